@@ -1,0 +1,197 @@
+/*
+ * slr_hip.h - C ABI of the MI355X spectral path tracer (libslr_hip.so).
+ *
+ * The library is a drop-in for the hot path of SLR: the scene it renders,
+ * the SBVH-collapsed-to-QBVH accelerator, the unidirectional spectral path
+ * tracer with MIS, and the film. Every entry point returns an int status
+ * (SLRH_OK or a negative error code); no C++ exception crosses the ABI;
+ * slrh_last_error() describes the last failure on a context. Host buffers
+ * are owned by the caller, device buffers by the library unless an entry
+ * point says it takes device pointers. A context is safe to use from one
+ * thread at a time per context.
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the SLR checkout).
+ */
+#ifndef SLR_HIP_H
+#define SLR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLRH_API __attribute__((visibility("default")))
+
+enum slrh_status {
+    SLRH_OK = 0,
+    SLRH_ERR_INVALID = -1,   /* bad argument */
+    SLRH_ERR_STATE = -2,     /* call out of order (e.g. query before build) */
+    SLRH_ERR_DEVICE = -3,    /* HIP runtime error */
+    SLRH_ERR_NOMEM = -4,     /* device or host allocation failed */
+    SLRH_ERR_CAPACITY = -5,  /* request exceeds a documented limit */
+    SLRH_ERR_NOTFOUND = -6   /* unknown ticket / id */
+};
+
+typedef struct slrh_context slrh_context;
+typedef struct slrh_scene slrh_scene;
+typedef struct slrh_film slrh_film;
+typedef struct slrh_server slrh_server;
+
+/* ------------------------------------------------------------ lifecycle */
+
+/* Creates a context on HIP device `device`. Replaces the process-wide state
+ * set up by SLR::initializeColorSystem() (libSLR/BasicTypes/spectrum_base.cpp:224)
+ * and the per-render thread pool (libSLR/Helper/ThreadPool.h:17-84). */
+SLRH_API int slrh_create(int device, slrh_context** out);
+SLRH_API int slrh_destroy(slrh_context* ctx);
+SLRH_API const char* slrh_last_error(const slrh_context* ctx);
+SLRH_API int slrh_version(void);
+
+/* ------------------------------------------------------------ spectra */
+
+enum slrh_spectrum_type {
+    SLRH_SPECTRUM_REFLECTANCE = 0,  /* SpectrumType::Reflectance */
+    SLRH_SPECTRUM_LIGHT_SOURCE = 1  /* SpectrumType::LightSource */
+};
+
+/* RGB (linear sRGB) upsampled to a spectrum by Meng et al. 2015; replaces
+ * UpsampledContinuousSpectrum(spType, ColorSpace::sRGB, r, g, b)
+ * (libSLR/BasicTypes/spectrum_types.cpp:482-543). */
+SLRH_API int slrh_spectrum_rgb(slrh_scene* s, int type, float r, float g, float b, int* id);
+/* Regularly sampled spectrum over [lambda_min, lambda_max] in nm; replaces
+ * RegularContinuousSpectrum (libSLR/BasicTypes/spectrum_types.h:88-108).
+ * n == 1 gives a constant. */
+SLRH_API int slrh_spectrum_regular(slrh_scene* s, float lambda_min, float lambda_max,
+                                   const float* values, int n, int* id);
+
+/* ------------------------------------------------------------ materials */
+
+/* DiffuseReflectionSurfaceMaterial without sigma -> LambertianBRDF
+ * (libSLR/SurfaceMaterial/basic_surface_materials.cpp:16-26). */
+SLRH_API int slrh_material_lambert(slrh_scene* s, int reflectance, int* id);
+/* DiffuseReflectionSurfaceMaterial with sigma -> OrenNayerBRDF
+ * (libSLR/BSDF/OrenNayerBRDF.h:28-30). */
+SLRH_API int slrh_material_oren_nayar(slrh_scene* s, int reflectance, float sigma, int* id);
+/* MicrofacetReflectionSurfaceMaterial, GGX (libSLR/SurfaceMaterial/microfacet_surface_materials.cpp:15-20). */
+SLRH_API int slrh_material_ggx_conductor(slrh_scene* s, int eta, int k, float alpha_x, float alpha_y, int* id);
+/* MicrofacetScatteringSurfaceMaterial, GGX (microfacet_surface_materials.cpp:24-30). */
+SLRH_API int slrh_material_ggx_dielectric(slrh_scene* s, int eta_ext, int eta_int, float alpha_x,
+                                          float alpha_y, int* id);
+/* EmitterSurfaceMaterial(base, DiffuseEmitterSurfaceProperty(emittance))
+ * (libSLR/Core/surface_material.h:55-70). base may be -1 only if no path
+ * ever needs to scatter off the emitter (the reference then dereferences null). */
+SLRH_API int slrh_material_diffuse_emitter(slrh_scene* s, int base_material, int emittance, int* id);
+
+/* ------------------------------------------------------------ geometry */
+
+/* Adds a triangle mesh in world space. Replaces SLR::TriangleMeshNode +
+ * MaterialGroupInTriangleMesh::setTriangles (libSLR/Scene/TriangleMeshNode.h:17-35)
+ * with its Vertex{position, normal, tangent, texCoord} (libSLR/Core/geometry.h:293-301).
+ * pos: 3*nverts floats; nrm/tan: 3*nverts floats or NULL (then the geometric
+ * normal / an orthogonal tangent is used); idx: 3*ntris; mat: ntris material
+ * ids, or NULL with `material` used for every triangle. */
+SLRH_API int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm,
+                           const float* tan, uint32_t ntris, const uint32_t* idx,
+                           const uint32_t* mat, int material, int* mesh_id);
+
+/* Camera-to-world 4x4 (column-major) and PerspectiveCamera parameters
+ * (libSLR/Camera/PerspectiveCamera.cpp:16-27, libSLR/Scene/camera_nodes.h:16-35). */
+SLRH_API int slrh_camera_perspective(slrh_scene* s, const float* l2w, float sensitivity, float aspect,
+                                     float fov_y, float lens_radius, float img_dist, float obj_dist);
+
+/* Builds the SBVH (libSLR/Accelerator/SBVH.h:380-417), collapses it into the
+ * QBVH (libSLR/Accelerator/QBVH.h:252-284) and uploads scene, accelerator and
+ * tables to HBM. Replaces SLR::Scene::build (libSLR/Scene/Scene.cpp:17-38). */
+SLRH_API int slrh_scene_create(slrh_context* ctx, slrh_scene** out);
+SLRH_API int slrh_scene_build(slrh_scene* s);
+SLRH_API int slrh_scene_destroy(slrh_scene* s);
+
+/* stats[0..7]: SBVH nodes, SBVH leaf refs, SBVH depth, SBVH SAH cost,
+ * QBVH nodes, QBVH leaf refs, QBVH depth, QBVH SAH cost; stats[8]: build seconds. */
+SLRH_API int slrh_scene_accel_stats(const slrh_scene* s, double* stats);
+/* Topology dumps in the same layout as the reference's node arrays (for
+ * parity tests): SBVH boxes[6*n], info[5*n] {c0,c1,offset,numLeaves,axis|0xFF},
+ * refs[leaf refs]; QBVH boxes[24*n], info[7*n] {children[4] in the reference's
+ * bitfield encoding, top/left/right axis}, refs. Buffers may be NULL to query n. */
+SLRH_API int slrh_scene_sbvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs,
+                                  uint32_t* num_nodes, uint32_t* num_refs);
+SLRH_API int slrh_scene_qbvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs,
+                                  uint32_t* num_nodes, uint32_t* num_refs);
+
+/* ------------------------------------------------------------ ray queries */
+
+/* Structure-of-arrays ray batch; all pointers refer to n-element arrays. */
+typedef struct slrh_rays {
+    const float* ox; const float* oy; const float* oz;
+    const float* dx; const float* dy; const float* dz;
+    const float* tmin; const float* tmax;
+} slrh_rays;
+
+typedef struct slrh_hits {
+    float* t;        /* hit distance, or tmax on a miss */
+    int32_t* prim;   /* global triangle index (mesh order, then triangle order), -1 on a miss */
+    float* b0;       /* surface parameters as SurfaceInteraction stores them: */
+    float* b1;       /*   b0 = 1 - u - v (weight of vertex 0), b1 = u (weight of vertex 1) */
+} slrh_hits;
+
+enum slrh_mem { SLRH_MEM_HOST = 0, SLRH_MEM_DEVICE = 1 };
+
+/* Closest hit for n rays. Replaces QBVH::intersect (libSLR/Accelerator/QBVH.h:355-365)
+ * over TriangleSurfaceShape::intersect (libSLR/SurfaceShape/TriangleSurfaceShape.cpp:205-257).
+ * mem says whether rays/hits are host or device pointers. stream: a hipStream_t or NULL. */
+SLRH_API int slrh_intersect(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, uint32_t n,
+                            int mem, void* stream);
+/* Visibility (1 unoccluded, 0 occluded) for n segments [tmin, tmax]. Replaces
+ * QBVH::testVisibility (QBVH.h:367-376) over TriangleSurfaceShape::testVisibility
+ * (TriangleSurfaceShape.cpp:259-299). */
+SLRH_API int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* visibility, uint32_t n,
+                           int mem, void* stream);
+
+/* ------------------------------------------------------------ rendering */
+
+enum slrh_integrator { SLRH_PT = 0 };
+
+typedef struct slrh_render_settings {
+    uint32_t width, height;   /* RenderSettingItem::ImageWidth/ImageHeight */
+    uint32_t spp;             /* PTRenderer(spp) */
+    uint32_t max_depth;       /* 0 = the reference's cap of 100 bounces; else a tighter cap */
+    uint32_t seed;            /* RenderSettingItem::RNGSeed */
+    uint32_t integrator;      /* slrh_integrator */
+    uint32_t slots_per_pixel; /* live paths per pixel (0 = automatic) */
+    uint32_t flags;           /* bit 0: time the traversal kernels with HIP events */
+} slrh_render_settings;
+
+/* Film: FP64 16-stratum spectral sums per pixel, the same quantity the
+ * reference's ImageSensor holds (libSLR/Core/ImageSensor.cpp:125-130,
+ * libSLR/BasicTypes/spectrum_types.h:716-741) but summed in FP64. */
+SLRH_API int slrh_film_create(slrh_context* ctx, uint32_t width, uint32_t height, slrh_film** out);
+SLRH_API int slrh_film_destroy(slrh_film* f);
+SLRH_API int slrh_film_clear(slrh_film* f);
+
+enum slrh_film_format {
+    SLRH_FILM_STRATA16_F64 = 0,  /* h*w*16 doubles: raw stratum sums */
+    SLRH_FILM_XYZ_F64 = 1,       /* h*w*3 doubles: CIE XYZ of the sums (DiscretizedSpectrum::getRGB's XYZ step) */
+    SLRH_FILM_SRGB8 = 2          /* h*w*3 bytes: ImageSensor::saveImage tone curve with scale */
+};
+/* Copies the film out. scale multiplies the sums first (saveImage passes
+ * brightness / spp, libSLR/Renderer/PTRenderer.cpp:84). */
+SLRH_API int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size_t dst_bytes);
+
+/* Renders into `film` (accumulating onto what it holds). Replaces
+ * PTRenderer::render (libSLR/Renderer/PTRenderer.cpp:26-101). stats, if not
+ * NULL, receives 13 doubles: [0] rays traced, [1] samples, [2] device seconds,
+ * [3] non-finite samples dropped, [4] wavefront iterations, [5] extension
+ * kernel seconds, [6] extension launches, [7] extension rays, [8] shadow
+ * kernel seconds, [9] shadow launches, [10] shadow rays, [11] QBVH nodes
+ * visited, [12] triangles tested. Kernel seconds are 0 unless flags bit 0. */
+SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
+                         double* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLR_HIP_H */

@@ -1,0 +1,42 @@
+# Builds the reference renderer (libSLR) from its own sources, where they lie
+# under /root/reference, plus this repository's harness around it.
+# Output goes only to oracle/_ref/ (git-ignored; it travels with the tree).
+#
+#   make -f oracle/ref.mk -j8
+#
+# Flags follow the reference's release build (-O3 -DNDEBUG, gnu++11, no -march,
+# so no FMA contraction on x86-64) plus the two portability flags libSLR needs
+# on Linux: SLR_Platform_OpenBSD routes SLR_memalign to posix_memalign and
+# -include cstring supplies std::memcpy.
+REF     ?= /root/reference
+SRC      = $(REF)/libSLR
+OUT      = oracle/_ref
+CXX     ?= g++
+CC      ?= gcc
+FLAGS    = -O3 -DNDEBUG -fPIC -DSLR_Platform_OpenBSD -w
+CXXFLAGS = -std=gnu++11 $(FLAGS) -include cstring -I$(REF)
+CFLAGS   = $(FLAGS)
+
+EXCL    := StackAllocator MSpaceAllocator dlmalloc AMCMCPPMRenderer
+CPPS    := $(filter-out $(foreach e,$(EXCL),%$(e).cpp),$(shell find $(SRC) -name '*.cpp'))
+CS      := $(filter-out $(foreach e,$(EXCL),%$(e).c),$(shell find $(SRC) -name '*.c'))
+OBJS    := $(patsubst $(SRC)/%.cpp,$(OUT)/obj/%.o,$(CPPS)) $(patsubst $(SRC)/%.c,$(OUT)/obj/%.o,$(CS))
+
+all: $(OUT)/libslr_ref.so $(OUT)/libslr_refdrv.so
+
+$(OUT)/obj/%.o: $(SRC)/%.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/obj/%.o: $(SRC)/%.c
+	@mkdir -p $(dir $@)
+	$(CC) $(CFLAGS) -c $< -o $@
+
+$(OUT)/libslr_ref.so: $(OBJS)
+	$(CXX) -shared -o $@ $(OBJS) -lpthread
+
+$(OUT)/libslr_refdrv.so: oracle/ref_driver.cpp $(OUT)/libslr_ref.so
+	$(CXX) $(CXXFLAGS) -shared -o $@ oracle/ref_driver.cpp -L$(OUT) -lslr_ref -Wl,-rpath,'$$ORIGIN' -lpthread
+
+clean:
+	rm -rf $(OUT)

@@ -1,0 +1,479 @@
+// C ABI of libslr_hip.so (declared in include/slr_hip.h).
+//
+// Host-side responsibilities: argument checking, scene assembly (meshes,
+// spectra, materials, camera), the SBVH/QBVH build, and turning device
+// results into the film formats. Errors are reported as status codes with a
+// message kept on the context; no exception leaves this file.
+#include "../../include/slr_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+
+using namespace slrh;
+
+struct slrh_context {
+    int device = 0;
+    std::string lastError;
+    std::mutex mu;
+};
+
+struct slrh_scene {
+    slrh_context* ctx = nullptr;
+    SceneData data;
+    DeviceScene* dev = nullptr;
+    bool built = false;
+    uint32_t numMeshes = 0;
+};
+
+struct slrh_film {
+    slrh_context* ctx = nullptr;
+    DeviceFilm* dev = nullptr;
+    uint32_t w = 0, h = 0;
+};
+
+namespace {
+
+int fail(slrh_context* c, int code, const std::string& msg) {
+    if (c) c->lastError = msg;
+    return code;
+}
+
+// makeCoordinateSystem of a unit normal (libSLR/BasicTypes/Vector3D.h:61-67)
+void tangentOf(const float n[3], float t[3]) {
+    float sign = n[2] >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sign + n[2]);
+    const float b = n[0] * n[1] * a;
+    (void)b;
+    t[0] = 1.0f + sign * n[0] * n[0] * a;
+    t[1] = sign * b;
+    t[2] = -sign * n[0];
+}
+
+int addSpectrum(slrh_scene* s, HostSpectrum&& h, int* id) {
+    s->data.spectra.push_back(std::move(h));
+    *id = (int)s->data.spectra.size() - 1;
+    return SLRH_OK;
+}
+
+bool validSpectrum(const slrh_scene* s, int id) { return id >= 0 && id < (int)s->data.spectra.size(); }
+
+int addMaterial(slrh_scene* s, const DevMaterial& m, int* id) {
+    s->data.materials.push_back(m);
+    *id = (int)s->data.materials.size() - 1;
+    return SLRH_OK;
+}
+
+#define CHECK_SCENE(s)                                                                  \
+    do {                                                                                \
+        if (!(s)) return SLRH_ERR_INVALID;                                              \
+        if ((s)->built) return fail((s)->ctx, SLRH_ERR_STATE, "scene is already built"); \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int slrh_version(void) { return 100; }
+
+int slrh_create(int device, slrh_context** out) {
+    if (!out) return SLRH_ERR_INVALID;
+    std::unique_ptr<slrh_context> c(new (std::nothrow) slrh_context());
+    if (!c) return SLRH_ERR_NOMEM;
+    c->device = device;
+    std::string err;
+    int r = deviceInit(device, &err);
+    if (r) return r;
+    *out = c.release();
+    return SLRH_OK;
+}
+
+int slrh_destroy(slrh_context* ctx) {
+    delete ctx;
+    return SLRH_OK;
+}
+
+const char* slrh_last_error(const slrh_context* ctx) { return ctx ? ctx->lastError.c_str() : "null context"; }
+
+int slrh_scene_create(slrh_context* ctx, slrh_scene** out) {
+    if (!ctx || !out) return SLRH_ERR_INVALID;
+    slrh_scene* s = new (std::nothrow) slrh_scene();
+    if (!s) return SLRH_ERR_NOMEM;
+    s->ctx = ctx;
+    *out = s;
+    return SLRH_OK;
+}
+
+int slrh_scene_destroy(slrh_scene* s) {
+    if (!s) return SLRH_ERR_INVALID;
+    if (s->dev) freeScene(s->dev);
+    delete s;
+    return SLRH_OK;
+}
+
+int slrh_spectrum_rgb(slrh_scene* s, int type, float r, float g, float b, int* id) {
+    CHECK_SCENE(s);
+    if (!id || (type != SLRH_SPECTRUM_REFLECTANCE && type != SLRH_SPECTRUM_LIGHT_SOURCE))
+        return fail(s->ctx, SLRH_ERR_INVALID, "slrh_spectrum_rgb: bad type");
+    if (!std::isfinite(r) || !std::isfinite(g) || !std::isfinite(b))
+        return fail(s->ctx, SLRH_ERR_INVALID, "slrh_spectrum_rgb: non-finite value");
+    MengSpectrum m = mengFromRGB(type, r, g, b);
+    return addSpectrum(s, mengToRegular(m), id);
+}
+
+int slrh_spectrum_regular(slrh_scene* s, float lo, float hi, const float* values, int n, int* id) {
+    CHECK_SCENE(s);
+    if (!id || !values || n < 1 || !(hi > lo) || (n > 1 && !(hi > lo)))
+        return fail(s->ctx, SLRH_ERR_INVALID, "slrh_spectrum_regular: bad arguments");
+    HostSpectrum h;
+    h.lo = lo;
+    h.hi = hi;
+    h.values.assign(values, values + n);
+    return addSpectrum(s, std::move(h), id);
+}
+
+int slrh_material_lambert(slrh_scene* s, int refl, int* id) {
+    CHECK_SCENE(s);
+    if (!id || !validSpectrum(s, refl)) return fail(s->ctx, SLRH_ERR_INVALID, "lambert: bad spectrum");
+    DevMaterial m{};
+    m.type = MAT_LAMBERT; m.spec0 = refl; m.spec1 = -1; m.emit = -1; m.cls = CLS_DIFFUSE;
+    return addMaterial(s, m, id);
+}
+
+int slrh_material_oren_nayar(slrh_scene* s, int refl, float sigma, int* id) {
+    CHECK_SCENE(s);
+    if (!id || !validSpectrum(s, refl) || !(sigma >= 0)) return fail(s->ctx, SLRH_ERR_INVALID, "oren_nayar: bad args");
+    DevMaterial m{};
+    m.type = MAT_OREN_NAYAR; m.spec0 = refl; m.spec1 = -1; m.emit = -1; m.cls = CLS_DIFFUSE;
+    // OrenNayerBRDF's A and B (libSLR/BSDF/OrenNayerBRDF.h:28-30), double literals as there
+    m.p0 = (float)(1.0f - 0.5f * sigma * sigma / (sigma * sigma + 0.33));
+    m.p1 = (float)(0.45 * sigma * sigma / (sigma * sigma + 0.09));
+    return addMaterial(s, m, id);
+}
+
+int slrh_material_ggx_conductor(slrh_scene* s, int eta, int k, float ax, float ay, int* id) {
+    CHECK_SCENE(s);
+    if (!id || !validSpectrum(s, eta) || !validSpectrum(s, k) || !(ax > 0) || !(ay > 0))
+        return fail(s->ctx, SLRH_ERR_INVALID, "ggx_conductor: bad args");
+    DevMaterial m{};
+    m.type = MAT_GGX_CONDUCTOR; m.spec0 = eta; m.spec1 = k; m.emit = -1; m.p0 = ax; m.p1 = ay; m.cls = CLS_CONDUCTOR;
+    return addMaterial(s, m, id);
+}
+
+int slrh_material_ggx_dielectric(slrh_scene* s, int etaExt, int etaInt, float ax, float ay, int* id) {
+    CHECK_SCENE(s);
+    if (!id || !validSpectrum(s, etaExt) || !validSpectrum(s, etaInt) || !(ax > 0) || !(ay > 0))
+        return fail(s->ctx, SLRH_ERR_INVALID, "ggx_dielectric: bad args");
+    DevMaterial m{};
+    m.type = MAT_GGX_DIELECTRIC; m.spec0 = etaExt; m.spec1 = etaInt; m.emit = -1; m.p0 = ax; m.p1 = ay;
+    m.cls = CLS_DIELECTRIC;
+    return addMaterial(s, m, id);
+}
+
+int slrh_material_diffuse_emitter(slrh_scene* s, int base, int emittance, int* id) {
+    CHECK_SCENE(s);
+    if (!id || !validSpectrum(s, emittance) || base >= (int)s->data.materials.size() || base < -1)
+        return fail(s->ctx, SLRH_ERR_INVALID, "diffuse_emitter: bad args");
+    DevMaterial m{};
+    if (base >= 0) {
+        m = s->data.materials[base];
+        if (m.emit >= 0) return fail(s->ctx, SLRH_ERR_INVALID, "diffuse_emitter: base material already emits");
+    } else {
+        m.type = MAT_NONE; m.spec0 = m.spec1 = -1; m.cls = CLS_MISS;
+    }
+    m.emit = emittance;
+    return addMaterial(s, m, id);
+}
+
+int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm, const float* tan,
+                  uint32_t ntris, const uint32_t* idx, const uint32_t* mat, int material, int* mesh_id) {
+    CHECK_SCENE(s);
+    if (!pos || !idx || nverts == 0 || ntris == 0) return fail(s->ctx, SLRH_ERR_INVALID, "mesh_add: empty mesh");
+    const uint32_t nm = (uint32_t)s->data.materials.size();
+    if (!mat && (material < 0 || (uint32_t)material >= nm)) return fail(s->ctx, SLRH_ERR_INVALID, "mesh_add: bad material");
+    for (uint64_t i = 0; i < 3ull * ntris; ++i)
+        if (idx[i] >= nverts) return fail(s->ctx, SLRH_ERR_INVALID, "mesh_add: vertex index out of range");
+    if (mat)
+        for (uint32_t i = 0; i < ntris; ++i)
+            if (mat[i] >= nm) return fail(s->ctx, SLRH_ERR_INVALID, "mesh_add: material id out of range");
+    for (uint64_t i = 0; i < 3ull * nverts; ++i)
+        if (!std::isfinite(pos[i])) return fail(s->ctx, SLRH_ERR_INVALID, "mesh_add: non-finite position");
+    SceneData& d = s->data;
+    const uint32_t base = (uint32_t)(d.vpos.size() / 3);
+    if ((uint64_t)base + nverts > 0xFFFFFFFFull) return fail(s->ctx, SLRH_ERR_CAPACITY, "mesh_add: too many vertices");
+    d.vpos.insert(d.vpos.end(), pos, pos + 3ull * nverts);
+    std::vector<float> n(3ull * nverts, 0.0f);
+    if (nrm) {
+        n.assign(nrm, nrm + 3ull * nverts);
+    } else {
+        // smooth normals: sum of unit face normals (libSLRSceneGraph/Scene/TriangleMeshNode.cpp:106-131)
+        for (uint32_t t = 0; t < ntris; ++t) {
+            const float* p0 = pos + 3ull * idx[3 * t];
+            const float* p1 = pos + 3ull * idx[3 * t + 1];
+            const float* p2 = pos + 3ull * idx[3 * t + 2];
+            float e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+            float e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+            float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            float l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            if (!(l > 0)) continue;
+            for (int k = 0; k < 3; ++k)
+                for (int j = 0; j < 3; ++j) n[3ull * idx[3 * t + k] + j] += c[j] / l;
+        }
+        for (uint32_t v = 0; v < nverts; ++v) {
+            float* q = &n[3ull * v];
+            float l = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+            if (l > 0) { q[0] /= l; q[1] /= l; q[2] /= l; } else { q[0] = 0; q[1] = 0; q[2] = 1; }
+        }
+    }
+    d.vnrm.insert(d.vnrm.end(), n.begin(), n.end());
+    if (tan) {
+        d.vtan.insert(d.vtan.end(), tan, tan + 3ull * nverts);
+    } else {
+        for (uint32_t v = 0; v < nverts; ++v) {
+            float t[3];
+            tangentOf(&n[3ull * v], t);
+            d.vtan.insert(d.vtan.end(), t, t + 3);
+        }
+    }
+    for (uint64_t i = 0; i < 3ull * ntris; ++i) d.idx.push_back(base + idx[i]);
+    for (uint32_t i = 0; i < ntris; ++i) d.triMat.push_back(mat ? mat[i] : (uint32_t)material);
+    if (mesh_id) *mesh_id = (int)s->numMeshes;
+    ++s->numMeshes;
+    return SLRH_OK;
+}
+
+int slrh_camera_perspective(slrh_scene* s, const float* l2w, float sensitivity, float aspect, float fovY,
+                            float lensRadius, float imgDist, float objDist) {
+    CHECK_SCENE(s);
+    if (!l2w || !(aspect > 0) || !(fovY > 0) || !(lensRadius >= 0) || !(imgDist > 0) || !(objDist > 0))
+        return fail(s->ctx, SLRH_ERR_INVALID, "camera_perspective: bad args");
+    CameraParams& c = s->data.camera;
+    memcpy(c.l2w, l2w, sizeof(c.l2w));
+    c.sensitivity = sensitivity; c.aspect = aspect; c.fovY = fovY;
+    c.lensRadius = lensRadius; c.imgDist = imgDist; c.objDist = objDist;
+    s->data.hasCamera = true;
+    return SLRH_OK;
+}
+
+int slrh_scene_build(slrh_scene* s) {
+    CHECK_SCENE(s);
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    try {
+        const size_t nt = s->data.idx.size() / 3;
+        std::vector<float> tp(9 * nt);
+        for (size_t t = 0; t < nt; ++t)
+            for (int k = 0; k < 3; ++k)
+                for (int j = 0; j < 3; ++j) tp[9 * t + 3 * k + j] = s->data.vpos[3ull * s->data.idx[3 * t + k] + j];
+        int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        buildAccel(tp, &s->data.accel, threads);
+        std::string err;
+        int r = uploadScene(s->data, &s->dev, &err);
+        if (r) return fail(s->ctx, r, "scene upload: " + err);
+    } catch (const std::bad_alloc&) {
+        return fail(s->ctx, SLRH_ERR_NOMEM, "scene build: out of host memory");
+    } catch (const std::exception& e) {
+        return fail(s->ctx, SLRH_ERR_INVALID, std::string("scene build: ") + e.what());
+    }
+    s->built = true;
+    return SLRH_OK;
+}
+
+int slrh_scene_accel_stats(const slrh_scene* s, double* st) {
+    if (!s || !st) return SLRH_ERR_INVALID;
+    if (!s->built) return SLRH_ERR_STATE;
+    const Accel& a = s->data.accel;
+    st[0] = (double)a.sbvh.size();
+    st[1] = (double)a.sbvhRefs.size();
+    st[2] = a.sbvhDepth;
+    st[3] = a.sbvhCost;
+    st[4] = (double)a.qbvh.size();
+    st[5] = (double)a.qbvhRefs.size();
+    st[6] = a.qbvhDepth;
+    st[7] = a.qbvhCost;
+    st[8] = a.buildSeconds;
+    return SLRH_OK;
+}
+
+int slrh_scene_sbvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs, uint32_t* nn,
+                         uint32_t* nr) {
+    if (!s) return SLRH_ERR_INVALID;
+    if (!s->built) return SLRH_ERR_STATE;
+    const Accel& a = s->data.accel;
+    if (nn) *nn = (uint32_t)a.sbvh.size();
+    if (nr) *nr = (uint32_t)a.sbvhRefs.size();
+    for (size_t i = 0; boxes && info && i < a.sbvh.size(); ++i) {
+        const SbvhNode& n = a.sbvh[i];
+        for (int k = 0; k < 3; ++k) { boxes[6 * i + k] = n.box.mn[k]; boxes[6 * i + 3 + k] = n.box.mx[k]; }
+        info[5 * i + 0] = n.c0; info[5 * i + 1] = n.c1; info[5 * i + 2] = n.first; info[5 * i + 3] = n.count;
+        info[5 * i + 4] = n.count ? 0xFFu : n.axis;
+    }
+    for (size_t i = 0; refs && i < a.sbvhRefs.size(); ++i) refs[i] = (int32_t)a.sbvhRefs[i];
+    return SLRH_OK;
+}
+
+int slrh_scene_qbvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs, uint32_t* nn,
+                         uint32_t* nr) {
+    if (!s) return SLRH_ERR_INVALID;
+    if (!s->built) return SLRH_ERR_STATE;
+    const Accel& a = s->data.accel;
+    if (nn) *nn = (uint32_t)a.qbvh.size();
+    if (nr) *nr = (uint32_t)a.qbvhRefs.size();
+    for (size_t i = 0; boxes && info && i < a.qbvh.size(); ++i) {
+        const QbvhNode& n = a.qbvh[i];
+        for (int ax = 0; ax < 3; ++ax)
+            for (int c = 0; c < 4; ++c) { boxes[24 * i + 4 * ax + c] = n.bmin[ax][c]; boxes[24 * i + 12 + 4 * ax + c] = n.bmax[ax][c]; }
+        for (int c = 0; c < 4; ++c) {
+            uint32_t w = 0xFFFFFFFFu;
+            if (n.childValid[c]) {
+                // the reference's Children bitfield {idx:27, numLeaves:4, isLeaf:1} (QBVH.h:26-39)
+                w = (n.index[c] & 0x07FFFFFFu) | ((n.count[c] & 0xFu) << 27) | ((n.isLeaf[c] ? 1u : 0u) << 31);
+            }
+            info[7 * i + c] = w;
+        }
+        info[7 * i + 4] = n.topAxis; info[7 * i + 5] = n.leftAxis; info[7 * i + 6] = n.rightAxis;
+    }
+    for (size_t i = 0; refs && i < a.qbvhRefs.size(); ++i) refs[i] = (int32_t)a.qbvhRefs[i];
+    return SLRH_OK;
+}
+
+static int doTrace(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, float* vis, uint32_t n, int mem,
+                   void* stream) {
+    if (!s || !rays) return SLRH_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "trace before slrh_scene_build");
+    if (mem != SLRH_MEM_HOST && mem != SLRH_MEM_DEVICE) return fail(s->ctx, SLRH_ERR_INVALID, "bad mem kind");
+    if (n == 0) return SLRH_OK;
+    TraceArgs a{};
+    a.ox = rays->ox; a.oy = rays->oy; a.oz = rays->oz; a.dx = rays->dx; a.dy = rays->dy; a.dz = rays->dz;
+    a.tmin = rays->tmin; a.tmax = rays->tmax;
+    if (!a.ox || !a.oy || !a.oz || !a.dx || !a.dy || !a.dz || !a.tmin || !a.tmax)
+        return fail(s->ctx, SLRH_ERR_INVALID, "null ray array");
+    if (hits) {
+        a.t = hits->t; a.prim = hits->prim; a.b0 = hits->b0; a.b1 = hits->b1;
+        if (!a.t || !a.prim || !a.b0 || !a.b1) return fail(s->ctx, SLRH_ERR_INVALID, "null hit array");
+    } else {
+        a.vis = vis;
+        if (!a.vis) return fail(s->ctx, SLRH_ERR_INVALID, "null visibility array");
+    }
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    std::string err;
+    int r = traceBatch(s->dev, a, n, hits == nullptr, mem == SLRH_MEM_DEVICE, stream, nullptr, nullptr, &err);
+    if (r) return fail(s->ctx, r, err);
+    return SLRH_OK;
+}
+
+int slrh_intersect(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, uint32_t n, int mem, void* stream) {
+    if (!hits) return s ? fail(s->ctx, SLRH_ERR_INVALID, "null hits") : SLRH_ERR_INVALID;
+    return doTrace(s, rays, hits, nullptr, n, mem, stream);
+}
+
+int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* vis, uint32_t n, int mem, void* stream) {
+    return doTrace(s, rays, nullptr, vis, n, mem, stream);
+}
+
+int slrh_film_create(slrh_context* ctx, uint32_t w, uint32_t h, slrh_film** out) {
+    if (!ctx || !out || w == 0 || h == 0) return SLRH_ERR_INVALID;
+    if ((uint64_t)w * h > (1ull << 27)) return fail(ctx, SLRH_ERR_CAPACITY, "film too large");
+    std::unique_ptr<slrh_film> f(new (std::nothrow) slrh_film());
+    if (!f) return SLRH_ERR_NOMEM;
+    f->ctx = ctx; f->w = w; f->h = h;
+    std::string err;
+    int r = filmCreate(w, h, &f->dev, &err);
+    if (r) return fail(ctx, r, err);
+    *out = f.release();
+    return SLRH_OK;
+}
+
+int slrh_film_destroy(slrh_film* f) {
+    if (!f) return SLRH_ERR_INVALID;
+    filmFree(f->dev);
+    delete f;
+    return SLRH_OK;
+}
+
+int slrh_film_clear(slrh_film* f) {
+    if (!f) return SLRH_ERR_INVALID;
+    std::string err;
+    int r = filmClear(f->dev, &err);
+    return r ? fail(f->ctx, r, err) : SLRH_OK;
+}
+
+int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size_t bytes) {
+    if (!f || !dst) return SLRH_ERR_INVALID;
+    const size_t px = (size_t)f->w * f->h;
+    size_t need = format == SLRH_FILM_STRATA16_F64 ? px * 16 * 8 : format == SLRH_FILM_XYZ_F64 ? px * 3 * 8
+                : format == SLRH_FILM_SRGB8 ? px * 3 : 0;
+    if (need == 0) return fail(f->ctx, SLRH_ERR_INVALID, "film_read: bad format");
+    if (bytes < need) return fail(f->ctx, SLRH_ERR_INVALID, "film_read: destination too small");
+    std::vector<double> st;
+    std::string err;
+    int r = filmDownload(f->dev, &st, &err);
+    if (r) return fail(f->ctx, r, err);
+    if (format == SLRH_FILM_STRATA16_F64) {
+        double* o = (double*)dst;
+        for (size_t i = 0; i < st.size(); ++i) o[i] = st[i] * scale;
+        return SLRH_OK;
+    }
+    float xb[16], yb[16], zb[16], integ;
+    cmfStrata(xb, yb, zb, &integ);
+    for (size_t p = 0; p < px; ++p) {
+        // DiscretizedSpectrum::getRGB's XYZ step (libSLR/BasicTypes/spectrum_types.h:666-690)
+        double X = 0, Y = 0, Z = 0;
+        for (int k = 0; k < 16; ++k) {
+            double v = st[16 * p + k] * scale;
+            X += xb[k] * v; Y += yb[k] * v; Z += zb[k] * v;
+        }
+        X /= integ; Y /= integ; Z /= integ;
+        if (format == SLRH_FILM_XYZ_F64) {
+            double* o = (double*)dst + 3 * p;
+            o[0] = X; o[1] = Y; o[2] = Z;
+        } else {
+            // XYZ -> sRGB, clamp, 1 - exp(-x) tone curve, gamma (ImageSensor::saveImage,
+            // libSLR/Core/ImageSensor.cpp:165-189)
+            double rgb[3] = {3.2404542 * X - 1.5371385 * Y - 0.4985314 * Z,
+                             -0.9692660 * X + 1.8760108 * Y + 0.0415560 * Z,
+                             0.0556434 * X - 0.2040259 * Y + 1.0572252 * Z};
+            uint8_t* o = (uint8_t*)dst + 3 * p;
+            for (int k = 0; k < 3; ++k) {
+                double v = rgb[k] < 0 ? 0 : rgb[k];
+                v = 1.0 - std::exp(-v);
+                v = v <= 0.0031308 ? 12.92 * v : 1.055 * std::pow(v, 1.0 / 2.4) - 0.055;
+                double q = 256.0 * v;
+                o[k] = (uint8_t)std::min(255.0, std::max(0.0, q));
+            }
+        }
+    }
+    return SLRH_OK;
+}
+
+int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film, double* stats) {
+    if (!s || !set || !film) return SLRH_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "render before slrh_scene_build");
+    if (!s->data.hasCamera) return fail(s->ctx, SLRH_ERR_STATE, "scene has no camera");
+    if (set->integrator != SLRH_PT) return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
+    if (set->width != film->w || set->height != film->h) return fail(s->ctx, SLRH_ERR_INVALID, "film size mismatch");
+    if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    RenderConfig cfg{set->width, set->height, set->spp, set->max_depth, set->seed, set->slots_per_pixel,
+                     (set->flags & 1u) != 0};
+    RenderStats st;
+    std::string err;
+    int r = slrh::render(s->dev, cfg, film->dev, &st, &err);
+    if (r) return fail(s->ctx, r, err);
+    if (stats) {
+        stats[0] = st.raysTraced; stats[1] = st.samples; stats[2] = st.kernelSeconds; stats[3] = st.nonFinite;
+        stats[4] = st.iterations; stats[5] = st.extendSeconds; stats[6] = st.extendLaunches; stats[7] = st.extendRays;
+        stats[8] = st.shadowSeconds; stats[9] = st.shadowLaunches; stats[10] = st.shadowRays;
+        stats[11] = st.nodesVisited; stats[12] = st.trisTested;
+    }
+    return SLRH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// Device-side data layout and math shared by the kernels.
+//
+// Layout in HBM (all arrays 16-byte aligned, structure-of-arrays where lanes
+// of a wave touch consecutive elements):
+//   QBVH node   : 8 x float4 = 128 B, one cache line. Planes 0-5 hold the four
+//                 children's bounds per axis (min x,y,z then max x,y,z, the
+//                 SoA-in-node form of libSLR/Accelerator/QBVH.h:41-53), plane 6
+//                 the four child words, plane 7 the packed split axes.
+//   child word  : 0xFFFFFFFF invalid; bit 31 set = leaf with count in bits
+//                 24-30 and first triangle reference in bits 0-23; otherwise
+//                 an internal node index.
+//   triangle ref: 3 x float4 = 48 B {v0.xyz, prim id}, {v1-v0}, {v2-v0}; one
+//                 per leaf reference, in leaf order, so a leaf is contiguous.
+//                 The edges are the same FP32 differences the reference forms
+//                 at each test (TriangleSurfaceShape.cpp:212-213).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace slrh {
+
+constexpr uint32_t kInvalidChild = 0xFFFFFFFFu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kMaxLeafCount = 127;
+constexpr uint32_t kMaxTriRefs = 1u << 24;
+constexpr int kMaxStack = 64;   // QBVH::commonProcedure's StackSize (QBVH.h:299)
+constexpr int kNumL = 16;       // wavelengths per path
+constexpr float kLambdaLowD = 360.0f;    // WavelengthLowBound
+constexpr float kLambdaHighD = 830.0f;   // WavelengthHighBound
+
+enum MaterialType : uint32_t {
+    MAT_LAMBERT = 0,
+    MAT_OREN_NAYAR = 1,
+    MAT_GGX_CONDUCTOR = 2,
+    MAT_GGX_DIELECTRIC = 3,
+    MAT_NONE = 4,   // emitter without a scattering base
+};
+// Shading queues: one per scattering model plus misses.
+enum ShadeClass : uint32_t {
+    CLS_DIFFUSE = 0,     // Lambert and Oren-Nayar share a queue (same sampling)
+    CLS_CONDUCTOR = 1,
+    CLS_DIELECTRIC = 2,
+    CLS_MISS = 3,        // no hit / emitter without base
+    kNumClasses = 4,
+};
+
+struct DevMaterial {
+    uint32_t type;
+    int32_t spec0, spec1;   // reflectance | eta, k | etaExt, etaInt
+    int32_t emit;           // emittance spectrum, -1 if not an emitter
+    float p0, p1;           // OrenNayar A, B | GGX alpha_x, alpha_y
+    uint32_t cls;
+    uint32_t pad;
+};
+
+struct DevSpectrum {
+    float lo, hi;
+    uint32_t offset, n;
+};
+
+struct DevCamera {
+    float l2w[12];        // rows of the 3x4 camera-to-world matrix
+    float nrm[9];         // rows of the inverse-transpose (for normals)
+    float opWidth, opHeight, objDist, imgDist, imgPlaneArea, lensRadius;
+    uint32_t width, height;
+};
+
+struct DevScene {
+    const float4* nodes;
+    const float4* tris;
+    const uint4* primVerts;   // i0, i1, i2, material
+    const float4* vpos;
+    const float4* vnrm;
+    const float4* vtan;
+    const DevMaterial* mats;
+    const DevSpectrum* spectra;
+    const float* specPool;
+    const uint32_t* lights;   // emitting prim ids
+    const float* lightCdf;    // numLights + 1 entries
+    const float* lightPmf;
+    const int32_t* primToLight;
+    uint32_t numLights;
+    uint32_t numPrims;
+    float lightImportance;    // SurfaceObjectAggregate::importance()
+    uint32_t pad;
+    DevCamera cam;
+};
+
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator-(float3 a) { return f3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float3 operator*(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 operator*(float s, float3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+// Vector3D::operator/(s) multiplies by the reciprocal (libSLR/BasicTypes/Vector3D.h:31).
+__device__ __forceinline__ float3 divs(float3 a, float s) { float r = 1.0f / s; return f3(a.x * r, a.y * r, a.z * r); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length3(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ float3 normalize3(float3 a) { return divs(a, length3(a)); }
+__device__ __forceinline__ float3 xyz(float4 a) { return f3(a.x, a.y, a.z); }
+
+}  // namespace slrh

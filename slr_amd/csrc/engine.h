@@ -1,0 +1,71 @@
+// Internal interface between the C ABI (capi.cpp) and the HIP side
+// (render.hip): host scene description in, device-resident scene out.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device_common.h"
+#include "host_scene.h"
+
+namespace slrh {
+
+struct CameraParams {
+    float l2w[16];   // column-major 4x4
+    float sensitivity = 1, aspect = 1, fovY = 0.5f, lensRadius = 0, imgDist = 1, objDist = 1;
+};
+
+struct SceneData {
+    std::vector<float> vpos, vnrm, vtan;   // 3 floats per vertex, world space
+    std::vector<uint32_t> idx;             // 3 per triangle (global vertex indices)
+    std::vector<uint32_t> triMat;          // material per triangle
+    std::vector<HostSpectrum> spectra;
+    std::vector<DevMaterial> materials;
+    bool hasCamera = false;
+    CameraParams camera;
+    Accel accel;
+};
+
+struct DeviceScene;
+struct DeviceFilm;
+
+struct TraceArgs {
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+    float* t;
+    int32_t* prim;
+    float *b0, *b1;
+    float* vis;
+};
+
+struct RenderConfig {
+    uint32_t width, height, spp, maxDepth, seed, slotsPerPixel;
+    bool timeKernels;
+};
+
+struct RenderStats {
+    double raysTraced = 0, samples = 0, kernelSeconds = 0, nonFinite = 0, iterations = 0;
+    double extendSeconds = 0, extendLaunches = 0, extendRays = 0;
+    double shadowSeconds = 0, shadowLaunches = 0, shadowRays = 0;
+    double nodesVisited = 0, trisTested = 0;
+    double overflow = 0;
+};
+
+int deviceInit(int device, std::string* err);
+int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err);
+void freeScene(DeviceScene* ds);
+size_t sceneDeviceBytes(const DeviceScene* ds);
+
+// any == false: closest hit into t/prim/b0/b1; any == true: visibility into vis.
+int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool devicePtrs, void* stream,
+               double* nodesVisited, double* trisTested, std::string* err);
+
+int filmCreate(uint32_t w, uint32_t h, DeviceFilm** out, std::string* err);
+void filmFree(DeviceFilm* f);
+int filmClear(DeviceFilm* f, std::string* err);
+int filmDownload(const DeviceFilm* f, std::vector<double>* strata, std::string* err);
+void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h);
+
+int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err);
+
+}  // namespace slrh

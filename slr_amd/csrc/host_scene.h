@@ -1,0 +1,118 @@
+// Host-side scene description, accelerator build and spectral helpers shared
+// by the C ABI (capi.cpp), the builder (bvh_build.cpp) and the device
+// orchestration (render.hip).
+#pragma once
+
+#include <cstdint>
+#include <cmath>
+#include <string>
+#include <vector>
+
+namespace slrh {
+
+constexpr int kNumLambdas = 16;            // NumSpectralSamples (libSLR/declarations.h:50)
+constexpr float kLambdaLow = 360.0f;       // WavelengthLowBound (libSLR/BasicTypes/spectrum_base.h:110)
+constexpr float kLambdaHigh = 830.0f;      // WavelengthHighBound (spectrum_base.h:111)
+
+struct V3 {
+    float x, y, z;
+};
+inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+
+// Axis-aligned box with the reference's conventions: empty = (+inf, -inf),
+// union/intersection through fmin/fmax (libSLR/BasicTypes/BoundingBox3D.h:25-163).
+struct Box {
+    float mn[3], mx[3];
+    Box() { mn[0] = mn[1] = mn[2] = INFINITY; mx[0] = mx[1] = mx[2] = -INFINITY; }
+    static Box point(const V3& p) { Box b; b.mn[0] = b.mx[0] = p.x; b.mn[1] = b.mx[1] = p.y; b.mn[2] = b.mx[2] = p.z; return b; }
+    void unify(const Box& o) {
+        for (int a = 0; a < 3; ++a) { mn[a] = std::fmin(mn[a], o.mn[a]); mx[a] = std::fmax(mx[a], o.mx[a]); }
+    }
+    void unify(const V3& p) {
+        const float q[3] = {p.x, p.y, p.z};
+        for (int a = 0; a < 3; ++a) { mn[a] = std::fmin(mn[a], q[a]); mx[a] = std::fmax(mx[a], q[a]); }
+    }
+    float area() const {
+        float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+        return 2 * (dx * dy + dy * dz + dz * dx);
+    }
+    bool valid() const { return (mx[0] - mn[0]) >= 0 && (mx[1] - mn[1]) >= 0 && (mx[2] - mn[2]) >= 0; }
+    int widestAxis() const {
+        float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+        if (dx > dy && dx > dz) return 0;
+        if (dy > dz) return 1;
+        return 2;
+    }
+    float center(int a) const { return (mn[a] + mx[a]) * 0.5f; }
+    V3 centroid() const { return v3((mn[0] + mx[0]) * 0.5f, (mn[1] + mx[1]) * 0.5f, (mn[2] + mx[2]) * 0.5f); }
+};
+inline Box intersect(const Box& a, const Box& b) {
+    Box r;
+    for (int k = 0; k < 3; ++k) { r.mn[k] = std::fmax(a.mn[k], b.mn[k]); r.mx[k] = std::fmin(a.mx[k], b.mx[k]); }
+    return r;
+}
+
+// Binary SBVH node in the reference's DFS pre-order layout (SBVH.h:21-43).
+struct SbvhNode {
+    Box box;
+    uint32_t c0 = 0, c1 = 0, first = 0, count = 0;
+    uint8_t axis = 0;
+};
+
+// Collapsed 4-wide node (QBVH.h:41-53), kept on the host in the reference's
+// logical form; the device copy is packed by the renderer.
+struct QbvhChild {
+    // invalid: all bits set. Otherwise leaf/internal with an index and count.
+    uint32_t raw = 0xFFFFFFFFu;   // the reference's bitfield {idx:27, numLeaves:4, isLeaf:1}
+    bool valid() const { return raw != 0xFFFFFFFFu; }
+};
+struct QbvhNode {
+    float bmin[3][4], bmax[3][4];
+    bool isLeaf[4];
+    uint32_t index[4];   // node index (internal) or first leaf ref (leaf)
+    uint32_t count[4];   // leaf prim count (0 for internal)
+    bool childValid[4];
+    uint8_t topAxis = 0, leftAxis = 0, rightAxis = 0;
+};
+
+struct Accel {
+    std::vector<SbvhNode> sbvh;
+    std::vector<uint32_t> sbvhRefs;   // leaf refs (prim ids) in DFS order
+    uint32_t sbvhDepth = 0;
+    float sbvhCost = 0;
+    uint32_t numFragments = 0;
+    std::vector<QbvhNode> qbvh;
+    std::vector<uint32_t> qbvhRefs;
+    uint32_t qbvhDepth = 0;
+    float qbvhCost = 0;
+    Box bounds;
+    double buildSeconds = 0;
+    uint32_t maxLeafCount = 0;
+};
+
+// Builds SBVH then QBVH over triangles given as 9 floats each (world space).
+// Deterministic; identical decisions to the reference builder.
+void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads);
+
+// ---------------------------------------------------------------- spectra
+
+// A spectrum as the device evaluates it: regular samples over [lo, hi].
+struct HostSpectrum {
+    float lo = kLambdaLow, hi = kLambdaHigh;
+    std::vector<float> values;  // n >= 1
+};
+// Meng upsampling in the reference's representation: up to four data-point
+// indices with bilinear (inside) or barycentric (fan) weights and a scale.
+struct MengSpectrum {
+    uint32_t adj = 0;     // packed indices, byte 3 == 0xFF for three points
+    float s = 0, t = 0, scale = 0;
+};
+MengSpectrum mengFromRGB(int type, float r, float g, float b);
+// Evaluates exactly as UpsampledContinuousSpectrum::evaluate does.
+float mengEvaluate(const MengSpectrum& m, float lambda);
+// Combines the basis into a 95-sample table for the device.
+HostSpectrum mengToRegular(const MengSpectrum& m);
+// CMF strata for the film (DiscretizedSpectrum::initialize).
+void cmfStrata(float xbar[16], float ybar[16], float zbar[16], float* integral);
+
+}  // namespace slrh

@@ -1,0 +1,1159 @@
+// Device side of the renderer: scene upload, ray-batch queries and the
+// wavefront spectral path tracer.
+//
+// Wavefront loop (one iteration = one bounce of every live path):
+//   regen   : retire finished paths into the film, start the pixel's next
+//             sample (camera ray) while samples remain.
+//   extend  : QBVH closest hit for the extension-ray queue; each slot is
+//             appended to the shading queue of the material it hit (wave
+//             ballot + one atomic per wave per queue).
+//   shade_k : one kernel per queue (diffuse, conductor, dielectric, miss):
+//             emission with MIS, Russian roulette, light sample for NEE
+//             (appended to the shadow queue) and BSDF sample (appended to the
+//             next extension queue) or retirement.
+//   shadow  : QBVH any hit for the shadow queue; unoccluded NEE contributions
+//             are added to the path's radiance.
+// A slot owns one pixel (and one film plane when several slots share a
+// pixel), so film accumulation needs no atomics: only the slot's own thread
+// ever adds into its film entry, in sample order, in FP64.
+//
+// The estimator is PTRenderer::Job::contribution
+// (libSLR/Renderer/PTRenderer.cpp:146-277) and the camera/weight of
+// PTRenderer::Job::kernel (:103-144).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bsdf.h"
+#include "engine.h"
+#include "traverse.h"
+
+namespace slrh {
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            if (err) *err = std::string(#expr) + ": " + hipGetErrorString(e_);     \
+            return SLRH_ERR_DEVICE_CODE;                                           \
+        }                                                                          \
+    } while (0)
+
+constexpr int SLRH_ERR_DEVICE_CODE = -3;
+constexpr int SLRH_ERR_NOMEM_CODE = -4;
+constexpr int SLRH_ERR_CAPACITY_CODE = -5;
+
+constexpr int kBlock = 256;
+constexpr int kLdsDepth = 16;
+
+static int g_numCUs = 256;
+
+// ------------------------------------------------------------------ buffers
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t count, std::string* err) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        n = count;
+        if (count == 0) return 0;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            if (err) *err = std::string("hipMalloc: ") + hipGetErrorString(e);
+            return SLRH_ERR_NOMEM_CODE;
+        }
+        return 0;
+    }
+    int upload(const std::vector<T>& v, std::string* err) {
+        int r = alloc(v.size(), err);
+        if (r) return r;
+        if (!v.empty()) {
+            hipError_t e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                if (err) *err = std::string("hipMemcpy: ") + hipGetErrorString(e);
+                return SLRH_ERR_DEVICE_CODE;
+            }
+        }
+        return 0;
+    }
+};
+
+struct DeviceScene {
+    DevBuf<float4> nodes, tris, vpos, vnrm, vtan;
+    DevBuf<uint4> primVerts;
+    DevBuf<DevMaterial> mats;
+    DevBuf<DevSpectrum> spectra;
+    DevBuf<float> specPool, lightCdf, lightPmf;
+    DevBuf<uint32_t> lights;
+    DevBuf<int32_t> primToLight;
+    DevBuf<uint32_t> spill;      // traversal stack spill for ray batches
+    DevBuf<unsigned long long> counters;
+    DevScene view;
+    bool hasCamera = false;
+    size_t bytes = 0;
+};
+
+struct DeviceFilm {
+    uint32_t w = 0, h = 0;
+    DevBuf<double> data;   // h*w*16
+};
+
+int deviceInit(int device, std::string* err) {
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    g_numCUs = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    return 0;
+}
+
+size_t sceneDeviceBytes(const DeviceScene* ds) { return ds->bytes; }
+
+static void cameraDerived(const CameraParams& c, uint32_t w, uint32_t h, DevCamera* d) {
+    // PerspectiveCamera::PerspectiveCamera (libSLR/Camera/PerspectiveCamera.cpp:16-27)
+    float opH = 2.0f * c.objDist * std::tan(c.fovY * 0.5f);
+    float opW = opH * c.aspect;
+    d->opHeight = opH;
+    d->opWidth = opW;
+    d->objDist = c.objDist;
+    d->imgDist = c.imgDist;
+    d->imgPlaneArea = (float)(opW * opH * std::pow(c.imgDist / c.objDist, 2));
+    d->lensRadius = c.lensRadius;
+    d->width = w;
+    d->height = h;
+    const float* m = c.l2w;   // column-major: m[col*4 + row]
+    for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 4; ++col) d->l2w[r * 4 + col] = m[col * 4 + r];
+    // inverse transpose of the upper 3x3 for normals
+    double a[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 3; ++col) a[r][col] = m[col * 4 + r];
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    double inv[3][3];
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+    inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+    inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 3; ++col) d->nrm[r * 3 + col] = (float)inv[col][r];
+}
+
+int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
+    std::unique_ptr<DeviceScene> ds(new DeviceScene());
+    const Accel& ac = sd.accel;
+    // QBVH nodes
+    std::vector<float4> nodes(ac.qbvh.size() * 8);
+    for (size_t i = 0; i < ac.qbvh.size(); ++i) {
+        const QbvhNode& q = ac.qbvh[i];
+        float4* o = &nodes[8 * i];
+        for (int a = 0; a < 3; ++a) {
+            o[a] = make_float4(q.bmin[a][0], q.bmin[a][1], q.bmin[a][2], q.bmin[a][3]);
+            o[3 + a] = make_float4(q.bmax[a][0], q.bmax[a][1], q.bmax[a][2], q.bmax[a][3]);
+        }
+        uint32_t ch[4];
+        for (int c = 0; c < 4; ++c) {
+            if (!q.childValid[c]) ch[c] = kInvalidChild;
+            else if (q.isLeaf[c]) {
+                if (q.count[c] > kMaxLeafCount || q.index[c] + q.count[c] > kMaxTriRefs) {
+                    if (err) *err = "QBVH leaf exceeds the device encoding (count <= 127, refs < 2^24)";
+                    return SLRH_ERR_CAPACITY_CODE;
+                }
+                ch[c] = kLeafBit | (q.count[c] << 24) | q.index[c];
+            } else ch[c] = q.index[c];
+        }
+        uint32_t axes = (uint32_t)q.topAxis | ((uint32_t)q.leftAxis << 2) | ((uint32_t)q.rightAxis << 4);
+        float4 cw, ax;
+        memcpy(&cw, ch, 16);
+        uint32_t axw[4] = {axes, 0, 0, 0};
+        memcpy(&ax, axw, 16);
+        o[6] = cw;
+        o[7] = ax;
+    }
+    // triangle references in leaf order
+    std::vector<float4> tris(ac.qbvhRefs.size() * 3);
+    for (size_t r = 0; r < ac.qbvhRefs.size(); ++r) {
+        uint32_t p = ac.qbvhRefs[r];
+        const uint32_t* v = &sd.idx[3 * (size_t)p];
+        const float* p0 = &sd.vpos[3 * (size_t)v[0]];
+        const float* p1 = &sd.vpos[3 * (size_t)v[1]];
+        const float* p2 = &sd.vpos[3 * (size_t)v[2]];
+        float pid;
+        memcpy(&pid, &p, 4);
+        tris[3 * r + 0] = make_float4(p0[0], p0[1], p0[2], pid);
+        tris[3 * r + 1] = make_float4(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], 0.0f);
+        tris[3 * r + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], 0.0f);
+    }
+    const size_t nv = sd.vpos.size() / 3, nt = sd.idx.size() / 3;
+    std::vector<float4> vp(nv), vn(nv), vt(nv);
+    for (size_t i = 0; i < nv; ++i) {
+        vp[i] = make_float4(sd.vpos[3 * i], sd.vpos[3 * i + 1], sd.vpos[3 * i + 2], 0);
+        vn[i] = make_float4(sd.vnrm[3 * i], sd.vnrm[3 * i + 1], sd.vnrm[3 * i + 2], 0);
+        vt[i] = make_float4(sd.vtan[3 * i], sd.vtan[3 * i + 1], sd.vtan[3 * i + 2], 0);
+    }
+    std::vector<uint4> pv(nt);
+    for (size_t i = 0; i < nt; ++i) pv[i] = make_uint4(sd.idx[3 * i], sd.idx[3 * i + 1], sd.idx[3 * i + 2], sd.triMat[i]);
+    // spectra
+    std::vector<DevSpectrum> specs;
+    std::vector<float> pool;
+    for (const HostSpectrum& h : sd.spectra) {
+        DevSpectrum d;
+        d.lo = h.lo; d.hi = h.hi; d.offset = (uint32_t)pool.size(); d.n = (uint32_t)h.values.size();
+        pool.insert(pool.end(), h.values.begin(), h.values.end());
+        specs.push_back(d);
+    }
+    if (pool.empty()) pool.push_back(0.0f);
+    // lights: every emitting triangle with importance 1, in object order
+    // (SurfaceObjectAggregate ctor, libSLR/Core/surface_object.cpp:379-398)
+    std::vector<uint32_t> lights;
+    std::vector<int32_t> primToLight(nt, -1);
+    for (size_t i = 0; i < nt; ++i) {
+        const DevMaterial& m = sd.materials[sd.triMat[i]];
+        if (m.emit >= 0) { primToLight[i] = (int32_t)lights.size(); lights.push_back((uint32_t)i); }
+    }
+    // DiscreteDistribution1D over unit importances with a compensated sum
+    // (libSLR/Core/distributions.cpp:99-123)
+    std::vector<float> cdf(lights.size() + 1), pmf(lights.size());
+    {
+        float sum = 0, comp = 0;
+        cdf[0] = 0;
+        for (size_t i = 0; i < lights.size(); ++i) {
+            float in = 1.0f - comp, tmp = sum + in;
+            comp = (tmp - sum) - in;
+            sum = tmp;
+            cdf[i + 1] = sum;
+        }
+        for (size_t i = 0; i < lights.size(); ++i) { pmf[i] = 1.0f / sum; cdf[i + 1] /= sum; }
+        ds->view.lightImportance = lights.empty() ? 0.0f : sum;
+    }
+    if (cdf.empty()) cdf.push_back(0);
+    if (pmf.empty()) pmf.push_back(0);
+    if (lights.empty()) lights.push_back(0);
+    int r;
+    if ((r = ds->nodes.upload(nodes, err)) || (r = ds->tris.upload(tris, err)) || (r = ds->vpos.upload(vp, err)) ||
+        (r = ds->vnrm.upload(vn, err)) || (r = ds->vtan.upload(vt, err)) || (r = ds->primVerts.upload(pv, err)) ||
+        (r = ds->mats.upload(sd.materials, err)) || (r = ds->spectra.upload(specs, err)) ||
+        (r = ds->specPool.upload(pool, err)) || (r = ds->lights.upload(lights, err)) ||
+        (r = ds->lightCdf.upload(cdf, err)) || (r = ds->lightPmf.upload(pmf, err)) ||
+        (r = ds->primToLight.upload(primToLight, err)))
+        return r;
+    if ((r = ds->counters.alloc(16, err))) return r;
+    ds->bytes = nodes.size() * 16 + tris.size() * 16 + (vp.size() * 3) * 16 + pv.size() * 16 + pool.size() * 4;
+    DevScene& v = ds->view;
+    v.nodes = ds->nodes.p; v.tris = ds->tris.p; v.primVerts = ds->primVerts.p;
+    v.vpos = ds->vpos.p; v.vnrm = ds->vnrm.p; v.vtan = ds->vtan.p;
+    v.mats = ds->mats.p; v.spectra = ds->spectra.p; v.specPool = ds->specPool.p;
+    v.lights = ds->lights.p; v.lightCdf = ds->lightCdf.p; v.lightPmf = ds->lightPmf.p;
+    v.primToLight = ds->primToLight.p;
+    v.numLights = (uint32_t)(cdf.size() - 1);
+    v.numPrims = (uint32_t)nt;
+    ds->hasCamera = sd.hasCamera;
+    if (sd.hasCamera) cameraDerived(sd.camera, 1, 1, &v.cam);
+    *out = ds.release();
+    return 0;
+}
+
+void freeScene(DeviceScene* ds) { delete ds; }
+
+// ------------------------------------------------------------------ wave helpers
+
+__device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
+
+// Appends the lanes with pred set to a queue; returns each such lane's slot.
+// Must be reached by every lane of the wave.
+__device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const uint32_t lane = laneId();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return base + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ void waveAdd64(unsigned long long* counter, unsigned long long v) {
+    // wave reduction then one atomic
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (laneId() == 0 && v) atomicAdd(counter, v);
+}
+
+// ------------------------------------------------------------------ ray batches
+
+__global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, TraceArgs a, uint32_t n, int any,
+                                                        uint32_t* spill, uint32_t spillStride,
+                                                        unsigned long long* counters) {
+    __shared__ uint32_t lds[kLdsDepth * kBlock];
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    TraceStack st;
+    st.lds = lds + threadIdx.x;
+    st.ldsStride = kBlock;
+    st.spill = spill + gtid;
+    st.spillStride = spillStride;
+    st.ldsDepth = kLdsDepth;
+    uint32_t overflow = 0;
+    unsigned long long nodes = 0, tested = 0;
+    for (uint32_t i = gtid; i < n; i += gridDim.x * kBlock) {
+        RayIn r;
+        r.o = f3(a.ox[i], a.oy[i], a.oz[i]);
+        r.d = f3(a.dx[i], a.dy[i], a.dz[i]);
+        r.tmin = a.tmin[i];
+        r.tmax = a.tmax[i];
+        HitOut h;
+        uint32_t tt = 0;
+        if (any) {
+            nodes += traceRay<true>(sc, r, st, &h, overflow, &tt);
+            a.vis[i] = h.prim >= 0 ? 0.0f : 1.0f;
+        } else {
+            nodes += traceRay<false>(sc, r, st, &h, overflow, &tt);
+            a.t[i] = h.t;
+            a.prim[i] = h.prim;
+            a.b0[i] = h.b0;
+            a.b1[i] = h.b1;
+        }
+        tested += tt;
+    }
+    waveAdd64(&counters[0], nodes);
+    waveAdd64(&counters[1], tested);
+    if (overflow) atomicOr((unsigned int*)&counters[2], 1u);
+}
+
+static uint32_t persistentBlocks(uint32_t work) {
+    uint32_t maxBlocks = (uint32_t)g_numCUs * 8;
+    uint32_t need = (work + kBlock - 1) / kBlock;
+    return std::max(1u, std::min(maxBlocks, need));
+}
+
+int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool devicePtrs, void* stream,
+               double* nodesVisited, double* trisTested, std::string* err) {
+    if (n == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    TraceArgs d = a;
+    std::vector<void*> temps;
+    auto cleanup = [&]() { for (void* p : temps) (void)hipFree(p); };
+    if (!devicePtrs) {
+        // stage host arrays through device memory
+        const float* hin[8] = {a.ox, a.oy, a.oz, a.dx, a.dy, a.dz, a.tmin, a.tmax};
+        const float** din[8] = {&d.ox, &d.oy, &d.oz, &d.dx, &d.dy, &d.dz, &d.tmin, &d.tmax};
+        for (int k = 0; k < 8; ++k) {
+            void* p = nullptr;
+            if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) { cleanup(); if (err) *err = "hipMalloc"; return SLRH_ERR_NOMEM_CODE; }
+            temps.push_back(p);
+            (void)hipMemcpyAsync(p, hin[k], n * sizeof(float), hipMemcpyHostToDevice, s);
+            *din[k] = (const float*)p;
+        }
+        int nout = any ? 1 : 4;
+        for (int k = 0; k < nout; ++k) {
+            void* p = nullptr;
+            if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) { cleanup(); if (err) *err = "hipMalloc"; return SLRH_ERR_NOMEM_CODE; }
+            temps.push_back(p);
+        }
+        if (any) d.vis = (float*)temps[8];
+        else { d.t = (float*)temps[8]; d.prim = (int32_t*)temps[9]; d.b0 = (float*)temps[10]; d.b1 = (float*)temps[11]; }
+    }
+    uint32_t blocks = persistentBlocks(n);
+    size_t spillNeed = (size_t)(kMaxStack - kLdsDepth) * blocks * kBlock;
+    if (ds->spill.n < spillNeed) {
+        int r = ds->spill.alloc(spillNeed, err);
+        if (r) { cleanup(); return r; }
+    }
+    (void)hipMemsetAsync(ds->counters.p, 0, 4 * sizeof(unsigned long long), s);
+    hipLaunchKernelGGL(k_trace_batch, dim3(blocks), dim3(kBlock), 0, s, ds->view, d, n, any ? 1 : 0, ds->spill.p,
+                       blocks * kBlock, ds->counters.p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { cleanup(); if (err) *err = std::string("k_trace_batch: ") + hipGetErrorString(e); return SLRH_ERR_DEVICE_CODE; }
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    (void)hipMemcpyAsync(cnt, ds->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s);
+    if (!devicePtrs) {
+        if (any) (void)hipMemcpyAsync(a.vis, d.vis, n * sizeof(float), hipMemcpyDeviceToHost, s);
+        else {
+            (void)hipMemcpyAsync(a.t, d.t, n * sizeof(float), hipMemcpyDeviceToHost, s);
+            (void)hipMemcpyAsync(a.prim, d.prim, n * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+            (void)hipMemcpyAsync(a.b0, d.b0, n * sizeof(float), hipMemcpyDeviceToHost, s);
+            (void)hipMemcpyAsync(a.b1, d.b1, n * sizeof(float), hipMemcpyDeviceToHost, s);
+        }
+    }
+    e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) { if (err) *err = std::string("trace sync: ") + hipGetErrorString(e); return SLRH_ERR_DEVICE_CODE; }
+    if (nodesVisited) *nodesVisited = (double)cnt[0];
+    if (trisTested) *trisTested = (double)cnt[1];
+    if (cnt[2]) { if (err) *err = "traversal stack overflow (depth > 64)"; return SLRH_ERR_CAPACITY_CODE; }
+    return 0;
+}
+
+// ------------------------------------------------------------------ films
+
+int filmCreate(uint32_t w, uint32_t h, DeviceFilm** out, std::string* err) {
+    std::unique_ptr<DeviceFilm> f(new DeviceFilm());
+    f->w = w; f->h = h;
+    int r = f->data.alloc((size_t)w * h * kNumL, err);
+    if (r) return r;
+    HIP_TRY(hipMemset(f->data.p, 0, (size_t)w * h * kNumL * sizeof(double)));
+    HIP_TRY(hipDeviceSynchronize());
+    *out = f.release();
+    return 0;
+}
+void filmFree(DeviceFilm* f) { delete f; }
+int filmClear(DeviceFilm* f, std::string* err) {
+    HIP_TRY(hipMemset(f->data.p, 0, (size_t)f->w * f->h * kNumL * sizeof(double)));
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+int filmDownload(const DeviceFilm* f, std::vector<double>* strata, std::string* err) {
+    strata->resize((size_t)f->w * f->h * kNumL);
+    HIP_TRY(hipMemcpy(strata->data(), f->data.p, strata->size() * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h) { *w = f->w; *h = f->h; }
+
+// ------------------------------------------------------------------ path state
+
+// Counter layout (uint32 each).
+enum Ctr : uint32_t {
+    C_EXT0 = 0, C_EXT1 = 1, C_DONE0 = 2, C_DONE1 = 3,
+    C_CLS0 = 4,   // .. C_CLS0 + kNumClasses - 1
+    C_SHADOW = 8,
+    C_NONFINITE = 9,
+    C_OVERFLOW = 10,
+    kNumCtr = 16
+};
+// 64-bit statistics
+enum Stat64 : uint32_t { S_RAYS_EXT = 0, S_RAYS_SHADOW = 1, S_NODES = 2, S_TRIS = 3, S_SAMPLES = 4, kNumStat = 8 };
+
+// flags in info.y
+constexpr uint32_t F_HAS_SAMPLE = 1u;    // slot holds a sample whose radiance must be filed
+constexpr uint32_t F_LAST_DELTA = 2u;    // last BSDF sample was a delta lobe
+
+struct PathState {
+    uint32_t n;          // slots
+    uint32_t pixels;     // W*H
+    float4 *ro, *rd;     // ray origin.xyz + tmin, direction.xyz
+    float4 *alpha, *sp, *nee;   // 4 planes each: plane p of slot s at [p*n + s]
+    float4 *so, *sd;     // shadow ray origin.xyz + tmin, direction.xyz + tmax
+    uint4* rng;
+    float4* misc;        // wl offset, initY, last BSDF pdf, camera weight
+    uint4* info;         // pixel, flags | hero << 8 | pathLength << 16, samples started, samples total
+    float4* hit;         // t, prim, b0, b1
+    double* film;        // [planes][pixels][16]
+    uint32_t* q;         // queues, each n entries: ext0, ext1, done0, done1, cls0..3, shadow
+    uint32_t* ctr;
+    unsigned long long* stat;
+    uint32_t* spill;
+    uint32_t spillStride;
+    uint32_t maxDepth;
+};
+
+enum QueueId : uint32_t { Q_EXT0 = 0, Q_EXT1 = 1, Q_DONE0 = 2, Q_DONE1 = 3, Q_CLS0 = 4, Q_SHADOW = 8, kNumQueues = 9 };
+
+__device__ __forceinline__ uint32_t* queue(const PathState& ps, uint32_t id) { return ps.q + (size_t)id * ps.n; }
+
+// XORShift128 (libSLR/RNG/XORShiftRNG.cpp:21-36) and its float conversion
+// (libSLR/Core/random_number_generator.cpp:12-15).
+__device__ __forceinline__ uint32_t xsNext(uint4& s) {
+    uint32_t t = s.x ^ (s.x << 11);
+    s.x = s.y; s.y = s.z; s.z = s.w;
+    s.w = (s.w ^ (s.w >> 19)) ^ (t ^ (t >> 8));
+    return s.w;
+}
+__device__ __forceinline__ float xsFloat(uint4& s) {
+    uint32_t bits = (xsNext(s) >> 9) | 0x3f800000u;
+    return __uint_as_float(bits) - 1.0f;
+}
+__device__ __forceinline__ uint4 xsSeed(uint32_t seed) {
+    uint32_t st[4];
+    for (int i = 0; i < 4; ++i) st[i] = seed = 1812433253u * (seed ^ (seed >> 30)) + (uint32_t)i;
+    uint4 s = make_uint4(st[0], st[1], st[2], st[3]);
+    for (int i = 0; i < 50; ++i) xsNext(s);
+    return s;
+}
+
+__device__ __forceinline__ void loadPlanes(const float4* base, uint32_t n, uint32_t s, float* v) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        float4 a = base[(size_t)p * n + s];
+        v[4 * p] = a.x; v[4 * p + 1] = a.y; v[4 * p + 2] = a.z; v[4 * p + 3] = a.w;
+    }
+}
+__device__ __forceinline__ void storePlanes(float4* base, uint32_t n, uint32_t s, const float* v) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) base[(size_t)p * n + s] = make_float4(v[4 * p], v[4 * p + 1], v[4 * p + 2], v[4 * p + 3]);
+}
+
+// wavelength i of a path: createWithEqualOffsets (libSLR/BasicTypes/spectrum_types.h:59-69)
+__device__ __forceinline__ float lambdaOf(int i, float off) {
+    return kLambdaLowD + (kLambdaHighD - kLambdaLowD) * (i + off) / kNumL;
+}
+
+// RegularContinuousSpectrum::evaluate (libSLR/BasicTypes/spectrum_types.cpp:82-100)
+__device__ __forceinline__ void evalSpectrum(const DevScene& sc, int32_t id, float off, float* out) {
+    const DevSpectrum sp = sc.spectra[id];
+    const float* v = sc.specPool + sp.offset;
+    const float span = sp.hi - sp.lo;
+    const float nm1 = (float)(sp.n - 1);
+#pragma unroll
+    for (int i = 0; i < kNumL; ++i) {
+        float binF = (lambdaOf(i, off) - sp.lo) / span * nm1;
+        float r;
+        if (binF <= 0.0f) r = v[0];
+        else if (binF >= nm1) r = v[sp.n - 1];
+        else {
+            int32_t b = (int32_t)binF;
+            float t = binF - b;
+            r = (1 - t) * v[b] + t * v[b + 1];
+        }
+        out[i] = r;
+    }
+}
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void k_init_slots(PathState ps, uint32_t spp, uint32_t slotsPerPixel, uint32_t seed) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < ps.n; s += gridDim.x * blockDim.x) {
+        uint32_t pixel = s % ps.pixels;
+        uint32_t plane = s / ps.pixels;
+        uint32_t total = spp > plane ? (spp - plane + slotsPerPixel - 1) / slotsPerPixel : 0;
+        ps.info[s] = make_uint4(pixel, 0u, 0u, total);
+        // one XORShift stream per slot, seeded like XORShiftRNG(seed')
+        uint32_t h = seed ^ (0x9E3779B9u * (s + 1u));
+        h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
+        ps.rng[s] = xsSeed(h);
+        queue(ps, Q_DONE0)[s] = s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int i = 0; i < kNumCtr; ++i) ps.ctr[i] = 0;
+        ps.ctr[C_DONE0] = ps.n;
+        for (int i = 0; i < kNumStat; ++i) ps.stat[i] = 0;
+    }
+}
+
+// Retire finished samples into the film, start the next sample of the slot.
+__global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps, uint32_t parity) {
+    const uint32_t count = ps.ctr[C_DONE0 + parity];
+    const uint32_t* qin = queue(ps, Q_DONE0 + parity);
+    uint32_t* qext = queue(ps, Q_EXT0 + parity);
+    const DevCamera cam = sc.cam;
+    const float recBinWidth = kNumL / (kLambdaHighD - kLambdaLowD);
+    unsigned long long samples = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < count;
+        bool emit = false;
+        uint32_t s = 0;
+        if (valid) {
+            s = qin[i];
+            uint4 info = ps.info[s];
+            float4 misc = ps.misc[s];
+            if (info.y & F_HAS_SAMPLE) {
+                // C *= weight; film add (PTRenderer.cpp:129-138, ImageSensor.cpp:125-130,
+                // SpectrumStorage::add spectrum_types.h:725-736)
+                float c[kNumL];
+                loadPlanes(ps.sp, ps.n, s, c);
+                bool finite = true;
+#pragma unroll
+                for (int k = 0; k < kNumL; ++k) { c[k] *= misc.w; finite = finite && isfinite(c[k]); }
+                if (finite) {
+                    float add[kNumL];
+#pragma unroll
+                    for (int k = 0; k < kNumL; ++k) add[k] = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < kNumL; ++k) {
+                        float l = lambdaOf(k, misc.x);
+                        uint32_t b = min((uint32_t)((l - kLambdaLowD) / (kLambdaHighD - kLambdaLowD) * kNumL), (uint32_t)(kNumL - 1));
+                        add[b] += c[k] * recBinWidth;
+                    }
+                    uint32_t plane = s / ps.pixels;
+                    double* f = ps.film + ((size_t)plane * ps.pixels + info.x) * kNumL;
+#pragma unroll
+                    for (int k = 0; k < kNumL; k += 2) {
+                        double2 v = *(double2*)(f + k);
+                        v.x += (double)add[k];
+                        v.y += (double)add[k + 1];
+                        *(double2*)(f + k) = v;
+                    }
+                } else {
+                    atomicAdd(&ps.ctr[C_NONFINITE], 1u);
+                }
+                ++samples;
+            }
+            info.y &= ~F_HAS_SAMPLE;
+            if (info.z < info.w) {
+                // new camera sample (PTRenderer.cpp:108-124, PerspectiveCamera.cpp:31-82)
+                uint4 rng = ps.rng[s];
+                const uint32_t px = info.x % cam.width, py = info.x / cam.width;
+                const float fx = px + xsFloat(rng);
+                const float fy = py + xsFloat(rng);
+                const float off = xsFloat(rng);
+                const float uL = xsFloat(rng);
+                const uint32_t hero = min((uint32_t)(uint16_t)(kNumL * uL), (uint32_t)(kNumL - 1));
+                const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
+                float lx = 0, ly = 0;
+                concentricDisk(lu0, lu1, &lx, &ly);
+                const float3 orgL = f3(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f);
+                const float* M = cam.l2w;
+                const float3 org = f3(M[0] * orgL.x + M[1] * orgL.y + M[2] * orgL.z + M[3],
+                                      M[4] * orgL.x + M[5] * orgL.y + M[6] * orgL.z + M[7],
+                                      M[8] * orgL.x + M[9] * orgL.y + M[10] * orgL.z + M[11]);
+                const float* N = cam.nrm;
+                const float3 fz = normalize3(f3(N[2], N[5], N[8]));
+                const float3 fxv = normalize3(f3(M[0], M[4], M[8]));
+                const float3 fyv = cross3(fz, fxv);
+                const float ux = fx / cam.width, uy = fy / cam.height;
+                const float3 pFocus = f3(cam.opWidth * (0.5f - ux), cam.opHeight * (0.5f - uy), cam.objDist);
+                const float3 dl = normalize3(pFocus - orgL);
+                const float dirPDF = cam.imgDist * cam.imgDist / ((dl.z * dl.z * dl.z) * cam.imgPlaneArea);
+                const float3 dir = f3(dot3(f3(fxv.x, fyv.x, fz.x), dl), dot3(f3(fxv.y, fyv.y, fz.y), dl),
+                                      dot3(f3(fxv.z, fyv.z, fz.z), dl));
+                const float areaPDF = cam.lensRadius > 0.0f
+                                          ? (float)(1.0 / (3.14159265358979323846 * cam.lensRadius * cam.lensRadius))
+                                          : 1.0f;
+                const float selectWLPDF = kNumL / (kLambdaHighD - kLambdaLowD);
+                const float cosT = fabsf(dot3(dir, fz));
+                const float weight = cosT / (areaPDF * dirPDF * selectWLPDF);
+                float one[kNumL], zero[kNumL];
+#pragma unroll
+                for (int k = 0; k < kNumL; ++k) { one[k] = 1.0f; zero[k] = 0.0f; }
+                storePlanes(ps.alpha, ps.n, s, one);
+                storePlanes(ps.sp, ps.n, s, zero);
+                const float initY = importance16(one, (int)hero);
+                ps.ro[s] = make_float4(org.x, org.y, org.z, 0.0f);
+                ps.rd[s] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+                ps.misc[s] = make_float4(off, initY, 0.0f, weight);
+                ps.rng[s] = rng;
+                info.y = F_HAS_SAMPLE | (hero << 8);   // pathLength 0
+                info.z += 1;
+                emit = true;
+            }
+            ps.info[s] = info;
+        }
+        const uint32_t slot = waveAppend(&ps.ctr[C_EXT0 + parity], emit);
+        if (emit) qext[slot] = s;
+    }
+    waveAdd64(&ps.stat[S_SAMPLES], samples);
+}
+
+// Closest hit for the extension queue, then classification into shading queues.
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PathState ps, uint32_t parity) {
+    __shared__ uint32_t lds[kLdsDepth * kBlock];
+    const uint32_t count = ps.ctr[C_EXT0 + parity];
+    const uint32_t* qin = queue(ps, Q_EXT0 + parity);
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    TraceStack st;
+    st.lds = lds + threadIdx.x;
+    st.ldsStride = kBlock;
+    st.spill = ps.spill + gtid;
+    st.spillStride = ps.spillStride;
+    st.ldsDepth = kLdsDepth;
+    uint32_t overflow = 0;
+    unsigned long long nodes = 0, tested = 0, rays = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < count;
+        uint32_t s = 0, cls = CLS_MISS;
+        if (valid) {
+            s = qin[i];
+            const float4 o = ps.ro[s], d = ps.rd[s];
+            RayIn r;
+            r.o = xyz(o);
+            r.d = xyz(d);
+            r.tmin = o.w;
+            r.tmax = INFINITY;
+            HitOut h;
+            uint32_t tt = 0;
+            nodes += traceRay<false>(sc, r, st, &h, overflow, &tt);
+            tested += tt;
+            ++rays;
+            ps.hit[s] = make_float4(h.t, __int_as_float(h.prim), h.b0, h.b1);
+            if (h.prim >= 0) cls = sc.mats[sc.primVerts[h.prim].w].cls;
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < kNumClasses; ++c) {
+            const bool mine = valid && cls == c;
+            const uint32_t slot = waveAppend(&ps.ctr[C_CLS0 + c], mine);
+            if (mine) queue(ps, Q_CLS0 + c)[slot] = s;
+        }
+    }
+    waveAdd64(&ps.stat[S_RAYS_EXT], rays);
+    waveAdd64(&ps.stat[S_NODES], nodes);
+    waveAdd64(&ps.stat[S_TRIS], tested);
+    if (overflow) atomicOr(&ps.ctr[C_OVERFLOW], 1u);
+}
+
+// Any hit for the shadow queue; unoccluded contributions join the radiance.
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PathState ps) {
+    __shared__ uint32_t lds[kLdsDepth * kBlock];
+    const uint32_t count = ps.ctr[C_SHADOW];
+    const uint32_t* qin = queue(ps, Q_SHADOW);
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    TraceStack st;
+    st.lds = lds + threadIdx.x;
+    st.ldsStride = kBlock;
+    st.spill = ps.spill + gtid;
+    st.spillStride = ps.spillStride;
+    st.ldsDepth = kLdsDepth;
+    uint32_t overflow = 0;
+    unsigned long long nodes = 0, tested = 0, rays = 0;
+    for (uint32_t i = gtid; i < count; i += gridDim.x * kBlock) {
+        const uint32_t s = qin[i];
+        const float4 o = ps.so[s], d = ps.sd[s];
+        RayIn r;
+        r.o = xyz(o);
+        r.d = xyz(d);
+        r.tmin = o.w;
+        r.tmax = d.w;
+        HitOut h;
+        uint32_t tt = 0;
+        nodes += traceRay<true>(sc, r, st, &h, overflow, &tt);
+        tested += tt;
+        ++rays;
+        if (h.prim < 0) {
+            float a[kNumL], c[kNumL];
+            loadPlanes(ps.sp, ps.n, s, a);
+            loadPlanes(ps.nee, ps.n, s, c);
+#pragma unroll
+            for (int k = 0; k < kNumL; ++k) a[k] += c[k];
+            storePlanes(ps.sp, ps.n, s, a);
+        }
+    }
+    waveAdd64(&ps.stat[S_RAYS_SHADOW], rays);
+    waveAdd64(&ps.stat[S_NODES], nodes);
+    waveAdd64(&ps.stat[S_TRIS], tested);
+    if (overflow) atomicOr(&ps.ctr[C_OVERFLOW], 1u);
+}
+
+struct SurfPt {
+    float3 p, gN;         // position, geometric normal (world)
+    float3 sx, sy, sz;    // shading frame
+    uint32_t mat;
+    uint32_t prim;
+};
+
+__device__ __forceinline__ float3 toLocal(const SurfPt& sp, const float3& v) {
+    return f3(dot3(sp.sx, v), dot3(sp.sy, v), dot3(sp.sz, v));
+}
+__device__ __forceinline__ float3 fromLocal(const SurfPt& sp, const float3& v) {
+    return f3(dot3(f3(sp.sx.x, sp.sy.x, sp.sz.x), v), dot3(f3(sp.sx.y, sp.sy.y, sp.sz.y), v),
+              dot3(f3(sp.sx.z, sp.sy.z, sp.sz.z), v));
+}
+
+// TriangleSurfaceShape::calculateSurfacePoint (TriangleSurfaceShape.cpp:301-346)
+__device__ __forceinline__ SurfPt surfaceAt(const DevScene& sc, uint32_t prim, float b0, float b1, float3 o, float3 d,
+                                            float t) {
+    SurfPt s;
+    const uint4 pv = sc.primVerts[prim];
+    const float3 p0 = xyz(sc.vpos[pv.x]), p1 = xyz(sc.vpos[pv.y]), p2 = xyz(sc.vpos[pv.z]);
+    const float3 n0 = xyz(sc.vnrm[pv.x]), n1 = xyz(sc.vnrm[pv.y]), n2 = xyz(sc.vnrm[pv.z]);
+    const float3 t0 = xyz(sc.vtan[pv.x]), t1 = xyz(sc.vtan[pv.y]), t2 = xyz(sc.vtan[pv.z]);
+    const float b2 = 1.0f - b0 - b1;
+    s.sz = normalize3(b0 * n0 + b1 * n1 + b2 * n2);
+    float3 x = b0 * t0 + b1 * t1 + b2 * t2;
+    const float dNT = dot3(s.sz, x);
+    s.sx = normalize3(x - dNT * s.sz);
+    s.sy = cross3(s.sz, s.sx);
+    s.p = o + d * t;
+    s.gN = normalize3(cross3(p1 - p0, p2 - p0));
+    s.mat = pv.w;
+    s.prim = prim;
+    return s;
+}
+
+__device__ __forceinline__ float triArea(const DevScene& sc, uint32_t prim) {
+    const uint4 pv = sc.primVerts[prim];
+    const float3 p0 = xyz(sc.vpos[pv.x]), p1 = xyz(sc.vpos[pv.y]), p2 = xyz(sc.vpos[pv.z]);
+    return 0.5f * length3(cross3(p1 - p0, p2 - p0));
+}
+
+// DiscreteDistribution1D::sample (libSLR/Core/distributions.cpp:138-161)
+__device__ __forceinline__ uint32_t sampleLight(const DevScene& sc, float u, float* prob) {
+    const uint32_t n = sc.numLights;
+    int idx = (int)n;
+    uint32_t d = 1;
+    while ((d << 1) <= n) d <<= 1;   // prevPowerOf2
+    for (; d > 0; d >>= 1)
+        if (idx - (int)d > 0 && sc.lightCdf[idx - d] >= u) idx -= (int)d;
+    --idx;
+    *prob = sc.lightPmf[idx];
+    return (uint32_t)idx;
+}
+
+// Probability of choosing the surface lights over other emitters
+// (Scene::selectSurfaceLight, libSLR/Scene/Scene.cpp:137-155): 1 without an
+// environment light.
+__device__ __forceinline__ float lightSelectScale(const DevScene&) { return 1.0f; }
+
+template <uint32_t CLS>
+__device__ __forceinline__ void evalBsdf(const DevMaterial& m, const float* A, const float* B, const BsdfQuery& q,
+                                         const float3& d, float* fs, float* pdf) {
+    if (CLS == CLS_DIFFUSE) {
+        Diffuse b{A, m.type == MAT_OREN_NAYAR, m.p0, m.p1};
+        b.evaluate(q, d, fs, pdf);
+    } else if (CLS == CLS_CONDUCTOR) {
+        Conductor b{A, B, GGX{m.p0, m.p1}};
+        b.evaluate(q, d, fs, pdf);
+    } else {
+        Dielectric b{A, B, GGX{m.p0, m.p1}};
+        b.evaluate(q, d, fs, pdf);
+    }
+}
+
+template <uint32_t CLS>
+__device__ __forceinline__ bool sampleBsdf(const DevMaterial& m, const float* A, const float* B, const BsdfQuery& q,
+                                           float uc, float u0, float u1, float* fs, BsdfSample* s) {
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) fs[k] = 0.0f;
+    s->pdf = 0.0f;
+    if (CLS == CLS_DIFFUSE) {
+        Diffuse b{A, m.type == MAT_OREN_NAYAR, m.p0, m.p1};
+        return b.sample(q, uc, u0, u1, fs, s);
+    } else if (CLS == CLS_CONDUCTOR) {
+        Conductor b{A, B, GGX{m.p0, m.p1}};
+        return b.sample(q, uc, u0, u1, fs, s);
+    } else {
+        Dielectric b{A, B, GGX{m.p0, m.p1}};
+        return b.sample(q, uc, u0, u1, fs, s);
+    }
+}
+
+// Shading of one queue. CLS selects the scattering model compiled in.
+template <uint32_t CLS>
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps, uint32_t parity) {
+    const uint32_t count = ps.ctr[C_CLS0 + CLS];
+    const uint32_t* qin = queue(ps, Q_CLS0 + CLS);
+    const uint32_t next = parity ^ 1u;
+    for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < count;
+        bool cont = false, shadow = false, done = false;
+        uint32_t s = 0;
+        if (valid) {
+            s = qin[i];
+            done = true;
+            if (CLS != CLS_MISS) {
+                uint4 info = ps.info[s];
+                float4 misc = ps.misc[s];
+                uint4 rng = ps.rng[s];
+                const float4 o4 = ps.ro[s], d4 = ps.rd[s], h = ps.hit[s];
+                const float3 ro = xyz(o4), rd = xyz(d4);
+                const uint32_t prim = (uint32_t)__float_as_int(h.y);
+                const SurfPt sp = surfaceAt(sc, prim, h.z, h.w, ro, rd, h.x);
+                const DevMaterial mat = sc.mats[sp.mat];
+                const int hero = (int)((info.y >> 8) & 0xFFu);
+                uint32_t pathLen = info.y >> 16;
+                float alpha[kNumL], rad[kNumL];
+                loadPlanes(ps.alpha, ps.n, s, alpha);
+                loadPlanes(ps.sp, ps.n, s, rad);
+                const float3 dirOut = toLocal(sp, -rd);
+                // emission (PTRenderer.cpp:162-166 first hit, :247-263 implicit with MIS)
+                if (mat.emit >= 0 && dirOut.z > 0.0f) {
+                    float Le[kNumL];
+                    evalSpectrum(sc, mat.emit, misc.x, Le);
+                    float misW = 1.0f;
+                    if (pathLen > 0 && !(info.y & F_LAST_DELTA)) {
+                        const float bsdfPDF = misc.z;
+                        const float3 dv = sp.p - ro;
+                        const float dist2 = dot3(dv, dv);
+                        const float lightProb = sc.lightPmf[sc.primToLight[prim]] * lightSelectScale(sc);
+                        const float lightPDF = lightProb * (1.0f / triArea(sc, prim)) * dist2 / fabsf(dot3(rd, sp.gN));
+                        misW = (bsdfPDF * bsdfPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+                    }
+                    const float edf = (float)(1.0 / 3.14159265358979323846);
+#pragma unroll
+                    for (int k = 0; k < kNumL; ++k) rad[k] += alpha[k] * (Le[k] * edf) * misW;
+                }
+                bool alive = true;
+                // Russian roulette (PTRenderer.cpp:267-273), from the second vertex on
+                if (pathLen > 0) {
+                    const float p = fminf(importance16(alpha, hero) / misc.y, 1.0f);
+                    if (xsFloat(rng) < p) {
+#pragma unroll
+                        for (int k = 0; k < kNumL; ++k) alpha[k] /= p;
+                    } else alive = false;
+                }
+                if (alive) {
+                    ++pathLen;
+                    if (pathLen >= ps.maxDepth || mat.type == MAT_NONE) alive = false;
+                }
+                if (alive) {
+                    const float3 gL = toLocal(sp, sp.gN);
+                    BsdfQuery q;
+                    q.dir = dirOut;
+                    q.ng = gL;
+                    q.hero = hero;
+                    float A[kNumL], B[kNumL];
+                    evalSpectrum(sc, mat.spec0, misc.x, A);
+                    if (CLS != CLS_DIFFUSE) evalSpectrum(sc, mat.spec1, misc.x, B);
+                    // ---- next event estimation (PTRenderer.cpp:179-218)
+                    {
+                        const float uSel = xsFloat(rng);
+                        const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
+                        float lprob;
+                        const uint32_t li = sampleLight(sc, uSel, &lprob);
+                        const uint32_t lprim = sc.lights[li];
+                        // TriangleSurfaceShape::sample (TriangleSurfaceShape.cpp:357-413)
+                        const float su = sqrtf(lu0);
+                        const float lb0 = 1.0f - su, lb1 = lu1 * su, lb2 = 1.0f - lb0 - lb1;
+                        const uint4 lv = sc.primVerts[lprim];
+                        const float3 q0 = xyz(sc.vpos[lv.x]), q1 = xyz(sc.vpos[lv.y]), q2 = xyz(sc.vpos[lv.z]);
+                        const float3 ln = normalize3(lb0 * xyz(sc.vnrm[lv.x]) + lb1 * xyz(sc.vnrm[lv.y]) +
+                                                     lb2 * xyz(sc.vnrm[lv.z]));
+                        const float3 lp = lb0 * q0 + lb1 * q1 + lb2 * q2;
+                        const float3 lg = normalize3(cross3(q1 - q0, q2 - q0));
+                        const float areaPDF = 1.0f / (0.5f * length3(cross3(q1 - q0, q2 - q0)));
+                        const float3 dv = lp - sp.p;
+                        const float dist2 = dot3(dv, dv);
+                        const float3 sdir = divs(dv, sqrtf(dist2));
+                        const float cosS = fabsf(dot3(sdir, sp.gN));
+                        const float cosL = fabsf(dot3(sdir, lg));
+                        // EDF is evaluated in the light's shading frame: only its z matters
+                        const float lz = dot3(ln, -sdir);
+                        if (cosS * cosL > 0 && lz > 0.0f) {
+                            const DevMaterial lm = sc.mats[lv.w];
+                            float M[kNumL], fs[kNumL];
+                            evalSpectrum(sc, lm.emit, misc.x, M);
+                            const float3 sdL = toLocal(sp, sdir);
+                            float bpdf = 0.0f;
+                            evalBsdf<CLS>(mat, A, B, q, sdL, fs, &bpdf);
+                            const float lightPDF = lprob * lightSelectScale(sc) * areaPDF;
+                            const float bsdfPDF = bpdf * cosL / dist2;
+                            const float mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+                            const float G = cosS * cosL / dist2;
+                            const float scal = G * mis / lightPDF;
+                            const float edf = (float)(1.0 / 3.14159265358979323846);
+                            float c[kNumL];
+                            bool any = false;
+#pragma unroll
+                            for (int k = 0; k < kNumL; ++k) {
+                                c[k] = alpha[k] * (M[k] * edf) * fs[k] * scal;
+                                any = any || c[k] != 0.0f;
+                            }
+                            if (any) {
+                                // Scene::testVisibility segment (libSLR/Scene/Scene.cpp:96-112)
+                                const float dist = length3(lp - sp.p);
+                                const float3 vdir = divs(lp - sp.p, dist);
+                                storePlanes(ps.nee, ps.n, s, c);
+                                ps.so[s] = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
+                                ps.sd[s] = make_float4(vdir.x, vdir.y, vdir.z, dist * (1 - 1e-4f));
+                                shadow = true;
+                            }
+                        }
+                    }
+                    // ---- BSDF sampling (PTRenderer.cpp:221-237)
+                    {
+                        const float uc = xsFloat(rng), u0 = xsFloat(rng), u1 = xsFloat(rng);
+                        float fs[kNumL];
+                        BsdfSample bs;
+                        bool ok = sampleBsdf<CLS>(mat, A, B, q, uc, u0, u1, fs, &bs);
+                        bool nonzero = false;
+#pragma unroll
+                        for (int k = 0; k < kNumL; ++k) nonzero = nonzero || fs[k] != 0.0f;
+                        if (ok && nonzero && bs.pdf != 0.0f) {
+                            const float w = fabsf(dot3(bs.dir, gL)) / bs.pdf;
+#pragma unroll
+                            for (int k = 0; k < kNumL; ++k) alpha[k] *= fs[k] * w;
+                            const float3 nd = fromLocal(sp, bs.dir);
+                            ps.ro[s] = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
+                            ps.rd[s] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                            misc.z = bs.pdf;
+                            info.y &= ~F_LAST_DELTA;
+                            cont = true;
+                        }
+                    }
+                }
+                storePlanes(ps.sp, ps.n, s, rad);
+                if (cont) storePlanes(ps.alpha, ps.n, s, alpha);
+                info.y = (info.y & 0xFFFFu) | (pathLen << 16);
+                ps.info[s] = info;
+                ps.misc[s] = misc;
+                ps.rng[s] = rng;
+                done = !cont;
+            }
+        }
+        {
+            const uint32_t slot = waveAppend(&ps.ctr[C_SHADOW], shadow);
+            if (shadow) queue(ps, Q_SHADOW)[slot] = s;
+        }
+        {
+            const uint32_t slot = waveAppend(&ps.ctr[C_EXT0 + next], cont);
+            if (cont) queue(ps, Q_EXT0 + next)[slot] = s;
+        }
+        {
+            const bool d = valid && done;
+            const uint32_t slot = waveAppend(&ps.ctr[C_DONE0 + next], d);
+            if (d) queue(ps, Q_DONE0 + next)[slot] = s;
+        }
+    }
+}
+
+__global__ void k_reset(PathState ps, uint32_t parity) {
+    if (threadIdx.x == 0) {
+        ps.ctr[C_EXT0 + parity] = 0;
+        ps.ctr[C_DONE0 + parity] = 0;
+        for (int c = 0; c < kNumClasses; ++c) ps.ctr[C_CLS0 + c] = 0;
+        ps.ctr[C_SHADOW] = 0;
+    }
+}
+
+// Sums film planes into the output film in plane order.
+__global__ void k_film_reduce(const double* planes, uint32_t numPlanes, size_t perPlane, double* out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < perPlane; i += (size_t)gridDim.x * blockDim.x) {
+        double acc = out[i];
+        for (uint32_t p = 0; p < numPlanes; ++p) acc += planes[(size_t)p * perPlane + i];
+        out[i] = acc;
+    }
+}
+
+// ------------------------------------------------------------------ render loop
+
+struct RenderState {
+    uint32_t n = 0, pixels = 0, planes = 0, blocks = 0;
+    DevBuf<float4> ro, rd, alpha, sp, nee, so, sd, misc, hit;
+    DevBuf<uint4> rng, info;
+    DevBuf<double> planeFilm;
+    DevBuf<uint32_t> q, ctr, spill;
+    DevBuf<unsigned long long> stat;
+    uint32_t* hostCtr = nullptr;
+    ~RenderState() { if (hostCtr) (void)hipHostFree(hostCtr); }
+};
+
+static int allocState(RenderState* rs, uint32_t n, uint32_t pixels, uint32_t planes, uint32_t blocks,
+                      std::string* err) {
+    int r;
+    if ((r = rs->ro.alloc(n, err)) || (r = rs->rd.alloc(n, err)) || (r = rs->alpha.alloc(4 * (size_t)n, err)) ||
+        (r = rs->sp.alloc(4 * (size_t)n, err)) || (r = rs->nee.alloc(4 * (size_t)n, err)) ||
+        (r = rs->so.alloc(n, err)) || (r = rs->sd.alloc(n, err)) || (r = rs->misc.alloc(n, err)) ||
+        (r = rs->hit.alloc(n, err)) || (r = rs->rng.alloc(n, err)) || (r = rs->info.alloc(n, err)) ||
+        (r = rs->q.alloc((size_t)kNumQueues * n, err)) || (r = rs->ctr.alloc(kNumCtr, err)) ||
+        (r = rs->stat.alloc(kNumStat, err)) ||
+        (r = rs->spill.alloc((size_t)(kMaxStack - kLdsDepth) * blocks * kBlock, err)))
+        return r;
+    if (planes > 1 && (r = rs->planeFilm.alloc((size_t)planes * pixels * kNumL, err))) return r;
+    if (!rs->hostCtr && hipHostMalloc((void**)&rs->hostCtr, 64 * sizeof(uint32_t)) != hipSuccess) {
+        if (err) *err = "hipHostMalloc";
+        return SLRH_ERR_NOMEM_CODE;
+    }
+    rs->n = n; rs->pixels = pixels; rs->planes = planes; rs->blocks = blocks;
+    return 0;
+}
+
+int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err) {
+    if (!ds->hasCamera) { if (err) *err = "scene has no camera"; return -2; }
+    if (film->w != cfg.width || film->h != cfg.height) { if (err) *err = "film size differs from settings"; return -1; }
+    const uint32_t W = cfg.width, H = cfg.height, P = W * H;
+    if (P == 0 || cfg.spp == 0) return 0;
+    uint32_t K = cfg.slotsPerPixel;
+    if (K == 0) {
+        const uint32_t target = 2u << 20;   // about two million live paths fill the chip
+        K = std::max(1u, std::min(cfg.spp, (target + P - 1) / P));
+        K = std::min(K, 16u);
+    }
+    K = std::min(K, cfg.spp);
+    const uint64_t n64 = (uint64_t)K * P;
+    if (n64 > (1ull << 26)) { if (err) *err = "too many path slots"; return SLRH_ERR_CAPACITY_CODE; }
+    const uint32_t N = (uint32_t)n64;
+    const uint32_t blocks = (uint32_t)g_numCUs * 8;
+
+    std::unique_ptr<RenderState> rs(new RenderState());
+    int r = allocState(rs.get(), N, P, K, blocks, err);
+    if (r) return r;
+
+    DevScene sc = ds->view;
+    sc.cam.width = W;
+    sc.cam.height = H;
+    PathState ps;
+    ps.n = N; ps.pixels = P;
+    ps.ro = rs->ro.p; ps.rd = rs->rd.p; ps.alpha = rs->alpha.p; ps.sp = rs->sp.p; ps.nee = rs->nee.p;
+    ps.so = rs->so.p; ps.sd = rs->sd.p; ps.rng = rs->rng.p; ps.misc = rs->misc.p; ps.info = rs->info.p;
+    ps.hit = rs->hit.p;
+    ps.film = K > 1 ? rs->planeFilm.p : film->data.p;
+    ps.q = rs->q.p; ps.ctr = rs->ctr.p; ps.stat = rs->stat.p;
+    ps.spill = rs->spill.p; ps.spillStride = blocks * kBlock;
+    ps.maxDepth = cfg.maxDepth == 0 ? 100u : cfg.maxDepth;
+
+    hipStream_t stream = 0;
+    if (K > 1) HIP_TRY(hipMemsetAsync(rs->planeFilm.p, 0, (size_t)K * P * kNumL * sizeof(double), stream));
+    hipLaunchKernelGGL(k_init_slots, dim3(blocks), dim3(kBlock), 0, stream, ps, cfg.spp, K, cfg.seed);
+    HIP_TRY(hipGetLastError());
+
+    std::vector<hipEvent_t> evs;
+    auto newEvent = [&]() { hipEvent_t e; (void)hipEventCreate(&e); evs.push_back(e); return e; };
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> extEv, shEv;
+    hipEvent_t tStart = newEvent(), tEnd = newEvent();
+    HIP_TRY(hipEventRecord(tStart, stream));
+
+    uint32_t it = 0;
+    const uint32_t checkEvery = 8;
+    for (;; ++it) {
+        const uint32_t par = it & 1u;
+        hipLaunchKernelGGL(k_regen, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (cfg.timeKernels) { e0 = newEvent(); HIP_TRY(hipEventRecord(e0, stream)); }
+        hipLaunchKernelGGL(k_extend, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
+        if (cfg.timeKernels) { e1 = newEvent(); HIP_TRY(hipEventRecord(e1, stream)); extEv.push_back({e0, e1}); }
+        hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
+        hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
+        hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
+        hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
+        if (cfg.timeKernels) { e0 = newEvent(); HIP_TRY(hipEventRecord(e0, stream)); }
+        hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
+        if (cfg.timeKernels) { e1 = newEvent(); HIP_TRY(hipEventRecord(e1, stream)); shEv.push_back({e0, e1}); }
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, ps, par);
+        HIP_TRY(hipGetLastError());
+        if ((it + 1) % checkEvery == 0) {
+            HIP_TRY(hipMemcpyAsync(rs->hostCtr, rs->ctr.p, kNumCtr * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            const uint32_t nx = par ^ 1u;
+            if (rs->hostCtr[C_EXT0 + nx] == 0 && rs->hostCtr[C_DONE0 + nx] == 0) break;
+            if (rs->hostCtr[C_OVERFLOW]) { if (err) *err = "traversal stack overflow (depth > 64)"; break; }
+        }
+    }
+    if (K > 1) {
+        hipLaunchKernelGGL(k_film_reduce, dim3(blocks), dim3(kBlock), 0, stream, rs->planeFilm.p, K,
+                           (size_t)P * kNumL, film->data.p);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(tEnd, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    unsigned long long stat[kNumStat];
+    HIP_TRY(hipMemcpy(stat, rs->stat.p, sizeof(stat), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rs->hostCtr, rs->ctr.p, kNumCtr * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, tStart, tEnd);
+    if (st) {
+        st->raysTraced = (double)(stat[S_RAYS_EXT] + stat[S_RAYS_SHADOW]);
+        st->extendRays = (double)stat[S_RAYS_EXT];
+        st->shadowRays = (double)stat[S_RAYS_SHADOW];
+        st->samples = (double)stat[S_SAMPLES];
+        st->nodesVisited = (double)stat[S_NODES];
+        st->trisTested = (double)stat[S_TRIS];
+        st->kernelSeconds = ms * 1e-3;
+        st->nonFinite = rs->hostCtr[C_NONFINITE];
+        st->overflow = rs->hostCtr[C_OVERFLOW];
+        st->iterations = it + 1;
+        double es = 0, ss = 0;
+        for (auto& p : extEv) { float t = 0; (void)hipEventElapsedTime(&t, p.first, p.second); es += t; }
+        for (auto& p : shEv) { float t = 0; (void)hipEventElapsedTime(&t, p.first, p.second); ss += t; }
+        st->extendSeconds = es * 1e-3;
+        st->shadowSeconds = ss * 1e-3;
+        st->extendLaunches = (double)extEv.size();
+        st->shadowLaunches = (double)shEv.size();
+    }
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    if (rs->hostCtr[C_OVERFLOW]) return SLRH_ERR_CAPACITY_CODE;
+    return 0;
+}
+
+}  // namespace slrh

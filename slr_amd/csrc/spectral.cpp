@@ -1,0 +1,191 @@
+// Host-side spectral helpers: RGB -> spectrum upsampling (Meng et al. 2015)
+// and the colour-matching-function strata of the film.
+//
+// The upsampling follows UpsampledContinuousSpectrum
+// (libSLR/BasicTypes/spectrum_types.cpp:408-543 for the construction and
+// :611-651 for evaluation) step for step, in the same precision: the colour
+// matrices and the xy -> uv map are evaluated in double and rounded to float
+// like the reference's float instantiation does. The device does not walk the
+// basis: mengToRegular() folds the (at most four) weighted basis spectra into
+// one 95-sample table that the kernels interpolate.
+#include "host_scene.h"
+
+#include <algorithm>
+
+#include "spectral_tables.inc"
+
+namespace slrh {
+namespace {
+
+constexpr float kMinWL = 360.0f, kMaxWL = 830.0f;
+constexpr uint32_t kNumWL = 95;
+constexpr uint32_t kGridW = 12, kGridH = 14;
+constexpr float kEqualEnergyReflectance = 0.009355121400914532f;
+
+void computeAdjacents(float u, float v, MengSpectrum* m) {
+    u = std::min(std::max(u, 0.0f), (float)kGridW);
+    v = std::min(std::max(v, 0.0f), (float)kGridH);
+    int32_t ui = (int32_t)u, vi = (int32_t)v;
+    ui = std::min(ui, (int32_t)kGridW - 1);
+    vi = std::min(vi, (int32_t)kGridH - 1);
+    const tables::MengCell& cell = tables::kMengCells[ui + kGridW * vi];
+    const uint8_t* idx = cell.idx;
+    const uint8_t np = cell.numPoints;
+    if (cell.inside) {
+        m->adj = ((uint32_t)idx[3] << 24) | ((uint32_t)idx[2] << 16) | ((uint32_t)idx[1] << 8) | (uint32_t)idx[0];
+        m->s = u - ui;
+        m->t = v - vi;
+        return;
+    }
+    const float ex = u - tables::kMengUV[idx[0]][0];
+    const float ey = v - tables::kMengUV[idx[0]][1];
+    float e0x = tables::kMengUV[idx[1]][0] - tables::kMengUV[idx[0]][0];
+    float e0y = tables::kMengUV[idx[1]][1] - tables::kMengUV[idx[0]][1];
+    float uu = e0x * ey - ex * e0y;
+    for (int i = 1; i < np; ++i) {
+        uint32_t k = idx[i % (np - 1) + 1];
+        float e1x = tables::kMengUV[k][0] - tables::kMengUV[idx[0]][0];
+        float e1y = tables::kMengUV[k][1] - tables::kMengUV[idx[0]][1];
+        float vv = ex * e1y - e1x * ey;
+        const float area = e0x * e1y - e1x * e0y;
+        const float b0 = uu / area;
+        const float b1 = vv / area;
+        float b2 = 1.0f - b0 - b1;
+        m->adj = (0xFFu << 24) | ((uint32_t)idx[0] << 16) | ((uint32_t)idx[i] << 8) | k;
+        m->s = b0;
+        m->t = b1;
+        if (b0 < -1e-6 || b1 < -1e-6 || b2 < -1e-6) {
+            uu = -vv;
+            e0x = e1x;
+            e0y = e1y;
+            continue;
+        }
+        break;
+    }
+}
+
+void weightsOf(const MengSpectrum& m, uint8_t idx[4], float w[4], int* n) {
+    for (int k = 0; k < 4; ++k) idx[k] = (m.adj >> (8 * k)) & 0xFF;
+    if (idx[3] != 0xFF) {
+        w[0] = (1 - m.s) * (1 - m.t);
+        w[1] = m.s * (1 - m.t);
+        w[2] = (1 - m.s) * m.t;
+        w[3] = m.s * m.t;
+        *n = 4;
+    } else {
+        w[0] = m.s;
+        w[1] = m.t;
+        w[2] = 1.0f - m.s - m.t;
+        w[3] = 0;
+        *n = 3;
+    }
+}
+
+}  // namespace
+
+MengSpectrum mengFromRGB(int type, float r, float g, float b) {
+    // linear sRGB -> XYZ: sRGB_E_to_XYZ for reflectances, sRGB_to_XYZ for
+    // light sources (libSLR/BasicTypes/spectrum_base.h:61-79).
+    float X, Y, Z;
+    if (type == 0) {
+        X = (float)(0.4969 * r + 0.3391 * g + 0.1640 * b);
+        Y = (float)(0.2562 * r + 0.6782 * g + 0.0656 * b);
+        Z = (float)(0.0233 * r + 0.1130 * g + 0.8637 * b);
+    } else {
+        X = (float)(0.4124564 * r + 0.3575761 * g + 0.1804375 * b);
+        Y = (float)(0.2126729 * r + 0.7151522 * g + 0.0721750 * b);
+        Z = (float)(0.0193339 * r + 0.1191920 * g + 0.9503041 * b);
+    }
+    MengSpectrum m;
+    float brightness = X + Y + Z;
+    if (brightness == 0) {
+        m.scale = 0;
+        computeAdjacents(6, 4, &m);
+        return m;
+    }
+    float x = X / brightness;
+    float y = Y / brightness;
+    m.scale = brightness / kEqualEnergyReflectance;
+    float u = (float)(16.730260708356887 * x + 7.7801960340706 * y - 2.170152247475828);
+    float v = (float)(-7.530081094743006 * x + 16.192422314095225 * y + 1.1125529268825947);
+    computeAdjacents(u, v, &m);
+    return m;
+}
+
+float mengEvaluate(const MengSpectrum& m, float lambda) {
+    uint8_t idx[4];
+    float w[4];
+    int n;
+    weightsOf(m, idx, w, &n);
+    float p = (lambda - kMinWL) / (kMaxWL - kMinWL);
+    p = std::min(std::max(p, 0.0f), 1.0f);
+    float binF = p * (kNumWL - 1);
+    uint32_t bin = std::min((uint32_t)binF, kNumWL - 1);
+    uint32_t next = std::min(bin + 1, kNumWL - 1);
+    float t = binF - bin;
+    float r = 0.0f;
+    for (int j = 0; j < n; ++j) {
+        const float* s = tables::kMengSpectra[idx[j]];
+        r += w[j] * (s[bin] * (1 - t) + s[next] * t);
+    }
+    return r * m.scale;
+}
+
+HostSpectrum mengToRegular(const MengSpectrum& m) {
+    uint8_t idx[4];
+    float w[4];
+    int n;
+    weightsOf(m, idx, w, &n);
+    HostSpectrum h;
+    h.lo = kMinWL;
+    h.hi = kMaxWL;
+    h.values.resize(kNumWL);
+    for (uint32_t k = 0; k < kNumWL; ++k) {
+        double acc = 0;
+        for (int j = 0; j < n; ++j) acc += (double)w[j] * tables::kMengSpectra[idx[j]][k];
+        h.values[k] = (float)(acc * m.scale);
+    }
+    return h;
+}
+
+// DiscretizedSpectrum::initialize (libSLR/BasicTypes/spectrum_types.cpp:945-990):
+// trapezoidal integration of the 1 nm CMF samples into 16 equal strata.
+void cmfStrata(float xbar[16], float ybar[16], float zbar[16], float* integral) {
+    const int N = tables::kNumCMFSamples;
+    const float* X = tables::kCIE1931_X;
+    const float* Y = tables::kCIE1931_Y;
+    const float* Z = tables::kCIE1931_Z;
+    uint32_t bin = 0;
+    float nextP = float(bin + 1) / 16;
+    float xs = 0, xp = X[0], ys = 0, yp = Y[0], zs = 0, zp = Z[0];
+    const float interval = 1;
+    for (int i = 1; i < N; ++i) {
+        float curP = float(i) / (N - 1);
+        float width = interval;
+        float xc = X[i], yc = Y[i], zc = Z[i];
+        if (curP >= nextP) {
+            width = (curP - nextP) * (kLambdaHigh - kLambdaLow);
+            float t = 1 - width / interval;
+            float xi = xp * (1 - t) + xc * t;
+            float yi = yp * (1 - t) + yc * t;
+            float zi = zp * (1 - t) + zc * t;
+            xs += (xp + xi) * (interval - width) * 0.5f;
+            ys += (yp + yi) * (interval - width) * 0.5f;
+            zs += (zp + zi) * (interval - width) * 0.5f;
+            if (bin < 16) { xbar[bin] = xs; ybar[bin] = ys; zbar[bin] = zs; }
+            xs = ys = zs = 0;
+            xp = xi; yp = yi; zp = zi;
+            ++bin;
+            nextP = float(bin + 1) / 16;
+        }
+        xs += (xp + xc) * width * 0.5f;
+        ys += (yp + yc) * width * 0.5f;
+        zs += (zp + zc) * width * 0.5f;
+        xp = xc; yp = yc; zp = zc;
+    }
+    float integ = 0.0f;
+    for (int i = 0; i < 16; ++i) integ += ybar[i];
+    *integral = integ;
+}
+
+}  // namespace slrh
