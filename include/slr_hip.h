@@ -42,7 +42,9 @@ typedef struct slrh_server slrh_server;
 
 /* ------------------------------------------------------------ lifecycle */
 
-/* Creates a context on HIP device `device`. Replaces the process-wide state
+/* Creates a context on HIP device `device` (device < 0: a host-only context
+ * that assembles scenes and builds/inspects the accelerator but cannot
+ * trace or render). Replaces the process-wide state
  * set up by SLR::initializeColorSystem() (libSLR/BasicTypes/spectrum_base.cpp:224)
  * and the per-render thread pool (libSLR/Helper/ThreadPool.h:17-84). */
 SLRH_API int slrh_create(int device, slrh_context** out);

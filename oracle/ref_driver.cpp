@@ -82,6 +82,22 @@
 
 using namespace SLR;
 
+// libSLR/BasicTypes/spectrum_types.cpp:922-925 (and the double copy below it)
+// writes the CMF-strata statics as explicit specialisations without an
+// initializer. ISO C++ reads those as declarations only, so g++ leaves the
+// symbols undefined; these are the zero-initialised definitions the
+// reference's own compilers produced from the same lines.
+namespace SLR {
+template <> std::unique_ptr<float[]> DiscretizedSpectrumTemplate<float, NumStrataForStorage>::xbar{};
+template <> std::unique_ptr<float[]> DiscretizedSpectrumTemplate<float, NumStrataForStorage>::ybar{};
+template <> std::unique_ptr<float[]> DiscretizedSpectrumTemplate<float, NumStrataForStorage>::zbar{};
+template <> float DiscretizedSpectrumTemplate<float, NumStrataForStorage>::integralCMF{};
+template <> std::unique_ptr<double[]> DiscretizedSpectrumTemplate<double, NumStrataForStorage>::xbar{};
+template <> std::unique_ptr<double[]> DiscretizedSpectrumTemplate<double, NumStrataForStorage>::ybar{};
+template <> std::unique_ptr<double[]> DiscretizedSpectrumTemplate<double, NumStrataForStorage>::zbar{};
+template <> double DiscretizedSpectrumTemplate<double, NumStrataForStorage>::integralCMF{};
+}  // namespace SLR
+
 namespace {
 
 struct RefScene {

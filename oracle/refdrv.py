@@ -1,0 +1,244 @@
+"""ctypes access to the compiled reference renderer (oracle/_ref/libslr_refdrv.so).
+
+Test infrastructure only: tests, the smoke check and the CPU-baseline leg of
+bench.py load it as a checker or a timed baseline. Build it with
+`make -f oracle/ref.mk -j8` where the reference sources are present; the
+built .so travels with the tree.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_ref", "libslr_refdrv.so")
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError("reference harness not built: make -f oracle/ref.mk")
+        L = ctypes.CDLL(LIB)
+        vp, i32, u32, f32, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_float, ctypes.c_double
+        sig = {
+            "refd_init": (None, []),
+            "refd_scene_new": (vp, []),
+            "refd_scene_free": (None, [vp]),
+            "refd_add_material_diffuse_rgb": (i32, [vp, f32, f32, f32, f32]),
+            "refd_add_material_ggx_conductor": (i32, [vp, f32, f32, f32, f32]),
+            "refd_add_material_ggx_dielectric": (i32, [vp, f32, f32, f32, f32]),
+            "refd_add_material_emitter_rgb": (i32, [vp, i32, f32, f32, f32]),
+            "refd_add_mesh": (i32, [vp, i32, vp, vp, vp, vp, i32, vp, i32]),
+            "refd_set_camera": (None, [vp, vp, f32, f32, f32, f32, f32, f32]),
+            "refd_build_accel": (i32, [vp, i32]),
+            "refd_accel_stats": (None, [vp, vp]),
+            "refd_sbvh_dump": (i32, [vp, vp, vp, vp]),
+            "refd_qbvh_dump": (i32, [vp, vp, vp, vp]),
+            "refd_intersect": (i32, [vp, i32, i32, vp, vp, vp, vp, vp]),
+            "refd_visibility": (i32, [vp, i32, i32, vp, vp]),
+            "refd_render": (f64, [vp, i32, i32, i32, u32, i32, vp, i32, vp]),
+            "refd_rng_stream": (None, [u32, i32, vp, vp]),
+            "refd_wavelengths": (None, [f32, f32, vp, vp, vp]),
+            "refd_upsample": (None, [i32, f32, f32, f32, i32, vp, vp]),
+            "refd_upsample16": (None, [i32, f32, f32, f32, vp, vp]),
+            "refd_cmf_strata": (None, [vp, vp, vp, vp]),
+            "refd_film_add": (None, [vp, vp, vp]),
+            "refd_bsdf": (i32, [i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp]),
+            "refd_sampling": (None, [i32, vp, vp, vp, vp]),
+        }
+        for k, (r, a) in sig.items():
+            fn = getattr(L, k)
+            fn.restype = r
+            fn.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class RefScene:
+    """A scene built from libSLR classes from the same description dict the
+    product consumes (slr_amd.scenes)."""
+
+    def __init__(self, desc):
+        L = lib()
+        self.L = L
+        self.h = L.refd_scene_new()
+        self.desc = desc
+        ids = []
+        for m in desc["materials"]:
+            k = m["kind"]
+            if k == "lambert":
+                ids.append(L.refd_add_material_diffuse_rgb(self.h, *m["rgb"], 0.0))
+            elif k == "oren_nayar":
+                ids.append(L.refd_add_material_diffuse_rgb(self.h, *m["rgb"], float(m["sigma"])))
+            elif k == "ggx_conductor":
+                ids.append(L.refd_add_material_ggx_conductor(self.h, m["alpha"], m["alpha"], m["eta"], m["k"]))
+            elif k == "ggx_dielectric":
+                ids.append(L.refd_add_material_ggx_dielectric(self.h, m["alpha"], m["alpha"], m["eta_ext"],
+                                                              m["eta_int"]))
+            elif k == "emitter":
+                base = ids[m["base"]] if m.get("base", -1) >= 0 else -1
+                ids.append(L.refd_add_material_emitter_rgb(self.h, base, *m["rgb"]))
+        self._keep = []
+        for mesh in desc["meshes"]:
+            mat = mesh["mat"]
+            if not np.isscalar(mat):
+                raise ValueError("reference harness meshes take one material each")
+            pos = np.ascontiguousarray(mesh["pos"], np.float32)
+            nrm = np.ascontiguousarray(mesh["nrm"], np.float32)
+            tan = np.ascontiguousarray(mesh["tan"], np.float32)
+            idx = np.ascontiguousarray(mesh["idx"], np.uint32)
+            self._keep += [pos, nrm, tan, idx]
+            L.refd_add_mesh(self.h, len(pos), _p(pos), _p(nrm), _p(tan), None, len(idx), _p(idx), ids[int(mat)])
+        cam = desc["camera"]
+        m = np.ascontiguousarray(np.asarray(cam["l2w"], np.float32).reshape(4, 4).T)
+        L.refd_set_camera(self.h, _p(m), cam.get("sensitivity", 1.0), cam["aspect"], cam["fov_y"],
+                          cam.get("lens_radius", 0.0), cam.get("img_dist", 1.0), cam.get("obj_dist", 1.0))
+        self.accel_built = False
+
+    def build_accel(self):
+        self.L.refd_build_accel(self.h, 1)
+        self.accel_built = True
+
+    def accel_stats(self):
+        st = np.zeros(8)
+        self.L.refd_accel_stats(self.h, _p(st))
+        keys = ["sbvh_nodes", "sbvh_refs", "sbvh_depth", "sbvh_cost", "qbvh_nodes", "qbvh_refs", "qbvh_depth",
+                "qbvh_cost"]
+        return dict(zip(keys, st.tolist()))
+
+    def sbvh_dump(self):
+        st = self.accel_stats()
+        n, r = int(st["sbvh_nodes"]), int(st["sbvh_refs"])
+        boxes = np.zeros((n, 6), np.float32)
+        info = np.zeros((n, 5), np.uint32)
+        refs = np.zeros(r, np.int32)
+        self.L.refd_sbvh_dump(self.h, _p(boxes), _p(info), _p(refs))
+        return boxes, info, refs
+
+    def qbvh_dump(self):
+        st = self.accel_stats()
+        n, r = int(st["qbvh_nodes"]), int(st["qbvh_refs"])
+        boxes = np.zeros((n, 24), np.float32)
+        info = np.zeros((n, 7), np.uint32)
+        refs = np.zeros(r, np.int32)
+        self.L.refd_qbvh_dump(self.h, _p(boxes), _p(info), _p(refs))
+        return boxes, info, refs
+
+    def intersect(self, rays, accel=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.int32)
+        b0 = np.zeros(n, np.float32)
+        b1 = np.zeros(n, np.float32)
+        self.L.refd_intersect(self.h, accel, n, _p(rays), _p(t), _p(prim), _p(b0), _p(b1))
+        return t, prim, b0, b1
+
+    def visibility(self, rays, accel=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        vis = np.zeros(len(rays), np.float32)
+        self.L.refd_visibility(self.h, accel, len(rays), _p(rays), _p(vis))
+        return vis
+
+    def render(self, width, height, spp, seed, threads=1):
+        """Runs PTRenderer; returns (film (h, w, 16) float32 sums, seconds, build seconds).
+        Cannot be combined with build_accel on the same object."""
+        film = np.zeros((height, width, 16), np.float32)
+        bs = np.zeros(1)
+        sec = self.L.refd_render(self.h, width, height, spp, seed & 0xFFFFFFFF, threads, _p(film), 1, _p(bs))
+        if sec < 0:
+            raise RuntimeError("reference render failed")
+        return film, sec, bs[0]
+
+    def close(self):
+        if self.h:
+            self.L.refd_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rng_stream(seed, n):
+    u = np.zeros(n, np.uint32)
+    f = np.zeros(n, np.float32)
+    lib().refd_rng_stream(seed, n, _p(u), _p(f))
+    return u, f
+
+
+def wavelengths(offset, u):
+    l = np.zeros(16, np.float32)
+    hero = np.zeros(1, np.int32)
+    pdf = np.zeros(1, np.float32)
+    lib().refd_wavelengths(offset, u, _p(l), _p(hero), _p(pdf))
+    return l, int(hero[0]), float(pdf[0])
+
+
+def upsample(kind, rgb, lambdas):
+    lam = np.ascontiguousarray(lambdas, np.float32)
+    out = np.zeros(len(lam), np.float32)
+    lib().refd_upsample(kind, *[float(x) for x in rgb], len(lam), _p(lam), _p(out))
+    return out
+
+
+def upsample16(kind, rgb, lambdas16):
+    lam = np.ascontiguousarray(lambdas16, np.float32)
+    out = np.zeros(16, np.float32)
+    lib().refd_upsample16(kind, *[float(x) for x in rgb], _p(lam), _p(out))
+    return out
+
+
+def cmf_strata():
+    x, y, z = np.zeros(16, np.float32), np.zeros(16, np.float32), np.zeros(16, np.float32)
+    integ = np.zeros(1, np.float32)
+    lib().refd_cmf_strata(_p(x), _p(y), _p(z), _p(integ))
+    return x, y, z, float(integ[0])
+
+
+def film_add(lambdas16, value16):
+    l = np.ascontiguousarray(lambdas16, np.float32)
+    v = np.ascontiguousarray(value16, np.float32)
+    out = np.zeros(16, np.float32)
+    lib().refd_film_add(_p(l), _p(v), _p(out))
+    return out
+
+
+def bsdf(kind, A, B, params, queries, wl, mode, u_or_dir):
+    """kind 0 Lambert, 1 Oren-Nayar, 2 GGX conductor, 3 GGX dielectric; mode 0
+    sample (u (n,3)), 1 evaluate (dir (n,3)). Returns dict of arrays."""
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 6)
+    n = len(q)
+    A = np.ascontiguousarray(A, np.float32)
+    B = None if B is None else np.ascontiguousarray(B, np.float32)
+    p = np.ascontiguousarray(np.resize(np.asarray(params, np.float32), 2))
+    wl = np.ascontiguousarray(np.broadcast_to(np.asarray(wl, np.int32), (n,)))
+    u = np.ascontiguousarray(u_or_dir, np.float32).reshape(-1, 3)
+    od = np.zeros((n, 3), np.float32)
+    op = np.zeros(n, np.float32)
+    fs = np.zeros((n, 16), np.float32)
+    ot = np.zeros(n, np.int32)
+    lib().refd_bsdf(kind, _p(A), _p(B), _p(p), n, _p(q), _p(wl), mode, _p(u), _p(od), _p(op), _p(fs), _p(ot))
+    return {"dir": od, "pdf": op, "fs": fs, "type": ot}
+
+
+def sampling(u):
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+    n = len(u)
+    disk = np.zeros((n, 2), np.float32)
+    hemi = np.zeros((n, 3), np.float32)
+    tri = np.zeros((n, 2), np.float32)
+    lib().refd_sampling(n, _p(u), _p(disk), _p(hemi), _p(tri))
+    return disk, hemi, tri

@@ -1,0 +1,372 @@
+"""Python host side of the MI355X spectral path tracer.
+
+A thin ctypes layer over libslr_hip.so (include/slr_hip.h), shaped like the
+reference's host sequence (HostProgram/main.cpp:22-70): build a Scene from
+meshes, spectra, materials and a PerspectiveCamera, build it (SBVH -> QBVH),
+then PTRenderer(spp).render(scene, settings) into a film. Ray batches go
+through Scene.intersect / Scene.occluded, the QBVH queries.
+
+The library is required: importing works without it, but every call that
+needs it raises RuntimeError when libslr_hip.so has not been built
+(python -m slr_amd.build). There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libslr_hip.so")
+
+SLRH_OK = 0
+SPECTRUM_REFLECTANCE = 0
+SPECTRUM_LIGHT_SOURCE = 1
+MEM_HOST = 0
+MEM_DEVICE = 1
+FILM_STRATA16_F64 = 0
+FILM_XYZ_F64 = 1
+FILM_SRGB8 = 2
+
+EXPORTS = [
+    "slrh_create", "slrh_destroy", "slrh_last_error", "slrh_version",
+    "slrh_spectrum_rgb", "slrh_spectrum_regular",
+    "slrh_material_lambert", "slrh_material_oren_nayar", "slrh_material_ggx_conductor",
+    "slrh_material_ggx_dielectric", "slrh_material_diffuse_emitter",
+    "slrh_mesh_add", "slrh_camera_perspective",
+    "slrh_scene_create", "slrh_scene_build", "slrh_scene_destroy",
+    "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
+    "slrh_intersect", "slrh_occluded",
+    "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read",
+    "slrh_render",
+]
+
+_lib = None
+
+
+class SlrhError(RuntimeError):
+    pass
+
+
+class _Rays(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")]
+
+
+class _Hits(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("t", "prim", "b0", "b1")]
+
+
+class _Settings(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ("width", "height", "spp", "max_depth", "seed", "integrator", "slots_per_pixel", "flags")]
+
+
+def load_library(path=None):
+    """Loads libslr_hip.so; raises RuntimeError if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError("libslr_hip.so not built (%s); run python -m slr_amd.build" % p)
+    lib = ctypes.CDLL(p)
+    vp, i32, u32, f32, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_float, ctypes.c_double
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    pi = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "slrh_create": (i32, [i32, pp]),
+        "slrh_destroy": (i32, [vp]),
+        "slrh_last_error": (ctypes.c_char_p, [vp]),
+        "slrh_version": (i32, []),
+        "slrh_spectrum_rgb": (i32, [vp, i32, f32, f32, f32, pi]),
+        "slrh_spectrum_regular": (i32, [vp, f32, f32, vp, i32, pi]),
+        "slrh_material_lambert": (i32, [vp, i32, pi]),
+        "slrh_material_oren_nayar": (i32, [vp, i32, f32, pi]),
+        "slrh_material_ggx_conductor": (i32, [vp, i32, i32, f32, f32, pi]),
+        "slrh_material_ggx_dielectric": (i32, [vp, i32, i32, f32, f32, pi]),
+        "slrh_material_diffuse_emitter": (i32, [vp, i32, i32, pi]),
+        "slrh_mesh_add": (i32, [vp, u32, vp, vp, vp, u32, vp, vp, i32, pi]),
+        "slrh_camera_perspective": (i32, [vp, vp, f32, f32, f32, f32, f32, f32]),
+        "slrh_scene_create": (i32, [vp, pp]),
+        "slrh_scene_build": (i32, [vp]),
+        "slrh_scene_destroy": (i32, [vp]),
+        "slrh_scene_accel_stats": (i32, [vp, vp]),
+        "slrh_scene_sbvh_dump": (i32, [vp, vp, vp, vp, vp, vp]),
+        "slrh_scene_qbvh_dump": (i32, [vp, vp, vp, vp, vp, vp]),
+        "slrh_intersect": (i32, [vp, vp, vp, u32, i32, vp]),
+        "slrh_occluded": (i32, [vp, vp, vp, u32, i32, vp]),
+        "slrh_film_create": (i32, [vp, u32, u32, pp]),
+        "slrh_film_destroy": (i32, [vp]),
+        "slrh_film_clear": (i32, [vp]),
+        "slrh_film_read": (i32, [vp, i32, f64, vp, ctypes.c_size_t]),
+        "slrh_render": (i32, [vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a
+
+
+class Context:
+    """slrh_create / slrh_destroy: one per HIP device."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        r = self.lib.slrh_create(device, ctypes.byref(h))
+        if r != SLRH_OK:
+            raise SlrhError("slrh_create failed: %d" % r)
+        self.h = h
+
+    def check(self, r, what):
+        if r != SLRH_OK:
+            msg = self.lib.slrh_last_error(self.h)
+            raise SlrhError("%s failed (%d): %s" % (what, r, msg.decode() if msg else ""))
+
+    def close(self):
+        if self.h:
+            self.lib.slrh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Scene assembled through the C ABI (libSLR Scene + nodes + materials)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = ctypes.c_void_p()
+        ctx.check(self.lib.slrh_scene_create(ctx.h, ctypes.byref(h)), "slrh_scene_create")
+        self.h = h
+        self.num_prims = 0
+
+    def _id(self, fn, name, *args):
+        out = ctypes.c_int()
+        self.ctx.check(fn(self.h, *args, ctypes.byref(out)), name)
+        return out.value
+
+    def spectrum_rgb(self, rgb, light=False):
+        t = SPECTRUM_LIGHT_SOURCE if light else SPECTRUM_REFLECTANCE
+        return self._id(self.lib.slrh_spectrum_rgb, "slrh_spectrum_rgb", t, *[float(x) for x in rgb])
+
+    def spectrum_regular(self, lo, hi, values):
+        v = _f32(np.atleast_1d(values))
+        return self._id(self.lib.slrh_spectrum_regular, "slrh_spectrum_regular", float(lo), float(hi), _ptr(v), len(v))
+
+    def spectrum_constant(self, value):
+        return self.spectrum_regular(360.0, 830.0, [value])
+
+    def lambert(self, refl):
+        return self._id(self.lib.slrh_material_lambert, "slrh_material_lambert", refl)
+
+    def oren_nayar(self, refl, sigma):
+        return self._id(self.lib.slrh_material_oren_nayar, "slrh_material_oren_nayar", refl, float(sigma))
+
+    def ggx_conductor(self, eta, k, ax, ay=None):
+        return self._id(self.lib.slrh_material_ggx_conductor, "slrh_material_ggx_conductor", eta, k, float(ax),
+                        float(ax if ay is None else ay))
+
+    def ggx_dielectric(self, eta_ext, eta_int, ax, ay=None):
+        return self._id(self.lib.slrh_material_ggx_dielectric, "slrh_material_ggx_dielectric", eta_ext, eta_int,
+                        float(ax), float(ax if ay is None else ay))
+
+    def diffuse_emitter(self, base, emittance):
+        return self._id(self.lib.slrh_material_diffuse_emitter, "slrh_material_diffuse_emitter", base, emittance)
+
+    def add_mesh(self, pos, idx, mat, nrm=None, tan=None):
+        pos = _f32(pos).reshape(-1, 3)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
+        nrm = None if nrm is None else _f32(nrm).reshape(-1, 3)
+        tan = None if tan is None else _f32(tan).reshape(-1, 3)
+        if np.isscalar(mat):
+            matp, matv = None, int(mat)
+        else:
+            matp, matv = np.ascontiguousarray(mat, dtype=np.uint32), 0
+        mid = self._id(self.lib.slrh_mesh_add, "slrh_mesh_add", len(pos), _ptr(pos), _ptr(nrm), _ptr(tan), len(idx),
+                       _ptr(idx), _ptr(matp), matv)
+        self.num_prims += len(idx)
+        return mid
+
+    def set_camera(self, l2w, aspect, fov_y, lens_radius=0.0, img_dist=1.0, obj_dist=1.0, sensitivity=1.0):
+        m = _f32(np.asarray(l2w, dtype=np.float32).reshape(4, 4).T.copy())  # column-major
+        self.ctx.check(self.lib.slrh_camera_perspective(self.h, _ptr(m), float(sensitivity), float(aspect),
+                                                        float(fov_y), float(lens_radius), float(img_dist),
+                                                        float(obj_dist)), "slrh_camera_perspective")
+
+    def build(self):
+        self.ctx.check(self.lib.slrh_scene_build(self.h), "slrh_scene_build")
+
+    def accel_stats(self):
+        st = np.zeros(9)
+        self.ctx.check(self.lib.slrh_scene_accel_stats(self.h, _ptr(st)), "slrh_scene_accel_stats")
+        keys = ["sbvh_nodes", "sbvh_refs", "sbvh_depth", "sbvh_cost", "qbvh_nodes", "qbvh_refs", "qbvh_depth",
+                "qbvh_cost", "build_seconds"]
+        return dict(zip(keys, st.tolist()))
+
+    def _dump(self, fn, nbox, ninfo, name):
+        nn, nr = ctypes.c_uint32(), ctypes.c_uint32()
+        self.ctx.check(fn(self.h, None, None, None, ctypes.byref(nn), ctypes.byref(nr)), name)
+        boxes = np.zeros((nn.value, nbox), np.float32)
+        info = np.zeros((nn.value, ninfo), np.uint32)
+        refs = np.zeros(nr.value, np.int32)
+        self.ctx.check(fn(self.h, _ptr(boxes), _ptr(info), _ptr(refs), None, None), name)
+        return boxes, info, refs
+
+    def sbvh_dump(self):
+        return self._dump(self.lib.slrh_scene_sbvh_dump, 6, 5, "slrh_scene_sbvh_dump")
+
+    def qbvh_dump(self):
+        return self._dump(self.lib.slrh_scene_qbvh_dump, 24, 7, "slrh_scene_qbvh_dump")
+
+    def intersect(self, rays):
+        """rays: (n, 8) float32 {org xyz, dir xyz, tmin, tmax} on the host.
+        Returns (t, prim, b0, b1)."""
+        rays = _f32(rays).reshape(-1, 8)
+        n = len(rays)
+        cols = [np.ascontiguousarray(rays[:, k]) for k in range(8)]
+        r = _Rays(*[c.ctypes.data for c in cols])
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.int32)
+        b0 = np.zeros(n, np.float32)
+        b1 = np.zeros(n, np.float32)
+        h = _Hits(t.ctypes.data, prim.ctypes.data, b0.ctypes.data, b1.ctypes.data)
+        self.ctx.check(self.lib.slrh_intersect(self.h, ctypes.byref(r), ctypes.byref(h), n, MEM_HOST, None),
+                       "slrh_intersect")
+        return t, prim, b0, b1
+
+    def intersect_device(self, cols, outs, n, stream=None):
+        """Device-pointer form: cols = 8 device addresses, outs = 4 device addresses."""
+        r = _Rays(*cols)
+        h = _Hits(*outs)
+        self.ctx.check(self.lib.slrh_intersect(self.h, ctypes.byref(r), ctypes.byref(h), n, MEM_DEVICE, stream),
+                       "slrh_intersect")
+
+    def occluded(self, rays):
+        rays = _f32(rays).reshape(-1, 8)
+        n = len(rays)
+        cols = [np.ascontiguousarray(rays[:, k]) for k in range(8)]
+        r = _Rays(*[c.ctypes.data for c in cols])
+        vis = np.zeros(n, np.float32)
+        self.ctx.check(self.lib.slrh_occluded(self.h, ctypes.byref(r), _ptr(vis), n, MEM_HOST, None), "slrh_occluded")
+        return vis
+
+    def close(self):
+        if self.h:
+            self.lib.slrh_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Film:
+    """FP64 16-stratum film (ImageSensor)."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.width, self.height = width, height
+        h = ctypes.c_void_p()
+        ctx.check(self.lib.slrh_film_create(ctx.h, width, height, ctypes.byref(h)), "slrh_film_create")
+        self.h = h
+
+    def clear(self):
+        self.ctx.check(self.lib.slrh_film_clear(self.h), "slrh_film_clear")
+
+    def strata(self, scale=1.0):
+        out = np.zeros((self.height, self.width, 16), np.float64)
+        self.ctx.check(self.lib.slrh_film_read(self.h, FILM_STRATA16_F64, float(scale), _ptr(out), out.nbytes),
+                       "slrh_film_read")
+        return out
+
+    def xyz(self, scale=1.0):
+        out = np.zeros((self.height, self.width, 3), np.float64)
+        self.ctx.check(self.lib.slrh_film_read(self.h, FILM_XYZ_F64, float(scale), _ptr(out), out.nbytes),
+                       "slrh_film_read")
+        return out
+
+    def srgb8(self, scale=1.0):
+        out = np.zeros((self.height, self.width, 3), np.uint8)
+        self.ctx.check(self.lib.slrh_film_read(self.h, FILM_SRGB8, float(scale), _ptr(out), out.nbytes),
+                       "slrh_film_read")
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.slrh_film_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "extend_seconds", "extend_launches",
+             "extend_rays", "shadow_seconds", "shadow_launches", "shadow_rays", "nodes_visited", "tris_tested"]
+
+
+class PTRenderer:
+    """PTRenderer(spp).render(scene, settings) (libSLR/Renderer/PTRenderer.h:16-44)."""
+
+    def __init__(self, spp):
+        self.spp = int(spp)
+
+    def render(self, scene, film, seed=1509761209, max_depth=0, slots_per_pixel=0, time_kernels=False):
+        s = _Settings(film.width, film.height, self.spp, max_depth, seed & 0xFFFFFFFF, 0, slots_per_pixel,
+                      1 if time_kernels else 0)
+        st = np.zeros(len(STAT_KEYS))
+        scene.ctx.check(scene.lib.slrh_render(scene.h, ctypes.byref(s), film.h, _ptr(st)), "slrh_render")
+        return dict(zip(STAT_KEYS, st.tolist()))
+
+
+def build_scene(ctx, desc):
+    """Builds a Scene from a description dict produced by slr_amd.scenes."""
+    sc = Scene(ctx)
+    mat_ids = []
+    for m in desc["materials"]:
+        kind = m["kind"]
+        if kind == "lambert":
+            mid = sc.lambert(sc.spectrum_rgb(m["rgb"]))
+        elif kind == "oren_nayar":
+            mid = sc.oren_nayar(sc.spectrum_rgb(m["rgb"]), m["sigma"])
+        elif kind == "ggx_conductor":
+            mid = sc.ggx_conductor(sc.spectrum_constant(m["eta"]), sc.spectrum_constant(m["k"]), m["alpha"])
+        elif kind == "ggx_dielectric":
+            mid = sc.ggx_dielectric(sc.spectrum_constant(m["eta_ext"]), sc.spectrum_constant(m["eta_int"]),
+                                    m["alpha"])
+        elif kind == "emitter":
+            base = mat_ids[m["base"]] if m.get("base", -1) >= 0 else -1
+            mid = sc.diffuse_emitter(base, sc.spectrum_rgb(m["rgb"], light=True))
+        else:
+            raise ValueError(kind)
+        mat_ids.append(mid)
+    for mesh in desc["meshes"]:
+        mat = mesh["mat"]
+        mat = mat_ids[mat] if np.isscalar(mat) else np.asarray(mat_ids, np.uint32)[np.asarray(mat)]
+        sc.add_mesh(mesh["pos"], mesh["idx"], mat, mesh.get("nrm"), mesh.get("tan"))
+    cam = desc["camera"]
+    sc.set_camera(cam["l2w"], cam["aspect"], cam["fov_y"], cam.get("lens_radius", 0.0), cam.get("img_dist", 1.0),
+                  cam.get("obj_dist", 1.0), cam.get("sensitivity", 1.0))
+    sc.build()
+    return sc

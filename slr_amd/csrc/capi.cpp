@@ -89,9 +89,11 @@ int slrh_create(int device, slrh_context** out) {
     std::unique_ptr<slrh_context> c(new (std::nothrow) slrh_context());
     if (!c) return SLRH_ERR_NOMEM;
     c->device = device;
-    std::string err;
-    int r = deviceInit(device, &err);
-    if (r) return r;
+    if (device >= 0) {   // device < 0: host-only context (scene assembly and accelerator inspection)
+        std::string err;
+        int r = deviceInit(device, &err);
+        if (r) return r;
+    }
     *out = c.release();
     return SLRH_OK;
 }
@@ -187,7 +189,7 @@ int slrh_material_diffuse_emitter(slrh_scene* s, int base, int emittance, int* i
         m = s->data.materials[base];
         if (m.emit >= 0) return fail(s->ctx, SLRH_ERR_INVALID, "diffuse_emitter: base material already emits");
     } else {
-        m.type = MAT_NONE; m.spec0 = m.spec1 = -1; m.cls = CLS_MISS;
+        m.type = MAT_NONE; m.spec0 = m.spec1 = -1; m.cls = CLS_DIFFUSE;   // emits, then the path ends
     }
     m.emit = emittance;
     return addMaterial(s, m, id);
@@ -274,9 +276,11 @@ int slrh_scene_build(slrh_scene* s) {
                 for (int j = 0; j < 3; ++j) tp[9 * t + 3 * k + j] = s->data.vpos[3ull * s->data.idx[3 * t + k] + j];
         int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         buildAccel(tp, &s->data.accel, threads);
-        std::string err;
-        int r = uploadScene(s->data, &s->dev, &err);
-        if (r) return fail(s->ctx, r, "scene upload: " + err);
+        if (s->ctx->device >= 0) {
+            std::string err;
+            int r = uploadScene(s->data, &s->dev, &err);
+            if (r) return fail(s->ctx, r, "scene upload: " + err);
+        }
     } catch (const std::bad_alloc&) {
         return fail(s->ctx, SLRH_ERR_NOMEM, "scene build: out of host memory");
     } catch (const std::exception& e) {
@@ -348,6 +352,7 @@ static int doTrace(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, 
                    void* stream) {
     if (!s || !rays) return SLRH_ERR_INVALID;
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "trace before slrh_scene_build");
+    if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
     if (mem != SLRH_MEM_HOST && mem != SLRH_MEM_DEVICE) return fail(s->ctx, SLRH_ERR_INVALID, "bad mem kind");
     if (n == 0) return SLRH_OK;
     TraceArgs a{};
@@ -380,6 +385,7 @@ int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* vis, uint32_t n, 
 
 int slrh_film_create(slrh_context* ctx, uint32_t w, uint32_t h, slrh_film** out) {
     if (!ctx || !out || w == 0 || h == 0) return SLRH_ERR_INVALID;
+    if (ctx->device < 0) return fail(ctx, SLRH_ERR_STATE, "host-only context has no device");
     if ((uint64_t)w * h > (1ull << 27)) return fail(ctx, SLRH_ERR_CAPACITY, "film too large");
     std::unique_ptr<slrh_film> f(new (std::nothrow) slrh_film());
     if (!f) return SLRH_ERR_NOMEM;
@@ -456,6 +462,7 @@ int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size
 int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film, double* stats) {
     if (!s || !set || !film) return SLRH_ERR_INVALID;
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "render before slrh_scene_build");
+    if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
     if (!s->data.hasCamera) return fail(s->ctx, SLRH_ERR_STATE, "scene has no camera");
     if (set->integrator != SLRH_PT) return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
     if (set->width != film->w || set->height != film->h) return fail(s->ctx, SLRH_ERR_INVALID, "film size mismatch");

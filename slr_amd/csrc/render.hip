@@ -474,9 +474,15 @@ __device__ __forceinline__ float xsFloat(uint4& s) {
     uint32_t bits = (xsNext(s) >> 9) | 0x3f800000u;
     return __uint_as_float(bits) - 1.0f;
 }
+// XORShiftRNG(seed): the seed is signed there (Types32bit::Int), so the
+// shift is arithmetic (XORShiftRNG.cpp:21-27, random_number_generator.h:15-19).
 __device__ __forceinline__ uint4 xsSeed(uint32_t seed) {
     uint32_t st[4];
-    for (int i = 0; i < 4; ++i) st[i] = seed = 1812433253u * (seed ^ (seed >> 30)) + (uint32_t)i;
+    int32_t si = (int32_t)seed;
+    for (int i = 0; i < 4; ++i) {
+        st[i] = 1812433253u * (uint32_t)(si ^ (si >> 30)) + (uint32_t)i;
+        si = (int32_t)st[i];
+    }
     uint4 s = make_uint4(st[0], st[1], st[2], st[3]);
     for (int i = 0; i < 50; ++i) xsNext(s);
     return s;

@@ -1,0 +1,245 @@
+"""Seeded synthetic scenes of the benchmark configurations.
+
+Every generator is deterministic for a given seed and returns a plain
+description dict that slr_amd.build_scene (and the test harness around the
+reference renderer) turns into a scene:
+
+    {"materials": [{"kind": ..., ...}, ...],
+     "meshes": [{"pos": (n,3) f32, "nrm": (n,3), "tan": (n,3), "idx": (m,3) u32,
+                 "mat": int or (m,) material indices}, ...],
+     "camera": {"l2w": 4x4, "aspect", "fov_y", "lens_radius", "img_dist", "obj_dist"}}
+
+Geometry is in the reference test scene's units scaled by 0.01
+(TestScenes/Original_Cornell_Box.txt:152-156): the box spans 5.5 units.
+"""
+import math
+
+import numpy as np
+
+SCALE = 0.01
+
+
+def _rot_y(deg):
+    a = math.radians(deg)
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]], np.float64)
+
+
+def _tangent_of(n):
+    # makeCoordinateSystem (libSLR/BasicTypes/Vector3D.h:61-67)
+    n = np.asarray(n, np.float64)
+    sign = np.where(n[..., 2] >= 0, 1.0, -1.0)
+    a = -1.0 / (sign + n[..., 2])
+    b = n[..., 0] * n[..., 1] * a
+    return np.stack([1.0 + sign * n[..., 0] ** 2 * a, sign * b, -sign * n[..., 0]], -1)
+
+
+def _quad(p0, p1, p2, p3, normal):
+    pos = np.array([p0, p1, p2, p3], np.float64)
+    nrm = np.tile(np.asarray(normal, np.float64), (4, 1))
+    return pos, nrm, np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+def _box(center, size, rot_deg):
+    """Axis-aligned box rotated about y; 6 faces, 12 triangles, flat normals."""
+    hx, hy, hz = [s * 0.5 for s in size]
+    R = _rot_y(rot_deg)
+    faces = [
+        ((+1, 0, 0), [(hx, -hy, -hz), (hx, hy, -hz), (hx, hy, hz), (hx, -hy, hz)]),
+        ((-1, 0, 0), [(-hx, -hy, hz), (-hx, hy, hz), (-hx, hy, -hz), (-hx, -hy, -hz)]),
+        ((0, +1, 0), [(-hx, hy, -hz), (-hx, hy, hz), (hx, hy, hz), (hx, hy, -hz)]),
+        ((0, -1, 0), [(-hx, -hy, hz), (-hx, -hy, -hz), (hx, -hy, -hz), (hx, -hy, hz)]),
+        ((0, 0, +1), [(hx, -hy, hz), (hx, hy, hz), (-hx, hy, hz), (-hx, -hy, hz)]),
+        ((0, 0, -1), [(-hx, -hy, -hz), (-hx, hy, -hz), (hx, hy, -hz), (hx, -hy, -hz)]),
+    ]
+    pos, nrm, idx = [], [], []
+    for k, (n, vs) in enumerate(faces):
+        for v in vs:
+            pos.append(R @ np.array(v) + np.asarray(center))
+            nrm.append(R @ np.array(n, np.float64))
+        b = 4 * k
+        idx += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+    return np.array(pos), np.array(nrm), np.array(idx, np.uint32)
+
+
+def _mesh(pos, nrm, idx, mat, scale=SCALE):
+    pos = np.asarray(pos, np.float64) * scale
+    nrm = np.asarray(nrm, np.float64)
+    nrm = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    return {"pos": pos.astype(np.float32), "nrm": nrm.astype(np.float32),
+            "tan": _tangent_of(nrm).astype(np.float32), "idx": np.asarray(idx, np.uint32), "mat": mat}
+
+
+def cornell_box(width=512, height=512, light_rgb=(17.0, 12.0, 4.0)):
+    """Cornell box of 5 walls, two boxes rotated +-18 degrees, one 130x105
+    quad light with diffuse emission, Lambert albedos white 0.75, red
+    (0.63, 0.06, 0.04), green (0.15, 0.48, 0.09); pinhole camera."""
+    W, R, G, L = 0, 1, 2, 3
+    materials = [
+        {"kind": "lambert", "rgb": (0.75, 0.75, 0.75)},
+        {"kind": "lambert", "rgb": (0.63, 0.06, 0.04)},
+        {"kind": "lambert", "rgb": (0.15, 0.48, 0.09)},
+        # the emitter scatters like the white walls: a path that hits the light
+        # keeps going (EmitterSurfaceMaterial over a base material)
+        {"kind": "emitter", "base": 0, "rgb": tuple(light_rgb)},
+    ]
+    S = 550.0
+    meshes = []
+    walls = [
+        (_quad((0, 0, 0), (0, 0, S), (S, 0, S), (S, 0, 0), (0, 1, 0)), W),        # floor
+        (_quad((0, S, 0), (S, S, 0), (S, S, S), (0, S, S), (0, -1, 0)), W),       # ceiling
+        (_quad((0, 0, S), (0, S, S), (S, S, S), (S, 0, S), (0, 0, -1)), W),       # back
+        (_quad((S, 0, 0), (S, 0, S), (S, S, S), (S, S, 0), (-1, 0, 0)), R),       # left (red)
+        (_quad((0, 0, 0), (0, S, 0), (0, S, S), (0, 0, S), (1, 0, 0)), G),        # right (green)
+    ]
+    for (pos, nrm, idx), m in walls:
+        meshes.append(_mesh(pos, nrm, idx, m))
+    ly = S - 1.0
+    pos, nrm, idx = _quad((213, ly, 227), (343, ly, 227), (343, ly, 332), (213, ly, 332), (0, -1, 0))
+    meshes.append(_mesh(pos, nrm, idx, L))
+    for center, size, rot in (((185.5, 82.5, 169.0), (165, 165, 165), -18.0),
+                              ((368.5, 165.0, 351.25), (165, 330, 165), 18.0)):
+        pos, nrm, idx = _box(center, size, rot)
+        meshes.append(_mesh(pos, nrm, idx, W))
+    l2w = np.eye(4)
+    l2w[:3, 3] = np.array([278.0, 273.0, -800.0]) * SCALE
+    camera = {"l2w": l2w, "aspect": width / height, "fov_y": 0.6860478808, "lens_radius": 0.0,
+              "img_dist": 1.0, "obj_dist": 4.5, "sensitivity": 1.0, "width": width, "height": height}
+    return {"materials": materials, "meshes": meshes, "camera": camera}
+
+
+# ------------------------------------------------------------------ icosphere
+
+def icosphere(subdiv):
+    """Unit icosphere: 20 * 4^subdiv triangles, shared vertices."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(subdiv):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        nf = len(f)
+        a = base + inv[:nf]
+        b = base + inv[nf:2 * nf]
+        c = base + inv[2 * nf:]
+        f = np.concatenate([np.stack([f[:, 0], a, c], 1), np.stack([a, f[:, 1], b], 1),
+                            np.stack([c, b, f[:, 2]], 1), np.stack([a, b, c], 1)])
+    return v, f.astype(np.uint32)
+
+
+def _perlin3(p, perm):
+    """Classic improved Perlin noise (Perlin 2002) with a seeded permutation."""
+    pi = np.floor(p).astype(np.int64)
+    pf = p - pi
+    xi, yi, zi = (pi[:, 0] & 255), (pi[:, 1] & 255), (pi[:, 2] & 255)
+    xf, yf, zf = pf[:, 0], pf[:, 1], pf[:, 2]
+
+    def fade(t):
+        return t * t * t * (t * (t * 6 - 15) + 10)
+
+    u, v, w = fade(xf), fade(yf), fade(zf)
+    P = np.concatenate([perm, perm])
+
+    def grad(h, x, y, z):
+        h = h & 15
+        uu = np.where(h < 8, x, y)
+        vv = np.where(h < 4, y, np.where((h == 12) | (h == 14), x, z))
+        return np.where(h & 1, -uu, uu) + np.where(h & 2, -vv, vv)
+
+    A = P[xi] + yi
+    AA, AB = P[A] + zi, P[A + 1] + zi
+    B = P[xi + 1] + yi
+    BA, BB = P[B] + zi, P[B + 1] + zi
+
+    def lerp(t, a, b):
+        return a + t * (b - a)
+
+    x1 = lerp(u, grad(P[AA], xf, yf, zf), grad(P[BA], xf - 1, yf, zf))
+    x2 = lerp(u, grad(P[AB], xf, yf - 1, zf), grad(P[BB], xf - 1, yf - 1, zf))
+    y1 = lerp(v, x1, x2)
+    x1 = lerp(u, grad(P[AA + 1], xf, yf, zf - 1), grad(P[BA + 1], xf - 1, yf, zf - 1))
+    x2 = lerp(u, grad(P[AB + 1], xf, yf - 1, zf - 1), grad(P[BB + 1], xf - 1, yf - 1, zf - 1))
+    y2 = lerp(v, x1, x2)
+    return lerp(w, y1, y2)
+
+
+def fbm(p, seed, octaves=4, lacunarity=2.0, gain=0.5, freq=2.0):
+    rng = np.random.RandomState(seed)
+    perm = rng.permutation(256).astype(np.int64)
+    total = np.zeros(len(p))
+    amp, f = 1.0, freq
+    norm = 0.0
+    for _ in range(octaves):
+        total += amp * _perlin3(p * f, perm)
+        norm += amp
+        amp *= gain
+        f *= lacunarity
+    return total / norm
+
+
+def smooth_normals(pos, idx):
+    p = pos.astype(np.float64)
+    fn = np.cross(p[idx[:, 1]] - p[idx[:, 0]], p[idx[:, 2]] - p[idx[:, 0]])
+    ln = np.linalg.norm(fn, axis=1, keepdims=True)
+    fn = np.where(ln > 0, fn / np.maximum(ln, 1e-30), 0)
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, idx[:, k], fn)
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    return np.where(ln > 0, n / np.maximum(ln, 1e-30), np.array([0.0, 0.0, 1.0]))
+
+
+def displaced_icosphere(subdiv, radius, center, seed=7, amplitude=0.05, octaves=4):
+    """Icosphere with radial displacement amplitude * r * fBm(p) (seeded Perlin)."""
+    v, f = icosphere(subdiv)
+    d = fbm(v, seed, octaves=octaves)
+    p = v * (radius * (1.0 + amplitude * d))[:, None] + np.asarray(center, np.float64)
+    return p, f
+
+
+def cornell_icosphere(width=1920, height=1080, subdiv=8, seed=7):
+    """Cornell box plus a displaced icosphere (GGX conductor, alpha 0.2,
+    eta 0.2, k 3.9), the 1.31M-triangle configuration at subdiv 8."""
+    desc = cornell_box(width, height)
+    desc["materials"].append({"kind": "ggx_conductor", "eta": 0.2, "k": 3.9, "alpha": 0.2})
+    mat = len(desc["materials"]) - 1
+    p, f = displaced_icosphere(subdiv, 100.0, (150.0, 320.0, 380.0), seed=seed)
+    p *= SCALE
+    n = smooth_normals(p, f)
+    desc["meshes"].append({"pos": p.astype(np.float32), "nrm": n.astype(np.float32),
+                           "tan": _tangent_of(n).astype(np.float32), "idx": f, "mat": mat})
+    return desc
+
+
+def random_rays(desc, n, seed=1, mode="mixed"):
+    """Seeded ray batch (n, 8) {org, dir, tmin, tmax} through the scene
+    bounds: origins inside the box, directions uniform on the sphere."""
+    rng = np.random.RandomState(seed)
+    allp = np.concatenate([m["pos"] for m in desc["meshes"]]).astype(np.float64)
+    lo, hi = allp.min(0), allp.max(0)
+    org = lo + (hi - lo) * rng.uniform(0.02, 0.98, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    if mode == "axis":
+        # a share of exactly axis-aligned and signed-zero directions
+        k = n // 4
+        ax = rng.randint(0, 3, size=k)
+        d[:k] = 0.0
+        d[np.arange(k), ax] = rng.choice([-1.0, 1.0], size=k)
+        d[:k][d[:k] == 0.0] = -0.0 * (rng.uniform(size=(d[:k] == 0.0).sum()) < 0.5)
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3] = org
+    r[:, 3:6] = d
+    r[:, 6] = 0.0
+    r[:, 7] = np.inf
+    return r
