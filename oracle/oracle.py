@@ -124,7 +124,10 @@ def reference_spectra(desc):
     for m in desc["materials"]:
         k = m["kind"]
         ent = {"kind": KIND.get(k, 4), "s0": -1, "s1": -1, "emit": -1, "p0": 0.0, "p1": 0.0}
-        if k in ("lambert", "oren_nayar"):
+        if k == "lambert" and "const" in m:
+            specs.append((360.0, 830.0, np.array([m["const"]], np.float32)))
+            ent["s0"] = len(specs) - 1
+        elif k in ("lambert", "oren_nayar"):
             specs.append((360.0, 830.0, refdrv.upsample(0, m["rgb"], GRID)))
             ent["s0"] = len(specs) - 1
             if k == "oren_nayar":
@@ -143,7 +146,10 @@ def reference_spectra(desc):
                 ent = dict(mats[base])
             else:
                 ent["kind"] = 4
-            specs.append((360.0, 830.0, refdrv.upsample(1, m["rgb"], GRID)))
+            if "const" in m:
+                specs.append((360.0, 830.0, np.array([m["const"]], np.float32)))
+            else:
+                specs.append((360.0, 830.0, refdrv.upsample(1, m["rgb"], GRID)))
             ent["emit"] = len(specs) - 1
         mats.append(ent)
     return mats, specs

@@ -346,7 +346,9 @@ def build_scene(ctx, desc):
     mat_ids = []
     for m in desc["materials"]:
         kind = m["kind"]
-        if kind == "lambert":
+        if kind == "lambert" and "const" in m:
+            mid = sc.lambert(sc.spectrum_constant(m["const"]))
+        elif kind == "lambert":
             mid = sc.lambert(sc.spectrum_rgb(m["rgb"]))
         elif kind == "oren_nayar":
             mid = sc.oren_nayar(sc.spectrum_rgb(m["rgb"]), m["sigma"])
@@ -357,7 +359,8 @@ def build_scene(ctx, desc):
                                     m["alpha"])
         elif kind == "emitter":
             base = mat_ids[m["base"]] if m.get("base", -1) >= 0 else -1
-            mid = sc.diffuse_emitter(base, sc.spectrum_rgb(m["rgb"], light=True))
+            le = sc.spectrum_constant(m["const"]) if "const" in m else sc.spectrum_rgb(m["rgb"], light=True)
+            mid = sc.diffuse_emitter(base, le)
         else:
             raise ValueError(kind)
         mat_ids.append(mid)

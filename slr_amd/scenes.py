@@ -108,6 +108,52 @@ def cornell_box(width=512, height=512, light_rgb=(17.0, 12.0, 4.0)):
     return {"materials": materials, "meshes": meshes, "camera": camera}
 
 
+def furnace(width=32, height=32, albedo=0.5, emittance=1.0):
+    """Closed cube, every face a diffuse emitter of constant spectrum
+    `emittance` over a Lambert base of constant `albedo`, camera inside.
+    Radiance everywhere is (emittance / pi) / (1 - albedo)."""
+    materials = [{"kind": "lambert", "const": albedo},
+                 {"kind": "emitter", "base": 0, "const": emittance}]
+    meshes = []
+    S = 100.0
+    faces = [
+        ((0, 0, 0), (0, S, 0), (S, S, 0), (S, 0, 0), (0, 0, 1)),
+        ((0, 0, S), (S, 0, S), (S, S, S), (0, S, S), (0, 0, -1)),
+        ((0, 0, 0), (0, 0, S), (0, S, S), (0, S, 0), (1, 0, 0)),
+        ((S, 0, 0), (S, S, 0), (S, S, S), (S, 0, S), (-1, 0, 0)),
+        ((0, 0, 0), (S, 0, 0), (S, 0, S), (0, 0, S), (0, 1, 0)),
+        ((0, S, 0), (0, S, S), (S, S, S), (S, S, 0), (0, -1, 0)),
+    ]
+    for f in faces:
+        pos, nrm, idx = _quad(*f)
+        meshes.append(_mesh(pos, nrm, idx, 1))
+    l2w = np.eye(4)
+    l2w[:3, 3] = np.array([50.0, 50.0, 20.0]) * SCALE
+    camera = {"l2w": l2w, "aspect": width / height, "fov_y": 1.0, "lens_radius": 0.0, "img_dist": 1.0,
+              "obj_dist": 1.0, "sensitivity": 1.0, "width": width, "height": height}
+    return {"materials": materials, "meshes": meshes, "camera": camera}
+
+
+def pixel_cos4(desc, width, height, n=64):
+    """Per-pixel mean of cos^4 of the pinhole ray against the optical axis
+    times imgPlaneArea / imgDist^2: the film value of a constant unit radiance
+    per sample and stratum (PTRenderer.cpp:129 with PerspectiveIDF's dirPDF,
+    PerspectiveCamera.cpp:71-82)."""
+    cam = desc["camera"]
+    opH = 2.0 * cam["obj_dist"] * math.tan(cam["fov_y"] * 0.5)
+    opW = opH * cam["aspect"]
+    area = opW * opH * (cam["img_dist"] / cam["obj_dist"]) ** 2
+    g = (np.arange(n) + 0.5) / n
+    ux = (np.arange(width)[:, None] + g[None, :]) / width
+    uy = (np.arange(height)[:, None] + g[None, :]) / height
+    x = opW * (0.5 - ux)
+    y = opH * (0.5 - uy)
+    z = cam["obj_dist"]
+    r2 = x[None, :, :, None] ** 2 + y[:, None, None, :] ** 2 + z * z
+    cos4 = (z * z / r2) ** 2
+    return cos4.mean(axis=(2, 3)) * area / cam["img_dist"] ** 2
+
+
 # ------------------------------------------------------------------ icosphere
 
 def icosphere(subdiv):
