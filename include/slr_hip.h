@@ -69,6 +69,14 @@ SLRH_API int slrh_spectrum_rgb(slrh_scene* s, int type, float r, float g, float 
 SLRH_API int slrh_spectrum_regular(slrh_scene* s, float lambda_min, float lambda_max,
                                    const float* values, int n, int* id);
 
+/* Evaluates spectrum `id` at n wavelengths (nm) on the host, as the device
+ * does during rendering (RegularContinuousSpectrum::evaluate,
+ * libSLR/BasicTypes/spectrum_types.cpp:82-100). */
+SLRH_API int slrh_spectrum_evaluate(const slrh_scene* s, int id, int n, const float* lambdas, float* out);
+/* The film's colour-matching strata and their luminance integral
+ * (DiscretizedSpectrum::initialize, libSLR/BasicTypes/spectrum_types.cpp:945-990). */
+SLRH_API int slrh_cmf_strata(float* xbar16, float* ybar16, float* zbar16, float* integral);
+
 /* ------------------------------------------------------------ materials */
 
 /* DiffuseReflectionSurfaceMaterial without sigma -> LambertianBRDF
@@ -184,11 +192,12 @@ SLRH_API int slrh_film_read(const slrh_film* f, int format, double scale, void* 
 
 /* Renders into `film` (accumulating onto what it holds). Replaces
  * PTRenderer::render (libSLR/Renderer/PTRenderer.cpp:26-101). stats, if not
- * NULL, receives 13 doubles: [0] rays traced, [1] samples, [2] device seconds,
+ * NULL, receives 15 doubles: [0] rays traced, [1] samples, [2] device seconds,
  * [3] non-finite samples dropped, [4] wavefront iterations, [5] extension
  * kernel seconds, [6] extension launches, [7] extension rays, [8] shadow
  * kernel seconds, [9] shadow launches, [10] shadow rays, [11] QBVH nodes
- * visited, [12] triangles tested. Kernel seconds are 0 unless flags bit 0. */
+ * visited and [12] triangles tested by extension rays, [13] nodes and [14]
+ * triangles by shadow rays. Kernel seconds are 0 unless flags bit 0. */
 SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
                          double* stats);
 

@@ -432,8 +432,7 @@ double refd_render(void* sp, int w, int h, int spp, uint32_t seed, int threads, 
             for (int k = 0; k < (int)NumStrataForStorage; ++k)
                 film_out[((size_t)y * w + x) * NumStrataForStorage + k] = px[k];
         }
-    scene.destory();
-    root.destroyRenderingData(&DefaultAllocator::instance());
+    scene.destory();   // also releases the node tree's rendering data (Scene.cpp:40-46)
     return std::chrono::duration<double>(t3 - t2).count();
 }
 

@@ -29,7 +29,7 @@ FILM_SRGB8 = 2
 
 EXPORTS = [
     "slrh_create", "slrh_destroy", "slrh_last_error", "slrh_version",
-    "slrh_spectrum_rgb", "slrh_spectrum_regular",
+    "slrh_spectrum_rgb", "slrh_spectrum_regular", "slrh_spectrum_evaluate", "slrh_cmf_strata",
     "slrh_material_lambert", "slrh_material_oren_nayar", "slrh_material_ggx_conductor",
     "slrh_material_ggx_dielectric", "slrh_material_diffuse_emitter",
     "slrh_mesh_add", "slrh_camera_perspective",
@@ -79,6 +79,8 @@ def load_library(path=None):
         "slrh_version": (i32, []),
         "slrh_spectrum_rgb": (i32, [vp, i32, f32, f32, f32, pi]),
         "slrh_spectrum_regular": (i32, [vp, f32, f32, vp, i32, pi]),
+        "slrh_spectrum_evaluate": (i32, [vp, i32, i32, vp, vp]),
+        "slrh_cmf_strata": (i32, [vp, vp, vp, vp]),
         "slrh_material_lambert": (i32, [vp, i32, pi]),
         "slrh_material_oren_nayar": (i32, [vp, i32, f32, pi]),
         "slrh_material_ggx_conductor": (i32, [vp, i32, i32, f32, f32, pi]),
@@ -171,6 +173,13 @@ class Scene:
 
     def spectrum_constant(self, value):
         return self.spectrum_regular(360.0, 830.0, [value])
+
+    def spectrum_evaluate(self, sid, lambdas):
+        lam = _f32(np.atleast_1d(lambdas))
+        out = np.zeros(len(lam), np.float32)
+        self.ctx.check(self.lib.slrh_spectrum_evaluate(self.h, sid, len(lam), _ptr(lam), _ptr(out)),
+                       "slrh_spectrum_evaluate")
+        return out
 
     def lambert(self, refl):
         return self._id(self.lib.slrh_material_lambert, "slrh_material_lambert", refl)
@@ -322,8 +331,17 @@ class Film:
             pass
 
 
+def cmf_strata():
+    lib = load_library()
+    x, y, z = np.zeros(16, np.float32), np.zeros(16, np.float32), np.zeros(16, np.float32)
+    integ = np.zeros(1, np.float32)
+    lib.slrh_cmf_strata(_ptr(x), _ptr(y), _ptr(z), _ptr(integ))
+    return x, y, z, float(integ[0])
+
+
 STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "extend_seconds", "extend_launches",
-             "extend_rays", "shadow_seconds", "shadow_launches", "shadow_rays", "nodes_visited", "tris_tested"]
+             "extend_rays", "shadow_seconds", "shadow_launches", "shadow_rays", "nodes_visited", "tris_tested",
+             "nodes_shadow", "tris_shadow"]
 
 
 class PTRenderer:

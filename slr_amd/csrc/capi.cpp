@@ -142,6 +142,28 @@ int slrh_spectrum_regular(slrh_scene* s, float lo, float hi, const float* values
     return addSpectrum(s, std::move(h), id);
 }
 
+int slrh_spectrum_evaluate(const slrh_scene* s, int id, int n, const float* lambdas, float* out) {
+    if (!s || !lambdas || !out || n < 0) return SLRH_ERR_INVALID;
+    if (!validSpectrum(s, id)) return fail(s->ctx, SLRH_ERR_INVALID, "spectrum_evaluate: bad id");
+    const HostSpectrum& h = s->data.spectra[id];
+    const int m = (int)h.values.size();
+    for (int i = 0; i < n; ++i) {
+        float binF = (lambdas[i] - h.lo) / (h.hi - h.lo) * (m - 1);
+        if (binF <= 0.0f) { out[i] = h.values[0]; continue; }
+        if (binF >= m - 1) { out[i] = h.values[m - 1]; continue; }
+        int b = (int)binF;
+        float t = binF - b;
+        out[i] = (1 - t) * h.values[b] + t * h.values[b + 1];
+    }
+    return SLRH_OK;
+}
+
+int slrh_cmf_strata(float* x, float* y, float* z, float* integral) {
+    if (!x || !y || !z || !integral) return SLRH_ERR_INVALID;
+    cmfStrata(x, y, z, integral);
+    return SLRH_OK;
+}
+
 int slrh_material_lambert(slrh_scene* s, int refl, int* id) {
     CHECK_SCENE(s);
     if (!id || !validSpectrum(s, refl)) return fail(s->ctx, SLRH_ERR_INVALID, "lambert: bad spectrum");
@@ -479,6 +501,7 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
         stats[4] = st.iterations; stats[5] = st.extendSeconds; stats[6] = st.extendLaunches; stats[7] = st.extendRays;
         stats[8] = st.shadowSeconds; stats[9] = st.shadowLaunches; stats[10] = st.shadowRays;
         stats[11] = st.nodesVisited; stats[12] = st.trisTested;
+        stats[13] = st.nodesShadow; stats[14] = st.trisShadow;
     }
     return SLRH_OK;
 }

@@ -47,7 +47,8 @@ struct RenderStats {
     double raysTraced = 0, samples = 0, kernelSeconds = 0, nonFinite = 0, iterations = 0;
     double extendSeconds = 0, extendLaunches = 0, extendRays = 0;
     double shadowSeconds = 0, shadowLaunches = 0, shadowRays = 0;
-    double nodesVisited = 0, trisTested = 0;
+    double nodesVisited = 0, trisTested = 0;   // extension rays
+    double nodesShadow = 0, trisShadow = 0;    // shadow rays
     double overflow = 0;
 };
 

@@ -433,7 +433,10 @@ enum Ctr : uint32_t {
     kNumCtr = 16
 };
 // 64-bit statistics
-enum Stat64 : uint32_t { S_RAYS_EXT = 0, S_RAYS_SHADOW = 1, S_NODES = 2, S_TRIS = 3, S_SAMPLES = 4, kNumStat = 8 };
+enum Stat64 : uint32_t {
+    S_RAYS_EXT = 0, S_RAYS_SHADOW = 1, S_NODES_EXT = 2, S_TRIS_EXT = 3, S_SAMPLES = 4, S_NODES_SH = 5, S_TRIS_SH = 6,
+    kNumStat = 8
+};
 
 // flags in info.y
 constexpr uint32_t F_HAS_SAMPLE = 1u;    // slot holds a sample whose radiance must be filed
@@ -694,8 +697,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PathState ps, ui
         }
     }
     waveAdd64(&ps.stat[S_RAYS_EXT], rays);
-    waveAdd64(&ps.stat[S_NODES], nodes);
-    waveAdd64(&ps.stat[S_TRIS], tested);
+    waveAdd64(&ps.stat[S_NODES_EXT], nodes);
+    waveAdd64(&ps.stat[S_TRIS_EXT], tested);
     if (overflow) atomicOr(&ps.ctr[C_OVERFLOW], 1u);
 }
 
@@ -736,8 +739,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PathState ps) {
         }
     }
     waveAdd64(&ps.stat[S_RAYS_SHADOW], rays);
-    waveAdd64(&ps.stat[S_NODES], nodes);
-    waveAdd64(&ps.stat[S_TRIS], tested);
+    waveAdd64(&ps.stat[S_NODES_SH], nodes);
+    waveAdd64(&ps.stat[S_TRIS_SH], tested);
     if (overflow) atomicOr(&ps.ctr[C_OVERFLOW], 1u);
 }
 
@@ -1143,8 +1146,10 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
         st->extendRays = (double)stat[S_RAYS_EXT];
         st->shadowRays = (double)stat[S_RAYS_SHADOW];
         st->samples = (double)stat[S_SAMPLES];
-        st->nodesVisited = (double)stat[S_NODES];
-        st->trisTested = (double)stat[S_TRIS];
+        st->nodesVisited = (double)stat[S_NODES_EXT];
+        st->trisTested = (double)stat[S_TRIS_EXT];
+        st->nodesShadow = (double)stat[S_NODES_SH];
+        st->trisShadow = (double)stat[S_TRIS_SH];
         st->kernelSeconds = ms * 1e-3;
         st->nonFinite = rs->hostCtr[C_NONFINITE];
         st->overflow = rs->hostCtr[C_OVERFLOW];

@@ -1,0 +1,103 @@
+"""Host side of the product against the compiled reference: accelerator
+topology, spectral upsampling, CMF strata, the C ABI surface and its error
+behaviour. No GPU needed (host-only context, device -1)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import slr_amd
+from slr_amd import scenes
+from tests.conftest import need_ref
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = slr_amd.load_library()
+    header = open(os.path.join(ROOT, "include", "slr_hip.h")).read()
+    declared = re.findall(r"SLRH_API\s+[^;(]*?\b(slrh_\w+)\s*\(", header)
+    assert len(declared) >= 25
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(slr_amd.EXPORTS)
+
+
+@pytest.mark.parametrize("which", ["cornell", "ico3", "ico5"])
+def test_accelerator_matches_reference_bit_for_bit(host_ctx, which):
+    refdrv = need_ref()
+    if which == "cornell":
+        desc = scenes.cornell_box(8, 8)
+    else:
+        desc = scenes.cornell_icosphere(8, 8, subdiv=int(which[-1]))
+    sc = slr_amd.build_scene(host_ctx, desc)
+    ref = refdrv.RefScene(desc)
+    ref.build_accel()
+    a, b = sc.accel_stats(), ref.accel_stats()
+    for k in b:
+        assert a[k] == b[k], (k, a[k], b[k])
+    for mine, theirs in ((sc.sbvh_dump(), ref.sbvh_dump()), (sc.qbvh_dump(), ref.qbvh_dump())):
+        for x, y in zip(mine, theirs):
+            assert x.shape == y.shape
+            assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+
+
+@pytest.mark.parametrize("rgb,kind", [((0.75, 0.75, 0.75), 0), ((0.63, 0.06, 0.04), 0), ((0.15, 0.48, 0.09), 0),
+                                      ((17.0, 12.0, 4.0), 1), ((0.0, 0.0, 0.0), 0), ((1.0, 0.0, 1.0), 0),
+                                      ((0.01, 0.9, 0.9), 0), ((5.0, 5.0, 5.0), 1)])
+def test_rgb_upsampling_matches_reference(host_ctx, rgb, kind):
+    refdrv = need_ref()
+    sc = slr_amd.Scene(host_ctx)
+    sid = sc.spectrum_rgb(rgb, light=(kind == 1))
+    lam = np.random.RandomState(0).uniform(355, 835, 2000).astype(np.float32)
+    mine = sc.spectrum_evaluate(sid, lam)
+    theirs = refdrv.upsample(kind, rgb, lam)
+    np.testing.assert_allclose(mine, theirs, rtol=2e-6, atol=1e-7 * max(1.0, float(np.abs(theirs).max())))
+
+
+def test_cmf_strata_match_reference():
+    refdrv = need_ref()
+    mine = slr_amd.cmf_strata()
+    theirs = refdrv.cmf_strata()
+    for a, b in zip(mine[:3], theirs[:3]):
+        assert np.array_equal(a, b)
+    assert mine[3] == theirs[3]
+
+
+def test_cmf_strata_integrate_the_cie_table():
+    # closed check: the strata partition the 360-830 nm integral of ybar (~106.86 nm)
+    x, y, z, integ = slr_amd.cmf_strata()
+    assert abs(integ - y.sum()) < 1e-4
+    assert 106.0 < integ < 107.5
+
+
+def test_errors_are_status_codes(host_ctx):
+    sc = slr_amd.Scene(host_ctx)
+    with pytest.raises(slr_amd.SlrhError):
+        sc.lambert(5)   # no such spectrum
+    sid = sc.spectrum_constant(0.5)
+    m = sc.lambert(sid)
+    with pytest.raises(slr_amd.SlrhError):
+        sc.add_mesh(np.zeros((3, 3)), np.array([[0, 1, 3]]), m)   # index out of range
+    with pytest.raises(slr_amd.SlrhError):
+        sc.add_mesh(np.full((3, 3), np.nan), np.array([[0, 1, 2]]), m)
+    with pytest.raises(slr_amd.SlrhError):
+        sc.intersect(np.zeros((1, 8), np.float32))   # before build
+    sc.add_mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]]), np.array([[0, 1, 2]]), m)
+    sc.build()
+    with pytest.raises(slr_amd.SlrhError):
+        sc.lambert(sid)   # after build
+    with pytest.raises(slr_amd.SlrhError):
+        sc.intersect(np.zeros((1, 8), np.float32))   # host-only context cannot trace
+
+
+def test_smooth_normals_when_omitted(host_ctx):
+    sc = slr_amd.Scene(host_ctx)
+    m = sc.lambert(sc.spectrum_constant(0.5))
+    p, f = scenes.icosphere(2)
+    sc.add_mesh(p, f, m)
+    sc.build()
+    st = sc.accel_stats()
+    assert st["qbvh_refs"] >= len(f)

@@ -1,0 +1,53 @@
+"""The oracle's integrator against the reference PTRenderer (statistical,
+3 sigma per pixel and XYZ channel over batch means) and against the
+white-furnace closed form."""
+import os
+
+import numpy as np
+
+from oracle import oracle
+from slr_amd import scenes
+from tests.conftest import need_ref
+
+
+def _xyz(strata, cmf):
+    x, y, z, integ = cmf
+    return np.stack([strata @ x, strata @ y, strata @ z], -1) / integ
+
+
+def test_white_furnace_closed_form():
+    W = H = 16
+    spp = 256
+    desc = scenes.furnace(W, H, albedo=0.7, emittance=2.0)
+    film, _ = oracle.OracleScene(desc).render(W, H, spp, 9, threads=8)
+    expect = scenes.pixel_cos4(desc, W, H) * (2.0 / np.pi) / (1 - 0.7) * spp
+    r = film.mean(axis=2) / expect
+    assert abs(r.mean() - 1) < 0.01, r.mean()
+
+
+def test_oracle_matches_reference_renderer():
+    refdrv = need_ref()
+    W = H = 16
+    B, spp = 16, 64
+    desc = scenes.cornell_box(W, H)
+    cmf = refdrv.cmf_strata()
+    threads = max(1, min(8, len(os.sched_getaffinity(0))))
+    rb, ob = [], []
+    orc = oracle.OracleScene(desc)
+    for b in range(B):
+        r = refdrv.RefScene(desc)
+        f, _, _ = r.render(W, H, spp, 500 + b, threads=threads)
+        r.close()
+        rb.append(_xyz(f.astype(np.float64), cmf) / spp)
+        f2, _ = orc.render(W, H, spp, 900 + b, threads=threads)
+        ob.append(_xyz(f2, cmf) / spp)
+    rb, ob = np.array(rb), np.array(ob)
+    mr, mo = rb.mean(0), ob.mean(0)
+    se = np.sqrt(rb.var(0, ddof=1) / B + ob.var(0, ddof=1) / B)
+    z = np.abs(mo - mr) / np.maximum(se, 1e-12)
+    assert np.mean(z > 3) < 0.03
+    # total energy: batch sums are heavy-tailed (light seen through BSDF
+    # sampling), so compare against their own spread rather than a fixed %
+    tr, to = rb.sum(axis=(1, 2, 3)), ob.sum(axis=(1, 2, 3))
+    se_t = np.sqrt(tr.var(ddof=1) / B + to.var(ddof=1) / B)
+    assert abs(to.mean() - tr.mean()) < 4 * se_t
