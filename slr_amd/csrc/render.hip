@@ -425,9 +425,9 @@ void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h) { *w = f->w; *h = f
 
 // Counter layout (uint32 each).
 enum Ctr : uint32_t {
-    C_EXT0 = 0, C_EXT1 = 1, C_DONE0 = 2, C_DONE1 = 3,
-    C_CLS0 = 4,   // .. C_CLS0 + kNumClasses - 1
-    C_SHADOW = 8,
+    C_SHADOW = 0, C_DONE = 1, C_EXT = 2,
+    C_CLS0 = 3,           // .. C_CLS0 + kNumClasses - 1
+    C_EXTPENDING = 7,     // continuing paths counted by the post-shade compaction
     C_NONFINITE = 9,
     C_OVERFLOW = 10,
     kNumCtr = 16
@@ -437,6 +437,9 @@ enum Stat64 : uint32_t {
     S_RAYS_EXT = 0, S_RAYS_SHADOW = 1, S_NODES_EXT = 2, S_TRIS_EXT = 3, S_SAMPLES = 4, S_NODES_SH = 5, S_TRIS_SH = 6,
     kNumStat = 8
 };
+// Per-slot work code, one byte per slot, rewritten by each stage and turned
+// into slot-ordered queues by k_compact.
+enum Code : uint8_t { CODE_IDLE = 0, CODE_EXT = 1, CODE_DONE = 2, CODE_CLS0 = 3, CODE_SHADOW = 0x80 };
 
 // flags in info.y
 constexpr uint32_t F_HAS_SAMPLE = 1u;    // slot holds a sample whose radiance must be filed
@@ -453,7 +456,8 @@ struct PathState {
     uint4* info;         // pixel, flags | hero << 8 | pathLength << 16, samples started, samples total
     float4* hit;         // t, prim, b0, b1
     double* film;        // [planes][pixels][16]
-    uint32_t* q;         // queues, each n entries: ext0, ext1, done0, done1, cls0..3, shadow
+    uint8_t* code;       // per-slot work code (Code)
+    uint32_t* q;         // queues, each n entries: shadow, done, ext, cls0..3
     uint32_t* ctr;
     unsigned long long* stat;
     uint32_t* spill;
@@ -461,7 +465,7 @@ struct PathState {
     uint32_t maxDepth;
 };
 
-enum QueueId : uint32_t { Q_EXT0 = 0, Q_EXT1 = 1, Q_DONE0 = 2, Q_DONE1 = 3, Q_CLS0 = 4, Q_SHADOW = 8, kNumQueues = 9 };
+enum QueueId : uint32_t { Q_SHADOW = 0, Q_DONE = 1, Q_EXT = 2, Q_CLS0 = 3, kNumQueues = 7 };
 
 __device__ __forceinline__ uint32_t* queue(const PathState& ps, uint32_t id) { return ps.q + (size_t)id * ps.n; }
 
@@ -541,20 +545,20 @@ __global__ void k_init_slots(PathState ps, uint32_t spp, uint32_t slotsPerPixel,
         uint32_t h = seed ^ (0x9E3779B9u * (s + 1u));
         h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
         ps.rng[s] = xsSeed(h);
-        queue(ps, Q_DONE0)[s] = s;
+        ps.code[s] = CODE_DONE;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int i = 0; i < kNumCtr; ++i) ps.ctr[i] = 0;
-        ps.ctr[C_DONE0] = ps.n;
         for (int i = 0; i < kNumStat; ++i) ps.stat[i] = 0;
     }
 }
 
 // Retire finished samples into the film, start the next sample of the slot.
-__global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps, uint32_t parity) {
-    const uint32_t count = ps.ctr[C_DONE0 + parity];
-    const uint32_t* qin = queue(ps, Q_DONE0 + parity);
-    uint32_t* qext = queue(ps, Q_EXT0 + parity);
+__global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
+    const uint32_t count = ps.ctr[C_DONE];
+    const uint32_t* qin = queue(ps, Q_DONE);
+    // the extension queue is rebuilt after this kernel
+    if (blockIdx.x == 0 && threadIdx.x == 0) ps.ctr[C_EXT] = 0;
     const DevCamera cam = sc.cam;
     const float recBinWidth = kNumL / (kLambdaHighD - kLambdaLowD);
     unsigned long long samples = 0;
@@ -648,18 +652,19 @@ __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps, uin
                 emit = true;
             }
             ps.info[s] = info;
+            ps.code[s] = emit ? CODE_EXT : CODE_IDLE;
         }
-        const uint32_t slot = waveAppend(&ps.ctr[C_EXT0 + parity], emit);
-        if (emit) qext[slot] = s;
     }
     waveAdd64(&ps.stat[S_SAMPLES], samples);
 }
 
 // Closest hit for the extension queue, then classification into shading queues.
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PathState ps, uint32_t parity) {
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PathState ps) {
     __shared__ uint32_t lds[kLdsDepth * kBlock];
-    const uint32_t count = ps.ctr[C_EXT0 + parity];
-    const uint32_t* qin = queue(ps, Q_EXT0 + parity);
+    const uint32_t count = ps.ctr[C_EXT];
+    const uint32_t* qin = queue(ps, Q_EXT);
+    // the shading queues are rebuilt after this kernel
+    if (blockIdx.x == 0 && threadIdx.x < kNumClasses) ps.ctr[C_CLS0 + threadIdx.x] = 0;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     TraceStack st;
     st.lds = lds + threadIdx.x;
@@ -688,12 +693,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PathState ps, ui
             ++rays;
             ps.hit[s] = make_float4(h.t, __int_as_float(h.prim), h.b0, h.b1);
             if (h.prim >= 0) cls = sc.mats[sc.primVerts[h.prim].w].cls;
-        }
-#pragma unroll
-        for (uint32_t c = 0; c < kNumClasses; ++c) {
-            const bool mine = valid && cls == c;
-            const uint32_t slot = waveAppend(&ps.ctr[C_CLS0 + c], mine);
-            if (mine) queue(ps, Q_CLS0 + c)[slot] = s;
+            ps.code[s] = (uint8_t)(CODE_CLS0 + cls);
         }
     }
     waveAdd64(&ps.stat[S_RAYS_EXT], rays);
@@ -839,10 +839,16 @@ __device__ __forceinline__ bool sampleBsdf(const DevMaterial& m, const float* A,
 
 // Shading of one queue. CLS selects the scattering model compiled in.
 template <uint32_t CLS>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps, uint32_t parity) {
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_CLS0 + CLS];
     const uint32_t* qin = queue(ps, Q_CLS0 + CLS);
-    const uint32_t next = parity ^ 1u;
+    // shadow/done queues of this iteration were consumed before the shading
+    // kernels; they are rebuilt by the compaction that follows them
+    if (CLS == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        ps.ctr[C_SHADOW] = 0;
+        ps.ctr[C_DONE] = 0;
+        ps.ctr[C_EXTPENDING] = 0;
+    }
     for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < count;
@@ -991,28 +997,79 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps, uin
                 done = !cont;
             }
         }
-        {
-            const uint32_t slot = waveAppend(&ps.ctr[C_SHADOW], shadow);
-            if (shadow) queue(ps, Q_SHADOW)[slot] = s;
-        }
-        {
-            const uint32_t slot = waveAppend(&ps.ctr[C_EXT0 + next], cont);
-            if (cont) queue(ps, Q_EXT0 + next)[slot] = s;
-        }
-        {
-            const bool d = valid && done;
-            const uint32_t slot = waveAppend(&ps.ctr[C_DONE0 + next], d);
-            if (d) queue(ps, Q_DONE0 + next)[slot] = s;
-        }
+        if (valid) ps.code[s] = (uint8_t)((cont ? CODE_EXT : CODE_DONE) | (shadow ? CODE_SHADOW : 0));
+        (void)done;
     }
 }
 
-__global__ void k_reset(PathState ps, uint32_t parity) {
-    if (threadIdx.x == 0) {
-        ps.ctr[C_EXT0 + parity] = 0;
-        ps.ctr[C_DONE0 + parity] = 0;
-        for (int c = 0; c < kNumClasses; ++c) ps.ctr[C_CLS0 + c] = 0;
-        ps.ctr[C_SHADOW] = 0;
+// Slot-ordered compaction of the per-slot codes into queues (one pass over
+// all slots; wave prefix sums, a block scan in LDS and one atomic per block
+// and queue). Each block's output is a sorted run of slot ids, so the
+// kernels that read the queues touch path state in near-sequential order.
+//   MODE 0 (after shading): shadow queue, done queue, count of continuing
+//          paths; clears the shadow bit.
+//   MODE 1 (after regeneration): extension queue.
+//   MODE 2 (after extension): one queue per shading class.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_compact(PathState ps) {
+    constexpr int PER = 8;
+    constexpr int NQ = MODE == 0 ? 3 : (MODE == 1 ? 1 : kNumClasses);
+    __shared__ uint32_t waveTot[NQ][4];
+    __shared__ uint32_t blockBase[NQ];
+    const uint32_t first = (blockIdx.x * 256u + threadIdx.x) * PER;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint8_t c[PER];
+    uint32_t cnt[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) cnt[q] = 0;
+    auto match = [](int q, uint8_t v) -> bool {
+        if (MODE == 0) return q == 0 ? (v & CODE_SHADOW) != 0 : q == 1 ? (v & 0x7F) == CODE_DONE : (v & 0x7F) == CODE_EXT;
+        if (MODE == 1) return v == CODE_EXT;
+        return v == CODE_CLS0 + q;
+    };
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t s = first + k;
+        c[k] = s < ps.n ? ps.code[s] : (uint8_t)CODE_IDLE;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) cnt[q] += match(q, c[k]) ? 1u : 0u;
+    }
+    uint32_t incl[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        uint32_t x = cnt[q];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(x, off);
+            if ((int)lane >= off) x += y;
+        }
+        incl[q] = x;
+        if (lane == 63) waveTot[q][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const int q = threadIdx.x;
+        const uint32_t tot = waveTot[q][0] + waveTot[q][1] + waveTot[q][2] + waveTot[q][3];
+        const uint32_t ctr = MODE == 0 ? (q == 0 ? C_SHADOW : q == 1 ? C_DONE : C_EXTPENDING)
+                           : MODE == 1 ? C_EXT : C_CLS0 + q;
+        blockBase[q] = tot ? atomicAdd(&ps.ctr[ctr], tot) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        if (MODE == 0 && q == 2) continue;   // counted, not queued
+        uint32_t off = blockBase[q] + incl[q] - cnt[q];
+        for (uint32_t w = 0; w < wave; ++w) off += waveTot[q][w];
+        const uint32_t qid = MODE == 0 ? (q == 0 ? Q_SHADOW : Q_DONE) : MODE == 1 ? Q_EXT : Q_CLS0 + q;
+        uint32_t* out = queue(ps, qid);
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (match(q, c[k])) out[off++] = first + k;
+    }
+    if (MODE == 0) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (c[k] & CODE_SHADOW) ps.code[first + k] = c[k] & 0x7F;
     }
 }
 
@@ -1033,6 +1090,7 @@ struct RenderState {
     DevBuf<uint4> rng, info;
     DevBuf<double> planeFilm;
     DevBuf<uint32_t> q, ctr, spill;
+    DevBuf<uint8_t> code;
     DevBuf<unsigned long long> stat;
     uint32_t* hostCtr = nullptr;
     ~RenderState() { if (hostCtr) (void)hipHostFree(hostCtr); }
@@ -1046,6 +1104,7 @@ static int allocState(RenderState* rs, uint32_t n, uint32_t pixels, uint32_t pla
         (r = rs->so.alloc(n, err)) || (r = rs->sd.alloc(n, err)) || (r = rs->misc.alloc(n, err)) ||
         (r = rs->hit.alloc(n, err)) || (r = rs->rng.alloc(n, err)) || (r = rs->info.alloc(n, err)) ||
         (r = rs->q.alloc((size_t)kNumQueues * n, err)) || (r = rs->ctr.alloc(kNumCtr, err)) ||
+        (r = rs->code.alloc(n, err)) ||
         (r = rs->stat.alloc(kNumStat, err)) ||
         (r = rs->spill.alloc((size_t)(kMaxStack - kLdsDepth) * blocks * kBlock, err)))
         return r;
@@ -1088,7 +1147,7 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
     ps.so = rs->so.p; ps.sd = rs->sd.p; ps.rng = rs->rng.p; ps.misc = rs->misc.p; ps.info = rs->info.p;
     ps.hit = rs->hit.p;
     ps.film = K > 1 ? rs->planeFilm.p : film->data.p;
-    ps.q = rs->q.p; ps.ctr = rs->ctr.p; ps.stat = rs->stat.p;
+    ps.q = rs->q.p; ps.ctr = rs->ctr.p; ps.stat = rs->stat.p; ps.code = rs->code.p;
     ps.spill = rs->spill.p; ps.spillStride = blocks * kBlock;
     ps.maxDepth = cfg.maxDepth == 0 ? 100u : cfg.maxDepth;
 
@@ -1103,29 +1162,30 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
     hipEvent_t tStart = newEvent(), tEnd = newEvent();
     HIP_TRY(hipEventRecord(tStart, stream));
 
+    const uint32_t cblocks = (N + 2047) / 2048;   // k_compact: 256 threads x 8 slots
+    hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, stream, ps);
     uint32_t it = 0;
     const uint32_t checkEvery = 8;
     for (;; ++it) {
-        const uint32_t par = it & 1u;
-        hipLaunchKernelGGL(k_regen, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (cfg.timeKernels) { e0 = newEvent(); HIP_TRY(hipEventRecord(e0, stream)); }
-        hipLaunchKernelGGL(k_extend, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
-        if (cfg.timeKernels) { e1 = newEvent(); HIP_TRY(hipEventRecord(e1, stream)); extEv.push_back({e0, e1}); }
-        hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
-        hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
-        hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
-        hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps, par);
-        if (cfg.timeKernels) { e0 = newEvent(); HIP_TRY(hipEventRecord(e0, stream)); }
+        if (cfg.timeKernels) { hipEvent_t e0 = newEvent(); HIP_TRY(hipEventRecord(e0, stream)); shEv.push_back({e0, nullptr}); }
         hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
-        if (cfg.timeKernels) { e1 = newEvent(); HIP_TRY(hipEventRecord(e1, stream)); shEv.push_back({e0, e1}); }
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, ps, par);
+        if (cfg.timeKernels) { hipEvent_t e1 = newEvent(); HIP_TRY(hipEventRecord(e1, stream)); shEv.back().second = e1; }
+        hipLaunchKernelGGL(k_regen, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
+        hipLaunchKernelGGL(k_compact<1>, dim3(cblocks), dim3(256), 0, stream, ps);
+        if (cfg.timeKernels) { hipEvent_t e0 = newEvent(); HIP_TRY(hipEventRecord(e0, stream)); extEv.push_back({e0, nullptr}); }
+        hipLaunchKernelGGL(k_extend, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
+        if (cfg.timeKernels) { hipEvent_t e1 = newEvent(); HIP_TRY(hipEventRecord(e1, stream)); extEv.back().second = e1; }
+        hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, stream, ps);
+        hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
+        hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
+        hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
+        hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
+        hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, stream, ps);
         HIP_TRY(hipGetLastError());
         if ((it + 1) % checkEvery == 0) {
             HIP_TRY(hipMemcpyAsync(rs->hostCtr, rs->ctr.p, kNumCtr * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipStreamSynchronize(stream));
-            const uint32_t nx = par ^ 1u;
-            if (rs->hostCtr[C_EXT0 + nx] == 0 && rs->hostCtr[C_DONE0 + nx] == 0) break;
+            if (rs->hostCtr[C_SHADOW] == 0 && rs->hostCtr[C_DONE] == 0 && rs->hostCtr[C_EXTPENDING] == 0) break;
             if (rs->hostCtr[C_OVERFLOW]) { if (err) *err = "traversal stack overflow (depth > 64)"; break; }
         }
     }
