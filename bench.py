@@ -35,10 +35,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(rays, nodes, tris):
-    # per ray: 32 B ray in + 16 B hit out, 128 B per QBVH node visited,
-    # 40 B per triangle tested (36 B positions + 4 B index)
-    return 48.0 * rays + 128.0 * nodes + 40.0 * tris
+def algorithmic_bytes(ext_rays, shadow_rays, nodes, tris):
+    # per extension ray: 4 B queue entry + 32 B ray in + 16 B hit out; per
+    # shadow ray: 4 B entry + 32 B ray; 128 B per QBVH node visited; 40 B per
+    # triangle tested (36 B positions + 4 B primitive id)
+    return 52.0 * ext_rays + 36.0 * shadow_rays + 128.0 * nodes + 40.0 * tris
 
 
 def cpu_baseline(desc, sample, rays_per_sample):
@@ -126,10 +127,11 @@ def main():
     samples = tot["samples"]
     mrays = rays / elapsed / 1e6
     sps = samples / elapsed
-    ext_bytes = algorithmic_bytes(tot["extend_rays"], tot["nodes_visited"], tot["tris_tested"])
-    ext_launches = max(1.0, tot["extend_launches"])
-    ext_avg = tot["extend_seconds"] / ext_launches
-    achieved = ext_bytes / ext_launches / ext_avg / 1e9 if ext_avg > 0 else 0.0
+    tr_bytes = algorithmic_bytes(tot["extend_rays"], tot["shadow_rays"], tot["nodes_visited"] + tot["nodes_shadow"],
+                                 tot["tris_tested"] + tot["tris_shadow"])
+    tr_launches = max(1.0, tot["trace_launches"])
+    tr_avg = tot["trace_seconds"] / tr_launches
+    achieved = tr_bytes / tr_launches / tr_avg / 1e9 if tr_avg > 0 else 0.0
     rays_per_sample = rays / samples
 
     result = {
@@ -151,20 +153,23 @@ def main():
         "rays_per_sample": round(rays_per_sample, 3),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_extend (QBVH closest hit)",
+            "kernel": "k_trace (QBVH closest hit for extension rays + any hit for shadow rays, one launch)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "avg_launch_ms": round(ext_avg * 1e3, 4),
-            "launches": int(ext_launches),
-            "bytes_per_launch": round(ext_bytes / ext_launches),
-            "nodes_per_ray": round(tot["nodes_visited"] / max(1, tot["extend_rays"]), 3),
-            "tris_per_ray": round(tot["tris_tested"] / max(1, tot["extend_rays"]), 3),
+            "avg_launch_ms": round(tr_avg * 1e3, 4),
+            "launches": int(tr_launches),
+            "bytes_per_launch": round(tr_bytes / tr_launches),
+            "nodes_per_ext_ray": round(tot["nodes_visited"] / max(1, tot["extend_rays"]), 3),
+            "tris_per_ext_ray": round(tot["tris_tested"] / max(1, tot["extend_rays"]), 3),
+            "nodes_per_shadow_ray": round(tot["nodes_shadow"] / max(1, tot["shadow_rays"]), 3),
+            "tris_per_shadow_ray": round(tot["tris_shadow"] / max(1, tot["shadow_rays"]), 3),
         },
-        "shadow_kernel": {"avg_launch_ms": round(tot["shadow_seconds"] / max(1, tot["shadow_launches"]) * 1e3, 4),
-                          "nodes_per_ray": round(tot["nodes_shadow"] / max(1, tot["shadow_rays"]), 3)},
+        "frame_ms_breakdown": {"trace": round(tot["trace_seconds"] / args.steps * 1e3, 2),
+                               "shade": round(tot["shade_seconds"] / args.steps * 1e3, 2),
+                               "regen_compact": round(tot["other_seconds"] / args.steps * 1e3, 2)},
         "accel": {"build_seconds": round(st_acc["build_seconds"], 3), "qbvh_nodes": int(st_acc["qbvh_nodes"]),
                   "qbvh_refs": int(st_acc["qbvh_refs"]), "qbvh_sah": round(st_acc["qbvh_cost"], 3)},
         "cpu_baseline": None,

@@ -171,8 +171,14 @@ typedef struct slrh_render_settings {
     uint32_t seed;            /* RenderSettingItem::RNGSeed */
     uint32_t integrator;      /* slrh_integrator */
     uint32_t slots_per_pixel; /* live paths per pixel (0 = automatic) */
-    uint32_t flags;           /* bit 0: time the traversal kernels with HIP events */
+    uint32_t flags;           /* bit 0: time the kernels with HIP events */
+    uint32_t tune_trace;      /* traversal kernel variant + 1 (0 = default) */
+    uint32_t reserved;        /* must be 0 */
 } slrh_render_settings;
+
+/* Traversal kernel variants: n = number of variants; for variant v, its LDS
+ * stack depth per lane and persistent-refill threshold (idle lanes). */
+SLRH_API int slrh_trace_variants(int* n, int* lds_depth, int* refill);
 
 /* Film: FP64 16-stratum spectral sums per pixel, the same quantity the
  * reference's ImageSensor holds (libSLR/Core/ImageSensor.cpp:125-130,
@@ -193,11 +199,12 @@ SLRH_API int slrh_film_read(const slrh_film* f, int format, double scale, void* 
 /* Renders into `film` (accumulating onto what it holds). Replaces
  * PTRenderer::render (libSLR/Renderer/PTRenderer.cpp:26-101). stats, if not
  * NULL, receives 15 doubles: [0] rays traced, [1] samples, [2] device seconds,
- * [3] non-finite samples dropped, [4] wavefront iterations, [5] extension
- * kernel seconds, [6] extension launches, [7] extension rays, [8] shadow
- * kernel seconds, [9] shadow launches, [10] shadow rays, [11] QBVH nodes
- * visited and [12] triangles tested by extension rays, [13] nodes and [14]
- * triangles by shadow rays. Kernel seconds are 0 unless flags bit 0. */
+ * [3] non-finite samples dropped, [4] wavefront iterations, [5] traversal
+ * kernel seconds (extension and shadow rays traced together), [6] traversal
+ * launches, [7] extension rays, [8] shading kernel seconds, [9] seconds of
+ * the other kernels (regeneration, compaction), [10] shadow rays, [11] QBVH
+ * nodes visited and [12] triangles tested by extension rays, [13] nodes and
+ * [14] triangles by shadow rays. Kernel seconds are 0 unless flags bit 0. */
 SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
                          double* stats);
 

@@ -37,7 +37,7 @@ EXPORTS = [
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
     "slrh_intersect", "slrh_occluded",
     "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read",
-    "slrh_render",
+    "slrh_render", "slrh_trace_variants",
 ]
 
 _lib = None
@@ -57,7 +57,8 @@ class _Hits(ctypes.Structure):
 
 class _Settings(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
-                ("width", "height", "spp", "max_depth", "seed", "integrator", "slots_per_pixel", "flags")]
+                ("width", "height", "spp", "max_depth", "seed", "integrator", "slots_per_pixel", "flags",
+                 "tune_trace", "reserved")]
 
 
 def load_library(path=None):
@@ -101,6 +102,7 @@ def load_library(path=None):
         "slrh_film_clear": (i32, [vp]),
         "slrh_film_read": (i32, [vp, i32, f64, vp, ctypes.c_size_t]),
         "slrh_render": (i32, [vp, vp, vp, vp]),
+        "slrh_trace_variants": (i32, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -339,8 +341,18 @@ def cmf_strata():
     return x, y, z, float(integ[0])
 
 
-STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "extend_seconds", "extend_launches",
-             "extend_rays", "shadow_seconds", "shadow_launches", "shadow_rays", "nodes_visited", "tris_tested",
+def trace_variants():
+    """[(lds_depth, refill_threshold)] of the traversal kernel variants."""
+    lib = load_library()
+    n = ctypes.c_int(64)
+    d = np.zeros(64, np.int32)
+    r = np.zeros(64, np.int32)
+    lib.slrh_trace_variants(ctypes.byref(n), _ptr(d), _ptr(r))
+    return list(zip(d[:n.value].tolist(), r[:n.value].tolist()))
+
+
+STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "trace_seconds", "trace_launches",
+             "extend_rays", "shade_seconds", "other_seconds", "shadow_rays", "nodes_visited", "tris_tested",
              "nodes_shadow", "tris_shadow"]
 
 
@@ -350,9 +362,10 @@ class PTRenderer:
     def __init__(self, spp):
         self.spp = int(spp)
 
-    def render(self, scene, film, seed=1509761209, max_depth=0, slots_per_pixel=0, time_kernels=False):
+    def render(self, scene, film, seed=1509761209, max_depth=0, slots_per_pixel=0, time_kernels=False,
+               tune_trace=0):
         s = _Settings(film.width, film.height, self.spp, max_depth, seed & 0xFFFFFFFF, 0, slots_per_pixel,
-                      1 if time_kernels else 0)
+                      1 if time_kernels else 0, tune_trace, 0)
         st = np.zeros(len(STAT_KEYS))
         scene.ctx.check(scene.lib.slrh_render(scene.h, ctypes.byref(s), film.h, _ptr(st)), "slrh_render")
         return dict(zip(STAT_KEYS, st.tolist()))

@@ -481,6 +481,15 @@ int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size
     return SLRH_OK;
 }
 
+int slrh_trace_variants(int* n, int* ldsDepth, int* refill) {
+    if (!n) return SLRH_ERR_INVALID;
+    const int m = numTraceVariants();
+    if (ldsDepth && refill)
+        for (int v = 0; v < std::min(*n, m); ++v) traceVariant(v, &ldsDepth[v], &refill[v]);
+    *n = m;
+    return SLRH_OK;
+}
+
 int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film, double* stats) {
     if (!s || !set || !film) return SLRH_ERR_INVALID;
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "render before slrh_scene_build");
@@ -492,14 +501,17 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     RenderConfig cfg{set->width, set->height, set->spp, set->max_depth, set->seed, set->slots_per_pixel,
                      (set->flags & 1u) != 0};
+    if (set->reserved != 0) return fail(s->ctx, SLRH_ERR_INVALID, "reserved settings field must be 0");
+    cfg.traceVariant = set->tune_trace == 0 ? 0 : (int)set->tune_trace - 1;
+    if (cfg.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     RenderStats st;
     std::string err;
     int r = slrh::render(s->dev, cfg, film->dev, &st, &err);
     if (r) return fail(s->ctx, r, err);
     if (stats) {
         stats[0] = st.raysTraced; stats[1] = st.samples; stats[2] = st.kernelSeconds; stats[3] = st.nonFinite;
-        stats[4] = st.iterations; stats[5] = st.extendSeconds; stats[6] = st.extendLaunches; stats[7] = st.extendRays;
-        stats[8] = st.shadowSeconds; stats[9] = st.shadowLaunches; stats[10] = st.shadowRays;
+        stats[4] = st.iterations; stats[5] = st.traceSeconds; stats[6] = st.traceLaunches; stats[7] = st.extendRays;
+        stats[8] = st.shadeSeconds; stats[9] = st.otherSeconds; stats[10] = st.shadowRays;
         stats[11] = st.nodesVisited; stats[12] = st.trisTested;
         stats[13] = st.nodesShadow; stats[14] = st.trisShadow;
     }

@@ -41,12 +41,13 @@ struct TraceArgs {
 struct RenderConfig {
     uint32_t width, height, spp, maxDepth, seed, slotsPerPixel;
     bool timeKernels;
+    int traceVariant = 0;   // traversal kernel variant (see traceVariant)
 };
 
 struct RenderStats {
     double raysTraced = 0, samples = 0, kernelSeconds = 0, nonFinite = 0, iterations = 0;
-    double extendSeconds = 0, extendLaunches = 0, extendRays = 0;
-    double shadowSeconds = 0, shadowLaunches = 0, shadowRays = 0;
+    double traceSeconds = 0, traceLaunches = 0, extendRays = 0, shadowRays = 0;
+    double shadeSeconds = 0, otherSeconds = 0;
     double nodesVisited = 0, trisTested = 0;   // extension rays
     double nodesShadow = 0, trisShadow = 0;    // shadow rays
     double overflow = 0;
@@ -68,5 +69,7 @@ int filmDownload(const DeviceFilm* f, std::vector<double>* strata, std::string* 
 void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h);
 
 int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err);
+int numTraceVariants();
+void traceVariant(int v, int* ldsDepth, int* refill);
 
 }  // namespace slrh

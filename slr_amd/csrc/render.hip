@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -50,7 +51,17 @@ constexpr int SLRH_ERR_NOMEM_CODE = -4;
 constexpr int SLRH_ERR_CAPACITY_CODE = -5;
 
 constexpr int kBlock = 256;
-constexpr int kLdsDepth = 16;
+constexpr int kLdsDepth = 16;       // LDS stack entries per lane for the ray-batch kernel
+constexpr int kMinLdsDepth = 8;     // smallest LDS depth among the traversal variants
+
+// Traversal kernel variants the autotuner chooses from: LDS stack depth per
+// lane and the idle-lane threshold of the persistent refill.
+struct TraceVariant { int ldsDepth, refill; };
+constexpr TraceVariant kVariants[] = {{16, 64}, {8, 64}, {24, 64}, {16, 16}, {8, 16}, {24, 16}, {16, 1}, {8, 1},
+                                      {24, 1}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+// refill threshold of the ray-batch entry points
+constexpr int kBatchRefill = 16;
 
 static int g_numCUs = 256;
 
@@ -155,7 +166,12 @@ static void cameraDerived(const CameraParams& c, uint32_t w, uint32_t h, DevCame
 int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     std::unique_ptr<DeviceScene> ds(new DeviceScene());
     const Accel& ac = sd.accel;
-    // QBVH nodes
+    // QBVH nodes; the traversal addresses nodes and triangle references with
+    // 32-bit byte offsets (128 B per node, 48 B per reference)
+    if (ac.qbvh.size() > (1u << 25)) {
+        if (err) *err = "QBVH exceeds 2^25 nodes";
+        return SLRH_ERR_CAPACITY_CODE;
+    }
     std::vector<float4> nodes(ac.qbvh.size() * 8);
     for (size_t i = 0; i < ac.qbvh.size(); ++i) {
         const QbvhNode& q = ac.qbvh[i];
@@ -194,7 +210,11 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         float pid;
         memcpy(&pid, &p, 4);
         tris[3 * r + 0] = make_float4(p0[0], p0[1], p0[2], pid);
-        tris[3 * r + 1] = make_float4(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], 0.0f);
+        // e1.w: shading class of the triangle's material, handed out with a hit
+        const uint32_t cls = sd.materials[sd.triMat[p]].cls;
+        float clsF;
+        memcpy(&clsF, &cls, 4);
+        tris[3 * r + 1] = make_float4(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], clsF);
         tris[3 * r + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], 0.0f);
     }
     const size_t nv = sd.vpos.size() / 3, nt = sd.idx.size() / 3;
@@ -300,36 +320,35 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, TraceArgs a
     __shared__ uint32_t lds[kLdsDepth * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     TraceStack st;
-    st.lds = lds + threadIdx.x;
+    st.lds = ldsColumn(lds, threadIdx.x);
     st.ldsStride = kBlock;
     st.spill = spill + gtid;
     st.spillStride = spillStride;
     st.ldsDepth = kLdsDepth;
-    uint32_t overflow = 0;
-    unsigned long long nodes = 0, tested = 0;
-    for (uint32_t i = gtid; i < n; i += gridDim.x * kBlock) {
-        RayIn r;
-        r.o = f3(a.ox[i], a.oy[i], a.oz[i]);
-        r.d = f3(a.dx[i], a.dy[i], a.dz[i]);
-        r.tmin = a.tmin[i];
-        r.tmax = a.tmax[i];
-        HitOut h;
-        uint32_t tt = 0;
-        if (any) {
-            nodes += traceRay<true>(sc, r, st, &h, overflow, &tt);
-            a.vis[i] = h.prim >= 0 ? 0.0f : 1.0f;
-        } else {
-            nodes += traceRay<false>(sc, r, st, &h, overflow, &tt);
-            a.t[i] = h.t;
-            a.prim[i] = h.prim;
-            a.b0[i] = h.b0;
-            a.b1[i] = h.b1;
-        }
-        tested += tt;
-    }
-    waveAdd64(&counters[0], nodes);
-    waveAdd64(&counters[1], tested);
-    if (overflow) atomicOr((unsigned int*)&counters[2], 1u);
+    TraceCounters tc;
+    uint32_t* fetchCtr = (uint32_t*)&counters[3];
+    const bool anyHit = any != 0;
+    traceLoop<kBatchRefill>(sc, st, n, fetchCtr, [](uint32_t i) { return i; },
+        [&](uint32_t i, RayIn& r) {
+            r.o = f3(a.ox[i], a.oy[i], a.oz[i]);
+            r.d = f3(a.dx[i], a.dy[i], a.dz[i]);
+            r.tmin = a.tmin[i];
+            r.tmax = a.tmax[i];
+            return anyHit;
+        },
+        [&](uint32_t i, bool isAny, const HitOut& h) {
+            if (isAny) {
+                a.vis[i] = h.prim >= 0 ? 0.0f : 1.0f;
+            } else {
+                a.t[i] = h.t;
+                a.prim[i] = h.prim;
+                a.b0[i] = h.b0;
+                a.b1[i] = h.b1;
+            }
+        }, tc);
+    waveAdd64(&counters[0], tc.nodes);
+    waveAdd64(&counters[1], tc.tris);
+    if (tc.overflow) atomicOr((unsigned int*)&counters[2], 1u);
 }
 
 static uint32_t persistentBlocks(uint32_t work) {
@@ -425,11 +444,13 @@ void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h) { *w = f->w; *h = f
 
 // Counter layout (uint32 each).
 enum Ctr : uint32_t {
-    C_SHADOW = 0, C_DONE = 1, C_EXT = 2,
-    C_CLS0 = 3,           // .. C_CLS0 + kNumClasses - 1
-    C_EXTPENDING = 7,     // continuing paths counted by the post-shade compaction
+    C_TRACE = 0,          // trace queue length (extension and shadow entries)
+    C_DONE = 1,           // done queue length
+    C_CLS0 = 3,           // .. C_CLS0 + kNumClasses - 1: shading queue lengths
+    C_PENDING = 7,        // slots with an extension or shadow ray pending after shading
     C_NONFINITE = 9,
     C_OVERFLOW = 10,
+    C_FETCH = 11,         // entry cursor of k_trace's persistent loop
     kNumCtr = 16
 };
 // 64-bit statistics
@@ -438,8 +459,12 @@ enum Stat64 : uint32_t {
     kNumStat = 8
 };
 // Per-slot work code, one byte per slot, rewritten by each stage and turned
-// into slot-ordered queues by k_compact.
+// into slot-ordered queues by k_compact. After shading a slot holds EXT or
+// DONE, plus SHADOW when it also emitted a shadow ray; the trace kernel turns
+// EXT into CLS0 + shading class.
 enum Code : uint8_t { CODE_IDLE = 0, CODE_EXT = 1, CODE_DONE = 2, CODE_CLS0 = 3, CODE_SHADOW = 0x80 };
+// trace queue entries: slot id, bit 31 set for the slot's shadow ray
+constexpr uint32_t kShadowEntry = 0x80000000u;
 
 // flags in info.y
 constexpr uint32_t F_HAS_SAMPLE = 1u;    // slot holds a sample whose radiance must be filed
@@ -457,7 +482,8 @@ struct PathState {
     float4* hit;         // t, prim, b0, b1
     double* film;        // [planes][pixels][16]
     uint8_t* code;       // per-slot work code (Code)
-    uint32_t* q;         // queues, each n entries: shadow, done, ext, cls0..3
+    uint8_t* vis;        // 1: the slot's shadow ray was traced and is unoccluded
+    uint32_t* q;         // queues: done (n entries), trace (2n), cls0..3 (n each)
     uint32_t* ctr;
     unsigned long long* stat;
     uint32_t* spill;
@@ -465,7 +491,8 @@ struct PathState {
     uint32_t maxDepth;
 };
 
-enum QueueId : uint32_t { Q_SHADOW = 0, Q_DONE = 1, Q_EXT = 2, Q_CLS0 = 3, kNumQueues = 7 };
+// queue offsets in units of n entries
+enum QueueId : uint32_t { Q_DONE = 0, Q_TRACE = 1, Q_CLS0 = 3, kNumQueues = 7 };
 
 __device__ __forceinline__ uint32_t* queue(const PathState& ps, uint32_t id) { return ps.q + (size_t)id * ps.n; }
 
@@ -533,6 +560,21 @@ __device__ __forceinline__ void evalSpectrum(const DevScene& sc, int32_t id, flo
     }
 }
 
+// An unoccluded shadow ray's contribution (already weighted by the shading
+// kernel that emitted it) joins the path radiance; the shadow kernel only
+// flags it. Called by the first kernel that reads the radiance afterwards,
+// which keeps the summation order of libSLR/Renderer/PTRenderer.cpp:190-214
+// (light sample before the next emission term).
+__device__ __forceinline__ void resolveShadow(const PathState& ps, uint32_t s, float* rad) {
+    if (ps.vis[s]) {
+        float c[kNumL];
+        loadPlanes(ps.nee, ps.n, s, c);
+#pragma unroll
+        for (int k = 0; k < kNumL; ++k) rad[k] += c[k];
+        ps.vis[s] = 0;
+    }
+}
+
 // ------------------------------------------------------------------ kernels
 
 __global__ void k_init_slots(PathState ps, uint32_t spp, uint32_t slotsPerPixel, uint32_t seed) {
@@ -546,6 +588,7 @@ __global__ void k_init_slots(PathState ps, uint32_t spp, uint32_t slotsPerPixel,
         h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
         ps.rng[s] = xsSeed(h);
         ps.code[s] = CODE_DONE;
+        ps.vis[s] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int i = 0; i < kNumCtr; ++i) ps.ctr[i] = 0;
@@ -557,8 +600,8 @@ __global__ void k_init_slots(PathState ps, uint32_t spp, uint32_t slotsPerPixel,
 __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_DONE];
     const uint32_t* qin = queue(ps, Q_DONE);
-    // the extension queue is rebuilt after this kernel
-    if (blockIdx.x == 0 && threadIdx.x == 0) ps.ctr[C_EXT] = 0;
+    // the trace queue is rebuilt after this kernel
+    if (blockIdx.x == 0 && threadIdx.x == 0) ps.ctr[C_TRACE] = 0;
     const DevCamera cam = sc.cam;
     const float recBinWidth = kNumL / (kLambdaHighD - kLambdaLowD);
     unsigned long long samples = 0;
@@ -576,6 +619,7 @@ __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
                 // SpectrumStorage::add spectrum_types.h:725-736)
                 float c[kNumL];
                 loadPlanes(ps.sp, ps.n, s, c);
+                resolveShadow(ps, s, c);
                 bool finite = true;
 #pragma unroll
                 for (int k = 0; k < kNumL; ++k) { c[k] *= misc.w; finite = finite && isfinite(c[k]); }
@@ -658,90 +702,63 @@ __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
     waveAdd64(&ps.stat[S_SAMPLES], samples);
 }
 
-// Closest hit for the extension queue, then classification into shading queues.
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PathState ps) {
-    __shared__ uint32_t lds[kLdsDepth * kBlock];
-    const uint32_t count = ps.ctr[C_EXT];
-    const uint32_t* qin = queue(ps, Q_EXT);
+// One traversal launch per iteration for the whole trace queue: closest hit
+// for extension entries (the hit and the hit material's shading class are
+// written for the classification that follows), any hit for shadow entries
+// (an unoccluded ray sets the slot's vis flag; its contribution is added by
+// resolveShadow in the next kernel that reads the path radiance, outside this
+// divergent loop). Mixing both kinds in one launch leaves one tail of long
+// rays per iteration instead of two.
+template <int D, int T>
+__global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathState ps) {
+    __shared__ uint32_t lds[D * kBlock];
+    const uint32_t count = ps.ctr[C_TRACE];
+    const uint32_t* qin = queue(ps, Q_TRACE);
     // the shading queues are rebuilt after this kernel
     if (blockIdx.x == 0 && threadIdx.x < kNumClasses) ps.ctr[C_CLS0 + threadIdx.x] = 0;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     TraceStack st;
-    st.lds = lds + threadIdx.x;
+    st.lds = ldsColumn(lds, threadIdx.x);
     st.ldsStride = kBlock;
     st.spill = ps.spill + gtid;
     st.spillStride = ps.spillStride;
-    st.ldsDepth = kLdsDepth;
-    uint32_t overflow = 0;
-    unsigned long long nodes = 0, tested = 0, rays = 0;
-    for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
-        const uint32_t i = base + threadIdx.x;
-        const bool valid = i < count;
-        uint32_t s = 0, cls = CLS_MISS;
-        if (valid) {
-            s = qin[i];
-            const float4 o = ps.ro[s], d = ps.rd[s];
-            RayIn r;
+    st.ldsDepth = D;
+    TraceCounters tc;
+    traceLoop<T>(sc, st, count, &ps.ctr[C_FETCH], [&](uint32_t i) { return qin[i]; },
+        [&](uint32_t e, RayIn& r) {
+            const uint32_t s = e & ~kShadowEntry;
+            const bool shadow = (e & kShadowEntry) != 0;
+            float4 o, d;
+            if (shadow) {
+                o = ps.so[s];
+                d = ps.sd[s];
+            } else {
+                o = ps.ro[s];
+                d = ps.rd[s];
+                d.w = INFINITY;
+            }
             r.o = xyz(o);
             r.d = xyz(d);
             r.tmin = o.w;
-            r.tmax = INFINITY;
-            HitOut h;
-            uint32_t tt = 0;
-            nodes += traceRay<false>(sc, r, st, &h, overflow, &tt);
-            tested += tt;
-            ++rays;
-            ps.hit[s] = make_float4(h.t, __int_as_float(h.prim), h.b0, h.b1);
-            if (h.prim >= 0) cls = sc.mats[sc.primVerts[h.prim].w].cls;
-            ps.code[s] = (uint8_t)(CODE_CLS0 + cls);
-        }
-    }
-    waveAdd64(&ps.stat[S_RAYS_EXT], rays);
-    waveAdd64(&ps.stat[S_NODES_EXT], nodes);
-    waveAdd64(&ps.stat[S_TRIS_EXT], tested);
-    if (overflow) atomicOr(&ps.ctr[C_OVERFLOW], 1u);
-}
-
-// Any hit for the shadow queue; unoccluded contributions join the radiance.
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PathState ps) {
-    __shared__ uint32_t lds[kLdsDepth * kBlock];
-    const uint32_t count = ps.ctr[C_SHADOW];
-    const uint32_t* qin = queue(ps, Q_SHADOW);
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    TraceStack st;
-    st.lds = lds + threadIdx.x;
-    st.ldsStride = kBlock;
-    st.spill = ps.spill + gtid;
-    st.spillStride = ps.spillStride;
-    st.ldsDepth = kLdsDepth;
-    uint32_t overflow = 0;
-    unsigned long long nodes = 0, tested = 0, rays = 0;
-    for (uint32_t i = gtid; i < count; i += gridDim.x * kBlock) {
-        const uint32_t s = qin[i];
-        const float4 o = ps.so[s], d = ps.sd[s];
-        RayIn r;
-        r.o = xyz(o);
-        r.d = xyz(d);
-        r.tmin = o.w;
-        r.tmax = d.w;
-        HitOut h;
-        uint32_t tt = 0;
-        nodes += traceRay<true>(sc, r, st, &h, overflow, &tt);
-        tested += tt;
-        ++rays;
-        if (h.prim < 0) {
-            float a[kNumL], c[kNumL];
-            loadPlanes(ps.sp, ps.n, s, a);
-            loadPlanes(ps.nee, ps.n, s, c);
-#pragma unroll
-            for (int k = 0; k < kNumL; ++k) a[k] += c[k];
-            storePlanes(ps.sp, ps.n, s, a);
-        }
-    }
-    waveAdd64(&ps.stat[S_RAYS_SHADOW], rays);
-    waveAdd64(&ps.stat[S_NODES_SH], nodes);
-    waveAdd64(&ps.stat[S_TRIS_SH], tested);
-    if (overflow) atomicOr(&ps.ctr[C_OVERFLOW], 1u);
+            r.tmax = d.w;
+            return shadow;
+        },
+        [&](uint32_t e, bool shadow, const HitOut& h) {
+            const uint32_t s = e & ~kShadowEntry;
+            if (shadow) {
+                if (h.prim < 0) ps.vis[s] = 1;
+            } else {
+                ps.hit[s] = make_float4(h.t, __int_as_float(h.prim), h.b0, h.b1);
+                ps.code[s] = (uint8_t)(CODE_CLS0 + (h.prim >= 0 ? h.aux : (uint32_t)CLS_MISS));
+            }
+        }, tc);
+    waveAdd64(&ps.stat[S_RAYS_EXT], tc.rays - tc.raysAny);
+    waveAdd64(&ps.stat[S_NODES_EXT], tc.nodes - tc.nodesAny);
+    waveAdd64(&ps.stat[S_TRIS_EXT], tc.tris - tc.trisAny);
+    waveAdd64(&ps.stat[S_RAYS_SHADOW], tc.raysAny);
+    waveAdd64(&ps.stat[S_NODES_SH], tc.nodesAny);
+    waveAdd64(&ps.stat[S_TRIS_SH], tc.trisAny);
+    if (tc.overflow) atomicOr(&ps.ctr[C_OVERFLOW], 1u);
 }
 
 struct SurfPt {
@@ -842,12 +859,12 @@ template <uint32_t CLS>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_CLS0 + CLS];
     const uint32_t* qin = queue(ps, Q_CLS0 + CLS);
-    // shadow/done queues of this iteration were consumed before the shading
-    // kernels; they are rebuilt by the compaction that follows them
+    // the done queue and the pending count of the previous iteration were
+    // consumed before the shading kernels; the compaction that follows them
+    // rebuilds both
     if (CLS == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
-        ps.ctr[C_SHADOW] = 0;
         ps.ctr[C_DONE] = 0;
-        ps.ctr[C_EXTPENDING] = 0;
+        ps.ctr[C_PENDING] = 0;
     }
     for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
         const uint32_t i = base + threadIdx.x;
@@ -871,6 +888,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
                 float alpha[kNumL], rad[kNumL];
                 loadPlanes(ps.alpha, ps.n, s, alpha);
                 loadPlanes(ps.sp, ps.n, s, rad);
+                resolveShadow(ps, s, rad);
                 const float3 dirOut = toLocal(sp, -rd);
                 // emission (PTRenderer.cpp:162-166 first hit, :247-263 implicit with MIS)
                 if (mat.emit >= 0 && dirOut.z > 0.0f) {
@@ -1006,33 +1024,38 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
 // all slots; wave prefix sums, a block scan in LDS and one atomic per block
 // and queue). Each block's output is a sorted run of slot ids, so the
 // kernels that read the queues touch path state in near-sequential order.
-//   MODE 0 (after shading): shadow queue, done queue, count of continuing
-//          paths; clears the shadow bit.
-//   MODE 1 (after regeneration): extension queue.
-//   MODE 2 (after extension): one queue per shading class.
+//   MODE 0 (after shading): done queue (DONE without a pending shadow ray),
+//          count of slots with an extension or shadow ray pending.
+//   MODE 1 (after regeneration): trace queue, the slot's extension entry
+//          and then its shadow entry; clears the shadow bit.
+//   MODE 2 (after tracing): one queue per shading class.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_compact(PathState ps) {
     constexpr int PER = 8;
-    constexpr int NQ = MODE == 0 ? 3 : (MODE == 1 ? 1 : kNumClasses);
+    constexpr int NQ = MODE == 0 ? 2 : (MODE == 1 ? 1 : kNumClasses);
     __shared__ uint32_t waveTot[NQ][4];
     __shared__ uint32_t blockBase[NQ];
     const uint32_t first = (blockIdx.x * 256u + threadIdx.x) * PER;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    // the traversal that consumes this compaction's queue starts its cursor at 0
+    if (MODE == 1 && blockIdx.x == 0 && threadIdx.x == 0) ps.ctr[C_FETCH] = 0;
     uint8_t c[PER];
     uint32_t cnt[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) cnt[q] = 0;
-    auto match = [](int q, uint8_t v) -> bool {
-        if (MODE == 0) return q == 0 ? (v & CODE_SHADOW) != 0 : q == 1 ? (v & 0x7F) == CODE_DONE : (v & 0x7F) == CODE_EXT;
-        if (MODE == 1) return v == CODE_EXT;
-        return v == CODE_CLS0 + q;
+    // entries slot s contributes to queue q
+    auto entries = [](int q, uint8_t v) -> uint32_t {
+        const bool ext = (v & 0x7F) == CODE_EXT, sh = (v & CODE_SHADOW) != 0;
+        if (MODE == 0) return q == 0 ? (v == CODE_DONE ? 1u : 0u) : (ext || sh ? 1u : 0u);
+        if (MODE == 1) return (ext ? 1u : 0u) + (sh ? 1u : 0u);
+        return v == CODE_CLS0 + q ? 1u : 0u;
     };
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const uint32_t s = first + k;
         c[k] = s < ps.n ? ps.code[s] : (uint8_t)CODE_IDLE;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) cnt[q] += match(q, c[k]) ? 1u : 0u;
+        for (int q = 0; q < NQ; ++q) cnt[q] += entries(q, c[k]);
     }
     uint32_t incl[NQ];
 #pragma unroll
@@ -1050,23 +1073,29 @@ __global__ __launch_bounds__(256) void k_compact(PathState ps) {
     if (threadIdx.x < NQ) {
         const int q = threadIdx.x;
         const uint32_t tot = waveTot[q][0] + waveTot[q][1] + waveTot[q][2] + waveTot[q][3];
-        const uint32_t ctr = MODE == 0 ? (q == 0 ? C_SHADOW : q == 1 ? C_DONE : C_EXTPENDING)
-                           : MODE == 1 ? C_EXT : C_CLS0 + q;
+        const uint32_t ctr = MODE == 0 ? (q == 0 ? C_DONE : C_PENDING) : MODE == 1 ? C_TRACE : C_CLS0 + q;
         blockBase[q] = tot ? atomicAdd(&ps.ctr[ctr], tot) : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        if (MODE == 0 && q == 2) continue;   // counted, not queued
+        if (MODE == 0 && q == 1) continue;   // counted, not queued
         uint32_t off = blockBase[q] + incl[q] - cnt[q];
         for (uint32_t w = 0; w < wave; ++w) off += waveTot[q][w];
-        const uint32_t qid = MODE == 0 ? (q == 0 ? Q_SHADOW : Q_DONE) : MODE == 1 ? Q_EXT : Q_CLS0 + q;
+        const uint32_t qid = MODE == 0 ? Q_DONE : MODE == 1 ? Q_TRACE : Q_CLS0 + q;
         uint32_t* out = queue(ps, qid);
 #pragma unroll
-        for (int k = 0; k < PER; ++k)
-            if (match(q, c[k])) out[off++] = first + k;
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t s = first + k;
+            if (MODE == 1) {
+                if ((c[k] & 0x7F) == CODE_EXT) out[off++] = s;
+                if (c[k] & CODE_SHADOW) out[off++] = s | kShadowEntry;
+            } else if (entries(q, c[k])) {
+                out[off++] = s;
+            }
+        }
     }
-    if (MODE == 0) {
+    if (MODE == 1) {
 #pragma unroll
         for (int k = 0; k < PER; ++k)
             if (c[k] & CODE_SHADOW) ps.code[first + k] = c[k] & 0x7F;
@@ -1090,7 +1119,7 @@ struct RenderState {
     DevBuf<uint4> rng, info;
     DevBuf<double> planeFilm;
     DevBuf<uint32_t> q, ctr, spill;
-    DevBuf<uint8_t> code;
+    DevBuf<uint8_t> code, vis;
     DevBuf<unsigned long long> stat;
     uint32_t* hostCtr = nullptr;
     ~RenderState() { if (hostCtr) (void)hipHostFree(hostCtr); }
@@ -1104,9 +1133,9 @@ static int allocState(RenderState* rs, uint32_t n, uint32_t pixels, uint32_t pla
         (r = rs->so.alloc(n, err)) || (r = rs->sd.alloc(n, err)) || (r = rs->misc.alloc(n, err)) ||
         (r = rs->hit.alloc(n, err)) || (r = rs->rng.alloc(n, err)) || (r = rs->info.alloc(n, err)) ||
         (r = rs->q.alloc((size_t)kNumQueues * n, err)) || (r = rs->ctr.alloc(kNumCtr, err)) ||
-        (r = rs->code.alloc(n, err)) ||
+        (r = rs->code.alloc(n, err)) || (r = rs->vis.alloc(n, err)) ||
         (r = rs->stat.alloc(kNumStat, err)) ||
-        (r = rs->spill.alloc((size_t)(kMaxStack - kLdsDepth) * blocks * kBlock, err)))
+        (r = rs->spill.alloc((size_t)(kMaxStack - kMinLdsDepth) * blocks * kBlock, err)))
         return r;
     if (planes > 1 && (r = rs->planeFilm.alloc((size_t)planes * pixels * kNumL, err))) return r;
     if (!rs->hostCtr && hipHostMalloc((void**)&rs->hostCtr, 64 * sizeof(uint32_t)) != hipSuccess) {
@@ -1115,6 +1144,21 @@ static int allocState(RenderState* rs, uint32_t n, uint32_t pixels, uint32_t pla
     }
     rs->n = n; rs->pixels = pixels; rs->planes = planes; rs->blocks = blocks;
     return 0;
+}
+
+template <int V>
+static void launchTraceV(uint32_t blocks, hipStream_t s, const DevScene& sc, const PathState& ps) {
+    hipLaunchKernelGGL((k_trace<kVariants[V].ldsDepth, kVariants[V].refill>), dim3(blocks), dim3(kBlock), 0, s, sc, ps);
+}
+typedef void (*LaunchFn)(uint32_t, hipStream_t, const DevScene&, const PathState&);
+static const LaunchFn kTraceLaunch[kNumVariants] = {launchTraceV<0>, launchTraceV<1>, launchTraceV<2>,
+                                                     launchTraceV<3>, launchTraceV<4>, launchTraceV<5>,
+                                                     launchTraceV<6>, launchTraceV<7>, launchTraceV<8>};
+
+int numTraceVariants() { return kNumVariants; }
+void traceVariant(int v, int* ldsDepth, int* refill) {
+    *ldsDepth = kVariants[v].ldsDepth;
+    *refill = kVariants[v].refill;
 }
 
 int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err) {
@@ -1148,6 +1192,7 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
     ps.hit = rs->hit.p;
     ps.film = K > 1 ? rs->planeFilm.p : film->data.p;
     ps.q = rs->q.p; ps.ctr = rs->ctr.p; ps.stat = rs->stat.p; ps.code = rs->code.p;
+    ps.vis = rs->vis.p;
     ps.spill = rs->spill.p; ps.spillStride = blocks * kBlock;
     ps.maxDepth = cfg.maxDepth == 0 ? 100u : cfg.maxDepth;
 
@@ -1158,34 +1203,42 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
 
     std::vector<hipEvent_t> evs;
     auto newEvent = [&]() { hipEvent_t e; (void)hipEventCreate(&e); evs.push_back(e); return e; };
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> extEv, shEv;
+    // with timeKernels: events before regen, trace, shading and the final compaction of each iteration
+    std::vector<std::array<hipEvent_t, 5>> itEv;
     hipEvent_t tStart = newEvent(), tEnd = newEvent();
     HIP_TRY(hipEventRecord(tStart, stream));
 
     const uint32_t cblocks = (N + 2047) / 2048;   // k_compact: 256 threads x 8 slots
+    const int tv = cfg.traceVariant >= 0 && cfg.traceVariant < kNumVariants ? cfg.traceVariant : 0;
+    auto mark = [&](int k) {
+        if (!cfg.timeKernels) return hipSuccess;
+        if (k == 0) itEv.push_back({});
+        itEv.back()[k] = newEvent();
+        return hipEventRecord(itEv.back()[k], stream);
+    };
     hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, stream, ps);
     uint32_t it = 0;
     const uint32_t checkEvery = 8;
     for (;; ++it) {
-        if (cfg.timeKernels) { hipEvent_t e0 = newEvent(); HIP_TRY(hipEventRecord(e0, stream)); shEv.push_back({e0, nullptr}); }
-        hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
-        if (cfg.timeKernels) { hipEvent_t e1 = newEvent(); HIP_TRY(hipEventRecord(e1, stream)); shEv.back().second = e1; }
+        HIP_TRY(mark(0));
         hipLaunchKernelGGL(k_regen, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
         hipLaunchKernelGGL(k_compact<1>, dim3(cblocks), dim3(256), 0, stream, ps);
-        if (cfg.timeKernels) { hipEvent_t e0 = newEvent(); HIP_TRY(hipEventRecord(e0, stream)); extEv.push_back({e0, nullptr}); }
-        hipLaunchKernelGGL(k_extend, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
-        if (cfg.timeKernels) { hipEvent_t e1 = newEvent(); HIP_TRY(hipEventRecord(e1, stream)); extEv.back().second = e1; }
+        HIP_TRY(mark(1));
+        kTraceLaunch[tv](blocks, stream, sc, ps);
+        HIP_TRY(mark(2));
         hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, stream, ps);
         hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
         hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
         hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
         hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
+        HIP_TRY(mark(3));
         hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, stream, ps);
+        HIP_TRY(mark(4));
         HIP_TRY(hipGetLastError());
         if ((it + 1) % checkEvery == 0) {
             HIP_TRY(hipMemcpyAsync(rs->hostCtr, rs->ctr.p, kNumCtr * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipStreamSynchronize(stream));
-            if (rs->hostCtr[C_SHADOW] == 0 && rs->hostCtr[C_DONE] == 0 && rs->hostCtr[C_EXTPENDING] == 0) break;
+            if (rs->hostCtr[C_DONE] == 0 && rs->hostCtr[C_PENDING] == 0) break;
             if (rs->hostCtr[C_OVERFLOW]) { if (err) *err = "traversal stack overflow (depth > 64)"; break; }
         }
     }
@@ -1214,13 +1267,17 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
         st->nonFinite = rs->hostCtr[C_NONFINITE];
         st->overflow = rs->hostCtr[C_OVERFLOW];
         st->iterations = it + 1;
-        double es = 0, ss = 0;
-        for (auto& p : extEv) { float t = 0; (void)hipEventElapsedTime(&t, p.first, p.second); es += t; }
-        for (auto& p : shEv) { float t = 0; (void)hipEventElapsedTime(&t, p.first, p.second); ss += t; }
-        st->extendSeconds = es * 1e-3;
-        st->shadowSeconds = ss * 1e-3;
-        st->extendLaunches = (double)extEv.size();
-        st->shadowLaunches = (double)shEv.size();
+        double tr = 0, sh = 0, ot = 0;
+        auto el = [](hipEvent_t a, hipEvent_t b) { float t = 0; (void)hipEventElapsedTime(&t, a, b); return (double)t; };
+        for (auto& e : itEv) {
+            ot += el(e[0], e[1]) + el(e[3], e[4]);
+            tr += el(e[1], e[2]);
+            sh += el(e[2], e[3]);
+        }
+        st->traceSeconds = tr * 1e-3;
+        st->shadeSeconds = sh * 1e-3;
+        st->otherSeconds = ot * 1e-3;
+        st->traceLaunches = (double)itEv.size();
     }
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     if (rs->hostCtr[C_OVERFLOW]) return SLRH_ERR_CAPACITY_CODE;
