@@ -42,6 +42,24 @@ def algorithmic_bytes(ext_rays, shadow_rays, nodes, tris):
     return 52.0 * ext_rays + 36.0 * shadow_rays + 128.0 * nodes + 40.0 * tris
 
 
+def calibration_copy(nbytes):
+    """One device-to-device hipMemcpy of nbytes (the runtime's copy kernel),
+    straight through the HIP runtime."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0:   # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def cpu_baseline(desc, sample, rays_per_sample):
     """Reference PTRenderer on the host cores over a bounded sample of the
     same scene; excludes its SBVH build (reported separately)."""
@@ -73,6 +91,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", default="960x540x2")
     ap.add_argument("--out", default="")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of known traffic that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -98,6 +119,9 @@ def main():
     log("scene generated: %d triangles in %.1fs" % (ntris, time.time() - t0))
 
     ctx = slr_amd.Context(0)
+    if args.calib_copy_mib > 0:
+        calibration_copy(args.calib_copy_mib << 20)
+        log("calibration copy: %d MiB device to device" % args.calib_copy_mib)
     t0 = time.time()
     sc = slr_amd.build_scene(ctx, desc)
     build_s = time.time() - t0

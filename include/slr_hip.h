@@ -176,9 +176,12 @@ typedef struct slrh_render_settings {
     uint32_t reserved;        /* must be 0 */
 } slrh_render_settings;
 
-/* Traversal kernel variants: n = number of variants; for variant v, its LDS
- * stack depth per lane and persistent-refill threshold (idle lanes). */
-SLRH_API int slrh_trace_variants(int* n, int* lds_depth, int* refill);
+/* Traversal kernel variants. On entry *n is the capacity of the arrays (any
+ * may be NULL), on return the number of variants; for variant v: its LDS
+ * stack depth per lane, persistent-refill threshold (idle lanes) and whether
+ * a node that opens a leaf has the leaf's first triangle tested in the same
+ * loop iteration. */
+SLRH_API int slrh_trace_variants(int* n, int* lds_depth, int* refill, int* fused);
 
 /* Film: FP64 16-stratum spectral sums per pixel, the same quantity the
  * reference's ImageSensor holds (libSLR/Core/ImageSensor.cpp:125-130,

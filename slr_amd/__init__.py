@@ -102,7 +102,7 @@ def load_library(path=None):
         "slrh_film_clear": (i32, [vp]),
         "slrh_film_read": (i32, [vp, i32, f64, vp, ctypes.c_size_t]),
         "slrh_render": (i32, [vp, vp, vp, vp]),
-        "slrh_trace_variants": (i32, [vp, vp, vp]),
+        "slrh_trace_variants": (i32, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -342,13 +342,14 @@ def cmf_strata():
 
 
 def trace_variants():
-    """[(lds_depth, refill_threshold)] of the traversal kernel variants."""
+    """[(lds_depth, refill_threshold, fused)] of the traversal kernel variants."""
     lib = load_library()
     n = ctypes.c_int(64)
     d = np.zeros(64, np.int32)
     r = np.zeros(64, np.int32)
-    lib.slrh_trace_variants(ctypes.byref(n), _ptr(d), _ptr(r))
-    return list(zip(d[:n.value].tolist(), r[:n.value].tolist()))
+    f = np.zeros(64, np.int32)
+    lib.slrh_trace_variants(ctypes.byref(n), _ptr(d), _ptr(r), _ptr(f))
+    return list(zip(d[:n.value].tolist(), r[:n.value].tolist(), f[:n.value].tolist()))
 
 
 STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "trace_seconds", "trace_launches",

@@ -481,11 +481,16 @@ int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size
     return SLRH_OK;
 }
 
-int slrh_trace_variants(int* n, int* ldsDepth, int* refill) {
+int slrh_trace_variants(int* n, int* ldsDepth, int* refill, int* fused) {
     if (!n) return SLRH_ERR_INVALID;
     const int m = numTraceVariants();
-    if (ldsDepth && refill)
-        for (int v = 0; v < std::min(*n, m); ++v) traceVariant(v, &ldsDepth[v], &refill[v]);
+    for (int v = 0; v < std::min(*n, m); ++v) {
+        int d, r, f;
+        traceVariant(v, &d, &r, &f);
+        if (ldsDepth) ldsDepth[v] = d;
+        if (refill) refill[v] = r;
+        if (fused) fused[v] = f;
+    }
     *n = m;
     return SLRH_OK;
 }

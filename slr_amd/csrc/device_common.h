@@ -2,13 +2,17 @@
 //
 // Layout in HBM (all arrays 16-byte aligned, structure-of-arrays where lanes
 // of a wave touch consecutive elements):
-//   QBVH node   : 8 x float4 = 128 B, one cache line. Planes 0-5 hold the four
-//                 children's bounds per axis (min x,y,z then max x,y,z, the
-//                 SoA-in-node form of libSLR/Accelerator/QBVH.h:41-53), plane 6
-//                 the four child words, plane 7 the packed split axes.
-//   child word  : 0xFFFFFFFF invalid; bit 31 set = leaf with count in bits
-//                 24-30 and first triangle reference in bits 0-23; otherwise
-//                 an internal node index.
+//   QBVH node   : 8 x float4 = 128 B, one cache line. Rows 0-5 hold the four
+//                 children's bounds per axis (min x, max x, min y, max y,
+//                 min z, max z: the SoA-in-node form of
+//                 libSLR/Accelerator/QBVH.h:41-53), row 6 the four child
+//                 words, row 7 is unused. Nodes are stored breadth-first for
+//                 the first levels (the part the traversal caches in LDS),
+//                 then in build order.
+//   child word  : bits 29-30 of child words 0, 1, 2 hold the node's top,
+//                 left and right split axes. Bit 31 set = leaf with count in
+//                 bits 24-28 and first triangle reference in bits 0-23 (count
+//                 0: no child); otherwise an internal node index (bits 0-24).
 //   triangle ref: 3 x float4 = 48 B {v0.xyz, prim id}, {v1-v0}, {v2-v0}; one
 //                 per leaf reference, in leaf order, so a leaf is contiguous.
 //                 The edges are the same FP32 differences the reference forms
@@ -20,10 +24,17 @@
 
 namespace slrh {
 
-constexpr uint32_t kInvalidChild = 0xFFFFFFFFu;
 constexpr uint32_t kLeafBit = 0x80000000u;
-constexpr uint32_t kMaxLeafCount = 127;
+constexpr uint32_t kNoChild = kLeafBit;          // an empty leaf
+constexpr uint32_t kAxisShift = 29;
+constexpr uint32_t kNodeIndexMask = 0x01FFFFFFu;
+constexpr uint32_t kLeafFirstMask = 0x00FFFFFFu;
+constexpr uint32_t kLeafCountShift = 24;
+constexpr uint32_t kLeafCountMask = 0x1Fu;
+constexpr uint32_t kMaxLeafCount = 31;
 constexpr uint32_t kMaxTriRefs = 1u << 24;
+constexpr uint32_t kMaxNodes = 1u << 25;
+constexpr uint32_t kTreeletRows = 7;             // node rows kept in the LDS copy (planes + child words)
 constexpr int kMaxStack = 64;   // QBVH::commonProcedure's StackSize (QBVH.h:299)
 constexpr int kNumL = 16;       // wavelengths per path
 constexpr float kLambdaLowD = 360.0f;    // WavelengthLowBound
@@ -83,7 +94,7 @@ struct DevScene {
     uint32_t numLights;
     uint32_t numPrims;
     float lightImportance;    // SurfaceObjectAggregate::importance()
-    uint32_t pad;
+    uint32_t numNodes;
     DevCamera cam;
 };
 

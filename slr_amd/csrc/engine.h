@@ -70,6 +70,6 @@ void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h);
 
 int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err);
 int numTraceVariants();
-void traceVariant(int v, int* ldsDepth, int* refill);
+void traceVariant(int v, int* ldsDepth, int* refill, int* fused);
 
 }  // namespace slrh

@@ -56,12 +56,19 @@ constexpr int kMinLdsDepth = 8;     // smallest LDS depth among the traversal va
 
 // Traversal kernel variants the autotuner chooses from: LDS stack depth per
 // lane and the idle-lane threshold of the persistent refill.
-struct TraceVariant { int ldsDepth, refill; };
-constexpr TraceVariant kVariants[] = {{16, 64}, {8, 64}, {24, 64}, {16, 16}, {8, 16}, {24, 16}, {16, 1}, {8, 1},
-                                      {24, 1}};
+// wavesPerEU: occupancy the register allocation is held to (0: compiler's
+// choice); treelet: nodes of the top of the tree each block copies into LDS.
+struct TraceVariant { int ldsDepth, refill; bool fuse; int wavesPerEU; int treelet; };
+constexpr TraceVariant kVariants[] = {{16, 16, true, 0, 256}, {16, 16, true, 0, 0},   {16, 16, true, 0, 128},
+                                      {16, 16, true, 0, 512}, {8, 16, true, 0, 256},  {8, 16, true, 0, 512},
+                                      {16, 16, true, 6, 256}, {16, 32, true, 0, 256}, {16, 16, false, 0, 256}};
+constexpr int kBatchTreelet = 128;
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // refill threshold of the ray-batch entry points
 constexpr int kBatchRefill = 16;
+// nodes placed breadth-first at the front of the node array (the largest
+// LDS treelet a traversal variant copies)
+constexpr uint32_t kBfsPrefix = 512;
 
 static int g_numCUs = 256;
 
@@ -168,36 +175,60 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     const Accel& ac = sd.accel;
     // QBVH nodes; the traversal addresses nodes and triangle references with
     // 32-bit byte offsets (128 B per node, 48 B per reference)
-    if (ac.qbvh.size() > (1u << 25)) {
+    const size_t nn = ac.qbvh.size();
+    if (nn > kMaxNodes) {
         if (err) *err = "QBVH exceeds 2^25 nodes";
         return SLRH_ERR_CAPACITY_CODE;
     }
-    std::vector<float4> nodes(ac.qbvh.size() * 8);
-    for (size_t i = 0; i < ac.qbvh.size(); ++i) {
-        const QbvhNode& q = ac.qbvh[i];
-        float4* o = &nodes[8 * i];
+    // device order: breadth-first from the root for the first levels (so a
+    // prefix of the array is the top of the tree, which the traversal copies
+    // into LDS), then the remaining nodes in build order
+    std::vector<uint32_t> order, newIndex(nn, UINT32_MAX);
+    order.reserve(nn);
+    if (nn) {
+        order.push_back(0);
+        newIndex[0] = 0;
+        for (size_t h = 0; h < order.size() && order.size() < kBfsPrefix; ++h) {
+            const QbvhNode& q = ac.qbvh[order[h]];
+            for (int c = 0; c < 4 && order.size() < kBfsPrefix; ++c)
+                if (q.childValid[c] && !q.isLeaf[c] && newIndex[q.index[c]] == UINT32_MAX) {
+                    newIndex[q.index[c]] = (uint32_t)order.size();
+                    order.push_back(q.index[c]);
+                }
+        }
+        for (size_t i = 0; i < nn; ++i)
+            if (newIndex[i] == UINT32_MAX) {
+                newIndex[i] = (uint32_t)order.size();
+                order.push_back((uint32_t)i);
+            }
+    }
+    std::vector<float4> nodes(nn * 8);
+    for (size_t k = 0; k < nn; ++k) {
+        const QbvhNode& q = ac.qbvh[order[k]];
+        float4* o = &nodes[8 * k];
+        // rows 0-5: min x, max x, min y, max y, min z, max z of the four children
         for (int a = 0; a < 3; ++a) {
-            o[a] = make_float4(q.bmin[a][0], q.bmin[a][1], q.bmin[a][2], q.bmin[a][3]);
-            o[3 + a] = make_float4(q.bmax[a][0], q.bmax[a][1], q.bmax[a][2], q.bmax[a][3]);
+            o[2 * a] = make_float4(q.bmin[a][0], q.bmin[a][1], q.bmin[a][2], q.bmin[a][3]);
+            o[2 * a + 1] = make_float4(q.bmax[a][0], q.bmax[a][1], q.bmax[a][2], q.bmax[a][3]);
         }
         uint32_t ch[4];
         for (int c = 0; c < 4; ++c) {
-            if (!q.childValid[c]) ch[c] = kInvalidChild;
+            if (!q.childValid[c]) ch[c] = kNoChild;
             else if (q.isLeaf[c]) {
                 if (q.count[c] > kMaxLeafCount || q.index[c] + q.count[c] > kMaxTriRefs) {
-                    if (err) *err = "QBVH leaf exceeds the device encoding (count <= 127, refs < 2^24)";
+                    if (err) *err = "QBVH leaf exceeds the device encoding (count <= 31, refs < 2^24)";
                     return SLRH_ERR_CAPACITY_CODE;
                 }
-                ch[c] = kLeafBit | (q.count[c] << 24) | q.index[c];
-            } else ch[c] = q.index[c];
+                ch[c] = kLeafBit | (q.count[c] << kLeafCountShift) | q.index[c];
+            } else ch[c] = newIndex[q.index[c]];
         }
-        uint32_t axes = (uint32_t)q.topAxis | ((uint32_t)q.leftAxis << 2) | ((uint32_t)q.rightAxis << 4);
-        float4 cw, ax;
+        ch[0] |= (uint32_t)q.topAxis << kAxisShift;
+        ch[1] |= (uint32_t)q.leftAxis << kAxisShift;
+        ch[2] |= (uint32_t)q.rightAxis << kAxisShift;
+        float4 cw;
         memcpy(&cw, ch, 16);
-        uint32_t axw[4] = {axes, 0, 0, 0};
-        memcpy(&ax, axw, 16);
         o[6] = cw;
-        o[7] = ax;
+        o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     // triangle references in leaf order
     std::vector<float4> tris(ac.qbvhRefs.size() * 3);
@@ -280,6 +311,7 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     v.primToLight = ds->primToLight.p;
     v.numLights = (uint32_t)(cdf.size() - 1);
     v.numPrims = (uint32_t)nt;
+    v.numNodes = (uint32_t)nn;
     ds->hasCamera = sd.hasCamera;
     if (sd.hasCamera) cameraDerived(sd.camera, 1, 1, &v.cam);
     *out = ds.release();
@@ -318,6 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, TraceArgs a
                                                         uint32_t* spill, uint32_t spillStride,
                                                         unsigned long long* counters) {
     __shared__ uint32_t lds[kLdsDepth * kBlock];
+    __shared__ float4 treelet[kBatchTreelet * kTreeletRows];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     TraceStack st;
     st.lds = ldsColumn(lds, threadIdx.x);
@@ -325,15 +358,15 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, TraceArgs a
     st.spill = spill + gtid;
     st.spillStride = spillStride;
     st.ldsDepth = kLdsDepth;
+    st.treeletNodes = (uint32_t)kBatchTreelet < sc.numNodes ? (uint32_t)kBatchTreelet : sc.numNodes;
+    st.treelet = loadTreelet(treelet, sc.nodes, st.treeletNodes);
     TraceCounters tc;
     uint32_t* fetchCtr = (uint32_t*)&counters[3];
     const bool anyHit = any != 0;
-    traceLoop<kBatchRefill>(sc, st, n, fetchCtr, [](uint32_t i) { return i; },
+    traceLoop<kBatchRefill, true, false>(sc, st, n, fetchCtr, [](uint32_t i) { return i; },
         [&](uint32_t i, RayIn& r) {
-            r.o = f3(a.ox[i], a.oy[i], a.oz[i]);
-            r.d = f3(a.dx[i], a.dy[i], a.dz[i]);
-            r.tmin = a.tmin[i];
-            r.tmax = a.tmax[i];
+            r.o = make_float4(a.ox[i], a.oy[i], a.oz[i], a.tmin[i]);
+            r.d = make_float4(a.dx[i], a.dy[i], a.dz[i], a.tmax[i]);
             return anyHit;
         },
         [&](uint32_t i, bool isAny, const HitOut& h) {
@@ -470,16 +503,39 @@ constexpr uint32_t kShadowEntry = 0x80000000u;
 constexpr uint32_t F_HAS_SAMPLE = 1u;    // slot holds a sample whose radiance must be filed
 constexpr uint32_t F_LAST_DELTA = 2u;    // last BSDF sample was a delta lobe
 
+// Per-slot path record: 24 float4 (384 B, three 128-B cache lines), slot s
+// at rec[24 * s]. Each kernel works on a sparse, slot-ordered subset of the
+// slots, so a slot's fields share lines instead of being spread over
+// per-field arrays (where a sparse subset would fetch a whole line per
+// 16-B field):
+//   line 0: what the traversal reads and writes: ray (ro, rd, rinv),
+//           shadow ray (so, sd, sinv), hit
+//   line 1: throughput alpha (16 wavelengths), radiance sp
+//   line 2: pending light-sample contribution nee, info, misc, rng
+// The producers of a ray (regeneration, shading) also store its reciprocal
+// direction and sign bits (rayInv), so the traversal loop starts a ray
+// without the three correctly rounded divisions.
+enum SlotField : uint32_t {
+    SF_RO = 0,      // ray origin.xyz + tmin
+    SF_RD = 1,      // ray direction.xyz + tmax (infinite)
+    SF_RINV = 2,    // rayInv(direction)
+    SF_SO = 3,      // shadow ray origin.xyz + tmin
+    SF_SD = 4,      // shadow ray direction.xyz + tmax
+    SF_SINV = 5,    // rayInv(shadow direction)
+    SF_HIT = 6,     // t, prim, b0, b1
+    SF_ALPHA = 8,   // 4 float4
+    SF_SP = 12,     // 4 float4
+    SF_NEE = 16,    // 4 float4
+    SF_INFO = 20,   // pixel, flags | hero << 8 | pathLength << 16, samples started, samples total
+    SF_MISC = 21,   // wl offset, initY, last BSDF pdf, camera weight
+    SF_RNG = 22,    // XORShift state
+    kSlotVec = 24
+};
+
 struct PathState {
     uint32_t n;          // slots
     uint32_t pixels;     // W*H
-    float4 *ro, *rd;     // ray origin.xyz + tmin, direction.xyz
-    float4 *alpha, *sp, *nee;   // 4 planes each: plane p of slot s at [p*n + s]
-    float4 *so, *sd;     // shadow ray origin.xyz + tmin, direction.xyz + tmax
-    uint4* rng;
-    float4* misc;        // wl offset, initY, last BSDF pdf, camera weight
-    uint4* info;         // pixel, flags | hero << 8 | pathLength << 16, samples started, samples total
-    float4* hit;         // t, prim, b0, b1
+    float4* rec;         // n slot records
     double* film;        // [planes][pixels][16]
     uint8_t* code;       // per-slot work code (Code)
     uint8_t* vis;        // 1: the slot's shadow ray was traced and is unoccluded
@@ -489,6 +545,8 @@ struct PathState {
     uint32_t* spill;
     uint32_t spillStride;
     uint32_t maxDepth;
+    __device__ float4& f4(uint32_t s, uint32_t f) const { return rec[(size_t)s * kSlotVec + f]; }
+    __device__ uint4& u4(uint32_t s, uint32_t f) const { return *(uint4*)&rec[(size_t)s * kSlotVec + f]; }
 };
 
 // queue offsets in units of n entries
@@ -522,16 +580,18 @@ __device__ __forceinline__ uint4 xsSeed(uint32_t seed) {
     return s;
 }
 
-__device__ __forceinline__ void loadPlanes(const float4* base, uint32_t n, uint32_t s, float* v) {
+__device__ __forceinline__ void loadPlanes(const PathState& ps, uint32_t s, uint32_t f, float* v) {
+    const float4* b = &ps.f4(s, f);
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-        float4 a = base[(size_t)p * n + s];
+        float4 a = b[p];
         v[4 * p] = a.x; v[4 * p + 1] = a.y; v[4 * p + 2] = a.z; v[4 * p + 3] = a.w;
     }
 }
-__device__ __forceinline__ void storePlanes(float4* base, uint32_t n, uint32_t s, const float* v) {
+__device__ __forceinline__ void storePlanes(const PathState& ps, uint32_t s, uint32_t f, const float* v) {
+    float4* b = &ps.f4(s, f);
 #pragma unroll
-    for (int p = 0; p < 4; ++p) base[(size_t)p * n + s] = make_float4(v[4 * p], v[4 * p + 1], v[4 * p + 2], v[4 * p + 3]);
+    for (int p = 0; p < 4; ++p) b[p] = make_float4(v[4 * p], v[4 * p + 1], v[4 * p + 2], v[4 * p + 3]);
 }
 
 // wavelength i of a path: createWithEqualOffsets (libSLR/BasicTypes/spectrum_types.h:59-69)
@@ -568,7 +628,7 @@ __device__ __forceinline__ void evalSpectrum(const DevScene& sc, int32_t id, flo
 __device__ __forceinline__ void resolveShadow(const PathState& ps, uint32_t s, float* rad) {
     if (ps.vis[s]) {
         float c[kNumL];
-        loadPlanes(ps.nee, ps.n, s, c);
+        loadPlanes(ps, s, SF_NEE, c);
 #pragma unroll
         for (int k = 0; k < kNumL; ++k) rad[k] += c[k];
         ps.vis[s] = 0;
@@ -582,11 +642,11 @@ __global__ void k_init_slots(PathState ps, uint32_t spp, uint32_t slotsPerPixel,
         uint32_t pixel = s % ps.pixels;
         uint32_t plane = s / ps.pixels;
         uint32_t total = spp > plane ? (spp - plane + slotsPerPixel - 1) / slotsPerPixel : 0;
-        ps.info[s] = make_uint4(pixel, 0u, 0u, total);
+        ps.u4(s, SF_INFO) = make_uint4(pixel, 0u, 0u, total);
         // one XORShift stream per slot, seeded like XORShiftRNG(seed')
         uint32_t h = seed ^ (0x9E3779B9u * (s + 1u));
         h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
-        ps.rng[s] = xsSeed(h);
+        ps.u4(s, SF_RNG) = xsSeed(h);
         ps.code[s] = CODE_DONE;
         ps.vis[s] = 0;
     }
@@ -612,13 +672,13 @@ __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
         uint32_t s = 0;
         if (valid) {
             s = qin[i];
-            uint4 info = ps.info[s];
-            float4 misc = ps.misc[s];
+            uint4 info = ps.u4(s, SF_INFO);
+            float4 misc = ps.f4(s, SF_MISC);
             if (info.y & F_HAS_SAMPLE) {
                 // C *= weight; film add (PTRenderer.cpp:129-138, ImageSensor.cpp:125-130,
                 // SpectrumStorage::add spectrum_types.h:725-736)
                 float c[kNumL];
-                loadPlanes(ps.sp, ps.n, s, c);
+                loadPlanes(ps, s, SF_SP, c);
                 resolveShadow(ps, s, c);
                 bool finite = true;
 #pragma unroll
@@ -650,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
             info.y &= ~F_HAS_SAMPLE;
             if (info.z < info.w) {
                 // new camera sample (PTRenderer.cpp:108-124, PerspectiveCamera.cpp:31-82)
-                uint4 rng = ps.rng[s];
+                uint4 rng = ps.u4(s, SF_RNG);
                 const uint32_t px = info.x % cam.width, py = info.x / cam.width;
                 const float fx = px + xsFloat(rng);
                 const float fy = py + xsFloat(rng);
@@ -684,18 +744,19 @@ __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
                 float one[kNumL], zero[kNumL];
 #pragma unroll
                 for (int k = 0; k < kNumL; ++k) { one[k] = 1.0f; zero[k] = 0.0f; }
-                storePlanes(ps.alpha, ps.n, s, one);
-                storePlanes(ps.sp, ps.n, s, zero);
+                storePlanes(ps, s, SF_ALPHA, one);
+                storePlanes(ps, s, SF_SP, zero);
                 const float initY = importance16(one, (int)hero);
-                ps.ro[s] = make_float4(org.x, org.y, org.z, 0.0f);
-                ps.rd[s] = make_float4(dir.x, dir.y, dir.z, 0.0f);
-                ps.misc[s] = make_float4(off, initY, 0.0f, weight);
-                ps.rng[s] = rng;
+                ps.f4(s, SF_RO) = make_float4(org.x, org.y, org.z, 0.0f);
+                ps.f4(s, SF_RD) = make_float4(dir.x, dir.y, dir.z, INFINITY);
+                ps.f4(s, SF_RINV) = rayInv(dir);
+                ps.f4(s, SF_MISC) = make_float4(off, initY, 0.0f, weight);
+                ps.u4(s, SF_RNG) = rng;
                 info.y = F_HAS_SAMPLE | (hero << 8);   // pathLength 0
                 info.z += 1;
                 emit = true;
             }
-            ps.info[s] = info;
+            ps.u4(s, SF_INFO) = info;
             ps.code[s] = emit ? CODE_EXT : CODE_IDLE;
         }
     }
@@ -709,9 +770,11 @@ __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
 // resolveShadow in the next kernel that reads the path radiance, outside this
 // divergent loop). Mixing both kinds in one launch leaves one tail of long
 // rays per iteration instead of two.
-template <int D, int T>
-__global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathState ps) {
+template <int D, int T, bool F, int W, int L>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1, W > 0 ? W : 10)))
+void k_trace(DevScene sc, PathState ps) {
     __shared__ uint32_t lds[D * kBlock];
+    __shared__ float4 treelet[L > 0 ? L * kTreeletRows : 1];
     const uint32_t count = ps.ctr[C_TRACE];
     const uint32_t* qin = queue(ps, Q_TRACE);
     // the shading queues are rebuilt after this kernel
@@ -723,24 +786,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathState ps) {
     st.spill = ps.spill + gtid;
     st.spillStride = ps.spillStride;
     st.ldsDepth = D;
+    st.treeletNodes = L < sc.numNodes ? (uint32_t)L : sc.numNodes;
+    st.treelet = loadTreelet(treelet, sc.nodes, st.treeletNodes);
     TraceCounters tc;
-    traceLoop<T>(sc, st, count, &ps.ctr[C_FETCH], [&](uint32_t i) { return qin[i]; },
+    traceLoop<T, F, true>(sc, st, count, &ps.ctr[C_FETCH], [&](uint32_t i) { return qin[i]; },
         [&](uint32_t e, RayIn& r) {
             const uint32_t s = e & ~kShadowEntry;
             const bool shadow = (e & kShadowEntry) != 0;
-            float4 o, d;
-            if (shadow) {
-                o = ps.so[s];
-                d = ps.sd[s];
-            } else {
-                o = ps.ro[s];
-                d = ps.rd[s];
-                d.w = INFINITY;
-            }
-            r.o = xyz(o);
-            r.d = xyz(d);
-            r.tmin = o.w;
-            r.tmax = d.w;
+            // the ray's three consecutive float4: origin + tmin, direction + tmax, rayInv
+            const float4* rp = &ps.f4(s, shadow ? SF_SO : SF_RO);
+            r.o = rp[0];
+            r.d = rp[1];
+            r.inv = rp[2];
             return shadow;
         },
         [&](uint32_t e, bool shadow, const HitOut& h) {
@@ -748,7 +805,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathState ps) {
             if (shadow) {
                 if (h.prim < 0) ps.vis[s] = 1;
             } else {
-                ps.hit[s] = make_float4(h.t, __int_as_float(h.prim), h.b0, h.b1);
+                ps.f4(s, SF_HIT) = make_float4(h.t, __int_as_float(h.prim), h.b0, h.b1);
                 ps.code[s] = (uint8_t)(CODE_CLS0 + (h.prim >= 0 ? h.aux : (uint32_t)CLS_MISS));
             }
         }, tc);
@@ -875,10 +932,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
             s = qin[i];
             done = true;
             if (CLS != CLS_MISS) {
-                uint4 info = ps.info[s];
-                float4 misc = ps.misc[s];
-                uint4 rng = ps.rng[s];
-                const float4 o4 = ps.ro[s], d4 = ps.rd[s], h = ps.hit[s];
+                uint4 info = ps.u4(s, SF_INFO);
+                float4 misc = ps.f4(s, SF_MISC);
+                uint4 rng = ps.u4(s, SF_RNG);
+                const float4 o4 = ps.f4(s, SF_RO), d4 = ps.f4(s, SF_RD), h = ps.f4(s, SF_HIT);
                 const float3 ro = xyz(o4), rd = xyz(d4);
                 const uint32_t prim = (uint32_t)__float_as_int(h.y);
                 const SurfPt sp = surfaceAt(sc, prim, h.z, h.w, ro, rd, h.x);
@@ -886,8 +943,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
                 const int hero = (int)((info.y >> 8) & 0xFFu);
                 uint32_t pathLen = info.y >> 16;
                 float alpha[kNumL], rad[kNumL];
-                loadPlanes(ps.alpha, ps.n, s, alpha);
-                loadPlanes(ps.sp, ps.n, s, rad);
+                loadPlanes(ps, s, SF_ALPHA, alpha);
+                loadPlanes(ps, s, SF_SP, rad);
                 resolveShadow(ps, s, rad);
                 const float3 dirOut = toLocal(sp, -rd);
                 // emission (PTRenderer.cpp:162-166 first hit, :247-263 implicit with MIS)
@@ -977,9 +1034,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
                                 // Scene::testVisibility segment (libSLR/Scene/Scene.cpp:96-112)
                                 const float dist = length3(lp - sp.p);
                                 const float3 vdir = divs(lp - sp.p, dist);
-                                storePlanes(ps.nee, ps.n, s, c);
-                                ps.so[s] = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
-                                ps.sd[s] = make_float4(vdir.x, vdir.y, vdir.z, dist * (1 - 1e-4f));
+                                storePlanes(ps, s, SF_NEE, c);
+                                ps.f4(s, SF_SO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
+                                ps.f4(s, SF_SD) = make_float4(vdir.x, vdir.y, vdir.z, dist * (1 - 1e-4f));
+                                ps.f4(s, SF_SINV) = rayInv(vdir);
                                 shadow = true;
                             }
                         }
@@ -998,20 +1056,21 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
 #pragma unroll
                             for (int k = 0; k < kNumL; ++k) alpha[k] *= fs[k] * w;
                             const float3 nd = fromLocal(sp, bs.dir);
-                            ps.ro[s] = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
-                            ps.rd[s] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                            ps.f4(s, SF_RO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
+                            ps.f4(s, SF_RD) = make_float4(nd.x, nd.y, nd.z, INFINITY);
+                            ps.f4(s, SF_RINV) = rayInv(nd);
                             misc.z = bs.pdf;
                             info.y &= ~F_LAST_DELTA;
                             cont = true;
                         }
                     }
                 }
-                storePlanes(ps.sp, ps.n, s, rad);
-                if (cont) storePlanes(ps.alpha, ps.n, s, alpha);
+                storePlanes(ps, s, SF_SP, rad);
+                if (cont) storePlanes(ps, s, SF_ALPHA, alpha);
                 info.y = (info.y & 0xFFFFu) | (pathLen << 16);
-                ps.info[s] = info;
-                ps.misc[s] = misc;
-                ps.rng[s] = rng;
+                ps.u4(s, SF_INFO) = info;
+                ps.f4(s, SF_MISC) = misc;
+                ps.u4(s, SF_RNG) = rng;
                 done = !cont;
             }
         }
@@ -1050,10 +1109,17 @@ __global__ __launch_bounds__(256) void k_compact(PathState ps) {
         if (MODE == 1) return (ext ? 1u : 0u) + (sh ? 1u : 0u);
         return v == CODE_CLS0 + q ? 1u : 0u;
     };
+    if (first + PER <= ps.n) {
+        // one 8-byte load per thread
+        const uint2 w = *(const uint2*)(ps.code + first);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) c[k] = (uint8_t)(((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xFFu);
+    } else {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) c[k] = first + k < ps.n ? ps.code[first + k] : (uint8_t)CODE_IDLE;
+    }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const uint32_t s = first + k;
-        c[k] = s < ps.n ? ps.code[s] : (uint8_t)CODE_IDLE;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) cnt[q] += entries(q, c[k]);
     }
@@ -1115,8 +1181,7 @@ __global__ void k_film_reduce(const double* planes, uint32_t numPlanes, size_t p
 
 struct RenderState {
     uint32_t n = 0, pixels = 0, planes = 0, blocks = 0;
-    DevBuf<float4> ro, rd, alpha, sp, nee, so, sd, misc, hit;
-    DevBuf<uint4> rng, info;
+    DevBuf<float4> rec;
     DevBuf<double> planeFilm;
     DevBuf<uint32_t> q, ctr, spill;
     DevBuf<uint8_t> code, vis;
@@ -1128,11 +1193,7 @@ struct RenderState {
 static int allocState(RenderState* rs, uint32_t n, uint32_t pixels, uint32_t planes, uint32_t blocks,
                       std::string* err) {
     int r;
-    if ((r = rs->ro.alloc(n, err)) || (r = rs->rd.alloc(n, err)) || (r = rs->alpha.alloc(4 * (size_t)n, err)) ||
-        (r = rs->sp.alloc(4 * (size_t)n, err)) || (r = rs->nee.alloc(4 * (size_t)n, err)) ||
-        (r = rs->so.alloc(n, err)) || (r = rs->sd.alloc(n, err)) || (r = rs->misc.alloc(n, err)) ||
-        (r = rs->hit.alloc(n, err)) || (r = rs->rng.alloc(n, err)) || (r = rs->info.alloc(n, err)) ||
-        (r = rs->q.alloc((size_t)kNumQueues * n, err)) || (r = rs->ctr.alloc(kNumCtr, err)) ||
+    if ((r = rs->rec.alloc((size_t)kSlotVec * n, err)) || (r = rs->q.alloc((size_t)kNumQueues * n, err)) || (r = rs->ctr.alloc(kNumCtr, err)) ||
         (r = rs->code.alloc(n, err)) || (r = rs->vis.alloc(n, err)) ||
         (r = rs->stat.alloc(kNumStat, err)) ||
         (r = rs->spill.alloc((size_t)(kMaxStack - kMinLdsDepth) * blocks * kBlock, err)))
@@ -1148,7 +1209,9 @@ static int allocState(RenderState* rs, uint32_t n, uint32_t pixels, uint32_t pla
 
 template <int V>
 static void launchTraceV(uint32_t blocks, hipStream_t s, const DevScene& sc, const PathState& ps) {
-    hipLaunchKernelGGL((k_trace<kVariants[V].ldsDepth, kVariants[V].refill>), dim3(blocks), dim3(kBlock), 0, s, sc, ps);
+    hipLaunchKernelGGL((k_trace<kVariants[V].ldsDepth, kVariants[V].refill, kVariants[V].fuse,
+                                kVariants[V].wavesPerEU, kVariants[V].treelet>),
+                       dim3(blocks), dim3(kBlock), 0, s, sc, ps);
 }
 typedef void (*LaunchFn)(uint32_t, hipStream_t, const DevScene&, const PathState&);
 static const LaunchFn kTraceLaunch[kNumVariants] = {launchTraceV<0>, launchTraceV<1>, launchTraceV<2>,
@@ -1156,9 +1219,10 @@ static const LaunchFn kTraceLaunch[kNumVariants] = {launchTraceV<0>, launchTrace
                                                      launchTraceV<6>, launchTraceV<7>, launchTraceV<8>};
 
 int numTraceVariants() { return kNumVariants; }
-void traceVariant(int v, int* ldsDepth, int* refill) {
+void traceVariant(int v, int* ldsDepth, int* refill, int* fused) {
     *ldsDepth = kVariants[v].ldsDepth;
     *refill = kVariants[v].refill;
+    *fused = kVariants[v].fuse ? 1 : 0;
 }
 
 int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err) {
@@ -1168,7 +1232,11 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
     if (P == 0 || cfg.spp == 0) return 0;
     uint32_t K = cfg.slotsPerPixel;
     if (K == 0) {
-        const uint32_t target = 2u << 20;   // about two million live paths fill the chip
+        // about 16M live paths: the traversal's per-launch tail of long rays
+        // is amortised over enough rays (measured on the 1080p/64 spp
+        // workload: 2M paths 258 ms per frame, 8M 224 ms, 16M 208 ms); path
+        // state is about 350 B per slot, 6 GB at 16M
+        const uint32_t target = 16u << 20;
         K = std::max(1u, std::min(cfg.spp, (target + P - 1) / P));
         K = std::min(K, 16u);
     }
@@ -1187,9 +1255,7 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
     sc.cam.height = H;
     PathState ps;
     ps.n = N; ps.pixels = P;
-    ps.ro = rs->ro.p; ps.rd = rs->rd.p; ps.alpha = rs->alpha.p; ps.sp = rs->sp.p; ps.nee = rs->nee.p;
-    ps.so = rs->so.p; ps.sd = rs->sd.p; ps.rng = rs->rng.p; ps.misc = rs->misc.p; ps.info = rs->info.p;
-    ps.hit = rs->hit.p;
+    ps.rec = rs->rec.p;
     ps.film = K > 1 ? rs->planeFilm.p : film->data.p;
     ps.q = rs->q.p; ps.ctr = rs->ctr.p; ps.stat = rs->stat.p; ps.code = rs->code.p;
     ps.vis = rs->vis.p;

@@ -23,6 +23,7 @@
 namespace slrh {
 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) const uint8_t lds_u8;
 
 struct TraceStack {
     lds_u32* lds;        // this lane's column: lds[d * ldsStride]
@@ -30,7 +31,27 @@ struct TraceStack {
     uint32_t* spill;     // this lane's column in the spill area
     uint32_t spillStride;
     int ldsDepth;
+    lds_u8* treelet;     // LDS copy of nodes 0..treeletNodes-1, kTreeletRows rows each
+    uint32_t treeletNodes;
 };
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 ldsf4(lds_u8* base, uint32_t off) {
+    const f32x4 v = *(__attribute__((address_space(3))) const f32x4*)(base + off);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Copies the first n nodes' rows 0..kTreeletRows-1 into LDS (called by the
+// whole block before the traversal loop; ends with a barrier).
+__device__ __forceinline__ lds_u8* loadTreelet(float4* shared, const float4* nodes, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n * kTreeletRows; i += blockDim.x) {
+        const uint32_t node = i / kTreeletRows, row = i - node * kTreeletRows;
+        shared[i] = nodes[8 * node + row];
+    }
+    __syncthreads();
+    return (lds_u8*)(const uint8_t*)shared;
+}
 
 __device__ __forceinline__ lds_u32* ldsColumn(uint32_t* shared, uint32_t lane) {
     return (lds_u32*)(shared + lane);
@@ -55,9 +76,15 @@ __device__ __forceinline__ uint32_t stackPop(const TraceStack& st, int& sp) {
     return v;
 }
 
+// A ray as it is stored and requested from memory: origin + tmin, direction
+// + tmax, rayInv(direction).
 struct RayIn {
-    float3 o, d;
-    float tmin, tmax;
+    float4 o, d, inv;
+    __device__ float3 org() const { return xyz(o); }
+    __device__ float3 dir() const { return xyz(d); }
+    __device__ float tmin() const { return o.w; }
+    __device__ float tmax() const { return d.w; }
+    __device__ uint32_t signs() const { return __float_as_uint(inv.w); }
 };
 
 struct HitOut {
@@ -67,23 +94,34 @@ struct HitOut {
     uint32_t aux;   // the hit triangle's per-reference payload (material class)
 };
 
+// Reciprocal direction of a ray and its sign bits, computed once by whoever
+// produces the ray: xyz = 1 / d (QBVH::Node::intersect's invRayDir, correctly
+// rounded), w = bits: bit a (0-2) set when 1/d[a] > 0, i.e. the min plane is
+// the near plane (QBVH.h:67-72); bit 3 + a set when d[a] >= 0 (the traversal
+// order's dirIsPositive, QBVH.h:296; differs from bit a for d[a] = -0).
+__device__ __forceinline__ float4 rayInv(float3 d) {
+    const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    const uint32_t bits = (ix > 0.0f ? 1u : 0u) | (iy > 0.0f ? 2u : 0u) | (iz > 0.0f ? 4u : 0u) |
+                          (d.x >= 0.0f ? 8u : 0u) | (d.y >= 0.0f ? 16u : 0u) | (d.z >= 0.0f ? 32u : 0u);
+    return make_float4(ix, iy, iz, __uint_as_float(bits));
+}
+
 // Per-ray constants of QBVH::Node::intersect and the order-table lookup.
+// Node rows (16 B each): 0 min x, 1 max x, 2 min y, 3 max y, 4 min z, 5 max z,
+// 6 child words (with the split axes); the far row of an axis is the near
+// row ^ 16.
 struct RayPre {
-    float3 inv;
-    uint32_t nearOff[3], farOff[3];   // byte offsets of the near/far plane rows per axis
-    uint32_t dirPos;                  // bit a set: dir[a] >= 0 (dirIsPositive)
+    uint32_t nearOff[3];   // byte offset of the near plane row per axis
+    uint32_t dirPos;       // bit a set: dir[a] >= 0 (dirIsPositive)
 };
 
 __device__ __forceinline__ RayPre prepareRay(const RayIn& r) {
     RayPre p;
-    p.inv = f3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
-    // QBVH.h:67-72 picks the min plane as the near plane when invRayDir > 0;
-    // node rows: 0-2 min x/y/z, 3-5 max x/y/z, 16 bytes each
-    const bool px = p.inv.x > 0.0f, py = p.inv.y > 0.0f, pz = p.inv.z > 0.0f;
-    p.nearOff[0] = px ? 0u : 48u;  p.farOff[0] = px ? 48u : 0u;
-    p.nearOff[1] = py ? 16u : 64u; p.farOff[1] = py ? 64u : 16u;
-    p.nearOff[2] = pz ? 32u : 80u; p.farOff[2] = pz ? 80u : 32u;
-    p.dirPos = (r.d.x >= 0 ? 1u : 0u) | (r.d.y >= 0 ? 2u : 0u) | (r.d.z >= 0 ? 4u : 0u);
+    const uint32_t b = r.signs();
+    p.nearOff[0] = (b & 1u) ? 0u : 16u;
+    p.nearOff[1] = (b & 2u) ? 32u : 48u;
+    p.nearOff[2] = (b & 4u) ? 64u : 80u;
+    p.dirPos = (b >> 3) & 7u;
     return p;
 }
 
@@ -97,24 +135,20 @@ __device__ __forceinline__ float4 ldg4(const char* base, uint32_t off) {
 // non-NaN values, so fmaxf/fminf (IEEE maxNum/minNum: a NaN term yields the
 // accumulator) select the same values up to the sign of a zero, which no
 // later comparison distinguishes; the hit masks are identical.
-__device__ __forceinline__ uint32_t slab4(const char* nodes, uint32_t nb, const RayIn& r, const RayPre& p,
-                                          float tmin, float tmax) {
-    const float4 nx = ldg4(nodes, nb + p.nearOff[0]), ny = ldg4(nodes, nb + p.nearOff[1]);
-    const float4 nz = ldg4(nodes, nb + p.nearOff[2]);
-    const float4 fx = ldg4(nodes, nb + p.farOff[0]), fy = ldg4(nodes, nb + p.farOff[1]);
-    const float4 fz = ldg4(nodes, nb + p.farOff[2]);
+__device__ __forceinline__ uint32_t slab4(const float4 nx, const float4 fx, const float4 ny, const float4 fy,
+                                          const float4 nz, const float4 fz, const RayIn& r, float tmin, float tmax) {
     const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
     const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
     const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
     uint32_t mask = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        float tn = fmaxf((nxa[c] - r.o.x) * p.inv.x, tmin);
-        tn = fmaxf((nya[c] - r.o.y) * p.inv.y, tn);
-        tn = fmaxf((nza[c] - r.o.z) * p.inv.z, tn);
-        float tf = fminf((fxa[c] - r.o.x) * p.inv.x, tmax);
-        tf = fminf((fya[c] - r.o.y) * p.inv.y, tf);
-        tf = fminf((fza[c] - r.o.z) * p.inv.z, tf);
+        float tn = fmaxf((nxa[c] - r.o.x) * r.inv.x, tmin);
+        tn = fmaxf((nya[c] - r.o.y) * r.inv.y, tn);
+        tn = fmaxf((nza[c] - r.o.z) * r.inv.z, tn);
+        float tf = fminf((fxa[c] - r.o.x) * r.inv.x, tmax);
+        tf = fminf((fya[c] - r.o.y) * r.inv.y, tf);
+        tf = fminf((fza[c] - r.o.z) * r.inv.z, tf);
         mask |= (tn <= tf ? 1u : 0u) << c;
     }
     return mask;
@@ -141,15 +175,16 @@ __device__ __forceinline__ void orderChildren(const uint32_t c[4], uint32_t t, u
 __device__ __forceinline__ bool triTest(const float4 a, const float4 e1v, const float4 e2v, const RayIn& r,
                                         float tmin, float tmax, float* tt, float* ob1, float* ob2) {
     const float3 v0 = xyz(a), e1 = xyz(e1v), e2 = xyz(e2v);
-    const float3 p = cross3(r.d, e2);
+    const float3 rd = r.dir();
+    const float3 p = cross3(rd, e2);
     const float det = dot3(e1, p);
     if (det == 0.0f) return false;
     const float invDet = 1.0f / det;
-    const float3 d = r.o - v0;
+    const float3 d = r.org() - v0;
     const float b1 = dot3(d, p) * invDet;
     if (b1 < 0.0f || b1 > 1.0f) return false;
     const float3 q = cross3(d, e1);
-    const float b2 = dot3(r.d, q) * invDet;
+    const float b2 = dot3(rd, q) * invDet;
     if (b2 < 0.0f || b1 + b2 > 1.0f) return false;
     const float t = dot3(e2, q) * invDet;
     if (t < tmin || t > tmax) return false;
@@ -167,43 +202,60 @@ struct TraceCounters {
 
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
+// Position of the k-th (0-based) set bit of m; m has more than k set bits.
+__device__ __forceinline__ uint32_t kthSetBit(unsigned long long m, uint32_t k) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t w = 32; w >= 1; w >>= 1) {
+        const uint32_t c = (uint32_t)__popcll((m >> pos) & ((1ull << w) - 1ull));
+        if (k >= c) { k -= c; pos += w; }
+    }
+    return pos;
+}
+
 // Persistent traversal loop over the queue entries 0..count-1.
 //
-// Lane stages: 0 idle; 1 queue entry requested; 2 ray requested; 3 tracing.
-// A lane moves one stage per iteration, so the loads a stage issues (the
-// queue entry, then the ray it names) are waited for together with the other
-// lanes' node and triangle loads of the same iteration instead of stalling
-// the wave on their own. Idle lanes are refilled once at least kRefill of the
-// wave's lanes are idle: the wave claims a chunk of entries with one atomic
-// on fetchCtr (zeroed before the launch) and hands them out in lane order,
-// so waves balance across the chip and lanes do not wait for the wave's
-// longest ray.
+// Feeding lanes: every lane holds, besides the ray it traces, the queue
+// entry of its next ray (entNext), claimed ahead of time. When a ray
+// finishes, the lane issues the loads of the next ray at once and starts
+// tracing it in the following iteration; those loads are waited for together
+// with the other lanes' node and triangle loads. Lanes that lack a next entry
+// are refilled once at least kRefill of them do: the wave claims a chunk of
+// entries with one atomic on fetchCtr (zeroed before the launch) and hands
+// them out in lane order, so waves balance across the chip. Once the queue
+// is exhausted, idle lanes take the next entries that busy lanes still hold,
+// so no ray waits behind another lane's long ray at the end of the launch.
 //
-// Tracing lanes advance their ray by ONE step per iteration, either one node
-// slab test or one triangle test, so lanes in a leaf and lanes between leaves
-// both do useful work in every iteration (a node-then-all-its-leaves step
-// would run the leaf loop at the width of the wave's longest leaf list). The
-// nearest internal child is kept in a register as the next node instead of
-// being pushed and popped back, which the reference's push-all-then-pop order
-// makes equivalent. Lane state while tracing: the next node (ni, or kNoNode
-// to pop), the ordered child words of the last node tested (v[4]) with a mask
-// of its leaf children still to intersect (leafPend, bit i = v[i]), and the
-// triangle range of the leaf being intersected [tcur, tend).
+// Tracing lanes advance their ray by one step per iteration, one node slab
+// test or one triangle test (with kFuse, a node test whose node opens a leaf
+// is followed by that leaf's first triangle test in the same iteration), so
+// lanes in a leaf and lanes between leaves both do useful work in every
+// iteration (a node-then-all-its-leaves step would run the leaf loop at the
+// width of the wave's longest leaf list). The nearest internal child is kept
+// in a register as the next node instead of being pushed and popped back,
+// which the reference's push-all-then-pop order makes equivalent. Lane state
+// while tracing: the next node (ni, or kNoNode to pop), the ordered child
+// words of the last node tested (v[4]) with a mask of its leaf children still
+// to intersect (leafPend, bit i = v[i]), and the triangle range of the leaf
+// being intersected [tcur, tend).
 //
-//   entry(i)          -> uint32_t, the i-th queue entry
-//   load(e, RayIn& r) -> bool, fills the ray of entry e; true for any hit
+//   entry(i)            -> uint32_t, the i-th queue entry
+//   load(e, RayIn& r)   -> bool, requests the ray of entry e (r.inv too when
+//                          kHasInv, else it is computed here); true for any hit
 //   finish(e, anyHit, const HitOut&)
-template <int kRefill, typename Entry, typename Load, typename Finish>
+template <int kRefill, bool kFuse, bool kHasInv, typename Entry, typename Load, typename Finish>
 __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& st, uint32_t count,
                                           uint32_t* fetchCtr, Entry&& entry, Load&& load, Finish&& finish,
                                           TraceCounters& tc) {
     constexpr uint32_t kChunk = 64;
     const char* nodes = (const char*)sc.nodes;
     const char* tris = (const char*)sc.tris;
+    const uint32_t lane = __lane_id();
     uint32_t next = 0, end = 0;   // wave-uniform chunk cursor
     bool exhausted = false;       // wave-uniform
-    uint32_t stage = 0;
-    uint32_t ent = 0;
+    uint32_t stage = 0;           // 0 idle, 2 ray requested, 3 tracing
+    uint32_t ent = 0, entNext = 0;
+    bool hasNext = false;         // entNext holds a claimed entry
     bool anyHit = false;
     RayIn r;
     RayPre p;
@@ -212,7 +264,7 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
     uint32_t aux = 0;
     int sp = 0;
     uint32_t ni = kNoNode, tcur = 0, tend = 0, leafPend = 0;
-    uint32_t v[4] = {kInvalidChild, kInvalidChild, kInvalidChild, kInvalidChild};
+    uint32_t v[4] = {kNoChild, kNoChild, kNoChild, kNoChild};
     // once the current leaf is exhausted, move to the next pending leaf child
     // in visiting order (skipping empty ones)
     auto nextLeaf = [&]() {
@@ -220,15 +272,21 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
             const uint32_t i = (uint32_t)__builtin_ctz(leafPend);
             leafPend &= leafPend - 1u;
             const uint32_t w = i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
-            tcur = w & 0x00FFFFFFu;
-            tend = tcur + ((w >> 24) & 0x7Fu);
+            tcur = w & kLeafFirstMask;
+            tend = tcur + ((w >> kLeafCountShift) & kLeafCountMask);
         }
+    };
+    auto startRay = [&](uint32_t e) {
+        ent = e;
+        anyHit = load(e, r);
+        stage = 2;
     };
     for (;;) {
         if (stage == 2) {
             // the ray requested last iteration starts at the root
+            if (!kHasInv) r.inv = rayInv(r.dir());
             p = prepareRay(r);
-            tmax = r.tmax;
+            tmax = r.tmax();
             prim = -1;
             aux = 0;
             sp = 0;
@@ -236,45 +294,126 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
             tcur = tend = 0;
             leafPend = 0;
             stage = 3;
-        } else if (stage == 1) {
-            anyHit = load(ent, r);
-            stage = 2;
         }
-        const unsigned long long idle = __ballot(stage == 0);
-        if (!exhausted && __popcll(idle) >= kRefill) {
+        // idle lanes start the entry they claimed in an earlier iteration
+        if (stage == 0 && hasNext) {
+            startRay(entNext);
+            hasNext = false;
+        }
+        if (exhausted) {
+            // the k-th idle lane takes the next entry of the k-th lane holding one
+            const unsigned long long idleM = __ballot(stage == 0);
+            const unsigned long long donorM = __ballot(hasNext);
+            if (idleM != 0ull && donorM != 0ull) {
+                const uint32_t nIdle = (uint32_t)__popcll(idleM), nDonor = (uint32_t)__popcll(donorM);
+                const uint32_t rI = __builtin_amdgcn_mbcnt_hi((uint32_t)(idleM >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idleM, 0u));
+                const uint32_t rD = __builtin_amdgcn_mbcnt_hi((uint32_t)(donorM >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)donorM, 0u));
+                const bool takes = stage == 0 && rI < nDonor;
+                const uint32_t src = takes ? kthSetBit(donorM, rI) : lane;
+                const uint32_t got = (uint32_t)__shfl((int)entNext, (int)src);
+                if (hasNext && rD < nIdle) hasNext = false;
+                if (takes) startRay(got);
+            }
+        }
+        // lanes without a next entry claim one; its load completes with this
+        // iteration's traversal loads and is first used in a later step
+        const unsigned long long lack = __ballot(!hasNext);
+        if (!exhausted && __popcll(lack) >= kRefill) {
             if (next >= end) {
                 uint32_t base = 0;
-                if (__lane_id() == 0) base = atomicAdd(fetchCtr, kChunk);
+                if (lane == 0) base = atomicAdd(fetchCtr, kChunk);
                 base = __shfl(base, 0);
                 next = base;
                 end = base + kChunk < count ? base + kChunk : count;
                 if (base >= count) { exhausted = true; next = end = count; }
             }
             const uint32_t avail = end - next;
-            // rank of this lane among the idle lanes
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-            if (stage == 0 && rank < avail) {
-                ent = entry(next + rank);
-                stage = 1;
+            // rank of this lane among the lanes lacking a next entry
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(lack >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)lack, 0u));
+            if (!hasNext && rank < avail) {
+                entNext = entry(next + rank);
+                hasNext = true;
             }
-            const uint32_t need = (uint32_t)__popcll(idle);
+            const uint32_t need = (uint32_t)__popcll(lack);
             next += need < avail ? need : avail;
         }
-        if (__ballot(stage != 0) == 0ull) {
+        if (__ballot(stage != 0 || hasNext) == 0ull) {
             if (exhausted) break;
             continue;
         }
         if (stage == 3) {
             bool done = false;
+            if (tcur >= tend) {
+                // one node
+                if (ni == kNoNode) ni = stackPop(st, sp);
+                ++tc.nodes;
+                tc.nodesAny += anyHit ? 1u : 0u;
+                // all rows the step needs are requested together: one round
+                // trip, from the LDS copy of the top of the tree when the node
+                // is there
+                float4 nx, fx, ny, fy, nz, fz;
+                uint4 cw;
+                if (ni < st.treeletNodes) {
+                    const lds_u8* q = st.treelet + ni * (kTreeletRows * 16u);
+                    nx = ldsf4(q, p.nearOff[0]); fx = ldsf4(q, p.nearOff[0] ^ 16u);
+                    ny = ldsf4(q, p.nearOff[1]); fy = ldsf4(q, p.nearOff[1] ^ 16u);
+                    nz = ldsf4(q, p.nearOff[2]); fz = ldsf4(q, p.nearOff[2] ^ 16u);
+                    const float4 w = ldsf4(q, 96u);
+                    cw = make_uint4(__float_as_uint(w.x), __float_as_uint(w.y), __float_as_uint(w.z),
+                                    __float_as_uint(w.w));
+                } else {
+                    const uint32_t nb = ni * 128u;
+                    nx = ldg4(nodes, nb + p.nearOff[0]); fx = ldg4(nodes, nb + (p.nearOff[0] ^ 16u));
+                    ny = ldg4(nodes, nb + p.nearOff[1]); fy = ldg4(nodes, nb + (p.nearOff[1] ^ 16u));
+                    nz = ldg4(nodes, nb + p.nearOff[2]); fz = ldg4(nodes, nb + (p.nearOff[2] ^ 16u));
+                    cw = *(const uint4*)(nodes + nb + 96u);
+                }
+                // keeps the child-word loads beside the plane loads (the
+                // compiler would otherwise issue them after the slab test)
+                asm volatile("" ::"v"(cw.x), "v"(cw.y), "v"(cw.z), "v"(cw.w));
+                const uint32_t hitMask = slab4(nx, fx, ny, fy, nz, fz, r, r.tmin(), tmax);
+                ni = kNoNode;
+                if (hitMask) {
+                    const uint32_t top = (cw.x >> kAxisShift) & 3u, left = (cw.y >> kAxisShift) & 3u,
+                                   right = (cw.z >> kAxisShift) & 3u;
+                    const uint32_t c[4] = {(hitMask & 1u) ? cw.x : kNoChild, (hitMask & 2u) ? cw.y : kNoChild,
+                                           (hitMask & 4u) ? cw.z : kNoChild, (hitMask & 8u) ? cw.w : kNoChild};
+                    orderChildren(c, (p.dirPos >> top) & 1u, (p.dirPos >> left) & 1u, (p.dirPos >> right) & 1u, v);
+                    // internal children (leaf bit clear): far-to-near, each
+                    // pushing the one held before it
+#pragma unroll
+                    for (int i = 3; i >= 0; --i) {
+                        if (!(v[i] & kLeafBit)) {
+                            if (ni != kNoNode) stackPush(st, sp, ni, tc.overflow);
+                            ni = v[i] & kNodeIndexMask;
+                        }
+                    }
+                    // leaf children with triangles, in visiting order
+                    leafPend = 0;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        leafPend |= ((v[i] & kLeafBit) && ((v[i] >> kLeafCountShift) & kLeafCountMask) ? 1u : 0u) << i;
+                    nextLeaf();
+                }
+                // kFuse: a node that opens a leaf has its first triangle tested in
+                // the same iteration; otherwise the next iteration does
+                if (!kFuse) goto stepDone;
+            }
             if (tcur < tend) {
                 // one triangle of the current leaf
                 const uint32_t tb = tcur * 48u;
                 const float4 a = ldg4(tris, tb), e1 = ldg4(tris, tb + 16u), e2 = ldg4(tris, tb + 32u);
+                // one round trip for the whole reference (the compiler would
+                // otherwise request v0 only after the determinant test)
+                asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(e1.x), "v"(e1.y), "v"(e1.z),
+                             "v"(e1.w), "v"(e2.x), "v"(e2.y), "v"(e2.z));
                 float t, b1, b2;
                 ++tc.tris;
                 tc.trisAny += anyHit ? 1u : 0u;
-                if (triTest(a, e1, e2, r, r.tmin, tmax, &t, &b1, &b2)) {
+                if (triTest(a, e1, e2, r, r.tmin(), tmax, &t, &b1, &b2)) {
                     tmax = t;
                     prim = (int32_t)__float_as_uint(a.w);
                     aux = __float_as_uint(e1.w);
@@ -284,36 +423,8 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
                 }
                 ++tcur;
                 nextLeaf();
-            } else {
-                if (ni == kNoNode) ni = stackPop(st, sp);
-                const uint32_t nb = ni * 128u;
-                ++tc.nodes;
-                tc.nodesAny += anyHit ? 1u : 0u;
-                const uint32_t hitMask = slab4(nodes, nb, r, p, r.tmin, tmax);
-                ni = kNoNode;
-                if (hitMask) {
-                    const uint4 cw = *(const uint4*)(nodes + nb + 96u);
-                    const uint32_t axes = *(const uint32_t*)(nodes + nb + 112u);
-                    const uint32_t c[4] = {(hitMask & 1u) ? cw.x : kInvalidChild, (hitMask & 2u) ? cw.y : kInvalidChild,
-                                           (hitMask & 4u) ? cw.z : kInvalidChild, (hitMask & 8u) ? cw.w : kInvalidChild};
-                    orderChildren(c, (p.dirPos >> (axes & 3u)) & 1u, (p.dirPos >> ((axes >> 2) & 3u)) & 1u,
-                                  (p.dirPos >> ((axes >> 4) & 3u)) & 1u, v);
-                    // internal children (leaf bit clear; the invalid word has it set):
-                    // far-to-near, each pushing the one held before it
-#pragma unroll
-                    for (int i = 3; i >= 0; --i) {
-                        if (!(v[i] & kLeafBit)) {
-                            if (ni != kNoNode) stackPush(st, sp, ni, tc.overflow);
-                            ni = v[i];
-                        }
-                    }
-                    leafPend = 0;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        leafPend |= ((v[i] & kLeafBit) && v[i] != kInvalidChild ? 1u : 0u) << i;
-                    nextLeaf();
-                }
             }
+        stepDone:
             if (done || (tcur >= tend && ni == kNoNode && sp == 0)) {
                 HitOut h;
                 h.t = tmax;
@@ -325,6 +436,10 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
                 ++tc.rays;
                 tc.raysAny += anyHit ? 1u : 0u;
                 stage = 0;
+                if (hasNext) {
+                    startRay(entNext);
+                    hasNext = false;
+                }
             }
         }
     }
