@@ -656,109 +656,145 @@ __global__ void k_init_slots(PathState ps, uint32_t spp, uint32_t slotsPerPixel,
     }
 }
 
-// Retire finished samples into the film, start the next sample of the slot.
+// C *= weight; film add (PTRenderer.cpp:129-138, ImageSensor.cpp:125-130,
+// SpectrumStorage::add spectrum_types.h:725-736). The slot owns its film cell
+// (plane s / pixels), so the add is a plain read-modify-write. A sample with
+// no radiance leaves the sums as they are and is not written.
+__device__ __forceinline__ void fileSample(const PathState& ps, uint32_t s, uint32_t pixel, const float4& misc,
+                                           float* c) {
+    const float recBinWidth = kNumL / (kLambdaHighD - kLambdaLowD);
+    bool finite = true, any = false;
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) {
+        c[k] *= misc.w;
+        finite = finite && isfinite(c[k]);
+        any = any || c[k] != 0.0f;
+    }
+    if (!finite) {
+        atomicAdd(&ps.ctr[C_NONFINITE], 1u);
+        return;
+    }
+    if (!any) return;
+    float add[kNumL];
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) add[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) {
+        float l = lambdaOf(k, misc.x);
+        uint32_t b = min((uint32_t)((l - kLambdaLowD) / (kLambdaHighD - kLambdaLowD) * kNumL), (uint32_t)(kNumL - 1));
+        add[b] += c[k] * recBinWidth;
+    }
+    const uint32_t plane = s / ps.pixels;
+    double* f = ps.film + ((size_t)plane * ps.pixels + pixel) * kNumL;
+#pragma unroll
+    for (int k = 0; k < kNumL; k += 2) {
+        double2 v = *(double2*)(f + k);
+        v.x += (double)add[k];
+        v.y += (double)add[k + 1];
+        *(double2*)(f + k) = v;
+    }
+}
+
+// The slot's next camera sample (PTRenderer.cpp:108-124,
+// PerspectiveCamera.cpp:31-82): draws pixel jitter, wavelength offset, hero
+// wavelength and lens position from the slot's stream, writes the ray and
+// misc, and sets info for a path of length 0. The throughput (all ones) and
+// radiance (zero) of a fresh path are implied by path length 0 and not
+// stored; the first shading kernel starts from them.
+__device__ __forceinline__ void startCameraSample(const DevScene& sc, const PathState& ps, uint32_t s, uint4& info,
+                                                  uint4& rng, float4& misc) {
+    const DevCamera& cam = sc.cam;
+    const uint32_t px = info.x % cam.width, py = info.x / cam.width;
+    const float fx = px + xsFloat(rng);
+    const float fy = py + xsFloat(rng);
+    const float off = xsFloat(rng);
+    const float uL = xsFloat(rng);
+    const uint32_t hero = min((uint32_t)(uint16_t)(kNumL * uL), (uint32_t)(kNumL - 1));
+    const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
+    float lx = 0, ly = 0;
+    concentricDisk(lu0, lu1, &lx, &ly);
+    const float3 orgL = f3(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f);
+    const float* M = cam.l2w;
+    const float3 org = f3(M[0] * orgL.x + M[1] * orgL.y + M[2] * orgL.z + M[3],
+                          M[4] * orgL.x + M[5] * orgL.y + M[6] * orgL.z + M[7],
+                          M[8] * orgL.x + M[9] * orgL.y + M[10] * orgL.z + M[11]);
+    const float* N = cam.nrm;
+    const float3 fz = normalize3(f3(N[2], N[5], N[8]));
+    const float3 fxv = normalize3(f3(M[0], M[4], M[8]));
+    const float3 fyv = cross3(fz, fxv);
+    const float ux = fx / cam.width, uy = fy / cam.height;
+    const float3 pFocus = f3(cam.opWidth * (0.5f - ux), cam.opHeight * (0.5f - uy), cam.objDist);
+    const float3 dl = normalize3(pFocus - orgL);
+    const float dirPDF = cam.imgDist * cam.imgDist / ((dl.z * dl.z * dl.z) * cam.imgPlaneArea);
+    const float3 dir = f3(dot3(f3(fxv.x, fyv.x, fz.x), dl), dot3(f3(fxv.y, fyv.y, fz.y), dl),
+                          dot3(f3(fxv.z, fyv.z, fz.z), dl));
+    const float areaPDF = cam.lensRadius > 0.0f
+                              ? (float)(1.0 / (3.14159265358979323846 * cam.lensRadius * cam.lensRadius))
+                              : 1.0f;
+    const float selectWLPDF = kNumL / (kLambdaHighD - kLambdaLowD);
+    const float cosT = fabsf(dot3(dir, fz));
+    const float weight = cosT / (areaPDF * dirPDF * selectWLPDF);
+    float one[kNumL];
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) one[k] = 1.0f;
+    const float initY = importance16(one, (int)hero);
+    ps.f4(s, SF_RO) = make_float4(org.x, org.y, org.z, 0.0f);
+    ps.f4(s, SF_RD) = make_float4(dir.x, dir.y, dir.z, INFINITY);
+    ps.f4(s, SF_RINV) = rayInv(dir);
+    misc = make_float4(off, initY, 0.0f, weight);
+    info.y = F_HAS_SAMPLE | (hero << 8);   // pathLength 0
+    info.z += 1;
+}
+
+// A path that ended: its sample is filed and, while the slot has samples
+// left, the next one starts. Returns the slot's new code (EXT or IDLE).
+// info, misc and rng are written back here.
+__device__ __forceinline__ uint8_t retireAndRestart(const DevScene& sc, const PathState& ps, uint32_t s, uint4 info,
+                                                    float4 misc, uint4 rng, float* rad, unsigned long long& samples) {
+    if (info.y & F_HAS_SAMPLE) {
+        fileSample(ps, s, info.x, misc, rad);
+        ++samples;
+    }
+    info.y &= ~F_HAS_SAMPLE;
+    uint8_t code = CODE_IDLE;
+    if (info.z < info.w) {
+        startCameraSample(sc, ps, s, info, rng, misc);
+        ps.f4(s, SF_MISC) = misc;
+        ps.u4(s, SF_RNG) = rng;
+        code = CODE_EXT;
+    }
+    ps.u4(s, SF_INFO) = info;
+    return code;
+}
+
+// Loads the radiance of a slot's current path (zero for a fresh path) with
+// any resolved light-sample contribution added.
+__device__ __forceinline__ void loadRadiance(const PathState& ps, uint32_t s, uint32_t pathLen, float* rad) {
+    if (pathLen == 0) {
+#pragma unroll
+        for (int k = 0; k < kNumL; ++k) rad[k] = 0.0f;
+    } else {
+        loadPlanes(ps, s, SF_SP, rad);
+    }
+    resolveShadow(ps, s, rad);
+}
+
+// Slots whose path ended with a light sample still being traced (and the
+// slots' first samples): file the finished sample, start the next one.
 __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_DONE];
     const uint32_t* qin = queue(ps, Q_DONE);
     // the trace queue is rebuilt after this kernel
     if (blockIdx.x == 0 && threadIdx.x == 0) ps.ctr[C_TRACE] = 0;
-    const DevCamera cam = sc.cam;
-    const float recBinWidth = kNumL / (kLambdaHighD - kLambdaLowD);
     unsigned long long samples = 0;
-    for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
-        const uint32_t i = base + threadIdx.x;
-        const bool valid = i < count;
-        bool emit = false;
-        uint32_t s = 0;
-        if (valid) {
-            s = qin[i];
-            uint4 info = ps.u4(s, SF_INFO);
-            float4 misc = ps.f4(s, SF_MISC);
-            if (info.y & F_HAS_SAMPLE) {
-                // C *= weight; film add (PTRenderer.cpp:129-138, ImageSensor.cpp:125-130,
-                // SpectrumStorage::add spectrum_types.h:725-736)
-                float c[kNumL];
-                loadPlanes(ps, s, SF_SP, c);
-                resolveShadow(ps, s, c);
-                bool finite = true;
-#pragma unroll
-                for (int k = 0; k < kNumL; ++k) { c[k] *= misc.w; finite = finite && isfinite(c[k]); }
-                if (finite) {
-                    float add[kNumL];
-#pragma unroll
-                    for (int k = 0; k < kNumL; ++k) add[k] = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < kNumL; ++k) {
-                        float l = lambdaOf(k, misc.x);
-                        uint32_t b = min((uint32_t)((l - kLambdaLowD) / (kLambdaHighD - kLambdaLowD) * kNumL), (uint32_t)(kNumL - 1));
-                        add[b] += c[k] * recBinWidth;
-                    }
-                    uint32_t plane = s / ps.pixels;
-                    double* f = ps.film + ((size_t)plane * ps.pixels + info.x) * kNumL;
-#pragma unroll
-                    for (int k = 0; k < kNumL; k += 2) {
-                        double2 v = *(double2*)(f + k);
-                        v.x += (double)add[k];
-                        v.y += (double)add[k + 1];
-                        *(double2*)(f + k) = v;
-                    }
-                } else {
-                    atomicAdd(&ps.ctr[C_NONFINITE], 1u);
-                }
-                ++samples;
-            }
-            info.y &= ~F_HAS_SAMPLE;
-            if (info.z < info.w) {
-                // new camera sample (PTRenderer.cpp:108-124, PerspectiveCamera.cpp:31-82)
-                uint4 rng = ps.u4(s, SF_RNG);
-                const uint32_t px = info.x % cam.width, py = info.x / cam.width;
-                const float fx = px + xsFloat(rng);
-                const float fy = py + xsFloat(rng);
-                const float off = xsFloat(rng);
-                const float uL = xsFloat(rng);
-                const uint32_t hero = min((uint32_t)(uint16_t)(kNumL * uL), (uint32_t)(kNumL - 1));
-                const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
-                float lx = 0, ly = 0;
-                concentricDisk(lu0, lu1, &lx, &ly);
-                const float3 orgL = f3(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f);
-                const float* M = cam.l2w;
-                const float3 org = f3(M[0] * orgL.x + M[1] * orgL.y + M[2] * orgL.z + M[3],
-                                      M[4] * orgL.x + M[5] * orgL.y + M[6] * orgL.z + M[7],
-                                      M[8] * orgL.x + M[9] * orgL.y + M[10] * orgL.z + M[11]);
-                const float* N = cam.nrm;
-                const float3 fz = normalize3(f3(N[2], N[5], N[8]));
-                const float3 fxv = normalize3(f3(M[0], M[4], M[8]));
-                const float3 fyv = cross3(fz, fxv);
-                const float ux = fx / cam.width, uy = fy / cam.height;
-                const float3 pFocus = f3(cam.opWidth * (0.5f - ux), cam.opHeight * (0.5f - uy), cam.objDist);
-                const float3 dl = normalize3(pFocus - orgL);
-                const float dirPDF = cam.imgDist * cam.imgDist / ((dl.z * dl.z * dl.z) * cam.imgPlaneArea);
-                const float3 dir = f3(dot3(f3(fxv.x, fyv.x, fz.x), dl), dot3(f3(fxv.y, fyv.y, fz.y), dl),
-                                      dot3(f3(fxv.z, fyv.z, fz.z), dl));
-                const float areaPDF = cam.lensRadius > 0.0f
-                                          ? (float)(1.0 / (3.14159265358979323846 * cam.lensRadius * cam.lensRadius))
-                                          : 1.0f;
-                const float selectWLPDF = kNumL / (kLambdaHighD - kLambdaLowD);
-                const float cosT = fabsf(dot3(dir, fz));
-                const float weight = cosT / (areaPDF * dirPDF * selectWLPDF);
-                float one[kNumL], zero[kNumL];
-#pragma unroll
-                for (int k = 0; k < kNumL; ++k) { one[k] = 1.0f; zero[k] = 0.0f; }
-                storePlanes(ps, s, SF_ALPHA, one);
-                storePlanes(ps, s, SF_SP, zero);
-                const float initY = importance16(one, (int)hero);
-                ps.f4(s, SF_RO) = make_float4(org.x, org.y, org.z, 0.0f);
-                ps.f4(s, SF_RD) = make_float4(dir.x, dir.y, dir.z, INFINITY);
-                ps.f4(s, SF_RINV) = rayInv(dir);
-                ps.f4(s, SF_MISC) = make_float4(off, initY, 0.0f, weight);
-                ps.u4(s, SF_RNG) = rng;
-                info.y = F_HAS_SAMPLE | (hero << 8);   // pathLength 0
-                info.z += 1;
-                emit = true;
-            }
-            ps.u4(s, SF_INFO) = info;
-            ps.code[s] = emit ? CODE_EXT : CODE_IDLE;
-        }
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+        const uint32_t s = qin[i];
+        const uint4 info = ps.u4(s, SF_INFO);
+        const float4 misc = ps.f4(s, SF_MISC);
+        const uint4 rng = ps.u4(s, SF_RNG);
+        float rad[kNumL];
+        if (info.y & F_HAS_SAMPLE) loadRadiance(ps, s, info.y >> 16, rad);
+        ps.code[s] = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
     }
     waveAdd64(&ps.stat[S_SAMPLES], samples);
 }
@@ -912,8 +948,12 @@ __device__ __forceinline__ bool sampleBsdf(const DevMaterial& m, const float* A,
 }
 
 // Shading of one queue. CLS selects the scattering model compiled in.
+// Register budget of the shading kernels: the diffuse kernel (most of the
+// work) is held to 3 waves per SIMD; the others use what they need.
 template <uint32_t CLS>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CLS == CLS_DIFFUSE ? 3 : 1,
+                                                                          CLS == CLS_DIFFUSE ? 3 : 10)))
+void k_shade(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_CLS0 + CLS];
     const uint32_t* qin = queue(ps, Q_CLS0 + CLS);
     // the done queue and the pending count of the previous iteration were
@@ -923,15 +963,24 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
         ps.ctr[C_DONE] = 0;
         ps.ctr[C_PENDING] = 0;
     }
+    unsigned long long samples = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < count;
-        bool cont = false, shadow = false, done = false;
+        uint8_t code = CODE_IDLE;
         uint32_t s = 0;
         if (valid) {
             s = qin[i];
-            done = true;
-            if (CLS != CLS_MISS) {
+            if (CLS == CLS_MISS) {
+                // the extension ray left the scene: the path ends here
+                const uint4 info = ps.u4(s, SF_INFO);
+                const float4 misc = ps.f4(s, SF_MISC);
+                const uint4 rng = ps.u4(s, SF_RNG);
+                float rad[kNumL];
+                loadRadiance(ps, s, info.y >> 16, rad);
+                code = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
+            } else {
+                bool cont = false, shadow = false;
                 uint4 info = ps.u4(s, SF_INFO);
                 float4 misc = ps.f4(s, SF_MISC);
                 uint4 rng = ps.u4(s, SF_RNG);
@@ -943,9 +992,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
                 const int hero = (int)((info.y >> 8) & 0xFFu);
                 uint32_t pathLen = info.y >> 16;
                 float alpha[kNumL], rad[kNumL];
-                loadPlanes(ps, s, SF_ALPHA, alpha);
-                loadPlanes(ps, s, SF_SP, rad);
-                resolveShadow(ps, s, rad);
+                if (pathLen == 0) {
+                    // a fresh path: throughput one (PTRenderer.cpp:115)
+#pragma unroll
+                    for (int k = 0; k < kNumL; ++k) alpha[k] = 1.0f;
+                } else {
+                    loadPlanes(ps, s, SF_ALPHA, alpha);
+                }
+                loadRadiance(ps, s, pathLen, rad);
                 const float3 dirOut = toLocal(sp, -rd);
                 // emission (PTRenderer.cpp:162-166 first hit, :247-263 implicit with MIS)
                 if (mat.emit >= 0 && dirOut.z > 0.0f) {
@@ -1065,18 +1119,24 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PathState ps) {
                         }
                     }
                 }
-                storePlanes(ps, s, SF_SP, rad);
-                if (cont) storePlanes(ps, s, SF_ALPHA, alpha);
                 info.y = (info.y & 0xFFFFu) | (pathLen << 16);
-                ps.u4(s, SF_INFO) = info;
-                ps.f4(s, SF_MISC) = misc;
-                ps.u4(s, SF_RNG) = rng;
-                done = !cont;
+                if (cont || shadow) {
+                    // the path goes on, or it ended with a light sample still to
+                    // trace: its sample is filed once the shadow ray is resolved
+                    storePlanes(ps, s, SF_SP, rad);
+                    if (cont) storePlanes(ps, s, SF_ALPHA, alpha);
+                    ps.u4(s, SF_INFO) = info;
+                    ps.f4(s, SF_MISC) = misc;
+                    ps.u4(s, SF_RNG) = rng;
+                    code = (uint8_t)((cont ? CODE_EXT : CODE_DONE) | (shadow ? CODE_SHADOW : 0));
+                } else {
+                    code = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
+                }
             }
         }
-        if (valid) ps.code[s] = (uint8_t)((cont ? CODE_EXT : CODE_DONE) | (shadow ? CODE_SHADOW : 0));
-        (void)done;
+        if (valid) ps.code[s] = code;
     }
+    waveAdd64(&ps.stat[S_SAMPLES], samples);
 }
 
 // Slot-ordered compaction of the per-slot codes into queues (one pass over
