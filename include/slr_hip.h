@@ -164,6 +164,11 @@ SLRH_API int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* visibili
 
 enum slrh_integrator { SLRH_PT = 0 };
 
+enum slrh_render_flags {
+    SLRH_FLAG_TIME_KERNELS = 1,   /* time the kernel groups with HIP events (stats 5, 8, 9) */
+    SLRH_FLAG_NO_GRAPH = 2        /* launch each iteration's kernels instead of replaying a recorded graph */
+};
+
 typedef struct slrh_render_settings {
     uint32_t width, height;   /* RenderSettingItem::ImageWidth/ImageHeight */
     uint32_t spp;             /* PTRenderer(spp) */
@@ -171,7 +176,7 @@ typedef struct slrh_render_settings {
     uint32_t seed;            /* RenderSettingItem::RNGSeed */
     uint32_t integrator;      /* slrh_integrator */
     uint32_t slots_per_pixel; /* live paths per pixel (0 = automatic) */
-    uint32_t flags;           /* bit 0: time the kernels with HIP events */
+    uint32_t flags;           /* SLRH_FLAG_* */
     uint32_t tune_trace;      /* traversal kernel variant + 1 (0 = default) */
     uint32_t reserved;        /* must be 0 */
 } slrh_render_settings;
@@ -210,6 +215,25 @@ SLRH_API int slrh_film_read(const slrh_film* f, int format, double scale, void* 
  * [14] triangles by shadow rays. Kernel seconds are 0 unless flags bit 0. */
 SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
                          double* stats);
+
+/* One frame of a batch: camera (as slrh_camera_perspective), resolution,
+ * samples per pixel, seed and the film it accumulates into. */
+typedef struct slrh_frame {
+    float l2w[16];
+    float sensitivity, aspect, fov_y, lens_radius, img_dist, obj_dist;
+    uint32_t width, height, spp, seed;
+    slrh_film* film;
+} slrh_frame;
+
+/* Renders n frames of the scene in one wavefront: their paths share one pool
+ * of path slots and every kernel launch (PTRenderer::render per frame,
+ * libSLR/Renderer/PTRenderer.cpp:26-101, batched). settings supplies
+ * max_depth, integrator, slots_per_pixel, flags and tune_trace; width,
+ * height, spp and seed come from the frames. A frame's film holds the same
+ * result as rendering it alone with the same seed and slots_per_pixel.
+ * stats as for slrh_render, summed over the batch. */
+SLRH_API int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n,
+                               const slrh_render_settings* settings, double* stats);
 
 #ifdef __cplusplus
 }

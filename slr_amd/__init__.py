@@ -37,7 +37,7 @@ EXPORTS = [
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
     "slrh_intersect", "slrh_occluded",
     "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read",
-    "slrh_render", "slrh_trace_variants",
+    "slrh_render", "slrh_render_batch", "slrh_trace_variants",
 ]
 
 _lib = None
@@ -53,6 +53,16 @@ class _Rays(ctypes.Structure):
 
 class _Hits(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in ("t", "prim", "b0", "b1")]
+
+
+class _Frame(ctypes.Structure):
+    _fields_ = [("l2w", ctypes.c_float * 16)] + [(n, ctypes.c_float) for n in
+                ("sensitivity", "aspect", "fov_y", "lens_radius", "img_dist", "obj_dist")] + \
+               [(n, ctypes.c_uint32) for n in ("width", "height", "spp", "seed")] + [("film", ctypes.c_void_p)]
+
+
+FLAG_TIME_KERNELS = 1
+FLAG_NO_GRAPH = 2
 
 
 class _Settings(ctypes.Structure):
@@ -102,6 +112,7 @@ def load_library(path=None):
         "slrh_film_clear": (i32, [vp]),
         "slrh_film_read": (i32, [vp, i32, f64, vp, ctypes.c_size_t]),
         "slrh_render": (i32, [vp, vp, vp, vp]),
+        "slrh_render_batch": (i32, [vp, vp, u32, vp, vp]),
         "slrh_trace_variants": (i32, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -364,11 +375,40 @@ class PTRenderer:
         self.spp = int(spp)
 
     def render(self, scene, film, seed=1509761209, max_depth=0, slots_per_pixel=0, time_kernels=False,
-               tune_trace=0):
+               tune_trace=0, graph=True):
+        flags = (FLAG_TIME_KERNELS if time_kernels else 0) | (0 if graph else FLAG_NO_GRAPH)
         s = _Settings(film.width, film.height, self.spp, max_depth, seed & 0xFFFFFFFF, 0, slots_per_pixel,
-                      1 if time_kernels else 0, tune_trace, 0)
+                      flags, tune_trace, 0)
         st = np.zeros(len(STAT_KEYS))
         scene.ctx.check(scene.lib.slrh_render(scene.h, ctypes.byref(s), film.h, _ptr(st)), "slrh_render")
+        return dict(zip(STAT_KEYS, st.tolist()))
+
+    def render_batch(self, scene, frames, max_depth=0, slots_per_pixel=0, time_kernels=False, tune_trace=0,
+                     graph=True):
+        """frames: list of dicts {film, camera: dict(l2w, aspect, fov_y, lens_radius, img_dist, obj_dist,
+        sensitivity), seed, spp (default self.spp)}; all frames in one wavefront."""
+        arr = (_Frame * max(1, len(frames)))()
+        for i, f in enumerate(frames):
+            cam = f["camera"]
+            m = np.asarray(cam["l2w"], dtype=np.float32).reshape(4, 4).T.ravel()  # column-major
+            fr = arr[i]
+            for k in range(16):
+                fr.l2w[k] = float(m[k])
+            fr.sensitivity = float(cam.get("sensitivity", 1.0))
+            fr.aspect = float(cam["aspect"])
+            fr.fov_y = float(cam["fov_y"])
+            fr.lens_radius = float(cam.get("lens_radius", 0.0))
+            fr.img_dist = float(cam.get("img_dist", 1.0))
+            fr.obj_dist = float(cam.get("obj_dist", 1.0))
+            fr.width, fr.height = f["film"].width, f["film"].height
+            fr.spp = int(f.get("spp", self.spp))
+            fr.seed = int(f.get("seed", 1509761209)) & 0xFFFFFFFF
+            fr.film = f["film"].h
+        flags = (FLAG_TIME_KERNELS if time_kernels else 0) | (0 if graph else FLAG_NO_GRAPH)
+        s = _Settings(0, 0, self.spp, max_depth, 0, 0, slots_per_pixel, flags, tune_trace, 0)
+        st = np.zeros(len(STAT_KEYS))
+        scene.ctx.check(scene.lib.slrh_render_batch(scene.h, arr, len(frames), ctypes.byref(s), _ptr(st)),
+                        "slrh_render_batch")
         return dict(zip(STAT_KEYS, st.tolist()))
 
 

@@ -495,6 +495,17 @@ int slrh_trace_variants(int* n, int* ldsDepth, int* refill, int* fused) {
     return SLRH_OK;
 }
 
+namespace {
+void copyStats(const RenderStats& st, double* stats) {
+    if (!stats) return;
+    stats[0] = st.raysTraced; stats[1] = st.samples; stats[2] = st.kernelSeconds; stats[3] = st.nonFinite;
+    stats[4] = st.iterations; stats[5] = st.traceSeconds; stats[6] = st.traceLaunches; stats[7] = st.extendRays;
+    stats[8] = st.shadeSeconds; stats[9] = st.otherSeconds; stats[10] = st.shadowRays;
+    stats[11] = st.nodesVisited; stats[12] = st.trisTested;
+    stats[13] = st.nodesShadow; stats[14] = st.trisShadow;
+}
+}  // namespace
+
 int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film, double* stats) {
     if (!s || !set || !film) return SLRH_ERR_INVALID;
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "render before slrh_scene_build");
@@ -509,17 +520,55 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     if (set->reserved != 0) return fail(s->ctx, SLRH_ERR_INVALID, "reserved settings field must be 0");
     cfg.traceVariant = set->tune_trace == 0 ? 0 : (int)set->tune_trace - 1;
     if (cfg.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
+    cfg.useGraph = (set->flags & SLRH_FLAG_NO_GRAPH) == 0;
     RenderStats st;
     std::string err;
     int r = slrh::render(s->dev, cfg, film->dev, &st, &err);
     if (r) return fail(s->ctx, r, err);
-    if (stats) {
-        stats[0] = st.raysTraced; stats[1] = st.samples; stats[2] = st.kernelSeconds; stats[3] = st.nonFinite;
-        stats[4] = st.iterations; stats[5] = st.traceSeconds; stats[6] = st.traceLaunches; stats[7] = st.extendRays;
-        stats[8] = st.shadeSeconds; stats[9] = st.otherSeconds; stats[10] = st.shadowRays;
-        stats[11] = st.nodesVisited; stats[12] = st.trisTested;
-        stats[13] = st.nodesShadow; stats[14] = st.trisShadow;
+    copyStats(st, stats);
+    return SLRH_OK;
+}
+
+int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const slrh_render_settings* set,
+                      double* stats) {
+    if (!s || !set || (n && !frames)) return SLRH_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "render before slrh_scene_build");
+    if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
+    if (set->integrator != SLRH_PT) return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
+    if (set->reserved != 0) return fail(s->ctx, SLRH_ERR_INVALID, "reserved settings field must be 0");
+    if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
+    std::vector<JobSpec> specs(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const slrh_frame& f = frames[i];
+        if (!f.film) return fail(s->ctx, SLRH_ERR_INVALID, "frame without a film");
+        if (f.width != f.film->w || f.height != f.film->h) return fail(s->ctx, SLRH_ERR_INVALID, "film size mismatch");
+        JobSpec& js = specs[i];
+        memcpy(js.camera.l2w, f.l2w, sizeof(js.camera.l2w));
+        js.camera.sensitivity = f.sensitivity;
+        js.camera.aspect = f.aspect;
+        js.camera.fovY = f.fov_y;
+        js.camera.lensRadius = f.lens_radius;
+        js.camera.imgDist = f.img_dist;
+        js.camera.objDist = f.obj_dist;
+        js.width = f.width;
+        js.height = f.height;
+        js.spp = f.spp;
+        js.seed = f.seed;
+        js.maxDepth = set->max_depth;
+        js.film = f.film->dev;
     }
+    RenderOptions opt;
+    opt.slotsPerPixel = set->slots_per_pixel;
+    opt.timeKernels = (set->flags & SLRH_FLAG_TIME_KERNELS) != 0;
+    opt.useGraph = (set->flags & SLRH_FLAG_NO_GRAPH) == 0;
+    opt.traceVariant = set->tune_trace == 0 ? 0 : (int)set->tune_trace - 1;
+    if (opt.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    RenderStats st;
+    std::string err;
+    int r = renderJobs(s->dev, specs.data(), n, opt, &st, &err);
+    if (r) return fail(s->ctx, r, err);
+    copyStats(st, stats);
     return SLRH_OK;
 }
 

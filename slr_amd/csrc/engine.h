@@ -42,6 +42,7 @@ struct RenderConfig {
     uint32_t width, height, spp, maxDepth, seed, slotsPerPixel;
     bool timeKernels;
     int traceVariant = 0;   // traversal kernel variant (see traceVariant)
+    bool useGraph = true;
 };
 
 struct RenderStats {
@@ -69,6 +70,23 @@ int filmDownload(const DeviceFilm* f, std::vector<double>* strata, std::string* 
 void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h);
 
 int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err);
+
+// One frame of a batch: its camera, resolution, samples, depth cap, seed
+// and the film it accumulates into.
+struct JobSpec {
+    CameraParams camera;
+    uint32_t width = 0, height = 0, spp = 0, maxDepth = 0, seed = 0;
+    DeviceFilm* film = nullptr;
+};
+struct RenderOptions {
+    uint32_t slotsPerPixel = 0;   // 0: automatic
+    bool timeKernels = false;
+    bool useGraph = true;         // replay recorded iteration blocks (ignored when timing kernels)
+    int traceVariant = 0;
+};
+// Renders all frames in one wavefront (one slot pool, shared kernel launches).
+int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOptions& opt, RenderStats* st,
+               std::string* err);
 int numTraceVariants();
 void traceVariant(int v, int* ldsDepth, int* refill, int* fused);
 

@@ -117,7 +117,9 @@ struct DeviceScene {
     DevBuf<unsigned long long> counters;
     DevScene view;
     bool hasCamera = false;
+    CameraParams camera;
     size_t bytes = 0;
+    std::shared_ptr<struct SlotPool> pool;   // wavefront state kept between renders
 };
 
 struct DeviceFilm {
@@ -313,6 +315,7 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     v.numPrims = (uint32_t)nt;
     v.numNodes = (uint32_t)nn;
     ds->hasCamera = sd.hasCamera;
+    ds->camera = sd.camera;
     if (sd.hasCamera) cameraDerived(sd.camera, 1, 1, &v.cam);
     *out = ds.release();
     return 0;
@@ -529,14 +532,28 @@ enum SlotField : uint32_t {
     SF_INFO = 20,   // pixel, flags | hero << 8 | pathLength << 16, samples started, samples total
     SF_MISC = 21,   // wl offset, initY, last BSDF pdf, camera weight
     SF_RNG = 22,    // XORShift state
+    SF_JOB = 23,    // job index, film plane of the slot
     kSlotVec = 24
 };
 
+// One render job (a frame or a served request) in the slot pool: its camera
+// (resolution included), its film planes and its path-length cap. Jobs own
+// contiguous slot ranges; slot base + plane * pixels + pixel.
+struct DevJob {
+    DevCamera cam;
+    double* film;        // [planes][pixels][16]
+    uint32_t pixels, planes, maxDepth, slotBase;
+};
+constexpr uint32_t kMaxJobs = 256;
+constexpr uint32_t kNoJob = 0xFFFFFFFFu;
+constexpr uint32_t kCompactSlots = 2048;   // slots per compaction block; jobs start at multiples of it
+
 struct PathState {
     uint32_t n;          // slots
-    uint32_t pixels;     // W*H
     float4* rec;         // n slot records
-    double* film;        // [planes][pixels][16]
+    const DevJob* jobs;  // kMaxJobs entries
+    uint32_t* jobActive; // per job: slots not idle, counted by the post-shade compaction
+    const uint32_t* blockJob;   // job owning compaction block b (slots 2048 b ..), kNoJob if none
     uint8_t* code;       // per-slot work code (Code)
     uint8_t* vis;        // 1: the slot's shadow ray was traced and is unoccluded
     uint32_t* q;         // queues: done (n entries), trace (2n), cls0..3 (n each)
@@ -544,7 +561,6 @@ struct PathState {
     unsigned long long* stat;
     uint32_t* spill;
     uint32_t spillStride;
-    uint32_t maxDepth;
     __device__ float4& f4(uint32_t s, uint32_t f) const { return rec[(size_t)s * kSlotVec + f]; }
     __device__ uint4& u4(uint32_t s, uint32_t f) const { return *(uint4*)&rec[(size_t)s * kSlotVec + f]; }
 };
@@ -637,22 +653,39 @@ __device__ __forceinline__ void resolveShadow(const PathState& ps, uint32_t s, f
 
 // ------------------------------------------------------------------ kernels
 
-__global__ void k_init_slots(PathState ps, uint32_t spp, uint32_t slotsPerPixel, uint32_t seed) {
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < ps.n; s += gridDim.x * blockDim.x) {
-        uint32_t pixel = s % ps.pixels;
-        uint32_t plane = s / ps.pixels;
-        uint32_t total = spp > plane ? (spp - plane + slotsPerPixel - 1) / slotsPerPixel : 0;
+// Puts job `job` (slots base .. base + planes * pixels - 1) into the pool:
+// slot (plane, pixel) takes samples plane, plane + planes, ... of the pixel,
+// and its XORShift stream is seeded from the job's seed and the slot's index
+// within the job, so a job renders the same alone or beside other jobs.
+__global__ void k_init_slots(PathState ps, uint32_t job, uint32_t base, uint32_t pixels, uint32_t planes, uint32_t spp,
+                             uint32_t seed) {
+    const uint32_t count = pixels * planes;
+    for (uint32_t local = blockIdx.x * blockDim.x + threadIdx.x; local < count; local += gridDim.x * blockDim.x) {
+        const uint32_t s = base + local;
+        const uint32_t pixel = local % pixels;
+        const uint32_t plane = local / pixels;
+        const uint32_t total = spp > plane ? (spp - plane + planes - 1) / planes : 0;
         ps.u4(s, SF_INFO) = make_uint4(pixel, 0u, 0u, total);
+        ps.u4(s, SF_JOB) = make_uint4(job, plane, 0u, 0u);
         // one XORShift stream per slot, seeded like XORShiftRNG(seed')
-        uint32_t h = seed ^ (0x9E3779B9u * (s + 1u));
+        uint32_t h = seed ^ (0x9E3779B9u * (local + 1u));
         h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
         ps.u4(s, SF_RNG) = xsSeed(h);
         ps.code[s] = CODE_DONE;
         ps.vis[s] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        for (int i = 0; i < kNumCtr; ++i) ps.ctr[i] = 0;
-        for (int i = 0; i < kNumStat; ++i) ps.stat[i] = 0;
+}
+
+// Clears the pool (all slots idle) and its counters.
+__global__ void k_reset_pool(PathState ps) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < ps.n; s += gridDim.x * blockDim.x) {
+        ps.code[s] = CODE_IDLE;
+        ps.vis[s] = 0;
+    }
+    if (blockIdx.x == 0) {
+        for (uint32_t i = threadIdx.x; i < kNumCtr; i += blockDim.x) ps.ctr[i] = 0;
+        for (uint32_t i = threadIdx.x; i < kNumStat; i += blockDim.x) ps.stat[i] = 0;
+        for (uint32_t i = threadIdx.x; i < kMaxJobs; i += blockDim.x) ps.jobActive[i] = 0;
     }
 }
 
@@ -662,6 +695,8 @@ __global__ void k_init_slots(PathState ps, uint32_t spp, uint32_t slotsPerPixel,
 // no radiance leaves the sums as they are and is not written.
 __device__ __forceinline__ void fileSample(const PathState& ps, uint32_t s, uint32_t pixel, const float4& misc,
                                            float* c) {
+    const uint4 jp = ps.u4(s, SF_JOB);
+    const DevJob& job = ps.jobs[jp.x];
     const float recBinWidth = kNumL / (kLambdaHighD - kLambdaLowD);
     bool finite = true, any = false;
 #pragma unroll
@@ -684,8 +719,7 @@ __device__ __forceinline__ void fileSample(const PathState& ps, uint32_t s, uint
         uint32_t b = min((uint32_t)((l - kLambdaLowD) / (kLambdaHighD - kLambdaLowD) * kNumL), (uint32_t)(kNumL - 1));
         add[b] += c[k] * recBinWidth;
     }
-    const uint32_t plane = s / ps.pixels;
-    double* f = ps.film + ((size_t)plane * ps.pixels + pixel) * kNumL;
+    double* f = job.film + ((size_t)jp.y * job.pixels + pixel) * kNumL;
 #pragma unroll
     for (int k = 0; k < kNumL; k += 2) {
         double2 v = *(double2*)(f + k);
@@ -703,7 +737,7 @@ __device__ __forceinline__ void fileSample(const PathState& ps, uint32_t s, uint
 // stored; the first shading kernel starts from them.
 __device__ __forceinline__ void startCameraSample(const DevScene& sc, const PathState& ps, uint32_t s, uint4& info,
                                                   uint4& rng, float4& misc) {
-    const DevCamera& cam = sc.cam;
+    const DevCamera& cam = ps.jobs[ps.u4(s, SF_JOB).x].cam;
     const uint32_t px = info.x % cam.width, py = info.x / cam.width;
     const float fx = px + xsFloat(rng);
     const float fy = py + xsFloat(rng);
@@ -959,9 +993,12 @@ void k_shade(DevScene sc, PathState ps) {
     // the done queue and the pending count of the previous iteration were
     // consumed before the shading kernels; the compaction that follows them
     // rebuilds both
-    if (CLS == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
-        ps.ctr[C_DONE] = 0;
-        ps.ctr[C_PENDING] = 0;
+    if (CLS == 0 && blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            ps.ctr[C_DONE] = 0;
+            ps.ctr[C_PENDING] = 0;
+        }
+        for (uint32_t j = threadIdx.x; j < kMaxJobs; j += blockDim.x) ps.jobActive[j] = 0;
     }
     unsigned long long samples = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
@@ -1029,7 +1066,7 @@ void k_shade(DevScene sc, PathState ps) {
                 }
                 if (alive) {
                     ++pathLen;
-                    if (pathLen >= ps.maxDepth || mat.type == MAT_NONE) alive = false;
+                    if (pathLen >= ps.jobs[ps.u4(s, SF_JOB).x].maxDepth || mat.type == MAT_NONE) alive = false;
                 }
                 if (alive) {
                     const float3 gL = toLocal(sp, sp.gN);
@@ -1144,14 +1181,15 @@ void k_shade(DevScene sc, PathState ps) {
 // and queue). Each block's output is a sorted run of slot ids, so the
 // kernels that read the queues touch path state in near-sequential order.
 //   MODE 0 (after shading): done queue (DONE without a pending shadow ray),
-//          count of slots with an extension or shadow ray pending.
+//          count of slots with an extension or shadow ray pending, and per
+//          job the count of slots that are not idle.
 //   MODE 1 (after regeneration): trace queue, the slot's extension entry
 //          and then its shadow entry; clears the shadow bit.
 //   MODE 2 (after tracing): one queue per shading class.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_compact(PathState ps) {
     constexpr int PER = 8;
-    constexpr int NQ = MODE == 0 ? 2 : (MODE == 1 ? 1 : kNumClasses);
+    constexpr int NQ = MODE == 0 ? 3 : (MODE == 1 ? 1 : kNumClasses);
     __shared__ uint32_t waveTot[NQ][4];
     __shared__ uint32_t blockBase[NQ];
     const uint32_t first = (blockIdx.x * 256u + threadIdx.x) * PER;
@@ -1165,7 +1203,7 @@ __global__ __launch_bounds__(256) void k_compact(PathState ps) {
     // entries slot s contributes to queue q
     auto entries = [](int q, uint8_t v) -> uint32_t {
         const bool ext = (v & 0x7F) == CODE_EXT, sh = (v & CODE_SHADOW) != 0;
-        if (MODE == 0) return q == 0 ? (v == CODE_DONE ? 1u : 0u) : (ext || sh ? 1u : 0u);
+        if (MODE == 0) return q == 0 ? (v == CODE_DONE ? 1u : 0u) : q == 1 ? (ext || sh ? 1u : 0u) : (v != CODE_IDLE ? 1u : 0u);
         if (MODE == 1) return (ext ? 1u : 0u) + (sh ? 1u : 0u);
         return v == CODE_CLS0 + q ? 1u : 0u;
     };
@@ -1199,13 +1237,19 @@ __global__ __launch_bounds__(256) void k_compact(PathState ps) {
     if (threadIdx.x < NQ) {
         const int q = threadIdx.x;
         const uint32_t tot = waveTot[q][0] + waveTot[q][1] + waveTot[q][2] + waveTot[q][3];
-        const uint32_t ctr = MODE == 0 ? (q == 0 ? C_DONE : C_PENDING) : MODE == 1 ? C_TRACE : C_CLS0 + q;
-        blockBase[q] = tot ? atomicAdd(&ps.ctr[ctr], tot) : 0u;
+        if (MODE == 0 && q == 2) {
+            const uint32_t job = ps.blockJob[blockIdx.x];
+            if (tot && job != kNoJob) atomicAdd(&ps.jobActive[job], tot);
+            blockBase[q] = 0;
+        } else {
+            const uint32_t ctr = MODE == 0 ? (q == 0 ? C_DONE : C_PENDING) : MODE == 1 ? C_TRACE : C_CLS0 + q;
+            blockBase[q] = tot ? atomicAdd(&ps.ctr[ctr], tot) : 0u;
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        if (MODE == 0 && q == 1) continue;   // counted, not queued
+        if (MODE == 0 && q >= 1) continue;   // counted, not queued
         uint32_t off = blockBase[q] + incl[q] - cnt[q];
         for (uint32_t w = 0; w < wave; ++w) off += waveTot[q][w];
         const uint32_t qid = MODE == 0 ? Q_DONE : MODE == 1 ? Q_TRACE : Q_CLS0 + q;
@@ -1239,31 +1283,135 @@ __global__ void k_film_reduce(const double* planes, uint32_t numPlanes, size_t p
 
 // ------------------------------------------------------------------ render loop
 
-struct RenderState {
-    uint32_t n = 0, pixels = 0, planes = 0, blocks = 0;
+// ------------------------------------------------------------------ slot pool
+
+// The wavefront's path state: a pool of slots that jobs are admitted into,
+// the queues, counters and job table, and the stream (and graph) the
+// per-iteration kernels run on.
+struct SlotPool {
+    uint32_t n = 0, blocks = 0;
     DevBuf<float4> rec;
-    DevBuf<double> planeFilm;
-    DevBuf<uint32_t> q, ctr, spill;
+    DevBuf<uint32_t> q, ctr, spill, jobActive, blockJob;
     DevBuf<uint8_t> code, vis;
     DevBuf<unsigned long long> stat;
-    uint32_t* hostCtr = nullptr;
-    ~RenderState() { if (hostCtr) (void)hipHostFree(hostCtr); }
+    DevBuf<DevJob> jobs;
+    std::vector<uint32_t> hostBlockJob;
+    uint32_t* hostCtr = nullptr;   // pinned: kNumCtr counters, then kMaxJobs active counts
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graphVariant = -1;
+    PathState ps;
+    DevScene sc;
+    ~SlotPool() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+        if (hostCtr) (void)hipHostFree(hostCtr);
+    }
 };
 
-static int allocState(RenderState* rs, uint32_t n, uint32_t pixels, uint32_t planes, uint32_t blocks,
-                      std::string* err) {
+// Iterations recorded per graph launch; the host looks at the counters
+// between launches.
+constexpr uint32_t kGraphIters = 8;
+
+static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<SlotPool>* out, std::string* err) {
+    if (capacity % kCompactSlots) capacity += kCompactSlots - capacity % kCompactSlots;
+    if (capacity == 0 || capacity > (1u << 26)) { if (err) *err = "slot pool size out of range"; return SLRH_ERR_CAPACITY_CODE; }
+    std::unique_ptr<SlotPool> p(new SlotPool());
+    const uint32_t blocks = (uint32_t)g_numCUs * 8;
     int r;
-    if ((r = rs->rec.alloc((size_t)kSlotVec * n, err)) || (r = rs->q.alloc((size_t)kNumQueues * n, err)) || (r = rs->ctr.alloc(kNumCtr, err)) ||
-        (r = rs->code.alloc(n, err)) || (r = rs->vis.alloc(n, err)) ||
-        (r = rs->stat.alloc(kNumStat, err)) ||
-        (r = rs->spill.alloc((size_t)(kMaxStack - kMinLdsDepth) * blocks * kBlock, err)))
+    if ((r = p->rec.alloc((size_t)kSlotVec * capacity, err)) || (r = p->q.alloc((size_t)kNumQueues * capacity, err)) ||
+        (r = p->ctr.alloc(kNumCtr, err)) || (r = p->code.alloc(capacity, err)) || (r = p->vis.alloc(capacity, err)) ||
+        (r = p->stat.alloc(kNumStat, err)) || (r = p->jobActive.alloc(kMaxJobs, err)) ||
+        (r = p->blockJob.alloc(capacity / kCompactSlots, err)) || (r = p->jobs.alloc(kMaxJobs, err)) ||
+        (r = p->spill.alloc((size_t)(kMaxStack - kMinLdsDepth) * blocks * kBlock, err)))
         return r;
-    if (planes > 1 && (r = rs->planeFilm.alloc((size_t)planes * pixels * kNumL, err))) return r;
-    if (!rs->hostCtr && hipHostMalloc((void**)&rs->hostCtr, 64 * sizeof(uint32_t)) != hipSuccess) {
+    if (hipHostMalloc((void**)&p->hostCtr, (kNumCtr + kMaxJobs) * sizeof(uint32_t)) != hipSuccess) {
         if (err) *err = "hipHostMalloc";
         return SLRH_ERR_NOMEM_CODE;
     }
-    rs->n = n; rs->pixels = pixels; rs->planes = planes; rs->blocks = blocks;
+    HIP_TRY(hipStreamCreate(&p->stream));
+    p->n = capacity;
+    p->blocks = blocks;
+    p->hostBlockJob.assign(capacity / kCompactSlots, kNoJob);
+    PathState& ps = p->ps;
+    ps.n = capacity;
+    ps.rec = p->rec.p;
+    ps.jobs = p->jobs.p;
+    ps.jobActive = p->jobActive.p;
+    ps.blockJob = p->blockJob.p;
+    ps.code = p->code.p;
+    ps.vis = p->vis.p;
+    ps.q = p->q.p;
+    ps.ctr = p->ctr.p;
+    ps.stat = p->stat.p;
+    ps.spill = p->spill.p;
+    ps.spillStride = blocks * kBlock;
+    p->sc = ds->view;
+    *out = std::move(p);
+    return 0;
+}
+
+// All slots idle, counters and statistics zero, no job owns a block.
+static int poolReset(SlotPool& p, std::string* err) {
+    hipLaunchKernelGGL(k_reset_pool, dim3(p.blocks), dim3(kBlock), 0, p.stream, p.ps);
+    HIP_TRY(hipGetLastError());
+    std::fill(p.hostBlockJob.begin(), p.hostBlockJob.end(), kNoJob);
+    HIP_TRY(hipMemcpyAsync(p.blockJob.p, p.hostBlockJob.data(), p.hostBlockJob.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, p.stream));
+    HIP_TRY(hipStreamSynchronize(p.stream));
+    return 0;
+}
+
+static uint32_t alignSlots(uint64_t n) { return (uint32_t)((n + kCompactSlots - 1) / kCompactSlots * kCompactSlots); }
+
+// A job in flight: its description, slot range and film planes.
+struct JobRun {
+    DevCamera cam;
+    DeviceFilm* out = nullptr;
+    uint32_t spp = 0, maxDepth = 100, seed = 0, planes = 1, base = 0, count = 0;
+    DevBuf<double> planeFilm;   // planes > 1: per-plane sums, reduced into out at the end
+};
+
+static int admitJob(SlotPool& p, uint32_t idx, JobRun& j, std::string* err) {
+    const uint32_t P = j.cam.width * j.cam.height;
+    DevJob dj;
+    dj.cam = j.cam;
+    dj.pixels = P;
+    dj.planes = j.planes;
+    dj.maxDepth = j.maxDepth;
+    dj.slotBase = j.base;
+    if (j.planes > 1) {
+        int r = j.planeFilm.alloc((size_t)j.planes * P * kNumL, err);
+        if (r) return r;
+        HIP_TRY(hipMemsetAsync(j.planeFilm.p, 0, (size_t)j.planes * P * kNumL * sizeof(double), p.stream));
+        dj.film = j.planeFilm.p;
+    } else {
+        dj.film = j.out->data.p;
+    }
+    HIP_TRY(hipStreamSynchronize(p.stream));
+    HIP_TRY(hipMemcpy(p.jobs.p + idx, &dj, sizeof(dj), hipMemcpyHostToDevice));
+    const uint32_t b0 = j.base / kCompactSlots, b1 = alignSlots((uint64_t)j.base + j.count) / kCompactSlots;
+    for (uint32_t b = b0; b < b1; ++b) p.hostBlockJob[b] = idx;
+    HIP_TRY(hipMemcpy(p.blockJob.p + b0, p.hostBlockJob.data() + b0, (b1 - b0) * sizeof(uint32_t),
+                      hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_init_slots, dim3(p.blocks), dim3(kBlock), 0, p.stream, p.ps, idx, j.base, P, j.planes, j.spp,
+                       j.seed);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// The job's planes are summed into its output film and its blocks freed.
+static int finishJob(SlotPool& p, JobRun& j, std::string* err) {
+    if (j.planes > 1) {
+        const uint32_t P = j.cam.width * j.cam.height;
+        hipLaunchKernelGGL(k_film_reduce, dim3(p.blocks), dim3(kBlock), 0, p.stream, j.planeFilm.p, j.planes,
+                           (size_t)P * kNumL, j.out->data.p);
+        HIP_TRY(hipGetLastError());
+    }
+    const uint32_t b0 = j.base / kCompactSlots, b1 = alignSlots((uint64_t)j.base + j.count) / kCompactSlots;
+    for (uint32_t b = b0; b < b1; ++b) p.hostBlockJob[b] = kNoJob;
+    HIP_TRY(hipMemcpyAsync(p.blockJob.p + b0, p.hostBlockJob.data() + b0, (b1 - b0) * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, p.stream));
     return 0;
 }
 
@@ -1285,99 +1433,139 @@ void traceVariant(int v, int* ldsDepth, int* refill, int* fused) {
     *fused = kVariants[v].fuse ? 1 : 0;
 }
 
-int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err) {
-    if (!ds->hasCamera) { if (err) *err = "scene has no camera"; return -2; }
-    if (film->w != cfg.width || film->h != cfg.height) { if (err) *err = "film size differs from settings"; return -1; }
-    const uint32_t W = cfg.width, H = cfg.height, P = W * H;
-    if (P == 0 || cfg.spp == 0) return 0;
-    uint32_t K = cfg.slotsPerPixel;
-    if (K == 0) {
-        // about 16M live paths: the traversal's per-launch tail of long rays
-        // is amortised over enough rays (measured on the 1080p/64 spp
-        // workload: 2M paths 258 ms per frame, 8M 224 ms, 16M 208 ms); path
-        // state is about 350 B per slot, 6 GB at 16M
-        const uint32_t target = 16u << 20;
-        K = std::max(1u, std::min(cfg.spp, (target + P - 1) / P));
-        K = std::min(K, 16u);
+// Events around the kernel groups of one iteration (kernel timing only).
+typedef std::array<hipEvent_t, 5> IterEvents;
+
+// One wavefront iteration: retire/restart, trace, classify, shade, compact.
+static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
+    const uint32_t cblocks = p.n / kCompactSlots;
+    hipStream_t s = p.stream;
+    if (ev) (void)hipEventRecord((*ev)[0], s);
+    hipLaunchKernelGGL(k_regen, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
+    hipLaunchKernelGGL(k_compact<1>, dim3(cblocks), dim3(256), 0, s, p.ps);
+    if (ev) (void)hipEventRecord((*ev)[1], s);
+    kTraceLaunch[tv](p.blocks, s, p.sc, p.ps);
+    if (ev) (void)hipEventRecord((*ev)[2], s);
+    hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, s, p.ps);
+    hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
+    hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
+    hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
+    hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
+    if (ev) (void)hipEventRecord((*ev)[3], s);
+    hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, s, p.ps);
+    if (ev) (void)hipEventRecord((*ev)[4], s);
+}
+
+// kGraphIters iterations recorded once per pool and variant, then replayed.
+// The kernels take their sizes from device counters, so one recording
+// serves every replay.
+static int launchGraphBlock(SlotPool& p, int tv, std::string* err) {
+    if (!p.graph || p.graphVariant != tv) {
+        if (p.graph) { (void)hipGraphExecDestroy(p.graph); p.graph = nullptr; }
+        hipGraph_t g;
+        HIP_TRY(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
+        for (uint32_t i = 0; i < kGraphIters; ++i) launchIteration(p, tv, nullptr);
+        HIP_TRY(hipStreamEndCapture(p.stream, &g));
+        hipError_t e = hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(e);
+        p.graphVariant = tv;
     }
-    K = std::min(K, cfg.spp);
-    const uint64_t n64 = (uint64_t)K * P;
-    if (n64 > (1ull << 26)) { if (err) *err = "too many path slots"; return SLRH_ERR_CAPACITY_CODE; }
-    const uint32_t N = (uint32_t)n64;
-    const uint32_t blocks = (uint32_t)g_numCUs * 8;
+    HIP_TRY(hipGraphLaunch(p.graph, p.stream));
+    return 0;
+}
 
-    std::unique_ptr<RenderState> rs(new RenderState());
-    int r = allocState(rs.get(), N, P, K, blocks, err);
+// Copies the counters and per-job active counts to the host (synchronous).
+static int readCounters(SlotPool& p, std::string* err) {
+    HIP_TRY(hipMemcpyAsync(p.hostCtr, p.ctr.p, kNumCtr * sizeof(uint32_t), hipMemcpyDeviceToHost, p.stream));
+    HIP_TRY(hipMemcpyAsync(p.hostCtr + kNumCtr, p.jobActive.p, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           p.stream));
+    HIP_TRY(hipStreamSynchronize(p.stream));
+    return 0;
+}
+
+// Live paths the wavefront aims for: the traversal's per-launch tail of long
+// rays is amortised over enough rays (measured on the 1080p/64 spp
+// workload: 2M paths 258 ms per frame, 8M 224 ms, 16M 208 ms); path state is
+// 384 B per slot, 6.4 GB at 16M.
+constexpr uint32_t kTargetSlots = 16u << 20;
+
+int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOptions& opt, RenderStats* st,
+               std::string* err) {
+    if (n == 0) return 0;
+    if (n > kMaxJobs) { if (err) *err = "too many frames in one batch"; return SLRH_ERR_CAPACITY_CODE; }
+    std::vector<JobRun> jobs(n);
+    uint64_t totalPixels = 0;
+    for (uint32_t i = 0; i < n; ++i) totalPixels += (uint64_t)specs[i].width * specs[i].height;
+    if (totalPixels == 0) return 0;
+    uint32_t K = opt.slotsPerPixel;
+    if (K == 0) K = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(16, (kTargetSlots + totalPixels - 1) / totalPixels));
+    uint64_t capacity = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const JobSpec& js = specs[i];
+        JobRun& j = jobs[i];
+        cameraDerived(js.camera, js.width, js.height, &j.cam);
+        j.out = js.film;
+        j.spp = js.spp;
+        j.maxDepth = js.maxDepth == 0 ? 100u : js.maxDepth;
+        j.seed = js.seed;
+        j.planes = std::max(1u, std::min(K, js.spp));
+        j.base = (uint32_t)std::min<uint64_t>(capacity, UINT32_MAX);
+        j.count = j.planes * js.width * js.height;
+        capacity += alignSlots(j.count);
+    }
+    if (capacity > (1u << 26)) { if (err) *err = "too many path slots"; return SLRH_ERR_CAPACITY_CODE; }
+    // the pool is kept between renders; it is rebuilt when too small, or when
+    // much larger than needed (every iteration scans all of its slots)
+    if (!ds->pool || ds->pool->n < capacity || ds->pool->n > 2 * capacity + 64 * kCompactSlots) {
+        ds->pool.reset();
+        std::unique_ptr<SlotPool> np;
+        int r = poolCreate(ds, (uint32_t)capacity, &np, err);
+        if (r) return r;
+        ds->pool = std::shared_ptr<SlotPool>(np.release());
+    }
+    SlotPool& p = *ds->pool;
+    const int tv = opt.traceVariant >= 0 && opt.traceVariant < kNumVariants ? opt.traceVariant : 0;
+    int r = poolReset(p, err);
     if (r) return r;
-
-    DevScene sc = ds->view;
-    sc.cam.width = W;
-    sc.cam.height = H;
-    PathState ps;
-    ps.n = N; ps.pixels = P;
-    ps.rec = rs->rec.p;
-    ps.film = K > 1 ? rs->planeFilm.p : film->data.p;
-    ps.q = rs->q.p; ps.ctr = rs->ctr.p; ps.stat = rs->stat.p; ps.code = rs->code.p;
-    ps.vis = rs->vis.p;
-    ps.spill = rs->spill.p; ps.spillStride = blocks * kBlock;
-    ps.maxDepth = cfg.maxDepth == 0 ? 100u : cfg.maxDepth;
-
-    hipStream_t stream = 0;
-    if (K > 1) HIP_TRY(hipMemsetAsync(rs->planeFilm.p, 0, (size_t)K * P * kNumL * sizeof(double), stream));
-    hipLaunchKernelGGL(k_init_slots, dim3(blocks), dim3(kBlock), 0, stream, ps, cfg.spp, K, cfg.seed);
-    HIP_TRY(hipGetLastError());
 
     std::vector<hipEvent_t> evs;
     auto newEvent = [&]() { hipEvent_t e; (void)hipEventCreate(&e); evs.push_back(e); return e; };
-    // with timeKernels: events before regen, trace, shading and the final compaction of each iteration
-    std::vector<std::array<hipEvent_t, 5>> itEv;
+    std::vector<IterEvents> itEv;
     hipEvent_t tStart = newEvent(), tEnd = newEvent();
-    HIP_TRY(hipEventRecord(tStart, stream));
-
-    const uint32_t cblocks = (N + 2047) / 2048;   // k_compact: 256 threads x 8 slots
-    const int tv = cfg.traceVariant >= 0 && cfg.traceVariant < kNumVariants ? cfg.traceVariant : 0;
-    auto mark = [&](int k) {
-        if (!cfg.timeKernels) return hipSuccess;
-        if (k == 0) itEv.push_back({});
-        itEv.back()[k] = newEvent();
-        return hipEventRecord(itEv.back()[k], stream);
-    };
-    hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, stream, ps);
+    for (uint32_t i = 0; i < n; ++i)
+        if ((r = admitJob(p, i, jobs[i], err))) return r;
+    HIP_TRY(hipEventRecord(tStart, p.stream));
+    hipLaunchKernelGGL(k_compact<0>, dim3(p.n / kCompactSlots), dim3(256), 0, p.stream, p.ps);
+    HIP_TRY(hipGetLastError());
     uint32_t it = 0;
-    const uint32_t checkEvery = 8;
-    for (;; ++it) {
-        HIP_TRY(mark(0));
-        hipLaunchKernelGGL(k_regen, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
-        hipLaunchKernelGGL(k_compact<1>, dim3(cblocks), dim3(256), 0, stream, ps);
-        HIP_TRY(mark(1));
-        kTraceLaunch[tv](blocks, stream, sc, ps);
-        HIP_TRY(mark(2));
-        hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, stream, ps);
-        hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
-        hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
-        hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
-        hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(blocks), dim3(kBlock), 0, stream, sc, ps);
-        HIP_TRY(mark(3));
-        hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, stream, ps);
-        HIP_TRY(mark(4));
-        HIP_TRY(hipGetLastError());
-        if ((it + 1) % checkEvery == 0) {
-            HIP_TRY(hipMemcpyAsync(rs->hostCtr, rs->ctr.p, kNumCtr * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-            HIP_TRY(hipStreamSynchronize(stream));
-            if (rs->hostCtr[C_DONE] == 0 && rs->hostCtr[C_PENDING] == 0) break;
-            if (rs->hostCtr[C_OVERFLOW]) { if (err) *err = "traversal stack overflow (depth > 64)"; break; }
+    for (;;) {
+        if (opt.timeKernels || !opt.useGraph) {
+            for (uint32_t k = 0; k < kGraphIters; ++k) {
+                IterEvents* e = nullptr;
+                if (opt.timeKernels) {
+                    itEv.push_back({});
+                    for (auto& x : itEv.back()) x = newEvent();
+                    e = &itEv.back();
+                }
+                launchIteration(p, tv, e);
+            }
+            HIP_TRY(hipGetLastError());
+        } else {
+            if ((r = launchGraphBlock(p, tv, err))) return r;
         }
+        it += kGraphIters;
+        if ((r = readCounters(p, err))) return r;
+        if (p.hostCtr[C_OVERFLOW]) { if (err) *err = "traversal stack overflow (depth > 64)"; break; }
+        if (p.hostCtr[C_DONE] == 0 && p.hostCtr[C_PENDING] == 0) break;
     }
-    if (K > 1) {
-        hipLaunchKernelGGL(k_film_reduce, dim3(blocks), dim3(kBlock), 0, stream, rs->planeFilm.p, K,
-                           (size_t)P * kNumL, film->data.p);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipEventRecord(tEnd, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
+    for (uint32_t i = 0; i < n; ++i)
+        if ((r = finishJob(p, jobs[i], err))) return r;
+    HIP_TRY(hipEventRecord(tEnd, p.stream));
+    HIP_TRY(hipStreamSynchronize(p.stream));
     unsigned long long stat[kNumStat];
-    HIP_TRY(hipMemcpy(stat, rs->stat.p, sizeof(stat), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(rs->hostCtr, rs->ctr.p, kNumCtr * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(stat, p.stat.p, sizeof(stat), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(p.hostCtr, p.ctr.p, kNumCtr * sizeof(uint32_t), hipMemcpyDeviceToHost));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, tStart, tEnd);
     if (st) {
@@ -1390,9 +1578,9 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
         st->nodesShadow = (double)stat[S_NODES_SH];
         st->trisShadow = (double)stat[S_TRIS_SH];
         st->kernelSeconds = ms * 1e-3;
-        st->nonFinite = rs->hostCtr[C_NONFINITE];
-        st->overflow = rs->hostCtr[C_OVERFLOW];
-        st->iterations = it + 1;
+        st->nonFinite = p.hostCtr[C_NONFINITE];
+        st->overflow = p.hostCtr[C_OVERFLOW];
+        st->iterations = it;
         double tr = 0, sh = 0, ot = 0;
         auto el = [](hipEvent_t a, hipEvent_t b) { float t = 0; (void)hipEventElapsedTime(&t, a, b); return (double)t; };
         for (auto& e : itEv) {
@@ -1406,8 +1594,28 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
         st->traceLaunches = (double)itEv.size();
     }
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
-    if (rs->hostCtr[C_OVERFLOW]) return SLRH_ERR_CAPACITY_CODE;
+    if (p.hostCtr[C_OVERFLOW]) return SLRH_ERR_CAPACITY_CODE;
     return 0;
+}
+
+int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err) {
+    if (!ds->hasCamera) { if (err) *err = "scene has no camera"; return -2; }
+    if (film->w != cfg.width || film->h != cfg.height) { if (err) *err = "film size differs from settings"; return -1; }
+    if (cfg.width * cfg.height == 0 || cfg.spp == 0) return 0;
+    JobSpec js;
+    js.camera = ds->camera;
+    js.width = cfg.width;
+    js.height = cfg.height;
+    js.spp = cfg.spp;
+    js.maxDepth = cfg.maxDepth;
+    js.seed = cfg.seed;
+    js.film = film;
+    RenderOptions opt;
+    opt.slotsPerPixel = cfg.slotsPerPixel;
+    opt.timeKernels = cfg.timeKernels;
+    opt.traceVariant = cfg.traceVariant;
+    opt.useGraph = cfg.useGraph;
+    return renderJobs(ds, &js, 1, opt, st, err);
 }
 
 }  // namespace slrh

@@ -1,0 +1,96 @@
+"""Multi-frame batches: several frames rendered in one wavefront (one slot
+pool, shared kernel launches) give each frame the film it gets alone.
+
+Slots are seeded from the frame's seed and the slot's index within the
+frame, and a slot's samples are filed in the same order either way, so the
+FP64 films are bit-identical, not merely close. Also checks that replaying
+recorded iteration graphs and launching each kernel give the same films.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+import slr_amd
+from slr_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _moved(cam, dx):
+    c = copy.deepcopy(cam)
+    m = np.array(c["l2w"], dtype=np.float64).reshape(4, 4).copy()
+    m[0, 3] += dx
+    c["l2w"] = m
+    return c
+
+
+def _frames(ctx, desc):
+    cams = [desc["camera"], _moved(desc["camera"], 0.3)]
+    sizes = [(40, 24), (24, 32)]
+    films = [slr_amd.Film(ctx, w, h) for w, h in sizes]
+    frames = [dict(film=f, camera=c, seed=s, spp=p) for f, c, s, p in zip(films, cams, (7, 1234567), (8, 12))]
+    return frames
+
+
+def test_batch_matches_frames_alone(gpu_ctx):
+    desc = scenes.cornell_box(40, 24)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    r = slr_amd.PTRenderer(8)
+    frames = _frames(gpu_ctx, desc)
+    st = r.render_batch(sc, frames, slots_per_pixel=2)
+    assert st["samples"] == sum(f["film"].width * f["film"].height * f["spp"] for f in frames)
+    together = [f["film"].strata().copy() for f in frames]
+    for f, img in zip(frames, together):
+        f["film"].clear()
+        r.render_batch(sc, [f], slots_per_pixel=2)
+        alone = f["film"].strata()
+        assert np.isfinite(alone).all() and alone.sum() > 0
+        assert np.array_equal(alone.view(np.uint64), img.view(np.uint64))
+
+
+def test_single_frame_batch_matches_render(gpu_ctx):
+    """slrh_render is the one-frame batch: same camera, seed and slots."""
+    W, H = 32, 20
+    desc = scenes.cornell_box(W, H)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    a = slr_amd.Film(gpu_ctx, W, H)
+    b = slr_amd.Film(gpu_ctx, W, H)
+    slr_amd.PTRenderer(6).render(sc, a, seed=99, slots_per_pixel=3)
+    slr_amd.PTRenderer(6).render_batch(sc, [dict(film=b, camera=desc["camera"], seed=99)], slots_per_pixel=3)
+    assert np.array_equal(a.strata().view(np.uint64), b.strata().view(np.uint64))
+
+
+def test_graph_replay_matches_direct_launches(gpu_ctx):
+    W, H = 48, 32
+    desc = scenes.cornell_box(W, H)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    a = slr_amd.Film(gpu_ctx, W, H)
+    b = slr_amd.Film(gpu_ctx, W, H)
+    r = slr_amd.PTRenderer(8)
+    # several renders in a row reuse the recorded graph of the same pool
+    for seed in (3, 4):
+        a.clear()
+        b.clear()
+        r.render(sc, a, seed=seed, slots_per_pixel=2, graph=True)
+        r.render(sc, b, seed=seed, slots_per_pixel=2, graph=False)
+        assert np.array_equal(a.strata().view(np.uint64), b.strata().view(np.uint64))
+
+
+def test_batch_rejects_film_size_mismatch(gpu_ctx):
+    desc = scenes.cornell_box(16, 16)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    film = slr_amd.Film(gpu_ctx, 16, 16)
+    fr = dict(film=film, camera=desc["camera"], seed=1)
+    bad = slr_amd._Frame
+    with pytest.raises(slr_amd.SlrhError):
+        # a film of another size than the frame asks for
+        f2 = slr_amd.Film(gpu_ctx, 8, 8)
+        frames = [fr, dict(film=f2, camera=desc["camera"], seed=2)]
+        arr = (bad * 2)()
+        for i, f in enumerate(frames):
+            arr[i].width, arr[i].height, arr[i].spp = 16, 16, 1
+            arr[i].aspect, arr[i].fov_y, arr[i].img_dist, arr[i].obj_dist = 1.0, 0.7, 1.0, 1.0
+            arr[i].film = f["film"].h
+        s = slr_amd._Settings(0, 0, 1, 0, 0, 0, 0, 0, 0, 0)
+        sc.ctx.check(sc.lib.slrh_render_batch(sc.h, arr, 2, slr_amd.ctypes.byref(s), None), "slrh_render_batch")
