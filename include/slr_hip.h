@@ -235,6 +235,47 @@ typedef struct slrh_frame {
 SLRH_API int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n,
                                const slrh_render_settings* settings, double* stats);
 
+/* ------------------------------------------------------------ serving */
+
+/* A render server over one resident scene (no counterpart in the reference,
+ * which renders one frame per PTRenderer::render call). Requests queue in
+ * order and are admitted while their path state fits the server's budget;
+ * a scheduler thread keeps one wavefront running and lets requests join and
+ * leave it between iteration blocks (continuous batching). A request's film
+ * holds what slrh_render_batch gives for that frame with the same seed and
+ * slots_per_pixel. The scene must outlive the server; films must outlive
+ * their requests (until DONE, CANCELLED or FAILED). */
+typedef struct slrh_server slrh_server;
+
+typedef struct slrh_server_config {
+    uint64_t path_state_bytes;   /* device memory for path state; bounds the live paths (~414 B each) */
+    uint32_t tune_trace;         /* traversal kernel variant + 1 (0 = default) */
+    uint32_t reserved;           /* must be 0 */
+} slrh_server_config;
+
+typedef struct slrh_request {
+    slrh_frame frame;            /* camera, resolution, spp, seed, film */
+    uint32_t max_depth;          /* 0 = 100 */
+    uint32_t slots_per_pixel;    /* 0 = chosen at admission from the load */
+} slrh_request;
+
+enum slrh_request_state {
+    SLRH_REQ_QUEUED = 0, SLRH_REQ_RUNNING = 1, SLRH_REQ_DONE = 2, SLRH_REQ_CANCELLED = 3, SLRH_REQ_FAILED = 4
+};
+
+SLRH_API int slrh_server_create(slrh_scene* s, const slrh_server_config* cfg, slrh_server** out);
+/* Stops the scheduler; requests not yet settled are abandoned. */
+SLRH_API int slrh_server_destroy(slrh_server* sv);
+SLRH_API int slrh_submit(slrh_server* sv, const slrh_request* req, uint64_t* ticket);
+SLRH_API int slrh_poll(slrh_server* sv, uint64_t ticket, int* state);
+/* Waits until the request settles or timeout_ms passes (< 0: no limit). */
+SLRH_API int slrh_wait(slrh_server* sv, uint64_t ticket, int timeout_ms, int* state);
+/* A cancelled request's film holds an unspecified part of its samples. */
+SLRH_API int slrh_cancel(slrh_server* sv, uint64_t ticket);
+/* stats[8]: completed requests, queued, running, p50 and p99 latency (ms,
+ * submit to done), mean latency (ms), samples filed, scheduler busy seconds. */
+SLRH_API int slrh_server_stats(slrh_server* sv, double* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,8 @@ EXPORTS = [
     "slrh_intersect", "slrh_occluded",
     "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read",
     "slrh_render", "slrh_render_batch", "slrh_trace_variants",
+    "slrh_server_create", "slrh_server_destroy", "slrh_submit", "slrh_poll", "slrh_wait", "slrh_cancel",
+    "slrh_server_stats",
 ]
 
 _lib = None
@@ -59,6 +61,14 @@ class _Frame(ctypes.Structure):
     _fields_ = [("l2w", ctypes.c_float * 16)] + [(n, ctypes.c_float) for n in
                 ("sensitivity", "aspect", "fov_y", "lens_radius", "img_dist", "obj_dist")] + \
                [(n, ctypes.c_uint32) for n in ("width", "height", "spp", "seed")] + [("film", ctypes.c_void_p)]
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("path_state_bytes", ctypes.c_uint64), ("tune_trace", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [("frame", _Frame), ("max_depth", ctypes.c_uint32), ("slots_per_pixel", ctypes.c_uint32)]
 
 
 FLAG_TIME_KERNELS = 1
@@ -113,6 +123,13 @@ def load_library(path=None):
         "slrh_film_read": (i32, [vp, i32, f64, vp, ctypes.c_size_t]),
         "slrh_render": (i32, [vp, vp, vp, vp]),
         "slrh_render_batch": (i32, [vp, vp, u32, vp, vp]),
+        "slrh_server_create": (i32, [vp, vp, pp]),
+        "slrh_server_destroy": (i32, [vp]),
+        "slrh_submit": (i32, [vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "slrh_poll": (i32, [vp, ctypes.c_uint64, pi]),
+        "slrh_wait": (i32, [vp, ctypes.c_uint64, i32, pi]),
+        "slrh_cancel": (i32, [vp, ctypes.c_uint64]),
+        "slrh_server_stats": (i32, [vp, vp]),
         "slrh_trace_variants": (i32, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -368,6 +385,23 @@ STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "tr
              "nodes_shadow", "tris_shadow"]
 
 
+def _fill_frame(fr, f, default_spp):
+    cam = f["camera"]
+    m = np.asarray(cam["l2w"], dtype=np.float32).reshape(4, 4).T.ravel()  # column-major
+    for k in range(16):
+        fr.l2w[k] = float(m[k])
+    fr.sensitivity = float(cam.get("sensitivity", 1.0))
+    fr.aspect = float(cam["aspect"])
+    fr.fov_y = float(cam["fov_y"])
+    fr.lens_radius = float(cam.get("lens_radius", 0.0))
+    fr.img_dist = float(cam.get("img_dist", 1.0))
+    fr.obj_dist = float(cam.get("obj_dist", 1.0))
+    fr.width, fr.height = f["film"].width, f["film"].height
+    fr.spp = int(f.get("spp", default_spp))
+    fr.seed = int(f.get("seed", 1509761209)) & 0xFFFFFFFF
+    fr.film = f["film"].h
+
+
 class PTRenderer:
     """PTRenderer(spp).render(scene, settings) (libSLR/Renderer/PTRenderer.h:16-44)."""
 
@@ -389,27 +423,71 @@ class PTRenderer:
         sensitivity), seed, spp (default self.spp)}; all frames in one wavefront."""
         arr = (_Frame * max(1, len(frames)))()
         for i, f in enumerate(frames):
-            cam = f["camera"]
-            m = np.asarray(cam["l2w"], dtype=np.float32).reshape(4, 4).T.ravel()  # column-major
-            fr = arr[i]
-            for k in range(16):
-                fr.l2w[k] = float(m[k])
-            fr.sensitivity = float(cam.get("sensitivity", 1.0))
-            fr.aspect = float(cam["aspect"])
-            fr.fov_y = float(cam["fov_y"])
-            fr.lens_radius = float(cam.get("lens_radius", 0.0))
-            fr.img_dist = float(cam.get("img_dist", 1.0))
-            fr.obj_dist = float(cam.get("obj_dist", 1.0))
-            fr.width, fr.height = f["film"].width, f["film"].height
-            fr.spp = int(f.get("spp", self.spp))
-            fr.seed = int(f.get("seed", 1509761209)) & 0xFFFFFFFF
-            fr.film = f["film"].h
+            _fill_frame(arr[i], f, self.spp)
         flags = (FLAG_TIME_KERNELS if time_kernels else 0) | (0 if graph else FLAG_NO_GRAPH)
         s = _Settings(0, 0, self.spp, max_depth, 0, 0, slots_per_pixel, flags, tune_trace, 0)
         st = np.zeros(len(STAT_KEYS))
         scene.ctx.check(scene.lib.slrh_render_batch(scene.h, arr, len(frames), ctypes.byref(s), _ptr(st)),
                         "slrh_render_batch")
         return dict(zip(STAT_KEYS, st.tolist()))
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
+SERVER_STAT_KEYS = ["completed", "queued", "running", "p50_ms", "p99_ms", "mean_ms", "samples", "busy_seconds"]
+
+
+class Server:
+    """Continuous-batching render server over a built scene (slrh_server_*)."""
+
+    def __init__(self, scene, path_state_bytes=2 << 30, tune_trace=0):
+        self.scene = scene
+        self.lib = scene.lib
+        h = ctypes.c_void_p()
+        cfg = _ServerConfig(int(path_state_bytes), int(tune_trace), 0)
+        scene.ctx.check(self.lib.slrh_server_create(scene.h, ctypes.byref(cfg), ctypes.byref(h)), "slrh_server_create")
+        self.h = h
+        self._films = {}
+
+    def submit(self, film, camera, spp, seed=1509761209, max_depth=0, slots_per_pixel=0):
+        req = _Request()
+        _fill_frame(req.frame, dict(film=film, camera=camera, spp=spp, seed=seed), spp)
+        req.max_depth = int(max_depth)
+        req.slots_per_pixel = int(slots_per_pixel)
+        t = ctypes.c_uint64()
+        self.scene.ctx.check(self.lib.slrh_submit(self.h, ctypes.byref(req), ctypes.byref(t)), "slrh_submit")
+        self._films[t.value] = film   # kept alive until the request settles
+        return t.value
+
+    def poll(self, ticket):
+        st = ctypes.c_int()
+        self.scene.ctx.check(self.lib.slrh_poll(self.h, ctypes.c_uint64(ticket), ctypes.byref(st)), "slrh_poll")
+        return st.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        st = ctypes.c_int()
+        self.scene.ctx.check(self.lib.slrh_wait(self.h, ctypes.c_uint64(ticket), int(timeout_ms), ctypes.byref(st)),
+                             "slrh_wait")
+        return st.value
+
+    def cancel(self, ticket):
+        self.scene.ctx.check(self.lib.slrh_cancel(self.h, ctypes.c_uint64(ticket)), "slrh_cancel")
+
+    def stats(self):
+        out = np.zeros(len(SERVER_STAT_KEYS))
+        self.scene.ctx.check(self.lib.slrh_server_stats(self.h, _ptr(out)), "slrh_server_stats")
+        return dict(zip(SERVER_STAT_KEYS, out.tolist()))
+
+    def close(self):
+        if self.h:
+            self.lib.slrh_server_destroy(self.h)
+            self.h = None
+            self._films.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def build_scene(ctx, desc):

@@ -15,13 +15,13 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libslr_hip.so")
 
-SOURCES = ["capi.cpp", "bvh_build.cpp", "spectral.cpp", "render.hip"]
+SOURCES = ["capi.cpp", "bvh_build.cpp", "spectral.cpp", "server.cpp", "render.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off keeps every multiply and add separately rounded, as the
 # reference's x86-64 build does; traversal parity depends on it.
-# Packed FP32 instructions are disabled (they have produced wrong results on
-# this hardware pool).
+# Packed FP32 instructions are disabled: some of their forms have given
+# silently wrong results on MI355X.
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "server.h"
 
 using namespace slrh;
 
@@ -38,6 +39,11 @@ struct slrh_film {
     slrh_context* ctx = nullptr;
     DeviceFilm* dev = nullptr;
     uint32_t w = 0, h = 0;
+};
+
+struct slrh_server {
+    slrh_context* ctx = nullptr;
+    std::unique_ptr<Server> srv;
 };
 
 namespace {
@@ -529,6 +535,27 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     return SLRH_OK;
 }
 
+namespace {
+int frameSpec(slrh_context* ctx, const slrh_frame& f, uint32_t maxDepth, JobSpec* js) {
+    if (!f.film) return fail(ctx, SLRH_ERR_INVALID, "frame without a film");
+    if (f.width != f.film->w || f.height != f.film->h) return fail(ctx, SLRH_ERR_INVALID, "film size mismatch");
+    memcpy(js->camera.l2w, f.l2w, sizeof(js->camera.l2w));
+    js->camera.sensitivity = f.sensitivity;
+    js->camera.aspect = f.aspect;
+    js->camera.fovY = f.fov_y;
+    js->camera.lensRadius = f.lens_radius;
+    js->camera.imgDist = f.img_dist;
+    js->camera.objDist = f.obj_dist;
+    js->width = f.width;
+    js->height = f.height;
+    js->spp = f.spp;
+    js->seed = f.seed;
+    js->maxDepth = maxDepth;
+    js->film = f.film->dev;
+    return SLRH_OK;
+}
+}  // namespace
+
 int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const slrh_render_settings* set,
                       double* stats) {
     if (!s || !set || (n && !frames)) return SLRH_ERR_INVALID;
@@ -539,23 +566,8 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
     std::vector<JobSpec> specs(n);
     for (uint32_t i = 0; i < n; ++i) {
-        const slrh_frame& f = frames[i];
-        if (!f.film) return fail(s->ctx, SLRH_ERR_INVALID, "frame without a film");
-        if (f.width != f.film->w || f.height != f.film->h) return fail(s->ctx, SLRH_ERR_INVALID, "film size mismatch");
-        JobSpec& js = specs[i];
-        memcpy(js.camera.l2w, f.l2w, sizeof(js.camera.l2w));
-        js.camera.sensitivity = f.sensitivity;
-        js.camera.aspect = f.aspect;
-        js.camera.fovY = f.fov_y;
-        js.camera.lensRadius = f.lens_radius;
-        js.camera.imgDist = f.img_dist;
-        js.camera.objDist = f.obj_dist;
-        js.width = f.width;
-        js.height = f.height;
-        js.spp = f.spp;
-        js.seed = f.seed;
-        js.maxDepth = set->max_depth;
-        js.film = f.film->dev;
+        int r = frameSpec(s->ctx, frames[i], set->max_depth, &specs[i]);
+        if (r) return r;
     }
     RenderOptions opt;
     opt.slotsPerPixel = set->slots_per_pixel;
@@ -569,6 +581,64 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     int r = renderJobs(s->dev, specs.data(), n, opt, &st, &err);
     if (r) return fail(s->ctx, r, err);
     copyStats(st, stats);
+    return SLRH_OK;
+}
+
+int slrh_server_create(slrh_scene* s, const slrh_server_config* cfg, slrh_server** out) {
+    if (!s || !cfg || !out) return SLRH_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "server before slrh_scene_build");
+    if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
+    if (cfg->reserved != 0) return fail(s->ctx, SLRH_ERR_INVALID, "reserved server field must be 0");
+    if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
+    const int tv = cfg->tune_trace == 0 ? 0 : (int)cfg->tune_trace - 1;
+    if (tv >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
+    std::unique_ptr<slrh_server> sv(new (std::nothrow) slrh_server());
+    if (!sv) return fail(s->ctx, SLRH_ERR_NOMEM, "out of host memory");
+    sv->ctx = s->ctx;
+    sv->srv.reset(new Server());
+    std::string err;
+    int r = sv->srv->start(s->dev, cfg->path_state_bytes, tv, &err);
+    if (r) return fail(s->ctx, r, err);
+    *out = sv.release();
+    return SLRH_OK;
+}
+
+int slrh_server_destroy(slrh_server* sv) {
+    delete sv;
+    return SLRH_OK;
+}
+
+int slrh_submit(slrh_server* sv, const slrh_request* req, uint64_t* ticket) {
+    if (!sv || !req || !ticket) return SLRH_ERR_INVALID;
+    JobSpec js;
+    int r = frameSpec(sv->ctx, req->frame, req->max_depth, &js);
+    if (r) return r;
+    std::string err;
+    if ((r = sv->srv->submit(js, req->slots_per_pixel, ticket, &err))) return fail(sv->ctx, r, err);
+    return SLRH_OK;
+}
+
+int slrh_poll(slrh_server* sv, uint64_t ticket, int* state) {
+    if (!sv || !state) return SLRH_ERR_INVALID;
+    if (sv->srv->state(ticket, state)) return fail(sv->ctx, SLRH_ERR_INVALID, "unknown ticket");
+    return SLRH_OK;
+}
+
+int slrh_wait(slrh_server* sv, uint64_t ticket, int timeout_ms, int* state) {
+    if (!sv || !state) return SLRH_ERR_INVALID;
+    if (sv->srv->wait(ticket, timeout_ms, state)) return fail(sv->ctx, SLRH_ERR_INVALID, "unknown ticket");
+    return SLRH_OK;
+}
+
+int slrh_cancel(slrh_server* sv, uint64_t ticket) {
+    if (!sv) return SLRH_ERR_INVALID;
+    if (sv->srv->cancel(ticket)) return fail(sv->ctx, SLRH_ERR_INVALID, "unknown ticket");
+    return SLRH_OK;
+}
+
+int slrh_server_stats(slrh_server* sv, double* stats) {
+    if (!sv || !stats) return SLRH_ERR_INVALID;
+    sv->srv->stats(stats);
     return SLRH_OK;
 }
 

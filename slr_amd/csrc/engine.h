@@ -87,6 +87,26 @@ struct RenderOptions {
 // Renders all frames in one wavefront (one slot pool, shared kernel launches).
 int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOptions& opt, RenderStats* st,
                std::string* err);
+
+// A slot pool that a scheduler admits jobs into while it runs (serving).
+// Jobs occupy planes * width * height slots from a base that is a multiple
+// of kPoolAlign; job indices are below kPoolMaxJobs. Between runs of
+// iteration blocks the scheduler may admit and cancel jobs and must then
+// call poolRecount; a job is finished once its active count reads 0.
+struct SlotPool;
+constexpr uint32_t kPoolAlign = 2048;
+constexpr uint32_t kPoolMaxJobs = 256;
+int poolOpen(DeviceScene* ds, uint32_t capacity, SlotPool** out, std::string* err);
+void poolClose(SlotPool* p);
+int poolBindThread(SlotPool* p);   // makes the pool's device current on the calling thread
+uint32_t poolCapacity(const SlotPool* p);
+int poolAdmit(SlotPool* p, uint32_t idx, const JobSpec& spec, uint32_t planes, uint32_t base, std::string* err);
+int poolCancel(SlotPool* p, uint32_t idx, std::string* err);
+int poolRecount(SlotPool* p, std::string* err);
+int poolRunBlock(SlotPool* p, int traceVariant, std::string* err);
+const uint32_t* poolActiveCounts(const SlotPool* p);
+int poolFinish(SlotPool* p, uint32_t idx, std::string* err);
+int poolSamples(SlotPool* p, double* samples, double* rays, std::string* err);
 int numTraceVariants();
 void traceVariant(int v, int* ldsDepth, int* refill, int* fused);
 

@@ -547,6 +547,7 @@ struct DevJob {
 constexpr uint32_t kMaxJobs = 256;
 constexpr uint32_t kNoJob = 0xFFFFFFFFu;
 constexpr uint32_t kCompactSlots = 2048;   // slots per compaction block; jobs start at multiples of it
+static_assert(kCompactSlots == kPoolAlign && kMaxJobs == kPoolMaxJobs, "pool constants");
 
 struct PathState {
     uint32_t n;          // slots
@@ -674,6 +675,24 @@ __global__ void k_init_slots(PathState ps, uint32_t job, uint32_t base, uint32_t
         ps.code[s] = CODE_DONE;
         ps.vis[s] = 0;
     }
+}
+
+// Makes slots base .. base + count - 1 idle (a cancelled job).
+__global__ void k_idle_slots(PathState ps, uint32_t base, uint32_t count) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        ps.code[base + i] = CODE_IDLE;
+        ps.vis[base + i] = 0;
+    }
+}
+
+// Zeroes what the post-shade compaction counts, so that it can be run again
+// after jobs were admitted or cancelled between iterations.
+__global__ void k_zero_counts(PathState ps) {
+    if (threadIdx.x == 0) {
+        ps.ctr[C_DONE] = 0;
+        ps.ctr[C_PENDING] = 0;
+    }
+    for (uint32_t j = threadIdx.x; j < kMaxJobs; j += blockDim.x) ps.jobActive[j] = 0;
 }
 
 // Clears the pool (all slots idle) and its counters.
@@ -1285,11 +1304,21 @@ __global__ void k_film_reduce(const double* planes, uint32_t numPlanes, size_t p
 
 // ------------------------------------------------------------------ slot pool
 
+// A job in flight: its description, slot range and film planes.
+struct JobRun {
+    DevCamera cam;
+    DeviceFilm* out = nullptr;
+    uint32_t spp = 0, maxDepth = 100, seed = 0, planes = 1, base = 0, count = 0;
+    DevBuf<double> planeFilm;   // planes > 1: per-plane sums, reduced into out at the end
+};
+
+
 // The wavefront's path state: a pool of slots that jobs are admitted into,
 // the queues, counters and job table, and the stream (and graph) the
 // per-iteration kernels run on.
 struct SlotPool {
     uint32_t n = 0, blocks = 0;
+    int device = 0;
     DevBuf<float4> rec;
     DevBuf<uint32_t> q, ctr, spill, jobActive, blockJob;
     DevBuf<uint8_t> code, vis;
@@ -1302,6 +1331,7 @@ struct SlotPool {
     int graphVariant = -1;
     PathState ps;
     DevScene sc;
+    JobRun runs[kMaxJobs];
     ~SlotPool() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1330,6 +1360,7 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
         return SLRH_ERR_NOMEM_CODE;
     }
     HIP_TRY(hipStreamCreate(&p->stream));
+    HIP_TRY(hipGetDevice(&p->device));
     p->n = capacity;
     p->blocks = blocks;
     p->hostBlockJob.assign(capacity / kCompactSlots, kNoJob);
@@ -1363,14 +1394,6 @@ static int poolReset(SlotPool& p, std::string* err) {
 }
 
 static uint32_t alignSlots(uint64_t n) { return (uint32_t)((n + kCompactSlots - 1) / kCompactSlots * kCompactSlots); }
-
-// A job in flight: its description, slot range and film planes.
-struct JobRun {
-    DevCamera cam;
-    DeviceFilm* out = nullptr;
-    uint32_t spp = 0, maxDepth = 100, seed = 0, planes = 1, base = 0, count = 0;
-    DevBuf<double> planeFilm;   // planes > 1: per-plane sums, reduced into out at the end
-};
 
 static int admitJob(SlotPool& p, uint32_t idx, JobRun& j, std::string* err) {
     const uint32_t P = j.cam.width * j.cam.height;
@@ -1484,6 +1507,79 @@ static int readCounters(SlotPool& p, std::string* err) {
     return 0;
 }
 
+int poolOpen(DeviceScene* ds, uint32_t capacity, SlotPool** out, std::string* err) {
+    std::unique_ptr<SlotPool> p;
+    int r = poolCreate(ds, capacity, &p, err);
+    if (r) return r;
+    if ((r = poolReset(*p, err))) return r;
+    *out = p.release();
+    return 0;
+}
+void poolClose(SlotPool* p) { delete p; }
+int poolBindThread(SlotPool* p) { return hipSetDevice(p->device) == hipSuccess ? 0 : SLRH_ERR_DEVICE_CODE; }
+uint32_t poolCapacity(const SlotPool* p) { return p->n; }
+
+int poolAdmit(SlotPool* p, uint32_t idx, const JobSpec& spec, uint32_t planes, uint32_t base, std::string* err) {
+    if (idx >= kMaxJobs || planes == 0 || base % kCompactSlots ||
+        (uint64_t)base + (uint64_t)planes * spec.width * spec.height > p->n) {
+        if (err) *err = "job does not fit the slot pool";
+        return SLRH_ERR_CAPACITY_CODE;
+    }
+    JobRun& j = p->runs[idx];
+    j.planeFilm.alloc(0, err);
+    cameraDerived(spec.camera, spec.width, spec.height, &j.cam);
+    j.out = spec.film;
+    j.spp = spec.spp;
+    j.maxDepth = spec.maxDepth == 0 ? 100u : spec.maxDepth;
+    j.seed = spec.seed;
+    j.planes = planes;
+    j.base = base;
+    j.count = planes * spec.width * spec.height;
+    return admitJob(*p, idx, j, err);
+}
+
+int poolCancel(SlotPool* p, uint32_t idx, std::string* err) {
+    JobRun& j = p->runs[idx];
+    hipLaunchKernelGGL(k_idle_slots, dim3(p->blocks), dim3(kBlock), 0, p->stream, p->ps, j.base, j.count);
+    HIP_TRY(hipGetLastError());
+    j.planes = 1;   // nothing to reduce
+    return finishJob(*p, j, err);
+}
+
+int poolRecount(SlotPool* p, std::string* err) {
+    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(256), 0, p->stream, p->ps);
+    hipLaunchKernelGGL(k_compact<0>, dim3(p->n / kCompactSlots), dim3(256), 0, p->stream, p->ps);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int poolRunBlock(SlotPool* p, int traceVariant, std::string* err) {
+    const int tv = traceVariant >= 0 && traceVariant < kNumVariants ? traceVariant : 0;
+    int r = launchGraphBlock(*p, tv, err);
+    if (r) return r;
+    if ((r = readCounters(*p, err))) return r;
+    if (p->hostCtr[C_OVERFLOW]) { if (err) *err = "traversal stack overflow (depth > 64)"; return SLRH_ERR_CAPACITY_CODE; }
+    return 0;
+}
+
+const uint32_t* poolActiveCounts(const SlotPool* p) { return p->hostCtr + kNumCtr; }
+
+int poolFinish(SlotPool* p, uint32_t idx, std::string* err) {
+    int r = finishJob(*p, p->runs[idx], err);
+    if (r) return r;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    p->runs[idx].planeFilm.alloc(0, err);
+    return 0;
+}
+
+int poolSamples(SlotPool* p, double* samples, double* rays, std::string* err) {
+    unsigned long long stat[kNumStat];
+    HIP_TRY(hipMemcpy(stat, p->stat.p, sizeof(stat), hipMemcpyDeviceToHost));
+    *samples = (double)stat[S_SAMPLES];
+    *rays = (double)(stat[S_RAYS_EXT] + stat[S_RAYS_SHADOW]);
+    return 0;
+}
+
 // Live paths the wavefront aims for: the traversal's per-launch tail of long
 // rays is amortised over enough rays (measured on the 1080p/64 spp
 // workload: 2M paths 258 ms per frame, 8M 224 ms, 16M 208 ms); path state is
@@ -1494,7 +1590,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
                std::string* err) {
     if (n == 0) return 0;
     if (n > kMaxJobs) { if (err) *err = "too many frames in one batch"; return SLRH_ERR_CAPACITY_CODE; }
-    std::vector<JobRun> jobs(n);
+    std::vector<uint32_t> planes(n), bases(n);
     uint64_t totalPixels = 0;
     for (uint32_t i = 0; i < n; ++i) totalPixels += (uint64_t)specs[i].width * specs[i].height;
     if (totalPixels == 0) return 0;
@@ -1503,16 +1599,9 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
     uint64_t capacity = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const JobSpec& js = specs[i];
-        JobRun& j = jobs[i];
-        cameraDerived(js.camera, js.width, js.height, &j.cam);
-        j.out = js.film;
-        j.spp = js.spp;
-        j.maxDepth = js.maxDepth == 0 ? 100u : js.maxDepth;
-        j.seed = js.seed;
-        j.planes = std::max(1u, std::min(K, js.spp));
-        j.base = (uint32_t)std::min<uint64_t>(capacity, UINT32_MAX);
-        j.count = j.planes * js.width * js.height;
-        capacity += alignSlots(j.count);
+        planes[i] = std::max(1u, std::min(K, js.spp));
+        bases[i] = (uint32_t)std::min<uint64_t>(capacity, UINT32_MAX);
+        capacity += alignSlots((uint64_t)planes[i] * js.width * js.height);
     }
     if (capacity > (1u << 26)) { if (err) *err = "too many path slots"; return SLRH_ERR_CAPACITY_CODE; }
     // the pool is kept between renders; it is rebuilt when too small, or when
@@ -1534,7 +1623,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
     std::vector<IterEvents> itEv;
     hipEvent_t tStart = newEvent(), tEnd = newEvent();
     for (uint32_t i = 0; i < n; ++i)
-        if ((r = admitJob(p, i, jobs[i], err))) return r;
+        if ((r = poolAdmit(&p, i, specs[i], planes[i], bases[i], err))) return r;
     HIP_TRY(hipEventRecord(tStart, p.stream));
     hipLaunchKernelGGL(k_compact<0>, dim3(p.n / kCompactSlots), dim3(256), 0, p.stream, p.ps);
     HIP_TRY(hipGetLastError());
@@ -1560,7 +1649,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
         if (p.hostCtr[C_DONE] == 0 && p.hostCtr[C_PENDING] == 0) break;
     }
     for (uint32_t i = 0; i < n; ++i)
-        if ((r = finishJob(p, jobs[i], err))) return r;
+        if ((r = finishJob(p, p.runs[i], err))) return r;
     HIP_TRY(hipEventRecord(tEnd, p.stream));
     HIP_TRY(hipStreamSynchronize(p.stream));
     unsigned long long stat[kNumStat];
