@@ -1,6 +1,6 @@
 """Benchmark of the spectral path tracer on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|batch8|serve]
                     [--no-cpu-baseline] [--cpu-sample WxHxSPP] [--out FILE]
 
 Workload (default, "c2"): the Cornell box plus a displaced icosphere at
@@ -15,6 +15,15 @@ the whole frame loop), samples/s beside it, the traversal kernel's roofline
 from HIP events and the traversal counters, and the reference renderer
 (compiled libSLR, oracle/_ref) timed on the host cores on a bounded sample of
 the same scene as the CPU baseline.
+
+Other workloads (not the headline line):
+  batch8  eight 1080p / 64 spp frames of the same scene from different camera
+          poses in one wavefront (slrh_render_batch); one step = the batch.
+  serve   the resident scene served to a seeded stream of requests (pose,
+          resolution from 256^2..1080p, 4/16/64 spp, depth 10) arriving as a
+          Poisson process; reports p50/p99 latency and served samples/s, and
+          the same requests rendered one at a time (slrh_render) for the
+          plain-GPU comparison.
 """
 import argparse
 import json
@@ -81,12 +90,124 @@ def cpu_baseline(desc, sample, rays_per_sample):
                                                                                           rays_per_sample)}
 
 
+def _scene_c2():
+    import slr_amd
+    from slr_amd import scenes
+    t0 = time.time()
+    desc = scenes.cornell_icosphere(1920, 1080, subdiv=8)
+    ctx = slr_amd.Context(0)
+    sc = slr_amd.build_scene(ctx, desc)
+    log("scene built in %.1fs" % (time.time() - t0))
+    return desc, ctx, sc
+
+
+def batch8(args):
+    """Eight 1080p x 64 spp frames (camera poses yawed -14..14 degrees) in
+    one slrh_render_batch call per step."""
+    import slr_amd
+    from slr_amd import scenes
+    desc, ctx, sc = _scene_c2()
+    W, H, spp = 1920, 1080, 64
+    frames = [dict(film=slr_amd.Film(ctx, W, H), camera=scenes.camera_pose(desc, yaw_deg=-14 + 4 * i), seed=17 + i)
+              for i in range(8)]
+    r = slr_amd.PTRenderer(spp)
+    for i in range(args.warmup):
+        for f in frames:
+            f["film"].clear()
+        r.render_batch(sc, frames)
+    tot = {}
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        for f in frames:
+            f["film"].clear()
+        st = r.render_batch(sc, frames)
+        for k, v in st.items():
+            tot[k] = tot.get(k, 0.0) + v
+        log("step %d: %.3fs device" % (i, st["device_seconds"]))
+    elapsed = time.perf_counter() - t_start
+    result = {
+        "metric": METRIC, "value": round(tot["rays"] / elapsed / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (film f64)",
+        "data": "synthetic: seeded procedural scene, 8 camera poses",
+        "config": {"workload": "8 frames x 1920x1080 x 64 spp of the cornell+displaced icosphere scene in one "
+                               "wavefront (slrh_render_batch)", "frames": 8, "width": W, "height": H, "spp": spp},
+        "samples_per_s": round(tot["samples"] / elapsed, 1),
+    }
+    print(json.dumps(result), flush=True)
+
+
+def serve(args):
+    """Serving: Poisson arrivals of seeded requests on the resident scene."""
+    import slr_amd
+    from slr_amd import scenes
+    desc, ctx, sc = _scene_c2()
+    reqs = scenes.random_requests(desc, args.requests, seed=3)
+    total_samples = sum(q["width"] * q["height"] * q["spp"] for q in reqs)
+    rng = np.random.RandomState(11)
+    gaps = rng.exponential(1.0 / args.rate, size=len(reqs)) if args.rate > 0 else np.zeros(len(reqs))
+    films = [slr_amd.Film(ctx, q["width"], q["height"]) for q in reqs]
+
+    # plain GPU: the same requests one at a time, each a full render of its
+    # own (a one-frame slrh_render_batch: slrh_render with the request's camera)
+    t0 = time.perf_counter()
+    for q, f in zip(reqs, films):
+        slr_amd.PTRenderer(q["spp"]).render_batch(sc, [dict(film=f, camera=q["camera"], seed=q["seed"])],
+                                                  max_depth=10)
+    r_plain = time.perf_counter() - t0
+    log("plain: %d requests one at a time in %.2fs" % (len(reqs), r_plain))
+
+    def run_server(gaps):
+        for f in films:
+            f.clear()
+        srv = slr_amd.Server(sc, path_state_bytes=8 << 30)
+        tickets = []
+        t0 = time.perf_counter()
+        for q, f, gap in zip(reqs, films, gaps):
+            if gap > 0:
+                time.sleep(gap)
+            tickets.append(srv.submit(f, q["camera"], spp=q["spp"], seed=q["seed"], max_depth=10))
+        states = [srv.wait(t) for t in tickets]
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        assert all(x == slr_amd.REQ_DONE for x in states)
+        return dt, st
+
+    # saturated: every request submitted at once (served throughput)
+    t_sat, st_sat = run_server(np.zeros(len(reqs)))
+    log("served, all submitted at once: %.2fs" % t_sat)
+    # Poisson arrivals at the given rate (latency under load)
+    t_poi, st_poi = run_server(gaps)
+    log("served, Poisson %.1f/s: %.2fs, p50 %.1f ms, p99 %.1f ms" % (args.rate, t_poi, st_poi["p50_ms"],
+                                                                     st_poi["p99_ms"]))
+    samples_s = total_samples / t_sat
+    result = {
+        "metric": METRIC, "value": round(samples_s, 1), "unit": "samples/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": round(t_sat * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (film f64)",
+        "data": "synthetic: seeded requests (pose, resolution, spp) on the procedural scene",
+        "config": {"workload": "serving %d requests (resolutions 256^2..1920x1080, spp 4/16/64, depth 10) on the "
+                               "resident cornell+displaced icosphere scene" % len(reqs),
+                   "requests": len(reqs), "rate": args.rate},
+        "served_samples_per_s": round(samples_s, 1),
+        "plain_samples_per_s": round(total_samples / r_plain, 1),
+        "served_over_plain": round(r_plain / t_sat, 3),
+        "poisson": {"rate_per_s": args.rate, "seconds": round(t_poi, 3), "p50_ms": round(st_poi["p50_ms"], 2),
+                    "p99_ms": round(st_poi["p99_ms"], 2), "mean_ms": round(st_poi["mean_ms"], 2)},
+        "saturated": {"p50_ms": round(st_sat["p50_ms"], 2), "p99_ms": round(st_sat["p99_ms"], 2)},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "batch8", "serve"])
+    ap.add_argument("--rate", type=float, default=16.0, help="serve: request arrivals per second")
+    ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", default="960x540x2")
@@ -103,6 +224,11 @@ def main():
 
     import slr_amd
     from slr_amd import scenes
+
+    if args.config == "serve":
+        return serve(args)
+    if args.config == "batch8":
+        return batch8(args)
 
     t0 = time.time()
     if args.config == "c2":

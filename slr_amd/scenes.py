@@ -267,6 +267,40 @@ def cornell_icosphere(width=1920, height=1080, subdiv=8, seed=7):
     return desc
 
 
+def camera_pose(desc, yaw_deg=0.0, dx=0.0, dy=0.0, width=None, height=None):
+    """desc's camera turned by yaw_deg about the vertical axis through the
+    scene's centre and shifted by (dx, dy) in its own plane; aspect from
+    width/height when given."""
+    cam = dict(desc["camera"])
+    allp = np.concatenate([m["pos"] for m in desc["meshes"]]).astype(np.float64)
+    centre = 0.5 * (allp.min(0) + allp.max(0))
+    l2w = np.array(cam["l2w"], dtype=np.float64).reshape(4, 4).copy()
+    t = math.radians(yaw_deg)
+    r = np.array([[math.cos(t), 0.0, math.sin(t)], [0.0, 1.0, 0.0], [-math.sin(t), 0.0, math.cos(t)]])
+    pos = l2w[:3, 3] + l2w[:3, 0] * dx + l2w[:3, 1] * dy
+    l2w[:3, :3] = r @ l2w[:3, :3]
+    l2w[:3, 3] = centre + r @ (pos - centre)
+    cam["l2w"] = l2w
+    if width and height:
+        cam["aspect"] = width / height
+    return cam
+
+
+def random_requests(desc, n, seed=3, sizes=((256, 256), (512, 512), (1024, 1024), (1920, 1080)),
+                    spps=(4, 16, 64)):
+    """Seeded serving workload: per request a camera pose, a resolution and
+    a sample count drawn uniformly from the lists."""
+    rng = np.random.RandomState(seed)
+    out = []
+    for i in range(n):
+        w, h = sizes[rng.randint(len(sizes))]
+        spp = int(spps[rng.randint(len(spps))])
+        cam = camera_pose(desc, yaw_deg=rng.uniform(-12, 12), dx=rng.uniform(-0.3, 0.3), dy=rng.uniform(-0.2, 0.2),
+                          width=w, height=h)
+        out.append({"width": w, "height": h, "spp": spp, "camera": cam, "seed": int(rng.randint(1, 2 ** 31))})
+    return out
+
+
 def random_rays(desc, n, seed=1, mode="mixed"):
     """Seeded ray batch (n, 8) {org, dir, tmin, tmax} through the scene
     bounds: origins inside the box, directions uniform on the sphere."""
