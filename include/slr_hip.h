@@ -109,6 +109,16 @@ SLRH_API int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, con
 
 /* Camera-to-world 4x4 (column-major) and PerspectiveCamera parameters
  * (libSLR/Camera/PerspectiveCamera.cpp:16-27, libSLR/Scene/camera_nodes.h:16-35). */
+/* Image-based environment light: an equirectangular image of width*height
+ * linear sRGB texels (row 0 at the +y pole, phi = 0 towards +z, as
+ * Vector3D::toPolarYUp maps directions), radiance scale. Replaces
+ * InfiniteSphereNode(scene, ImageSpectrumTexture, scale) set with
+ * Scene::setEnvironmentNode (libSLR/Scene/node.h:133, Scene.h:33); the
+ * texels are kept as the reference keeps an RGBA16F light-source image
+ * (uvs truncated to half precision, libSLR/Core/image_2d.h:270-289). Call
+ * before slrh_scene_build. */
+SLRH_API int slrh_environment(slrh_scene* s, const float* rgb, uint32_t width, uint32_t height, float scale);
+
 SLRH_API int slrh_camera_perspective(slrh_scene* s, const float* l2w, float sensitivity, float aspect,
                                      float fov_y, float lens_radius, float img_dist, float obj_dist);
 

@@ -62,6 +62,10 @@
 #include <libSLR/Scene/TriangleMeshNode.h>
 #include <libSLR/Scene/camera_nodes.h>
 #include <libSLR/Texture/constant_textures.h>
+#include <libSLR/Texture/image_textures.h>
+#include <libSLR/Core/image_2d.h>
+#include <libSLR/Core/textures.h>
+#include <libSLR/SurfaceMaterial/IBLEmitterSurfaceProperty.h>
 #include <libSLR/SurfaceMaterial/basic_surface_materials.h>
 #include <libSLR/SurfaceMaterial/microfacet_surface_materials.h>
 #include <libSLR/SurfaceMaterial/basic_emitter_surface_properties.h>
@@ -116,6 +120,13 @@ struct RefScene {
     float l2w[16];
     float sensitivity = 1, aspect = 1, fovY = 0.5f, lensRadius = 0, imgDist = 1, objDist = 1;
 
+    // environment: an RGBA16F equirectangular image stored as a light-source
+    // uvs texture (image_2d.h:270-289) behind the default texture mapping
+    TiledImage2D* envImage = nullptr;
+    Texture2DMapping envMapping;
+    std::unique_ptr<ImageSpectrumTexture> envTex;
+    float envScale = 1.0f;
+
     // accelerator for ray batches
     bool built = false;
     std::vector<SurfaceObject*> objs;
@@ -124,6 +135,8 @@ struct RefScene {
     QBVH* qbvh = nullptr;
 
     ~RefScene() {
+        envTex.reset();
+        if (envImage) DefaultAllocator::instance().destroy(envImage);
         delete qbvh;
         delete sbvh;
         if (built)
@@ -370,6 +383,23 @@ int refd_visibility(void* sp, int accel, int n, const float* rays, float* vis) {
     return 0;
 }
 
+// Equirectangular environment of w x h RGB floats (rows from the +y pole),
+// converted to half as an RGBA16F image (alpha 1) and attached to the scene
+// by InfiniteSphereNode (libSLR/Scene/node.cpp:201-206) with the given scale.
+int refd_set_environment(void* sp, int w, int h, const float* rgb, float scale) {
+    RefScene* s = (RefScene*)sp;
+    if (w < 4 || h < 4 || s->envTex) return -1;
+    std::vector<RGBA16Fx4> px((size_t)w * h);
+    for (size_t i = 0; i < px.size(); ++i)
+        px[i] = RGBA16Fx4{half(rgb[3 * i]), half(rgb[3 * i + 1]), half(rgb[3 * i + 2]), half(1.0f)};
+    s->envImage = DefaultAllocator::instance().create<TiledImage2D>(
+        (const void*)px.data(), (uint32_t)w, (uint32_t)h, ColorFormat::RGBA16Fx4, &DefaultAllocator::instance(),
+        ImageStoreMode::AsIs, SpectrumType::LightSource);
+    s->envTex.reset(new ImageSpectrumTexture(s->envImage, &s->envMapping));
+    s->envScale = scale;
+    return 0;
+}
+
 // PTRenderer::render over the scene; film_out receives h*w*16 floats (row-major
 // pixels, 16 strata each) of the raw sensor sums. Returns render seconds
 // (excluding Scene::build) or a negative value on error.
@@ -389,6 +419,11 @@ double refd_render(void* sp, int w, int h, int spp, uint32_t seed, int threads, 
     root.addChildNode(&camNode);
 
     Scene scene(&root);
+    std::unique_ptr<InfiniteSphereNode> envNode;
+    if (s->envTex) {
+        envNode.reset(new InfiniteSphereNode(&scene, s->envTex.get(), s->envScale));
+        scene.setEnvironmentNode(envNode.get());
+    }
     auto t0 = std::chrono::steady_clock::now();
     scene.build(&DefaultAllocator::instance());
     auto t1 = std::chrono::steady_clock::now();

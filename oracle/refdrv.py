@@ -42,6 +42,7 @@ def lib():
             "refd_qbvh_dump": (i32, [vp, vp, vp, vp]),
             "refd_intersect": (i32, [vp, i32, i32, vp, vp, vp, vp, vp]),
             "refd_visibility": (i32, [vp, i32, i32, vp, vp]),
+            "refd_set_environment": (i32, [vp, i32, i32, vp, f32]),
             "refd_render": (f64, [vp, i32, i32, i32, u32, i32, vp, i32, vp]),
             "refd_rng_stream": (None, [u32, i32, vp, vp]),
             "refd_wavelengths": (None, [f32, f32, vp, vp, vp]),
@@ -99,6 +100,12 @@ class RefScene:
             idx = np.ascontiguousarray(mesh["idx"], np.uint32)
             self._keep += [pos, nrm, tan, idx]
             L.refd_add_mesh(self.h, len(pos), _p(pos), _p(nrm), _p(tan), None, len(idx), _p(idx), ids[int(mat)])
+        env = desc.get("environment")
+        if env is not None:
+            rgb = np.ascontiguousarray(env["rgb"], np.float32)
+            self._keep.append(rgb)
+            if L.refd_set_environment(self.h, rgb.shape[1], rgb.shape[0], _p(rgb), float(env.get("scale", 1.0))):
+                raise ValueError("reference environment needs at least 4x4 texels")
         cam = desc["camera"]
         m = np.ascontiguousarray(np.asarray(cam["l2w"], np.float32).reshape(4, 4).T)
         L.refd_set_camera(self.h, _p(m), cam.get("sensitivity", 1.0), cam["aspect"], cam["fov_y"],

@@ -37,7 +37,7 @@ EXPORTS = [
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
     "slrh_intersect", "slrh_occluded",
     "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read",
-    "slrh_render", "slrh_render_batch", "slrh_trace_variants",
+    "slrh_environment", "slrh_render", "slrh_render_batch", "slrh_trace_variants",
     "slrh_server_create", "slrh_server_destroy", "slrh_submit", "slrh_poll", "slrh_wait", "slrh_cancel",
     "slrh_server_stats",
 ]
@@ -123,6 +123,7 @@ def load_library(path=None):
         "slrh_film_read": (i32, [vp, i32, f64, vp, ctypes.c_size_t]),
         "slrh_render": (i32, [vp, vp, vp, vp]),
         "slrh_render_batch": (i32, [vp, vp, u32, vp, vp]),
+        "slrh_environment": (i32, [vp, vp, u32, u32, f32]),
         "slrh_server_create": (i32, [vp, vp, pp]),
         "slrh_server_destroy": (i32, [vp]),
         "slrh_submit": (i32, [vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
@@ -247,6 +248,12 @@ class Scene:
         self.ctx.check(self.lib.slrh_camera_perspective(self.h, _ptr(m), float(sensitivity), float(aspect),
                                                         float(fov_y), float(lens_radius), float(img_dist),
                                                         float(obj_dist)), "slrh_camera_perspective")
+
+    def set_environment(self, rgb, scale=1.0):
+        """rgb: (height, width, 3) linear sRGB equirectangular image."""
+        img = _f32(np.asarray(rgb, dtype=np.float32))
+        h, w = img.shape[0], img.shape[1]
+        self.ctx.check(self.lib.slrh_environment(self.h, _ptr(img), w, h, float(scale)), "slrh_environment")
 
     def build(self):
         self.ctx.check(self.lib.slrh_scene_build(self.h), "slrh_scene_build")
@@ -521,5 +528,8 @@ def build_scene(ctx, desc):
     cam = desc["camera"]
     sc.set_camera(cam["l2w"], cam["aspect"], cam["fov_y"], cam.get("lens_radius", 0.0), cam.get("img_dist", 1.0),
                   cam.get("obj_dist", 1.0), cam.get("sensitivity", 1.0))
+    if desc.get("environment") is not None:
+        env = desc["environment"]
+        sc.set_environment(env["rgb"], env.get("scale", 1.0))
     sc.build()
     return sc

@@ -293,6 +293,14 @@ int slrh_camera_perspective(slrh_scene* s, const float* l2w, float sensitivity, 
     return SLRH_OK;
 }
 
+int slrh_environment(slrh_scene* s, const float* rgb, uint32_t width, uint32_t height, float scale) {
+    CHECK_SCENE(s);
+    if (!rgb || width == 0 || height == 0 || !(scale >= 0) || (uint64_t)width * height > (1ull << 28))
+        return fail(s->ctx, SLRH_ERR_INVALID, "environment: bad args");
+    buildEnvironment(rgb, width, height, scale, &s->data.env);
+    return SLRH_OK;
+}
+
 int slrh_scene_build(slrh_scene* s) {
     CHECK_SCENE(s);
     std::lock_guard<std::mutex> lk(s->ctx->mu);

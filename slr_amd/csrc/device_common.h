@@ -96,6 +96,13 @@ struct DevScene {
     float lightImportance;    // SurfaceObjectAggregate::importance()
     uint32_t numNodes;
     DevCamera cam;
+    // environment light (image-based); hasEnv = 0 without one
+    const float4* envTexel;   // per texel: Meng basis indices (bits), s, t, scale
+    const float* mengBasis;   // [kMengPoints][95]
+    const float *envCondPdf, *envCondCdf, *envMargPdf, *envMargCdf;
+    uint32_t envW, envH, envMapW, envMapH;
+    float envScale;
+    uint32_t hasEnv;
 };
 
 __device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }

@@ -24,6 +24,7 @@ struct SceneData {
     std::vector<DevMaterial> materials;
     bool hasCamera = false;
     CameraParams camera;
+    EnvData env;
     Accel accel;
 };
 

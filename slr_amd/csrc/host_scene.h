@@ -115,4 +115,27 @@ HostSpectrum mengToRegular(const MengSpectrum& m);
 // CMF strata for the film (DiscretizedSpectrum::initialize).
 void cmfStrata(float xbar[16], float ybar[16], float zbar[16], float* integral);
 
+// Environment light (image-based): the texels as the reference stores an
+// RGBA16F image for a light source (u, v and scaled s truncated to half
+// precision, libSLR/Core/image_2d.h:270-289), each decoded into its Meng
+// basis weights, and a piecewise-constant importance map over (phi, theta).
+struct EnvData {
+    bool present = false;
+    uint32_t width = 0, height = 0;
+    float scale = 1.0f;
+    std::vector<MengSpectrum> texels;   // row-major, row = theta, column = phi
+    uint32_t mapW = 0, mapH = 0;
+    std::vector<float> condPdf, condCdf;   // per map row: mapW densities, mapW + 1 CDF values
+    std::vector<float> margPdf, margCdf;   // over rows: mapH densities, mapH + 1 CDF values
+    std::vector<float> rowIntegral;
+};
+// rgb: width * height linear sRGB triples (row 0 = theta 0, +y up).
+void buildEnvironment(const float* rgb, uint32_t width, uint32_t height, float scale, EnvData* env);
+// Value of a float truncated toward zero to IEEE half precision (overflow to
+// infinity), the conversion of the reference's half type (External/half.hpp
+// 1.12 with its default rounding style).
+float truncToHalf(double x);
+// The Meng basis table and grid constants the device evaluates texels with.
+const float* mengBasisTable(uint32_t* numPoints, uint32_t* numLambdas);
+
 }  // namespace slrh

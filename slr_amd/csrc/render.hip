@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -114,6 +115,8 @@ struct DeviceScene {
     DevBuf<uint32_t> lights;
     DevBuf<int32_t> primToLight;
     DevBuf<uint32_t> spill;      // traversal stack spill for ray batches
+    DevBuf<float4> envTexel;
+    DevBuf<float> mengBasis, envCondPdf, envCondCdf, envMargPdf, envMargCdf;
     DevBuf<unsigned long long> counters;
     DevScene view;
     bool hasCamera = false;
@@ -304,6 +307,40 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         (r = ds->primToLight.upload(primToLight, err)))
         return r;
     if ((r = ds->counters.alloc(16, err))) return r;
+    DevScene& ve = ds->view;
+    ve.hasEnv = sd.env.present ? 1u : 0u;
+    ve.envTexel = nullptr;
+    ve.mengBasis = nullptr;
+    ve.envCondPdf = ve.envCondCdf = ve.envMargPdf = ve.envMargCdf = nullptr;
+    ve.envW = ve.envH = ve.envMapW = ve.envMapH = 0;
+    ve.envScale = 0.0f;
+    if (sd.env.present) {
+        const EnvData& e = sd.env;
+        std::vector<float4> tex(e.texels.size());
+        for (size_t i = 0; i < tex.size(); ++i) {
+            float adj;
+            memcpy(&adj, &e.texels[i].adj, 4);
+            tex[i] = make_float4(adj, e.texels[i].s, e.texels[i].t, e.texels[i].scale);
+        }
+        uint32_t np = 0, nl = 0;
+        const float* basis = mengBasisTable(&np, &nl);
+        std::vector<float> b(basis, basis + (size_t)np * nl);
+        if ((r = ds->envTexel.upload(tex, err)) || (r = ds->mengBasis.upload(b, err)) ||
+            (r = ds->envCondPdf.upload(e.condPdf, err)) || (r = ds->envCondCdf.upload(e.condCdf, err)) ||
+            (r = ds->envMargPdf.upload(e.margPdf, err)) || (r = ds->envMargCdf.upload(e.margCdf, err)))
+            return r;
+        ve.envTexel = ds->envTexel.p;
+        ve.mengBasis = ds->mengBasis.p;
+        ve.envCondPdf = ds->envCondPdf.p;
+        ve.envCondCdf = ds->envCondCdf.p;
+        ve.envMargPdf = ds->envMargPdf.p;
+        ve.envMargCdf = ds->envMargCdf.p;
+        ve.envW = e.width;
+        ve.envH = e.height;
+        ve.envMapW = e.mapW;
+        ve.envMapH = e.mapH;
+        ve.envScale = e.scale;
+    }
     ds->bytes = nodes.size() * 16 + tris.size() * 16 + (vp.size() * 3) * 16 + pv.size() * 16 + pool.size() * 4;
     DevScene& v = ds->view;
     v.nodes = ds->nodes.p; v.tris = ds->tris.p; v.primVerts = ds->primVerts.p;
@@ -652,6 +689,113 @@ __device__ __forceinline__ void resolveShadow(const PathState& ps, uint32_t s, f
     }
 }
 
+// ------------------------------------------------------------------ environment light
+
+constexpr double kPiD = 3.14159265358979323846;
+
+// UpsampledContinuousSpectrum::evaluate (libSLR/BasicTypes/spectrum_types.cpp:
+// 611-651) of one environment texel at the path's 16 wavelengths: the three
+// or four weighted basis spectra, each interpolated linearly in wavelength.
+__device__ __forceinline__ void envTexelSpectrum(const DevScene& sc, uint32_t texel, float off, float* out) {
+    const float4 tx = sc.envTexel[texel];
+    const uint32_t adj = __float_as_uint(tx.x);
+    uint32_t idx[4];
+    float w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) idx[k] = (adj >> (8 * k)) & 0xFFu;
+    int n;
+    if (idx[3] != 0xFFu) {
+        w[0] = (1 - tx.y) * (1 - tx.z);
+        w[1] = tx.y * (1 - tx.z);
+        w[2] = (1 - tx.y) * tx.z;
+        w[3] = tx.y * tx.z;
+        n = 4;
+    } else {
+        w[0] = tx.y;
+        w[1] = tx.z;
+        w[2] = 1.0f - tx.y - tx.z;
+        w[3] = 0.0f;
+        n = 3;
+    }
+#pragma unroll
+    for (int i = 0; i < kNumL; ++i) {
+        float p = (lambdaOf(i, off) - kLambdaLowD) / (kLambdaHighD - kLambdaLowD);
+        p = fminf(fmaxf(p, 0.0f), 1.0f);
+        const float binF = p * 94;
+        const uint32_t bin = min((uint32_t)binF, 94u), nxt = min(bin + 1, 94u);
+        const float t = binF - bin;
+        float r = 0.0f;
+        for (int j = 0; j < n; ++j) {
+            const float* sp = sc.mengBasis + idx[j] * 95;
+            r += w[j] * (sp[bin] * (1 - t) + sp[nxt] * t);
+        }
+        out[i] = r * tx.w;
+    }
+}
+
+// Radiance reaching a path from the environment at texture coordinate
+// (tu, tv): IBLEmitterSurfaceProperty::emittance (pi * texture * scale,
+// IBLEmitterSurfaceProperty.cpp:16-18) times IBLEDF's 1/pi (IBLEDF.cpp:18-22),
+// the texel chosen as ImageSpectrumTexture::evaluate does
+// (libSLR/Texture/image_textures.cpp:14-21).
+__device__ __forceinline__ void envRadiance(const DevScene& sc, float tu, float tv, float off, float* Le) {
+    float u = fmodf(tu, 1.0f), v = fmodf(tv, 1.0f);
+    u += u < 0 ? 1.0f : 0.0f;
+    v += v < 0 ? 1.0f : 0.0f;
+    const uint32_t px = min((uint32_t)(sc.envW * u), sc.envW - 1);
+    const uint32_t py = min((uint32_t)(sc.envH * v), sc.envH - 1);
+    envTexelSpectrum(sc, py * sc.envW + px, off, Le);
+    const float pi = (float)kPiD, invPi = (float)(1.0f / kPiD);
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) Le[k] = ((pi * Le[k]) * sc.envScale) * invPi;
+}
+
+// Vector3D::toPolarYUp (libSLR/BasicTypes/Vector3D.h:78-81).
+__device__ __forceinline__ void toPolarYUp(float3 d, float* theta, float* phi) {
+    *theta = acosf(fminf(fmaxf(d.y, -1.0f), 1.0f));
+    *phi = fmodf((float)(atan2f(-d.x, d.z) + 2 * kPiD), (float)(2 * kPiD));
+}
+
+__device__ __forceinline__ uint32_t prevPowerOf2(uint32_t n) {
+    uint32_t d = 1;
+    while ((d << 1) <= n) d <<= 1;
+    return d;
+}
+
+// RegularConstantContinuousDistribution1D::sample (libSLR/Core/distributions.cpp:212-223).
+__device__ __forceinline__ float dist1DSample(const float* cdf, const float* pdf, uint32_t n, float u, float* p) {
+    int idx = (int)n;
+    for (uint32_t d = prevPowerOf2(n); d > 0; d >>= 1)
+        if (idx - (int)d > 0 && cdf[idx - d] >= u) idx -= (int)d;
+    --idx;
+    *p = pdf[idx];
+    const float t = (u - cdf[idx]) / (cdf[idx + 1] - cdf[idx]);
+    return (idx + t) / n;
+}
+
+// RegularConstantContinuousDistribution2D::sample / evaluatePDF
+// (distributions.cpp:264-280) over the environment's importance map:
+// d1 (theta / pi) from the row marginal, d0 (phi / 2 pi) from the row.
+__device__ __forceinline__ void envSample(const DevScene& sc, float u0, float u1, float* d0, float* d1, float* pdf) {
+    float topPdf;
+    *d1 = dist1DSample(sc.envMargCdf, sc.envMargPdf, sc.envMapH, u1, &topPdf);
+    const uint32_t row = min((uint32_t)(sc.envMapH * *d1), sc.envMapH - 1);
+    *d0 = dist1DSample(sc.envCondCdf + row * (sc.envMapW + 1), sc.envCondPdf + row * sc.envMapW, sc.envMapW, u0, pdf);
+    *pdf *= topPdf;
+}
+__device__ __forceinline__ float envPdf(const DevScene& sc, float d0, float d1) {
+    const uint32_t row = min((uint32_t)(sc.envMapH * d1), sc.envMapH - 1);
+    const uint32_t col = min((uint32_t)(d0 * sc.envMapW), sc.envMapW - 1);
+    const uint32_t r1 = min((uint32_t)(d1 * sc.envMapH), sc.envMapH - 1);
+    return sc.envMargPdf[r1] * sc.envCondPdf[row * sc.envMapW + col];
+}
+
+// InfiniteSphereSurfaceObject::evaluateAreaPDF (libSLR/Core/surface_object.cpp:281-286).
+__device__ __forceinline__ float envAreaPdf(const DevScene& sc, float phi, float theta) {
+    const float uvPdf = envPdf(sc, (float)(phi / (2 * kPiD)), (float)(theta / kPiD));
+    return (float)(uvPdf / (2 * kPiD * kPiD * sinf(theta)));
+}
+
 // ------------------------------------------------------------------ kernels
 
 // Puts job `job` (slots base .. base + planes * pixels - 1) into the pool:
@@ -965,7 +1109,13 @@ __device__ __forceinline__ uint32_t sampleLight(const DevScene& sc, float u, flo
 // Probability of choosing the surface lights over other emitters
 // (Scene::selectSurfaceLight, libSLR/Scene/Scene.cpp:137-155): 1 without an
 // environment light.
-__device__ __forceinline__ float lightSelectScale(const DevScene&) { return 1.0f; }
+__device__ __forceinline__ float lightSelectScale(const DevScene& sc) {
+    if (!sc.hasEnv) return 1.0f;
+    // InfiniteSphereSurfaceObject::importance() is 1 (surface_object.cpp:197-199)
+    return sc.lightImportance / (sc.lightImportance + 1.0f);
+}
+// ... and of choosing the environment light.
+__device__ __forceinline__ float envSelectProb(const DevScene& sc) { return 1.0f / (sc.lightImportance + 1.0f); }
 
 template <uint32_t CLS>
 __device__ __forceinline__ void evalBsdf(const DevMaterial& m, const float* A, const float* B, const BsdfQuery& q,
@@ -1028,12 +1178,38 @@ void k_shade(DevScene sc, PathState ps) {
         if (valid) {
             s = qin[i];
             if (CLS == CLS_MISS) {
-                // the extension ray left the scene: the path ends here
+                // the extension ray left the scene: the path ends here, after
+                // the environment's emission when there is one (Scene::intersect's
+                // fallback, libSLR/Scene/Scene.cpp:57-62; PTRenderer.cpp:160-166
+                // for a camera ray, :247-263 with MIS otherwise)
                 const uint4 info = ps.u4(s, SF_INFO);
                 const float4 misc = ps.f4(s, SF_MISC);
                 const uint4 rng = ps.u4(s, SF_RNG);
+                const uint32_t pathLen = info.y >> 16;
                 float rad[kNumL];
-                loadRadiance(ps, s, info.y >> 16, rad);
+                loadRadiance(ps, s, pathLen, rad);
+                if (sc.hasEnv) {
+                    const float3 rd = xyz(ps.f4(s, SF_RD));
+                    float theta, phi;
+                    toPolarYUp(rd, &theta, &phi);
+                    float Le[kNumL], alpha[kNumL];
+                    envRadiance(sc, (float)(phi / (2 * kPiD)), (float)(theta / kPiD), misc.x, Le);
+                    float misW = 1.0f;
+                    if (pathLen > 0 && !(info.y & F_LAST_DELTA)) {
+                        const float bsdfPDF = misc.z;
+                        const float lightProb = envSelectProb(sc);
+                        const float lightPDF = lightProb * envAreaPdf(sc, phi, theta) * 1.0f / fabsf(dot3(rd, -rd));
+                        misW = (bsdfPDF * bsdfPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+                    }
+                    if (pathLen == 0) {
+#pragma unroll
+                        for (int k = 0; k < kNumL; ++k) alpha[k] = 1.0f;
+                    } else {
+                        loadPlanes(ps, s, SF_ALPHA, alpha);
+                    }
+#pragma unroll
+                    for (int k = 0; k < kNumL; ++k) rad[k] += alpha[k] * Le[k] * misW;
+                }
                 code = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
             } else {
                 bool cont = false, shadow = false;
@@ -1097,11 +1273,60 @@ void k_shade(DevScene sc, PathState ps) {
                     evalSpectrum(sc, mat.spec0, misc.x, A);
                     if (CLS != CLS_DIFFUSE) evalSpectrum(sc, mat.spec1, misc.x, B);
                     // ---- next event estimation (PTRenderer.cpp:179-218)
-                    {
-                        const float uSel = xsFloat(rng);
-                        const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
+                    const float uSel = xsFloat(rng);
+                    const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
+                    // Scene::selectSurfaceLight (libSLR/Scene/Scene.cpp:137-155):
+                    // surface lights or the environment by importance
+                    bool envLight = false;
+                    float uSurf = uSel;
+                    if (sc.hasEnv) {
+                        const float surfImp = sc.lightImportance, sumImps = surfImp + 1.0f;
+                        if (sumImps * uSel < surfImp) uSurf = uSel / (surfImp / sumImps);
+                        else envLight = true;
+                    }
+                    if (envLight) {
+                        // InfiniteSphereSurfaceObject::sample (surface_object.cpp:201-238)
+                        float d0, d1, uvPdf;
+                        envSample(sc, lu0, lu1, &d0, &d1, &uvPdf);
+                        const float phi = (float)(d0 * (2 * kPiD)), theta = (float)(d1 * kPiD);
+                        const float sinT = sinf(theta);
+                        const float3 pos = f3(-sinf(phi) * sinT, cosf(theta), cosf(phi) * sinT);
+                        const float areaPDF = (float)(uvPdf / (2 * kPiD * kPiD * sinT));
+                        // a point at infinity: unit distance, geometric normal -pos
+                        // (SurfacePoint::getDirectionFrom, geometry.h:147-151)
+                        const float3 sdir = normalize3(pos);
+                        const float cosS = fabsf(dot3(sdir, sp.gN));
+                        const float cosL = fabsf(dot3(sdir, -pos));
+                        if (cosS * cosL > 0) {
+                            float Le[kNumL], fs[kNumL];
+                            envRadiance(sc, (float)(phi / (2 * kPiD)), (float)(theta / kPiD), misc.x, Le);
+                            const float3 sdL = toLocal(sp, sdir);
+                            float bpdf = 0.0f;
+                            evalBsdf<CLS>(mat, A, B, q, sdL, fs, &bpdf);
+                            const float lightPDF = envSelectProb(sc) * areaPDF;
+                            const float bsdfPDF = bpdf * cosL;
+                            const float mis = isinf(areaPDF) ? 1.0f
+                                                             : (lightPDF * lightPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+                            const float scal = (cosS * cosL) * mis / lightPDF;
+                            float c[kNumL];
+                            bool any = false;
+#pragma unroll
+                            for (int k = 0; k < kNumL; ++k) {
+                                c[k] = alpha[k] * Le[k] * fs[k] * scal;
+                                any = any || c[k] != 0.0f;
+                            }
+                            if (any) {
+                                // Scene::testVisibility toward infinity: [Epsilon, FLT_MAX]
+                                storePlanes(ps, s, SF_NEE, c);
+                                ps.f4(s, SF_SO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
+                                ps.f4(s, SF_SD) = make_float4(sdir.x, sdir.y, sdir.z, FLT_MAX);
+                                ps.f4(s, SF_SINV) = rayInv(sdir);
+                                shadow = true;
+                            }
+                        }
+                    } else {
                         float lprob;
-                        const uint32_t li = sampleLight(sc, uSel, &lprob);
+                        const uint32_t li = sampleLight(sc, uSurf, &lprob);
                         const uint32_t lprim = sc.lights[li];
                         // TriangleSurfaceShape::sample (TriangleSurfaceShape.cpp:357-413)
                         const float su = sqrtf(lu0);

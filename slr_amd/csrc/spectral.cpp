@@ -11,6 +11,7 @@
 #include "host_scene.h"
 
 #include <algorithm>
+#include <cmath>
 
 #include "spectral_tables.inc"
 
@@ -146,6 +147,110 @@ HostSpectrum mengToRegular(const MengSpectrum& m) {
         h.values[k] = (float)(acc * m.scale);
     }
     return h;
+}
+
+float truncToHalf(double x) {
+    if (std::isnan(x)) return (float)x;
+    const double ax = std::fabs(x);
+    if (ax >= 65536.0) return std::copysign(INFINITY, (float)x);   // beyond the largest half: infinity
+    if (ax == 0.0) return (float)x;
+    int e;
+    std::frexp(ax, &e);   // ax = m * 2^e, m in [0.5, 1)
+    // half: 10 fraction bits for normals (exponent >= -14), fixed 2^-24 steps below
+    const int shift = e - 1 >= -14 ? (e - 1) - 10 : -24;
+    const double q = std::ldexp(std::floor(std::ldexp(ax, -shift)), shift);
+    if (q >= 65520.0) return std::copysign(INFINITY, (float)x);
+    return (float)std::copysign(q, x);
+}
+
+const float* mengBasisTable(uint32_t* numPoints, uint32_t* numLambdas) {
+    *numPoints = tables::kMengNumPoints;
+    *numLambdas = tables::kMengNumLambdas;
+    return &tables::kMengSpectra[0][0];
+}
+
+namespace {
+// RegularConstantContinuousDistribution1D's construction
+// (libSLR/Core/distributions.cpp:172-191): densities over [0, 1) and a CDF
+// built with a compensated sum, both normalised by the integral when positive.
+float buildDist1D(const float* values, uint32_t n, float* pdf, float* cdf) {
+    float sum = 0, comp = 0;
+    cdf[0] = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        pdf[i] = values[i];
+        const float in = values[i] / n - comp, tmp = sum + in;
+        comp = (tmp - sum) - in;
+        sum = tmp;
+        cdf[i + 1] = sum;
+    }
+    if (sum > 0)
+        for (uint32_t i = 0; i < n; ++i) {
+            pdf[i] /= sum;
+            cdf[i + 1] /= sum;
+        }
+    return sum;
+}
+}  // namespace
+
+void buildEnvironment(const float* rgb, uint32_t w, uint32_t h, float scale, EnvData* env) {
+    constexpr double kScaleFactor = 0.009355121400914532;   // UPSAMPLED_CONTINOUS_SPECTRUM_SCALE_FACTOR
+    env->present = true;
+    env->width = w;
+    env->height = h;
+    env->scale = scale;
+    env->texels.resize((size_t)w * h);
+    std::vector<float> lum((size_t)w * h);
+    for (size_t i = 0; i < (size_t)w * h; ++i) {
+        // RGBA16F input, clamped at zero, to uvs (sRGB_to_uvs for a light
+        // source, spectrum_types.h:219-242), stored as half
+        float c[3];
+        for (int k = 0; k < 3; ++k) c[k] = std::max(truncToHalf(rgb[3 * i + k]), 0.0f);
+        const float X = (float)(0.4124564 * c[0] + 0.3575761 * c[1] + 0.1804375 * c[2]);
+        const float Y = (float)(0.2126729 * c[0] + 0.7151522 * c[1] + 0.0721750 * c[2]);
+        const float Z = (float)(0.0193339 * c[0] + 0.1191920 * c[1] + 0.9503041 * c[2]);
+        const float b = X + Y + Z;
+        float x = X / b, y = Y / b;
+        if (b == 0) x = y = (float)(1.0f / 3.0);
+        const float u = (float)(16.730260708356887 * x + 7.7801960340706 * y - 2.170152247475828);
+        const float v = (float)(-7.530081094743006 * x + 16.192422314095225 * y + 1.1125529268825947);
+        const float sv = (float)(b / kEqualEnergyReflectance);
+        const float uh = truncToHalf(u), vh = truncToHalf(v), sh = truncToHalf((double)sv * kScaleFactor);
+        // ImageSpectrumTexture::evaluate: UpsampledContinuousSpectrum(u, v, s / scale factor)
+        MengSpectrum& m = env->texels[i];
+        computeAdjacents(uh, vh, &m);
+        m.scale = (float)((double)sh / kScaleFactor);
+        // luminance Y of the stored texel (uvs_to_luminance, spectrum_types.h:267-272)
+        const float yy = (float)(0.022853819546830627 * uh + 0.05077639329371236 * vh - 0.006895157122499944);
+        lum[i] = yy * (float)(m.scale * kEqualEnergyReflectance);
+    }
+    // importance map at a quarter of the resolution (as createIBLImportanceMap,
+    // image_textures.cpp:142-199): per cell the mean texel luminance times the
+    // sine of the cell's polar angle
+    const uint32_t mw = std::max(1u, w / 4), mh = std::max(1u, h / 4);
+    env->mapW = mw;
+    env->mapH = mh;
+    std::vector<float> cell((size_t)mw * mh);
+    for (uint32_t cy = 0; cy < mh; ++cy)
+        for (uint32_t cx = 0; cx < mw; ++cx) {
+            const uint32_t x0 = (uint32_t)((uint64_t)cx * w / mw), x1 = (uint32_t)((uint64_t)(cx + 1) * w / mw);
+            const uint32_t y0 = (uint32_t)((uint64_t)cy * h / mh), y1 = (uint32_t)((uint64_t)(cy + 1) * h / mh);
+            double acc = 0;
+            for (uint32_t yy = y0; yy < y1; ++yy)
+                for (uint32_t xx = x0; xx < x1; ++xx) acc += lum[(size_t)yy * w + xx];
+            const double mean = acc / ((double)(x1 - x0) * (y1 - y0));
+            cell[(size_t)cy * mw + cx] = (float)(std::sin(M_PI * (cy + 0.5f) / mh) * mean);
+        }
+    // RegularConstantContinuousDistribution2D (distributions.cpp:242-262):
+    // one distribution per row over phi, and one over the rows' integrals
+    env->condPdf.resize((size_t)mw * mh);
+    env->condCdf.resize((size_t)(mw + 1) * mh);
+    env->rowIntegral.resize(mh);
+    for (uint32_t cy = 0; cy < mh; ++cy)
+        env->rowIntegral[cy] = buildDist1D(&cell[(size_t)cy * mw], mw, &env->condPdf[(size_t)cy * mw],
+                                           &env->condCdf[(size_t)cy * (mw + 1)]);
+    env->margPdf.resize(mh);
+    env->margCdf.resize(mh + 1);
+    buildDist1D(env->rowIntegral.data(), mh, env->margPdf.data(), env->margCdf.data());
 }
 
 // DiscretizedSpectrum::initialize (libSLR/BasicTypes/spectrum_types.cpp:945-990):

@@ -267,6 +267,76 @@ def cornell_icosphere(width=1920, height=1080, subdiv=8, seed=7):
     return desc
 
 
+def sky_environment(width=256, height=128, sun_dir=(0.4, 0.8, 0.45), sun_radiance=1e4, sun_radius_deg=1.5,
+                    zenith=(0.25, 0.45, 1.0), horizon=(1.0, 0.95, 0.85), ground=(0.2, 0.18, 0.15)):
+    """Equirectangular sky: a zenith-to-horizon gradient, a darker ground and a
+    small sun disc of the given relative radiance. Rows run from the +y pole
+    (theta = 0) down; phi = 0 looks along +z (Vector3D::toPolarYUp)."""
+    theta = (np.arange(height) + 0.5) / height * np.pi
+    phi = (np.arange(width) + 0.5) / width * 2 * np.pi
+    th, ph = np.meshgrid(theta, phi, indexing="ij")
+    d = np.stack([-np.sin(ph) * np.sin(th), np.cos(th), np.cos(ph) * np.sin(th)], -1)
+    up = np.clip(d[..., 1], 0, 1)[..., None]
+    sky = np.array(horizon) * (1 - up) + np.array(zenith) * up
+    img = np.where(d[..., 1:2] >= 0, sky, np.array(ground))
+    s = np.asarray(sun_dir, np.float64)
+    s /= np.linalg.norm(s)
+    cosang = d @ s
+    img = img + (cosang > math.cos(math.radians(sun_radius_deg)))[..., None] * sun_radiance
+    return img.astype(np.float32)
+
+
+def constant_environment(value=1.0, width=32, height=16):
+    return np.full((height, width, 3), value, np.float32)
+
+
+def env_furnace(width=32, height=32, albedo=0.5, env_value=1.0, quad_in_view=True):
+    """A large Lambert quad of constant albedo under a uniform environment.
+    In view (quad_in_view) it fills the image and, with nothing shadowing its
+    hemisphere, reflects albedo times the environment's radiance; with the
+    quad behind the camera the image is the environment itself. The two
+    films therefore differ by the factor albedo."""
+    materials = [{"kind": "lambert", "const": albedo}]
+    z = 500.0 if quad_in_view else -500.0
+    n = (0, 0, -1) if quad_in_view else (0, 0, 1)
+    S = 5000.0
+    if quad_in_view:
+        pos, nrm, idx = _quad((-S, -S, z), (-S, S, z), (S, S, z), (S, -S, z), n)
+    else:
+        pos, nrm, idx = _quad((-S, -S, z), (S, -S, z), (S, S, z), (-S, S, z), n)
+    meshes = [_mesh(pos, nrm, idx, 0)]
+    l2w = np.eye(4)
+    camera = {"l2w": l2w, "aspect": width / height, "fov_y": 1.0, "lens_radius": 0.0, "img_dist": 1.0,
+              "obj_dist": 1.0, "sensitivity": 1.0, "width": width, "height": height}
+    return {"materials": materials, "meshes": meshes, "camera": camera,
+            "environment": {"rgb": constant_environment(env_value), "scale": 1.0}}
+
+
+def sky_scene(width=64, height=48, env_width=64, env_height=32, sun_radiance=50.0, sun_radius_deg=6.0):
+    """Open scene under the equirectangular sky of sky_environment: a ground
+    quad, a rotated box and a GGX conductor box, camera looking slightly
+    down. Every material is one the reference test harness also builds."""
+    materials = [{"kind": "lambert", "rgb": (0.6, 0.6, 0.55)},
+                 {"kind": "lambert", "rgb": (0.63, 0.06, 0.04)},
+                 {"kind": "ggx_conductor", "eta": 0.2, "k": 3.9, "alpha": 0.25}]
+    G = 2000.0
+    pos, nrm, idx = _quad((-G, 0, -G), (-G, 0, G), (G, 0, G), (G, 0, -G), (0, 1, 0))
+    meshes = [_mesh(pos, nrm, idx, 0)]
+    for (center, size, rot), m in ((((-120.0, 80.0, 300.0), (160, 160, 160), 25.0), 1),
+                                   (((150.0, 120.0, 420.0), (140, 240, 140), -15.0), 2)):
+        pos, nrm, idx = _box(center, size, rot)
+        meshes.append(_mesh(pos, nrm, idx, m))
+    l2w = np.eye(4)
+    t = math.radians(-12.0)   # pitched down about x
+    l2w[:3, :3] = np.array([[1, 0, 0], [0, math.cos(t), -math.sin(t)], [0, math.sin(t), math.cos(t)]])
+    l2w[:3, 3] = np.array([0.0, 220.0, -350.0]) * SCALE
+    camera = {"l2w": l2w, "aspect": width / height, "fov_y": 0.9, "lens_radius": 0.0, "img_dist": 1.0,
+              "obj_dist": 1.0, "sensitivity": 1.0, "width": width, "height": height}
+    env = sky_environment(env_width, env_height, sun_radiance=sun_radiance, sun_radius_deg=sun_radius_deg)
+    return {"materials": materials, "meshes": meshes, "camera": camera,
+            "environment": {"rgb": env, "scale": 1.0}}
+
+
 def camera_pose(desc, yaw_deg=0.0, dx=0.0, dy=0.0, width=None, height=None):
     """desc's camera turned by yaw_deg about the vertical axis through the
     scene's centre and shifted by (dx, dy) in its own plane; aspect from

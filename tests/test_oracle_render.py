@@ -51,3 +51,26 @@ def test_oracle_matches_reference_renderer():
     tr, to = rb.sum(axis=(1, 2, 3)), ob.sum(axis=(1, 2, 3))
     se_t = np.sqrt(tr.var(ddof=1) / B + to.var(ddof=1) / B)
     assert abs(to.mean() - tr.mean()) < 4 * se_t
+
+
+def test_reference_environment_closed_form():
+    """The harness's environment (RGBA16F image, InfiniteSphereNode) renders a
+    uniform white sky as luminance 1 per unit cos^4 weight when seen
+    directly, and albedo times that off a Lambert plane that fills the view.
+    The image is a whole number of the reference sensor's 8-pixel tiles: the
+    reference traces whole tiles and files the samples of pixels beyond the
+    image into the edge pixels (ImageSensor.cpp:125-130)."""
+    refdrv = need_ref()
+    W = H = 16
+    spp = 32
+    cmf = refdrv.cmf_strata()
+    out = []
+    for in_view in (False, True):
+        desc = scenes.env_furnace(W, H, quad_in_view=in_view)
+        desc["materials"] = [{"kind": "lambert", "rgb": (0.5, 0.5, 0.5)}]
+        r = refdrv.RefScene(desc)
+        f, _, _ = r.render(W, H, spp, 3, threads=max(1, min(8, len(os.sched_getaffinity(0)))))
+        r.close()
+        out.append(_xyz(f.astype(np.float64), cmf)[..., 1] / spp / scenes.pixel_cos4(desc, W, H))
+    assert abs(out[0].mean() - 1) < 0.01, out[0].mean()
+    assert abs(out[1].mean() - 0.5) < 0.01, out[1].mean()
