@@ -107,6 +107,43 @@ SLRH_API int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, con
                            const float* tan, uint32_t ntris, const uint32_t* idx,
                            const uint32_t* mat, int material, int* mesh_id);
 
+/* Adds a mesh that appears only through instances (a prototype), with the
+ * arguments of slrh_mesh_add, in object space. Replaces a TriangleMeshNode
+ * wrapped in a ReferenceNode (libSLR/Scene/node.h:68-82, node.cpp ReferenceNode::
+ * createRenderingData), which builds the mesh's own aggregate once and shares
+ * it between every place it is referenced from. */
+SLRH_API int slrh_prototype_add(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm,
+                                const float* tan, uint32_t ntris, const uint32_t* idx,
+                                const uint32_t* mat, int material, int* proto_id);
+/* Places a copy of prototype proto_id with the object-to-world affine
+ * transform l2w (12 floats: 3 rows of 4, row-major; the bottom row is
+ * 0 0 0 1). material >= 0 replaces every material of the prototype for this
+ * copy, -1 keeps them (an extension: the reference binds materials to the
+ * mesh). Replaces InternalNode(StaticTransform) over a ReferenceNode, which
+ * becomes a TransformedSurfaceObject (libSLR/Scene/node.cpp InternalNode::
+ * createRenderingData; libSLR/Core/surface_object.cpp:290-369): rays are
+ * taken into object space by the inverse transform, hits and light samples
+ * back by the transform. Instance ids count from 0 in call order. */
+SLRH_API int slrh_instance_add(slrh_scene* s, int proto_id, const float* l2w, int material, int* inst_id);
+
+/* The transform of instance inst_id as stored (12 floats each, row-major
+ * 3x4; either pointer may be NULL): the object-to-world rows given, and the
+ * world-to-object rows rays are transformed with, computed as the reference
+ * inverts a StaticTransform's matrix (Matrix4x4 invert,
+ * libSLR/BasicTypes/Matrix4x4.cpp:36-87). */
+SLRH_API int slrh_instance_get(const slrh_scene* s, int inst_id, float* l2w, float* w2l);
+
+enum slrh_compat_flags {
+    /* shadow rays test an instance with the untransformed world ray, as
+     * TransformedSurfaceObject::testVisibility does (libSLR/Core/
+     * surface_object.cpp:363-369); off: the ray is transformed as for
+     * closest hits */
+    SLRH_COMPAT_INSTANCE_SHADOW = 1
+};
+/* Reproduces reference behaviours that differ from the correct result, for
+ * parity checks (flags from slrh_compat_flags). Call before slrh_scene_build. */
+SLRH_API int slrh_scene_compat(slrh_scene* s, uint32_t flags);
+
 /* Camera-to-world 4x4 (column-major) and PerspectiveCamera parameters
  * (libSLR/Camera/PerspectiveCamera.cpp:16-27, libSLR/Scene/camera_nodes.h:16-35). */
 /* Image-based environment light: an equirectangular image of width*height
@@ -135,7 +172,9 @@ SLRH_API int slrh_scene_accel_stats(const slrh_scene* s, double* stats);
 /* Topology dumps in the same layout as the reference's node arrays (for
  * parity tests): SBVH boxes[6*n], info[5*n] {c0,c1,offset,numLeaves,axis|0xFF},
  * refs[leaf refs]; QBVH boxes[24*n], info[7*n] {children[4] in the reference's
- * bitfield encoding, top/left/right axis}, refs. Buffers may be NULL to query n. */
+ * bitfield encoding, top/left/right axis}, refs. Buffers may be NULL to query n.
+ * These are the world-level trees; a ref is a global triangle index, or
+ * INT32_MIN + i (bit 31 set) for instance i. */
 SLRH_API int slrh_scene_sbvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs,
                                   uint32_t* num_nodes, uint32_t* num_refs);
 SLRH_API int slrh_scene_qbvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs,
@@ -152,9 +191,12 @@ typedef struct slrh_rays {
 
 typedef struct slrh_hits {
     float* t;        /* hit distance, or tmax on a miss */
-    int32_t* prim;   /* global triangle index (mesh order, then triangle order), -1 on a miss */
+    int32_t* prim;   /* global triangle index (meshes and prototypes in the order they were
+                        added, then triangle order), -1 on a miss */
     float* b0;       /* surface parameters as SurfaceInteraction stores them: */
     float* b1;       /*   b0 = 1 - u - v (weight of vertex 0), b1 = u (weight of vertex 1) */
+    int32_t* inst;   /* instance of the hit triangle, -1 for world-space triangles and
+                        misses; may be NULL */
 } slrh_hits;
 
 enum slrh_mem { SLRH_MEM_HOST = 0, SLRH_MEM_DEVICE = 1 };

@@ -46,6 +46,16 @@
 
 #include <libSLR/BasicTypes/spectrum_base.h>
 #include <libSLR/BasicTypes/spectrum_types.h>
+#include <libSLR/BasicTypes/rgb_types.h>
+#include <libSLR/BasicTypes/TexCoord2D.h>
+#include <libSLR/Core/transform.h>
+// SurfaceInteraction keeps its hit triangle in a member declared before any
+// access label; spelling its classes as structs (same layout, same mangling)
+// makes it readable here. Its dependencies are included above, so only
+// geometry.h's own declarations see the change.
+#define class struct
+#include <libSLR/Core/geometry.h>
+#undef class
 #include <libSLR/MemoryAllocators/Allocator.h>
 #include <libSLR/MemoryAllocators/ArenaAllocator.h>
 #include <libSLR/Core/surface_object.h>
@@ -114,6 +124,16 @@ struct RefScene {
     std::vector<TriangleMeshNode*> meshes;
     std::vector<uint32_t> meshPrimBase;
     uint32_t numPrims = 0;
+    // prototypes (meshes placed only through instances) and instances
+    // (prototype, object-to-world rows)
+    std::vector<TriangleMeshNode*> protos;
+    std::vector<uint32_t> protoPrimBase;
+    struct Inst { int proto; float rows[12]; };
+    std::vector<Inst> insts;
+    std::vector<std::unique_ptr<SurfaceObjectAggregate>> protoAggs;
+    std::vector<std::unique_ptr<StaticTransform>> instTFs;
+    std::vector<std::unique_ptr<TransformedSurfaceObject>> instObjs;
+    std::map<const SurfaceObject*, int32_t> objToInst;
 
     // camera
     bool hasCamera = false;
@@ -139,11 +159,20 @@ struct RefScene {
         if (envImage) DefaultAllocator::instance().destroy(envImage);
         delete qbvh;
         delete sbvh;
-        if (built)
-            for (auto m : meshes)
-                m->destroyRenderingData(&DefaultAllocator::instance());
-        for (auto m : meshes)
-            delete m;
+        instObjs.clear();
+        protoAggs.clear();
+        if (built) {
+            for (auto m : meshes) m->destroyRenderingData(&DefaultAllocator::instance());
+            for (auto m : protos) m->destroyRenderingData(&DefaultAllocator::instance());
+        }
+        for (auto m : meshes) delete m;
+        for (auto m : protos) delete m;
+    }
+    // Matrix4x4 (column-major array) of instance k's object-to-world rows.
+    Matrix4x4 instMatrix(size_t k) const {
+        const float* r = insts[k].rows;
+        float m[16] = {r[0], r[4], r[8], 0, r[1], r[5], r[9], 0, r[2], r[6], r[10], 0, r[3], r[7], r[11], 1};
+        return Matrix4x4(m);
     }
 
     const SpectrumTexture* constTexRGB(SpectrumType type, float r, float g, float b) {
@@ -243,12 +272,11 @@ int refd_add_material_emitter_rgb(void* sp, int baseMat, float r, float g, float
     return s->addMaterial(new EmitterSurfaceMaterial(base, s->emitters.back().get()));
 }
 
-// One triangle mesh with one material group. Positions, normals and tangents
-// are world space (3 floats per vertex); uvs may be null.
-int refd_add_mesh(void* sp, int nv, const float* pos, const float* nrm, const float* tan,
-                  const float* uv, int nt, const uint32_t* idx, int material) {
-    RefScene* s = (RefScene*)sp;
-    if (s->built) return -1;
+}  // extern "C"
+
+namespace {
+TriangleMeshNode* makeMesh(RefScene* s, int nv, const float* pos, const float* nrm, const float* tan,
+                           const float* uv, int nt, const uint32_t* idx, int material) {
     TriangleMeshNode* m = new TriangleMeshNode((uint32_t)nv, 1, false, -1);
     Vertex* vs = m->getVertexArray();
     for (int i = 0; i < nv; ++i) {
@@ -262,10 +290,48 @@ int refd_add_mesh(void* sp, int nv, const float* pos, const float* nrm, const fl
     std::unique_ptr<Vertex*[]> refs(new Vertex*[3 * (size_t)nt]);
     for (int i = 0; i < 3 * nt; ++i) refs[i] = vs + idx[i];
     g.setTriangles(refs, (uint32_t)nt);
-    s->meshes.push_back(m);
+    return m;
+}
+}  // namespace
+
+extern "C" {
+
+// One triangle mesh with one material group. Positions, normals and tangents
+// are world space (3 floats per vertex); uvs may be null.
+int refd_add_mesh(void* sp, int nv, const float* pos, const float* nrm, const float* tan,
+                  const float* uv, int nt, const uint32_t* idx, int material) {
+    RefScene* s = (RefScene*)sp;
+    if (s->built) return -1;
+    s->meshes.push_back(makeMesh(s, nv, pos, nrm, tan, uv, nt, idx, material));
     s->meshPrimBase.push_back(s->numPrims);
     s->numPrims += (uint32_t)nt;
     return (int)s->meshes.size() - 1;
+}
+
+// A mesh in object space placed only through refd_add_instance (a
+// TriangleMeshNode under ReferenceNodes). Its triangles take the next prim
+// ids, as a mesh would.
+int refd_add_prototype(void* sp, int nv, const float* pos, const float* nrm, const float* tan,
+                       const float* uv, int nt, const uint32_t* idx, int material) {
+    RefScene* s = (RefScene*)sp;
+    if (s->built) return -1;
+    s->protos.push_back(makeMesh(s, nv, pos, nrm, tan, uv, nt, idx, material));
+    s->protoPrimBase.push_back(s->numPrims);
+    s->numPrims += (uint32_t)nt;
+    return (int)s->protos.size() - 1;
+}
+
+// Places prototype `proto` with the object-to-world affine rows (12 floats,
+// row-major 3x4): InternalNode(StaticTransform) over the prototype's
+// ReferenceNode when rendering, TransformedSurfaceObject for ray batches.
+int refd_add_instance(void* sp, int proto, const float* rows) {
+    RefScene* s = (RefScene*)sp;
+    if (s->built || proto < 0 || (size_t)proto >= s->protos.size()) return -1;
+    RefScene::Inst in;
+    in.proto = proto;
+    memcpy(in.rows, rows, sizeof(in.rows));
+    s->insts.push_back(in);
+    return (int)s->insts.size() - 1;
 }
 
 // Camera-to-world matrix, column-major 4x4 (Matrix4x4.h:42-46).
@@ -289,6 +355,29 @@ int refd_build_accel(void* sp, int quiet) {
         m->createRenderingData(&DefaultAllocator::instance(), nullptr, &rd);
     s->objs = rd.surfObjs;
     for (size_t i = 0; i < s->objs.size(); ++i) s->objToPrim[s->objs[i]] = (int32_t)i;
+    // prototypes: one aggregate each (ReferenceNode::createRenderingData),
+    // instances as TransformedSurfaceObjects after the world triangles
+    std::vector<SurfaceObject*> protoObj;
+    for (size_t k = 0; k < s->protos.size(); ++k) {
+        RenderingData pd(nullptr);
+        s->protos[k]->createRenderingData(&DefaultAllocator::instance(), nullptr, &pd);
+        for (size_t i = 0; i < pd.surfObjs.size(); ++i)
+            s->objToPrim[pd.surfObjs[i]] = (int32_t)(s->protoPrimBase[k] + i);
+        if (pd.surfObjs.size() > 1) {
+            s->protoAggs.emplace_back(new SurfaceObjectAggregate(pd.surfObjs));
+            protoObj.push_back(s->protoAggs.back().get());
+        } else {
+            protoObj.push_back(pd.surfObjs[0]);
+        }
+    }
+    for (size_t k = 0; k < s->insts.size(); ++k) {
+        s->instTFs.emplace_back(new StaticTransform(s->instMatrix(k)));
+        s->instObjs.emplace_back(new TransformedSurfaceObject(protoObj[s->insts[k].proto], s->instTFs.back().get()));
+        SurfaceObject* o = s->instObjs.back().get();
+        s->objs.push_back(o);
+        s->objToInst[o] = (int32_t)k;
+        s->objToPrim[o] = (int32_t)(0x80000000u | (uint32_t)k);
+    }
     s->sbvh = new SBVH(s->objs);
     s->qbvh = new QBVH(*s->sbvh);
     s->built = true;
@@ -344,10 +433,28 @@ int refd_qbvh_dump(void* sp, float* boxes, uint32_t* info, int32_t* refs) {
     return (int)n.size();
 }
 
+// World-to-object rows (12 floats, row-major 3x4) of instance k as
+// StaticTransform computes its inverse (libSLR/Core/transform.h:38).
+int refd_instance_inverse(void* sp, int k, float* rows) {
+    RefScene* s = (RefScene*)sp;
+    if (k < 0 || (size_t)k >= s->insts.size()) return -1;
+    const StaticTransform inv = invert(StaticTransform(s->instMatrix(k)));
+    // column c of the inverse is the inverse applied to the unit vector e_c
+    for (int c = 0; c < 4; ++c) {
+        const Vector4D col = inv * Vector4D(c == 0 ? 1.0f : 0.0f, c == 1 ? 1.0f : 0.0f, c == 2 ? 1.0f : 0.0f,
+                                            c == 3 ? 1.0f : 0.0f);
+        rows[c] = col.x;
+        rows[4 + c] = col.y;
+        rows[8 + c] = col.z;
+    }
+    return 0;
+}
+
 // Closest hit for n rays, each 8 floats {org xyz, dir xyz, tmin, tmax}.
-// accel: 0 = QBVH, 1 = SBVH. Miss: prim = -1, t = tmax.
+// accel: 0 = QBVH, 1 = SBVH. Miss: prim = -1, t = tmax. inst (may be null):
+// the instance of the hit triangle, -1 in world space.
 int refd_intersect(void* sp, int accel, int n, const float* rays, float* t, int32_t* prim,
-                   float* b0, float* b1) {
+                   float* b0, float* b1, int32_t* inst) {
     RefScene* s = (RefScene*)sp;
     IndependentLightPathSampler sampler(1u);
     for (int i = 0; i < n; ++i) {
@@ -360,12 +467,14 @@ int refd_intersect(void* sp, int accel, int n, const float* rays, float* t, int3
                              : s->sbvh->intersect(ray, seg, sampler, &si, &hit);
         if (ok) {
             t[i] = si.getDistance();
-            prim[i] = s->objToPrim.at(hit);
+            prim[i] = s->objToPrim.at(si.m_obj);
             float u, v;
             si.getSurfaceParameter(&u, &v);
             b0[i] = u; b1[i] = v;
+            if (inst) inst[i] = s->objToInst.count(hit) ? s->objToInst.at(hit) : -1;
         } else {
             t[i] = r[7]; prim[i] = -1; b0[i] = 0; b1[i] = 0;
+            if (inst) inst[i] = -1;
         }
     }
     return 0;
@@ -411,6 +520,18 @@ double refd_render(void* sp, int w, int h, int spp, uint32_t seed, int threads, 
 
     InternalNode root(new StaticTransform());
     for (auto m : s->meshes) root.addChildNode(m);
+    // instances: InternalNode(transform) over the prototype's ReferenceNode,
+    // one ReferenceNode per prototype shared by its instances
+    std::vector<std::unique_ptr<ReferenceNode>> refNodes;
+    for (auto m : s->protos) refNodes.emplace_back(new ReferenceNode(m));
+    std::vector<std::unique_ptr<StaticTransform>> instTF;
+    std::vector<std::unique_ptr<InternalNode>> instNodes;
+    for (size_t k = 0; k < s->insts.size(); ++k) {
+        instTF.emplace_back(new StaticTransform(s->instMatrix(k)));
+        instNodes.emplace_back(new InternalNode(instTF.back().get()));
+        instNodes.back()->addChildNode(refNodes[s->insts[k].proto].get());
+        root.addChildNode(instNodes.back().get());
+    }
     float mat[16];
     memcpy(mat, s->l2w, sizeof(mat));
     InternalNode camNode(new StaticTransform(Matrix4x4(mat)));

@@ -35,12 +35,15 @@ def lib():
             "refd_add_material_ggx_dielectric": (i32, [vp, f32, f32, f32, f32]),
             "refd_add_material_emitter_rgb": (i32, [vp, i32, f32, f32, f32]),
             "refd_add_mesh": (i32, [vp, i32, vp, vp, vp, vp, i32, vp, i32]),
+            "refd_add_prototype": (i32, [vp, i32, vp, vp, vp, vp, i32, vp, i32]),
+            "refd_add_instance": (i32, [vp, i32, vp]),
+            "refd_instance_inverse": (i32, [vp, i32, vp]),
             "refd_set_camera": (None, [vp, vp, f32, f32, f32, f32, f32, f32]),
             "refd_build_accel": (i32, [vp, i32]),
             "refd_accel_stats": (None, [vp, vp]),
             "refd_sbvh_dump": (i32, [vp, vp, vp, vp]),
             "refd_qbvh_dump": (i32, [vp, vp, vp, vp]),
-            "refd_intersect": (i32, [vp, i32, i32, vp, vp, vp, vp, vp]),
+            "refd_intersect": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp]),
             "refd_visibility": (i32, [vp, i32, i32, vp, vp]),
             "refd_set_environment": (i32, [vp, i32, i32, vp, f32]),
             "refd_render": (f64, [vp, i32, i32, i32, u32, i32, vp, i32, vp]),
@@ -90,16 +93,23 @@ class RefScene:
                 base = ids[m["base"]] if m.get("base", -1) >= 0 else -1
                 ids.append(L.refd_add_material_emitter_rgb(self.h, base, *m["rgb"]))
         self._keep = []
-        for mesh in desc["meshes"]:
-            mat = mesh["mat"]
-            if not np.isscalar(mat):
-                raise ValueError("reference harness meshes take one material each")
-            pos = np.ascontiguousarray(mesh["pos"], np.float32)
-            nrm = np.ascontiguousarray(mesh["nrm"], np.float32)
-            tan = np.ascontiguousarray(mesh["tan"], np.float32)
-            idx = np.ascontiguousarray(mesh["idx"], np.uint32)
-            self._keep += [pos, nrm, tan, idx]
-            L.refd_add_mesh(self.h, len(pos), _p(pos), _p(nrm), _p(tan), None, len(idx), _p(idx), ids[int(mat)])
+        for fn, meshes in ((L.refd_add_mesh, desc["meshes"]), (L.refd_add_prototype, desc.get("prototypes", []))):
+            for mesh in meshes:
+                mat = mesh["mat"]
+                if not np.isscalar(mat):
+                    raise ValueError("reference harness meshes take one material each")
+                pos = np.ascontiguousarray(mesh["pos"], np.float32)
+                nrm = np.ascontiguousarray(mesh["nrm"], np.float32)
+                tan = np.ascontiguousarray(mesh["tan"], np.float32)
+                idx = np.ascontiguousarray(mesh["idx"], np.uint32)
+                self._keep += [pos, nrm, tan, idx]
+                fn(self.h, len(pos), _p(pos), _p(nrm), _p(tan), None, len(idx), _p(idx), ids[int(mat)])
+        for inst in desc.get("instances", []):
+            if inst.get("mat", -1) >= 0:
+                raise ValueError("the reference binds materials to the mesh: no per-instance material")
+            rows = np.ascontiguousarray(np.asarray(inst["l2w"], np.float32).reshape(-1, 4)[:3])
+            if L.refd_add_instance(self.h, int(inst["proto"]), _p(rows)) < 0:
+                raise ValueError("bad instance")
         env = desc.get("environment")
         if env is not None:
             rgb = np.ascontiguousarray(env["rgb"], np.float32)
@@ -141,15 +151,22 @@ class RefScene:
         self.L.refd_qbvh_dump(self.h, _p(boxes), _p(info), _p(refs))
         return boxes, info, refs
 
-    def intersect(self, rays, accel=0):
+    def instance_inverse(self, k):
+        rows = np.zeros(12, np.float32)
+        if self.L.refd_instance_inverse(self.h, k, _p(rows)):
+            raise IndexError(k)
+        return rows.reshape(3, 4)
+
+    def intersect(self, rays, accel=0, with_inst=False):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         n = len(rays)
         t = np.zeros(n, np.float32)
         prim = np.zeros(n, np.int32)
         b0 = np.zeros(n, np.float32)
         b1 = np.zeros(n, np.float32)
-        self.L.refd_intersect(self.h, accel, n, _p(rays), _p(t), _p(prim), _p(b0), _p(b1))
-        return t, prim, b0, b1
+        inst = np.zeros(n, np.int32)
+        self.L.refd_intersect(self.h, accel, n, _p(rays), _p(t), _p(prim), _p(b0), _p(b1), _p(inst))
+        return (t, prim, b0, b1, inst) if with_inst else (t, prim, b0, b1)
 
     def visibility(self, rays, accel=0):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

@@ -32,7 +32,7 @@ EXPORTS = [
     "slrh_spectrum_rgb", "slrh_spectrum_regular", "slrh_spectrum_evaluate", "slrh_cmf_strata",
     "slrh_material_lambert", "slrh_material_oren_nayar", "slrh_material_ggx_conductor",
     "slrh_material_ggx_dielectric", "slrh_material_diffuse_emitter",
-    "slrh_mesh_add", "slrh_camera_perspective",
+    "slrh_mesh_add", "slrh_prototype_add", "slrh_instance_add", "slrh_instance_get", "slrh_scene_compat", "slrh_camera_perspective",
     "slrh_scene_create", "slrh_scene_build", "slrh_scene_destroy",
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
     "slrh_intersect", "slrh_occluded",
@@ -54,7 +54,7 @@ class _Rays(ctypes.Structure):
 
 
 class _Hits(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in ("t", "prim", "b0", "b1")]
+    _fields_ = [(n, ctypes.c_void_p) for n in ("t", "prim", "b0", "b1", "inst")]
 
 
 class _Frame(ctypes.Structure):
@@ -71,6 +71,7 @@ class _Request(ctypes.Structure):
     _fields_ = [("frame", _Frame), ("max_depth", ctypes.c_uint32), ("slots_per_pixel", ctypes.c_uint32)]
 
 
+COMPAT_INSTANCE_SHADOW = 1
 FLAG_TIME_KERNELS = 1
 FLAG_NO_GRAPH = 2
 
@@ -108,6 +109,10 @@ def load_library(path=None):
         "slrh_material_ggx_dielectric": (i32, [vp, i32, i32, f32, f32, pi]),
         "slrh_material_diffuse_emitter": (i32, [vp, i32, i32, pi]),
         "slrh_mesh_add": (i32, [vp, u32, vp, vp, vp, u32, vp, vp, i32, pi]),
+        "slrh_prototype_add": (i32, [vp, u32, vp, vp, vp, u32, vp, vp, i32, pi]),
+        "slrh_instance_add": (i32, [vp, i32, vp, i32, pi]),
+        "slrh_scene_compat": (i32, [vp, u32]),
+        "slrh_instance_get": (i32, [vp, i32, vp, vp]),
         "slrh_camera_perspective": (i32, [vp, vp, f32, f32, f32, f32, f32, f32]),
         "slrh_scene_create": (i32, [vp, pp]),
         "slrh_scene_build": (i32, [vp]),
@@ -243,6 +248,38 @@ class Scene:
         self.num_prims += len(idx)
         return mid
 
+    def add_prototype(self, pos, idx, mat, nrm=None, tan=None):
+        """A mesh in object space that appears only through add_instance."""
+        pos = _f32(pos).reshape(-1, 3)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
+        nrm = None if nrm is None else _f32(nrm).reshape(-1, 3)
+        tan = None if tan is None else _f32(tan).reshape(-1, 3)
+        if np.isscalar(mat):
+            matp, matv = None, int(mat)
+        else:
+            matp, matv = np.ascontiguousarray(mat, dtype=np.uint32), 0
+        pid = self._id(self.lib.slrh_prototype_add, "slrh_prototype_add", len(pos), _ptr(pos), _ptr(nrm), _ptr(tan),
+                       len(idx), _ptr(idx), _ptr(matp), matv)
+        self.num_prims += len(idx)
+        return pid
+
+    def add_instance(self, proto, l2w, material=-1):
+        """Places prototype `proto` with the object-to-world affine transform
+        l2w (3x4 or 4x4, row-major); material >= 0 replaces its materials."""
+        m = np.asarray(l2w, dtype=np.float64)
+        m = _f32(m.reshape(-1, 4)[:3].copy())
+        return self._id(self.lib.slrh_instance_add, "slrh_instance_add", int(proto), _ptr(m), int(material))
+
+    def instance_transform(self, inst):
+        """(object-to-world rows, world-to-object rows), each 3x4 float32."""
+        a = np.zeros(12, np.float32)
+        b = np.zeros(12, np.float32)
+        self.ctx.check(self.lib.slrh_instance_get(self.h, int(inst), _ptr(a), _ptr(b)), "slrh_instance_get")
+        return a.reshape(3, 4), b.reshape(3, 4)
+
+    def set_compat(self, flags):
+        self.ctx.check(self.lib.slrh_scene_compat(self.h, int(flags)), "slrh_scene_compat")
+
     def set_camera(self, l2w, aspect, fov_y, lens_radius=0.0, img_dist=1.0, obj_dist=1.0, sensitivity=1.0):
         m = _f32(np.asarray(l2w, dtype=np.float32).reshape(4, 4).T.copy())  # column-major
         self.ctx.check(self.lib.slrh_camera_perspective(self.h, _ptr(m), float(sensitivity), float(aspect),
@@ -280,9 +317,9 @@ class Scene:
     def qbvh_dump(self):
         return self._dump(self.lib.slrh_scene_qbvh_dump, 24, 7, "slrh_scene_qbvh_dump")
 
-    def intersect(self, rays):
+    def intersect(self, rays, with_inst=False):
         """rays: (n, 8) float32 {org xyz, dir xyz, tmin, tmax} on the host.
-        Returns (t, prim, b0, b1)."""
+        Returns (t, prim, b0, b1), and inst (-1: world space) with with_inst."""
         rays = _f32(rays).reshape(-1, 8)
         n = len(rays)
         cols = [np.ascontiguousarray(rays[:, k]) for k in range(8)]
@@ -291,15 +328,18 @@ class Scene:
         prim = np.zeros(n, np.int32)
         b0 = np.zeros(n, np.float32)
         b1 = np.zeros(n, np.float32)
-        h = _Hits(t.ctypes.data, prim.ctypes.data, b0.ctypes.data, b1.ctypes.data)
+        inst = np.zeros(n, np.int32) if with_inst else None
+        h = _Hits(t.ctypes.data, prim.ctypes.data, b0.ctypes.data, b1.ctypes.data,
+                  inst.ctypes.data if with_inst else None)
         self.ctx.check(self.lib.slrh_intersect(self.h, ctypes.byref(r), ctypes.byref(h), n, MEM_HOST, None),
                        "slrh_intersect")
-        return t, prim, b0, b1
+        return (t, prim, b0, b1, inst) if with_inst else (t, prim, b0, b1)
 
     def intersect_device(self, cols, outs, n, stream=None):
-        """Device-pointer form: cols = 8 device addresses, outs = 4 device addresses."""
+        """Device-pointer form: cols = 8 device addresses, outs = 4 (or 5, with
+        instance ids) device addresses."""
         r = _Rays(*cols)
-        h = _Hits(*outs)
+        h = _Hits(*(list(outs) + [None] * (5 - len(outs))))
         self.ctx.check(self.lib.slrh_intersect(self.h, ctypes.byref(r), ctypes.byref(h), n, MEM_DEVICE, stream),
                        "slrh_intersect")
 
@@ -525,6 +565,15 @@ def build_scene(ctx, desc):
         mat = mesh["mat"]
         mat = mat_ids[mat] if np.isscalar(mat) else np.asarray(mat_ids, np.uint32)[np.asarray(mat)]
         sc.add_mesh(mesh["pos"], mesh["idx"], mat, mesh.get("nrm"), mesh.get("tan"))
+    for mesh in desc.get("prototypes", []):
+        mat = mesh["mat"]
+        mat = mat_ids[mat] if np.isscalar(mat) else np.asarray(mat_ids, np.uint32)[np.asarray(mat)]
+        sc.add_prototype(mesh["pos"], mesh["idx"], mat, mesh.get("nrm"), mesh.get("tan"))
+    for inst in desc.get("instances", []):
+        m = inst.get("mat", -1)
+        sc.add_instance(inst["proto"], inst["l2w"], mat_ids[m] if m >= 0 else -1)
+    if desc.get("compat_instance_shadow"):
+        sc.set_compat(COMPAT_INSTANCE_SHADOW)
     cam = desc["camera"]
     sc.set_camera(cam["l2w"], cam["aspect"], cam["fov_y"], cam.get("lens_radius", 0.0), cam.get("img_dist", 1.0),
                   cam.get("obj_dist", 1.0), cam.get("sensitivity", 1.0))

@@ -109,6 +109,27 @@ void splitBounds(const Tri& tri, int axis, float pos, Box* left, Box* right) {
     for (uint32_t i = 0; i < n; ++i) { left->unify(isect[i]); right->unify(isect[i]); }
 }
 
+// Object::choppedBounds / splitBounds (libSLR/Core/object.h:50-80): the
+// default for objects that are not triangles, here the world bounds of an
+// instance.
+Box choppedBoxBounds(const Box& b, int axis, float minChop, float maxChop) {
+    if (maxChop < b.mn[axis]) return Box();
+    if (minChop > b.mx[axis]) return Box();
+    if (minChop < b.mn[axis] && maxChop > b.mx[axis]) return b;
+    Box r = b;
+    r.mn[axis] = std::max(minChop, r.mn[axis]);
+    r.mx[axis] = std::min(maxChop, r.mx[axis]);
+    return r;
+}
+void splitBoxBounds(const Box& b, int axis, float pos, Box* left, Box* right) {
+    if (pos < b.mn[axis]) { *left = Box(); *right = b; return; }
+    if (pos > b.mx[axis]) { *left = b; *right = Box(); return; }
+    *left = b;
+    left->mx[axis] = std::min(left->mx[axis], pos);
+    *right = b;
+    right->mn[axis] = std::max(right->mn[axis], pos);
+}
+
 // A subtree in DFS pre-order with local indices.
 struct Subtree {
     std::vector<SbvhNode> nodes;
@@ -119,13 +140,28 @@ struct Subtree {
 
 class Builder {
 public:
-    Builder(const std::vector<Tri>& tris, float rootArea, int parallelDepth)
-        : m_tris(tris), m_rootArea(rootArea), m_parDepth(parallelDepth) {}
+    Builder(const std::vector<Tri>& tris, const std::vector<Box>& boxes, const std::vector<float>& boxCosts,
+            float rootArea, int parallelDepth)
+        : m_tris(tris), m_boxes(boxes), m_boxCosts(boxCosts), m_rootArea(rootArea), m_parDepth(parallelDepth) {}
 
     void build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out);
 
 private:
+    // primitives 0..nt-1 are triangles, nt.. the boxes
+    Box chopped(uint32_t prim, int axis, float lo, float hi) const {
+        return prim < m_tris.size() ? choppedBounds(m_tris[prim], axis, lo, hi)
+                                    : choppedBoxBounds(m_boxes[prim - m_tris.size()], axis, lo, hi);
+    }
+    void split(uint32_t prim, int axis, float pos, Box* l, Box* r) const {
+        if (prim < m_tris.size()) splitBounds(m_tris[prim], axis, pos, l, r);
+        else splitBoxBounds(m_boxes[prim - m_tris.size()], axis, pos, l, r);
+    }
+    // costForIntersect: 1 for a triangle (TriangleSurfaceShape.h:25), the
+    // prototype tree's SAH cost for an instance (SBVH::costForIntersect)
+    float cost(uint32_t prim) const { return prim < m_tris.size() ? 1.0f : m_boxCosts[prim - m_tris.size()]; }
     const std::vector<Tri>& m_tris;
+    const std::vector<Box>& m_boxes;
+    const std::vector<float>& m_boxCosts;
     float m_rootArea;
     int m_parDepth;
 
@@ -181,7 +217,7 @@ void Builder::build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out) {
     for (const Frag& f : frags) {
         parent.unify(f.box);
         centroids.unify(f.box.centroid());
-        leafCost += 1.0f;
+        leafCost += cost(f.prim);
     }
     const int axisOP = centroids.widestAxis();
     const int axisSP = parent.widestAxis();
@@ -213,7 +249,7 @@ void Builder::build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out) {
             uint32_t b = (uint32_t)(kObjBins * ((f.box.center(axisOP) - pcMin) / (pcMax - pcMin)));
             b = std::min(b, kObjBins - 1);
             ++ob[b].n;
-            ob[b].cost += 1.0f;
+            ob[b].cost += cost(f.prim);
             ob[b].box.unify(f.box);
         }
         for (uint32_t i = 0; i < kObjBins - 1; ++i) {
@@ -247,11 +283,11 @@ void Builder::build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out) {
             xb = std::min(xb, kSpBins - 1);
             ++sb[eb].nIn;
             ++sb[xb].nOut;
-            sb[eb].cIn += 1.0f;
-            sb[xb].cOut += 1.0f;
+            sb[eb].cIn += cost(f.prim);
+            sb[xb].cOut += cost(f.prim);
             for (int b = (int)eb; b <= (int)xb; ++b) {
                 float lo = b * binWidth + pMin;
-                Box c = choppedBounds(m_tris[f.prim], axisSP, lo, lo + binWidth);
+                Box c = chopped(f.prim, axisSP, lo, lo + binWidth);
                 sb[b].box.unify(intersect(c, f.box));
             }
         }
@@ -298,7 +334,7 @@ void Builder::build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out) {
             right.push_back(f);
         } else {
             Box l, r;
-            splitBounds(m_tris[f.prim], axisSP, splitPos, &l, &r);
+            split(f.prim, axisSP, splitPos, &l, &r);
             if (l.valid()) left.push_back(Frag{f.prim, intersect(l, f.box)});
             if (r.valid()) right.push_back(Frag{f.prim, intersect(r, f.box)});
         }
@@ -308,8 +344,10 @@ void Builder::build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out) {
     recurse(std::move(left), std::move(right), depth, out, nodeIdx, parent, axisSP);
 }
 
-// SBVH::calcSAHCost (SBVH.h:351-377).
-float sbvhCost(const std::vector<SbvhNode>& nodes) {
+// SBVH::calcSAHCost (SBVH.h:351-377); primCost(ref) is the referenced
+// primitive's intersection cost.
+template <typename PrimCost>
+float sbvhCost(const std::vector<SbvhNode>& nodes, const std::vector<uint32_t>& refs, PrimCost&& primCost) {
     const float Ci = 1.2f, Cl = 0.0f;
     float costInt = 0.0f, costLeaf = 0.0f, costObj = 0.0f;
     for (const SbvhNode& n : nodes) {
@@ -319,7 +357,7 @@ float sbvhCost(const std::vector<SbvhNode>& nodes) {
         } else {
             costLeaf += a;
             float prims = 0.0f;
-            for (uint32_t j = 0; j < n.count; ++j) prims += 1.0f;
+            for (uint32_t j = 0; j < n.count; ++j) prims += primCost(refs[n.first + j]);
             costObj += a * prims;
         }
     }
@@ -398,7 +436,8 @@ private:
 };
 
 // QBVH::calcSAHCost (QBVH.h:203-249).
-float qbvhCost(const Accel& a) {
+template <typename PrimCost>
+float qbvhCost(const Accel& a, PrimCost&& primCost) {
     const float Ci = 1.2f;
     float costInt = 0.0f, costObj = 0.0f;
     for (const QbvhNode& n : a.qbvh) {
@@ -413,7 +452,7 @@ float qbvhCost(const Accel& a) {
         for (int c = 0; c < 4; ++c) {
             if (!n.childValid[c] || !n.isLeaf[c]) continue;
             float prims = 0.0f;
-            for (uint32_t j = 0; j < n.count[c]; ++j) prims += 1.0f;
+            for (uint32_t j = 0; j < n.count[c]; ++j) prims += primCost(a.qbvhRefs[n.index[c] + j]);
             costObj += cb[c].area() * prims;
         }
     }
@@ -426,15 +465,20 @@ float qbvhCost(const Accel& a) {
 }  // namespace
 
 void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads) {
+    buildAccel(triPos, std::vector<Box>(), std::vector<float>(), out, numThreads);
+}
+
+void buildAccel(const std::vector<float>& triPos, const std::vector<Box>& boxes, const std::vector<float>& boxCosts,
+                Accel* out, int numThreads) {
     auto t0 = std::chrono::steady_clock::now();
-    const size_t nt = triPos.size() / 9;
+    const size_t nt = triPos.size() / 9, np = nt + boxes.size();
     std::vector<Tri> tris(nt);
     for (size_t i = 0; i < nt; ++i)
         for (int k = 0; k < 3; ++k)
             tris[i].p[k] = v3(triPos[9 * i + 3 * k], triPos[9 * i + 3 * k + 1], triPos[9 * i + 3 * k + 2]);
 
     *out = Accel();
-    if (nt == 0) {
+    if (np == 0) {
         // SBVH(objs) with no objects: one empty leaf (SBVH.h:386-393).
         out->sbvh.emplace_back();
         out->sbvhDepth = 1;
@@ -447,23 +491,24 @@ void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads) {
         return;
     }
 
-    std::vector<Frag> frags(nt);
-    for (size_t i = 0; i < nt; ++i) {
+    std::vector<Frag> frags(np);
+    for (size_t i = 0; i < np; ++i) {
         frags[i].prim = (uint32_t)i;
-        frags[i].box = triBounds(tris[i]);
+        frags[i].box = i < nt ? triBounds(tris[i]) : boxes[i - nt];
         out->bounds.unify(frags[i].box);
     }
     int parDepth = 0;
     for (int t = std::max(1, numThreads); t > 1; t >>= 1) ++parDepth;
-    Builder b(tris, out->bounds.area(), parDepth + 1);
+    Builder b(tris, boxes, boxCosts, out->bounds.area(), parDepth + 1);
+    auto primCost = [&](uint32_t prim) { return prim < nt ? 1.0f : boxCosts[prim - nt]; };
     Subtree st;
-    st.nodes.reserve(2 * nt);
+    st.nodes.reserve(2 * np);
     b.build(std::move(frags), 0, &st);
     out->sbvh = std::move(st.nodes);
     out->sbvhRefs = std::move(st.refs);
     out->sbvhDepth = st.depth;
     out->numFragments = st.fragments;
-    out->sbvhCost = sbvhCost(out->sbvh);
+    out->sbvhCost = sbvhCost(out->sbvh, out->sbvhRefs, primCost);
 
     Collapser c(*out, out);
     Collapser::Res r = c.collapse(0, 0);
@@ -479,7 +524,7 @@ void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads) {
         q.childValid[0] = true; q.isLeaf[0] = true; q.index[0] = r.idx; q.count[0] = r.count;
         q.topAxis = q.leftAxis = q.rightAxis = 0;
     }
-    out->qbvhCost = qbvhCost(*out);
+    out->qbvhCost = qbvhCost(*out, primCost);
     out->buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

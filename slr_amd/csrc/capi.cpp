@@ -223,9 +223,13 @@ int slrh_material_diffuse_emitter(slrh_scene* s, int base, int emittance, int* i
     return addMaterial(s, m, id);
 }
 
-int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm, const float* tan,
-                  uint32_t ntris, const uint32_t* idx, const uint32_t* mat, int material, int* mesh_id) {
-    CHECK_SCENE(s);
+}  // extern "C"
+
+namespace {
+// Appends a mesh's vertices and triangles to the scene's global arrays:
+// in world space (proto < 0) or as prototype `proto` in object space.
+int addMesh(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm, const float* tan, uint32_t ntris,
+            const uint32_t* idx, const uint32_t* mat, int material, int32_t proto) {
     if (!pos || !idx || nverts == 0 || ntris == 0) return fail(s->ctx, SLRH_ERR_INVALID, "mesh_add: empty mesh");
     const uint32_t nm = (uint32_t)s->data.materials.size();
     if (!mat && (material < 0 || (uint32_t)material >= nm)) return fail(s->ctx, SLRH_ERR_INVALID, "mesh_add: bad material");
@@ -275,8 +279,135 @@ int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, const float*
     }
     for (uint64_t i = 0; i < 3ull * ntris; ++i) d.idx.push_back(base + idx[i]);
     for (uint32_t i = 0; i < ntris; ++i) d.triMat.push_back(mat ? mat[i] : (uint32_t)material);
+    d.triProto.insert(d.triProto.end(), ntris, proto);
+    return SLRH_OK;
+}
+
+// StaticTransform * Point3D for an affine matrix (Matrix4x4::operator*,
+// libSLR/BasicTypes/Matrix4x4.h:68-74): each row dotted with (p, 1) in order.
+V3 affinePoint(const float* m, const V3& p) {
+    float o[3];
+    for (int r = 0; r < 3; ++r) o[r] = m[4 * r] * p.x + m[4 * r + 1] * p.y + m[4 * r + 2] * p.z + m[4 * r + 3] * 1.0f;
+    return v3(o[0], o[1], o[2]);
+}
+// Matrix4x4 invert (libSLR/BasicTypes/Matrix4x4.cpp:36-87) restated in
+// FP32 step by step: Gauss-Jordan elimination with full pivoting over the
+// columns not yet done, then the recorded row swaps undone as column swaps
+// in reverse order. The instance's world-to-object rows are this inverse,
+// so rays reach object space with exactly the reference's operands.
+// m, out: [column][row]. Returns false for a zero pivot.
+bool invertMatrix(const float m[4][4], float out[4][4]) {
+    float a[4][4];
+    memcpy(a, m, sizeof(a));
+    bool colDone[4] = {false, false, false, false};
+    int swapA[4] = {0, 0, 0, 0}, swapB[4] = {0, 0, 0, 0};
+    for (int pass = 0; pass < 4; ++pass) {
+        int pvCol = 0, pvRow = 0;
+        float maxPivot = -1.0f;
+        for (int c = 0; c < 4; ++c) {
+            if (colDone[c]) continue;
+            for (int r = 0; r < 4; ++r) {
+                if (colDone[r]) continue;
+                const float v = std::fabs(a[c][r]);
+                if (v > maxPivot) { pvCol = c; pvRow = r; maxPivot = v; }
+            }
+        }
+        if (pvRow != pvCol)
+            for (int k = 0; k < 4; ++k) std::swap(a[k][pvRow], a[k][pvCol]);
+        swapA[pass] = pvRow;
+        swapB[pass] = pvCol;
+        const float pivot = a[pvCol][pvCol];
+        if (pivot == 0.0f) return false;
+        a[pvCol][pvCol] = 1.0f;
+        const float scale = 1.0f / pivot;
+        for (int k = 0; k < 4; ++k) a[k][pvCol] *= scale;
+        const float addend[4] = {a[0][pvCol], a[1][pvCol], a[2][pvCol], a[3][pvCol]};
+        for (int r = 0; r < 4; ++r) {
+            if (r == pvCol) continue;
+            const float sv = a[pvCol][r];
+            a[pvCol][r] = 0.0f;
+            for (int k = 0; k < 4; ++k) a[k][r] += -sv * addend[k];
+        }
+        colDone[pvCol] = true;
+    }
+    for (int pass = 3; pass >= 0; --pass)
+        if (swapA[pass] != swapB[pass])
+            for (int k = 0; k < 4; ++k) std::swap(a[swapA[pass]][k], a[swapB[pass]][k]);
+    memcpy(out, a, sizeof(a));
+    return true;
+}
+}  // namespace
+
+extern "C" {
+
+int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm, const float* tan,
+                  uint32_t ntris, const uint32_t* idx, const uint32_t* mat, int material, int* mesh_id) {
+    CHECK_SCENE(s);
+    int r = addMesh(s, nverts, pos, nrm, tan, ntris, idx, mat, material, -1);
+    if (r) return r;
     if (mesh_id) *mesh_id = (int)s->numMeshes;
     ++s->numMeshes;
+    return SLRH_OK;
+}
+
+int slrh_prototype_add(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm, const float* tan,
+                       uint32_t ntris, const uint32_t* idx, const uint32_t* mat, int material, int* proto_id) {
+    CHECK_SCENE(s);
+    SceneData& d = s->data;
+    if (d.protos.size() >= (1u << 20)) return fail(s->ctx, SLRH_ERR_CAPACITY, "prototype_add: too many prototypes");
+    Prototype p;
+    p.triFirst = (uint32_t)d.triMat.size();
+    p.triCount = ntris;
+    int r = addMesh(s, nverts, pos, nrm, tan, ntris, idx, mat, material, (int32_t)d.protos.size());
+    if (r) return r;
+    d.protos.push_back(std::move(p));
+    if (proto_id) *proto_id = (int)d.protos.size() - 1;
+    return SLRH_OK;
+}
+
+int slrh_instance_add(slrh_scene* s, int proto_id, const float* l2w, int material, int* inst_id) {
+    CHECK_SCENE(s);
+    SceneData& d = s->data;
+    if (proto_id < 0 || (size_t)proto_id >= d.protos.size()) return fail(s->ctx, SLRH_ERR_INVALID, "instance_add: bad prototype");
+    if (!l2w) return fail(s->ctx, SLRH_ERR_INVALID, "instance_add: null transform");
+    if (material < -1 || (material >= 0 && (size_t)material >= d.materials.size()))
+        return fail(s->ctx, SLRH_ERR_INVALID, "instance_add: bad material");
+    if (d.instances.size() >= (1u << 24)) return fail(s->ctx, SLRH_ERR_CAPACITY, "instance_add: too many instances");
+    Instance in;
+    in.proto = (uint32_t)proto_id;
+    in.material = material;
+    float m[4][4], inv[4][4];   // [column][row]
+    for (int k = 0; k < 12; ++k) {
+        if (!std::isfinite(l2w[k])) return fail(s->ctx, SLRH_ERR_INVALID, "instance_add: non-finite transform");
+        in.l2w[k] = l2w[k];
+        m[k % 4][k / 4] = l2w[k];
+    }
+    m[0][3] = m[1][3] = m[2][3] = 0.0f;
+    m[3][3] = 1.0f;
+    if (!invertMatrix(m, inv)) return fail(s->ctx, SLRH_ERR_INVALID, "instance_add: singular transform");
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) {
+            if (!std::isfinite(inv[c][r])) return fail(s->ctx, SLRH_ERR_INVALID, "instance_add: singular transform");
+            in.w2l[4 * r + c] = inv[c][r];
+        }
+    d.instances.push_back(in);
+    if (inst_id) *inst_id = (int)d.instances.size() - 1;
+    return SLRH_OK;
+}
+
+int slrh_instance_get(const slrh_scene* s, int inst_id, float* l2w, float* w2l) {
+    if (!s) return SLRH_ERR_INVALID;
+    if (inst_id < 0 || (size_t)inst_id >= s->data.instances.size()) return SLRH_ERR_INVALID;
+    const Instance& in = s->data.instances[inst_id];
+    if (l2w) memcpy(l2w, in.l2w, sizeof(in.l2w));
+    if (w2l) memcpy(w2l, in.w2l, sizeof(in.w2l));
+    return SLRH_OK;
+}
+
+int slrh_scene_compat(slrh_scene* s, uint32_t flags) {
+    CHECK_SCENE(s);
+    if (flags & ~(uint32_t)SLRH_COMPAT_INSTANCE_SHADOW) return fail(s->ctx, SLRH_ERR_INVALID, "scene_compat: unknown flag");
+    s->data.compatInstanceShadow = (flags & SLRH_COMPAT_INSTANCE_SHADOW) != 0;
     return SLRH_OK;
 }
 
@@ -305,13 +436,49 @@ int slrh_scene_build(slrh_scene* s) {
     CHECK_SCENE(s);
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     try {
-        const size_t nt = s->data.idx.size() / 3;
-        std::vector<float> tp(9 * nt);
-        for (size_t t = 0; t < nt; ++t)
-            for (int k = 0; k < 3; ++k)
-                for (int j = 0; j < 3; ++j) tp[9 * t + 3 * k + j] = s->data.vpos[3ull * s->data.idx[3 * t + k] + j];
+        SceneData& d = s->data;
+        const size_t nt = d.idx.size() / 3;
+        auto positions = [&](const std::vector<uint32_t>& ids) {
+            std::vector<float> tp(9 * ids.size());
+            for (size_t i = 0; i < ids.size(); ++i)
+                for (int k = 0; k < 3; ++k)
+                    for (int j = 0; j < 3; ++j) tp[9 * i + 3 * k + j] = d.vpos[3ull * d.idx[3 * (size_t)ids[i] + k] + j];
+            return tp;
+        };
         int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        buildAccel(tp, &s->data.accel, threads);
+        // one accelerator per prototype, over its object-space triangles
+        for (Prototype& p : d.protos) {
+            std::vector<uint32_t> ids(p.triCount);
+            for (uint32_t i = 0; i < p.triCount; ++i) ids[i] = p.triFirst + i;
+            buildAccel(positions(ids), &p.accel, threads);
+            for (uint32_t& r : p.accel.sbvhRefs) r = ids[r];
+            for (uint32_t& r : p.accel.qbvhRefs) r = ids[r];
+        }
+        // the world level over world-space triangles and the instances'
+        // world bounds (TransformedSurfaceObject::bounds, surface_object.cpp:290-292:
+        // the eight transformed corners of the prototype's bounds)
+        std::vector<uint32_t> world;
+        for (size_t t = 0; t < nt; ++t)
+            if (d.triProto[t] < 0) world.push_back((uint32_t)t);
+        std::vector<Box> boxes;
+        std::vector<float> costs;
+        for (const Instance& in : d.instances) {
+            // a one-triangle prototype is referenced as the triangle itself
+            // (ReferenceNode::createRenderingData), otherwise as its aggregate
+            const Prototype& pr = d.protos[in.proto];
+            costs.push_back(pr.triCount > 1 ? pr.accel.sbvhCost : 1.0f);
+            const Box& b = d.protos[in.proto].accel.bounds;
+            Box w;
+            for (int c = 0; c < 8; ++c)
+                w.unify(affinePoint(in.l2w, v3((c & 4) ? b.mx[0] : b.mn[0], (c & 2) ? b.mx[1] : b.mn[1],
+                                               (c & 1) ? b.mx[2] : b.mn[2])));
+            boxes.push_back(w);
+        }
+        buildAccel(positions(world), boxes, costs, &d.accel, threads);
+        const uint32_t nw = (uint32_t)world.size();
+        auto code = [&](uint32_t r) { return r < nw ? world[r] : (kRefInstance | (r - nw)); };
+        for (uint32_t& r : d.accel.sbvhRefs) r = code(r);
+        for (uint32_t& r : d.accel.qbvhRefs) r = code(r);
         if (s->ctx->device >= 0) {
             std::string err;
             int r = uploadScene(s->data, &s->dev, &err);
@@ -397,7 +564,7 @@ static int doTrace(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, 
     if (!a.ox || !a.oy || !a.oz || !a.dx || !a.dy || !a.dz || !a.tmin || !a.tmax)
         return fail(s->ctx, SLRH_ERR_INVALID, "null ray array");
     if (hits) {
-        a.t = hits->t; a.prim = hits->prim; a.b0 = hits->b0; a.b1 = hits->b1;
+        a.t = hits->t; a.prim = hits->prim; a.b0 = hits->b0; a.b1 = hits->b1; a.inst = hits->inst;
         if (!a.t || !a.prim || !a.b0 || !a.b1) return fail(s->ctx, SLRH_ERR_INVALID, "null hit array");
     } else {
         a.vis = vis;

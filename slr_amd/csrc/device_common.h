@@ -13,10 +13,14 @@
 //                 left and right split axes. Bit 31 set = leaf with count in
 //                 bits 24-28 and first triangle reference in bits 0-23 (count
 //                 0: no child); otherwise an internal node index (bits 0-24).
-//   triangle ref: 3 x float4 = 48 B {v0.xyz, prim id}, {v1-v0}, {v2-v0}; one
-//                 per leaf reference, in leaf order, so a leaf is contiguous.
-//                 The edges are the same FP32 differences the reference forms
-//                 at each test (TriangleSurfaceShape.cpp:212-213).
+//   triangle ref: 3 x float4 = 48 B {v0.xyz, prim id}, {v1-v0, class},
+//                 {v2-v0, 0}; one per leaf reference, in leaf order, so a leaf
+//                 is contiguous. The edges are the same FP32 differences the
+//                 reference forms at each test (TriangleSurfaceShape.cpp:212-213).
+//   instance ref: the same 48 B with w words {0x80000000 | instance id},
+//                 {root node of the prototype's tree}, {class override}.
+//                 With instances the node and reference arrays hold the
+//                 world tree, then each prototype's tree.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -77,6 +81,8 @@ struct DevCamera {
     uint32_t width, height;
 };
 
+constexpr uint32_t kNoInstance = 0xFFFFFFFFu;
+
 struct DevScene {
     const float4* nodes;
     const float4* tris;
@@ -87,7 +93,7 @@ struct DevScene {
     const DevMaterial* mats;
     const DevSpectrum* spectra;
     const float* specPool;
-    const uint32_t* lights;   // emitting prim ids
+    const uint2* lights;      // emitting triangles: global id, instance (kNoInstance: world space)
     const float* lightCdf;    // numLights + 1 entries
     const float* lightPmf;
     const int32_t* primToLight;
@@ -95,6 +101,12 @@ struct DevScene {
     uint32_t numPrims;
     float lightImportance;    // SurfaceObjectAggregate::importance()
     uint32_t numNodes;
+    // instances: per instance 3 rows object-to-world, then 3 rows
+    // world-to-object (affine, row-major, translation in w)
+    const float4* instXf;
+    const int32_t* instMat;      // material replacing the prototype's, -1: none
+    uint32_t numInstances;
+    uint32_t compatInstShadow;   // shadow rays test instances with the world ray
     DevCamera cam;
     // environment light (image-based); hasEnv = 0 without one
     const float4* envTexel;   // per texel: Meng basis indices (bits), s, t, scale

@@ -16,16 +16,44 @@ struct CameraParams {
     float sensitivity = 1, aspect = 1, fovY = 0.5f, lensRadius = 0, imgDist = 1, objDist = 1;
 };
 
+// A mesh placed only through instances: its triangles (a range of the
+// global arrays, in object space) and its own accelerator over them, whose
+// references are global triangle ids.
+struct Prototype {
+    uint32_t triFirst = 0, triCount = 0;
+    Accel accel;
+};
+
+// A placed copy of a prototype: object-to-world affine transform (3 rows of
+// 4, row-major), its inverse, and a material replacing the prototype's
+// materials (-1: keep them).
+struct Instance {
+    uint32_t proto = 0;
+    float l2w[12], w2l[12];
+    int32_t material = -1;
+};
+
+// Accelerator references and hit ids: a world-space triangle by its global
+// id, an instance by kRefInstance | instance id.
+constexpr uint32_t kRefInstance = 0x80000000u;
+
 struct SceneData {
-    std::vector<float> vpos, vnrm, vtan;   // 3 floats per vertex, world space
+    std::vector<float> vpos, vnrm, vtan;   // 3 floats per vertex (object space for prototype triangles)
     std::vector<uint32_t> idx;             // 3 per triangle (global vertex indices)
     std::vector<uint32_t> triMat;          // material per triangle
+    std::vector<int32_t> triProto;         // per triangle: its prototype, or -1 in world space
     std::vector<HostSpectrum> spectra;
     std::vector<DevMaterial> materials;
     bool hasCamera = false;
     CameraParams camera;
     EnvData env;
-    Accel accel;
+    std::vector<Prototype> protos;
+    std::vector<Instance> instances;
+    // the reference's instance shadow test: the world ray against the
+    // object-space prototype (TransformedSurfaceObject::testVisibility,
+    // libSLR/Core/surface_object.cpp:363-369)
+    bool compatInstanceShadow = false;
+    Accel accel;   // world level: world-space triangles and instances, refs as above
 };
 
 struct DeviceScene;
@@ -36,6 +64,7 @@ struct TraceArgs {
     float* t;
     int32_t* prim;
     float *b0, *b1;
+    int32_t* inst;   // may be null
     float* vis;
 };
 

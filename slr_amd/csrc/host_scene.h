@@ -93,6 +93,11 @@ struct Accel {
 // Builds SBVH then QBVH over triangles given as 9 floats each (world space).
 // Deterministic; identical decisions to the reference builder.
 void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads);
+// The same over triangles followed by boxes (instances): primitive ids
+// 0..ntris-1 are the triangles, ntris + i is boxes[i], whose intersection
+// cost in the SAH is boxCosts[i] (a triangle's is 1).
+void buildAccel(const std::vector<float>& triPos, const std::vector<Box>& boxes, const std::vector<float>& boxCosts,
+                Accel* out, int numThreads);
 
 // ---------------------------------------------------------------- spectra
 

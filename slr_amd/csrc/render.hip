@@ -112,8 +112,9 @@ struct DeviceScene {
     DevBuf<DevMaterial> mats;
     DevBuf<DevSpectrum> spectra;
     DevBuf<float> specPool, lightCdf, lightPmf;
-    DevBuf<uint32_t> lights;
-    DevBuf<int32_t> primToLight;
+    DevBuf<uint2> lights;
+    DevBuf<int32_t> primToLight, instMat;
+    DevBuf<float4> instXf;
     DevBuf<uint32_t> spill;      // traversal stack spill for ray batches
     DevBuf<float4> envTexel;
     DevBuf<float> mengBasis, envCondPdf, envCondCdf, envMargPdf, envMargCdf;
@@ -177,10 +178,32 @@ static void cameraDerived(const CameraParams& c, uint32_t w, uint32_t h, DevCame
 
 int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     std::unique_ptr<DeviceScene> ds(new DeviceScene());
-    const Accel& ac = sd.accel;
+    // With instances the device holds one node array and one reference array:
+    // the world tree first (its root is node 0), then each prototype's tree
+    // with its child and reference indices shifted.
+    std::vector<QbvhNode> mergedNodes;
+    std::vector<uint32_t> mergedRefs, protoRoot(sd.protos.size(), 0);
+    const bool twoLevel = !sd.instances.empty();
+    if (twoLevel) {
+        mergedNodes = sd.accel.qbvh;
+        mergedRefs = sd.accel.qbvhRefs;
+        for (size_t k = 0; k < sd.protos.size(); ++k) {
+            const Accel& pa = sd.protos[k].accel;
+            const uint32_t nodeBase = (uint32_t)mergedNodes.size(), refBase = (uint32_t)mergedRefs.size();
+            protoRoot[k] = nodeBase;
+            for (QbvhNode q : pa.qbvh) {
+                for (int c = 0; c < 4; ++c)
+                    if (q.childValid[c]) q.index[c] += q.isLeaf[c] ? refBase : nodeBase;
+                mergedNodes.push_back(q);
+            }
+            mergedRefs.insert(mergedRefs.end(), pa.qbvhRefs.begin(), pa.qbvhRefs.end());
+        }
+    }
+    const std::vector<QbvhNode>& qnodes = twoLevel ? mergedNodes : sd.accel.qbvh;
+    const std::vector<uint32_t>& qrefs = twoLevel ? mergedRefs : sd.accel.qbvhRefs;
     // QBVH nodes; the traversal addresses nodes and triangle references with
     // 32-bit byte offsets (128 B per node, 48 B per reference)
-    const size_t nn = ac.qbvh.size();
+    const size_t nn = qnodes.size();
     if (nn > kMaxNodes) {
         if (err) *err = "QBVH exceeds 2^25 nodes";
         return SLRH_ERR_CAPACITY_CODE;
@@ -194,7 +217,7 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         order.push_back(0);
         newIndex[0] = 0;
         for (size_t h = 0; h < order.size() && order.size() < kBfsPrefix; ++h) {
-            const QbvhNode& q = ac.qbvh[order[h]];
+            const QbvhNode& q = qnodes[order[h]];
             for (int c = 0; c < 4 && order.size() < kBfsPrefix; ++c)
                 if (q.childValid[c] && !q.isLeaf[c] && newIndex[q.index[c]] == UINT32_MAX) {
                     newIndex[q.index[c]] = (uint32_t)order.size();
@@ -209,7 +232,7 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     }
     std::vector<float4> nodes(nn * 8);
     for (size_t k = 0; k < nn; ++k) {
-        const QbvhNode& q = ac.qbvh[order[k]];
+        const QbvhNode& q = qnodes[order[k]];
         float4* o = &nodes[8 * k];
         // rows 0-5: min x, max x, min y, max y, min z, max z of the four children
         for (int a = 0; a < 3; ++a) {
@@ -236,9 +259,22 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     // triangle references in leaf order
-    std::vector<float4> tris(ac.qbvhRefs.size() * 3);
-    for (size_t r = 0; r < ac.qbvhRefs.size(); ++r) {
-        uint32_t p = ac.qbvhRefs[r];
+    std::vector<float4> tris(qrefs.size() * 3);
+    for (size_t r = 0; r < qrefs.size(); ++r) {
+        uint32_t p = qrefs[r];
+        if (p & kRefInstance) {
+            // an instance: id, root of its prototype's tree, class override
+            const Instance& in = sd.instances[p & ~kRefInstance];
+            const uint32_t root = newIndex[protoRoot[in.proto]];
+            const uint32_t cls = in.material >= 0 ? sd.materials[in.material].cls : kNoClassOverride;
+            float w[3];
+            const uint32_t bits[3] = {p, root, cls};
+            memcpy(w, bits, sizeof(w));
+            tris[3 * r + 0] = make_float4(0.0f, 0.0f, 0.0f, w[0]);
+            tris[3 * r + 1] = make_float4(0.0f, 0.0f, 0.0f, w[1]);
+            tris[3 * r + 2] = make_float4(0.0f, 0.0f, 0.0f, w[2]);
+            continue;
+        }
         const uint32_t* v = &sd.idx[3 * (size_t)p];
         const float* p0 = &sd.vpos[3 * (size_t)v[0]];
         const float* p1 = &sd.vpos[3 * (size_t)v[1]];
@@ -273,12 +309,37 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     }
     if (pool.empty()) pool.push_back(0.0f);
     // lights: every emitting triangle with importance 1, in object order
-    // (SurfaceObjectAggregate ctor, libSLR/Core/surface_object.cpp:379-398)
-    std::vector<uint32_t> lights;
+    // (SurfaceObjectAggregate ctor, libSLR/Core/surface_object.cpp:379-398):
+    // the world-space ones, then per instance its prototype's (an instance
+    // carries its prototype's importance, TransformedSurfaceObject::importance,
+    // surface_object.cpp:298-300)
+    std::vector<uint2> lights;
     std::vector<int32_t> primToLight(nt, -1);
     for (size_t i = 0; i < nt; ++i) {
         const DevMaterial& m = sd.materials[sd.triMat[i]];
-        if (m.emit >= 0) { primToLight[i] = (int32_t)lights.size(); lights.push_back((uint32_t)i); }
+        if (sd.triProto[i] < 0 && m.emit >= 0) {
+            primToLight[i] = (int32_t)lights.size();
+            lights.push_back(make_uint2((uint32_t)i, kNoInstance));
+        }
+    }
+    for (size_t k = 0; k < sd.instances.size(); ++k) {
+        const Instance& in = sd.instances[k];
+        const Prototype& pr = sd.protos[in.proto];
+        for (uint32_t i = pr.triFirst; i < pr.triFirst + pr.triCount; ++i) {
+            const uint32_t mat = in.material >= 0 ? (uint32_t)in.material : sd.triMat[i];
+            if (sd.materials[mat].emit >= 0) lights.push_back(make_uint2(i, (uint32_t)k));
+        }
+    }
+    // per instance: object-to-world rows, world-to-object rows; material override
+    std::vector<float4> instXf(std::max<size_t>(1, 6 * sd.instances.size()), make_float4(0, 0, 0, 0));
+    std::vector<int32_t> instMat(std::max<size_t>(1, sd.instances.size()), -1);
+    for (size_t k = 0; k < sd.instances.size(); ++k) {
+        const Instance& in = sd.instances[k];
+        for (int r = 0; r < 3; ++r) {
+            instXf[6 * k + r] = make_float4(in.l2w[4 * r], in.l2w[4 * r + 1], in.l2w[4 * r + 2], in.l2w[4 * r + 3]);
+            instXf[6 * k + 3 + r] = make_float4(in.w2l[4 * r], in.w2l[4 * r + 1], in.w2l[4 * r + 2], in.w2l[4 * r + 3]);
+        }
+        instMat[k] = in.material;
     }
     // DiscreteDistribution1D over unit importances with a compensated sum
     // (libSLR/Core/distributions.cpp:99-123)
@@ -297,14 +358,15 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     }
     if (cdf.empty()) cdf.push_back(0);
     if (pmf.empty()) pmf.push_back(0);
-    if (lights.empty()) lights.push_back(0);
+    if (lights.empty()) lights.push_back(make_uint2(0, kNoInstance));
     int r;
     if ((r = ds->nodes.upload(nodes, err)) || (r = ds->tris.upload(tris, err)) || (r = ds->vpos.upload(vp, err)) ||
         (r = ds->vnrm.upload(vn, err)) || (r = ds->vtan.upload(vt, err)) || (r = ds->primVerts.upload(pv, err)) ||
         (r = ds->mats.upload(sd.materials, err)) || (r = ds->spectra.upload(specs, err)) ||
         (r = ds->specPool.upload(pool, err)) || (r = ds->lights.upload(lights, err)) ||
         (r = ds->lightCdf.upload(cdf, err)) || (r = ds->lightPmf.upload(pmf, err)) ||
-        (r = ds->primToLight.upload(primToLight, err)))
+        (r = ds->primToLight.upload(primToLight, err)) || (r = ds->instXf.upload(instXf, err)) ||
+        (r = ds->instMat.upload(instMat, err)))
         return r;
     if ((r = ds->counters.alloc(16, err))) return r;
     DevScene& ve = ds->view;
@@ -348,6 +410,10 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     v.mats = ds->mats.p; v.spectra = ds->spectra.p; v.specPool = ds->specPool.p;
     v.lights = ds->lights.p; v.lightCdf = ds->lightCdf.p; v.lightPmf = ds->lightPmf.p;
     v.primToLight = ds->primToLight.p;
+    v.instXf = ds->instXf.p;
+    v.instMat = ds->instMat.p;
+    v.numInstances = (uint32_t)sd.instances.size();
+    v.compatInstShadow = sd.compatInstanceShadow ? 1u : 0u;
     v.numLights = (uint32_t)(cdf.size() - 1);
     v.numPrims = (uint32_t)nt;
     v.numNodes = (uint32_t)nn;
@@ -386,6 +452,7 @@ __device__ __forceinline__ void waveAdd64(unsigned long long* counter, unsigned 
 
 // ------------------------------------------------------------------ ray batches
 
+template <bool I>
 __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, TraceArgs a, uint32_t n, int any,
                                                         uint32_t* spill, uint32_t spillStride,
                                                         unsigned long long* counters) {
@@ -403,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, TraceArgs a
     TraceCounters tc;
     uint32_t* fetchCtr = (uint32_t*)&counters[3];
     const bool anyHit = any != 0;
-    traceLoop<kBatchRefill, true, false>(sc, st, n, fetchCtr, [](uint32_t i) { return i; },
+    traceLoop<kBatchRefill, true, false, I>(sc, st, n, fetchCtr, [](uint32_t i) { return i; },
         [&](uint32_t i, RayIn& r) {
             r.o = make_float4(a.ox[i], a.oy[i], a.oz[i], a.tmin[i]);
             r.d = make_float4(a.dx[i], a.dy[i], a.dz[i], a.tmax[i]);
@@ -417,6 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, TraceArgs a
                 a.prim[i] = h.prim;
                 a.b0[i] = h.b0;
                 a.b1[i] = h.b1;
+                if (a.inst) a.inst[i] = h.inst;
             }
         }, tc);
     waveAdd64(&counters[0], tc.nodes);
@@ -448,14 +516,17 @@ int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool d
             (void)hipMemcpyAsync(p, hin[k], n * sizeof(float), hipMemcpyHostToDevice, s);
             *din[k] = (const float*)p;
         }
-        int nout = any ? 1 : 4;
+        int nout = any ? 1 : (a.inst ? 5 : 4);
         for (int k = 0; k < nout; ++k) {
             void* p = nullptr;
             if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) { cleanup(); if (err) *err = "hipMalloc"; return SLRH_ERR_NOMEM_CODE; }
             temps.push_back(p);
         }
         if (any) d.vis = (float*)temps[8];
-        else { d.t = (float*)temps[8]; d.prim = (int32_t*)temps[9]; d.b0 = (float*)temps[10]; d.b1 = (float*)temps[11]; }
+        else {
+            d.t = (float*)temps[8]; d.prim = (int32_t*)temps[9]; d.b0 = (float*)temps[10]; d.b1 = (float*)temps[11];
+            d.inst = a.inst ? (int32_t*)temps[12] : nullptr;
+        }
     }
     uint32_t blocks = persistentBlocks(n);
     size_t spillNeed = (size_t)(kMaxStack - kLdsDepth) * blocks * kBlock;
@@ -464,8 +535,12 @@ int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool d
         if (r) { cleanup(); return r; }
     }
     (void)hipMemsetAsync(ds->counters.p, 0, 4 * sizeof(unsigned long long), s);
-    hipLaunchKernelGGL(k_trace_batch, dim3(blocks), dim3(kBlock), 0, s, ds->view, d, n, any ? 1 : 0, ds->spill.p,
-                       blocks * kBlock, ds->counters.p);
+    if (ds->view.numInstances)
+        hipLaunchKernelGGL(k_trace_batch<true>, dim3(blocks), dim3(kBlock), 0, s, ds->view, d, n, any ? 1 : 0,
+                           ds->spill.p, blocks * kBlock, ds->counters.p);
+    else
+        hipLaunchKernelGGL(k_trace_batch<false>, dim3(blocks), dim3(kBlock), 0, s, ds->view, d, n, any ? 1 : 0,
+                           ds->spill.p, blocks * kBlock, ds->counters.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { cleanup(); if (err) *err = std::string("k_trace_batch: ") + hipGetErrorString(e); return SLRH_ERR_DEVICE_CODE; }
     unsigned long long cnt[4] = {0, 0, 0, 0};
@@ -477,6 +552,7 @@ int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool d
             (void)hipMemcpyAsync(a.prim, d.prim, n * sizeof(int32_t), hipMemcpyDeviceToHost, s);
             (void)hipMemcpyAsync(a.b0, d.b0, n * sizeof(float), hipMemcpyDeviceToHost, s);
             (void)hipMemcpyAsync(a.b1, d.b1, n * sizeof(float), hipMemcpyDeviceToHost, s);
+            if (a.inst) (void)hipMemcpyAsync(a.inst, d.inst, n * sizeof(int32_t), hipMemcpyDeviceToHost, s);
         }
     }
     e = hipStreamSynchronize(s);
@@ -563,6 +639,7 @@ enum SlotField : uint32_t {
     SF_SD = 4,      // shadow ray direction.xyz + tmax
     SF_SINV = 5,    // rayInv(shadow direction)
     SF_HIT = 6,     // t, prim, b0, b1
+    SF_HITX = 7,    // instance of the hit (x; scenes with instances only)
     SF_ALPHA = 8,   // 4 float4
     SF_SP = 12,     // 4 float4
     SF_NEE = 16,    // 4 float4
@@ -1003,7 +1080,7 @@ __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
 // resolveShadow in the next kernel that reads the path radiance, outside this
 // divergent loop). Mixing both kinds in one launch leaves one tail of long
 // rays per iteration instead of two.
-template <int D, int T, bool F, int W, int L>
+template <int D, int T, bool F, int W, int L, bool I>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1, W > 0 ? W : 10)))
 void k_trace(DevScene sc, PathState ps) {
     __shared__ uint32_t lds[D * kBlock];
@@ -1022,7 +1099,7 @@ void k_trace(DevScene sc, PathState ps) {
     st.treeletNodes = L < sc.numNodes ? (uint32_t)L : sc.numNodes;
     st.treelet = loadTreelet(treelet, sc.nodes, st.treeletNodes);
     TraceCounters tc;
-    traceLoop<T, F, true>(sc, st, count, &ps.ctr[C_FETCH], [&](uint32_t i) { return qin[i]; },
+    traceLoop<T, F, true, I>(sc, st, count, &ps.ctr[C_FETCH], [&](uint32_t i) { return qin[i]; },
         [&](uint32_t e, RayIn& r) {
             const uint32_t s = e & ~kShadowEntry;
             const bool shadow = (e & kShadowEntry) != 0;
@@ -1039,6 +1116,7 @@ void k_trace(DevScene sc, PathState ps) {
                 if (h.prim < 0) ps.vis[s] = 1;
             } else {
                 ps.f4(s, SF_HIT) = make_float4(h.t, __int_as_float(h.prim), h.b0, h.b1);
+                if (I) *(int32_t*)&ps.f4(s, SF_HITX) = h.inst;
                 ps.code[s] = (uint8_t)(CODE_CLS0 + (h.prim >= 0 ? h.aux : (uint32_t)CLS_MISS));
             }
         }, tc);
@@ -1066,9 +1144,45 @@ __device__ __forceinline__ float3 fromLocal(const SurfPt& sp, const float3& v) {
               dot3(f3(sp.sx.z, sp.sy.z, sp.sz.z), v));
 }
 
-// TriangleSurfaceShape::calculateSurfacePoint (TriangleSurfaceShape.cpp:301-346)
+// Instance transforms: object-to-world rows X[0..2], world-to-object rows
+// X[3..5]. Vectors go by the linear part, normals by the inverse transpose
+// (StaticTransform * Vector3D / Normal3D, libSLR/Core/transform.h:41-49).
+__device__ __forceinline__ float3 xfPoint(const float4* X, float3 p) {
+    return f3(X[0].x * p.x + X[0].y * p.y + X[0].z * p.z + X[0].w, X[1].x * p.x + X[1].y * p.y + X[1].z * p.z + X[1].w,
+              X[2].x * p.x + X[2].y * p.y + X[2].z * p.z + X[2].w);
+}
+__device__ __forceinline__ float3 xfVector(const float4* X, float3 v) {
+    return f3(X[0].x * v.x + X[0].y * v.y + X[0].z * v.z, X[1].x * v.x + X[1].y * v.y + X[1].z * v.z,
+              X[2].x * v.x + X[2].y * v.y + X[2].z * v.z);
+}
+__device__ __forceinline__ float3 xfNormal(const float4* X, float3 n) {
+    return f3(X[3].x * n.x + X[4].x * n.y + X[5].x * n.z, X[3].y * n.x + X[4].y * n.y + X[5].y * n.z,
+              X[3].z * n.x + X[4].z * n.y + X[5].z * n.z);
+}
+
+// A triangle's vertices in world space (inst < 0: already there).
+__device__ __forceinline__ void worldVerts(const DevScene& sc, uint32_t prim, int32_t inst, float3* q0, float3* q1,
+                                           float3* q2, uint4* pvOut) {
+    const uint4 pv = sc.primVerts[prim];
+    *q0 = xyz(sc.vpos[pv.x]);
+    *q1 = xyz(sc.vpos[pv.y]);
+    *q2 = xyz(sc.vpos[pv.z]);
+    if (inst >= 0) {
+        const float4* X = sc.instXf + 6u * (uint32_t)inst;
+        *q0 = xfPoint(X, *q0);
+        *q1 = xfPoint(X, *q1);
+        *q2 = xfPoint(X, *q2);
+    }
+    *pvOut = pv;
+}
+
+// TriangleSurfaceShape::calculateSurfacePoint (TriangleSurfaceShape.cpp:301-346);
+// for a triangle of an instance the object-space point is then taken to
+// world space as SurfacePoint::applyTransform does (libSLR/Core/geometry.cpp:
+// 40-45, 73-77): frame vectors by the transform, the normal and geometric
+// normal by its inverse transpose, each renormalised.
 __device__ __forceinline__ SurfPt surfaceAt(const DevScene& sc, uint32_t prim, float b0, float b1, float3 o, float3 d,
-                                            float t) {
+                                            float t, int32_t inst = -1) {
     SurfPt s;
     const uint4 pv = sc.primVerts[prim];
     const float3 p0 = xyz(sc.vpos[pv.x]), p1 = xyz(sc.vpos[pv.y]), p2 = xyz(sc.vpos[pv.z]);
@@ -1084,12 +1198,24 @@ __device__ __forceinline__ SurfPt surfaceAt(const DevScene& sc, uint32_t prim, f
     s.gN = normalize3(cross3(p1 - p0, p2 - p0));
     s.mat = pv.w;
     s.prim = prim;
+    if (inst >= 0) {
+        const float4* X = sc.instXf + 6u * (uint32_t)inst;
+        s.sx = normalize3(xfVector(X, s.sx));
+        s.sy = normalize3(xfVector(X, s.sy));
+        s.sz = normalize3(xfNormal(X, s.sz));
+        s.gN = normalize3(xfNormal(X, s.gN));
+        const int32_t om = sc.instMat[inst];
+        if (om >= 0) s.mat = (uint32_t)om;
+    }
     return s;
 }
 
-__device__ __forceinline__ float triArea(const DevScene& sc, uint32_t prim) {
-    const uint4 pv = sc.primVerts[prim];
-    const float3 p0 = xyz(sc.vpos[pv.x]), p1 = xyz(sc.vpos[pv.y]), p2 = xyz(sc.vpos[pv.z]);
+// World-space area (the reference's light area PDF uses the object-space
+// area, which differs only under a scaling instance transform).
+__device__ __forceinline__ float triArea(const DevScene& sc, uint32_t prim, int32_t inst = -1) {
+    float3 p0, p1, p2;
+    uint4 pv;
+    worldVerts(sc, prim, inst, &p0, &p1, &p2, &pv);
     return 0.5f * length3(cross3(p1 - p0, p2 - p0));
 }
 
@@ -1219,7 +1345,8 @@ void k_shade(DevScene sc, PathState ps) {
                 const float4 o4 = ps.f4(s, SF_RO), d4 = ps.f4(s, SF_RD), h = ps.f4(s, SF_HIT);
                 const float3 ro = xyz(o4), rd = xyz(d4);
                 const uint32_t prim = (uint32_t)__float_as_int(h.y);
-                const SurfPt sp = surfaceAt(sc, prim, h.z, h.w, ro, rd, h.x);
+                const int32_t hinst = sc.numInstances ? *(const int32_t*)&ps.f4(s, SF_HITX) : -1;
+                const SurfPt sp = surfaceAt(sc, prim, h.z, h.w, ro, rd, h.x, hinst);
                 const DevMaterial mat = sc.mats[sp.mat];
                 const int hero = (int)((info.y >> 8) & 0xFFu);
                 uint32_t pathLen = info.y >> 16;
@@ -1242,8 +1369,12 @@ void k_shade(DevScene sc, PathState ps) {
                         const float bsdfPDF = misc.z;
                         const float3 dv = sp.p - ro;
                         const float dist2 = dot3(dv, dv);
-                        const float lightProb = sc.lightPmf[sc.primToLight[prim]] * lightSelectScale(sc);
-                        const float lightPDF = lightProb * (1.0f / triArea(sc, prim)) * dist2 / fabsf(dot3(rd, sp.gN));
+                        // every light has importance one: an instance's triangles
+                        // share the world triangles' selection probability
+                        const float lightProb =
+                            sc.lightPmf[hinst >= 0 ? 0 : sc.primToLight[prim]] * lightSelectScale(sc);
+                        const float lightPDF =
+                            lightProb * (1.0f / triArea(sc, prim, hinst)) * dist2 / fabsf(dot3(rd, sp.gN));
                         misW = (bsdfPDF * bsdfPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
                     }
                     const float edf = (float)(1.0 / 3.14159265358979323846);
@@ -1327,14 +1458,22 @@ void k_shade(DevScene sc, PathState ps) {
                     } else {
                         float lprob;
                         const uint32_t li = sampleLight(sc, uSurf, &lprob);
-                        const uint32_t lprim = sc.lights[li];
-                        // TriangleSurfaceShape::sample (TriangleSurfaceShape.cpp:357-413)
+                        const uint2 lent = sc.lights[li];
+                        const uint32_t lprim = lent.x;
+                        const int32_t linst = lent.y == kNoInstance ? -1 : (int32_t)lent.y;
+                        // TriangleSurfaceShape::sample (TriangleSurfaceShape.cpp:357-413),
+                        // then the instance's transform (TransformedSurfaceObject::selectLight)
                         const float su = sqrtf(lu0);
                         const float lb0 = 1.0f - su, lb1 = lu1 * su, lb2 = 1.0f - lb0 - lb1;
-                        const uint4 lv = sc.primVerts[lprim];
-                        const float3 q0 = xyz(sc.vpos[lv.x]), q1 = xyz(sc.vpos[lv.y]), q2 = xyz(sc.vpos[lv.z]);
-                        const float3 ln = normalize3(lb0 * xyz(sc.vnrm[lv.x]) + lb1 * xyz(sc.vnrm[lv.y]) +
-                                                     lb2 * xyz(sc.vnrm[lv.z]));
+                        float3 q0, q1, q2;
+                        uint4 lv;
+                        worldVerts(sc, lprim, linst, &q0, &q1, &q2, &lv);
+                        float3 ln = normalize3(lb0 * xyz(sc.vnrm[lv.x]) + lb1 * xyz(sc.vnrm[lv.y]) +
+                                               lb2 * xyz(sc.vnrm[lv.z]));
+                        if (linst >= 0) {
+                            ln = normalize3(xfNormal(sc.instXf + 6u * (uint32_t)linst, ln));
+                            if (sc.instMat[linst] >= 0) lv.w = (uint32_t)sc.instMat[linst];
+                        }
                         const float3 lp = lb0 * q0 + lb1 * q1 + lb2 * q2;
                         const float3 lg = normalize3(cross3(q1 - q0, q2 - q0));
                         const float areaPDF = 1.0f / (0.5f * length3(cross3(q1 - q0, q2 - q0)));
@@ -1665,9 +1804,14 @@ static int finishJob(SlotPool& p, JobRun& j, std::string* err) {
 
 template <int V>
 static void launchTraceV(uint32_t blocks, hipStream_t s, const DevScene& sc, const PathState& ps) {
-    hipLaunchKernelGGL((k_trace<kVariants[V].ldsDepth, kVariants[V].refill, kVariants[V].fuse,
-                                kVariants[V].wavesPerEU, kVariants[V].treelet>),
-                       dim3(blocks), dim3(kBlock), 0, s, sc, ps);
+    if (sc.numInstances)
+        hipLaunchKernelGGL((k_trace<kVariants[V].ldsDepth, kVariants[V].refill, kVariants[V].fuse,
+                                    kVariants[V].wavesPerEU, kVariants[V].treelet, true>),
+                           dim3(blocks), dim3(kBlock), 0, s, sc, ps);
+    else
+        hipLaunchKernelGGL((k_trace<kVariants[V].ldsDepth, kVariants[V].refill, kVariants[V].fuse,
+                                    kVariants[V].wavesPerEU, kVariants[V].treelet, false>),
+                           dim3(blocks), dim3(kBlock), 0, s, sc, ps);
 }
 typedef void (*LaunchFn)(uint32_t, hipStream_t, const DevScene&, const PathState&);
 static const LaunchFn kTraceLaunch[kNumVariants] = {launchTraceV<0>, launchTraceV<1>, launchTraceV<2>,

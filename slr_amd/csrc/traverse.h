@@ -92,7 +92,23 @@ struct HitOut {
     int32_t prim;
     float b0, b1;
     uint32_t aux;   // the hit triangle's per-reference payload (material class)
+    int32_t inst;   // instance of the hit triangle, -1 in world space
 };
+
+// Two-level traversal (instances). A world-level reference whose first word
+// has kRefInstBit set is an instance: word 3 of its first float4 holds the
+// instance id, of the second the root node of its prototype's tree, of the
+// third the material class that replaces the prototype's (kNoClassOverride:
+// none). Entering one suspends the rest of the current node (the held next
+// node, the pending leaf children, the rest of the current leaf) onto the
+// stack as node and leaf words under a kInstMarker, takes the ray into
+// object space and traverses the prototype's tree; popping the marker
+// restores the world ray. Hit distances are the same in both spaces (the
+// direction is transformed, not renormalised).
+constexpr uint32_t kRefInstBit = 0x80000000u;
+constexpr uint32_t kInstMarker = 0x7FFFFFFFu;   // stack word: back to the world ray
+constexpr uint32_t kEnterInst = 0x7FFFFFFEu;    // next node: the instance transform is on its way
+constexpr uint32_t kNoClassOverride = 0xFFFFFFFFu;
 
 // Reciprocal direction of a ray and its sign bits, computed once by whoever
 // produces the ray: xyz = 1 / d (QBVH::Node::intersect's invRayDir, correctly
@@ -243,7 +259,8 @@ __device__ __forceinline__ uint32_t kthSetBit(unsigned long long m, uint32_t k) 
 //   load(e, RayIn& r)   -> bool, requests the ray of entry e (r.inv too when
 //                          kHasInv, else it is computed here); true for any hit
 //   finish(e, anyHit, const HitOut&)
-template <int kRefill, bool kFuse, bool kHasInv, typename Entry, typename Load, typename Finish>
+template <int kRefill, bool kFuse, bool kHasInv, bool kInst = false, typename Entry, typename Load,
+          typename Finish>
 __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& st, uint32_t count,
                                           uint32_t* fetchCtr, Entry&& entry, Load&& load, Finish&& finish,
                                           TraceCounters& tc) {
@@ -262,6 +279,11 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
     float tmax = 0.0f, hb1 = 0.0f, hb2 = 0.0f;
     int32_t prim = -1;
     uint32_t aux = 0;
+    // instances: the one being traversed, its root and class override, the
+    // hit's instance, and the world-to-object rows requested on entry
+    int32_t inst = -1, hitInst = -1;
+    uint32_t instRoot = 0, instCls = kNoClassOverride;
+    float4 x0 = make_float4(0, 0, 0, 0), x1 = x0, x2 = x0;
     int sp = 0;
     uint32_t ni = kNoNode, tcur = 0, tend = 0, leafPend = 0;
     uint32_t v[4] = {kNoChild, kNoChild, kNoChild, kNoChild};
@@ -293,6 +315,16 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
             ni = 0;
             tcur = tend = 0;
             leafPend = 0;
+            if (kInst) {
+                inst = hitInst = -1;
+                instCls = kNoClassOverride;
+            }
+            stage = 3;
+        }
+        if (kInst && stage == 4) {
+            // the world ray reloaded after an instance: traversal goes on
+            if (!kHasInv) r.inv = rayInv(r.dir());
+            p = prepareRay(r);
             stage = 3;
         }
         // idle lanes start the entry they claimed in an earlier iteration
@@ -349,6 +381,39 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
             if (tcur >= tend) {
                 // one node
                 if (ni == kNoNode) ni = stackPop(st, sp);
+                if (kInst) {
+                    if (ni == kInstMarker) {
+                        // the instance is done: request the world ray again
+                        inst = -1;
+                        instCls = kNoClassOverride;
+                        ni = kNoNode;
+                        load(ent, r);
+                        stage = 4;
+                        goto stepDone;
+                    }
+                    if (ni == kEnterInst) {
+                        // object-space ray: origin by the affine rows, direction
+                        // by their linear part (StaticTransform * Ray)
+                        if (!(sc.compatInstShadow && anyHit)) {
+                            const float3 o = r.org(), d = r.dir();
+                            r.o = make_float4(x0.x * o.x + x0.y * o.y + x0.z * o.z + x0.w,
+                                              x1.x * o.x + x1.y * o.y + x1.z * o.z + x1.w,
+                                              x2.x * o.x + x2.y * o.y + x2.z * o.z + x2.w, r.o.w);
+                            r.d = make_float4(x0.x * d.x + x0.y * d.y + x0.z * d.z, x1.x * d.x + x1.y * d.y + x1.z * d.z,
+                                              x2.x * d.x + x2.y * d.y + x2.z * d.z, r.d.w);
+                            r.inv = rayInv(r.dir());
+                            p = prepareRay(r);
+                        }
+                        ni = instRoot;
+                    } else if (ni & kLeafBit) {
+                        // a suspended leaf range: intersect it next
+                        tcur = ni & kLeafFirstMask;
+                        tend = tcur + ((ni >> kLeafCountShift) & kLeafCountMask);
+                        leafPend = 0;
+                        ni = kNoNode;
+                        goto triStep;
+                    }
+                }
                 ++tc.nodes;
                 tc.nodesAny += anyHit ? 1u : 0u;
                 // all rows the step needs are requested together: one round
@@ -402,6 +467,7 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
                 // the same iteration; otherwise the next iteration does
                 if (!kFuse) goto stepDone;
             }
+        triStep:
             if (tcur < tend) {
                 // one triangle of the current leaf
                 const uint32_t tb = tcur * 48u;
@@ -410,19 +476,48 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
                 // otherwise request v0 only after the determinant test)
                 asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(e1.x), "v"(e1.y), "v"(e1.z),
                              "v"(e1.w), "v"(e2.x), "v"(e2.y), "v"(e2.z));
-                float t, b1, b2;
-                ++tc.tris;
-                tc.trisAny += anyHit ? 1u : 0u;
-                if (triTest(a, e1, e2, r, r.tmin(), tmax, &t, &b1, &b2)) {
-                    tmax = t;
-                    prim = (int32_t)__float_as_uint(a.w);
-                    aux = __float_as_uint(e1.w);
-                    hb1 = b1;
-                    hb2 = b2;
-                    done = anyHit;
+                if (kInst && (__float_as_uint(a.w) & kRefInstBit)) {
+                    // an instance: suspend this node's remaining work (held
+                    // next node deepest, then pending leaves far to near, then
+                    // the rest of this leaf) under the marker, and request the
+                    // instance's world-to-object rows
+                    ++tcur;
+                    if (ni != kNoNode) stackPush(st, sp, ni, tc.overflow);
+#pragma unroll
+                    for (int i = 3; i >= 0; --i)
+                        if ((leafPend >> i) & 1u) stackPush(st, sp, i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3],
+                                                           tc.overflow);
+                    if (tcur < tend) stackPush(st, sp, kLeafBit | ((tend - tcur) << kLeafCountShift) | tcur, tc.overflow);
+                    stackPush(st, sp, kInstMarker, tc.overflow);
+                    leafPend = 0;
+                    tcur = tend;
+                    inst = (int32_t)(__float_as_uint(a.w) & ~kRefInstBit);
+                    instRoot = __float_as_uint(e1.w);
+                    instCls = __float_as_uint(e2.w);
+                    const float4* xr = sc.instXf + 6u * (uint32_t)inst + 3u;
+                    x0 = xr[0];
+                    x1 = xr[1];
+                    x2 = xr[2];
+                    ni = kEnterInst;
+                } else {
+                    float t, b1, b2;
+                    ++tc.tris;
+                    tc.trisAny += anyHit ? 1u : 0u;
+                    if (triTest(a, e1, e2, r, r.tmin(), tmax, &t, &b1, &b2)) {
+                        tmax = t;
+                        prim = (int32_t)__float_as_uint(a.w);
+                        aux = __float_as_uint(e1.w);
+                        if (kInst) {
+                            if (instCls != kNoClassOverride) aux = instCls;
+                            hitInst = inst;
+                        }
+                        hb1 = b1;
+                        hb2 = b2;
+                        done = anyHit;
+                    }
+                    ++tcur;
+                    nextLeaf();
                 }
-                ++tcur;
-                nextLeaf();
             }
         stepDone:
             if (done || (tcur >= tend && ni == kNoNode && sp == 0)) {
@@ -432,6 +527,7 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
                 h.b0 = prim >= 0 ? 1.0f - hb1 - hb2 : 0.0f;
                 h.b1 = prim >= 0 ? hb1 : 0.0f;
                 h.aux = aux;
+                h.inst = kInst ? hitInst : -1;
                 finish(ent, anyHit, h);
                 ++tc.rays;
                 tc.raysAny += anyHit ? 1u : 0u;

@@ -267,6 +267,99 @@ def cornell_icosphere(width=1920, height=1080, subdiv=8, seed=7):
     return desc
 
 
+def _rotation(axis, angle):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    x, y, z = axis
+    c, s = math.cos(angle), math.sin(angle)
+    C = 1 - c
+    return np.array([[c + x * x * C, x * y * C - z * s, x * z * C + y * s],
+                     [y * x * C + z * s, c + y * y * C, y * z * C - x * s],
+                     [z * x * C - y * s, z * y * C + x * s, c + z * z * C]])
+
+
+def random_affine(rng, center_lo, center_hi, scale_lo=0.6, scale_hi=1.3, uniform=True):
+    """Seeded object-to-world affine transform (3x4): a random rotation, a
+    scale (uniform or per axis) and a translation in [center_lo, center_hi]."""
+    R = _rotation(rng.normal(size=3), rng.uniform(0, 2 * math.pi))
+    sc = rng.uniform(scale_lo, scale_hi, size=1 if uniform else 3) * np.ones(3)
+    m = np.zeros((3, 4))
+    m[:, :3] = R @ np.diag(sc)
+    m[:, 3] = rng.uniform(center_lo, center_hi)
+    return m
+
+
+def flatten_instances(desc):
+    """The same scene with every instance's triangles transformed into world
+    space as plain meshes (for checking two-level traversal against the
+    single-level one). Returns (desc, prim_map): prim_map[(inst, proto prim)]
+    -> flattened prim id is given as an array per instance."""
+    out = {k: v for k, v in desc.items() if k not in ("prototypes", "instances")}
+    meshes = list(desc["meshes"])
+    base = sum(len(m["idx"]) for m in meshes)
+    proto_first = []
+    first = base
+    for pr in desc.get("prototypes", []):
+        proto_first.append(first)
+        first += len(pr["idx"])
+    maps = []
+    nflat = base
+    for inst in desc.get("instances", []):
+        pr = desc["prototypes"][inst["proto"]]
+        m = np.asarray(inst["l2w"], np.float64).reshape(-1, 4)[:3]
+        L, t = m[:, :3], m[:, 3]
+        pos = np.asarray(pr["pos"], np.float64) @ L.T + t
+        N = np.linalg.inv(L).T
+        nrm = np.asarray(pr["nrm"], np.float64) @ N.T
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        tan = np.asarray(pr["tan"], np.float64) @ L.T
+        tan /= np.linalg.norm(tan, axis=1, keepdims=True)
+        mat = inst["mat"] if inst.get("mat", -1) >= 0 else pr["mat"]
+        meshes.append({"pos": pos.astype(np.float32), "nrm": nrm.astype(np.float32), "tan": tan.astype(np.float32),
+                       "idx": pr["idx"], "mat": mat})
+        maps.append(nflat + np.arange(len(pr["idx"])))
+        nflat += len(pr["idx"])
+    out["meshes"] = meshes
+    return out, maps, proto_first
+
+
+def instanced_cornell(width=64, height=64, n_instances=8, seed=11, uniform_scale=True, instanced_light=False):
+    """The Cornell box walls and light with the two boxes replaced by
+    instances of two prototypes, a box (12 triangles, white Lambert) and an
+    icosphere of subdivision 2 (320 triangles, GGX conductor), under seeded
+    random rotations, scales and positions inside the box. instanced_light:
+    the ceiling light is a third prototype placed by a rigid instance."""
+    desc = cornell_box(width, height)
+    light = desc["meshes"][5]
+    desc["meshes"] = desc["meshes"][:5 if instanced_light else 6]
+    desc["materials"].append({"kind": "ggx_conductor", "eta": 0.2, "k": 3.9, "alpha": 0.2})
+    pos, nrm, idx = _box((0.0, 0.0, 0.0), (100.0, 100.0, 100.0), 0.0)
+    box = _mesh(pos, nrm, idx, 0)
+    v, f = icosphere(2)
+    sph = _mesh(v * 60.0, v, f, len(desc["materials"]) - 1)
+    desc["prototypes"] = [box, sph]
+    rng = np.random.default_rng(seed)
+    insts = []
+    for i in range(n_instances):
+        m = random_affine(rng, np.array([100.0, 60.0, 100.0]) * SCALE, np.array([450.0, 300.0, 450.0]) * SCALE,
+                          uniform=uniform_scale)
+        insts.append({"proto": i % 2, "l2w": m})
+    if instanced_light:
+        # the light quad about its centre, placed back by a quarter turn
+        # about y and a translation (its footprint is symmetric enough to
+        # stay under the ceiling)
+        c = np.asarray(light["pos"], np.float64).mean(0)
+        desc["prototypes"].append({"pos": (np.asarray(light["pos"], np.float64) - c).astype(np.float32),
+                                   "nrm": light["nrm"], "tan": light["tan"], "idx": light["idx"],
+                                   "mat": light["mat"]})
+        m = np.zeros((3, 4))
+        m[:, :3] = _rot_y(90.0)
+        m[:, 3] = c
+        insts.append({"proto": 2, "l2w": m})
+    desc["instances"] = insts
+    return desc
+
+
 def sky_environment(width=256, height=128, sun_dir=(0.4, 0.8, 0.45), sun_radiance=1e4, sun_radius_deg=1.5,
                     zenith=(0.25, 0.45, 1.0), horizon=(1.0, 0.95, 0.85), ground=(0.2, 0.18, 0.15)):
     """Equirectangular sky: a zenith-to-horizon gradient, a darker ground and a

@@ -1,0 +1,88 @@
+"""Instancing on the host: the world-level SBVH and QBVH over world
+triangles and instances, built as the reference builds its aggregate over
+triangles and TransformedSurfaceObjects (spatial splits clip an instance's
+world box, libSLR/Core/object.h:50-80; an instance costs its prototype
+tree's SAH cost, SBVH.h:402,419), are node-for-node the reference's; the
+world-to-object rows are the reference's FP32 inverse bit for bit; the
+C ABI rejects bad prototypes, materials and singular transforms."""
+import numpy as np
+import pytest
+
+import slr_amd
+from slr_amd import scenes
+from tests.conftest import need_ref
+
+
+@pytest.mark.parametrize("uniform,light", [(True, False), (False, False), (False, True)])
+def test_world_tree_matches_reference(host_ctx, uniform, light):
+    refdrv = need_ref()
+    desc = scenes.instanced_cornell(16, 16, n_instances=12, seed=3, uniform_scale=uniform, instanced_light=light)
+    sc = slr_amd.build_scene(host_ctx, desc)
+    ref = refdrv.RefScene(desc)
+    ref.build_accel()
+    mine, theirs = sc.accel_stats(), ref.accel_stats()
+    for k in theirs:
+        assert mine[k] == theirs[k], (k, mine[k], theirs[k])
+    for a, b in zip(sc.sbvh_dump(), ref.sbvh_dump()):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for a, b in zip(sc.qbvh_dump(), ref.qbvh_dump()):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    # the refs name instances (bit 31) as well as world triangles
+    refs = sc.qbvh_dump()[2]
+    assert (refs < 0).sum() >= 12 + (1 if light else 0)
+
+
+def test_instance_inverse_matches_reference(host_ctx):
+    refdrv = need_ref()
+    desc = scenes.instanced_cornell(16, 16, n_instances=24, seed=9, uniform_scale=False)
+    sc = slr_amd.build_scene(host_ctx, desc)
+    ref = refdrv.RefScene(desc)
+    for k in range(24):
+        l2w, w2l = sc.instance_transform(k)
+        assert np.array_equal(l2w, np.asarray(desc["instances"][k]["l2w"], np.float32))
+        assert np.array_equal(w2l.view(np.uint32), ref.instance_inverse(k).view(np.uint32)), k
+        # and it is an inverse
+        m = np.vstack([np.asarray(l2w, np.float64), [0, 0, 0, 1]])
+        mi = np.vstack([np.asarray(w2l, np.float64), [0, 0, 0, 1]])
+        assert np.allclose(m @ mi, np.eye(4), atol=1e-5)
+
+
+def test_instance_api_errors(host_ctx):
+    sc = slr_amd.Scene(host_ctx)
+    mat = sc.lambert(sc.spectrum_constant(0.5))
+    v, f = scenes.icosphere(1)
+    p = sc.add_prototype(v, f, mat)
+    good = np.hstack([np.eye(3), np.zeros((3, 1))])
+    with pytest.raises(slr_amd.SlrhError):
+        sc.add_instance(p + 1, good)
+    with pytest.raises(slr_amd.SlrhError):
+        sc.add_instance(p, np.zeros((3, 4)))                 # singular
+    bad = good.copy()
+    bad[0, 0] = np.nan
+    with pytest.raises(slr_amd.SlrhError):
+        sc.add_instance(p, bad)
+    with pytest.raises(slr_amd.SlrhError):
+        sc.add_instance(p, good, material=mat + 5)
+    assert sc.add_instance(p, good) == 0
+    assert sc.add_instance(p, good, material=mat) == 1
+    with pytest.raises(slr_amd.SlrhError):
+        sc.instance_transform(2)
+    with pytest.raises(slr_amd.SlrhError):
+        sc.set_compat(4)
+    sc.set_compat(slr_amd.COMPAT_INSTANCE_SHADOW)
+    sc.set_camera(np.eye(4), 1.0, 0.5)
+    sc.build()
+    with pytest.raises(slr_amd.SlrhError):
+        sc.add_instance(p, good)                              # after build
+
+
+def test_flattened_scene_places_the_same_triangles():
+    desc = scenes.instanced_cornell(16, 16, n_instances=4, seed=2, uniform_scale=False)
+    flat, maps, first = scenes.flatten_instances(desc)
+    assert len(flat["meshes"]) == len(desc["meshes"]) + 4
+    for k, inst in enumerate(desc["instances"]):
+        pr = desc["prototypes"][inst["proto"]]
+        m = np.asarray(inst["l2w"], np.float64)
+        want = np.asarray(pr["pos"], np.float64) @ m[:, :3].T + m[:, 3]
+        np.testing.assert_allclose(flat["meshes"][len(desc["meshes"]) + k]["pos"], want, rtol=1e-6, atol=1e-6)
+        assert len(maps[k]) == len(pr["idx"])
