@@ -126,11 +126,13 @@ SLRH_API int slrh_prototype_add(slrh_scene* s, uint32_t nverts, const float* pos
  * back by the transform. Instance ids count from 0 in call order. */
 SLRH_API int slrh_instance_add(slrh_scene* s, int proto_id, const float* l2w, int material, int* inst_id);
 
-/* The transform of instance inst_id as stored (12 floats each, row-major
- * 3x4; either pointer may be NULL): the object-to-world rows given, and the
- * world-to-object rows rays are transformed with, computed as the reference
- * inverts a StaticTransform's matrix (Matrix4x4 invert,
- * libSLR/BasicTypes/Matrix4x4.cpp:36-87). */
+/* The transform of instance inst_id as stored (either pointer may be NULL):
+ * l2w, the 12 object-to-world floats given (row-major 3x4), and w2l, the 16
+ * world-to-object floats (row-major 4x4) rays are transformed with,
+ * computed as the reference inverts a StaticTransform's matrix (Matrix4x4
+ * invert, libSLR/BasicTypes/Matrix4x4.cpp:36-87). Its bottom row carries the
+ * elimination's FP32 rounding residue; a point is divided by the w it gives
+ * when w != 1 (Matrix4x4 * Point3D, Matrix4x4.h:68-74). */
 SLRH_API int slrh_instance_get(const slrh_scene* s, int inst_id, float* l2w, float* w2l);
 
 enum slrh_compat_flags {

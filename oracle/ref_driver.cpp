@@ -433,7 +433,7 @@ int refd_qbvh_dump(void* sp, float* boxes, uint32_t* info, int32_t* refs) {
     return (int)n.size();
 }
 
-// World-to-object rows (12 floats, row-major 3x4) of instance k as
+// World-to-object matrix (16 floats, row-major 4x4) of instance k as
 // StaticTransform computes its inverse (libSLR/Core/transform.h:38).
 int refd_instance_inverse(void* sp, int k, float* rows) {
     RefScene* s = (RefScene*)sp;
@@ -446,6 +446,7 @@ int refd_instance_inverse(void* sp, int k, float* rows) {
         rows[c] = col.x;
         rows[4 + c] = col.y;
         rows[8 + c] = col.z;
+        rows[12 + c] = col.w;
     }
     return 0;
 }

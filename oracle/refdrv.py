@@ -151,11 +151,11 @@ class RefScene:
         self.L.refd_qbvh_dump(self.h, _p(boxes), _p(info), _p(refs))
         return boxes, info, refs
 
-    def instance_inverse(self, k):
-        rows = np.zeros(12, np.float32)
+    def instance_inverse(self, k, full=False):
+        rows = np.zeros(16, np.float32)
         if self.L.refd_instance_inverse(self.h, k, _p(rows)):
             raise IndexError(k)
-        return rows.reshape(3, 4)
+        return rows.reshape(4, 4) if full else rows.reshape(4, 4)[:3].copy()
 
     def intersect(self, rays, accel=0, with_inst=False):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

@@ -271,11 +271,11 @@ class Scene:
         return self._id(self.lib.slrh_instance_add, "slrh_instance_add", int(proto), _ptr(m), int(material))
 
     def instance_transform(self, inst):
-        """(object-to-world rows, world-to-object rows), each 3x4 float32."""
+        """(object-to-world rows 3x4, world-to-object matrix 4x4), float32."""
         a = np.zeros(12, np.float32)
-        b = np.zeros(12, np.float32)
+        b = np.zeros(16, np.float32)
         self.ctx.check(self.lib.slrh_instance_get(self.h, int(inst), _ptr(a), _ptr(b)), "slrh_instance_get")
-        return a.reshape(3, 4), b.reshape(3, 4)
+        return a.reshape(3, 4), b.reshape(4, 4)
 
     def set_compat(self, flags):
         self.ctx.check(self.lib.slrh_scene_compat(self.h, int(flags)), "slrh_scene_compat")

@@ -385,7 +385,7 @@ int slrh_instance_add(slrh_scene* s, int proto_id, const float* l2w, int materia
     m[0][3] = m[1][3] = m[2][3] = 0.0f;
     m[3][3] = 1.0f;
     if (!invertMatrix(m, inv)) return fail(s->ctx, SLRH_ERR_INVALID, "instance_add: singular transform");
-    for (int r = 0; r < 3; ++r)
+    for (int r = 0; r < 4; ++r)
         for (int c = 0; c < 4; ++c) {
             if (!std::isfinite(inv[c][r])) return fail(s->ctx, SLRH_ERR_INVALID, "instance_add: singular transform");
             in.w2l[4 * r + c] = inv[c][r];

@@ -82,6 +82,7 @@ struct DevCamera {
 };
 
 constexpr uint32_t kNoInstance = 0xFFFFFFFFu;
+constexpr uint32_t kInstXfRows = 8;
 
 struct DevScene {
     const float4* nodes;
@@ -101,8 +102,8 @@ struct DevScene {
     uint32_t numPrims;
     float lightImportance;    // SurfaceObjectAggregate::importance()
     uint32_t numNodes;
-    // instances: per instance 3 rows object-to-world, then 3 rows
-    // world-to-object (affine, row-major, translation in w)
+    // instances: kInstXfRows float4 each: 3 rows object-to-world (affine,
+    // row-major, translation in w), then the 4 rows of world-to-object
     const float4* instXf;
     const int32_t* instMat;      // material replacing the prototype's, -1: none
     uint32_t numInstances;

@@ -25,11 +25,13 @@ struct Prototype {
 };
 
 // A placed copy of a prototype: object-to-world affine transform (3 rows of
-// 4, row-major), its inverse, and a material replacing the prototype's
+// 4, row-major), its inverse as the reference computes it (4 rows: the FP32
+// elimination leaves rounding residue in the bottom row, and points are
+// divided by the w it yields), and a material replacing the prototype's
 // materials (-1: keep them).
 struct Instance {
     uint32_t proto = 0;
-    float l2w[12], w2l[12];
+    float l2w[12], w2l[16];
     int32_t material = -1;
 };
 

@@ -331,14 +331,17 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         }
     }
     // per instance: object-to-world rows, world-to-object rows; material override
-    std::vector<float4> instXf(std::max<size_t>(1, 6 * sd.instances.size()), make_float4(0, 0, 0, 0));
+    std::vector<float4> instXf(std::max<size_t>(1, kInstXfRows * sd.instances.size()), make_float4(0, 0, 0, 0));
     std::vector<int32_t> instMat(std::max<size_t>(1, sd.instances.size()), -1);
     for (size_t k = 0; k < sd.instances.size(); ++k) {
         const Instance& in = sd.instances[k];
         for (int r = 0; r < 3; ++r) {
-            instXf[6 * k + r] = make_float4(in.l2w[4 * r], in.l2w[4 * r + 1], in.l2w[4 * r + 2], in.l2w[4 * r + 3]);
-            instXf[6 * k + 3 + r] = make_float4(in.w2l[4 * r], in.w2l[4 * r + 1], in.w2l[4 * r + 2], in.w2l[4 * r + 3]);
+            instXf[kInstXfRows * k + r] =
+                make_float4(in.l2w[4 * r], in.l2w[4 * r + 1], in.l2w[4 * r + 2], in.l2w[4 * r + 3]);
         }
+        for (int r = 0; r < 4; ++r)
+            instXf[kInstXfRows * k + 3 + r] =
+                make_float4(in.w2l[4 * r], in.w2l[4 * r + 1], in.w2l[4 * r + 2], in.w2l[4 * r + 3]);
         instMat[k] = in.material;
     }
     // DiscreteDistribution1D over unit importances with a compensated sum
@@ -1168,7 +1171,7 @@ __device__ __forceinline__ void worldVerts(const DevScene& sc, uint32_t prim, in
     *q1 = xyz(sc.vpos[pv.y]);
     *q2 = xyz(sc.vpos[pv.z]);
     if (inst >= 0) {
-        const float4* X = sc.instXf + 6u * (uint32_t)inst;
+        const float4* X = sc.instXf + kInstXfRows * (uint32_t)inst;
         *q0 = xfPoint(X, *q0);
         *q1 = xfPoint(X, *q1);
         *q2 = xfPoint(X, *q2);
@@ -1199,7 +1202,7 @@ __device__ __forceinline__ SurfPt surfaceAt(const DevScene& sc, uint32_t prim, f
     s.mat = pv.w;
     s.prim = prim;
     if (inst >= 0) {
-        const float4* X = sc.instXf + 6u * (uint32_t)inst;
+        const float4* X = sc.instXf + kInstXfRows * (uint32_t)inst;
         s.sx = normalize3(xfVector(X, s.sx));
         s.sy = normalize3(xfVector(X, s.sy));
         s.sz = normalize3(xfNormal(X, s.sz));
@@ -1471,7 +1474,7 @@ void k_shade(DevScene sc, PathState ps) {
                         float3 ln = normalize3(lb0 * xyz(sc.vnrm[lv.x]) + lb1 * xyz(sc.vnrm[lv.y]) +
                                                lb2 * xyz(sc.vnrm[lv.z]));
                         if (linst >= 0) {
-                            ln = normalize3(xfNormal(sc.instXf + 6u * (uint32_t)linst, ln));
+                            ln = normalize3(xfNormal(sc.instXf + kInstXfRows * (uint32_t)linst, ln));
                             if (sc.instMat[linst] >= 0) lv.w = (uint32_t)sc.instMat[linst];
                         }
                         const float3 lp = lb0 * q0 + lb1 * q1 + lb2 * q2;

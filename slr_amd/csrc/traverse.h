@@ -283,7 +283,7 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
     // hit's instance, and the world-to-object rows requested on entry
     int32_t inst = -1, hitInst = -1;
     uint32_t instRoot = 0, instCls = kNoClassOverride;
-    float4 x0 = make_float4(0, 0, 0, 0), x1 = x0, x2 = x0;
+    float4 x0 = make_float4(0, 0, 0, 0), x1 = x0, x2 = x0, x3 = x0;
     int sp = 0;
     uint32_t ni = kNoNode, tcur = 0, tend = 0, leafPend = 0;
     uint32_t v[4] = {kNoChild, kNoChild, kNoChild, kNoChild};
@@ -396,9 +396,19 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
                         // by their linear part (StaticTransform * Ray)
                         if (!(sc.compatInstShadow && anyHit)) {
                             const float3 o = r.org(), d = r.dir();
-                            r.o = make_float4(x0.x * o.x + x0.y * o.y + x0.z * o.z + x0.w,
-                                              x1.x * o.x + x1.y * o.y + x1.z * o.z + x1.w,
-                                              x2.x * o.x + x2.y * o.y + x2.z * o.z + x2.w, r.o.w);
+                            // Matrix4x4 * Point3D: rows dotted with (o, 1), then
+                            // divided by w unless it is exactly one
+                            float px = x0.x * o.x + x0.y * o.y + x0.z * o.z + x0.w;
+                            float py = x1.x * o.x + x1.y * o.y + x1.z * o.z + x1.w;
+                            float pz = x2.x * o.x + x2.y * o.y + x2.z * o.z + x2.w;
+                            const float pw = x3.x * o.x + x3.y * o.y + x3.z * o.z + x3.w;
+                            if (pw != 1.0f) {
+                                const float rw = 1.0f / pw;
+                                px *= rw;
+                                py *= rw;
+                                pz *= rw;
+                            }
+                            r.o = make_float4(px, py, pz, r.o.w);
                             r.d = make_float4(x0.x * d.x + x0.y * d.y + x0.z * d.z, x1.x * d.x + x1.y * d.y + x1.z * d.z,
                                               x2.x * d.x + x2.y * d.y + x2.z * d.z, r.d.w);
                             r.inv = rayInv(r.dir());
@@ -494,10 +504,11 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
                     inst = (int32_t)(__float_as_uint(a.w) & ~kRefInstBit);
                     instRoot = __float_as_uint(e1.w);
                     instCls = __float_as_uint(e2.w);
-                    const float4* xr = sc.instXf + 6u * (uint32_t)inst + 3u;
+                    const float4* xr = sc.instXf + kInstXfRows * (uint32_t)inst + 3u;
                     x0 = xr[0];
                     x1 = xr[1];
                     x2 = xr[2];
+                    x3 = xr[3];
                     ni = kEnterInst;
                 } else {
                     float t, b1, b2;
