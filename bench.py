@@ -51,6 +51,17 @@ def algorithmic_bytes(ext_rays, shadow_rays, nodes, tris):
     return 52.0 * ext_rays + 36.0 * shadow_rays + 128.0 * nodes + 40.0 * tris
 
 
+def measured_traffic(config):
+    """HBM bytes per k_trace launch measured with rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy), as
+    committed under profiles/ for this workload; None for other workloads."""
+    path = os.path.join(ROOT, "profiles", "k_trace_%s_traffic.json" % config)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return round(json.load(f)["traffic_bytes_per_dispatch"])
+
+
 def calibration_copy(nbytes):
     """One device-to-device hipMemcpy of nbytes (the runtime's copy kernel),
     straight through the HIP runtime."""
@@ -210,7 +221,7 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", default="960x540x2")
+    ap.add_argument("--cpu-sample", default="1920x1080x8")
     ap.add_argument("--out", default="")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known traffic that "
@@ -308,7 +319,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.config),
             "avg_launch_ms": round(tr_avg * 1e3, 4),
             "launches": int(tr_launches),
             "bytes_per_launch": round(tr_bytes / tr_launches),
