@@ -231,7 +231,10 @@ typedef struct slrh_render_settings {
     uint32_t integrator;      /* slrh_integrator */
     uint32_t slots_per_pixel; /* live paths per pixel (0 = automatic) */
     uint32_t flags;           /* SLRH_FLAG_* */
-    uint32_t tune_trace;      /* traversal kernel variant + 1 (0 = default) */
+    uint32_t tune_trace;      /* traversal kernel variant + 1; 0 = automatic: the first render
+                                 of a scene with a pool of >= 2^20 slots renders one pilot
+                                 sample per slot with each variant (scratch film) and keeps
+                                 the fastest per traced ray for that scene and pool size */
     uint32_t reserved;        /* must be 0 */
 } slrh_render_settings;
 
@@ -260,13 +263,14 @@ SLRH_API int slrh_film_read(const slrh_film* f, int format, double scale, void* 
 
 /* Renders into `film` (accumulating onto what it holds). Replaces
  * PTRenderer::render (libSLR/Renderer/PTRenderer.cpp:26-101). stats, if not
- * NULL, receives 15 doubles: [0] rays traced, [1] samples, [2] device seconds,
+ * NULL, receives 16 doubles: [0] rays traced, [1] samples, [2] device seconds,
  * [3] non-finite samples dropped, [4] wavefront iterations, [5] traversal
  * kernel seconds (extension and shadow rays traced together), [6] traversal
  * launches, [7] extension rays, [8] shading kernel seconds, [9] seconds of
  * the other kernels (regeneration, compaction), [10] shadow rays, [11] QBVH
  * nodes visited and [12] triangles tested by extension rays, [13] nodes and
- * [14] triangles by shadow rays. Kernel seconds are 0 unless flags bit 0. */
+ * [14] triangles by shadow rays, [15] traversal variant used. Kernel seconds
+ * are 0 unless flags bit 0. */
 SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
                          double* stats);
 
@@ -303,7 +307,7 @@ typedef struct slrh_server slrh_server;
 
 typedef struct slrh_server_config {
     uint64_t path_state_bytes;   /* device memory for path state; bounds the live paths (~414 B each) */
-    uint32_t tune_trace;         /* traversal kernel variant + 1 (0 = default) */
+    uint32_t tune_trace;         /* traversal kernel variant + 1 (0 = the one tuned by a render) */
     uint32_t reserved;           /* must be 0 */
 } slrh_server_config;
 

@@ -429,7 +429,7 @@ def trace_variants():
 
 STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "trace_seconds", "trace_launches",
              "extend_rays", "shade_seconds", "other_seconds", "shadow_rays", "nodes_visited", "tris_tested",
-             "nodes_shadow", "tris_shadow"]
+             "nodes_shadow", "tris_shadow", "trace_variant"]
 
 
 def _fill_frame(fr, f, default_spp):

@@ -684,6 +684,7 @@ void copyStats(const RenderStats& st, double* stats) {
     stats[8] = st.shadeSeconds; stats[9] = st.otherSeconds; stats[10] = st.shadowRays;
     stats[11] = st.nodesVisited; stats[12] = st.trisTested;
     stats[13] = st.nodesShadow; stats[14] = st.trisShadow;
+    stats[15] = st.traceVariant;
 }
 }  // namespace
 
@@ -699,7 +700,7 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     RenderConfig cfg{set->width, set->height, set->spp, set->max_depth, set->seed, set->slots_per_pixel,
                      (set->flags & 1u) != 0};
     if (set->reserved != 0) return fail(s->ctx, SLRH_ERR_INVALID, "reserved settings field must be 0");
-    cfg.traceVariant = set->tune_trace == 0 ? 0 : (int)set->tune_trace - 1;
+    cfg.traceVariant = set->tune_trace == 0 ? -1 : (int)set->tune_trace - 1;
     if (cfg.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     cfg.useGraph = (set->flags & SLRH_FLAG_NO_GRAPH) == 0;
     RenderStats st;
@@ -748,7 +749,7 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     opt.slotsPerPixel = set->slots_per_pixel;
     opt.timeKernels = (set->flags & SLRH_FLAG_TIME_KERNELS) != 0;
     opt.useGraph = (set->flags & SLRH_FLAG_NO_GRAPH) == 0;
-    opt.traceVariant = set->tune_trace == 0 ? 0 : (int)set->tune_trace - 1;
+    opt.traceVariant = set->tune_trace == 0 ? -1 : (int)set->tune_trace - 1;
     if (opt.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     RenderStats st;
@@ -765,7 +766,8 @@ int slrh_server_create(slrh_scene* s, const slrh_server_config* cfg, slrh_server
     if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
     if (cfg->reserved != 0) return fail(s->ctx, SLRH_ERR_INVALID, "reserved server field must be 0");
     if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
-    const int tv = cfg->tune_trace == 0 ? 0 : (int)cfg->tune_trace - 1;
+    // 0: the variant the autotuner chose for this scene in a render (0 if none)
+    const int tv = cfg->tune_trace == 0 ? tunedTraceVariant(s->dev) : (int)cfg->tune_trace - 1;
     if (tv >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     std::unique_ptr<slrh_server> sv(new (std::nothrow) slrh_server());
     if (!sv) return fail(s->ctx, SLRH_ERR_NOMEM, "out of host memory");

@@ -73,7 +73,7 @@ struct TraceArgs {
 struct RenderConfig {
     uint32_t width, height, spp, maxDepth, seed, slotsPerPixel;
     bool timeKernels;
-    int traceVariant = 0;   // traversal kernel variant (see traceVariant)
+    int traceVariant = 0;   // traversal kernel variant (see traceVariant), -1: autotuned
     bool useGraph = true;
 };
 
@@ -84,6 +84,7 @@ struct RenderStats {
     double nodesVisited = 0, trisTested = 0;   // extension rays
     double nodesShadow = 0, trisShadow = 0;    // shadow rays
     double overflow = 0;
+    double traceVariant = 0;   // traversal variant the render ran with
 };
 
 int deviceInit(int device, std::string* err);
@@ -114,8 +115,10 @@ struct RenderOptions {
     uint32_t slotsPerPixel = 0;   // 0: automatic
     bool timeKernels = false;
     bool useGraph = true;         // replay recorded iteration blocks (ignored when timing kernels)
-    int traceVariant = 0;
+    int traceVariant = 0;         // -1: chosen by the autotuner
 };
+// The variant the autotuner chose for the scene (0 before any tuning).
+int tunedTraceVariant(const DeviceScene* ds);
 // Renders all frames in one wavefront (one slot pool, shared kernel launches).
 int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOptions& opt, RenderStats* st,
                std::string* err);

@@ -124,6 +124,9 @@ struct DeviceScene {
     CameraParams camera;
     size_t bytes = 0;
     std::shared_ptr<struct SlotPool> pool;   // wavefront state kept between renders
+    // traversal variant chosen by the autotuner for pools of tunedSlots slots
+    int tunedVariant = -1;
+    uint32_t tunedSlots = 0;
 };
 
 struct DeviceFilm {
@@ -1958,6 +1961,70 @@ int poolSamples(SlotPool* p, double* samples, double* rays, std::string* err) {
 // 384 B per slot, 6.4 GB at 16M.
 constexpr uint32_t kTargetSlots = 16u << 20;
 
+// Traversal autotuner. For each variant a pilot renders one sample per slot
+// of the first job (its camera and resolution, as many samples per pixel as
+// the job has slot planes, into a scratch film): the same pool, the same
+// wavefront width and the same scene, so the traversal launches see the
+// real mix of camera, bounce and shadow rays. Each variant's traversal
+// kernel time (HIP events around every launch) per traced ray is compared
+// and the fastest is kept for the scene and pool size. Pilots change no
+// film the caller owns; the variants give identical hits, so the choice
+// changes timing only. Pools under kTuneMinSlots keep variant 0.
+constexpr uint32_t kTuneMinSlots = 1u << 20;
+
+static int tuneTrace(DeviceScene* ds, SlotPool& p, const JobSpec& first, uint32_t planes, uint32_t base,
+                     std::string* err) {
+    DeviceFilm* scratch = nullptr;
+    int r = filmCreate(first.width, first.height, &scratch, err);
+    if (r) return r;
+    std::unique_ptr<DeviceFilm, void (*)(DeviceFilm*)> hold(scratch, filmFree);
+    JobSpec pilot = first;
+    pilot.spp = planes;
+    pilot.film = scratch;
+    std::vector<hipEvent_t> evs;
+    auto cleanup = [&]() { for (hipEvent_t e : evs) (void)hipEventDestroy(e); };
+    double best = 0;
+    int bestV = 0;
+    for (int v = 0; v < kNumVariants; ++v) {
+        if ((r = poolReset(p, err))) { cleanup(); return r; }
+        if ((r = poolAdmit(&p, 0, pilot, planes, base, err))) { cleanup(); return r; }
+        hipLaunchKernelGGL(k_compact<0>, dim3(p.n / kCompactSlots), dim3(256), 0, p.stream, p.ps);
+        std::vector<IterEvents> itEv;
+        for (;;) {
+            for (uint32_t k = 0; k < kGraphIters; ++k) {
+                itEv.push_back({});
+                for (auto& x : itEv.back()) {
+                    (void)hipEventCreate(&x);
+                    evs.push_back(x);
+                }
+                launchIteration(p, v, &itEv.back());
+            }
+            if ((r = readCounters(p, err))) { cleanup(); return r; }
+            if (p.hostCtr[C_OVERFLOW] || (p.hostCtr[C_DONE] == 0 && p.hostCtr[C_PENDING] == 0)) break;
+        }
+        if ((r = finishJob(p, p.runs[0], err))) { cleanup(); return r; }
+        HIP_TRY(hipStreamSynchronize(p.stream));
+        unsigned long long stat[kNumStat];
+        HIP_TRY(hipMemcpy(stat, p.stat.p, sizeof(stat), hipMemcpyDeviceToHost));
+        double ms = 0;
+        for (auto& e : itEv) {
+            float t = 0;
+            (void)hipEventElapsedTime(&t, e[1], e[2]);
+            ms += t;
+        }
+        const double rays = (double)(stat[S_RAYS_EXT] + stat[S_RAYS_SHADOW]);
+        const double perRay = rays > 0 ? ms / rays : 0;
+        if (v == 0 || perRay < best) { best = perRay; bestV = v; }
+        for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+        evs.clear();
+    }
+    ds->tunedVariant = bestV;
+    ds->tunedSlots = p.n;
+    return 0;
+}
+
+int tunedTraceVariant(const DeviceScene* ds) { return ds->tunedVariant >= 0 ? ds->tunedVariant : 0; }
+
 int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOptions& opt, RenderStats* st,
                std::string* err) {
     if (n == 0) return 0;
@@ -1986,9 +2053,15 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
         ds->pool = std::shared_ptr<SlotPool>(np.release());
     }
     SlotPool& p = *ds->pool;
-    const int tv = opt.traceVariant >= 0 && opt.traceVariant < kNumVariants ? opt.traceVariant : 0;
-    int r = poolReset(p, err);
-    if (r) return r;
+    int r = 0;
+    int tv = opt.traceVariant >= 0 && opt.traceVariant < kNumVariants ? opt.traceVariant : 0;
+    if (opt.traceVariant < 0) {
+        // automatic: tuned once per scene and pool size
+        if (p.n >= kTuneMinSlots && (ds->tunedVariant < 0 || ds->tunedSlots != p.n))
+            if ((r = tuneTrace(ds, p, specs[0], planes[0], bases[0], err))) return r;
+        tv = p.n >= kTuneMinSlots && ds->tunedSlots == p.n ? ds->tunedVariant : 0;
+    }
+    if ((r = poolReset(p, err))) return r;
 
     std::vector<hipEvent_t> evs;
     auto newEvent = [&]() { hipEvent_t e; (void)hipEventCreate(&e); evs.push_back(e); return e; };
@@ -2053,6 +2126,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
         st->shadeSeconds = sh * 1e-3;
         st->otherSeconds = ot * 1e-3;
         st->traceLaunches = (double)itEv.size();
+        st->traceVariant = tv;
     }
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     if (p.hostCtr[C_OVERFLOW]) return SLRH_ERR_CAPACITY_CODE;

@@ -94,3 +94,36 @@ def test_batch_rejects_film_size_mismatch(gpu_ctx):
             arr[i].film = f["film"].h
         s = slr_amd._Settings(0, 0, 1, 0, 0, 0, 0, 0, 0, 0)
         sc.ctx.check(sc.lib.slrh_render_batch(sc.h, arr, 2, slr_amd.ctypes.byref(s), None), "slrh_render_batch")
+
+
+def test_every_traversal_variant_gives_the_same_film(gpu_ctx):
+    """The variants differ in stack depth, refill, LDS treelet and step
+    fusion only: hits are the reference's in every one, so films are equal."""
+    W, H = 64, 48
+    desc = scenes.cornell_icosphere(W, H, subdiv=4)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    ref = None
+    for v in range(len(slr_amd.trace_variants())):
+        f = slr_amd.Film(gpu_ctx, W, H)
+        st = slr_amd.PTRenderer(4).render(sc, f, seed=3, slots_per_pixel=2, tune_trace=v + 1)
+        assert int(st["trace_variant"]) == v
+        bits = f.strata().view(np.uint64).copy()
+        if ref is None:
+            ref = bits
+        assert np.array_equal(bits, ref), v
+
+
+def test_autotuner_picks_a_variant_and_keeps_the_film(gpu_ctx):
+    W, H = 1024, 1024      # 2^20 slots at one slot per pixel: large enough to tune
+    desc = scenes.cornell_box(W, H)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    a = slr_amd.Film(gpu_ctx, W, H)
+    st = slr_amd.PTRenderer(2).render(sc, a, seed=5, slots_per_pixel=1)
+    v = int(st["trace_variant"])
+    assert 0 <= v < len(slr_amd.trace_variants())
+    # the choice is kept for the scene, and an explicit variant gives the same film
+    st2 = slr_amd.PTRenderer(2).render(sc, slr_amd.Film(gpu_ctx, W, H), seed=5, slots_per_pixel=1)
+    assert int(st2["trace_variant"]) == v
+    b = slr_amd.Film(gpu_ctx, W, H)
+    slr_amd.PTRenderer(2).render(sc, b, seed=5, slots_per_pixel=1, tune_trace=(v + 1) % len(slr_amd.trace_variants()) + 1)
+    assert np.array_equal(a.strata().view(np.uint64), b.strata().view(np.uint64))
