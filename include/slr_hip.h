@@ -140,8 +140,34 @@ enum slrh_compat_flags {
      * TransformedSurfaceObject::testVisibility does (libSLR/Core/
      * surface_object.cpp:363-369); off: the ray is transformed as for
      * closest hits */
-    SLRH_COMPAT_INSTANCE_SHADOW = 1
+    SLRH_COMPAT_INSTANCE_SHADOW = 1,
+    /* SLRH_VOL_PT renders treat a light sample in the plane of the shading
+     * surface as the reference VolumetricPTRenderer does: its next-event
+     * estimate has no cosine guard (libSLR/Renderer/VolumetricPTRenderer.cpp:
+     * 190-231), the BSDF's shading-normal correction is then 0/0
+     * (libSLR/Core/directional_distribution_functions.h:322-325), and the NaN
+     * drops the whole sample (VolumetricPTRenderer.cpp:130-138). Such samples
+     * are counted in the non-finite statistic. Off: the estimate is zero, as
+     * in SLRH_PT (PTRenderer.cpp:199), and a scene without media renders
+     * exactly as with SLRH_PT. */
+    SLRH_COMPAT_VOLPT_TANGENT_NEE = 2
 };
+/* A heterogeneous medium on a density lattice: region bounds[6] (min xyz,
+ * max xyz), nx*ny*nz densities (x fastest, then y, then z; trilinear
+ * between lattice points spanning the region), base scattering and
+ * extinction spectra (the extinction constant over wavelength) and the
+ * Henyey-Greenstein asymmetry g. Replaces DensityGridMediumNode(region,
+ * base_sigma_s, base_sigma_e, density_grid, nx, ny, nz,
+ * HenyeyGreensteinScatteringMediumMaterial(g)) (libSLR/Scene/medium_nodes.h:
+ * 33-46). Free paths are sampled against a majorant grid of the maximum
+ * density over blocks of 8^3 lattice cells (the reference uses trilinear
+ * supervoxels with a per-cell correction, DensityGridMediumDistribution.cpp:
+ * 45-205; both bound the density, so both are unbiased). One medium per
+ * scene; it takes part in SLRH_VOL_PT renders. */
+SLRH_API int slrh_medium_density_grid(slrh_scene* s, const float* bounds, const float* density, uint32_t nx,
+                                      uint32_t ny, uint32_t nz, int sigma_s, int sigma_e, float g,
+                                      int* medium_id);
+
 /* Reproduces reference behaviours that differ from the correct result, for
  * parity checks (flags from slrh_compat_flags). Call before slrh_scene_build. */
 SLRH_API int slrh_scene_compat(slrh_scene* s, uint32_t flags);
@@ -214,9 +240,28 @@ SLRH_API int slrh_intersect(slrh_scene* s, const slrh_rays* rays, const slrh_hit
 SLRH_API int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* visibility, uint32_t n,
                            int mem, void* stream);
 
+enum slrh_medium_query_mode {
+    SLRH_MEDIUM_TRANSMITTANCE = 0,  /* one ratio-tracking estimate of the segment's transmittance */
+    SLRH_MEDIUM_FREE_PATH = 1       /* one delta-tracking scattering distance, tmax if none */
+};
+/* Ray batches through the scene's medium (surfaces are not tested): per ray
+ * an unbiased transmittance estimate of [tmin, tmax], as Scene::testVisibility
+ * takes from MediumObjectAggregate::evaluateTransmittance (libSLR/Core/
+ * medium_object.cpp:348-399), or a sampled free-path distance, as
+ * Scene::interact takes from MediumObjectAggregate::interact (:300-347).
+ * Ray i uses its own random stream seeded from (seed, i). out: n floats. */
+SLRH_API int slrh_medium_query(slrh_scene* s, const slrh_rays* rays, uint32_t n, int mode, uint32_t seed,
+                               float* out, int mem, void* stream);
+
 /* ------------------------------------------------------------ rendering */
 
-enum slrh_integrator { SLRH_PT = 0 };
+/* SLRH_PT: PTRenderer (libSLR/Renderer/PTRenderer.cpp), media ignored.
+ * SLRH_VOL_PT: VolumetricPTRenderer (libSLR/Renderer/VolumetricPTRenderer.cpp:
+ * 146-296): free paths sampled by delta tracking in the scene's medium,
+ * scattering by its phase function, light samples attenuated by the
+ * medium's transmittance (ratio tracking), no fixed path-length cap (Russian
+ * roulette; max_depth still applies when set). */
+enum slrh_integrator { SLRH_PT = 0, SLRH_VOL_PT = 1 };
 
 enum slrh_render_flags {
     SLRH_FLAG_TIME_KERNELS = 1,   /* time the kernel groups with HIP events (stats 5, 8, 9) */
@@ -263,14 +308,15 @@ SLRH_API int slrh_film_read(const slrh_film* f, int format, double scale, void* 
 
 /* Renders into `film` (accumulating onto what it holds). Replaces
  * PTRenderer::render (libSLR/Renderer/PTRenderer.cpp:26-101). stats, if not
- * NULL, receives 16 doubles: [0] rays traced, [1] samples, [2] device seconds,
+ * NULL, receives 18 doubles: [0] rays traced, [1] samples, [2] device seconds,
  * [3] non-finite samples dropped, [4] wavefront iterations, [5] traversal
  * kernel seconds (extension and shadow rays traced together), [6] traversal
  * launches, [7] extension rays, [8] shading kernel seconds, [9] seconds of
  * the other kernels (regeneration, compaction), [10] shadow rays, [11] QBVH
  * nodes visited and [12] triangles tested by extension rays, [13] nodes and
- * [14] triangles by shadow rays, [15] traversal variant used. Kernel seconds
- * are 0 unless flags bit 0. */
+ * [14] triangles by shadow rays, [15] traversal variant used, [16] scattering
+ * events in the medium and [17] light samples attenuated by it (SLRH_VOL_PT).
+ * Kernel seconds are 0 unless flags bit 0. */
 SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
                          double* stats);
 

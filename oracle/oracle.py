@@ -47,6 +47,7 @@ def lib():
             "orc_scene_new": (vp, [i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp,
                                    i32, vp, vp, f32, f32, f32, f32, f32]),
             "orc_scene_free": (None, [vp]),
+            "orc_scene_compat": (None, [vp, i32]),
             "orc_intersect": (i32, [vp, i32, vp, vp, vp, vp, vp]),
             "orc_visibility": (i32, [vp, i32, vp, vp]),
             "orc_render": (i64, [vp, u32, u32, u32, u32, u32, u32, u32, u32, vp, vp]),
@@ -190,6 +191,7 @@ class OracleScene:
                                  _p(self.n), len(self.pool), _p(self.pool), _p(self.l2w), cam["aspect"], cam["fov_y"],
                                  cam.get("lens_radius", 0.0), cam.get("img_dist", 1.0), cam.get("obj_dist", 1.0))
         self.L = L
+        self.tangent_nee = bool(desc.get("compat_volpt_tangent_nee"))
 
     def intersect(self, rays):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
@@ -205,9 +207,15 @@ class OracleScene:
         self.L.orc_visibility(self.h, len(rays), _p(rays), _p(vis))
         return vis
 
-    def render(self, W, H, spp, seed, K=1, max_depth=0, slots=None, threads=1):
+    def render(self, W, H, spp, seed, K=1, max_depth=0, slots=None, threads=1, volumetric=False):
         """Film (H, W, 16) float64 of raw sums, rays traced. threads > 1 splits
-        the slots over a thread pool (the C call releases the GIL)."""
+        the slots over a thread pool (the C call releases the GIL).
+        volumetric: VolumetricPTRenderer over a scene without media (no path
+        length cap; with the scene's compat_volpt_tangent_nee, the reference's
+        NaN for a light sample in the shading plane)."""
+        self.L.orc_scene_compat(self.h, 1 if volumetric and self.tangent_nee else 0)
+        if volumetric and max_depth == 0:
+            max_depth = 0xFFFFFFFF
         film = np.zeros((H, W, 16), np.float64)
         nonfin = np.zeros(1, np.int64)
         total = K * W * H

@@ -83,6 +83,10 @@
 #include <libSLR/BSDF/OrenNayerBRDF.h>
 #include <libSLR/BSDF/microfacet_bsdfs.h>
 #include <libSLR/Renderer/PTRenderer.h>
+#include <libSLR/Renderer/VolumetricPTRenderer.h>
+#include <libSLR/Scene/medium_nodes.h>
+#include <libSLR/MediumDistribution/DensityGridMediumDistribution.h>
+#include <libSLR/MediumMaterial/basic_medium_materials.h>
 #include <libSLR/RNG/XORShiftRNG.h>
 // SBVH and QBVH are declared with the class keyword and no access label, so
 // their members default to private; spelling them as structs (same layout,
@@ -146,6 +150,14 @@ struct RefScene {
     Texture2DMapping envMapping;
     std::unique_ptr<ImageSpectrumTexture> envTex;
     float envScale = 1.0f;
+
+    // heterogeneous medium (DensityGridMediumNode) and the integrator
+    bool hasMedium = false;
+    float medBounds[6];
+    uint32_t medN[3];
+    std::vector<std::vector<float>> medGrid;   // per z slice, x fastest
+    float medSigmaS = 0, medSigmaE = 0, medG = 0;
+    int integrator = 0;   // 0 PTRenderer, 1 VolumetricPTRenderer
 
     // accelerator for ray batches
     bool built = false;
@@ -433,6 +445,81 @@ int refd_qbvh_dump(void* sp, float* boxes, uint32_t* info, int32_t* refs) {
     return (int)n.size();
 }
 
+// A DensityGridMediumNode over bounds[6] with nx*ny*nz densities (x fastest,
+// then y, then z), constant base sigma_s / sigma_e and a Henyey-Greenstein
+// phase function of asymmetry g (libSLR/Scene/medium_nodes.h:33-46).
+int refd_set_medium(void* sp, const float* bounds, const float* density, int nx, int ny, int nz, float sigmaS,
+                    float sigmaE, float g) {
+    RefScene* s = (RefScene*)sp;
+    if (s->built || s->hasMedium || nx < 2 || ny < 2 || nz < 2) return -1;
+    memcpy(s->medBounds, bounds, sizeof(s->medBounds));
+    s->medN[0] = nx; s->medN[1] = ny; s->medN[2] = nz;
+    s->medGrid.assign(nz, std::vector<float>());
+    for (int z = 0; z < nz; ++z)
+        s->medGrid[z].assign(density + (size_t)z * nx * ny, density + (size_t)(z + 1) * nx * ny);
+    s->medSigmaS = sigmaS; s->medSigmaE = sigmaE; s->medG = g;
+    s->hasMedium = true;
+    return 0;
+}
+
+// Probe: density of the scene's medium at n points given in the region's
+// local coordinates (xyz in [0, 1]), through DensityGridMediumDistribution::
+// evaluateExtinctionCoefficient with a unit extinction spectrum (its value is
+// calcDensity, DensityGridMediumDistribution.cpp:369-395, 749-752).
+int refd_medium_density(void* sp, int n, const float* params, float* out) {
+    RefScene* s = (RefScene*)sp;
+    if (!s->hasMedium) return -1;
+    float one = 1.0f;
+    RegularContinuousSpectrum unit(WavelengthLowBound, WavelengthHighBound, &one, 1);
+    BoundingBox3D region(Point3D(s->medBounds[0], s->medBounds[1], s->medBounds[2]),
+                         Point3D(s->medBounds[3], s->medBounds[4], s->medBounds[5]));
+    DensityGridMediumDistribution dist(region, &unit, &unit, s->medGrid, s->medN[0], s->medN[1], s->medN[2]);
+    float pdf;
+    WavelengthSamples wls = WavelengthSamples::createWithEqualOffsets(0.5f, 0.5f, &pdf);
+    for (int i = 0; i < n; ++i) {
+        SampledSpectrum e = dist.evaluateExtinctionCoefficient(Point3D(params[3 * i], params[3 * i + 1], params[3 * i + 2]), wls);
+        out[i] = e[0];
+    }
+    return 0;
+}
+
+// Probe: per ray (8 floats {org, dir, tmin, tmax}) the reference medium's
+// transmittance estimate (mode 0, DensityGridMediumDistribution::
+// evaluateTransmittance, hero wavelength entry) or free-path distance
+// (mode 1, DensityGridMediumDistribution::interact: the scattering distance,
+// tmax if none), with sigma_s / sigma_e of the scene's medium; one
+// IndependentLightPathSampler stream for the whole batch.
+int refd_medium_track(void* sp, int mode, int n, const float* rays, uint32_t seed, float* out) {
+    RefScene* s = (RefScene*)sp;
+    if (!s->hasMedium) return -1;
+    RegularContinuousSpectrum ss(WavelengthLowBound, WavelengthHighBound, &s->medSigmaS, 1);
+    RegularContinuousSpectrum se(WavelengthLowBound, WavelengthHighBound, &s->medSigmaE, 1);
+    BoundingBox3D region(Point3D(s->medBounds[0], s->medBounds[1], s->medBounds[2]),
+                         Point3D(s->medBounds[3], s->medBounds[4], s->medBounds[5]));
+    DensityGridMediumDistribution dist(region, &ss, &se, s->medGrid, s->medN[0], s->medN[1], s->medN[2]);
+    IndependentLightPathSampler sampler(seed);
+    float pdf;
+    WavelengthSamples wls = WavelengthSamples::createWithEqualOffsets(0.5f, 0.5f, &pdf);
+    for (int i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        Ray ray(Point3D(r[0], r[1], r[2]), Vector3D(r[3], r[4], r[5]), 0.0f);
+        RaySegment seg(r[6], r[7]);
+        bool single = false;
+        if (mode == 0) {
+            SampledSpectrum T = dist.evaluateTransmittance(ray, seg, wls, sampler, &single);
+            out[i] = T[wls.selectedLambdaIndex];
+        } else {
+            MediumInteraction mi;
+            SampledSpectrum thr;
+            out[i] = dist.interact(ray, seg, wls, sampler, &mi, &thr, &single) ? mi.getDistance() : r[7];
+        }
+    }
+    return 0;
+}
+
+// 0: PTRenderer, 1: VolumetricPTRenderer for refd_render.
+void refd_set_integrator(void* sp, int integrator) { ((RefScene*)sp)->integrator = integrator; }
+
 // World-to-object matrix (16 floats, row-major 4x4) of instance k as
 // StaticTransform computes its inverse (libSLR/Core/transform.h:38).
 int refd_instance_inverse(void* sp, int k, float* rows) {
@@ -540,6 +627,20 @@ double refd_render(void* sp, int w, int h, int spp, uint32_t seed, int threads, 
     camNode.addChildNode(&cam);
     root.addChildNode(&camNode);
 
+    std::unique_ptr<DensityGridMediumNode> medNode;
+    std::unique_ptr<HenyeyGreensteinScatteringMediumMaterial> medMat;
+    if (s->hasMedium) {
+        BoundingBox3D region(Point3D(s->medBounds[0], s->medBounds[1], s->medBounds[2]),
+                             Point3D(s->medBounds[3], s->medBounds[4], s->medBounds[5]));
+        s->spectra.emplace_back(new RegularContinuousSpectrum(WavelengthLowBound, WavelengthHighBound, &s->medSigmaS, 1));
+        const AssetSpectrum* ss = s->spectra.back().get();
+        s->spectra.emplace_back(new RegularContinuousSpectrum(WavelengthLowBound, WavelengthHighBound, &s->medSigmaE, 1));
+        const AssetSpectrum* se = s->spectra.back().get();
+        medMat.reset(new HenyeyGreensteinScatteringMediumMaterial(s->constFloat(s->medG)));
+        medNode.reset(new DensityGridMediumNode(region, ss, se, s->medGrid, s->medN[0], s->medN[1], s->medN[2],
+                                                medMat.get()));
+        root.addChildNode(medNode.get());
+    }
     Scene scene(&root);
     std::unique_ptr<InfiniteSphereNode> envNode;
     if (s->envTex) {
@@ -567,9 +668,14 @@ double refd_render(void* sp, int w, int h, int spp, uint32_t seed, int threads, 
     char* dir = mkdtemp(tmpl);
     if (dir && chdir(dir) != 0) dir = nullptr;
 
-    PTRenderer renderer((uint32_t)spp);
     auto t2 = std::chrono::steady_clock::now();
-    renderer.render(scene, settings);
+    if (s->integrator == 1) {
+        VolumetricPTRenderer renderer((uint32_t)spp);
+        renderer.render(scene, settings);
+    } else {
+        PTRenderer renderer((uint32_t)spp);
+        renderer.render(scene, settings);
+    }
     auto t3 = std::chrono::steady_clock::now();
 
     if (dir) {

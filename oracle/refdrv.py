@@ -46,6 +46,10 @@ def lib():
             "refd_intersect": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp]),
             "refd_visibility": (i32, [vp, i32, i32, vp, vp]),
             "refd_set_environment": (i32, [vp, i32, i32, vp, f32]),
+            "refd_set_medium": (i32, [vp, vp, vp, i32, i32, i32, f32, f32, f32]),
+            "refd_set_integrator": (None, [vp, i32]),
+            "refd_medium_density": (i32, [vp, i32, vp, vp]),
+            "refd_medium_track": (i32, [vp, i32, i32, vp, u32, vp]),
             "refd_render": (f64, [vp, i32, i32, i32, u32, i32, vp, i32, vp]),
             "refd_rng_stream": (None, [u32, i32, vp, vp]),
             "refd_wavelengths": (None, [f32, f32, vp, vp, vp]),
@@ -116,6 +120,15 @@ class RefScene:
             self._keep.append(rgb)
             if L.refd_set_environment(self.h, rgb.shape[1], rgb.shape[0], _p(rgb), float(env.get("scale", 1.0))):
                 raise ValueError("reference environment needs at least 4x4 texels")
+        med = desc.get("medium")
+        if med is not None:
+            b = np.ascontiguousarray(np.asarray(med["bounds"], np.float32).ravel())
+            dens = np.ascontiguousarray(med["density"], np.float32)
+            self._keep += [b, dens]
+            nz, ny, nx = dens.shape
+            if L.refd_set_medium(self.h, _p(b), _p(dens), nx, ny, nz, float(med["sigma_s"]), float(med["sigma_e"]),
+                                 float(med.get("g", 0.0))):
+                raise ValueError("bad medium")
         cam = desc["camera"]
         m = np.ascontiguousarray(np.asarray(cam["l2w"], np.float32).reshape(4, 4).T)
         L.refd_set_camera(self.h, _p(m), cam.get("sensitivity", 1.0), cam["aspect"], cam["fov_y"],
@@ -151,6 +164,23 @@ class RefScene:
         self.L.refd_qbvh_dump(self.h, _p(boxes), _p(info), _p(refs))
         return boxes, info, refs
 
+    def medium_density(self, params):
+        """Density of the medium at (n, 3) local coordinates (calcDensity)."""
+        p = np.ascontiguousarray(params, np.float32).reshape(-1, 3)
+        out = np.zeros(len(p), np.float32)
+        if self.L.refd_medium_density(self.h, len(p), _p(p), _p(out)):
+            raise ValueError("scene has no medium")
+        return out
+
+    def medium_track(self, rays, mode=0, seed=1):
+        """Reference transmittance estimates (mode 0) or free-path distances
+        (mode 1) for (n, 8) rays through the scene's medium."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(len(r), np.float32)
+        if self.L.refd_medium_track(self.h, int(mode), len(r), _p(r), seed & 0xFFFFFFFF, _p(out)):
+            raise ValueError("scene has no medium")
+        return out
+
     def instance_inverse(self, k, full=False):
         rows = np.zeros(16, np.float32)
         if self.L.refd_instance_inverse(self.h, k, _p(rows)):
@@ -174,9 +204,11 @@ class RefScene:
         self.L.refd_visibility(self.h, accel, len(rays), _p(rays), _p(vis))
         return vis
 
-    def render(self, width, height, spp, seed, threads=1):
-        """Runs PTRenderer; returns (film (h, w, 16) float32 sums, seconds, build seconds).
-        Cannot be combined with build_accel on the same object."""
+    def render(self, width, height, spp, seed, threads=1, volumetric=False):
+        """Runs PTRenderer (VolumetricPTRenderer with volumetric); returns
+        (film (h, w, 16) float32 sums, seconds, build seconds). Cannot be
+        combined with build_accel on the same object."""
+        self.L.refd_set_integrator(self.h, 1 if volumetric else 0)
         film = np.zeros((height, width, 16), np.float32)
         bs = np.zeros(1)
         sec = self.L.refd_render(self.h, width, height, spp, seed & 0xFFFFFFFF, threads, _p(film), 1, _p(bs))

@@ -470,6 +470,7 @@ typedef struct {
     /* camera */
     float l2w[12], nrm3[9];
     float opW, opH, objDist, imgDist, area, lensR;
+    int tangentNeeNaN;        /* see orc_scene_compat */
 } scene_t;
 
 static void* dupa(const void* p, size_t bytes) {
@@ -539,6 +540,13 @@ void* orc_scene_new(int nv, const float* pos, const float* nrm, const float* tan
         for (int c = 0; c < 3; ++c) s->nrm3[3 * r + c] = (float)inv[c][r];
     return s;
 }
+
+/* flags bit 0: the reference VolumetricPTRenderer's treatment of a light
+ * sample in the shading point's plane (the sample becomes NaN and is not
+ * filed); with it and an unlimited path length the integrator is
+ * VolumetricPTRenderer::Job::contribution (VolumetricPTRenderer.cpp:145-293)
+ * over a scene without media. */
+void orc_scene_compat(void* p, int flags) { ((scene_t*)p)->tangentNeeNaN = flags & 1; }
 
 void orc_scene_free(void* p) {
     scene_t* s = (scene_t*)p;
@@ -767,7 +775,17 @@ static void trace_sample(const scene_t* s, uint32_t st[4], uint32_t px, uint32_t
             v3 sdir = divs(dv, sqrtf(dist2));
             float cosS = fabsf(dot(sdir, f.gN)), cosL = fabsf(dot(sdir, lg));
             float lz = dot(ln, neg(sdir));
-            if (cosS * cosL > 0 && lz > 0.0f) {
+            if (s->tangentNeeNaN && cosS == 0.0f) {
+                /* VolumetricPTRenderer.cpp:198-230 has no cosine guard: a visible
+                 * light point in the shading point's plane makes the BSDF's
+                 * shading-normal correction 0/0 (directional_distribution_functions.h:322-325),
+                 * a NaN that drops the whole sample (VolumetricPTRenderer.cpp:130-138) */
+                float dist = len(sub(lpt, f.p));
+                v3 vd = divs(sub(lpt, f.p), dist);
+                ++*rays;
+                if (!occluded(s, f.p, vd, 1e-4f, dist * (1 - 1e-4f)))
+                    for (int k = 0; k < NL; ++k) sp[k] = NAN;
+            } else if (cosS * cosL > 0 && lz > 0.0f) {
                 float Mv[NL], fs[NL], bpdf;
                 spec(s, s->memit[s->tmat[lp_]], lam, Mv);
                 bsdf_eval(&b, dirOut, gL, hero, to_local(&f, sdir), fs, &bpdf);

@@ -32,7 +32,8 @@ EXPORTS = [
     "slrh_spectrum_rgb", "slrh_spectrum_regular", "slrh_spectrum_evaluate", "slrh_cmf_strata",
     "slrh_material_lambert", "slrh_material_oren_nayar", "slrh_material_ggx_conductor",
     "slrh_material_ggx_dielectric", "slrh_material_diffuse_emitter",
-    "slrh_mesh_add", "slrh_prototype_add", "slrh_instance_add", "slrh_instance_get", "slrh_scene_compat", "slrh_camera_perspective",
+    "slrh_mesh_add", "slrh_prototype_add", "slrh_instance_add", "slrh_instance_get", "slrh_scene_compat",
+    "slrh_medium_density_grid", "slrh_medium_query", "slrh_camera_perspective",
     "slrh_scene_create", "slrh_scene_build", "slrh_scene_destroy",
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
     "slrh_intersect", "slrh_occluded",
@@ -72,6 +73,7 @@ class _Request(ctypes.Structure):
 
 
 COMPAT_INSTANCE_SHADOW = 1
+COMPAT_VOLPT_TANGENT_NEE = 2
 FLAG_TIME_KERNELS = 1
 FLAG_NO_GRAPH = 2
 
@@ -112,6 +114,8 @@ def load_library(path=None):
         "slrh_prototype_add": (i32, [vp, u32, vp, vp, vp, u32, vp, vp, i32, pi]),
         "slrh_instance_add": (i32, [vp, i32, vp, i32, pi]),
         "slrh_scene_compat": (i32, [vp, u32]),
+        "slrh_medium_density_grid": (i32, [vp, vp, vp, u32, u32, u32, i32, i32, f32, pi]),
+        "slrh_medium_query": (i32, [vp, vp, u32, i32, u32, vp, i32, vp]),
         "slrh_instance_get": (i32, [vp, i32, vp, vp]),
         "slrh_camera_perspective": (i32, [vp, vp, f32, f32, f32, f32, f32, f32]),
         "slrh_scene_create": (i32, [vp, pp]),
@@ -277,6 +281,30 @@ class Scene:
         self.ctx.check(self.lib.slrh_instance_get(self.h, int(inst), _ptr(a), _ptr(b)), "slrh_instance_get")
         return a.reshape(3, 4), b.reshape(4, 4)
 
+    def set_medium(self, bounds, density, sigma_s, sigma_e, g=0.0):
+        """Heterogeneous medium: bounds (min xyz, max xyz), density (nz, ny, nx)
+        lattice, sigma_s / sigma_e spectrum ids (extinction constant), HG g."""
+        b = _f32(np.asarray(bounds, np.float32).ravel())
+        d = _f32(np.asarray(density, np.float32))
+        nz, ny, nx = d.shape
+        mid = ctypes.c_int()
+        self.ctx.check(self.lib.slrh_medium_density_grid(self.h, _ptr(b), _ptr(d), nx, ny, nz, int(sigma_s),
+                                                         int(sigma_e), float(g), ctypes.byref(mid)),
+                       "slrh_medium_density_grid")
+        return mid.value
+
+    def medium_query(self, rays, mode=0, seed=1):
+        """Per ray (n, 8) {org, dir, tmin, tmax}: mode 0 a transmittance
+        estimate, 1 a sampled free-path distance (tmax if none)."""
+        rays = _f32(rays).reshape(-1, 8)
+        n = len(rays)
+        cols = [np.ascontiguousarray(rays[:, k]) for k in range(8)]
+        r = _Rays(*[c.ctypes.data for c in cols])
+        out = np.zeros(n, np.float32)
+        self.ctx.check(self.lib.slrh_medium_query(self.h, ctypes.byref(r), n, int(mode), int(seed) & 0xFFFFFFFF,
+                                                  _ptr(out), MEM_HOST, None), "slrh_medium_query")
+        return out
+
     def set_compat(self, flags):
         self.ctx.check(self.lib.slrh_scene_compat(self.h, int(flags)), "slrh_scene_compat")
 
@@ -429,7 +457,7 @@ def trace_variants():
 
 STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "trace_seconds", "trace_launches",
              "extend_rays", "shade_seconds", "other_seconds", "shadow_rays", "nodes_visited", "tris_tested",
-             "nodes_shadow", "tris_shadow", "trace_variant"]
+             "nodes_shadow", "tris_shadow", "trace_variant", "medium_events", "medium_shadows"]
 
 
 def _fill_frame(fr, f, default_spp):
@@ -455,11 +483,13 @@ class PTRenderer:
     def __init__(self, spp):
         self.spp = int(spp)
 
+    integrator = 0   # SLRH_PT
+
     def render(self, scene, film, seed=1509761209, max_depth=0, slots_per_pixel=0, time_kernels=False,
                tune_trace=0, graph=True):
         flags = (FLAG_TIME_KERNELS if time_kernels else 0) | (0 if graph else FLAG_NO_GRAPH)
-        s = _Settings(film.width, film.height, self.spp, max_depth, seed & 0xFFFFFFFF, 0, slots_per_pixel,
-                      flags, tune_trace, 0)
+        s = _Settings(film.width, film.height, self.spp, max_depth, seed & 0xFFFFFFFF, self.integrator,
+                      slots_per_pixel, flags, tune_trace, 0)
         st = np.zeros(len(STAT_KEYS))
         scene.ctx.check(scene.lib.slrh_render(scene.h, ctypes.byref(s), film.h, _ptr(st)), "slrh_render")
         return dict(zip(STAT_KEYS, st.tolist()))
@@ -472,11 +502,17 @@ class PTRenderer:
         for i, f in enumerate(frames):
             _fill_frame(arr[i], f, self.spp)
         flags = (FLAG_TIME_KERNELS if time_kernels else 0) | (0 if graph else FLAG_NO_GRAPH)
-        s = _Settings(0, 0, self.spp, max_depth, 0, 0, slots_per_pixel, flags, tune_trace, 0)
+        s = _Settings(0, 0, self.spp, max_depth, 0, self.integrator, slots_per_pixel, flags, tune_trace, 0)
         st = np.zeros(len(STAT_KEYS))
         scene.ctx.check(scene.lib.slrh_render_batch(scene.h, arr, len(frames), ctypes.byref(s), _ptr(st)),
                         "slrh_render_batch")
         return dict(zip(STAT_KEYS, st.tolist()))
+
+
+class VolumetricPTRenderer(PTRenderer):
+    """VolumetricPTRenderer(spp) (libSLR/Renderer/VolumetricPTRenderer.h:16-44):
+    the scene's medium scatters and attenuates; no fixed path-length cap."""
+    integrator = 1   # SLRH_VOL_PT
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
@@ -572,8 +608,14 @@ def build_scene(ctx, desc):
     for inst in desc.get("instances", []):
         m = inst.get("mat", -1)
         sc.add_instance(inst["proto"], inst["l2w"], mat_ids[m] if m >= 0 else -1)
-    if desc.get("compat_instance_shadow"):
-        sc.set_compat(COMPAT_INSTANCE_SHADOW)
+    flags = (COMPAT_INSTANCE_SHADOW if desc.get("compat_instance_shadow") else 0) | \
+        (COMPAT_VOLPT_TANGENT_NEE if desc.get("compat_volpt_tangent_nee") else 0)
+    if flags:
+        sc.set_compat(flags)
+    med = desc.get("medium")
+    if med is not None:
+        sc.set_medium(med["bounds"], med["density"], sc.spectrum_constant(med["sigma_s"]),
+                      sc.spectrum_constant(med["sigma_e"]), med.get("g", 0.0))
     cam = desc["camera"]
     sc.set_camera(cam["l2w"], cam["aspect"], cam["fov_y"], cam.get("lens_radius", 0.0), cam.get("img_dist", 1.0),
                   cam.get("obj_dist", 1.0), cam.get("sensitivity", 1.0))

@@ -404,10 +404,51 @@ int slrh_instance_get(const slrh_scene* s, int inst_id, float* l2w, float* w2l) 
     return SLRH_OK;
 }
 
+int slrh_medium_density_grid(slrh_scene* s, const float* bounds, const float* density, uint32_t nx, uint32_t ny,
+                             uint32_t nz, int sigma_s, int sigma_e, float g, int* medium_id) {
+    CHECK_SCENE(s);
+    SceneData& d = s->data;
+    if (d.medium.present) return fail(s->ctx, SLRH_ERR_CAPACITY, "medium: one medium per scene");
+    if (!bounds || !density || nx < 2 || ny < 2 || nz < 2)
+        return fail(s->ctx, SLRH_ERR_INVALID, "medium: bounds, density and at least 2 lattice points per axis");
+    if ((uint64_t)nx * ny * nz > (1ull << 30)) return fail(s->ctx, SLRH_ERR_CAPACITY, "medium: lattice too large");
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(bounds[a]) || !std::isfinite(bounds[3 + a]) || !(bounds[3 + a] > bounds[a]))
+            return fail(s->ctx, SLRH_ERR_INVALID, "medium: bad bounds");
+    const int ns = (int)d.spectra.size();
+    if (sigma_s < 0 || sigma_s >= ns || sigma_e < 0 || sigma_e >= ns)
+        return fail(s->ctx, SLRH_ERR_INVALID, "medium: bad spectrum id");
+    const HostSpectrum& se = d.spectra[sigma_e];
+    for (float v : se.values)
+        if (v != se.values[0])
+            return fail(s->ctx, SLRH_ERR_INVALID, "medium: the extinction spectrum must be constant over wavelength");
+    if (!(se.values[0] >= 0) || !std::isfinite(se.values[0])) return fail(s->ctx, SLRH_ERR_INVALID, "medium: bad extinction");
+    for (float v : d.spectra[sigma_s].values)
+        if (!(v >= 0 && v <= se.values[0])) return fail(s->ctx, SLRH_ERR_INVALID, "medium: scattering must be in [0, extinction]");
+    if (!(g > -1.0f && g < 1.0f)) return fail(s->ctx, SLRH_ERR_INVALID, "medium: g must be in (-1, 1)");
+    const size_t cnt = (size_t)nx * ny * nz;
+    for (size_t i = 0; i < cnt; ++i)
+        if (!(density[i] >= 0) || !std::isfinite(density[i])) return fail(s->ctx, SLRH_ERR_INVALID, "medium: bad density");
+    MediumData& m = d.medium;
+    for (int a = 0; a < 3; ++a) { m.bmin[a] = bounds[a]; m.bmax[a] = bounds[3 + a]; }
+    m.n[0] = nx; m.n[1] = ny; m.n[2] = nz;
+    m.density.assign(density, density + cnt);
+    m.sigmaS = sigma_s;
+    m.sigmaE = sigma_e;
+    m.sigmaEValue = se.values[0];
+    m.g = g;
+    buildMajorants(&m);
+    m.present = true;
+    if (medium_id) *medium_id = 0;
+    return SLRH_OK;
+}
+
 int slrh_scene_compat(slrh_scene* s, uint32_t flags) {
     CHECK_SCENE(s);
-    if (flags & ~(uint32_t)SLRH_COMPAT_INSTANCE_SHADOW) return fail(s->ctx, SLRH_ERR_INVALID, "scene_compat: unknown flag");
+    if (flags & ~(uint32_t)(SLRH_COMPAT_INSTANCE_SHADOW | SLRH_COMPAT_VOLPT_TANGENT_NEE))
+        return fail(s->ctx, SLRH_ERR_INVALID, "scene_compat: unknown flag");
     s->data.compatInstanceShadow = (flags & SLRH_COMPAT_INSTANCE_SHADOW) != 0;
+    s->data.compatTangentNee = (flags & SLRH_COMPAT_VOLPT_TANGENT_NEE) != 0;
     return SLRH_OK;
 }
 
@@ -586,6 +627,28 @@ int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* vis, uint32_t n, 
     return doTrace(s, rays, nullptr, vis, n, mem, stream);
 }
 
+int slrh_medium_query(slrh_scene* s, const slrh_rays* rays, uint32_t n, int mode, uint32_t seed, float* out,
+                      int mem, void* stream) {
+    if (!s || !rays || !out) return SLRH_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "medium query before slrh_scene_build");
+    if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
+    if (!s->data.medium.present) return fail(s->ctx, SLRH_ERR_STATE, "scene has no medium");
+    if (mode != SLRH_MEDIUM_TRANSMITTANCE && mode != SLRH_MEDIUM_FREE_PATH)
+        return fail(s->ctx, SLRH_ERR_INVALID, "medium query: unknown mode");
+    if (mem != SLRH_MEM_HOST && mem != SLRH_MEM_DEVICE) return fail(s->ctx, SLRH_ERR_INVALID, "bad mem kind");
+    if (n == 0) return SLRH_OK;
+    TraceArgs a{};
+    a.ox = rays->ox; a.oy = rays->oy; a.oz = rays->oz; a.dx = rays->dx; a.dy = rays->dy; a.dz = rays->dz;
+    a.tmin = rays->tmin; a.tmax = rays->tmax;
+    if (!a.ox || !a.oy || !a.oz || !a.dx || !a.dy || !a.dz || !a.tmin || !a.tmax)
+        return fail(s->ctx, SLRH_ERR_INVALID, "null ray array");
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    std::string err;
+    int r = mediumQuery(s->dev, a, n, mode, seed, out, mem == SLRH_MEM_DEVICE, stream, &err);
+    if (r) return fail(s->ctx, r, err);
+    return SLRH_OK;
+}
+
 int slrh_film_create(slrh_context* ctx, uint32_t w, uint32_t h, slrh_film** out) {
     if (!ctx || !out || w == 0 || h == 0) return SLRH_ERR_INVALID;
     if (ctx->device < 0) return fail(ctx, SLRH_ERR_STATE, "host-only context has no device");
@@ -685,6 +748,7 @@ void copyStats(const RenderStats& st, double* stats) {
     stats[11] = st.nodesVisited; stats[12] = st.trisTested;
     stats[13] = st.nodesShadow; stats[14] = st.trisShadow;
     stats[15] = st.traceVariant;
+    stats[16] = st.mediumEvents; stats[17] = st.mediumShadows;
 }
 }  // namespace
 
@@ -693,7 +757,8 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "render before slrh_scene_build");
     if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
     if (!s->data.hasCamera) return fail(s->ctx, SLRH_ERR_STATE, "scene has no camera");
-    if (set->integrator != SLRH_PT) return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
+    if (set->integrator != SLRH_PT && set->integrator != SLRH_VOL_PT)
+        return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
     if (set->width != film->w || set->height != film->h) return fail(s->ctx, SLRH_ERR_INVALID, "film size mismatch");
     if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
     std::lock_guard<std::mutex> lk(s->ctx->mu);
@@ -703,6 +768,7 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     cfg.traceVariant = set->tune_trace == 0 ? -1 : (int)set->tune_trace - 1;
     if (cfg.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     cfg.useGraph = (set->flags & SLRH_FLAG_NO_GRAPH) == 0;
+    cfg.volumetric = set->integrator == SLRH_VOL_PT;
     RenderStats st;
     std::string err;
     int r = slrh::render(s->dev, cfg, film->dev, &st, &err);
@@ -737,7 +803,8 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     if (!s || !set || (n && !frames)) return SLRH_ERR_INVALID;
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "render before slrh_scene_build");
     if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
-    if (set->integrator != SLRH_PT) return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
+    if (set->integrator != SLRH_PT && set->integrator != SLRH_VOL_PT)
+        return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
     if (set->reserved != 0) return fail(s->ctx, SLRH_ERR_INVALID, "reserved settings field must be 0");
     if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
     std::vector<JobSpec> specs(n);
@@ -750,6 +817,7 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     opt.timeKernels = (set->flags & SLRH_FLAG_TIME_KERNELS) != 0;
     opt.useGraph = (set->flags & SLRH_FLAG_NO_GRAPH) == 0;
     opt.traceVariant = set->tune_trace == 0 ? -1 : (int)set->tune_trace - 1;
+    opt.volumetric = set->integrator == SLRH_VOL_PT;
     if (opt.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     RenderStats st;

@@ -57,7 +57,8 @@ enum ShadeClass : uint32_t {
     CLS_CONDUCTOR = 1,
     CLS_DIELECTRIC = 2,
     CLS_MISS = 3,        // no hit / emitter without base
-    kNumClasses = 4,
+    CLS_MEDIUM = 4,      // scattering in the medium (volumetric integrator)
+    kNumClasses = 5,
 };
 
 struct DevMaterial {
@@ -108,6 +109,8 @@ struct DevScene {
     const int32_t* instMat;      // material replacing the prototype's, -1: none
     uint32_t numInstances;
     uint32_t compatInstShadow;   // shadow rays test instances with the world ray
+    uint32_t compatTangentNee;   // the scene asks for the reference's volumetric tangent-light NaN
+    uint32_t tangentNeeNaN;      // set per render: compatTangentNee with the volumetric integrator
     DevCamera cam;
     // environment light (image-based); hasEnv = 0 without one
     const float4* envTexel;   // per texel: Meng basis indices (bits), s, t, scale
@@ -116,7 +119,18 @@ struct DevScene {
     uint32_t envW, envH, envMapW, envMapH;
     float envScale;
     uint32_t hasEnv;
+    // heterogeneous medium (volumetric integrator)
+    const float* medDensity;    // lattice densities, x fastest, then y, then z
+    const float* medMajorant;   // maximum density per supervoxel (kMedSuperVoxel^3 lattice cells)
+    float medMin[3], medMax[3]; // region
+    uint32_t medN[3], medCells[3];
+    float medSigmaE;            // extinction per unit density (constant over wavelength)
+    int32_t medSigmaS;          // scattering per unit density (spectrum id)
+    float medG;                 // Henyey-Greenstein asymmetry
+    uint32_t hasMedium;
+    uint32_t volume;            // set per render: the volumetric integrator with a medium
 };
+constexpr uint32_t kMedSuperVoxel = 8;
 
 __device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
 __device__ __forceinline__ float3 operator+(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }

@@ -39,6 +39,8 @@ struct Instance {
 // id, an instance by kRefInstance | instance id.
 constexpr uint32_t kRefInstance = 0x80000000u;
 
+
+
 struct SceneData {
     std::vector<float> vpos, vnrm, vtan;   // 3 floats per vertex (object space for prototype triangles)
     std::vector<uint32_t> idx;             // 3 per triangle (global vertex indices)
@@ -55,6 +57,8 @@ struct SceneData {
     // object-space prototype (TransformedSurfaceObject::testVisibility,
     // libSLR/Core/surface_object.cpp:363-369)
     bool compatInstanceShadow = false;
+    bool compatTangentNee = false;   // volumetric integrator: NaN for a light sample in the shading plane
+    MediumData medium;
     Accel accel;   // world level: world-space triangles and instances, refs as above
 };
 
@@ -73,6 +77,7 @@ struct TraceArgs {
 struct RenderConfig {
     uint32_t width, height, spp, maxDepth, seed, slotsPerPixel;
     bool timeKernels;
+    bool volumetric = false;   // VolumetricPTRenderer: the medium takes part
     int traceVariant = 0;   // traversal kernel variant (see traceVariant), -1: autotuned
     bool useGraph = true;
 };
@@ -85,6 +90,7 @@ struct RenderStats {
     double nodesShadow = 0, trisShadow = 0;    // shadow rays
     double overflow = 0;
     double traceVariant = 0;   // traversal variant the render ran with
+    double mediumEvents = 0, mediumShadows = 0;   // scattering events, light samples through the medium
 };
 
 int deviceInit(int device, std::string* err);
@@ -95,6 +101,10 @@ size_t sceneDeviceBytes(const DeviceScene* ds);
 // any == false: closest hit into t/prim/b0/b1; any == true: visibility into vis.
 int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool devicePtrs, void* stream,
                double* nodesVisited, double* trisTested, std::string* err);
+
+// mode 0: a transmittance estimate per ray, 1: a free-path distance per ray.
+int mediumQuery(DeviceScene* ds, const TraceArgs& a, uint32_t n, int mode, uint32_t seed, float* out,
+                bool devicePtrs, void* stream, std::string* err);
 
 int filmCreate(uint32_t w, uint32_t h, DeviceFilm** out, std::string* err);
 void filmFree(DeviceFilm* f);
@@ -116,6 +126,7 @@ struct RenderOptions {
     bool timeKernels = false;
     bool useGraph = true;         // replay recorded iteration blocks (ignored when timing kernels)
     int traceVariant = 0;         // -1: chosen by the autotuner
+    bool volumetric = false;      // VolumetricPTRenderer: media scatter and attenuate
 };
 // The variant the autotuner chose for the scene (0 before any tuning).
 int tunedTraceVariant(const DeviceScene* ds);

@@ -136,6 +136,27 @@ struct EnvData {
 };
 // rgb: width * height linear sRGB triples (row 0 = theta 0, +y up).
 void buildEnvironment(const float* rgb, uint32_t width, uint32_t height, float scale, EnvData* env);
+// A heterogeneous medium on a density lattice (DensityGridMediumDistribution,
+// libSLR/MediumDistribution/DensityGridMediumDistribution.h:16-60): region,
+// nx*ny*nz densities (x fastest, then y, then z), base scattering and
+// extinction spectra, Henyey-Greenstein asymmetry. Majorants: the maximum
+// density over each supervoxel of kSuperVoxel^3 lattice cells (its
+// (kSuperVoxel+1)^3 lattice points bound the trilinear density inside it).
+constexpr uint32_t kSuperVoxel = 8;
+struct MediumData {
+    bool present = false;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    uint32_t n[3] = {0, 0, 0};
+    std::vector<float> density;
+    int32_t sigmaS = -1;      // spectrum ids
+    int32_t sigmaE = -1;
+    float sigmaEValue = 0;    // the extinction spectrum's (constant) value
+    float g = 0;
+    uint32_t cells[3] = {0, 0, 0};
+    std::vector<float> majorant;   // cells[0]*cells[1]*cells[2] maximum densities
+};
+void buildMajorants(MediumData* m);
+
 // Value of a float truncated toward zero to IEEE half precision (overflow to
 // infinity), the conversion of the reference's half type (External/half.hpp
 // 1.12 with its default rounding style).

@@ -47,6 +47,7 @@ namespace slrh {
         }                                                                          \
     } while (0)
 
+constexpr int SLRH_ERR_STATE_CODE = -2;
 constexpr int SLRH_ERR_DEVICE_CODE = -3;
 constexpr int SLRH_ERR_NOMEM_CODE = -4;
 constexpr int SLRH_ERR_CAPACITY_CODE = -5;
@@ -118,6 +119,7 @@ struct DeviceScene {
     DevBuf<uint32_t> spill;      // traversal stack spill for ray batches
     DevBuf<float4> envTexel;
     DevBuf<float> mengBasis, envCondPdf, envCondCdf, envMargPdf, envMargCdf;
+    DevBuf<float> medDensity, medMajorant;
     DevBuf<unsigned long long> counters;
     DevScene view;
     bool hasCamera = false;
@@ -409,6 +411,26 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         ve.envMapH = e.mapH;
         ve.envScale = e.scale;
     }
+    ve.medDensity = ve.medMajorant = nullptr;
+    ve.hasMedium = 0;
+    ve.volume = 0;
+    if (sd.medium.present) {
+        static_assert(kMedSuperVoxel == kSuperVoxel, "supervoxel size");
+        const MediumData& m = sd.medium;
+        if ((r = ds->medDensity.upload(m.density, err)) || (r = ds->medMajorant.upload(m.majorant, err))) return r;
+        ve.medDensity = ds->medDensity.p;
+        ve.medMajorant = ds->medMajorant.p;
+        for (int a = 0; a < 3; ++a) {
+            ve.medMin[a] = m.bmin[a];
+            ve.medMax[a] = m.bmax[a];
+            ve.medN[a] = m.n[a];
+            ve.medCells[a] = m.cells[a];
+        }
+        ve.medSigmaE = m.sigmaEValue;
+        ve.medSigmaS = m.sigmaS;
+        ve.medG = m.g;
+        ve.hasMedium = 1;
+    }
     ds->bytes = nodes.size() * 16 + tris.size() * 16 + (vp.size() * 3) * 16 + pv.size() * 16 + pool.size() * 4;
     DevScene& v = ds->view;
     v.nodes = ds->nodes.p; v.tris = ds->tris.p; v.primVerts = ds->primVerts.p;
@@ -420,6 +442,8 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     v.instMat = ds->instMat.p;
     v.numInstances = (uint32_t)sd.instances.size();
     v.compatInstShadow = sd.compatInstanceShadow ? 1u : 0u;
+    v.compatTangentNee = sd.compatTangentNee ? 1u : 0u;
+    v.tangentNeeNaN = 0;
     v.numLights = (uint32_t)(cdf.size() - 1);
     v.numPrims = (uint32_t)nt;
     v.numNodes = (uint32_t)nn;
@@ -602,7 +626,7 @@ enum Ctr : uint32_t {
     C_TRACE = 0,          // trace queue length (extension and shadow entries)
     C_DONE = 1,           // done queue length
     C_CLS0 = 3,           // .. C_CLS0 + kNumClasses - 1: shading queue lengths
-    C_PENDING = 7,        // slots with an extension or shadow ray pending after shading
+    C_PENDING = 8,        // slots with an extension or shadow ray pending after shading
     C_NONFINITE = 9,
     C_OVERFLOW = 10,
     C_FETCH = 11,         // entry cursor of k_trace's persistent loop
@@ -611,7 +635,9 @@ enum Ctr : uint32_t {
 // 64-bit statistics
 enum Stat64 : uint32_t {
     S_RAYS_EXT = 0, S_RAYS_SHADOW = 1, S_NODES_EXT = 2, S_TRIS_EXT = 3, S_SAMPLES = 4, S_NODES_SH = 5, S_TRIS_SH = 6,
-    kNumStat = 8
+    S_MED_EVENTS = 7,     // scattering events in the medium
+    S_MED_SHADOWS = 8,    // light samples whose segment crossed the medium
+    kNumStat = 10
 };
 // Per-slot work code, one byte per slot, rewritten by each stage and turned
 // into slot-ordered queues by k_compact. After shading a slot holds EXT or
@@ -687,7 +713,8 @@ struct PathState {
 };
 
 // queue offsets in units of n entries
-enum QueueId : uint32_t { Q_DONE = 0, Q_TRACE = 1, Q_CLS0 = 3, kNumQueues = 7 };
+enum QueueId : uint32_t { Q_DONE = 0, Q_TRACE = 1, Q_CLS0 = 3, kNumQueues = Q_CLS0 + kNumClasses };
+static_assert(C_CLS0 + kNumClasses <= C_PENDING, "counter layout");
 
 __device__ __forceinline__ uint32_t* queue(const PathState& ps, uint32_t id) { return ps.q + (size_t)id * ps.n; }
 
@@ -877,6 +904,184 @@ __device__ __forceinline__ float envPdf(const DevScene& sc, float d0, float d1) 
 __device__ __forceinline__ float envAreaPdf(const DevScene& sc, float phi, float theta) {
     const float uvPdf = envPdf(sc, (float)(phi / (2 * kPiD)), (float)(theta / kPiD));
     return (float)(uvPdf / (2 * kPiD * kPiD * sinf(theta)));
+}
+
+// ------------------------------------------------------------------ participating medium
+
+// DensityGridMediumDistribution::calcDensity (libSLR/MediumDistribution/
+// DensityGridMediumDistribution.cpp:369-395): the point's coordinates in the
+// region (BoundingBox3D::calculateLocalCoordinates), zero outside it,
+// trilinear between the lattice points around it, in the reference's term
+// order.
+__device__ __forceinline__ float mediumDensity(const DevScene& sc, float3 p) {
+    const float px = (p.x - sc.medMin[0]) / (sc.medMax[0] - sc.medMin[0]);
+    const float py = (p.y - sc.medMin[1]) / (sc.medMax[1] - sc.medMin[1]);
+    const float pz = (p.z - sc.medMin[2]) / (sc.medMax[2] - sc.medMin[2]);
+    // outside the region (and any NaN coordinate) reads zero
+    if (!(px >= 0 && py >= 0 && pz >= 0 && px <= 1 && py <= 1 && pz <= 1)) return 0.0f;
+    const uint32_t nx = sc.medN[0], ny = sc.medN[1], nz = sc.medN[2];
+    const uint32_t lx = min((uint32_t)(px * (nx - 1)), nx - 1), ux = min(lx + 1, nx - 1);
+    const uint32_t ly = min((uint32_t)(py * (ny - 1)), ny - 1), uy = min(ly + 1, ny - 1);
+    const uint32_t lz = min((uint32_t)(pz * (nz - 1)), nz - 1), uz = min(lz + 1, nz - 1);
+    const float wux = px * (nx - 1) - lx, wlx = 1 - wux;
+    const float wuy = py * (ny - 1) - ly, wly = 1 - wuy;
+    const float wuz = pz * (nz - 1) - lz, wlz = 1 - wuz;
+    const float* g = sc.medDensity;
+    const size_t zl = (size_t)lz * ny * nx, zu = (size_t)uz * ny * nx;
+    return g[zl + (size_t)nx * ly + lx] * wlz * wly * wlx + g[zl + (size_t)nx * ly + ux] * wlz * wly * wux +
+           g[zl + (size_t)nx * uy + lx] * wlz * wuy * wlx + g[zl + (size_t)nx * uy + ux] * wlz * wuy * wux +
+           g[zu + (size_t)nx * ly + lx] * wuz * wly * wlx + g[zu + (size_t)nx * ly + ux] * wuz * wly * wux +
+           g[zu + (size_t)nx * uy + lx] * wuz * wuy * wlx + g[zu + (size_t)nx * uy + ux] * wuz * wuy * wux;
+}
+
+// Clips [t0, t1] of the ray to the medium region; false if nothing is left.
+__device__ __forceinline__ bool clipToMedium(const DevScene& sc, float3 o, float3 d, float* t0, float* t1) {
+    const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float inv = 1.0f / da[a];
+        float ta = (sc.medMin[a] - oa[a]) * inv, tb = (sc.medMax[a] - oa[a]) * inv;
+        if (ta > tb) { const float x = ta; ta = tb; tb = x; }
+        *t0 = fmaxf(*t0, ta);   // a NaN (origin on a plane, zero direction) keeps the bound
+        *t1 = fminf(*t1, tb);
+    }
+    return *t0 < *t1;
+}
+
+// Upper bound on the number of supervoxel steps plus tentative collisions
+// one tracking walk takes before it gives up (a guard; optical depths in the
+// configurations stay far below it).
+constexpr int kMaxTrackSteps = 1 << 16;
+
+// Walks the supervoxels that the part of [t0, t1] inside the medium crosses
+// (3D DDA in lattice space, cells of kMedSuperVoxel lattice intervals) and
+// draws tentative collisions against the piecewise-constant majorant
+// mu = sigma_e * max density of the cell: exponential free paths whose
+// optical depth carries across cells, as the reference's supervoxel
+// traversal does with its trilinear majorant (DensityGridMediumDistribution.cpp:
+// 208-367). collide(t) is called at each tentative collision with the
+// majorant there and returns true to stop the walk (the walk then returns
+// true); false: the walk reached t1.
+template <typename Collide>
+__device__ __forceinline__ bool trackMedium(const DevScene& sc, float3 o, float3 d, float t0, float t1, uint4& rng,
+                                            Collide&& collide) {
+    if (!clipToMedium(sc, o, d, &t0, &t1)) return false;
+    const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
+    int cell[3], step[3];
+    float tNext[3], tDelta[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float scale = (float)(sc.medN[a] - 1) / (sc.medMax[a] - sc.medMin[a]);   // lattice units per length
+        const float l0 = (oa[a] + da[a] * t0 - sc.medMin[a]) * scale;
+        int c = (int)floorf(l0 / kMedSuperVoxel);
+        c = max(0, min((int)sc.medCells[a] - 1, c));
+        cell[a] = c;
+        const float dl = da[a] * scale;
+        if (dl > 0.0f) {
+            step[a] = 1;
+            tNext[a] = t0 + ((float)((c + 1) * (int)kMedSuperVoxel) - l0) / dl;
+            tDelta[a] = kMedSuperVoxel / dl;
+        } else if (dl < 0.0f) {
+            step[a] = -1;
+            tNext[a] = t0 + ((float)(c * (int)kMedSuperVoxel) - l0) / dl;
+            tDelta[a] = -(float)kMedSuperVoxel / dl;
+        } else {
+            step[a] = 0;
+            tNext[a] = INFINITY;
+            tDelta[a] = INFINITY;
+        }
+    }
+    float t = t0;
+    float tau = -logf(1.0f - xsFloat(rng));
+    for (int it = 0; it < kMaxTrackSteps; ++it) {
+        const float mu =
+            sc.medSigmaE * sc.medMajorant[((size_t)cell[2] * sc.medCells[1] + cell[1]) * sc.medCells[0] + cell[0]];
+        const float tExit = fminf(fminf(tNext[0], tNext[1]), fminf(tNext[2], t1));
+        const float span = fmaxf(tExit - t, 0.0f);
+        if (mu > 0.0f && tau < mu * span) {
+            t += tau / mu;
+            if (collide(t, mu)) return true;
+            tau = -logf(1.0f - xsFloat(rng));
+            continue;
+        }
+        if (mu > 0.0f) tau -= mu * span;
+        t = fmaxf(t, tExit);
+        if (t >= t1) return false;
+        const int a = tNext[0] < tNext[1] ? (tNext[0] < tNext[2] ? 0 : 2) : (tNext[1] < tNext[2] ? 1 : 2);
+        cell[a] += step[a];
+        if (cell[a] < 0 || cell[a] >= (int)sc.medCells[a]) return false;
+        tNext[a] += tDelta[a];
+    }
+    return false;
+}
+
+// Free-path sampling for the hero wavelength by delta tracking (the
+// medium's extinction is grey, so the free path is the same at every
+// wavelength and the reference's ratio-tracked throughput of the others,
+// DensityGridMediumDistribution.cpp:454-486, is one). Returns true with the
+// scattering distance in *ts when a real collision happens before t1.
+__device__ __forceinline__ bool sampleFreePath(const DevScene& sc, float3 o, float3 d, float t0, float t1, uint4& rng,
+                                               float* ts) {
+    return trackMedium(sc, o, d, t0, t1, rng, [&](float t, float mu) {
+        const float sigma = sc.medSigmaE * mediumDensity(sc, o + d * t);
+        if (xsFloat(rng) < sigma / mu) {
+            *ts = t;
+            return true;
+        }
+        return false;
+    });
+}
+
+// Transmittance of [t0, t1] by ratio tracking with Russian roulette below
+// 0.1, as the reference's estimateTransmittance (DensityGridMediumDistribution.cpp:
+// 618-655); one estimate serves all wavelengths (grey extinction).
+__device__ __forceinline__ float mediumTransmittance(const DevScene& sc, float3 o, float3 d, float t0, float t1,
+                                                     uint4& rng) {
+    float T = 1.0f;
+    trackMedium(sc, o, d, t0, t1, rng, [&](float t, float mu) {
+        const float sigma = sc.medSigmaE * mediumDensity(sc, o + d * t);
+        T *= 1.0f - fminf(sigma / mu, 1.0f);
+        if (T < 0.1f) {
+            if (xsFloat(rng) < T) T = 1.0f;
+            else {
+                T = 0.0f;
+                return true;
+            }
+        }
+        return false;
+    });
+    return T;
+}
+
+// HenyeyGreensteinPhaseFunction (libSLR/PhaseFunction/basic_phase_functions.cpp:
+// 36-64): cosTheta is taken against the direction of propagation; g = 0 is
+// the isotropic limit (IsotropicPhaseFunction, :13-33).
+__device__ __forceinline__ float hgValue(float g, float cosTheta) {
+    if (g == 0.0f) return (float)(1.0 / (4.0 * kPiD));
+    return (1 - g * g) / ((float)(4 * kPiD) * powf(1 + g * g - 2 * g * cosTheta, 1.5f));
+}
+__device__ __forceinline__ float3 hgSample(float g, float u0, float u1, float* value) {
+    if (g == 0.0f) {
+        // uniformSampleSphere (libSLR/Core/distributions.h:54-58)
+        const float phi = (float)(2 * kPiD) * u1, theta = acosf(1 - 2 * u0);
+        *value = (float)(1.0 / (4.0 * kPiD));
+        return f3(cosf(phi) * sinf(theta), sinf(phi) * sinf(theta), cosf(theta));
+    }
+    const float sqTerm = (1 - g * g) / (1 - g + 2 * g * u1);
+    const float cosTheta = fminf(fmaxf(0.5f / g * (1 + g * g - sqTerm * sqTerm), -1.0f), 1.0f);
+    const float phi = (float)(2 * kPiD) * u0;
+    *value = (1 - g * g) / ((float)(4 * kPiD) * powf(1 + g * g - 2 * g * cosTheta, 1.5f));
+    const float sinTheta = sqrtf(1 - cosTheta * cosTheta);
+    return f3(cosf(phi) * sinTheta, sinf(phi) * sinTheta, cosTheta);
+}
+// Vector3D::makeCoordinateSystem (libSLR/BasicTypes/Vector3D.h:61-67): the
+// frame a medium point's incoming direction spans (ReferenceFrame(z)).
+__device__ __forceinline__ void coordinateSystem(float3 z, float3* x, float3* y) {
+    const float sign = z.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sign + z.z);
+    const float b = z.x * z.y * a;
+    *x = f3(1.0f + sign * z.x * z.x * a, sign * b, -sign * z.x);
+    *y = f3(b, sign + z.y * z.y * a, -z.y);
 }
 
 // ------------------------------------------------------------------ kernels
@@ -1416,6 +1621,22 @@ void k_shade(DevScene sc, PathState ps) {
                     // surface lights or the environment by importance
                     bool envLight = false;
                     float uSurf = uSel;
+                    // VolumetricPTRenderer.cpp:190-231 (with SLRH_COMPAT_VOLPT_TANGENT_NEE):
+                    // no cosine guard, so a light sample in the shading plane
+                    // gives 0/0 in the BSDF's shading-normal correction
+                    // (directional_distribution_functions.h:322-325); if it is
+                    // visible the NaN reaches the radiance and the sample is
+                    // not filed
+                    auto tangentNaN = [&](float3 dir, float tmax) {
+                        float c[kNumL];
+#pragma unroll
+                        for (int k = 0; k < kNumL; ++k) c[k] = __builtin_nanf("");
+                        storePlanes(ps, s, SF_NEE, c);
+                        ps.f4(s, SF_SO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
+                        ps.f4(s, SF_SD) = make_float4(dir.x, dir.y, dir.z, tmax);
+                        ps.f4(s, SF_SINV) = rayInv(dir);
+                        shadow = true;
+                    };
                     if (sc.hasEnv) {
                         const float surfImp = sc.lightImportance, sumImps = surfImp + 1.0f;
                         if (sumImps * uSel < surfImp) uSurf = uSel / (surfImp / sumImps);
@@ -1434,7 +1655,9 @@ void k_shade(DevScene sc, PathState ps) {
                         const float3 sdir = normalize3(pos);
                         const float cosS = fabsf(dot3(sdir, sp.gN));
                         const float cosL = fabsf(dot3(sdir, -pos));
-                        if (cosS * cosL > 0) {
+                        if (sc.tangentNeeNaN && cosS == 0.0f) {
+                            tangentNaN(sdir, FLT_MAX);
+                        } else if (cosS * cosL > 0) {
                             float Le[kNumL], fs[kNumL];
                             envRadiance(sc, (float)(phi / (2 * kPiD)), (float)(theta / kPiD), misc.x, Le);
                             const float3 sdL = toLocal(sp, sdir);
@@ -1490,7 +1713,10 @@ void k_shade(DevScene sc, PathState ps) {
                         const float cosL = fabsf(dot3(sdir, lg));
                         // EDF is evaluated in the light's shading frame: only its z matters
                         const float lz = dot3(ln, -sdir);
-                        if (cosS * cosL > 0 && lz > 0.0f) {
+                        if (sc.tangentNeeNaN && cosS == 0.0f) {
+                            const float dist = length3(lp - sp.p);
+                            tangentNaN(divs(lp - sp.p, dist), dist * (1 - 1e-4f));
+                        } else if (cosS * cosL > 0 && lz > 0.0f) {
                             const DevMaterial lm = sc.mats[lv.w];
                             float M[kNumL], fs[kNumL];
                             evalSpectrum(sc, lm.emit, misc.x, M);
@@ -1575,6 +1801,304 @@ void k_shade(DevScene sc, PathState ps) {
 //   MODE 1 (after regeneration): trace queue, the slot's extension entry
 //          and then its shadow entry; clears the shadow bit.
 //   MODE 2 (after tracing): one queue per shading class.
+// Volumetric integrator, after the traversal: the extension rays' free
+// paths through the medium (Scene::interact, libSLR/Scene/Scene.cpp:66-94:
+// the medium over [tmin, surface hit]) by delta tracking, and the light
+// samples' transmittance (Scene::testVisibility, :114-135) by ratio tracking,
+// which scales the stored contribution before it is resolved. A slot's
+// extension and shadow entries are adjacent in the trace queue; the thread
+// of the extension entry handles both, so one thread owns the slot's RNG.
+__global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
+    const uint32_t count = ps.ctr[C_TRACE];
+    const uint32_t* qin = queue(ps, Q_TRACE);
+    unsigned long long events = 0, shadows = 0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+        const uint32_t e = qin[i];
+        const uint32_t s = e & ~kShadowEntry;
+        bool doExt = false, doShadow = false;
+        if (e & kShadowEntry) {
+            doShadow = !(i > 0 && qin[i - 1] == s);
+        } else {
+            doExt = true;
+            doShadow = i + 1 < count && qin[i + 1] == (s | kShadowEntry);
+        }
+        // an occluded light sample needs no transmittance
+        doShadow = doShadow && ps.vis[s] != 0;
+        if (!doExt && !doShadow) continue;
+        uint4 rng = ps.u4(s, SF_RNG);
+        bool used = false;
+        if (doExt) {
+            const float4 o4 = ps.f4(s, SF_RO), d4 = ps.f4(s, SF_RD), h = ps.f4(s, SF_HIT);
+            const float3 o = xyz(o4), d = xyz(d4);
+            float t0 = o4.w, t1 = h.x;
+            if (clipToMedium(sc, o, d, &t0, &t1)) {
+                used = true;
+                float ts;
+                if (sampleFreePath(sc, o, d, o4.w, h.x, rng, &ts)) {
+                    ps.f4(s, SF_HIT) = make_float4(ts, __int_as_float(-2), 0.0f, 0.0f);
+                    ps.code[s] = (uint8_t)(CODE_CLS0 + CLS_MEDIUM);
+                }
+            }
+        }
+        if (doShadow) {
+            const float4 o4 = ps.f4(s, SF_SO), d4 = ps.f4(s, SF_SD);
+            const float3 o = xyz(o4), d = xyz(d4);
+            float t0 = o4.w, t1 = d4.w;
+            if (clipToMedium(sc, o, d, &t0, &t1)) {
+                used = true;
+                ++shadows;
+                const float T = mediumTransmittance(sc, o, d, o4.w, d4.w, rng);
+                if (T != 1.0f) {
+                    float c[kNumL];
+                    loadPlanes(ps, s, SF_NEE, c);
+#pragma unroll
+                    for (int k = 0; k < kNumL; ++k) c[k] *= T;
+                    storePlanes(ps, s, SF_NEE, c);
+                }
+            }
+        }
+        if (used) ps.u4(s, SF_RNG) = rng;
+        events += doExt && ps.code[s] == CODE_CLS0 + CLS_MEDIUM ? 1u : 0u;
+    }
+    waveAdd64(&ps.stat[S_MED_EVENTS], events);
+    waveAdd64(&ps.stat[S_MED_SHADOWS], shadows);
+}
+
+// Shading at a scattering point in the medium (VolumetricPTRenderer::Job::
+// contribution, libSLR/Renderer/VolumetricPTRenderer.cpp:179-296, for a
+// MediumPoint): Russian roulette, next-event estimation with the phase
+// function (its value is the reference's VolumetricBSDF value albedo *
+// phase, the geometry term has no cosine at the medium point, the light
+// sample's transmittance is applied by k_medium), then a new direction
+// sampled from the phase function, which multiplies the throughput by the
+// albedo sigma_s / sigma_e. The free-path estimator's sigma_e / sigma_e
+// (hero) factor is one for grey extinction.
+__global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState ps) {
+    const uint32_t count = ps.ctr[C_CLS0 + CLS_MEDIUM];
+    const uint32_t* qin = queue(ps, Q_CLS0 + CLS_MEDIUM);
+    unsigned long long samples = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        if (i >= count) continue;
+        const uint32_t s = qin[i];
+        uint8_t code = CODE_IDLE;
+        bool cont = false, shadow = false;
+        uint4 info = ps.u4(s, SF_INFO);
+        float4 misc = ps.f4(s, SF_MISC);
+        uint4 rng = ps.u4(s, SF_RNG);
+        const float4 o4 = ps.f4(s, SF_RO), d4 = ps.f4(s, SF_RD), h = ps.f4(s, SF_HIT);
+        const float3 rd = xyz(d4);
+        const float3 pos = xyz(o4) + rd * h.x;
+        const int hero = (int)((info.y >> 8) & 0xFFu);
+        uint32_t pathLen = info.y >> 16;
+        float alpha[kNumL], rad[kNumL], albedo[kNumL];
+        if (pathLen == 0) {
+#pragma unroll
+            for (int k = 0; k < kNumL; ++k) alpha[k] = 1.0f;
+        } else {
+            loadPlanes(ps, s, SF_ALPHA, alpha);
+        }
+        loadRadiance(ps, s, pathLen, rad);
+        bool alive = true;
+        if (pathLen > 0) {
+            const float p = fminf(importance16(alpha, hero) / misc.y, 1.0f);
+            if (xsFloat(rng) < p) {
+#pragma unroll
+                for (int k = 0; k < kNumL; ++k) alpha[k] /= p;
+            } else alive = false;
+        }
+        if (alive) {
+            ++pathLen;
+            if (pathLen >= ps.jobs[ps.u4(s, SF_JOB).x].maxDepth) alive = false;
+        }
+        if (alive) {
+            // albedo: sigma_s / sigma_e (DensityGridMediumDistribution::evaluateAlbedo)
+            evalSpectrum(sc, sc.medSigmaS, misc.x, albedo);
+            const float invE = 1.0f / sc.medSigmaE;
+#pragma unroll
+            for (int k = 0; k < kNumL; ++k) albedo[k] = albedo[k] * invE;
+            // ---- next event estimation
+            const float uSel = xsFloat(rng);
+            const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
+            bool envLight = false;
+            float uSurf = uSel;
+            if (sc.hasEnv) {
+                const float surfImp = sc.lightImportance, sumImps = surfImp + 1.0f;
+                if (sumImps * uSel < surfImp) uSurf = uSel / (surfImp / sumImps);
+                else envLight = true;
+            }
+            float3 sdir = f3(0, 0, 1);
+            float tmax = 0.0f, lightPDF = 0.0f, pdfConv = 1.0f, geo = 0.0f;
+            float Le[kNumL];
+            bool ok = false;
+            if (envLight) {
+                float d0, d1, uvPdf;
+                envSample(sc, lu0, lu1, &d0, &d1, &uvPdf);
+                const float phi = (float)(d0 * (2 * kPiD)), theta = (float)(d1 * kPiD);
+                const float sinT = sinf(theta);
+                const float3 lpos = f3(-sinf(phi) * sinT, cosf(theta), cosf(phi) * sinT);
+                const float areaPDF = (float)(uvPdf / (2 * kPiD * kPiD * sinT));
+                sdir = normalize3(lpos);
+                envRadiance(sc, (float)(phi / (2 * kPiD)), (float)(theta / kPiD), misc.x, Le);
+                lightPDF = envSelectProb(sc) * areaPDF;
+                pdfConv = 1.0f;   // unit distance, cosine one at infinity
+                geo = 1.0f;
+                tmax = FLT_MAX;
+                ok = areaPDF > 0.0f && isfinite(areaPDF);
+            } else if (sc.numLights > 0 && sc.lightImportance > 0.0f) {
+                float lprob;
+                const uint32_t li = sampleLight(sc, uSurf, &lprob);
+                const uint2 lent = sc.lights[li];
+                const int32_t linst = lent.y == kNoInstance ? -1 : (int32_t)lent.y;
+                const float su = sqrtf(lu0);
+                const float lb0 = 1.0f - su, lb1 = lu1 * su, lb2 = 1.0f - lb0 - lb1;
+                float3 q0, q1, q2;
+                uint4 lv;
+                worldVerts(sc, lent.x, linst, &q0, &q1, &q2, &lv);
+                float3 ln = normalize3(lb0 * xyz(sc.vnrm[lv.x]) + lb1 * xyz(sc.vnrm[lv.y]) + lb2 * xyz(sc.vnrm[lv.z]));
+                if (linst >= 0) {
+                    ln = normalize3(xfNormal(sc.instXf + kInstXfRows * (uint32_t)linst, ln));
+                    if (sc.instMat[linst] >= 0) lv.w = (uint32_t)sc.instMat[linst];
+                }
+                const float3 lp = lb0 * q0 + lb1 * q1 + lb2 * q2;
+                const float3 lg = normalize3(cross3(q1 - q0, q2 - q0));
+                const float areaPDF = 1.0f / (0.5f * length3(cross3(q1 - q0, q2 - q0)));
+                const float3 dv = lp - pos;
+                const float dist2 = dot3(dv, dv);
+                const float dist = sqrtf(dist2);
+                sdir = divs(dv, dist);
+                const float cosL = fabsf(dot3(sdir, lg));
+                if (cosL > 0.0f && dot3(ln, -sdir) > 0.0f) {
+                    evalSpectrum(sc, sc.mats[lv.w].emit, misc.x, Le);
+                    const float edf = (float)(1.0 / 3.14159265358979323846);
+#pragma unroll
+                    for (int k = 0; k < kNumL; ++k) Le[k] *= edf;
+                    lightPDF = lprob * lightSelectScale(sc) * areaPDF;
+                    pdfConv = cosL / dist2;
+                    geo = cosL / dist2;
+                    tmax = dist * (1 - 1e-4f);
+                    ok = true;
+                }
+            }
+            if (ok) {
+                const float ph = hgValue(sc.medG, dot3(rd, sdir));
+                const float phPDF = ph * pdfConv;
+                const float mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + phPDF * phPDF);
+                const float scal = ph * geo * mis / lightPDF;
+                float c[kNumL];
+                bool any = false;
+#pragma unroll
+                for (int k = 0; k < kNumL; ++k) {
+                    c[k] = alpha[k] * Le[k] * albedo[k] * scal;
+                    any = any || c[k] != 0.0f;
+                }
+                if (any) {
+                    storePlanes(ps, s, SF_NEE, c);
+                    ps.f4(s, SF_SO) = make_float4(pos.x, pos.y, pos.z, 1e-4f);
+                    ps.f4(s, SF_SD) = make_float4(sdir.x, sdir.y, sdir.z, tmax);
+                    ps.f4(s, SF_SINV) = rayInv(sdir);
+                    shadow = true;
+                }
+            }
+            // ---- phase-function sampling
+            {
+                const float u0 = xsFloat(rng), u1 = xsFloat(rng);
+                float value;
+                const float3 local = hgSample(sc.medG, u0, u1, &value);
+                float3 fx, fy;
+                coordinateSystem(rd, &fx, &fy);
+                const float3 nd = normalize3(fx * local.x + fy * local.y + rd * local.z);
+                bool nonzero = false;
+#pragma unroll
+                for (int k = 0; k < kNumL; ++k) {
+                    alpha[k] *= albedo[k];   // albedo * phase / pdf, the phase value being its pdf
+                    nonzero = nonzero || alpha[k] != 0.0f;
+                }
+                if (nonzero && value > 0.0f) {
+                    ps.f4(s, SF_RO) = make_float4(pos.x, pos.y, pos.z, 1e-4f);
+                    ps.f4(s, SF_RD) = make_float4(nd.x, nd.y, nd.z, INFINITY);
+                    ps.f4(s, SF_RINV) = rayInv(nd);
+                    misc.z = value;
+                    info.y &= ~F_LAST_DELTA;
+                    cont = true;
+                }
+            }
+        }
+        info.y = (info.y & 0xFFFFu) | (pathLen << 16);
+        if (cont || shadow) {
+            storePlanes(ps, s, SF_SP, rad);
+            if (cont) storePlanes(ps, s, SF_ALPHA, alpha);
+            ps.u4(s, SF_INFO) = info;
+            ps.f4(s, SF_MISC) = misc;
+            ps.u4(s, SF_RNG) = rng;
+            code = (uint8_t)((cont ? CODE_EXT : CODE_DONE) | (shadow ? CODE_SHADOW : 0));
+        } else {
+            code = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
+        }
+        ps.code[s] = code;
+    }
+    waveAdd64(&ps.stat[S_SAMPLES], samples);
+}
+
+// Ray batches through the medium: mode 0 one ratio-tracking transmittance
+// estimate of [tmin, tmax] per ray (MediumObjectAggregate::evaluateTransmittance,
+// libSLR/Core/medium_object.cpp:348-399, over the density grid), mode 1 one
+// delta-tracking free-path distance (the scattering distance, or tmax when
+// the ray leaves [tmin, tmax] unscattered; MediumObjectAggregate::interact,
+// :300-347). Ray i draws from its own XORShift stream seeded from (seed, i).
+__global__ __launch_bounds__(kBlock) void k_medium_query(DevScene sc, TraceArgs a, uint32_t n, int mode,
+                                                         uint32_t seed, float* out) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float3 o = f3(a.ox[i], a.oy[i], a.oz[i]), d = f3(a.dx[i], a.dy[i], a.dz[i]);
+        const float t0 = a.tmin[i], t1 = a.tmax[i];
+        uint32_t h = seed ^ (0x9E3779B9u * (i + 1u));
+        h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
+        uint4 rng = xsSeed(h);
+        if (mode == 0) {
+            out[i] = mediumTransmittance(sc, o, d, t0, t1, rng);
+        } else {
+            float ts = t1;
+            if (!sampleFreePath(sc, o, d, t0, t1, rng, &ts)) ts = t1;
+            out[i] = ts;
+        }
+    }
+}
+
+int mediumQuery(DeviceScene* ds, const TraceArgs& a, uint32_t n, int mode, uint32_t seed, float* out,
+                bool devicePtrs, void* stream, std::string* err) {
+    if (n == 0) return 0;
+    if (!ds->view.hasMedium) { if (err) *err = "scene has no medium"; return SLRH_ERR_STATE_CODE; }
+    hipStream_t s = (hipStream_t)stream;
+    TraceArgs d = a;
+    float* dout = out;
+    std::vector<void*> temps;
+    auto cleanup = [&]() { for (void* p : temps) (void)hipFree(p); };
+    if (!devicePtrs) {
+        const float* hin[8] = {a.ox, a.oy, a.oz, a.dx, a.dy, a.dz, a.tmin, a.tmax};
+        const float** din[8] = {&d.ox, &d.oy, &d.oz, &d.dx, &d.dy, &d.dz, &d.tmin, &d.tmax};
+        for (int k = 0; k < 9; ++k) {
+            void* p = nullptr;
+            if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) { cleanup(); if (err) *err = "hipMalloc"; return SLRH_ERR_NOMEM_CODE; }
+            temps.push_back(p);
+            if (k < 8) {
+                (void)hipMemcpyAsync(p, hin[k], n * sizeof(float), hipMemcpyHostToDevice, s);
+                *din[k] = (const float*)p;
+            } else {
+                dout = (float*)p;
+            }
+        }
+    }
+    const uint32_t blocks = std::max(1u, std::min((uint32_t)g_numCUs * 8, (n + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_medium_query, dim3(blocks), dim3(kBlock), 0, s, ds->view, d, n, mode, seed, dout);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { cleanup(); if (err) *err = std::string("k_medium_query: ") + hipGetErrorString(e); return SLRH_ERR_DEVICE_CODE; }
+    if (!devicePtrs) (void)hipMemcpyAsync(out, dout, n * sizeof(float), hipMemcpyDeviceToHost, s);
+    e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) { if (err) *err = std::string("medium query: ") + hipGetErrorString(e); return SLRH_ERR_DEVICE_CODE; }
+    return 0;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_compact(PathState ps) {
     constexpr int PER = 8;
@@ -1699,6 +2223,7 @@ struct SlotPool {
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graphVariant = -1;
+    uint32_t graphMode = 0;     // volume | tangentNeeNaN << 1, as the graph was recorded
     PathState ps;
     DevScene sc;
     JobRun runs[kMaxJobs];
@@ -1844,11 +2369,13 @@ static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
     if (ev) (void)hipEventRecord((*ev)[1], s);
     kTraceLaunch[tv](p.blocks, s, p.sc, p.ps);
     if (ev) (void)hipEventRecord((*ev)[2], s);
+    if (p.sc.volume) hipLaunchKernelGGL(k_medium, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, s, p.ps);
     hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
+    if (p.sc.volume) hipLaunchKernelGGL(k_shade_medium, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     if (ev) (void)hipEventRecord((*ev)[3], s);
     hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, s, p.ps);
     if (ev) (void)hipEventRecord((*ev)[4], s);
@@ -1858,7 +2385,8 @@ static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
 // The kernels take their sizes from device counters, so one recording
 // serves every replay.
 static int launchGraphBlock(SlotPool& p, int tv, std::string* err) {
-    if (!p.graph || p.graphVariant != tv) {
+    const uint32_t mode = p.sc.volume | p.sc.tangentNeeNaN << 1;
+    if (!p.graph || p.graphVariant != tv || p.graphMode != mode) {
         if (p.graph) { (void)hipGraphExecDestroy(p.graph); p.graph = nullptr; }
         hipGraph_t g;
         HIP_TRY(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
@@ -1868,6 +2396,7 @@ static int launchGraphBlock(SlotPool& p, int tv, std::string* err) {
         (void)hipGraphDestroy(g);
         HIP_TRY(e);
         p.graphVariant = tv;
+        p.graphMode = mode;
     }
     HIP_TRY(hipGraphLaunch(p.graph, p.stream));
     return 0;
@@ -2054,6 +2583,16 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
     }
     SlotPool& p = *ds->pool;
     int r = 0;
+    // the volumetric integrator: media take part, and the path length has no
+    // fixed cap (VolumetricPTRenderer.cpp:184-293 ends paths by Russian
+    // roulette only) unless the settings set one
+    p.sc.volume = opt.volumetric && ds->view.hasMedium ? 1u : 0u;
+    p.sc.tangentNeeNaN = opt.volumetric && ds->view.compatTangentNee ? 1u : 0u;
+    std::vector<JobSpec> jobSpecs(specs, specs + n);
+    if (opt.volumetric)
+        for (JobSpec& js : jobSpecs)
+            if (js.maxDepth == 0) js.maxDepth = 0xFFFFu;
+    specs = jobSpecs.data();
     int tv = opt.traceVariant >= 0 && opt.traceVariant < kNumVariants ? opt.traceVariant : 0;
     if (opt.traceVariant < 0) {
         // automatic: tuned once per scene and pool size
@@ -2111,6 +2650,8 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
         st->trisTested = (double)stat[S_TRIS_EXT];
         st->nodesShadow = (double)stat[S_NODES_SH];
         st->trisShadow = (double)stat[S_TRIS_SH];
+        st->mediumEvents = (double)stat[S_MED_EVENTS];
+        st->mediumShadows = (double)stat[S_MED_SHADOWS];
         st->kernelSeconds = ms * 1e-3;
         st->nonFinite = p.hostCtr[C_NONFINITE];
         st->overflow = p.hostCtr[C_OVERFLOW];
@@ -2150,6 +2691,7 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
     opt.timeKernels = cfg.timeKernels;
     opt.traceVariant = cfg.traceVariant;
     opt.useGraph = cfg.useGraph;
+    opt.volumetric = cfg.volumetric;
     return renderJobs(ds, &js, 1, opt, st, err);
 }
 

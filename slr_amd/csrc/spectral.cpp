@@ -293,4 +293,24 @@ void cmfStrata(float xbar[16], float ybar[16], float zbar[16], float* integral) 
     *integral = integ;
 }
 
+// Supervoxel majorants. Cell c of an axis covers lattice intervals
+// [kSuperVoxel c, kSuperVoxel (c + 1)] (clipped to the lattice); trilinear
+// interpolation anywhere in it is a convex combination of the lattice points
+// in that range, so their maximum bounds the density there.
+void buildMajorants(MediumData* m) {
+    for (int a = 0; a < 3; ++a) m->cells[a] = std::max(1u, (m->n[a] - 1 + kSuperVoxel - 1) / kSuperVoxel);
+    m->majorant.assign((size_t)m->cells[0] * m->cells[1] * m->cells[2], 0.0f);
+    const uint32_t nx = m->n[0], ny = m->n[1], nz = m->n[2];
+    for (uint32_t cz = 0; cz < m->cells[2]; ++cz)
+        for (uint32_t cy = 0; cy < m->cells[1]; ++cy)
+            for (uint32_t cx = 0; cx < m->cells[0]; ++cx) {
+                float mx = 0.0f;
+                for (uint32_t z = cz * kSuperVoxel; z <= std::min(nz - 1, (cz + 1) * kSuperVoxel); ++z)
+                    for (uint32_t y = cy * kSuperVoxel; y <= std::min(ny - 1, (cy + 1) * kSuperVoxel); ++y)
+                        for (uint32_t x = cx * kSuperVoxel; x <= std::min(nx - 1, (cx + 1) * kSuperVoxel); ++x)
+                            mx = std::max(mx, m->density[((size_t)z * ny + y) * nx + x]);
+                m->majorant[((size_t)cz * m->cells[1] + cy) * m->cells[0] + cx] = mx;
+            }
+}
+
 }  // namespace slrh

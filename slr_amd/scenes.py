@@ -360,6 +360,77 @@ def instanced_cornell(width=64, height=64, n_instances=8, seed=11, uniform_scale
     return desc
 
 
+def fbm_density(n, seed=5, octaves=5, freq=2.0, bias=0.1, scale=2.5):
+    """(n, n, n) lattice (z, y, x) of max(0, scale * (fBm(p) + bias)) over the
+    unit cube: a seeded Perlin fBm cloud of densities mostly in [0, 1]."""
+    g = (np.arange(n) + 0.0) / (n - 1)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], -1)
+    d = np.maximum(0.0, scale * (fbm(p, seed, octaves=octaves, freq=freq) + bias))
+    return d.reshape(n, n, n).astype(np.float32)
+
+
+def cornell_medium(width=64, height=64, n=32, sigma_t=4.0, albedo=0.9, g=0.7, seed=5, octaves=5):
+    """The Cornell box walls and light enclosing a heterogeneous medium on an
+    n^3 lattice (seeded fBm densities times sigma_t, albedo, Henyey-Greenstein
+    g) that fills most of the box."""
+    desc = cornell_box(width, height)
+    desc["meshes"] = desc["meshes"][:6]          # walls and light
+    lo = np.array([30.0, 10.0, 30.0]) * SCALE
+    hi = np.array([520.0, 500.0, 520.0]) * SCALE
+    desc["medium"] = {"bounds": np.concatenate([lo, hi]).astype(np.float32),
+                      "density": fbm_density(n, seed=seed, octaves=octaves),
+                      "sigma_s": albedo * sigma_t, "sigma_e": sigma_t, "g": g}
+    return desc
+
+
+def absorbing_slab(width=16, height=16, sigma=1.5, box=((-1.0, -1.0, 1.0), (1.0, 1.0, 3.0))):
+    """A uniform white environment seen through a box of constant density
+    that absorbs (sigma_s = 0, sigma_e = sigma): each camera ray keeps
+    exp(-sigma * length inside the box) of the sky (see pixel_transmittance)."""
+    desc = env_furnace(width, height, quad_in_view=False)
+    lo, hi = np.asarray(box[0], np.float32), np.asarray(box[1], np.float32)
+    desc["medium"] = {"bounds": np.concatenate([lo, hi]).astype(np.float32),
+                      "density": np.ones((4, 4, 4), np.float32), "sigma_s": 0.0, "sigma_e": sigma, "g": 0.0}
+    return desc
+
+
+def pixel_transmittance(desc, width, height, n=32):
+    """pixel_cos4's per-pixel weight with each sub-pixel ray attenuated by
+    exp(-sigma_e * density * chord length) through the medium box (constant
+    density): the film of absorbing_slab under a unit sky."""
+    cam = desc["camera"]
+    med = desc["medium"]
+    sig = med["sigma_e"] * float(np.asarray(med["density"]).max())
+    lo, hi = np.asarray(med["bounds"][:3], np.float64), np.asarray(med["bounds"][3:], np.float64)
+    opH = 2.0 * cam["obj_dist"] * math.tan(cam["fov_y"] * 0.5)
+    opW = opH * cam["aspect"]
+    area = opW * opH * (cam["img_dist"] / cam["obj_dist"]) ** 2
+    l2w = np.asarray(cam["l2w"], np.float64).reshape(4, 4)
+    g = (np.arange(n) + 0.5) / n
+    out = np.zeros((height, width))
+    for py in range(height):
+        for px in range(width):
+            ux = (px + g[:, None] + 0 * g[None, :]) / width
+            uy = (py + g[None, :] + 0 * g[:, None]) / height
+            x = opW * (0.5 - ux)
+            y = opH * (0.5 - uy)
+            z = np.full_like(x, cam["obj_dist"])
+            dl = np.stack([x, y, z], -1).reshape(-1, 3)
+            cos4 = (dl[:, 2] ** 2 / (dl ** 2).sum(1)) ** 2
+            d = dl @ l2w[:3, :3].T
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            o = l2w[:3, 3]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t0 = (lo - o) / d
+                t1 = (hi - o) / d
+            tn = np.maximum(np.minimum(t0, t1).max(1), 0.0)
+            tf = np.maximum(t0, t1).min(1)
+            L = np.maximum(tf - tn, 0.0)
+            out[py, px] = np.mean(cos4 * np.exp(-sig * L)) * area / cam["img_dist"] ** 2
+    return out
+
+
 def sky_environment(width=256, height=128, sun_dir=(0.4, 0.8, 0.45), sun_radiance=1e4, sun_radius_deg=1.5,
                     zenith=(0.25, 0.45, 1.0), horizon=(1.0, 0.95, 0.85), ground=(0.2, 0.18, 0.15)):
     """Equirectangular sky: a zenith-to-horizon gradient, a darker ground and a

@@ -74,3 +74,39 @@ def test_reference_environment_closed_form():
         out.append(_xyz(f.astype(np.float64), cmf)[..., 1] / spp / scenes.pixel_cos4(desc, W, H))
     assert abs(out[0].mean() - 1) < 0.01, out[0].mean()
     assert abs(out[1].mean() - 0.5) < 0.01, out[1].mean()
+
+
+def test_oracle_volumetric_compat_matches_reference_volumetric_renderer():
+    """The reference VolumetricPTRenderer over the Cornell box without media
+    against the oracle in its volumetric compat mode (a visible light sample
+    in the shading plane turns the sample into NaN, which is not filed), 3
+    sigma per pixel and channel. The same oracle without the compat mode is
+    the path tracer, whose image carries about twice the energy: the light
+    is an emitter over a Lambert base, so a path that reaches it samples a
+    light point in its own plane, and the reference drops every sample that
+    sees the light directly."""
+    refdrv = need_ref()
+    W = H = 16
+    B, spp = 12, 32
+    desc = dict(scenes.cornell_box(W, H), compat_volpt_tangent_nee=True)
+    cmf = refdrv.cmf_strata()
+    threads = max(1, min(8, len(os.sched_getaffinity(0))))
+    rb, ob, pb = [], [], []
+    orc = oracle.OracleScene(desc)
+    for b in range(B):
+        r = refdrv.RefScene(desc)
+        f, _, _ = r.render(W, H, spp, 700 + b, threads=threads, volumetric=True)
+        r.close()
+        rb.append(_xyz(f.astype(np.float64), cmf) / spp)
+        f2, _ = orc.render(W, H, spp, 300 + b, threads=threads, volumetric=True)
+        ob.append(_xyz(f2, cmf) / spp)
+        f3, _ = orc.render(W, H, spp, 300 + b, threads=threads)
+        pb.append(_xyz(f3, cmf) / spp)
+    rb, ob, pb = np.array(rb), np.array(ob), np.array(pb)
+    se = np.sqrt(rb.var(0, ddof=1) / B + ob.var(0, ddof=1) / B)
+    z = np.abs(ob.mean(0) - rb.mean(0)) / np.maximum(se, 1e-12)
+    assert np.mean(z > 3) < 0.03, np.mean(z > 3)
+    tr, to, tp = rb.sum(axis=(1, 2, 3)), ob.sum(axis=(1, 2, 3)), pb.sum(axis=(1, 2, 3))
+    se_t = np.sqrt(tr.var(ddof=1) / B + to.var(ddof=1) / B)
+    assert abs(to.mean() - tr.mean()) < 4 * se_t, (to.mean(), tr.mean(), se_t)
+    assert tr.mean() < 0.7 * tp.mean(), (tr.mean(), tp.mean())
