@@ -145,8 +145,8 @@ enum slrh_compat_flags {
      * surface as the reference VolumetricPTRenderer does: its next-event
      * estimate has no cosine guard (libSLR/Renderer/VolumetricPTRenderer.cpp:
      * 190-231), the BSDF's shading-normal correction is then 0/0
-     * (libSLR/Core/directional_distribution_functions.h:322-325), and the NaN
-     * drops the whole sample (VolumetricPTRenderer.cpp:130-138). Such samples
+     * (libSLR/Core/directional_distribution_functions.h:325-327), and the NaN
+     * drops the whole sample (VolumetricPTRenderer.cpp:133-138). Such samples
      * are counted in the non-finite statistic. Off: the estimate is zero, as
      * in SLRH_PT (PTRenderer.cpp:199), and a scene without media renders
      * exactly as with SLRH_PT. */
@@ -352,7 +352,7 @@ SLRH_API int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t
 typedef struct slrh_server slrh_server;
 
 typedef struct slrh_server_config {
-    uint64_t path_state_bytes;   /* device memory for path state; bounds the live paths (~414 B each) */
+    uint64_t path_state_bytes;   /* device memory for path state; bounds the live paths (~418 B each) */
     uint32_t tune_trace;         /* traversal kernel variant + 1 (0 = the one tuned by a render) */
     uint32_t reserved;           /* must be 0 */
 } slrh_server_config;

@@ -17,11 +17,17 @@
  * rule (TriangleSurfaceShape.cpp:205-299), which yields the same closest hit
  * as any correct traversal except for exact ties in t.
  *
+ * With orc_scene_compat bit 0 and no path-length cap the estimator is
+ * VolumetricPTRenderer::Job::contribution (VolumetricPTRenderer.cpp:145-296)
+ * over a scene without media, including its NaN for a light sample in the
+ * shading plane.
+ *
  * Pinned by: tests/test_oracle.py (XORShift streams, wavelength samples,
  * sampling helpers, film binning and BSDF sample/evaluate/PDF tuples against
  * the compiled reference; ray hits against the reference QBVH) and
- * tests/test_oracle_render.py (images against the reference PTRenderer
- * within 3 sigma, white-furnace closed form).
+ * tests/test_oracle_render.py (images against the reference PTRenderer and,
+ * in the compat mode, VolumetricPTRenderer within 3 sigma; white-furnace
+ * closed form).
  */
 #include <math.h>
 #include <stdint.h>
@@ -778,8 +784,8 @@ static void trace_sample(const scene_t* s, uint32_t st[4], uint32_t px, uint32_t
             if (s->tangentNeeNaN && cosS == 0.0f) {
                 /* VolumetricPTRenderer.cpp:198-230 has no cosine guard: a visible
                  * light point in the shading point's plane makes the BSDF's
-                 * shading-normal correction 0/0 (directional_distribution_functions.h:322-325),
-                 * a NaN that drops the whole sample (VolumetricPTRenderer.cpp:130-138) */
+                 * shading-normal correction 0/0 (directional_distribution_functions.h:325-327),
+                 * a NaN that drops the whole sample (VolumetricPTRenderer.cpp:133-138) */
                 float dist = len(sub(lpt, f.p));
                 v3 vd = divs(sub(lpt, f.p), dist);
                 ++*rays;

@@ -1624,7 +1624,7 @@ void k_shade(DevScene sc, PathState ps) {
                     // VolumetricPTRenderer.cpp:190-231 (with SLRH_COMPAT_VOLPT_TANGENT_NEE):
                     // no cosine guard, so a light sample in the shading plane
                     // gives 0/0 in the BSDF's shading-normal correction
-                    // (directional_distribution_functions.h:322-325); if it is
+                    // (directional_distribution_functions.h:325-327); if it is
                     // visible the NaN reaches the radiance and the sample is
                     // not filed
                     auto tangentNaN = [&](float3 dir, float tmax) {

@@ -360,14 +360,132 @@ def instanced_cornell(width=64, height=64, n_instances=8, seed=11, uniform_scale
     return desc
 
 
-def fbm_density(n, seed=5, octaves=5, freq=2.0, bias=0.1, scale=2.5):
+def fbm_density(n, seed=5, octaves=5, freq=2.0, bias=0.1, scale=2.5, chunk_points=1 << 20):
     """(n, n, n) lattice (z, y, x) of max(0, scale * (fBm(p) + bias)) over the
-    unit cube: a seeded Perlin fBm cloud of densities mostly in [0, 1]."""
+    unit cube: a seeded Perlin fBm cloud of densities mostly in [0, 1].
+    Evaluated a few z slices at a time to bound the temporaries."""
     g = (np.arange(n) + 0.0) / (n - 1)
-    z, y, x = np.meshgrid(g, g, g, indexing="ij")
-    p = np.stack([x.ravel(), y.ravel(), z.ravel()], -1)
-    d = np.maximum(0.0, scale * (fbm(p, seed, octaves=octaves, freq=freq) + bias))
-    return d.reshape(n, n, n).astype(np.float32)
+    y, x = np.meshgrid(g, g, indexing="ij")
+    out = np.empty((n, n, n), np.float32)
+    step = max(1, chunk_points // (n * n))
+    for z0 in range(0, n, step):
+        zs = g[z0:z0 + step]
+        p = np.stack([np.tile(x.ravel(), len(zs)), np.tile(y.ravel(), len(zs)), np.repeat(zs, n * n)], -1)
+        d = np.maximum(0.0, scale * (fbm(p, seed, octaves=octaves, freq=freq) + bias))
+        out[z0:z0 + len(zs)] = d.reshape(len(zs), n, n)
+    return out
+
+
+def terrain(n=316, size=4.0, height=0.6, seed=13, octaves=5):
+    """Seeded height field: an (n+1)^2 vertex grid over a size x size square
+    in xz centred at the origin, heights height * fBm; 2 n^2 triangles
+    (199,712 at n = 316), smooth normals."""
+    g = (np.arange(n + 1) / n - 0.5) * size
+    z, x = np.meshgrid(g, g, indexing="ij")
+    q = np.stack([x.ravel() / size * 3.0, np.zeros(x.size), z.ravel() / size * 3.0], -1)
+    y = height * fbm(q + 0.37, seed, octaves=octaves, freq=1.0)
+    pos = np.stack([x.ravel(), y, z.ravel()], -1)
+    i = np.arange(n)
+    a = (i[:, None] * (n + 1) + i[None, :]).ravel()
+    b, c, d = a + 1, a + n + 1, a + n + 2
+    idx = np.concatenate([np.stack([a, c, b], 1), np.stack([b, c, d], 1)]).astype(np.uint32)
+    return pos, idx
+
+
+def torus_knot(p=2, q=3, segments=500, sides=50, radius=1.0, tube=0.15):
+    """(p, q) torus knot swept by a circular tube: segments * sides * 2
+    triangles (50,000 by default), smooth normals from the sweep frame."""
+    t = np.arange(segments) / segments * 2 * np.pi
+
+    def curve(t):
+        r = radius * (2 + np.cos(q * t)) / 3
+        return np.stack([r * np.cos(p * t), radius * np.sin(q * t) / 3, r * np.sin(p * t)], -1)
+
+    c = curve(t)
+    tg = curve(t + 1e-4) - curve(t - 1e-4)
+    tg /= np.linalg.norm(tg, axis=1, keepdims=True)
+    nb = np.cross(tg, c)
+    nb /= np.linalg.norm(nb, axis=1, keepdims=True)
+    nn = np.cross(nb, tg)
+    a = np.arange(sides) / sides * 2 * np.pi
+    nrm = np.cos(a)[None, :, None] * nn[:, None, :] + np.sin(a)[None, :, None] * nb[:, None, :]
+    pos = c[:, None, :] + tube * nrm
+    s_, k_ = np.meshgrid(np.arange(segments), np.arange(sides), indexing="ij")
+    v00 = s_ * sides + k_
+    v10 = ((s_ + 1) % segments) * sides + k_
+    v01 = s_ * sides + (k_ + 1) % sides
+    v11 = ((s_ + 1) % segments) * sides + (k_ + 1) % sides
+    idx = np.concatenate([np.stack([v00.ravel(), v01.ravel(), v10.ravel()], 1),
+                          np.stack([v10.ravel(), v01.ravel(), v11.ravel()], 1)]).astype(np.uint32)
+    return pos.reshape(-1, 3), nrm.reshape(-1, 3), idx
+
+
+def _proto(pos, idx, nrm=None, mat=0):
+    pos = np.asarray(pos, np.float64)
+    nrm = smooth_normals(pos, idx) if nrm is None else np.asarray(nrm, np.float64)
+    nrm = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    return {"pos": pos.astype(np.float32), "nrm": nrm.astype(np.float32), "tan": _tangent_of(nrm).astype(np.float32),
+            "idx": np.asarray(idx, np.uint32), "mat": mat}
+
+
+def random_material(rng):
+    """One of Lambert (grey albedo U[0.1, 0.9]), GGX conductor (alpha U[0.05,
+    0.5]), GGX dielectric (IOR 1.5, alpha 0.1) and Oren-Nayar (sigma 0.3),
+    drawn uniformly."""
+    k = rng.randint(4)
+    if k == 0:
+        a = float(rng.uniform(0.1, 0.9))
+        return {"kind": "lambert", "rgb": (a, a, a)}
+    if k == 1:
+        return {"kind": "ggx_conductor", "eta": 0.2, "k": 3.9, "alpha": float(rng.uniform(0.05, 0.5))}
+    if k == 2:
+        return {"kind": "ggx_dielectric", "eta_ext": 1.0, "eta_int": 1.5, "alpha": 0.1}
+    return {"kind": "oren_nayar", "rgb": (0.6, 0.5, 0.4), "sigma": 0.3}
+
+
+def instanced_field(width=1920, height=1080, n_instances=256, seed=21, subdiv=8, terrain_n=316,
+                    knot=(500, 50), env_size=(2048, 1024), sun_radiance=1e4, materials_per_instance=True):
+    """Instances under an image-based sky: n_instances seeded random affine
+    instances of four prototypes, a displaced icosphere (subdivision subdiv,
+    1,310,720 triangles at 8), a height-field terrain (199,712 triangles at
+    316), a torus knot (50,000 triangles at 500 x 50) and a 12-triangle box,
+    each instance with a material drawn by random_material, lit by
+    sky_environment at env_size with a sun of sun_radiance. Without
+    materials_per_instance every (prototype, material) pair is its own
+    prototype, the form a scene graph without per-instance materials takes."""
+    rng = np.random.RandomState(seed)
+    sph_p, sph_f = displaced_icosphere(subdiv, 1.0, (0.0, 0.0, 0.0), seed=seed)
+    ter_p, ter_f = terrain(terrain_n, seed=seed + 1)
+    kp, kn, kf = torus_knot(segments=knot[0], sides=knot[1])
+    bp, bn, bf = _box((0.0, 0.0, 0.0), (1.6, 1.6, 1.6), 0.0)
+    shapes = [(sph_p, sph_f, None), (ter_p, ter_f, None), (kp, kf, kn), (bp, bf, bn)]
+    materials, insts = [], []
+    for i in range(n_instances):
+        m = random_affine(rng, np.array([-12.0, 0.0, 4.0]), np.array([12.0, 3.0, 30.0]), scale_lo=0.5,
+                          scale_hi=1.5)
+        materials.append(random_material(rng))
+        insts.append({"proto": int(rng.randint(4)), "l2w": m, "mat": i})
+    if materials_per_instance:
+        protos = [_proto(p, f, n, 0) for p, f, n in shapes]
+    else:
+        protos, key = [], {}
+        for inst in insts:
+            k = (inst["proto"], inst["mat"])
+            if k not in key:
+                key[k] = len(protos)
+                p, f, n = shapes[inst["proto"]]
+                protos.append(_proto(p, f, n, inst["mat"]))
+            inst["proto"] = key[k]
+            inst.pop("mat")
+    l2w = np.eye(4)
+    t = math.radians(-14.0)
+    l2w[:3, :3] = np.array([[1, 0, 0], [0, math.cos(t), -math.sin(t)], [0, math.sin(t), math.cos(t)]])
+    l2w[:3, 3] = np.array([0.0, 6.0, -8.0])
+    camera = {"l2w": l2w, "aspect": width / height, "fov_y": 0.9, "lens_radius": 0.0, "img_dist": 1.0,
+              "obj_dist": 1.0, "sensitivity": 1.0, "width": width, "height": height}
+    env = sky_environment(env_size[0], env_size[1], sun_radiance=sun_radiance)
+    return {"materials": materials, "meshes": [], "prototypes": protos, "instances": insts, "camera": camera,
+            "environment": {"rgb": env, "scale": 1.0}}
 
 
 def cornell_medium(width=64, height=64, n=32, sigma_t=4.0, albedo=0.9, g=0.7, seed=5, octaves=5):
@@ -506,7 +624,9 @@ def camera_pose(desc, yaw_deg=0.0, dx=0.0, dy=0.0, width=None, height=None):
     scene's centre and shifted by (dx, dy) in its own plane; aspect from
     width/height when given."""
     cam = dict(desc["camera"])
-    allp = np.concatenate([m["pos"] for m in desc["meshes"]]).astype(np.float64)
+    pts = [np.asarray(m["pos"], np.float64) for m in desc["meshes"]]
+    pts += [np.asarray(i["l2w"], np.float64).reshape(-1, 4)[:3, 3][None] for i in desc.get("instances", [])]
+    allp = np.concatenate(pts)
     centre = 0.5 * (allp.min(0) + allp.max(0))
     l2w = np.array(cam["l2w"], dtype=np.float64).reshape(4, 4).copy()
     t = math.radians(yaw_deg)

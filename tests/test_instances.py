@@ -87,3 +87,37 @@ def test_flattened_scene_places_the_same_triangles():
         want = np.asarray(pr["pos"], np.float64) @ m[:, :3].T + m[:, 3]
         np.testing.assert_allclose(flat["meshes"][len(desc["meshes"]) + k]["pos"], want, rtol=1e-6, atol=1e-6)
         assert len(maps[k]) == len(pr["idx"])
+
+
+def test_instanced_field_generators():
+    """The instanced-field meshes have their configured sizes, the scene is
+    seed-deterministic, and its per-prototype form carries the same
+    instances and materials as the per-instance form."""
+    p, f = scenes.terrain()
+    assert len(f) == 199712 and f.max() < len(p)
+    kp, kn, kf = scenes.torus_knot()
+    assert len(kf) == 50000 and kf.max() < len(kp)
+    assert np.allclose(np.linalg.norm(kn, axis=1), 1.0)
+    kw = dict(n_instances=24, seed=9, subdiv=2, terrain_n=10, knot=(30, 6), env_size=(32, 16))
+    a = scenes.instanced_field(32, 32, **kw)
+    b = scenes.instanced_field(32, 32, materials_per_instance=False, **kw)
+    a2 = scenes.instanced_field(32, 32, **kw)
+    assert len(a["instances"]) == len(b["instances"]) == 24 and len(a["prototypes"]) == 4
+    kinds = {m["kind"] for m in a["materials"]}
+    assert kinds <= {"lambert", "ggx_conductor", "ggx_dielectric", "oren_nayar"} and len(kinds) >= 3
+    for ia, ib, ia2 in zip(a["instances"], b["instances"], a2["instances"]):
+        assert np.array_equal(ia["l2w"], ib["l2w"]) and np.array_equal(ia["l2w"], ia2["l2w"])
+        pb = b["prototypes"][ib["proto"]]
+        pa = a["prototypes"][ia["proto"]]
+        assert pb["mat"] == ia["mat"]
+        assert np.array_equal(pa["pos"], pb["pos"]) and np.array_equal(pa["idx"], pb["idx"])
+
+
+def test_fbm_density_matches_direct_evaluation():
+    n = 24
+    d = scenes.fbm_density(n, seed=3, octaves=3, chunk_points=5 * n * n)   # 5 z slices per chunk
+    g = np.arange(n) / (n - 1)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], -1)
+    want = np.maximum(0.0, 2.5 * (scenes.fbm(p, 3, octaves=3, freq=2.0) + 0.1)).reshape(n, n, n)
+    assert np.allclose(d, want.astype(np.float32), rtol=0, atol=1e-6)
