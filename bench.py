@@ -324,6 +324,9 @@ def serve(args):
         "saturated": {"p50_ms": round(st_sat["p50_ms"], 2), "p99_ms": round(st_sat["p99_ms"], 2)},
     }
     print(json.dumps(result), flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(json.dumps(result) + "\n")
 
 
 def main():

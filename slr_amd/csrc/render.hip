@@ -480,6 +480,16 @@ __device__ __forceinline__ void waveAdd64(unsigned long long* counter, unsigned 
     if (laneId() == 0 && v) atomicAdd(counter, v);
 }
 
+// Render statistics are spread over kStatSpread copies per counter (waves
+// pick one by their index; the host sums them): same-address atomics
+// serialise, and one per wave of a persistent grid on a single address
+// costs a visible share of a short kernel.
+constexpr uint32_t kStatSpread = 32;
+__device__ __forceinline__ void statAdd(unsigned long long* stat, uint32_t which, unsigned long long v) {
+    const uint32_t w = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kStatSpread - 1);
+    waveAdd64(stat + which * kStatSpread + w, v);
+}
+
 // ------------------------------------------------------------------ ray batches
 
 template <bool I>
@@ -650,6 +660,7 @@ constexpr uint32_t kShadowEntry = 0x80000000u;
 // flags in info.y
 constexpr uint32_t F_HAS_SAMPLE = 1u;    // slot holds a sample whose radiance must be filed
 constexpr uint32_t F_LAST_DELTA = 2u;    // last BSDF sample was a delta lobe
+constexpr uint32_t F_RAD = 4u;           // SF_SP holds the path's radiance so far (zero when clear)
 
 // Per-slot path record: 24 float4 (384 B, three 128-B cache lines), slot s
 // at rec[24 * s]. Each kernel works on a sparse, slot-ordered subset of the
@@ -1135,7 +1146,7 @@ __global__ void k_reset_pool(PathState ps) {
     }
     if (blockIdx.x == 0) {
         for (uint32_t i = threadIdx.x; i < kNumCtr; i += blockDim.x) ps.ctr[i] = 0;
-        for (uint32_t i = threadIdx.x; i < kNumStat; i += blockDim.x) ps.stat[i] = 0;
+        for (uint32_t i = threadIdx.x; i < kNumStat * kStatSpread; i += blockDim.x) ps.stat[i] = 0;
         for (uint32_t i = threadIdx.x; i < kMaxJobs; i += blockDim.x) ps.jobActive[i] = 0;
     }
 }
@@ -1164,11 +1175,17 @@ __device__ __forceinline__ void fileSample(const PathState& ps, uint32_t s, uint
     float add[kNumL];
 #pragma unroll
     for (int k = 0; k < kNumL; ++k) add[k] = 0.0f;
+    // the bin of wavelength k is k or k + 1 (k + 1 when rounding carries
+    // (k + offset) / 16 over the next bin edge; checked exhaustively over every
+    // offset the random stream yields), so the adds keep static indices and
+    // the sums stay in registers, in the same order as indexing add[b]
 #pragma unroll
     for (int k = 0; k < kNumL; ++k) {
         float l = lambdaOf(k, misc.x);
         uint32_t b = min((uint32_t)((l - kLambdaLowD) / (kLambdaHighD - kLambdaLowD) * kNumL), (uint32_t)(kNumL - 1));
-        add[b] += c[k] * recBinWidth;
+        const float v = c[k] * recBinWidth;
+        add[k] += b == (uint32_t)k ? v : 0.0f;
+        if (k + 1 < kNumL) add[k + 1] += b == (uint32_t)(k + 1) ? v : 0.0f;
     }
     double* f = job.film + ((size_t)jp.y * job.pixels + pixel) * kNumL;
 #pragma unroll
@@ -1240,7 +1257,7 @@ __device__ __forceinline__ uint8_t retireAndRestart(const DevScene& sc, const Pa
         fileSample(ps, s, info.x, misc, rad);
         ++samples;
     }
-    info.y &= ~F_HAS_SAMPLE;
+    info.y &= ~(F_HAS_SAMPLE | F_RAD);
     uint8_t code = CODE_IDLE;
     if (info.z < info.w) {
         startCameraSample(sc, ps, s, info, rng, misc);
@@ -1254,12 +1271,16 @@ __device__ __forceinline__ uint8_t retireAndRestart(const DevScene& sc, const Pa
 
 // Loads the radiance of a slot's current path (zero for a fresh path) with
 // any resolved light-sample contribution added.
-__device__ __forceinline__ void loadRadiance(const PathState& ps, uint32_t s, uint32_t pathLen, float* rad) {
-    if (pathLen == 0) {
+// The path's radiance so far plus its resolved light sample. The stored sum
+// is read only when `read` is set (the caller adds to it or files it); a
+// vertex that adds nothing leaves SF_SP untouched, and F_RAD says whether it
+// holds anything.
+__device__ __forceinline__ void loadRadiance(const PathState& ps, uint32_t s, uint32_t infoY, bool read, float* rad) {
+    if (read && (infoY & F_RAD)) {
+        loadPlanes(ps, s, SF_SP, rad);
+    } else {
 #pragma unroll
         for (int k = 0; k < kNumL; ++k) rad[k] = 0.0f;
-    } else {
-        loadPlanes(ps, s, SF_SP, rad);
     }
     resolveShadow(ps, s, rad);
 }
@@ -1278,10 +1299,10 @@ __global__ __launch_bounds__(kBlock) void k_regen(DevScene sc, PathState ps) {
         const float4 misc = ps.f4(s, SF_MISC);
         const uint4 rng = ps.u4(s, SF_RNG);
         float rad[kNumL];
-        if (info.y & F_HAS_SAMPLE) loadRadiance(ps, s, info.y >> 16, rad);
+        if (info.y & F_HAS_SAMPLE) loadRadiance(ps, s, info.y, true, rad);
         ps.code[s] = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
     }
-    waveAdd64(&ps.stat[S_SAMPLES], samples);
+    statAdd(ps.stat, S_SAMPLES, samples);
 }
 
 // One traversal launch per iteration for the whole trace queue: closest hit
@@ -1331,12 +1352,12 @@ void k_trace(DevScene sc, PathState ps) {
                 ps.code[s] = (uint8_t)(CODE_CLS0 + (h.prim >= 0 ? h.aux : (uint32_t)CLS_MISS));
             }
         }, tc);
-    waveAdd64(&ps.stat[S_RAYS_EXT], tc.rays - tc.raysAny);
-    waveAdd64(&ps.stat[S_NODES_EXT], tc.nodes - tc.nodesAny);
-    waveAdd64(&ps.stat[S_TRIS_EXT], tc.tris - tc.trisAny);
-    waveAdd64(&ps.stat[S_RAYS_SHADOW], tc.raysAny);
-    waveAdd64(&ps.stat[S_NODES_SH], tc.nodesAny);
-    waveAdd64(&ps.stat[S_TRIS_SH], tc.trisAny);
+    statAdd(ps.stat, S_RAYS_EXT, tc.rays - tc.raysAny);
+    statAdd(ps.stat, S_NODES_EXT, tc.nodes - tc.nodesAny);
+    statAdd(ps.stat, S_TRIS_EXT, tc.tris - tc.trisAny);
+    statAdd(ps.stat, S_RAYS_SHADOW, tc.raysAny);
+    statAdd(ps.stat, S_NODES_SH, tc.nodesAny);
+    statAdd(ps.stat, S_TRIS_SH, tc.trisAny);
     if (tc.overflow) atomicOr(&ps.ctr[C_OVERFLOW], 1u);
 }
 
@@ -1490,9 +1511,12 @@ __device__ __forceinline__ bool sampleBsdf(const DevMaterial& m, const float* A,
 // Shading of one queue. CLS selects the scattering model compiled in.
 // Register budget of the shading kernels: the diffuse kernel (most of the
 // work) is held to 3 waves per SIMD; the others use what they need.
+#ifndef SLRH_DIFFUSE_WAVES
+#define SLRH_DIFFUSE_WAVES 3
+#endif
 template <uint32_t CLS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CLS == CLS_DIFFUSE ? 3 : 1,
-                                                                          CLS == CLS_DIFFUSE ? 3 : 10)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CLS == CLS_DIFFUSE ? SLRH_DIFFUSE_WAVES : 1,
+                                                                          CLS == CLS_DIFFUSE ? SLRH_DIFFUSE_WAVES : 10)))
 void k_shade(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_CLS0 + CLS];
     const uint32_t* qin = queue(ps, Q_CLS0 + CLS);
@@ -1524,7 +1548,7 @@ void k_shade(DevScene sc, PathState ps) {
                 const uint4 rng = ps.u4(s, SF_RNG);
                 const uint32_t pathLen = info.y >> 16;
                 float rad[kNumL];
-                loadRadiance(ps, s, pathLen, rad);
+                loadRadiance(ps, s, info.y, true, rad);
                 if (sc.hasEnv) {
                     const float3 rd = xyz(ps.f4(s, SF_RD));
                     float theta, phi;
@@ -1569,10 +1593,13 @@ void k_shade(DevScene sc, PathState ps) {
                 } else {
                     loadPlanes(ps, s, SF_ALPHA, alpha);
                 }
-                loadRadiance(ps, s, pathLen, rad);
                 const float3 dirOut = toLocal(sp, -rd);
+                const bool emits = mat.emit >= 0 && dirOut.z > 0.0f;
+                // the stored radiance is read here only when this vertex adds to it
+                const bool adds = emits || ps.vis[s] != 0;
+                loadRadiance(ps, s, info.y, adds, rad);
                 // emission (PTRenderer.cpp:162-166 first hit, :247-263 implicit with MIS)
-                if (mat.emit >= 0 && dirOut.z > 0.0f) {
+                if (emits) {
                     float Le[kNumL];
                     evalSpectrum(sc, mat.emit, misc.x, Le);
                     float misW = 1.0f;
@@ -1775,20 +1802,25 @@ void k_shade(DevScene sc, PathState ps) {
                 if (cont || shadow) {
                     // the path goes on, or it ended with a light sample still to
                     // trace: its sample is filed once the shadow ray is resolved
-                    storePlanes(ps, s, SF_SP, rad);
+                    if (adds) {
+                        storePlanes(ps, s, SF_SP, rad);
+                        info.y |= F_RAD;
+                    }
                     if (cont) storePlanes(ps, s, SF_ALPHA, alpha);
                     ps.u4(s, SF_INFO) = info;
                     ps.f4(s, SF_MISC) = misc;
                     ps.u4(s, SF_RNG) = rng;
                     code = (uint8_t)((cont ? CODE_EXT : CODE_DONE) | (shadow ? CODE_SHADOW : 0));
                 } else {
+                    // nothing added here: the sample's radiance is the stored sum
+                    if (!adds && (info.y & F_RAD)) loadPlanes(ps, s, SF_SP, rad);
                     code = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
                 }
             }
         }
         if (valid) ps.code[s] = code;
     }
-    waveAdd64(&ps.stat[S_SAMPLES], samples);
+    statAdd(ps.stat, S_SAMPLES, samples);
 }
 
 // Slot-ordered compaction of the per-slot codes into queues (one pass over
@@ -1860,8 +1892,8 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
         if (used) ps.u4(s, SF_RNG) = rng;
         events += doExt && ps.code[s] == CODE_CLS0 + CLS_MEDIUM ? 1u : 0u;
     }
-    waveAdd64(&ps.stat[S_MED_EVENTS], events);
-    waveAdd64(&ps.stat[S_MED_SHADOWS], shadows);
+    statAdd(ps.stat, S_MED_EVENTS, events);
+    statAdd(ps.stat, S_MED_SHADOWS, shadows);
 }
 
 // Shading at a scattering point in the medium (VolumetricPTRenderer::Job::
@@ -1898,7 +1930,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
         } else {
             loadPlanes(ps, s, SF_ALPHA, alpha);
         }
-        loadRadiance(ps, s, pathLen, rad);
+        const bool adds = ps.vis[s] != 0;
+        loadRadiance(ps, s, info.y, adds, rad);
         bool alive = true;
         if (pathLen > 0) {
             const float p = fminf(importance16(alpha, hero) / misc.y, 1.0f);
@@ -2026,18 +2059,22 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
         }
         info.y = (info.y & 0xFFFFu) | (pathLen << 16);
         if (cont || shadow) {
-            storePlanes(ps, s, SF_SP, rad);
+            if (adds) {
+                storePlanes(ps, s, SF_SP, rad);
+                info.y |= F_RAD;
+            }
             if (cont) storePlanes(ps, s, SF_ALPHA, alpha);
             ps.u4(s, SF_INFO) = info;
             ps.f4(s, SF_MISC) = misc;
             ps.u4(s, SF_RNG) = rng;
             code = (uint8_t)((cont ? CODE_EXT : CODE_DONE) | (shadow ? CODE_SHADOW : 0));
         } else {
+            if (!adds && (info.y & F_RAD)) loadPlanes(ps, s, SF_SP, rad);
             code = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
         }
         ps.code[s] = code;
     }
-    waveAdd64(&ps.stat[S_SAMPLES], samples);
+    statAdd(ps.stat, S_SAMPLES, samples);
 }
 
 // Ray batches through the medium: mode 0 one ratio-tracking transmittance
@@ -2099,20 +2136,22 @@ int mediumQuery(DeviceScene* ds, const TraceArgs& a, uint32_t n, int mode, uint3
     return 0;
 }
 
+// 2048-slot groups per compaction block: each block takes one counter
+// atomic per queue, and same-address atomics serialise, so fewer and larger
+// blocks shorten the pass.
+constexpr uint32_t kCompactGroups = 4;
+
 template <int MODE>
-__global__ __launch_bounds__(256) void k_compact(PathState ps) {
+__global__ __launch_bounds__(256) void k_compact(PathState ps, uint32_t groups) {
     constexpr int PER = 8;
+    constexpr int G = (int)kCompactGroups;
     constexpr int NQ = MODE == 0 ? 3 : (MODE == 1 ? 1 : kNumClasses);
-    __shared__ uint32_t waveTot[NQ][4];
+    __shared__ uint32_t waveTot[G][NQ][4];
+    __shared__ uint32_t groupBase[G][NQ];
     __shared__ uint32_t blockBase[NQ];
-    const uint32_t first = (blockIdx.x * 256u + threadIdx.x) * PER;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     // the traversal that consumes this compaction's queue starts its cursor at 0
     if (MODE == 1 && blockIdx.x == 0 && threadIdx.x == 0) ps.ctr[C_FETCH] = 0;
-    uint8_t c[PER];
-    uint32_t cnt[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) cnt[q] = 0;
     // entries slot s contributes to queue q
     auto entries = [](int q, uint8_t v) -> uint32_t {
         const bool ext = (v & 0x7F) == CODE_EXT, sh = (v & CODE_SHADOW) != 0;
@@ -2120,39 +2159,63 @@ __global__ __launch_bounds__(256) void k_compact(PathState ps) {
         if (MODE == 1) return (ext ? 1u : 0u) + (sh ? 1u : 0u);
         return v == CODE_CLS0 + q ? 1u : 0u;
     };
-    if (first + PER <= ps.n) {
-        // one 8-byte load per thread
-        const uint2 w = *(const uint2*)(ps.code + first);
+    auto codeOf = [](const uint2& w, int k) -> uint8_t {
+        return (uint8_t)(((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xFFu);
+    };
+    const uint32_t g0 = blockIdx.x * (uint32_t)G;
+    uint2 w[G];
+    uint32_t cnt[G][NQ], incl[G][NQ];
 #pragma unroll
-        for (int k = 0; k < PER; ++k) c[k] = (uint8_t)(((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xFFu);
-    } else {
+    for (int g = 0; g < G; ++g) {
+        // the pool is a whole number of groups: one 8-byte load per thread
+        const uint32_t first = ((g0 + g) * 256u + threadIdx.x) * PER;
+        w[g] = g0 + g < groups ? *(const uint2*)(ps.code + first) : make_uint2(0u, 0u);   // 0: CODE_IDLE
 #pragma unroll
-        for (int k = 0; k < PER; ++k) c[k] = first + k < ps.n ? ps.code[first + k] : (uint8_t)CODE_IDLE;
-    }
+        for (int q = 0; q < NQ; ++q) cnt[g][q] = 0;
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
+        for (int k = 0; k < PER; ++k) {
+            const uint8_t c = codeOf(w[g], k);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) cnt[q] += entries(q, c[k]);
-    }
-    uint32_t incl[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        uint32_t x = cnt[q];
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t y = __shfl_up(x, off);
-            if ((int)lane >= off) x += y;
+            for (int q = 0; q < NQ; ++q) cnt[g][q] += entries(q, c);
         }
-        incl[q] = x;
-        if (lane == 63) waveTot[q][wave] = x;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            uint32_t x = cnt[g][q];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                uint32_t y = __shfl_up(x, off);
+                if ((int)lane >= off) x += y;
+            }
+            incl[g][q] = x;
+            if (lane == 63) waveTot[g][q][wave] = x;
+        }
     }
     __syncthreads();
     if (threadIdx.x < NQ) {
         const int q = threadIdx.x;
-        const uint32_t tot = waveTot[q][0] + waveTot[q][1] + waveTot[q][2] + waveTot[q][3];
+        uint32_t tot = 0;
+        uint32_t gt[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            gt[g] = waveTot[g][q][0] + waveTot[g][q][1] + waveTot[g][q][2] + waveTot[g][q][3];
+            groupBase[g][q] = tot;
+            tot += gt[g];
+        }
         if (MODE == 0 && q == 2) {
-            const uint32_t job = ps.blockJob[blockIdx.x];
-            if (tot && job != kNoJob) atomicAdd(&ps.jobActive[job], tot);
+            // live slots per job: one atomic per run of groups of the same job
+            uint32_t runJob = kNoJob, runTot = 0;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (g0 + g >= groups) break;
+                const uint32_t job = ps.blockJob[g0 + g];
+                if (job != runJob) {
+                    if (runTot && runJob != kNoJob) atomicAdd(&ps.jobActive[runJob], runTot);
+                    runJob = job;
+                    runTot = 0;
+                }
+                runTot += gt[g];
+            }
+            if (runTot && runJob != kNoJob) atomicAdd(&ps.jobActive[runJob], runTot);
             blockBase[q] = 0;
         } else {
             const uint32_t ctr = MODE == 0 ? (q == 0 ? C_DONE : C_PENDING) : MODE == 1 ? C_TRACE : C_CLS0 + q;
@@ -2161,28 +2224,38 @@ __global__ __launch_bounds__(256) void k_compact(PathState ps) {
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        if (MODE == 0 && q >= 1) continue;   // counted, not queued
-        uint32_t off = blockBase[q] + incl[q] - cnt[q];
-        for (uint32_t w = 0; w < wave; ++w) off += waveTot[q][w];
-        const uint32_t qid = MODE == 0 ? Q_DONE : MODE == 1 ? Q_TRACE : Q_CLS0 + q;
-        uint32_t* out = queue(ps, qid);
+    for (int g = 0; g < G; ++g) {
+        const uint32_t first = ((g0 + g) * 256u + threadIdx.x) * PER;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const uint32_t s = first + k;
-            if (MODE == 1) {
-                if ((c[k] & 0x7F) == CODE_EXT) out[off++] = s;
-                if (c[k] & CODE_SHADOW) out[off++] = s | kShadowEntry;
-            } else if (entries(q, c[k])) {
-                out[off++] = s;
+        for (int q = 0; q < NQ; ++q) {
+            if (MODE == 0 && q >= 1) continue;   // counted, not queued
+            uint32_t off = blockBase[q] + groupBase[g][q] + incl[g][q] - cnt[g][q];
+            for (uint32_t v = 0; v < wave; ++v) off += waveTot[g][q][v];
+            const uint32_t qid = MODE == 0 ? Q_DONE : MODE == 1 ? Q_TRACE : Q_CLS0 + q;
+            uint32_t* out = queue(ps, qid);
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const uint8_t c = codeOf(w[g], k);
+                const uint32_t s = first + k;
+                if (MODE == 1) {
+                    if ((c & 0x7F) == CODE_EXT) out[off++] = s;
+                    if (c & CODE_SHADOW) out[off++] = s | kShadowEntry;
+                } else if (entries(q, c)) {
+                    out[off++] = s;
+                }
             }
         }
+        if (MODE == 1 && g0 + g < groups) {
+            // shadow bits cleared in the codes: one 8-byte store when any was set
+            const uint2 nw = make_uint2(w[g].x & 0x7F7F7F7Fu, w[g].y & 0x7F7F7F7Fu);
+            if (nw.x != w[g].x || nw.y != w[g].y) *(uint2*)(ps.code + first) = nw;
+        }
     }
-    if (MODE == 1) {
-#pragma unroll
-        for (int k = 0; k < PER; ++k)
-            if (c[k] & CODE_SHADOW) ps.code[first + k] = c[k] & 0x7F;
-    }
+}
+
+// Blocks of a compaction pass over a pool of n slots (a multiple of kCompactSlots).
+static inline uint32_t compactBlocks(uint32_t n) {
+    return (n / kCompactSlots + kCompactGroups - 1) / kCompactGroups;
 }
 
 // Sums film planes into the output film in plane order.
@@ -2246,7 +2319,7 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
     int r;
     if ((r = p->rec.alloc((size_t)kSlotVec * capacity, err)) || (r = p->q.alloc((size_t)kNumQueues * capacity, err)) ||
         (r = p->ctr.alloc(kNumCtr, err)) || (r = p->code.alloc(capacity, err)) || (r = p->vis.alloc(capacity, err)) ||
-        (r = p->stat.alloc(kNumStat, err)) || (r = p->jobActive.alloc(kMaxJobs, err)) ||
+        (r = p->stat.alloc(kNumStat * kStatSpread, err)) || (r = p->jobActive.alloc(kMaxJobs, err)) ||
         (r = p->blockJob.alloc(capacity / kCompactSlots, err)) || (r = p->jobs.alloc(kMaxJobs, err)) ||
         (r = p->spill.alloc((size_t)(kMaxStack - kMinLdsDepth) * blocks * kBlock, err)))
         return r;
@@ -2361,23 +2434,24 @@ typedef std::array<hipEvent_t, 5> IterEvents;
 
 // One wavefront iteration: retire/restart, trace, classify, shade, compact.
 static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
-    const uint32_t cblocks = p.n / kCompactSlots;
+    const uint32_t groups = p.n / kCompactSlots;
+    const uint32_t cblocks = (groups + kCompactGroups - 1) / kCompactGroups;
     hipStream_t s = p.stream;
     if (ev) (void)hipEventRecord((*ev)[0], s);
     hipLaunchKernelGGL(k_regen, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
-    hipLaunchKernelGGL(k_compact<1>, dim3(cblocks), dim3(256), 0, s, p.ps);
+    hipLaunchKernelGGL(k_compact<1>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
     if (ev) (void)hipEventRecord((*ev)[1], s);
     kTraceLaunch[tv](p.blocks, s, p.sc, p.ps);
     if (ev) (void)hipEventRecord((*ev)[2], s);
     if (p.sc.volume) hipLaunchKernelGGL(k_medium, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
-    hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, s, p.ps);
+    hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
     hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     if (p.sc.volume) hipLaunchKernelGGL(k_shade_medium, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
     if (ev) (void)hipEventRecord((*ev)[3], s);
-    hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, s, p.ps);
+    hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
     if (ev) (void)hipEventRecord((*ev)[4], s);
 }
 
@@ -2452,7 +2526,7 @@ int poolCancel(SlotPool* p, uint32_t idx, std::string* err) {
 
 int poolRecount(SlotPool* p, std::string* err) {
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(256), 0, p->stream, p->ps);
-    hipLaunchKernelGGL(k_compact<0>, dim3(p->n / kCompactSlots), dim3(256), 0, p->stream, p->ps);
+    hipLaunchKernelGGL(k_compact<0>, dim3(compactBlocks(p->n)), dim3(256), 0, p->stream, p->ps, p->n / kCompactSlots);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -2476,9 +2550,20 @@ int poolFinish(SlotPool* p, uint32_t idx, std::string* err) {
     return 0;
 }
 
+// The render statistics summed over their spread copies (synchronous).
+static int readStats(const SlotPool* p, unsigned long long* stat, std::string* err) {
+    std::vector<unsigned long long> all((size_t)kNumStat * kStatSpread);
+    HIP_TRY(hipMemcpy(all.data(), p->stat.p, all.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < kNumStat; ++i) {
+        stat[i] = 0;
+        for (uint32_t j = 0; j < kStatSpread; ++j) stat[i] += all[(size_t)i * kStatSpread + j];
+    }
+    return 0;
+}
+
 int poolSamples(SlotPool* p, double* samples, double* rays, std::string* err) {
     unsigned long long stat[kNumStat];
-    HIP_TRY(hipMemcpy(stat, p->stat.p, sizeof(stat), hipMemcpyDeviceToHost));
+    if (int r = readStats(p, stat, err)) return r;
     *samples = (double)stat[S_SAMPLES];
     *rays = (double)(stat[S_RAYS_EXT] + stat[S_RAYS_SHADOW]);
     return 0;
@@ -2517,7 +2602,7 @@ static int tuneTrace(DeviceScene* ds, SlotPool& p, const JobSpec& first, uint32_
     for (int v = 0; v < kNumVariants; ++v) {
         if ((r = poolReset(p, err))) { cleanup(); return r; }
         if ((r = poolAdmit(&p, 0, pilot, planes, base, err))) { cleanup(); return r; }
-        hipLaunchKernelGGL(k_compact<0>, dim3(p.n / kCompactSlots), dim3(256), 0, p.stream, p.ps);
+        hipLaunchKernelGGL(k_compact<0>, dim3(compactBlocks(p.n)), dim3(256), 0, p.stream, p.ps, p.n / kCompactSlots);
         std::vector<IterEvents> itEv;
         for (;;) {
             for (uint32_t k = 0; k < kGraphIters; ++k) {
@@ -2534,7 +2619,7 @@ static int tuneTrace(DeviceScene* ds, SlotPool& p, const JobSpec& first, uint32_
         if ((r = finishJob(p, p.runs[0], err))) { cleanup(); return r; }
         HIP_TRY(hipStreamSynchronize(p.stream));
         unsigned long long stat[kNumStat];
-        HIP_TRY(hipMemcpy(stat, p.stat.p, sizeof(stat), hipMemcpyDeviceToHost));
+        if ((r = readStats(&p, stat, err))) { cleanup(); return r; }
         double ms = 0;
         for (auto& e : itEv) {
             float t = 0;
@@ -2609,7 +2694,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
     for (uint32_t i = 0; i < n; ++i)
         if ((r = poolAdmit(&p, i, specs[i], planes[i], bases[i], err))) return r;
     HIP_TRY(hipEventRecord(tStart, p.stream));
-    hipLaunchKernelGGL(k_compact<0>, dim3(p.n / kCompactSlots), dim3(256), 0, p.stream, p.ps);
+    hipLaunchKernelGGL(k_compact<0>, dim3(compactBlocks(p.n)), dim3(256), 0, p.stream, p.ps, p.n / kCompactSlots);
     HIP_TRY(hipGetLastError());
     uint32_t it = 0;
     for (;;) {
@@ -2637,7 +2722,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
     HIP_TRY(hipEventRecord(tEnd, p.stream));
     HIP_TRY(hipStreamSynchronize(p.stream));
     unsigned long long stat[kNumStat];
-    HIP_TRY(hipMemcpy(stat, p.stat.p, sizeof(stat), hipMemcpyDeviceToHost));
+    if (int rs = readStats(&p, stat, err)) return rs;
     HIP_TRY(hipMemcpy(p.hostCtr, p.ctr.p, kNumCtr * sizeof(uint32_t), hipMemcpyDeviceToHost));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, tStart, tEnd);

@@ -18,6 +18,11 @@
 // interleaved by lane. Total depth is the reference's 64.
 #pragma once
 
+// Queue entries a wave claims per atomic on the traversal's fetch cursor.
+#ifndef SLRH_FETCH_CHUNK
+#define SLRH_FETCH_CHUNK 256
+#endif
+
 #include "device_common.h"
 
 namespace slrh {
@@ -264,7 +269,7 @@ template <int kRefill, bool kFuse, bool kHasInv, bool kInst = false, typename En
 __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& st, uint32_t count,
                                           uint32_t* fetchCtr, Entry&& entry, Load&& load, Finish&& finish,
                                           TraceCounters& tc) {
-    constexpr uint32_t kChunk = 64;
+    constexpr uint32_t kChunk = SLRH_FETCH_CHUNK;
     const char* nodes = (const char*)sc.nodes;
     const char* tris = (const char*)sc.tris;
     const uint32_t lane = __lane_id();
