@@ -279,7 +279,8 @@ typedef struct slrh_render_settings {
     uint32_t tune_trace;      /* traversal kernel variant + 1; 0 = automatic: the first render
                                  of a scene with a pool of >= 2^20 slots renders one pilot
                                  sample per slot with each variant (scratch film) and keeps
-                                 the fastest per traced ray for that scene and pool size */
+                                 the fastest per traced ray for every later render of the
+                                 scene (variant 0 until then) */
     uint32_t reserved;        /* must be 0 */
 } slrh_render_settings;
 

@@ -126,7 +126,7 @@ struct DeviceScene {
     CameraParams camera;
     size_t bytes = 0;
     std::shared_ptr<struct SlotPool> pool;   // wavefront state kept between renders
-    // traversal variant chosen by the autotuner for pools of tunedSlots slots
+    // traversal variant chosen by the autotuner (on a pool of tunedSlots slots)
     int tunedVariant = -1;
     uint32_t tunedSlots = 0;
 };
@@ -639,7 +639,6 @@ enum Ctr : uint32_t {
     C_PENDING = 8,        // slots with an extension or shadow ray pending after shading
     C_NONFINITE = 9,
     C_OVERFLOW = 10,
-    C_FETCH = 11,         // entry cursor of k_trace's persistent loop
     kNumCtr = 16
 };
 // 64-bit statistics
@@ -716,6 +715,7 @@ struct PathState {
     uint8_t* vis;        // 1: the slot's shadow ray was traced and is unoccluded
     uint32_t* q;         // queues: done (n entries), trace (2n), cls0..3 (n each)
     uint32_t* ctr;
+    uint32_t* fetch;     // k_trace's entry cursors, kFetchStride apart (one per queue part)
     unsigned long long* stat;
     uint32_t* spill;
     uint32_t spillStride;
@@ -1331,7 +1331,7 @@ void k_trace(DevScene sc, PathState ps) {
     st.treeletNodes = L < sc.numNodes ? (uint32_t)L : sc.numNodes;
     st.treelet = loadTreelet(treelet, sc.nodes, st.treeletNodes);
     TraceCounters tc;
-    traceLoop<T, F, true, I>(sc, st, count, &ps.ctr[C_FETCH], [&](uint32_t i) { return qin[i]; },
+    traceLoop<T, F, true, I, SLRH_FETCH_PARTS>(sc, st, count, ps.fetch, [&](uint32_t i) { return qin[i]; },
         [&](uint32_t e, RayIn& r) {
             const uint32_t s = e & ~kShadowEntry;
             const bool shadow = (e & kShadowEntry) != 0;
@@ -2151,7 +2151,7 @@ __global__ __launch_bounds__(256) void k_compact(PathState ps, uint32_t groups) 
     __shared__ uint32_t blockBase[NQ];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     // the traversal that consumes this compaction's queue starts its cursor at 0
-    if (MODE == 1 && blockIdx.x == 0 && threadIdx.x == 0) ps.ctr[C_FETCH] = 0;
+    if (MODE == 1 && blockIdx.x == 0 && threadIdx.x < SLRH_FETCH_PARTS) ps.fetch[threadIdx.x * kFetchStride] = 0;
     // entries slot s contributes to queue q
     auto entries = [](int q, uint8_t v) -> uint32_t {
         const bool ext = (v & 0x7F) == CODE_EXT, sh = (v & CODE_SHADOW) != 0;
@@ -2287,7 +2287,7 @@ struct SlotPool {
     uint32_t n = 0, blocks = 0;
     int device = 0;
     DevBuf<float4> rec;
-    DevBuf<uint32_t> q, ctr, spill, jobActive, blockJob;
+    DevBuf<uint32_t> q, ctr, fetch, spill, jobActive, blockJob;
     DevBuf<uint8_t> code, vis;
     DevBuf<unsigned long long> stat;
     DevBuf<DevJob> jobs;
@@ -2318,7 +2318,8 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
     const uint32_t blocks = (uint32_t)g_numCUs * 8;
     int r;
     if ((r = p->rec.alloc((size_t)kSlotVec * capacity, err)) || (r = p->q.alloc((size_t)kNumQueues * capacity, err)) ||
-        (r = p->ctr.alloc(kNumCtr, err)) || (r = p->code.alloc(capacity, err)) || (r = p->vis.alloc(capacity, err)) ||
+        (r = p->ctr.alloc(kNumCtr, err)) || (r = p->fetch.alloc(SLRH_FETCH_PARTS * kFetchStride, err)) ||
+        (r = p->code.alloc(capacity, err)) || (r = p->vis.alloc(capacity, err)) ||
         (r = p->stat.alloc(kNumStat * kStatSpread, err)) || (r = p->jobActive.alloc(kMaxJobs, err)) ||
         (r = p->blockJob.alloc(capacity / kCompactSlots, err)) || (r = p->jobs.alloc(kMaxJobs, err)) ||
         (r = p->spill.alloc((size_t)(kMaxStack - kMinLdsDepth) * blocks * kBlock, err)))
@@ -2342,6 +2343,7 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
     ps.vis = p->vis.p;
     ps.q = p->q.p;
     ps.ctr = p->ctr.p;
+    ps.fetch = p->fetch.p;
     ps.stat = p->stat.p;
     ps.spill = p->spill.p;
     ps.spillStride = blocks * kBlock;
@@ -2680,10 +2682,13 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
     specs = jobSpecs.data();
     int tv = opt.traceVariant >= 0 && opt.traceVariant < kNumVariants ? opt.traceVariant : 0;
     if (opt.traceVariant < 0) {
-        // automatic: tuned once per scene and pool size
-        if (p.n >= kTuneMinSlots && (ds->tunedVariant < 0 || ds->tunedSlots != p.n))
+        // automatic: tuned once per scene, by the first render whose pool is
+        // large enough for the pilot to time the traversal at full occupancy;
+        // later renders of any size reuse the choice (a server's requests
+        // vary in size, and a pilot per pool size would cost more than it saves)
+        if (p.n >= kTuneMinSlots && ds->tunedVariant < 0)
             if ((r = tuneTrace(ds, p, specs[0], planes[0], bases[0], err))) return r;
-        tv = p.n >= kTuneMinSlots && ds->tunedSlots == p.n ? ds->tunedVariant : 0;
+        tv = ds->tunedVariant >= 0 ? ds->tunedVariant : 0;
     }
     if ((r = poolReset(p, err))) return r;
 

@@ -22,6 +22,13 @@
 #ifndef SLRH_FETCH_CHUNK
 #define SLRH_FETCH_CHUNK 256
 #endif
+// The queue can be split into parts with a cursor each (kFetchStride uint32
+// apart, so on separate cache lines): block b starts on part b % parts, the
+// blocks of one XCD on the same part, and moves on when its part runs dry.
+#ifndef SLRH_FETCH_PARTS
+#define SLRH_FETCH_PARTS 1
+#endif
+constexpr uint32_t kFetchStride = 32;
 
 #include "device_common.h"
 
@@ -264,12 +271,14 @@ __device__ __forceinline__ uint32_t kthSetBit(unsigned long long m, uint32_t k) 
 //   load(e, RayIn& r)   -> bool, requests the ray of entry e (r.inv too when
 //                          kHasInv, else it is computed here); true for any hit
 //   finish(e, anyHit, const HitOut&)
-template <int kRefill, bool kFuse, bool kHasInv, bool kInst = false, typename Entry, typename Load,
-          typename Finish>
+template <int kRefill, bool kFuse, bool kHasInv, bool kInst = false, uint32_t kParts = 1, typename Entry,
+          typename Load, typename Finish>
 __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& st, uint32_t count,
                                           uint32_t* fetchCtr, Entry&& entry, Load&& load, Finish&& finish,
                                           TraceCounters& tc) {
     constexpr uint32_t kChunk = SLRH_FETCH_CHUNK;
+    uint32_t part = kParts > 1 ? blockIdx.x % kParts : 0;   // wave-uniform
+    uint32_t partsLeft = kParts;
     const char* nodes = (const char*)sc.nodes;
     const char* tris = (const char*)sc.tris;
     const uint32_t lane = __lane_id();
@@ -359,12 +368,33 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
         const unsigned long long lack = __ballot(!hasNext);
         if (!exhausted && __popcll(lack) >= kRefill) {
             if (next >= end) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(fetchCtr, kChunk);
-                base = __shfl(base, 0);
-                next = base;
-                end = base + kChunk < count ? base + kChunk : count;
-                if (base >= count) { exhausted = true; next = end = count; }
+                if (kParts == 1) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(fetchCtr, kChunk);
+                    base = __shfl(base, 0);
+                    next = base;
+                    end = base + kChunk < count ? base + kChunk : count;
+                    if (base >= count) { exhausted = true; next = end = count; }
+                } else {
+                    for (;;) {
+                        const uint32_t pb = (uint32_t)((uint64_t)count * part / kParts);
+                        const uint32_t pe = (uint32_t)((uint64_t)count * (part + 1) / kParts);
+                        uint32_t base = 0;
+                        if (lane == 0) base = atomicAdd(fetchCtr + part * kFetchStride, kChunk);
+                        base = __shfl(base, 0);
+                        if (base < pe - pb) {
+                            next = pb + base;
+                            end = pe - next > kChunk ? next + kChunk : pe;
+                            break;
+                        }
+                        if (--partsLeft == 0) {
+                            exhausted = true;
+                            next = end = count;
+                            break;
+                        }
+                        part = part + 1 < kParts ? part + 1 : 0;
+                    }
+                }
             }
             const uint32_t avail = end - next;
             // rank of this lane among the lanes lacking a next entry
