@@ -114,7 +114,7 @@ struct DevScene {
     DevCamera cam;
     // environment light (image-based); hasEnv = 0 without one
     const float4* envTexel;   // per texel: Meng basis indices (bits), s, t, scale
-    const float* mengBasis;   // [kMengPoints][95]
+    const float2* mengBasis;  // [kMengPoints][95]: each bin with the next (the last with itself)
     const float *envCondPdf, *envCondCdf, *envMargPdf, *envMargCdf;
     uint32_t envW, envH, envMapW, envMapH;
     float envScale;

@@ -118,7 +118,8 @@ struct DeviceScene {
     DevBuf<float4> instXf;
     DevBuf<uint32_t> spill;      // traversal stack spill for ray batches
     DevBuf<float4> envTexel;
-    DevBuf<float> mengBasis, envCondPdf, envCondCdf, envMargPdf, envMargCdf;
+    DevBuf<float2> mengBasis;
+    DevBuf<float> envCondPdf, envCondCdf, envMargPdf, envMargCdf;
     DevBuf<float> medDensity, medMajorant;
     DevBuf<unsigned long long> counters;
     DevScene view;
@@ -394,7 +395,12 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         }
         uint32_t np = 0, nl = 0;
         const float* basis = mengBasisTable(&np, &nl);
-        std::vector<float> b(basis, basis + (size_t)np * nl);
+        // each bin with its upper neighbour (the last bin with itself): the
+        // linear interpolation between two bins is then one 8-byte load
+        std::vector<float2> b((size_t)np * nl);
+        for (uint32_t q = 0; q < np; ++q)
+            for (uint32_t i = 0; i < nl; ++i)
+                b[(size_t)q * nl + i] = make_float2(basis[(size_t)q * nl + i], basis[(size_t)q * nl + std::min(i + 1, nl - 1)]);
         if ((r = ds->envTexel.upload(tex, err)) || (r = ds->mengBasis.upload(b, err)) ||
             (r = ds->envCondPdf.upload(e.condPdf, err)) || (r = ds->envCondCdf.upload(e.condCdf, err)) ||
             (r = ds->envMargPdf.upload(e.margPdf, err)) || (r = ds->envMargCdf.upload(e.margCdf, err)))
@@ -688,7 +694,7 @@ enum SlotField : uint32_t {
     SF_INFO = 20,   // pixel, flags | hero << 8 | pathLength << 16, samples started, samples total
     SF_MISC = 21,   // wl offset, initY, last BSDF pdf, camera weight
     SF_RNG = 22,    // XORShift state
-    SF_JOB = 23,    // job index, film plane of the slot
+    SF_JOB = 23,    // job index, film plane of the slot, seed of its light-sample transmittance streams
     kSlotVec = 24
 };
 
@@ -743,6 +749,25 @@ __device__ __forceinline__ float xsFloat(uint4& s) {
 }
 // XORShiftRNG(seed): the seed is signed there (Types32bit::Int), so the
 // shift is arithmetic (XORShiftRNG.cpp:21-27, random_number_generator.h:15-19).
+// MurmurHash3's 32-bit finaliser.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
+    return h;
+}
+
+// A stream of its own for the transmittance estimate of one light sample:
+// hashed from the slot's seed, its sample count and the path length, none of
+// which change while the sample's rays are in flight, so the estimate can
+// run beside the extension ray's free-path sampling, which owns the slot's
+// stream.
+__device__ __forceinline__ uint4 shadowStream(uint32_t slotSeed, uint4 info) {
+    const uint32_t a = fmix32(slotSeed ^ fmix32(info.z * 0x9E3779B9u ^ (info.y >> 16)));
+    uint4 st = make_uint4(fmix32(a ^ 0x1B873593u), fmix32(a ^ 0xCC9E2D51u), fmix32(a ^ 0xE6546B64u),
+                          fmix32(a ^ 0x85EBCA6Bu));
+    if ((st.x | st.y | st.z | st.w) == 0u) st.x = 1u;
+    return st;
+}
+
 __device__ __forceinline__ uint4 xsSeed(uint32_t seed) {
     uint32_t st[4];
     int32_t si = (int32_t)seed;
@@ -843,12 +868,12 @@ __device__ __forceinline__ void envTexelSpectrum(const DevScene& sc, uint32_t te
         float p = (lambdaOf(i, off) - kLambdaLowD) / (kLambdaHighD - kLambdaLowD);
         p = fminf(fmaxf(p, 0.0f), 1.0f);
         const float binF = p * 94;
-        const uint32_t bin = min((uint32_t)binF, 94u), nxt = min(bin + 1, 94u);
+        const uint32_t bin = min((uint32_t)binF, 94u);
         const float t = binF - bin;
         float r = 0.0f;
         for (int j = 0; j < n; ++j) {
-            const float* sp = sc.mengBasis + idx[j] * 95;
-            r += w[j] * (sp[bin] * (1 - t) + sp[nxt] * t);
+            const float2 pr = sc.mengBasis[idx[j] * 95 + bin];   // bins bin and min(bin + 1, 94)
+            r += w[j] * (pr.x * (1 - t) + pr.y * t);
         }
         out[i] = r * tx.w;
     }
@@ -1110,11 +1135,11 @@ __global__ void k_init_slots(PathState ps, uint32_t job, uint32_t base, uint32_t
         const uint32_t plane = local / pixels;
         const uint32_t total = spp > plane ? (spp - plane + planes - 1) / planes : 0;
         ps.u4(s, SF_INFO) = make_uint4(pixel, 0u, 0u, total);
-        ps.u4(s, SF_JOB) = make_uint4(job, plane, 0u, 0u);
         // one XORShift stream per slot, seeded like XORShiftRNG(seed')
         uint32_t h = seed ^ (0x9E3779B9u * (local + 1u));
         h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
         ps.u4(s, SF_RNG) = xsSeed(h);
+        ps.u4(s, SF_JOB) = make_uint4(job, plane, fmix32(h ^ 0x68E31DA4u), 0u);
         ps.code[s] = CODE_DONE;
         ps.vis[s] = 0;
     }
@@ -1837,9 +1862,10 @@ void k_shade(DevScene sc, PathState ps) {
 // paths through the medium (Scene::interact, libSLR/Scene/Scene.cpp:66-94:
 // the medium over [tmin, surface hit]) by delta tracking, and the light
 // samples' transmittance (Scene::testVisibility, :114-135) by ratio tracking,
-// which scales the stored contribution before it is resolved. A slot's
-// extension and shadow entries are adjacent in the trace queue; the thread
-// of the extension entry handles both, so one thread owns the slot's RNG.
+// which scales the stored contribution before it is resolved. Each trace
+// queue entry has its own lane: the extension ray advances the slot's
+// stream, the light sample draws from a stream hashed for it (shadowStream),
+// so the two long tracking loops of a slot run side by side.
 __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_TRACE];
     const uint32_t* qin = queue(ps, Q_TRACE);
@@ -1847,50 +1873,38 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
         const uint32_t e = qin[i];
         const uint32_t s = e & ~kShadowEntry;
-        bool doExt = false, doShadow = false;
         if (e & kShadowEntry) {
-            doShadow = !(i > 0 && qin[i - 1] == s);
-        } else {
-            doExt = true;
-            doShadow = i + 1 < count && qin[i + 1] == (s | kShadowEntry);
-        }
-        // an occluded light sample needs no transmittance
-        doShadow = doShadow && ps.vis[s] != 0;
-        if (!doExt && !doShadow) continue;
-        uint4 rng = ps.u4(s, SF_RNG);
-        bool used = false;
-        if (doExt) {
-            const float4 o4 = ps.f4(s, SF_RO), d4 = ps.f4(s, SF_RD), h = ps.f4(s, SF_HIT);
-            const float3 o = xyz(o4), d = xyz(d4);
-            float t0 = o4.w, t1 = h.x;
-            if (clipToMedium(sc, o, d, &t0, &t1)) {
-                used = true;
-                float ts;
-                if (sampleFreePath(sc, o, d, o4.w, h.x, rng, &ts)) {
-                    ps.f4(s, SF_HIT) = make_float4(ts, __int_as_float(-2), 0.0f, 0.0f);
-                    ps.code[s] = (uint8_t)(CODE_CLS0 + CLS_MEDIUM);
-                }
-            }
-        }
-        if (doShadow) {
+            // the light sample (an occluded one needs no transmittance), on a
+            // stream of its own so that it runs beside the slot's extension ray
+            if (ps.vis[s] == 0) continue;
             const float4 o4 = ps.f4(s, SF_SO), d4 = ps.f4(s, SF_SD);
             const float3 o = xyz(o4), d = xyz(d4);
             float t0 = o4.w, t1 = d4.w;
-            if (clipToMedium(sc, o, d, &t0, &t1)) {
-                used = true;
-                ++shadows;
-                const float T = mediumTransmittance(sc, o, d, o4.w, d4.w, rng);
-                if (T != 1.0f) {
-                    float c[kNumL];
-                    loadPlanes(ps, s, SF_NEE, c);
+            if (!clipToMedium(sc, o, d, &t0, &t1)) continue;
+            ++shadows;
+            uint4 rng = shadowStream(ps.u4(s, SF_JOB).z, ps.u4(s, SF_INFO));
+            const float T = mediumTransmittance(sc, o, d, o4.w, d4.w, rng);
+            if (T != 1.0f) {
+                float c[kNumL];
+                loadPlanes(ps, s, SF_NEE, c);
 #pragma unroll
-                    for (int k = 0; k < kNumL; ++k) c[k] *= T;
-                    storePlanes(ps, s, SF_NEE, c);
-                }
+                for (int k = 0; k < kNumL; ++k) c[k] *= T;
+                storePlanes(ps, s, SF_NEE, c);
             }
+        } else {
+            const float4 o4 = ps.f4(s, SF_RO), d4 = ps.f4(s, SF_RD), h = ps.f4(s, SF_HIT);
+            const float3 o = xyz(o4), d = xyz(d4);
+            float t0 = o4.w, t1 = h.x;
+            if (!clipToMedium(sc, o, d, &t0, &t1)) continue;
+            uint4 rng = ps.u4(s, SF_RNG);
+            float ts;
+            if (sampleFreePath(sc, o, d, o4.w, h.x, rng, &ts)) {
+                ps.f4(s, SF_HIT) = make_float4(ts, __int_as_float(-2), 0.0f, 0.0f);
+                ps.code[s] = (uint8_t)(CODE_CLS0 + CLS_MEDIUM);
+                ++events;
+            }
+            ps.u4(s, SF_RNG) = rng;
         }
-        if (used) ps.u4(s, SF_RNG) = rng;
-        events += doExt && ps.code[s] == CODE_CLS0 + CLS_MEDIUM ? 1u : 0u;
     }
     statAdd(ps.stat, S_MED_EVENTS, events);
     statAdd(ps.stat, S_MED_SHADOWS, shadows);

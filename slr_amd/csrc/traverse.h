@@ -20,13 +20,13 @@
 
 // Queue entries a wave claims per atomic on the traversal's fetch cursor.
 #ifndef SLRH_FETCH_CHUNK
-#define SLRH_FETCH_CHUNK 256
+#define SLRH_FETCH_CHUNK 128
 #endif
 // The queue can be split into parts with a cursor each (kFetchStride uint32
 // apart, so on separate cache lines): block b starts on part b % parts, the
 // blocks of one XCD on the same part, and moves on when its part runs dry.
 #ifndef SLRH_FETCH_PARTS
-#define SLRH_FETCH_PARTS 1
+#define SLRH_FETCH_PARTS 8
 #endif
 constexpr uint32_t kFetchStride = 32;
 
