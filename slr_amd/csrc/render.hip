@@ -1539,9 +1539,12 @@ __device__ __forceinline__ bool sampleBsdf(const DevMaterial& m, const float* A,
 #ifndef SLRH_DIFFUSE_WAVES
 #define SLRH_DIFFUSE_WAVES 3
 #endif
+#ifndef SLRH_SHADE_MIN_WAVES
+#define SLRH_SHADE_MIN_WAVES 1
+#endif
 template <uint32_t CLS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CLS == CLS_DIFFUSE ? SLRH_DIFFUSE_WAVES : 1,
-                                                                          CLS == CLS_DIFFUSE ? SLRH_DIFFUSE_WAVES : 10)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
+    CLS == CLS_DIFFUSE ? SLRH_DIFFUSE_WAVES : SLRH_SHADE_MIN_WAVES, CLS == CLS_DIFFUSE ? SLRH_DIFFUSE_WAVES : 10)))
 void k_shade(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_CLS0 + CLS];
     const uint32_t* qin = queue(ps, Q_CLS0 + CLS);
@@ -2299,6 +2302,9 @@ struct JobRun {
 // per-iteration kernels run on.
 struct SlotPool {
     uint32_t n = 0, blocks = 0;
+    // grid of each grid-stride kernel: the blocks that can be resident at
+    // once (more would wait for a second round and leave a tail)
+    uint32_t gridShade[kNumClasses] = {}, gridRegen = 0, gridMedium = 0;
     int device = 0;
     DevBuf<float4> rec;
     DevBuf<uint32_t> q, ctr, fetch, spill, jobActive, blockJob;
@@ -2325,6 +2331,14 @@ struct SlotPool {
 // between launches.
 constexpr uint32_t kGraphIters = 8;
 
+// Blocks of kBlock threads of `kernel` that fit on the device at once
+// (registers and LDS as compiled), at most `cap`.
+static uint32_t residentGrid(const void* kernel, uint32_t cap) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || per <= 0) return cap;
+    return std::max(1u, std::min(cap, (uint32_t)g_numCUs * (uint32_t)per));
+}
+
 static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<SlotPool>* out, std::string* err) {
     if (capacity % kCompactSlots) capacity += kCompactSlots - capacity % kCompactSlots;
     if (capacity == 0 || capacity > (1u << 26)) { if (err) *err = "slot pool size out of range"; return SLRH_ERR_CAPACITY_CODE; }
@@ -2346,6 +2360,13 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
     HIP_TRY(hipGetDevice(&p->device));
     p->n = capacity;
     p->blocks = blocks;
+    p->gridShade[CLS_DIFFUSE] = residentGrid((const void*)&k_shade<CLS_DIFFUSE>, blocks);
+    p->gridShade[CLS_CONDUCTOR] = residentGrid((const void*)&k_shade<CLS_CONDUCTOR>, blocks);
+    p->gridShade[CLS_DIELECTRIC] = residentGrid((const void*)&k_shade<CLS_DIELECTRIC>, blocks);
+    p->gridShade[CLS_MISS] = residentGrid((const void*)&k_shade<CLS_MISS>, blocks);
+    p->gridShade[CLS_MEDIUM] = residentGrid((const void*)&k_shade_medium, blocks);
+    p->gridRegen = residentGrid((const void*)&k_regen, blocks);
+    p->gridMedium = residentGrid((const void*)&k_medium, blocks);
     p->hostBlockJob.assign(capacity / kCompactSlots, kNoJob);
     PathState& ps = p->ps;
     ps.n = capacity;
@@ -2454,18 +2475,19 @@ static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
     const uint32_t cblocks = (groups + kCompactGroups - 1) / kCompactGroups;
     hipStream_t s = p.stream;
     if (ev) (void)hipEventRecord((*ev)[0], s);
-    hipLaunchKernelGGL(k_regen, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
+    hipLaunchKernelGGL(k_regen, dim3(p.gridRegen), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_compact<1>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
     if (ev) (void)hipEventRecord((*ev)[1], s);
     kTraceLaunch[tv](p.blocks, s, p.sc, p.ps);
     if (ev) (void)hipEventRecord((*ev)[2], s);
-    if (p.sc.volume) hipLaunchKernelGGL(k_medium, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
+    if (p.sc.volume) hipLaunchKernelGGL(k_medium, dim3(p.gridMedium), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
-    hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
-    hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
-    hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
-    hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
-    if (p.sc.volume) hipLaunchKernelGGL(k_shade_medium, dim3(p.blocks), dim3(kBlock), 0, s, p.sc, p.ps);
+    hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(p.gridShade[CLS_DIFFUSE]), dim3(kBlock), 0, s, p.sc, p.ps);
+    hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(p.gridShade[CLS_CONDUCTOR]), dim3(kBlock), 0, s, p.sc, p.ps);
+    hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(p.gridShade[CLS_DIELECTRIC]), dim3(kBlock), 0, s, p.sc, p.ps);
+    hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(p.gridShade[CLS_MISS]), dim3(kBlock), 0, s, p.sc, p.ps);
+    if (p.sc.volume)
+        hipLaunchKernelGGL(k_shade_medium, dim3(p.gridShade[CLS_MEDIUM]), dim3(kBlock), 0, s, p.sc, p.ps);
     if (ev) (void)hipEventRecord((*ev)[3], s);
     hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
     if (ev) (void)hipEventRecord((*ev)[4], s);
