@@ -8,6 +8,9 @@
 * cornell_16x16_image.npz: the reference PTRenderer's image of the Cornell
   box at 16 x 16: per pixel CIE XYZ, the mean and the standard error over
   32 independent batches of 512 samples per pixel (16384 spp in all).
+* ref_<case>.npz for each case of tests/refcases.py: the reference
+  renderer's image statistics (PTRenderer, or VolumetricPTRenderer for the
+  volumetric cases) that the 3-sigma parity tests compare against.
 
 Run from the repository root: python tests/golden/make_golden.py
 """
@@ -21,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 from oracle import refdrv  # noqa: E402
 from slr_amd import scenes  # noqa: E402
+from tests import refcases  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -52,8 +56,31 @@ def image(W=16, H=16, B=32, spp=512):
                         se=out.std(0, ddof=1) / np.sqrt(B), batches=B, spp=spp)
 
 
+def case(name):
+    make, W, H, B, spp, seed, vol = refcases.CASES[name]
+    desc = make()
+    cmf = refdrv.cmf_strata()
+    threads = max(1, min(8, len(os.sched_getaffinity(0))))
+    out = []
+    for b in range(B):
+        r = refdrv.RefScene(desc)
+        f, _, _ = r.render(W, H, spp, seed + b, threads=threads, volumetric=vol)
+        r.close()
+        out.append(refcases.xyz(f.astype(np.float64), cmf) / spp)
+    out = np.array(out)
+    tot = out.sum(axis=(1, 2, 3))
+    np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), mean=out.mean(0), se=out.std(0, ddof=1) / np.sqrt(B),
+                        tot_mean=tot.mean(), tot_se=tot.std(ddof=1) / np.sqrt(B), batches=B, spp=spp)
+
+
 if __name__ == "__main__":
     if not refdrv.available():
         sys.exit("the compiled reference is missing: make -f oracle/ref.mk")
-    hits()
-    image()
+    which = sys.argv[1:]
+    if not which or "hits" in which:
+        hits()
+    if not which or "image" in which:
+        image()
+    for name in refcases.CASES:
+        if not which or name in which:
+            case(name)

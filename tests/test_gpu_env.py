@@ -10,14 +10,12 @@ sampling by MIS.
 * an open scene under a sky with a sun matches the compiled reference
   PTRenderer per pixel and XYZ channel within 3 combined sigma.
 """
-import os
-
 import numpy as np
 import pytest
 
 import slr_amd
 from slr_amd import scenes
-from tests.conftest import need_ref
+from tests import refcases
 
 pytestmark = pytest.mark.gpu
 
@@ -56,43 +54,19 @@ def test_uniform_environment_furnace(gpu_ctx):
     assert abs(films[1].sum() / films[0].sum() / albedo - 1) < 0.005
 
 
-def _xyz(strata):
-    from oracle import refdrv
-    x, y, z, integ = refdrv.cmf_strata()
-    return np.stack([strata @ x, strata @ y, strata @ z], -1) / integ
-
-
 def test_sky_matches_reference_renderer_3sigma(gpu_ctx):
-    """Reference PTRenderer vs GPU on an open scene under a sky with a sun,
-    32x24, 16 batches of 32 spp each side."""
-    refdrv = need_ref()
-    W, H = 32, 24
-    B, spp = 16, 32
-    desc = scenes.sky_scene(W, H)
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    rb = []
-    for b in range(B):
-        r = refdrv.RefScene(desc)
-        f, _, _ = r.render(W, H, spp, 2000 + b, threads=threads)
-        rb.append(_xyz(f.astype(np.float64)) / spp)
-        r.close()
-    sc = slr_amd.build_scene(gpu_ctx, desc)
+    """Reference PTRenderer image of an open scene under a sky with a sun
+    (32 x 24, 32 batches of 32 spp, tests/golden/ref_sky.npz) vs 16 GPU
+    batches of 32 spp."""
+    make, W, H, _, spp, _, _ = refcases.CASES["sky"]
+    sc = slr_amd.build_scene(gpu_ctx, make())
+    cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(B):
+    for b in range(16):
         film = slr_amd.Film(gpu_ctx, W, H)
         slr_amd.PTRenderer(spp).render(sc, film, seed=300 + b)
-        gb.append(_xyz(film.strata()) / spp)
-    rb, gb = np.array(rb), np.array(gb)
-    mr, mg = rb.mean(0), gb.mean(0)
-    se = np.sqrt(rb.var(0, ddof=1) / B + gb.var(0, ddof=1) / B)
-    z = np.abs(mg - mr) / np.maximum(se, 1e-12)
-    fail = np.mean(z > 3.0)
-    relmse = np.mean((mg - mr) ** 2 / (mr ** 2 + 1e-3))
-    assert fail < 0.03, (fail, relmse)
-    tr, tg = rb.sum(axis=(1, 2, 3)), gb.sum(axis=(1, 2, 3))
-    se_t = np.sqrt(tr.var(ddof=1) / B + tg.var(ddof=1) / B)
-    assert abs(tg.mean() - tr.mean()) < 4 * se_t, (tg.mean(), tr.mean(), se_t)
-    assert relmse < 0.05
+        gb.append(refcases.xyz(film.strata(), cmf) / spp)
+    refcases.check_3sigma(np.array(gb), refcases.load("sky"))
 
 
 def test_environment_adds_light(gpu_ctx):

@@ -15,13 +15,12 @@
   reference PTRenderer within 3 sigma (compat mode), instanced lights and
   per-instance materials included.
 """
-import os
-
 import numpy as np
 import pytest
 
 import slr_amd
 from slr_amd import scenes
+from tests import refcases
 from tests.conftest import need_ref
 
 pytestmark = pytest.mark.gpu
@@ -116,49 +115,23 @@ def test_render_matches_flattened_scene(gpu_ctx):
     assert np.mean(rel < 1e-2) > 0.9, np.mean(rel < 1e-2)
 
 
-def _xyz(strata):
-    from oracle import refdrv
-    x, y, z, integ = refdrv.cmf_strata()
-    return np.stack([strata @ x, strata @ y, strata @ z], -1) / integ
-
-
 def test_render_matches_reference_renderer_3sigma(gpu_ctx):
-    """Reference PTRenderer (instances through InternalNode + ReferenceNode)
-    vs the GPU in compat mode, 32x32, 16 batches of 32 spp each side."""
-    refdrv = need_ref()
-    W = H = 32
-    B, spp = 16, 32
-    desc = scenes.instanced_cornell(W, H, n_instances=10, seed=12, uniform_scale=True, instanced_light=True)
-    desc["compat_instance_shadow"] = True
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    rb = []
-    for b in range(B):
-        r = refdrv.RefScene(desc)
-        f, _, _ = r.render(W, H, spp, 4000 + b, threads=threads)
-        rb.append(_xyz(f.astype(np.float64)) / spp)
-        r.close()
-    sc = slr_amd.build_scene(gpu_ctx, desc)
+    """Reference PTRenderer image (instances through InternalNode +
+    ReferenceNode, 32 x 32, 32 batches of 32 spp, tests/golden/ref_instanced.npz)
+    vs 16 GPU batches of 32 spp in compat mode."""
+    make, W, H, _, spp, _, _ = refcases.CASES["instanced"]
+    sc = slr_amd.build_scene(gpu_ctx, make())
+    cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(B):
+    for b in range(16):
         film = slr_amd.Film(gpu_ctx, W, H)
         slr_amd.PTRenderer(spp).render(sc, film, seed=500 + b)
-        gb.append(_xyz(film.strata()) / spp)
-    rb, gb = np.array(rb), np.array(gb)
-    mr, mg = rb.mean(0), gb.mean(0)
-    se = np.sqrt(rb.var(0, ddof=1) / B + gb.var(0, ddof=1) / B)
-    z = np.abs(mg - mr) / np.maximum(se, 1e-12)
-    fail = np.mean(z > 3.0)
-    relmse = np.mean((mg - mr) ** 2 / (mr ** 2 + 1e-3))
-    assert fail < 0.03, (fail, relmse)
-    tr, tg = rb.sum(axis=(1, 2, 3)), gb.sum(axis=(1, 2, 3))
-    se_t = np.sqrt(tr.var(ddof=1) / B + tg.var(ddof=1) / B)
-    assert abs(tg.mean() - tr.mean()) < 4 * se_t, (tg.mean(), tr.mean(), se_t)
-    assert relmse < 0.05
+        gb.append(refcases.xyz(film.strata(), cmf) / spp)
+    refcases.check_3sigma(np.array(gb), refcases.load("instanced"))
 
 
 def _field(W, H, per_instance):
-    return scenes.instanced_field(W, H, n_instances=16, seed=5, subdiv=3, terrain_n=20, knot=(40, 8),
-                                  env_size=(64, 32), sun_radiance=200.0, materials_per_instance=per_instance)
+    return refcases.field(W, H, per_instance)
 
 
 def test_per_instance_materials_equal_per_prototype_materials(gpu_ctx):
@@ -179,34 +152,15 @@ def test_per_instance_materials_equal_per_prototype_materials(gpu_ctx):
 def test_instanced_field_matches_reference_renderer_3sigma(gpu_ctx):
     """Instances of four meshes with mixed Lambert, Oren-Nayar, GGX conductor
     and GGX dielectric materials under an equirectangular sky with a sun:
-    the reference PTRenderer (one prototype per mesh and material) vs the
-    GPU (materials on the instances) in compat mode, 32x32, 16 batches of
-    16 spp each side."""
-    refdrv = need_ref()
-    W = H = 32
-    B, spp = 16, 16
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    dref = dict(_field(W, H, False), compat_instance_shadow=True)
-    rb = []
-    for b in range(B):
-        r = refdrv.RefScene(dref)
-        f, _, _ = r.render(W, H, spp, 7000 + b, threads=threads)
-        rb.append(_xyz(f.astype(np.float64)) / spp)
-        r.close()
+    the reference PTRenderer's image (one prototype per mesh and material,
+    32 x 32, 32 batches of 16 spp, tests/golden/ref_field.npz) vs 16 GPU
+    batches of 16 spp with the materials on the instances, compat mode."""
+    _, W, H, _, spp, _, _ = refcases.CASES["field"]
     sc = slr_amd.build_scene(gpu_ctx, dict(_field(W, H, True), compat_instance_shadow=True))
+    cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(B):
+    for b in range(16):
         film = slr_amd.Film(gpu_ctx, W, H)
         slr_amd.PTRenderer(spp).render(sc, film, seed=800 + b)
-        gb.append(_xyz(film.strata()) / spp)
-    rb, gb = np.array(rb), np.array(gb)
-    mr, mg = rb.mean(0), gb.mean(0)
-    se = np.sqrt(rb.var(0, ddof=1) / B + gb.var(0, ddof=1) / B)
-    z = np.abs(mg - mr) / np.maximum(se, 1e-12)
-    fail = np.mean(z > 3.0)
-    relmse = np.mean((mg - mr) ** 2 / (mr ** 2 + 1e-3))
-    assert fail < 0.03, (fail, relmse)
-    tr, tg = rb.sum(axis=(1, 2, 3)), gb.sum(axis=(1, 2, 3))
-    se_t = np.sqrt(tr.var(ddof=1) / B + tg.var(ddof=1) / B)
-    assert abs(tg.mean() - tr.mean()) < 4 * se_t, (tg.mean(), tr.mean(), se_t)
-    assert relmse < 0.05
+        gb.append(refcases.xyz(film.strata(), cmf) / spp)
+    refcases.check_3sigma(np.array(gb), refcases.load("field"))

@@ -14,7 +14,7 @@ import pytest
 import slr_amd
 from oracle import oracle
 from slr_amd import scenes
-from tests.conftest import need_ref
+from tests import refcases
 
 pytestmark = pytest.mark.gpu
 
@@ -51,44 +51,22 @@ def test_white_furnace(gpu_ctx):
 
 
 def _xyz(strata):
-    from oracle import refdrv
-    x, y, z, integ = refdrv.cmf_strata()
-    return np.stack([strata @ x, strata @ y, strata @ z], -1) / integ
+    # the film's CMF strata (checked against the reference's in test_host)
+    return refcases.xyz(strata, slr_amd.cmf_strata())
 
 
 def test_matches_reference_renderer_3sigma(gpu_ctx):
-    """Reference PTRenderer vs GPU on the Cornell box, 32x32, 16 batches of
-    64 spp each side; per-pixel XYZ within 3 combined sigma."""
-    refdrv = need_ref()
-    W = H = 32
-    B, spp = 16, 64
-    desc = scenes.cornell_box(W, H)
-    import os
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    rb = []
-    for b in range(B):
-        r = refdrv.RefScene(desc)
-        f, _, _ = r.render(W, H, spp, 1000 + b, threads=threads)
-        rb.append(_xyz(f.astype(np.float64)) / spp)
-        r.close()
-    sc = slr_amd.build_scene(gpu_ctx, desc)
+    """Reference PTRenderer image of the Cornell box (32 x 32, 32 batches of
+    64 spp, tests/golden/ref_cornell32.npz) vs 16 GPU batches of 64 spp."""
+    make, W, H, _, spp, _, _ = refcases.CASES["cornell32"]
+    sc = slr_amd.build_scene(gpu_ctx, make())
+    cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(B):
+    for b in range(16):
         film = slr_amd.Film(gpu_ctx, W, H)
-        slr_amd.PTRenderer(spp).render(sc, film, seed=77 + b)
-        gb.append(_xyz(film.strata()) / spp)
-    rb, gb = np.array(rb), np.array(gb)
-    mr, mg = rb.mean(0), gb.mean(0)
-    se = np.sqrt(rb.var(0, ddof=1) / B + gb.var(0, ddof=1) / B)
-    z = np.abs(mg - mr) / np.maximum(se, 1e-12)
-    fail = np.mean(z > 3.0)
-    relmse = np.mean((mg - mr) ** 2 / (mr ** 2 + 1e-3))
-    # 3 sigma with 15-dof variance estimates: expect ~1% chance failures
-    assert fail < 0.03, (fail, relmse)
-    tr, tg = rb.sum(axis=(1, 2, 3)), gb.sum(axis=(1, 2, 3))
-    se_t = np.sqrt(tr.var(ddof=1) / B + tg.var(ddof=1) / B)
-    assert abs(tg.mean() - tr.mean()) < 4 * se_t
-    assert relmse < 0.05
+        slr_amd.PTRenderer(spp).render(sc, film, seed=100 + b)
+        gb.append(refcases.xyz(film.strata(), cmf) / spp)
+    refcases.check_3sigma(np.array(gb), refcases.load("cornell32"))
 
 
 def test_slots_per_pixel_do_not_change_the_estimate(gpu_ctx):

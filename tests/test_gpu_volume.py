@@ -13,15 +13,13 @@
 * without a medium and the flag the volumetric integrator gives the path
   tracer's film.
 """
-import os
-
 import numpy as np
 import pytest
 
 import slr_amd
 from oracle import medium
 from slr_amd import scenes
-from tests.conftest import need_ref
+from tests import refcases
 
 pytestmark = pytest.mark.gpu
 
@@ -94,46 +92,21 @@ def test_absorbing_slab_closed_form(gpu_ctx):
     assert z.max() < 4.5, (z.max(), worst, y[worst], expect[worst], se[worst])
 
 
-def _xyz(strata):
-    from oracle import refdrv
-    x, y, z, integ = refdrv.cmf_strata()
-    return np.stack([strata @ x, strata @ y, strata @ z], -1) / integ
-
-
 def test_medium_matches_reference_renderer_3sigma(gpu_ctx):
-    """Reference VolumetricPTRenderer (DensityGridMediumNode in the Cornell
-    box) vs the GPU in the reference's compat mode, 32x32, 16 batches of 16
-    spp each side."""
-    refdrv = need_ref()
-    W = H = 32
-    B, spp = 16, 16
-    desc = scenes.cornell_medium(W, H, n=32, sigma_t=1.0, albedo=0.9, g=0.7)
-    desc["compat_volpt_tangent_nee"] = True
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    rb = []
-    for b in range(B):
-        r = refdrv.RefScene(desc)
-        f, _, _ = r.render(W, H, spp, 6000 + b, threads=threads, volumetric=True)
-        rb.append(_xyz(f.astype(np.float64)) / spp)
-        r.close()
-    sc = slr_amd.build_scene(gpu_ctx, desc)
+    """Reference VolumetricPTRenderer image (DensityGridMediumNode in the
+    Cornell box, 32 x 32, 32 batches of 16 spp, tests/golden/ref_medium.npz)
+    vs 16 GPU batches of 16 spp in the reference's compat mode."""
+    make, W, H, _, spp, _, _ = refcases.CASES["medium"]
+    sc = slr_amd.build_scene(gpu_ctx, make())
+    cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(B):
+    for b in range(16):
         film = slr_amd.Film(gpu_ctx, W, H)
         slr_amd.VolumetricPTRenderer(spp).render(sc, film, seed=900 + b)
-        gb.append(_xyz(film.strata()) / spp)
-    rb, gb = np.array(rb), np.array(gb)
+        gb.append(refcases.xyz(film.strata(), cmf) / spp)
+    gb = np.array(gb)
     assert np.isfinite(gb).all()
-    mr, mg = rb.mean(0), gb.mean(0)
-    se = np.sqrt(rb.var(0, ddof=1) / B + gb.var(0, ddof=1) / B)
-    z = np.abs(mg - mr) / np.maximum(se, 1e-12)
-    fail = np.mean(z > 3.0)
-    relmse = np.mean((mg - mr) ** 2 / (mr ** 2 + 1e-3))
-    assert fail < 0.03, (fail, relmse)
-    tr, tg = rb.sum(axis=(1, 2, 3)), gb.sum(axis=(1, 2, 3))
-    se_t = np.sqrt(tr.var(ddof=1) / B + tg.var(ddof=1) / B)
-    assert abs(tg.mean() - tr.mean()) < 4 * se_t, (tg.mean(), tr.mean(), se_t)
-    assert relmse < 0.05
+    refcases.check_3sigma(gb, refcases.load("medium"))
 
 
 def test_volumetric_without_medium_is_the_path_tracer(gpu_ctx):

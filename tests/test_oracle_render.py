@@ -1,12 +1,15 @@
-"""The oracle's integrator against the reference PTRenderer (statistical,
-3 sigma per pixel and XYZ channel over batch means) and against the
-white-furnace closed form."""
+"""The oracle's integrator against the reference renderers' image statistics
+(tests/golden/ref_*.npz; 3 sigma per pixel and XYZ channel over batch
+means), the white-furnace closed form, and the reference harness's
+environment against its closed form."""
 import os
 
 import numpy as np
 
+import slr_amd
 from oracle import oracle
 from slr_amd import scenes
+from tests import refcases
 from tests.conftest import need_ref
 
 
@@ -26,31 +29,19 @@ def test_white_furnace_closed_form():
 
 
 def test_oracle_matches_reference_renderer():
-    refdrv = need_ref()
-    W = H = 16
-    B, spp = 16, 64
-    desc = scenes.cornell_box(W, H)
-    cmf = refdrv.cmf_strata()
+    """The oracle's Cornell box (32 x 32, 16 batches of 64 spp) against the
+    reference PTRenderer's image statistics (tests/golden/ref_cornell32.npz):
+    per pixel and XYZ channel within 3 combined standard errors, image total
+    within 4 (batch sums are heavy-tailed: light seen through BSDF sampling)."""
+    make, W, H, _, spp, _, _ = refcases.CASES["cornell32"]
+    cmf = slr_amd.cmf_strata()
     threads = max(1, min(8, len(os.sched_getaffinity(0))))
-    rb, ob = [], []
-    orc = oracle.OracleScene(desc)
-    for b in range(B):
-        r = refdrv.RefScene(desc)
-        f, _, _ = r.render(W, H, spp, 500 + b, threads=threads)
-        r.close()
-        rb.append(_xyz(f.astype(np.float64), cmf) / spp)
-        f2, _ = orc.render(W, H, spp, 900 + b, threads=threads)
-        ob.append(_xyz(f2, cmf) / spp)
-    rb, ob = np.array(rb), np.array(ob)
-    mr, mo = rb.mean(0), ob.mean(0)
-    se = np.sqrt(rb.var(0, ddof=1) / B + ob.var(0, ddof=1) / B)
-    z = np.abs(mo - mr) / np.maximum(se, 1e-12)
-    assert np.mean(z > 3) < 0.03
-    # total energy: batch sums are heavy-tailed (light seen through BSDF
-    # sampling), so compare against their own spread rather than a fixed %
-    tr, to = rb.sum(axis=(1, 2, 3)), ob.sum(axis=(1, 2, 3))
-    se_t = np.sqrt(tr.var(ddof=1) / B + to.var(ddof=1) / B)
-    assert abs(to.mean() - tr.mean()) < 4 * se_t
+    orc = oracle.OracleScene(make())
+    ob = []
+    for b in range(16):
+        f, _ = orc.render(W, H, spp, 900 + b, threads=threads)
+        ob.append(refcases.xyz(f, cmf) / spp)
+    refcases.check_3sigma(np.array(ob), refcases.load("cornell32"))
 
 
 def test_reference_environment_closed_form():
@@ -78,35 +69,23 @@ def test_reference_environment_closed_form():
 
 def test_oracle_volumetric_compat_matches_reference_volumetric_renderer():
     """The reference VolumetricPTRenderer over the Cornell box without media
-    against the oracle in its volumetric compat mode (a visible light sample
-    in the shading plane turns the sample into NaN, which is not filed), 3
-    sigma per pixel and channel. The same oracle without the compat mode is
-    the path tracer, whose image carries about twice the energy: the light
-    is an emitter over a Lambert base, so a path that reaches it samples a
-    light point in its own plane, and the reference drops every sample that
-    sees the light directly."""
-    refdrv = need_ref()
-    W = H = 16
-    B, spp = 12, 32
-    desc = dict(scenes.cornell_box(W, H), compat_volpt_tangent_nee=True)
-    cmf = refdrv.cmf_strata()
+    (16 x 16, tests/golden/ref_volpt_cornell16.npz) against the oracle in its
+    volumetric compat mode (a visible light sample in the shading plane turns
+    the sample into NaN, which is not filed), 3 sigma per pixel and channel.
+    The same oracle without the compat mode is the path tracer, whose image
+    carries about twice the energy: the light is an emitter over a Lambert
+    base, so a path that reaches it samples a light point in its own plane,
+    and the reference drops every sample that sees the light directly."""
+    make, W, H, _, spp, _, _ = refcases.CASES["volpt_cornell16"]
+    cmf = slr_amd.cmf_strata()
     threads = max(1, min(8, len(os.sched_getaffinity(0))))
-    rb, ob, pb = [], [], []
-    orc = oracle.OracleScene(desc)
-    for b in range(B):
-        r = refdrv.RefScene(desc)
-        f, _, _ = r.render(W, H, spp, 700 + b, threads=threads, volumetric=True)
-        r.close()
-        rb.append(_xyz(f.astype(np.float64), cmf) / spp)
+    orc = oracle.OracleScene(make())
+    ob, pb = [], []
+    for b in range(12):
         f2, _ = orc.render(W, H, spp, 300 + b, threads=threads, volumetric=True)
-        ob.append(_xyz(f2, cmf) / spp)
+        ob.append(refcases.xyz(f2, cmf) / spp)
         f3, _ = orc.render(W, H, spp, 300 + b, threads=threads)
-        pb.append(_xyz(f3, cmf) / spp)
-    rb, ob, pb = np.array(rb), np.array(ob), np.array(pb)
-    se = np.sqrt(rb.var(0, ddof=1) / B + ob.var(0, ddof=1) / B)
-    z = np.abs(ob.mean(0) - rb.mean(0)) / np.maximum(se, 1e-12)
-    assert np.mean(z > 3) < 0.03, np.mean(z > 3)
-    tr, to, tp = rb.sum(axis=(1, 2, 3)), ob.sum(axis=(1, 2, 3)), pb.sum(axis=(1, 2, 3))
-    se_t = np.sqrt(tr.var(ddof=1) / B + to.var(ddof=1) / B)
-    assert abs(to.mean() - tr.mean()) < 4 * se_t, (to.mean(), tr.mean(), se_t)
-    assert tr.mean() < 0.7 * tp.mean(), (tr.mean(), tp.mean())
+        pb.append(refcases.xyz(f3, cmf) / spp)
+    ref = refcases.load("volpt_cornell16")
+    refcases.check_3sigma(np.array(ob), ref)
+    assert float(ref["tot_mean"]) < 0.7 * np.array(pb).sum(axis=(1, 2, 3)).mean()
