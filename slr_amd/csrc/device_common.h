@@ -83,6 +83,7 @@ struct DevCamera {
 };
 
 constexpr uint32_t kNoInstance = 0xFFFFFFFFu;
+constexpr uint32_t kLightRecRows = 7;
 constexpr uint32_t kInstXfRows = 8;
 
 struct DevScene {
@@ -96,6 +97,8 @@ struct DevScene {
     const DevSpectrum* spectra;
     const float* specPool;
     const uint2* lights;      // emitting triangles: global id, instance (kNoInstance: world space)
+    const float4* lightRec;   // per light kLightRecRows float4: world vertices (material, instance and area
+                              // PDF in w), geometric normal, the three object-space vertex normals
     const float* lightCdf;    // numLights + 1 entries
     const float* lightPmf;
     const int32_t* primToLight;

@@ -114,6 +114,7 @@ struct DeviceScene {
     DevBuf<DevSpectrum> spectra;
     DevBuf<float> specPool, lightCdf, lightPmf;
     DevBuf<uint2> lights;
+    DevBuf<float4> lightRec;
     DevBuf<int32_t> primToLight, instMat;
     DevBuf<float4> instXf;
     DevBuf<uint32_t> spill;      // traversal stack spill for ray batches
@@ -181,6 +182,8 @@ static void cameraDerived(const CameraParams& c, uint32_t w, uint32_t h, DevCame
     for (int r = 0; r < 3; ++r)
         for (int col = 0; col < 3; ++col) d->nrm[r * 3 + col] = (float)inv[col][r];
 }
+
+static int buildLightRecords(DeviceScene* ds, std::string* err);
 
 int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     std::unique_ptr<DeviceScene> ds(new DeviceScene());
@@ -456,6 +459,7 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     ds->hasCamera = sd.hasCamera;
     ds->camera = sd.camera;
     if (sd.hasCamera) cameraDerived(sd.camera, 1, 1, &v.cam);
+    if (int rl = buildLightRecords(ds.get(), err)) return rl;
     *out = ds.release();
     return 0;
 }
@@ -1419,6 +1423,72 @@ __device__ __forceinline__ float3 xfNormal(const float4* X, float3 n) {
 
 // A triangle's vertices in world space (inst < 0: already there).
 __device__ __forceinline__ void worldVerts(const DevScene& sc, uint32_t prim, int32_t inst, float3* q0, float3* q1,
+                                           float3* q2, uint4* pvOut);
+
+// The per-light records the next-event estimate reads (one gather instead of
+// the light list, triangle, vertex and normal chain), computed once per
+// scene on the device with the expressions the estimate used inline, so
+// the values are the same bits.
+__global__ void k_light_records(DevScene sc, float4* out) {
+    for (uint32_t li = blockIdx.x * blockDim.x + threadIdx.x; li < sc.numLights; li += gridDim.x * blockDim.x) {
+        const uint2 lent = sc.lights[li];
+        const int32_t linst = lent.y == kNoInstance ? -1 : (int32_t)lent.y;
+        float3 q0, q1, q2;
+        uint4 lv;
+        worldVerts(sc, lent.x, linst, &q0, &q1, &q2, &lv);
+        uint32_t mat = lv.w;
+        if (linst >= 0 && sc.instMat[linst] >= 0) mat = (uint32_t)sc.instMat[linst];
+        const float3 lg = normalize3(cross3(q1 - q0, q2 - q0));
+        const float areaPDF = 1.0f / (0.5f * length3(cross3(q1 - q0, q2 - q0)));
+        float4* R = out + (size_t)kLightRecRows * li;
+        R[0] = make_float4(q0.x, q0.y, q0.z, __uint_as_float(mat));
+        R[1] = make_float4(q1.x, q1.y, q1.z, __int_as_float(linst));
+        R[2] = make_float4(q2.x, q2.y, q2.z, areaPDF);
+        R[3] = make_float4(lg.x, lg.y, lg.z, 0.0f);
+        R[4] = sc.vnrm[lv.x];
+        R[5] = sc.vnrm[lv.y];
+        R[6] = sc.vnrm[lv.z];
+    }
+}
+
+static int buildLightRecords(DeviceScene* ds, std::string* err) {
+    const uint32_t n = ds->view.numLights;
+    ds->view.lightRec = nullptr;
+    if (n == 0) return 0;
+    if (int r = ds->lightRec.alloc((size_t)n * kLightRecRows, err)) return r;
+    hipLaunchKernelGGL(k_light_records, dim3((n + 255) / 256), dim3(256), 0, 0, ds->view, ds->lightRec.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    ds->view.lightRec = ds->lightRec.p;
+    return 0;
+}
+
+// A light sample on a triangle light (TriangleSurfaceShape::sample,
+// TriangleSurfaceShape.cpp:357-413, then the instance's transform,
+// TransformedSurfaceObject::selectLight): point, geometric normal, shading
+// normal, area PDF and material.
+struct LightPoint {
+    float3 lp, lg, ln;
+    float areaPDF;
+    uint32_t mat;
+};
+__device__ __forceinline__ LightPoint lightPoint(const DevScene& sc, uint32_t li, float lu0, float lu1) {
+    const float4* R = sc.lightRec + (size_t)kLightRecRows * li;
+    const float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3], n0 = R[4], n1 = R[5], n2 = R[6];
+    const float su = sqrtf(lu0);
+    const float lb0 = 1.0f - su, lb1 = lu1 * su, lb2 = 1.0f - lb0 - lb1;
+    const int32_t linst = __float_as_int(r1.w);
+    LightPoint L;
+    L.ln = normalize3(lb0 * xyz(n0) + lb1 * xyz(n1) + lb2 * xyz(n2));
+    if (linst >= 0) L.ln = normalize3(xfNormal(sc.instXf + kInstXfRows * (uint32_t)linst, L.ln));
+    L.lp = lb0 * xyz(r0) + lb1 * xyz(r1) + lb2 * xyz(r2);
+    L.lg = xyz(r3);
+    L.areaPDF = r2.w;
+    L.mat = __float_as_uint(r0.w);
+    return L;
+}
+
+__device__ __forceinline__ void worldVerts(const DevScene& sc, uint32_t prim, int32_t inst, float3* q0, float3* q1,
                                            float3* q2, uint4* pvOut) {
     const uint4 pv = sc.primVerts[prim];
     *q0 = xyz(sc.vpos[pv.x]);
@@ -1742,25 +1812,9 @@ void k_shade(DevScene sc, PathState ps) {
                     } else {
                         float lprob;
                         const uint32_t li = sampleLight(sc, uSurf, &lprob);
-                        const uint2 lent = sc.lights[li];
-                        const uint32_t lprim = lent.x;
-                        const int32_t linst = lent.y == kNoInstance ? -1 : (int32_t)lent.y;
-                        // TriangleSurfaceShape::sample (TriangleSurfaceShape.cpp:357-413),
-                        // then the instance's transform (TransformedSurfaceObject::selectLight)
-                        const float su = sqrtf(lu0);
-                        const float lb0 = 1.0f - su, lb1 = lu1 * su, lb2 = 1.0f - lb0 - lb1;
-                        float3 q0, q1, q2;
-                        uint4 lv;
-                        worldVerts(sc, lprim, linst, &q0, &q1, &q2, &lv);
-                        float3 ln = normalize3(lb0 * xyz(sc.vnrm[lv.x]) + lb1 * xyz(sc.vnrm[lv.y]) +
-                                               lb2 * xyz(sc.vnrm[lv.z]));
-                        if (linst >= 0) {
-                            ln = normalize3(xfNormal(sc.instXf + kInstXfRows * (uint32_t)linst, ln));
-                            if (sc.instMat[linst] >= 0) lv.w = (uint32_t)sc.instMat[linst];
-                        }
-                        const float3 lp = lb0 * q0 + lb1 * q1 + lb2 * q2;
-                        const float3 lg = normalize3(cross3(q1 - q0, q2 - q0));
-                        const float areaPDF = 1.0f / (0.5f * length3(cross3(q1 - q0, q2 - q0)));
+                        const LightPoint L = lightPoint(sc, li, lu0, lu1);
+                        const float3 lp = L.lp, lg = L.lg, ln = L.ln;
+                        const float areaPDF = L.areaPDF;
                         const float3 dv = lp - sp.p;
                         const float dist2 = dot3(dv, dv);
                         const float3 sdir = divs(dv, sqrtf(dist2));
@@ -1772,7 +1826,7 @@ void k_shade(DevScene sc, PathState ps) {
                             const float dist = length3(lp - sp.p);
                             tangentNaN(divs(lp - sp.p, dist), dist * (1 - 1e-4f));
                         } else if (cosS * cosL > 0 && lz > 0.0f) {
-                            const DevMaterial lm = sc.mats[lv.w];
+                            const DevMaterial lm = sc.mats[L.mat];
                             float M[kNumL], fs[kNumL];
                             evalSpectrum(sc, lm.emit, misc.x, M);
                             const float3 sdL = toLocal(sp, sdir);
@@ -1998,28 +2052,16 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
             } else if (sc.numLights > 0 && sc.lightImportance > 0.0f) {
                 float lprob;
                 const uint32_t li = sampleLight(sc, uSurf, &lprob);
-                const uint2 lent = sc.lights[li];
-                const int32_t linst = lent.y == kNoInstance ? -1 : (int32_t)lent.y;
-                const float su = sqrtf(lu0);
-                const float lb0 = 1.0f - su, lb1 = lu1 * su, lb2 = 1.0f - lb0 - lb1;
-                float3 q0, q1, q2;
-                uint4 lv;
-                worldVerts(sc, lent.x, linst, &q0, &q1, &q2, &lv);
-                float3 ln = normalize3(lb0 * xyz(sc.vnrm[lv.x]) + lb1 * xyz(sc.vnrm[lv.y]) + lb2 * xyz(sc.vnrm[lv.z]));
-                if (linst >= 0) {
-                    ln = normalize3(xfNormal(sc.instXf + kInstXfRows * (uint32_t)linst, ln));
-                    if (sc.instMat[linst] >= 0) lv.w = (uint32_t)sc.instMat[linst];
-                }
-                const float3 lp = lb0 * q0 + lb1 * q1 + lb2 * q2;
-                const float3 lg = normalize3(cross3(q1 - q0, q2 - q0));
-                const float areaPDF = 1.0f / (0.5f * length3(cross3(q1 - q0, q2 - q0)));
+                const LightPoint L = lightPoint(sc, li, lu0, lu1);
+                const float3 lp = L.lp, lg = L.lg, ln = L.ln;
+                const float areaPDF = L.areaPDF;
                 const float3 dv = lp - pos;
                 const float dist2 = dot3(dv, dv);
                 const float dist = sqrtf(dist2);
                 sdir = divs(dv, dist);
                 const float cosL = fabsf(dot3(sdir, lg));
                 if (cosL > 0.0f && dot3(ln, -sdir) > 0.0f) {
-                    evalSpectrum(sc, sc.mats[lv.w].emit, misc.x, Le);
+                    evalSpectrum(sc, sc.mats[L.mat].emit, misc.x, Le);
                     const float edf = (float)(1.0 / 3.14159265358979323846);
 #pragma unroll
                     for (int k = 0; k < kNumL; ++k) Le[k] *= edf;
