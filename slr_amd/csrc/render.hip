@@ -2656,8 +2656,9 @@ int poolSamples(SlotPool* p, double* samples, double* rays, std::string* err) {
 constexpr uint32_t kTargetSlots = 16u << 20;
 
 // Traversal autotuner. For each variant a pilot renders one sample per slot
-// of the first job (its camera and resolution, as many samples per pixel as
-// the job has slot planes, into a scratch film): the same pool, the same
+// of every job of the render (their cameras and resolutions, as many
+// samples per pixel as each job has slot planes, into scratch films): the
+// same pool, the same
 // wavefront width and the same scene, so the traversal launches see the
 // real mix of camera, bounce and shadow rays. Each variant's traversal
 // kernel time (HIP events around every launch) per traced ray is compared
@@ -2666,22 +2667,28 @@ constexpr uint32_t kTargetSlots = 16u << 20;
 // changes timing only. Pools under kTuneMinSlots keep variant 0.
 constexpr uint32_t kTuneMinSlots = 1u << 20;
 
-static int tuneTrace(DeviceScene* ds, SlotPool& p, const JobSpec& first, uint32_t planes, uint32_t base,
-                     std::string* err) {
-    DeviceFilm* scratch = nullptr;
-    int r = filmCreate(first.width, first.height, &scratch, err);
-    if (r) return r;
-    std::unique_ptr<DeviceFilm, void (*)(DeviceFilm*)> hold(scratch, filmFree);
-    JobSpec pilot = first;
-    pilot.spp = planes;
-    pilot.film = scratch;
+static int tuneTrace(DeviceScene* ds, SlotPool& p, const JobSpec* specs, uint32_t n, const uint32_t* planes,
+                     const uint32_t* bases, std::string* err) {
+    // every job of the render takes part (one sample per slot, scratch
+    // films), so the pilot's wavefront is as wide as the render's
+    std::vector<std::unique_ptr<DeviceFilm, void (*)(DeviceFilm*)>> scratch;
+    std::vector<JobSpec> pilots(specs, specs + n);
+    int r = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        DeviceFilm* f = nullptr;
+        if ((r = filmCreate(specs[i].width, specs[i].height, &f, err))) return r;
+        scratch.emplace_back(f, filmFree);
+        pilots[i].spp = planes[i];
+        pilots[i].film = f;
+    }
     std::vector<hipEvent_t> evs;
     auto cleanup = [&]() { for (hipEvent_t e : evs) (void)hipEventDestroy(e); };
     double best = 0;
     int bestV = 0;
     for (int v = 0; v < kNumVariants; ++v) {
         if ((r = poolReset(p, err))) { cleanup(); return r; }
-        if ((r = poolAdmit(&p, 0, pilot, planes, base, err))) { cleanup(); return r; }
+        for (uint32_t i = 0; i < n; ++i)
+            if ((r = poolAdmit(&p, i, pilots[i], planes[i], bases[i], err))) { cleanup(); return r; }
         hipLaunchKernelGGL(k_compact<0>, dim3(compactBlocks(p.n)), dim3(256), 0, p.stream, p.ps, p.n / kCompactSlots);
         std::vector<IterEvents> itEv;
         for (;;) {
@@ -2696,7 +2703,8 @@ static int tuneTrace(DeviceScene* ds, SlotPool& p, const JobSpec& first, uint32_
             if ((r = readCounters(p, err))) { cleanup(); return r; }
             if (p.hostCtr[C_OVERFLOW] || (p.hostCtr[C_DONE] == 0 && p.hostCtr[C_PENDING] == 0)) break;
         }
-        if ((r = finishJob(p, p.runs[0], err))) { cleanup(); return r; }
+        for (uint32_t i = 0; i < n; ++i)
+            if ((r = finishJob(p, p.runs[i], err))) { cleanup(); return r; }
         HIP_TRY(hipStreamSynchronize(p.stream));
         unsigned long long stat[kNumStat];
         if ((r = readStats(&p, stat, err))) { cleanup(); return r; }
@@ -2765,7 +2773,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
         // later renders of any size reuse the choice (a server's requests
         // vary in size, and a pilot per pool size would cost more than it saves)
         if (p.n >= kTuneMinSlots && ds->tunedVariant < 0)
-            if ((r = tuneTrace(ds, p, specs[0], planes[0], bases[0], err))) return r;
+            if ((r = tuneTrace(ds, p, specs, n, planes.data(), bases.data(), err))) return r;
         tv = ds->tunedVariant >= 0 ? ds->tunedVariant : 0;
     }
     if ((r = poolReset(p, err))) return r;
