@@ -120,6 +120,25 @@ def trace_roofline(tot, config, inst_note=False):
     return r
 
 
+def medium_roofline(tot):
+    """k_medium's algorithmic bytes per launch over its mean launch time (HIP
+    events around each launch): 32 B per density lookup (the 8 lattice
+    values of a trilinear lookup), 4 B per majorant cell entered, 48 B per
+    tracked ray (origin, direction, segment, hit). None without a medium."""
+    if tot.get("medium_lookups", 0) <= 0 or tot.get("medium_seconds", 0) <= 0:
+        return None
+    launches = max(1.0, tot["trace_launches"])
+    b = 32.0 * tot["medium_lookups"] + 4.0 * tot["medium_cells"] + 48.0 * tot["medium_rays"]
+    avg = tot["medium_seconds"] / launches
+    achieved = b / launches / avg / 1e9
+    return {"bound": "hbm", "kernel": "k_medium (delta tracking of extension rays, ratio tracking of light samples)",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": round(avg * 1e3, 4),
+            "bytes_per_launch": round(b / launches),
+            "lookups_per_ray": round(tot["medium_lookups"] / max(1.0, tot["medium_rays"]), 3),
+            "cells_per_ray": round(tot["medium_cells"] / max(1.0, tot["medium_rays"]), 3)}
+
+
 def frame_breakdown(tot, steps):
     return {"trace": round(tot["trace_seconds"] / steps * 1e3, 2),
             "shade": round(tot["shade_seconds"] / steps * 1e3, 2),
@@ -188,6 +207,7 @@ def run_batch(args, sc, frames, renderer, max_depth, config, workload, data, ins
         "roofline": trace_roofline(tot, args.config, inst_note),
         "frame_ms_breakdown": frame_breakdown(tot, args.steps),
         "medium_events_per_sample": round(tot["medium_events"] / max(1.0, tot["samples"]), 3),
+        "medium_roofline": medium_roofline(tot),
         "trace_variant": int(tot["trace_variant"] / max(1, args.steps)),
         "cpu_baseline": None,
     }

@@ -309,14 +309,16 @@ SLRH_API int slrh_film_read(const slrh_film* f, int format, double scale, void* 
 
 /* Renders into `film` (accumulating onto what it holds). Replaces
  * PTRenderer::render (libSLR/Renderer/PTRenderer.cpp:26-101). stats, if not
- * NULL, receives 18 doubles: [0] rays traced, [1] samples, [2] device seconds,
+ * NULL, receives 22 doubles: [0] rays traced, [1] samples, [2] device seconds,
  * [3] non-finite samples dropped, [4] wavefront iterations, [5] traversal
  * kernel seconds (extension and shadow rays traced together), [6] traversal
  * launches, [7] extension rays, [8] shading kernel seconds, [9] seconds of
  * the other kernels (regeneration, compaction), [10] shadow rays, [11] QBVH
  * nodes visited and [12] triangles tested by extension rays, [13] nodes and
  * [14] triangles by shadow rays, [15] traversal variant used, [16] scattering
- * events in the medium and [17] light samples attenuated by it (SLRH_VOL_PT).
+ * events in the medium and [17] light samples attenuated by it (SLRH_VOL_PT),
+ * [18] seconds of the medium tracking kernel, and its [19] density lookups
+ * (tentative collisions), [20] majorant cells entered and [21] rays tracked.
  * Kernel seconds are 0 unless flags bit 0. */
 SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
                          double* stats);

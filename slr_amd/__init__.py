@@ -457,7 +457,8 @@ def trace_variants():
 
 STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "trace_seconds", "trace_launches",
              "extend_rays", "shade_seconds", "other_seconds", "shadow_rays", "nodes_visited", "tris_tested",
-             "nodes_shadow", "tris_shadow", "trace_variant", "medium_events", "medium_shadows"]
+             "nodes_shadow", "tris_shadow", "trace_variant", "medium_events", "medium_shadows", "medium_seconds",
+             "medium_lookups", "medium_cells", "medium_rays"]
 
 
 def _fill_frame(fr, f, default_spp):

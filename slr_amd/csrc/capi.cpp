@@ -749,6 +749,8 @@ void copyStats(const RenderStats& st, double* stats) {
     stats[13] = st.nodesShadow; stats[14] = st.trisShadow;
     stats[15] = st.traceVariant;
     stats[16] = st.mediumEvents; stats[17] = st.mediumShadows;
+    stats[18] = st.mediumSeconds; stats[19] = st.mediumLookups; stats[20] = st.mediumCells;
+    stats[21] = st.mediumRays;
 }
 }  // namespace
 

@@ -91,6 +91,8 @@ struct RenderStats {
     double overflow = 0;
     double traceVariant = 0;   // traversal variant the render ran with
     double mediumEvents = 0, mediumShadows = 0;   // scattering events, light samples through the medium
+    double mediumSeconds = 0;                     // k_medium time (with kernel timing)
+    double mediumLookups = 0, mediumCells = 0, mediumRays = 0;   // k_medium work: density lookups, majorant cells, entries
 };
 
 int deviceInit(int device, std::string* err);

@@ -656,7 +656,10 @@ enum Stat64 : uint32_t {
     S_RAYS_EXT = 0, S_RAYS_SHADOW = 1, S_NODES_EXT = 2, S_TRIS_EXT = 3, S_SAMPLES = 4, S_NODES_SH = 5, S_TRIS_SH = 6,
     S_MED_EVENTS = 7,     // scattering events in the medium
     S_MED_SHADOWS = 8,    // light samples whose segment crossed the medium
-    kNumStat = 10
+    S_MED_LOOKUPS = 9,    // tentative collisions (density lookups) in k_medium
+    S_MED_CELLS = 10,     // majorant cells entered in k_medium
+    S_MED_RAYS = 11,      // trace entries k_medium tracked through the medium
+    kNumStat = 12
 };
 // Per-slot work code, one byte per slot, rewritten by each stage and turned
 // into slot-ordered queues by k_compact. After shading a slot holds EXT or
@@ -1002,9 +1005,14 @@ constexpr int kMaxTrackSteps = 1 << 16;
 // 208-367). collide(t) is called at each tentative collision with the
 // majorant there and returns true to stop the walk (the walk then returns
 // true); false: the walk reached t1.
+// Work counters of the tracking walks (majorant cells entered, density lookups).
+struct MedCount {
+    uint32_t cells = 0, lookups = 0;
+};
+
 template <typename Collide>
 __device__ __forceinline__ bool trackMedium(const DevScene& sc, float3 o, float3 d, float t0, float t1, uint4& rng,
-                                            Collide&& collide) {
+                                            Collide&& collide, MedCount* mc = nullptr) {
     if (!clipToMedium(sc, o, d, &t0, &t1)) return false;
     const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
     int cell[3], step[3];
@@ -1036,10 +1044,12 @@ __device__ __forceinline__ bool trackMedium(const DevScene& sc, float3 o, float3
     for (int it = 0; it < kMaxTrackSteps; ++it) {
         const float mu =
             sc.medSigmaE * sc.medMajorant[((size_t)cell[2] * sc.medCells[1] + cell[1]) * sc.medCells[0] + cell[0]];
+        if (mc) ++mc->cells;
         const float tExit = fminf(fminf(tNext[0], tNext[1]), fminf(tNext[2], t1));
         const float span = fmaxf(tExit - t, 0.0f);
         if (mu > 0.0f && tau < mu * span) {
             t += tau / mu;
+            if (mc) ++mc->lookups;
             if (collide(t, mu)) return true;
             tau = -logf(1.0f - xsFloat(rng));
             continue;
@@ -1061,7 +1071,7 @@ __device__ __forceinline__ bool trackMedium(const DevScene& sc, float3 o, float3
 // DensityGridMediumDistribution.cpp:454-486, is one). Returns true with the
 // scattering distance in *ts when a real collision happens before t1.
 __device__ __forceinline__ bool sampleFreePath(const DevScene& sc, float3 o, float3 d, float t0, float t1, uint4& rng,
-                                               float* ts) {
+                                               float* ts, MedCount* mc = nullptr) {
     return trackMedium(sc, o, d, t0, t1, rng, [&](float t, float mu) {
         const float sigma = sc.medSigmaE * mediumDensity(sc, o + d * t);
         if (xsFloat(rng) < sigma / mu) {
@@ -1069,14 +1079,14 @@ __device__ __forceinline__ bool sampleFreePath(const DevScene& sc, float3 o, flo
             return true;
         }
         return false;
-    });
+    }, mc);
 }
 
 // Transmittance of [t0, t1] by ratio tracking with Russian roulette below
 // 0.1, as the reference's estimateTransmittance (DensityGridMediumDistribution.cpp:
 // 618-655); one estimate serves all wavelengths (grey extinction).
 __device__ __forceinline__ float mediumTransmittance(const DevScene& sc, float3 o, float3 d, float t0, float t1,
-                                                     uint4& rng) {
+                                                     uint4& rng, MedCount* mc = nullptr) {
     float T = 1.0f;
     trackMedium(sc, o, d, t0, t1, rng, [&](float t, float mu) {
         const float sigma = sc.medSigmaE * mediumDensity(sc, o + d * t);
@@ -1089,7 +1099,7 @@ __device__ __forceinline__ float mediumTransmittance(const DevScene& sc, float3 
             }
         }
         return false;
-    });
+    }, mc);
     return T;
 }
 
@@ -1926,7 +1936,8 @@ void k_shade(DevScene sc, PathState ps) {
 __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_TRACE];
     const uint32_t* qin = queue(ps, Q_TRACE);
-    unsigned long long events = 0, shadows = 0;
+    unsigned long long events = 0, shadows = 0, rays = 0;
+    MedCount mc;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
         const uint32_t e = qin[i];
         const uint32_t s = e & ~kShadowEntry;
@@ -1939,8 +1950,9 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
             float t0 = o4.w, t1 = d4.w;
             if (!clipToMedium(sc, o, d, &t0, &t1)) continue;
             ++shadows;
+            ++rays;
             uint4 rng = shadowStream(ps.u4(s, SF_JOB).z, ps.u4(s, SF_INFO));
-            const float T = mediumTransmittance(sc, o, d, o4.w, d4.w, rng);
+            const float T = mediumTransmittance(sc, o, d, o4.w, d4.w, rng, &mc);
             if (T != 1.0f) {
                 float c[kNumL];
                 loadPlanes(ps, s, SF_NEE, c);
@@ -1955,7 +1967,8 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
             if (!clipToMedium(sc, o, d, &t0, &t1)) continue;
             uint4 rng = ps.u4(s, SF_RNG);
             float ts;
-            if (sampleFreePath(sc, o, d, o4.w, h.x, rng, &ts)) {
+            ++rays;
+            if (sampleFreePath(sc, o, d, o4.w, h.x, rng, &ts, &mc)) {
                 ps.f4(s, SF_HIT) = make_float4(ts, __int_as_float(-2), 0.0f, 0.0f);
                 ps.code[s] = (uint8_t)(CODE_CLS0 + CLS_MEDIUM);
                 ++events;
@@ -1965,6 +1978,9 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
     }
     statAdd(ps.stat, S_MED_EVENTS, events);
     statAdd(ps.stat, S_MED_SHADOWS, shadows);
+    statAdd(ps.stat, S_MED_LOOKUPS, mc.lookups);
+    statAdd(ps.stat, S_MED_CELLS, mc.cells);
+    statAdd(ps.stat, S_MED_RAYS, rays);
 }
 
 // Shading at a scattering point in the medium (VolumetricPTRenderer::Job::
@@ -2509,7 +2525,10 @@ void traceVariant(int v, int* ldsDepth, int* refill, int* fused) {
 }
 
 // Events around the kernel groups of one iteration (kernel timing only).
-typedef std::array<hipEvent_t, 5> IterEvents;
+// [0] start, [1] after regeneration + compaction, [2] after the traversal,
+// [3] after shading, [4] after the last compaction; [5], [6] around k_medium
+// (recorded only with a medium)
+typedef std::array<hipEvent_t, 7> IterEvents;
 
 // One wavefront iteration: retire/restart, trace, classify, shade, compact.
 static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
@@ -2522,7 +2541,11 @@ static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
     if (ev) (void)hipEventRecord((*ev)[1], s);
     kTraceLaunch[tv](p.blocks, s, p.sc, p.ps);
     if (ev) (void)hipEventRecord((*ev)[2], s);
-    if (p.sc.volume) hipLaunchKernelGGL(k_medium, dim3(p.gridMedium), dim3(kBlock), 0, s, p.sc, p.ps);
+    if (p.sc.volume) {
+        if (ev) (void)hipEventRecord((*ev)[5], s);
+        hipLaunchKernelGGL(k_medium, dim3(p.gridMedium), dim3(kBlock), 0, s, p.sc, p.ps);
+        if (ev) (void)hipEventRecord((*ev)[6], s);
+    }
     hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
     hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(p.gridShade[CLS_DIFFUSE]), dim3(kBlock), 0, s, p.sc, p.ps);
     hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(p.gridShade[CLS_CONDUCTOR]), dim3(kBlock), 0, s, p.sc, p.ps);
@@ -2828,17 +2851,22 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
         st->trisShadow = (double)stat[S_TRIS_SH];
         st->mediumEvents = (double)stat[S_MED_EVENTS];
         st->mediumShadows = (double)stat[S_MED_SHADOWS];
+        st->mediumLookups = (double)stat[S_MED_LOOKUPS];
+        st->mediumCells = (double)stat[S_MED_CELLS];
+        st->mediumRays = (double)stat[S_MED_RAYS];
         st->kernelSeconds = ms * 1e-3;
         st->nonFinite = p.hostCtr[C_NONFINITE];
         st->overflow = p.hostCtr[C_OVERFLOW];
         st->iterations = it;
-        double tr = 0, sh = 0, ot = 0;
+        double tr = 0, sh = 0, ot = 0, md = 0;
         auto el = [](hipEvent_t a, hipEvent_t b) { float t = 0; (void)hipEventElapsedTime(&t, a, b); return (double)t; };
         for (auto& e : itEv) {
             ot += el(e[0], e[1]) + el(e[3], e[4]);
             tr += el(e[1], e[2]);
             sh += el(e[2], e[3]);
+            if (p.sc.volume) md += el(e[5], e[6]);
         }
+        st->mediumSeconds = md * 1e-3;
         st->traceSeconds = tr * 1e-3;
         st->shadeSeconds = sh * 1e-3;
         st->otherSeconds = ot * 1e-3;
