@@ -1943,15 +1943,19 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
         const uint32_t s = e & ~kShadowEntry;
         if (e & kShadowEntry) {
             // the light sample (an occluded one needs no transmittance), on a
-            // stream of its own so that it runs beside the slot's extension ray
-            if (ps.vis[s] == 0) continue;
+            // stream of its own so that it runs beside the slot's extension ray;
+            // everything the entry needs is loaded at once, before the tests
+            const uint8_t vis = ps.vis[s];
             const float4 o4 = ps.f4(s, SF_SO), d4 = ps.f4(s, SF_SD);
+            const uint32_t slotSeed = ps.u4(s, SF_JOB).z;
+            const uint4 info = ps.u4(s, SF_INFO);
+            if (vis == 0) continue;
             const float3 o = xyz(o4), d = xyz(d4);
             float t0 = o4.w, t1 = d4.w;
             if (!clipToMedium(sc, o, d, &t0, &t1)) continue;
             ++shadows;
             ++rays;
-            uint4 rng = shadowStream(ps.u4(s, SF_JOB).z, ps.u4(s, SF_INFO));
+            uint4 rng = shadowStream(slotSeed, info);
             const float T = mediumTransmittance(sc, o, d, o4.w, d4.w, rng, &mc);
             if (T != 1.0f) {
                 float c[kNumL];
@@ -1962,10 +1966,10 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
             }
         } else {
             const float4 o4 = ps.f4(s, SF_RO), d4 = ps.f4(s, SF_RD), h = ps.f4(s, SF_HIT);
+            uint4 rng = ps.u4(s, SF_RNG);
             const float3 o = xyz(o4), d = xyz(d4);
             float t0 = o4.w, t1 = h.x;
             if (!clipToMedium(sc, o, d, &t0, &t1)) continue;
-            uint4 rng = ps.u4(s, SF_RNG);
             float ts;
             ++rays;
             if (sampleFreePath(sc, o, d, o4.w, h.x, rng, &ts, &mc)) {
