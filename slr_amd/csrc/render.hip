@@ -1041,10 +1041,10 @@ __device__ __forceinline__ bool trackMedium(const DevScene& sc, float3 o, float3
     }
     float t = t0;
     float tau = -logf(1.0f - xsFloat(rng));
+    if (mc) ++mc->cells;
     for (int it = 0; it < kMaxTrackSteps; ++it) {
         const float mu =
             sc.medSigmaE * sc.medMajorant[((size_t)cell[2] * sc.medCells[1] + cell[1]) * sc.medCells[0] + cell[0]];
-        if (mc) ++mc->cells;
         const float tExit = fminf(fminf(tNext[0], tNext[1]), fminf(tNext[2], t1));
         const float span = fmaxf(tExit - t, 0.0f);
         if (mu > 0.0f && tau < mu * span) {
@@ -1061,6 +1061,7 @@ __device__ __forceinline__ bool trackMedium(const DevScene& sc, float3 o, float3
         cell[a] += step[a];
         if (cell[a] < 0 || cell[a] >= (int)sc.medCells[a]) return false;
         tNext[a] += tDelta[a];
+        if (mc) ++mc->cells;
     }
     return false;
 }
