@@ -376,6 +376,9 @@ def main():
     import slr_amd
     from slr_amd import scenes
 
+    if args.calib_copy_mib > 0:
+        calibration_copy(args.calib_copy_mib << 20)
+        log("calibration copy: %d MiB device to device" % args.calib_copy_mib)
     if args.config == "serve":
         return serve(args)
     if args.config == "batch8":
@@ -401,9 +404,6 @@ def main():
     log("scene generated: %d triangles in %.1fs" % (ntris, time.time() - t0))
 
     ctx = slr_amd.Context(0)
-    if args.calib_copy_mib > 0:
-        calibration_copy(args.calib_copy_mib << 20)
-        log("calibration copy: %d MiB device to device" % args.calib_copy_mib)
     t0 = time.time()
     sc = slr_amd.build_scene(ctx, desc)
     build_s = time.time() - t0

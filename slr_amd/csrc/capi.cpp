@@ -813,6 +813,11 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     for (uint32_t i = 0; i < n; ++i) {
         int r = frameSpec(s->ctx, frames[i], set->max_depth, &specs[i]);
         if (r) return r;
+        // a slot adds into its film cell with a plain read-modify-write; two
+        // frames of one wavefront sharing a film would race on its cells
+        for (uint32_t j = 0; j < i; ++j)
+            if (frames[j].film == frames[i].film)
+                return fail(s->ctx, SLRH_ERR_INVALID, "two frames of a batch share a film");
     }
     RenderOptions opt;
     opt.slotsPerPixel = set->slots_per_pixel;

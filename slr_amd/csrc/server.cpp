@@ -51,7 +51,8 @@ Server::~Server() {
 int Server::submit(const JobSpec& spec, uint32_t planes, uint64_t* ticket, std::string* err) {
     const uint64_t pixels = (uint64_t)spec.width * spec.height;
     if (pixels == 0 || spec.spp == 0 || !spec.film) { if (err) *err = "empty request"; return -1; }
-    if (pixels > poolCapacity(pool_)) { if (err) *err = "request larger than the path-state budget"; return -5; }
+    const uint64_t need = (pixels * planesFor(spec, planes) + kPoolAlign - 1) / kPoolAlign * kPoolAlign;
+    if (need > poolCapacity(pool_)) { if (err) *err = "request larger than the path-state budget"; return -5; }
     std::lock_guard<std::mutex> lk(mu_);
     if (failed_) { if (err) *err = failure_; return -4; }
     Request q;
@@ -144,30 +145,37 @@ void Server::freeBlocks(uint32_t first, uint32_t nblocks) {
     for (uint32_t k = first; k < first + nblocks; ++k) blockUsed_[k] = false;
 }
 
+// Slots per pixel of a request: the caller's count, or with 0 a count that
+// depends only on the request and the pool (never on the load, so a seeded
+// request renders the same film whenever it is admitted): about
+// kServeSlotsPerRequest live paths, at most its spp and 16, and no more
+// than the whole pool holds.
+uint32_t Server::planesFor(const JobSpec& spec, uint32_t requested) const {
+    const uint64_t pixels = (uint64_t)spec.width * spec.height;
+    uint64_t planes = requested;
+    if (planes == 0) {
+        planes = std::max<uint64_t>(1, std::min<uint64_t>({kServeSlotsPerRequest / pixels, spec.spp, 16}));
+        const uint64_t fit = std::max<uint64_t>(1, poolCapacity(pool_) / pixels);
+        planes = std::min(planes, fit);
+    }
+    return (uint32_t)std::min<uint64_t>(planes, spec.spp);
+}
+
 // Admits queued requests in order while they fit (holding mu_). A request
-// without a fixed slots-per-pixel count gets enough planes to fill about
-// kTargetServeSlots live paths among the requests in the pool, at most its
-// spp and 16, and fewer if only a smaller range is free.
+// whose film a running request still accumulates into waits for it: slots
+// add into their film cells without atomics.
 bool Server::admitQueued(std::string* err, int* rc) {
     bool any = false;
     while (!queue_.empty() && !freeJobs_.empty()) {
         Request& q = requests_[queue_.front()];
         const uint64_t pixels = (uint64_t)q.spec.width * q.spec.height;
-        uint32_t planes = q.planes;
-        if (planes == 0) {
-            const uint64_t share = kTargetServeSlots / (uint64_t)(running_.size() + queue_.size());
-            planes = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({share / pixels, q.spec.spp, 16}));
-        }
-        planes = std::min(planes, q.spec.spp);
-        uint32_t first = 0, nblocks = 0;
-        bool ok = false;
-        for (;;) {
-            nblocks = (uint32_t)((planes * pixels + kPoolAlign - 1) / kPoolAlign);
-            if (allocBlocks(nblocks, &first)) { ok = true; break; }
-            if (q.planes != 0 || planes == 1) break;
-            planes = std::max(1u, planes / 2);
-        }
-        if (!ok) break;   // FIFO: wait for room
+        bool filmBusy = false;
+        for (uint64_t t : running_) filmBusy = filmBusy || requests_[t].spec.film == q.spec.film;
+        if (filmBusy) break;   // FIFO: wait until that request is done
+        const uint32_t planes = planesFor(q.spec, q.planes);
+        uint32_t first = 0;
+        const uint32_t nblocks = (uint32_t)((planes * pixels + kPoolAlign - 1) / kPoolAlign);
+        if (!allocBlocks(nblocks, &first)) break;   // FIFO: wait for room
         const uint32_t job = freeJobs_.back();
         freeJobs_.pop_back();
         int r = poolAdmit(pool_, job, q.spec, planes, first * kPoolAlign, err);

@@ -18,7 +18,8 @@ class Server {
 public:
     enum State { kQueued = 0, kRunning = 1, kDone = 2, kCancelled = 3, kFailed = 4 };
     static constexpr uint64_t kSlotBytes = 384 + 4 * 8 + 2;   // slot record, queue entries, code and flag bytes
-    static constexpr uint64_t kTargetServeSlots = 16u << 20;
+    // live paths a request without a fixed slots-per-pixel count aims for
+    static constexpr uint64_t kServeSlotsPerRequest = 2u << 20;
 
     ~Server();
     int start(DeviceScene* ds, uint64_t pathBytes, int traceVariant, std::string* err);
@@ -41,6 +42,7 @@ private:
     };
     void loop();
     bool admitQueued(std::string* err, int* rc);
+    uint32_t planesFor(const JobSpec& spec, uint32_t requested) const;
     bool allocBlocks(uint32_t nblocks, uint32_t* first);
     void freeBlocks(uint32_t first, uint32_t nblocks);
     void fail(const std::string& msg);

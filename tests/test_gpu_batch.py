@@ -127,3 +127,14 @@ def test_autotuner_picks_a_variant_and_keeps_the_film(gpu_ctx):
     b = slr_amd.Film(gpu_ctx, W, H)
     slr_amd.PTRenderer(2).render(sc, b, seed=5, slots_per_pixel=1, tune_trace=(v + 1) % len(slr_amd.trace_variants()) + 1)
     assert np.array_equal(a.strata().view(np.uint64), b.strata().view(np.uint64))
+
+
+def test_batch_rejects_a_shared_film(gpu_ctx):
+    """Slots add into their film cells without atomics, so two frames of one
+    wavefront may not accumulate into the same film."""
+    desc = scenes.cornell_box(16, 16)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    film = slr_amd.Film(gpu_ctx, 16, 16)
+    frames = [dict(film=film, camera=desc["camera"], seed=1), dict(film=film, camera=desc["camera"], seed=2)]
+    with pytest.raises(slr_amd.SlrhError, match="share a film"):
+        slr_amd.PTRenderer(2).render_batch(sc, frames)

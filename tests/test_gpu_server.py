@@ -118,3 +118,40 @@ def test_cancel_frees_slots(gpu_ctx, cornell):
     b = slr_amd.Film(gpu_ctx, 32, 32)
     slr_amd.PTRenderer(2).render_batch(sc, [dict(film=b, camera=desc["camera"], seed=2)], slots_per_pixel=1)
     assert np.array_equal(_bits(small), _bits(b))
+
+
+def test_requests_sharing_a_film_accumulate_in_turn(gpu_ctx, cornell):
+    """Two requests into one film: the second waits for the first (their
+    slots would race on the film cells), so the film holds the sum of the
+    two renders, bit for bit."""
+    desc, sc = cornell
+    srv = slr_amd.Server(sc, path_state_bytes=256 << 20)
+    try:
+        shared = slr_amd.Film(gpu_ctx, 64, 64)
+        ts = [srv.submit(shared, desc["camera"], spp=4, seed=s, slots_per_pixel=2) for s in (21, 22)]
+        assert [srv.wait(t, 60000) for t in ts] == [slr_amd.REQ_DONE] * 2
+    finally:
+        srv.close()
+    a = slr_amd.Film(gpu_ctx, 64, 64)
+    slr_amd.PTRenderer(4).render_batch(sc, [dict(film=a, camera=desc["camera"], seed=21)], slots_per_pixel=2)
+    slr_amd.PTRenderer(4).render_batch(sc, [dict(film=a, camera=desc["camera"], seed=22)], slots_per_pixel=2)
+    assert np.array_equal(_bits(shared), _bits(a))
+
+
+def test_automatic_slots_per_pixel_do_not_depend_on_load(gpu_ctx, cornell):
+    """Without a fixed slots-per-pixel count a request gets one chosen from
+    the request alone, so its film is the same on an idle and a busy server."""
+    desc, sc = cornell
+    cam = _moved(desc["camera"], 0.15)
+    srv = slr_amd.Server(sc, path_state_bytes=256 << 20)
+    try:
+        idle = slr_amd.Film(gpu_ctx, 40, 32)
+        assert srv.wait(srv.submit(idle, cam, spp=12, seed=5), 60000) == slr_amd.REQ_DONE
+        busy = slr_amd.Film(gpu_ctx, 40, 32)
+        load = [slr_amd.Film(gpu_ctx, 64, 64) for _ in range(4)]
+        tl = [srv.submit(f, desc["camera"], spp=16, seed=50 + i) for i, f in enumerate(load)]
+        tb = srv.submit(busy, cam, spp=12, seed=5)
+        assert [srv.wait(t, 60000) for t in tl + [tb]] == [slr_amd.REQ_DONE] * 5
+    finally:
+        srv.close()
+    assert np.array_equal(_bits(idle), _bits(busy))
