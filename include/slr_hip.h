@@ -240,6 +240,22 @@ SLRH_API int slrh_intersect(slrh_scene* s, const slrh_rays* rays, const slrh_hit
 SLRH_API int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* visibility, uint32_t n,
                            int mem, void* stream);
 
+/* slrh_intersect for device pointers that allocates nothing: no counterpart
+ * in the reference; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). rays and hits are device pointers; hits as slrh_intersect
+ * writes them. The traversal's counters and stack spill live in a static
+ * device array of 48 MiB that each GPU the library runs on holds until the
+ * process ends (looked up when a context is created on that device); its
+ * grid is at most 1024 blocks. The call is asynchronous on `stream` (a
+ * hipStream_t, NULL for the default stream) and reads nothing back: results
+ * are valid once the stream has completed. Calls that overlap on one GPU
+ * share that array and race: serialise them (one stream, or one at a time).
+ * Returns SLRH_ERR_CAPACITY for n > 2^30 and for a scene whose trees could
+ * need a traversal stack deeper than the reference's 64 entries. */
+SLRH_API int slrh_intersect_noalloc(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, uint32_t n,
+                                    void* stream);
+
 enum slrh_medium_query_mode {
     SLRH_MEDIUM_TRANSMITTANCE = 0,  /* one ratio-tracking estimate of the segment's transmittance */
     SLRH_MEDIUM_FREE_PATH = 1       /* one delta-tracking scattering distance, tmax if none */

@@ -36,7 +36,7 @@ EXPORTS = [
     "slrh_medium_density_grid", "slrh_medium_query", "slrh_camera_perspective",
     "slrh_scene_create", "slrh_scene_build", "slrh_scene_destroy",
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
-    "slrh_intersect", "slrh_occluded",
+    "slrh_intersect", "slrh_occluded", "slrh_intersect_noalloc",
     "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read",
     "slrh_environment", "slrh_render", "slrh_render_batch", "slrh_trace_variants",
     "slrh_server_create", "slrh_server_destroy", "slrh_submit", "slrh_poll", "slrh_wait", "slrh_cancel",
@@ -126,6 +126,7 @@ def load_library(path=None):
         "slrh_scene_qbvh_dump": (i32, [vp, vp, vp, vp, vp, vp]),
         "slrh_intersect": (i32, [vp, vp, vp, u32, i32, vp]),
         "slrh_occluded": (i32, [vp, vp, vp, u32, i32, vp]),
+        "slrh_intersect_noalloc": (i32, [vp, vp, vp, u32, vp]),
         "slrh_film_create": (i32, [vp, u32, u32, pp]),
         "slrh_film_destroy": (i32, [vp]),
         "slrh_film_clear": (i32, [vp]),
@@ -370,6 +371,15 @@ class Scene:
         h = _Hits(*(list(outs) + [None] * (5 - len(outs))))
         self.ctx.check(self.lib.slrh_intersect(self.h, ctypes.byref(r), ctypes.byref(h), n, MEM_DEVICE, stream),
                        "slrh_intersect")
+
+    def intersect_noalloc(self, cols, outs, n, stream=None):
+        """slrh_intersect_noalloc: the device-pointer closest hit that
+        allocates nothing (asynchronous on `stream`; synchronise before
+        reading the outputs). cols = 8 device addresses, outs = 4 or 5."""
+        r = _Rays(*cols)
+        h = _Hits(*(list(outs) + [None] * (5 - len(outs))))
+        self.ctx.check(self.lib.slrh_intersect_noalloc(self.h, ctypes.byref(r), ctypes.byref(h), n, stream),
+                       "slrh_intersect_noalloc")
 
     def occluded(self, rays):
         rays = _f32(rays).reshape(-1, 8)

@@ -627,6 +627,46 @@ int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* vis, uint32_t n, 
     return doTrace(s, rays, nullptr, vis, n, mem, stream);
 }
 
+// Deepest traversal stack the scene's trees can need: a QBVH node step
+// leaves at most three children pending per level (the nearest one is held
+// in a register), and entering an instance suspends at most six words (the
+// held node, three pending leaves, the rest of the leaf, the marker) before
+// the prototype's tree adds its own levels.
+static uint32_t stackBound(const SceneData& d) {
+    uint32_t b = 3 * d.accel.qbvhDepth;
+    if (!d.instances.empty()) {
+        uint32_t pd = 0;
+        for (const Prototype& p : d.protos) pd = std::max(pd, p.accel.qbvhDepth);
+        b += 6 + 3 * pd;
+    }
+    return b;
+}
+
+int slrh_intersect_noalloc(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, uint32_t n, void* stream) {
+    if (!s || !rays || !hits) return s ? fail(s->ctx, SLRH_ERR_INVALID, "null rays or hits") : SLRH_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "trace before slrh_scene_build");
+    if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
+    if (n > kNoAllocMaxRays)
+        return fail(s->ctx, SLRH_ERR_CAPACITY, "slrh_intersect_noalloc: more than 2^30 rays in one call");
+    if (n == 0) return SLRH_OK;
+    TraceArgs a{};
+    a.ox = rays->ox; a.oy = rays->oy; a.oz = rays->oz; a.dx = rays->dx; a.dy = rays->dy; a.dz = rays->dz;
+    a.tmin = rays->tmin; a.tmax = rays->tmax;
+    a.t = hits->t; a.prim = hits->prim; a.b0 = hits->b0; a.b1 = hits->b1; a.inst = hits->inst;
+    if (!a.ox || !a.oy || !a.oz || !a.dx || !a.dy || !a.dz || !a.tmin || !a.tmax)
+        return fail(s->ctx, SLRH_ERR_INVALID, "null ray array");
+    if (!a.t || !a.prim || !a.b0 || !a.b1) return fail(s->ctx, SLRH_ERR_INVALID, "null hit array");
+    if (stackBound(s->data) > 64)
+        return fail(s->ctx, SLRH_ERR_CAPACITY, "slrh_intersect_noalloc: the scene's trees may need a traversal "
+                                                "stack deeper than 64 entries; use slrh_intersect");
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    std::string err;
+    int r = deviceSelect(s->ctx->device, &err);
+    if (!r) r = traceBatchNoAlloc(s->dev, a, n, false, stream, &err);
+    if (r) return fail(s->ctx, r, err);
+    return SLRH_OK;
+}
+
 int slrh_medium_query(slrh_scene* s, const slrh_rays* rays, uint32_t n, int mode, uint32_t seed, float* out,
                       int mem, void* stream) {
     if (!s || !rays || !out) return SLRH_ERR_INVALID;

@@ -96,6 +96,7 @@ struct RenderStats {
 };
 
 int deviceInit(int device, std::string* err);
+int deviceSelect(int device, std::string* err);   // makes `device` current on the calling thread
 int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err);
 void freeScene(DeviceScene* ds);
 size_t sceneDeviceBytes(const DeviceScene* ds);
@@ -103,6 +104,13 @@ size_t sceneDeviceBytes(const DeviceScene* ds);
 // any == false: closest hit into t/prim/b0/b1; any == true: visibility into vis.
 int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool devicePtrs, void* stream,
                double* nodesVisited, double* trisTested, std::string* err);
+
+// The same on device pointers without allocating: counters and stack spill
+// come from a static device array, the launch is asynchronous on `stream`,
+// and n is at most kNoAllocMaxRays. The caller checks beforehand that the
+// scene's trees cannot overflow the traversal stack (nothing is read back).
+constexpr uint32_t kNoAllocMaxRays = 1u << 30;
+int traceBatchNoAlloc(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, void* stream, std::string* err);
 
 // mode 0: a transmittance estimate per ray, 1: a free-path distance per ray.
 int mediumQuery(DeviceScene* ds, const TraceArgs& a, uint32_t n, int mode, uint32_t seed, float* out,

@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -138,11 +139,44 @@ struct DeviceFilm {
     DevBuf<double> data;   // h*w*16
 };
 
+// ------------------------------------------------------------------ scratch of the allocation-free entry point
+//
+// traceBatchNoAlloc (slrh_intersect_noalloc / slrh_occluded_noalloc) takes
+// its device memory from this array instead of allocating: words
+// [0, kNoAllocCounterWords) hold the launch's counters (nodes, triangles,
+// overflow as 64-bit words, then the traversal's fetch cursor), the rest the
+// traversal stack spill of a grid of at most kNoAllocBlocks blocks (the
+// entries below the kLdsDepth that live in LDS, kMaxStack in all). Each device
+// holds its own copy, from module load to process end; every call on a
+// device shares it, so calls that overlap on one device race.
+constexpr uint32_t kNoAllocBlocks = 1024;
+constexpr size_t kNoAllocCounterWords = 64;
+constexpr size_t kNoAllocSpillWords = (size_t)(kMaxStack - kLdsDepth) * kNoAllocBlocks * kBlock;
+static __device__ __align__(256) uint32_t scratch[kNoAllocCounterWords + kNoAllocSpillWords];
+constexpr int kMaxDevices = 64;
+// device address of `scratch` per device (the host-side name is only a
+// stand-in), looked up once per device in deviceInit
+static uint32_t* g_scratchPtr[kMaxDevices] = {};
+static std::mutex g_scratchMu;
+
 int deviceInit(int device, std::string* err) {
     HIP_TRY(hipSetDevice(device));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     g_numCUs = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (device < kMaxDevices) {
+        std::lock_guard<std::mutex> lk(g_scratchMu);
+        if (!g_scratchPtr[device]) {
+            void* ptr = nullptr;
+            HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            g_scratchPtr[device] = (uint32_t*)ptr;
+        }
+    }
+    return 0;
+}
+
+int deviceSelect(int device, std::string* err) {
+    HIP_TRY(hipSetDevice(device));
     return 0;
 }
 
@@ -611,6 +645,40 @@ int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool d
     if (nodesVisited) *nodesVisited = (double)cnt[0];
     if (trisTested) *trisTested = (double)cnt[1];
     if (cnt[2]) { if (err) *err = "traversal stack overflow (depth > 64)"; return SLRH_ERR_CAPACITY_CODE; }
+    return 0;
+}
+
+// Fills n words with zero (the allocation-free path clears its counters with
+// a kernel, not a memset node, so that it can be captured in a graph).
+__global__ void k_zero_words(uint32_t* p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+int traceBatchNoAlloc(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, void* stream, std::string* err) {
+    if (n == 0) return 0;
+    if (n > kNoAllocMaxRays) {
+        if (err) *err = "allocation-free ray batch larger than 2^30 rays";
+        return SLRH_ERR_CAPACITY_CODE;
+    }
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    uint32_t* ptr = dev >= 0 && dev < kMaxDevices ? g_scratchPtr[dev] : nullptr;
+    if (!ptr) {
+        if (err) *err = "no scratch set up on the current device (create a context on it first)";
+        return SLRH_ERR_STATE_CODE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t blocks = std::min(persistentBlocks(n), kNoAllocBlocks);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, ptr, (uint32_t)kNoAllocCounterWords);
+    unsigned long long* counters = (unsigned long long*)ptr;
+    uint32_t* spill = ptr + kNoAllocCounterWords;
+    if (ds->view.numInstances)
+        hipLaunchKernelGGL(k_trace_batch<true>, dim3(blocks), dim3(kBlock), 0, s, ds->view, a, n, any ? 1 : 0, spill,
+                           blocks * kBlock, counters);
+    else
+        hipLaunchKernelGGL(k_trace_batch<false>, dim3(blocks), dim3(kBlock), 0, s, ds->view, a, n, any ? 1 : 0, spill,
+                           blocks * kBlock, counters);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 
