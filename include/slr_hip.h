@@ -320,12 +320,26 @@ enum slrh_film_format {
     SLRH_FILM_SRGB8 = 2          /* h*w*3 bytes: ImageSensor::saveImage tone curve with scale */
 };
 /* Copies the film out. scale multiplies the sums first (saveImage passes
- * brightness / spp, libSLR/Renderer/PTRenderer.cpp:84). */
+ * brightness / spp, libSLR/Renderer/PTRenderer.cpp:84). SLRH_FILM_SRGB8 is
+ * the pixel bytes ImageSensor::saveImage writes (libSLR/Core/ImageSensor.cpp:
+ * 139-195) for a sensor holding the film's sums rounded to FP32, with its FP32
+ * and double arithmetic, row 0 first, R, G, B. */
 SLRH_API int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size_t dst_bytes);
+/* Replaces the film's sums with w*h*16 doubles (pixel-major, 16 strata each):
+ * to resume accumulating onto saved sums, or to encode sums made elsewhere. */
+SLRH_API int slrh_film_load(slrh_film* f, const double* strata, size_t bytes);
+/* The BMP file ImageSensor::saveImage(path, scale) writes for the film
+ * (sensor sensitivity `sensitivity`, infinity counting as 1): saveBMP's
+ * 54-byte header and bottom-up BGR rows of 3 * width + width % 4 bytes
+ * (libSLR/Helper/bmp_exporter.cpp:15-54; the reference leaves the row padding
+ * uninitialised, here it is zero). *size receives the file size; dst may be
+ * NULL to query it. */
+SLRH_API int slrh_film_bmp(const slrh_film* f, float scale, float sensitivity, void* dst, size_t dst_bytes,
+                           size_t* size);
 
 /* Renders into `film` (accumulating onto what it holds). Replaces
  * PTRenderer::render (libSLR/Renderer/PTRenderer.cpp:26-101). stats, if not
- * NULL, receives 22 doubles: [0] rays traced, [1] samples, [2] device seconds,
+ * NULL, receives 25 doubles: [0] rays traced, [1] samples, [2] device seconds,
  * [3] non-finite samples dropped, [4] wavefront iterations, [5] traversal
  * kernel seconds (extension and shadow rays traced together), [6] traversal
  * launches, [7] extension rays, [8] shading kernel seconds, [9] seconds of
@@ -334,10 +348,28 @@ SLRH_API int slrh_film_read(const slrh_film* f, int format, double scale, void* 
  * [14] triangles by shadow rays, [15] traversal variant used, [16] scattering
  * events in the medium and [17] light samples attenuated by it (SLRH_VOL_PT),
  * [18] seconds of the medium tracking kernel, and its [19] density lookups
- * (tentative collisions), [20] majorant cells entered and [21] rays tracked.
+ * (tentative collisions), [20] majorant cells entered, [21] rays tracked
+ * and [22] tracking walks stopped by the step guard (2^16 steps; expected 0),
+ * [23] QBVH nodes read from the traversal's LDS copy of the top of the tree
+ * (included in [11] and [13]), [24] instances entered by the traversal.
  * Kernel seconds are 0 unless flags bit 0. */
 SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
                          double* stats);
+
+/* Progressive rendering: renders settings->spp samples per pixel into
+ * `film` in passes of 1, 1, 2, 4, ... samples (the last pass cut to spp) and
+ * calls snapshot(user, film, samples_done) after each pass, so the film
+ * holds 1, 2, 4, 8, ... samples per pixel at the calls, where
+ * PTRenderer::render saves its 000.bmp, 001.bmp, ... snapshots
+ * (libSLR/Renderer/PTRenderer.cpp:77-94; the host saves one with
+ * slrh_film_bmp(film, brightness / samples_done, ...)). A non-zero return
+ * from snapshot stops the render there. Pass k draws from seed +
+ * 0x9E3779B9 * k, so the final film is a valid render of spp samples, not
+ * the same bits as one slrh_render call. stats summed over the passes
+ * (stats[15], the traversal variant, averaged). snapshot may be NULL. */
+typedef int (*slrh_snapshot_fn)(void* user, const slrh_film* film, uint32_t samples_done);
+SLRH_API int slrh_render_progressive(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
+                                     slrh_snapshot_fn snapshot, void* user, double* stats);
 
 /* One frame of a batch: camera (as slrh_camera_perspective), resolution,
  * samples per pixel, seed and the film it accumulates into. */

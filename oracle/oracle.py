@@ -136,11 +136,11 @@ def reference_spectra(desc):
         elif k == "ggx_conductor":
             specs.append((360.0, 830.0, np.array([m["eta"]], np.float32)))
             specs.append((360.0, 830.0, np.array([m["k"]], np.float32)))
-            ent.update(s0=len(specs) - 2, s1=len(specs) - 1, p0=m["alpha"], p1=m["alpha"])
+            ent.update(s0=len(specs) - 2, s1=len(specs) - 1, p0=m["alpha"], p1=m.get("alpha_y", m["alpha"]))
         elif k == "ggx_dielectric":
             specs.append((360.0, 830.0, np.array([m["eta_ext"]], np.float32)))
             specs.append((360.0, 830.0, np.array([m["eta_int"]], np.float32)))
-            ent.update(s0=len(specs) - 2, s1=len(specs) - 1, p0=m["alpha"], p1=m["alpha"])
+            ent.update(s0=len(specs) - 2, s1=len(specs) - 1, p0=m["alpha"], p1=m.get("alpha_y", m["alpha"]))
         elif k == "emitter":
             base = m.get("base", -1)
             if base >= 0:

@@ -600,8 +600,8 @@ int refd_set_environment(void* sp, int w, int h, const float* rgb, float scale) 
 // PTRenderer::render over the scene; film_out receives h*w*16 floats (row-major
 // pixels, 16 strata each) of the raw sensor sums. Returns render seconds
 // (excluding Scene::build) or a negative value on error.
-double refd_render(void* sp, int w, int h, int spp, uint32_t seed, int threads, float* film_out,
-                   int quiet, double* build_seconds) {
+double refd_render_batches(void* sp, int w, int h, int spp, uint32_t seed, int batches, int threads,
+                           float* film_out, int quiet, double* build_seconds) {
     RefScene* s = (RefScene*)sp;
     if (!s->hasCamera || s->built) return -1.0;
     QuietStdout q(quiet != 0);
@@ -668,15 +668,30 @@ double refd_render(void* sp, int w, int h, int spp, uint32_t seed, int threads, 
     char* dir = mkdtemp(tmpl);
     if (dir && chdir(dir) != 0) dir = nullptr;
 
-    auto t2 = std::chrono::steady_clock::now();
-    if (s->integrator == 1) {
-        VolumetricPTRenderer renderer((uint32_t)spp);
-        renderer.render(scene, settings);
-    } else {
-        PTRenderer renderer((uint32_t)spp);
-        renderer.render(scene, settings);
+    // batch b: the renderer again with seed + b (PTRenderer::render starts
+    // with ImageSensor::init, so each batch's sensor sums start at zero)
+    double renderSeconds = 0;
+    const ImageSensor* sensor = scene.getCamera()->getSensor();
+    for (int b = 0; b < batches; ++b) {
+        settings.addItem(RenderSettingItem::RNGSeed, (int32_t)(seed + (uint32_t)b));
+        auto t2 = std::chrono::steady_clock::now();
+        if (s->integrator == 1) {
+            VolumetricPTRenderer renderer((uint32_t)spp);
+            renderer.render(scene, settings);
+        } else {
+            PTRenderer renderer((uint32_t)spp);
+            renderer.render(scene, settings);
+        }
+        auto t3 = std::chrono::steady_clock::now();
+        renderSeconds += std::chrono::duration<double>(t3 - t2).count();
+        float* out = film_out + (size_t)b * w * h * NumStrataForStorage;
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                DiscretizedSpectrum px = sensor->pixel((uint32_t)x, (uint32_t)y);
+                for (int k = 0; k < (int)NumStrataForStorage; ++k)
+                    out[((size_t)y * w + x) * NumStrataForStorage + k] = px[k];
+            }
     }
-    auto t3 = std::chrono::steady_clock::now();
 
     if (dir) {
         char cmd[256];
@@ -688,15 +703,30 @@ double refd_render(void* sp, int w, int h, int spp, uint32_t seed, int threads, 
         rmdir(dir);
     }
 
-    const ImageSensor* sensor = scene.getCamera()->getSensor();
-    for (int y = 0; y < h; ++y)
-        for (int x = 0; x < w; ++x) {
-            DiscretizedSpectrum px = sensor->pixel((uint32_t)x, (uint32_t)y);
-            for (int k = 0; k < (int)NumStrataForStorage; ++k)
-                film_out[((size_t)y * w + x) * NumStrataForStorage + k] = px[k];
-        }
     scene.destory();   // also releases the node tree's rendering data (Scene.cpp:40-46)
-    return std::chrono::duration<double>(t3 - t2).count();
+    return renderSeconds;
+}
+
+double refd_render(void* sp, int w, int h, int spp, uint32_t seed, int threads, float* film_out, int quiet,
+                   double* build_seconds) {
+    return refd_render_batches(sp, w, h, spp, seed, 1, threads, film_out, quiet, build_seconds);
+}
+
+// ImageSensor::saveImage (libSLR/Core/ImageSensor.cpp:139-195) of a sensor
+// holding the given sums: w*h pixels of 16 strata (row-major), written as a
+// BMP to `path` with the sensor's sensitivity and the scale PTRenderer passes
+// (brightness / samples). Returns 0, or -1 if the file could not be read back.
+int refd_save_image(int w, int h, const float* strata, float sensitivity, float scale, const char* path) {
+    ImageSensor sensor((uint32_t)w, (uint32_t)h, sensitivity);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x)
+            sensor.pixel((uint32_t)x, (uint32_t)y) =
+                SpectrumStorage(DiscretizedSpectrum(strata + ((size_t)y * w + x) * NumStrataForStorage));
+    sensor.saveImage(path, scale);
+    FILE* f = fopen(path, "rb");
+    if (!f) return -1;
+    fclose(f);
+    return 0;
 }
 
 // ---------------------------------------------------------------- probes

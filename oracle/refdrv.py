@@ -51,6 +51,8 @@ def lib():
             "refd_medium_density": (i32, [vp, i32, vp, vp]),
             "refd_medium_track": (i32, [vp, i32, i32, vp, u32, vp]),
             "refd_render": (f64, [vp, i32, i32, i32, u32, i32, vp, i32, vp]),
+            "refd_render_batches": (f64, [vp, i32, i32, i32, u32, i32, i32, vp, i32, vp]),
+            "refd_save_image": (i32, [i32, i32, vp, f32, f32, ctypes.c_char_p]),
             "refd_rng_stream": (None, [u32, i32, vp, vp]),
             "refd_wavelengths": (None, [f32, f32, vp, vp, vp]),
             "refd_upsample": (None, [i32, f32, f32, f32, i32, vp, vp]),
@@ -89,10 +91,11 @@ class RefScene:
             elif k == "oren_nayar":
                 ids.append(L.refd_add_material_diffuse_rgb(self.h, *m["rgb"], float(m["sigma"])))
             elif k == "ggx_conductor":
-                ids.append(L.refd_add_material_ggx_conductor(self.h, m["alpha"], m["alpha"], m["eta"], m["k"]))
+                ids.append(L.refd_add_material_ggx_conductor(self.h, m["alpha"], m.get("alpha_y", m["alpha"]), m["eta"],
+                                                             m["k"]))
             elif k == "ggx_dielectric":
-                ids.append(L.refd_add_material_ggx_dielectric(self.h, m["alpha"], m["alpha"], m["eta_ext"],
-                                                              m["eta_int"]))
+                ids.append(L.refd_add_material_ggx_dielectric(self.h, m["alpha"], m.get("alpha_y", m["alpha"]),
+                                                              m["eta_ext"], m["eta_int"]))
             elif k == "emitter":
                 base = ids[m["base"]] if m.get("base", -1) >= 0 else -1
                 ids.append(L.refd_add_material_emitter_rgb(self.h, base, *m["rgb"]))
@@ -216,6 +219,19 @@ class RefScene:
             raise RuntimeError("reference render failed")
         return film, sec, bs[0]
 
+    def render_batches(self, width, height, spp, seed, batches, threads=1, volumetric=False):
+        """`batches` renders on one scene build (seeds seed, seed + 1, ...);
+        returns (films (batches, h, w, 16) float32 sums, render seconds,
+        build seconds). Cannot be combined with build_accel or render."""
+        self.L.refd_set_integrator(self.h, 1 if volumetric else 0)
+        films = np.zeros((batches, height, width, 16), np.float32)
+        bs = np.zeros(1)
+        sec = self.L.refd_render_batches(self.h, width, height, spp, seed & 0xFFFFFFFF, batches, threads, _p(films), 1,
+                                         _p(bs))
+        if sec < 0:
+            raise RuntimeError("reference render failed")
+        return films, sec, bs[0]
+
     def close(self):
         if self.h:
             self.L.refd_scene_free(self.h)
@@ -226,6 +242,17 @@ class RefScene:
             self.close()
         except Exception:
             pass
+
+
+def save_image(strata, path, scale=1.0, sensitivity=1.0):
+    """ImageSensor::saveImage of a sensor holding `strata` ((h, w, 16) sums)
+    into a BMP file at path; returns the file's bytes."""
+    st = np.ascontiguousarray(strata, np.float32)
+    h, w = st.shape[:2]
+    if lib().refd_save_image(w, h, _p(st), float(sensitivity), float(scale), path.encode()) != 0:
+        raise RuntimeError("reference saveImage wrote nothing")
+    with open(path, "rb") as f:
+        return f.read()
 
 
 def rng_stream(seed, n):

@@ -37,7 +37,8 @@ EXPORTS = [
     "slrh_scene_create", "slrh_scene_build", "slrh_scene_destroy",
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
     "slrh_intersect", "slrh_occluded", "slrh_intersect_noalloc",
-    "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read",
+    "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read", "slrh_film_load", "slrh_film_bmp",
+    "slrh_render_progressive",
     "slrh_environment", "slrh_render", "slrh_render_batch", "slrh_trace_variants",
     "slrh_server_create", "slrh_server_destroy", "slrh_submit", "slrh_poll", "slrh_wait", "slrh_cancel",
     "slrh_server_stats",
@@ -131,6 +132,9 @@ def load_library(path=None):
         "slrh_film_destroy": (i32, [vp]),
         "slrh_film_clear": (i32, [vp]),
         "slrh_film_read": (i32, [vp, i32, f64, vp, ctypes.c_size_t]),
+        "slrh_film_load": (i32, [vp, vp, ctypes.c_size_t]),
+        "slrh_film_bmp": (i32, [vp, f32, f32, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+        "slrh_render_progressive": (i32, [vp, vp, vp, vp, vp, vp]),
         "slrh_render": (i32, [vp, vp, vp, vp]),
         "slrh_render_batch": (i32, [vp, vp, u32, vp, vp]),
         "slrh_environment": (i32, [vp, vp, u32, u32, f32]),
@@ -434,6 +438,21 @@ class Film:
                        "slrh_film_read")
         return out
 
+    def load(self, strata):
+        """Replaces the sums with (height, width, 16) float64 values."""
+        a = np.ascontiguousarray(strata, np.float64).reshape(self.height, self.width, 16)
+        self.ctx.check(self.lib.slrh_film_load(self.h, _ptr(a), a.nbytes), "slrh_film_load")
+
+    def bmp(self, scale=1.0, sensitivity=1.0):
+        """The BMP file bytes ImageSensor::saveImage would write for these sums."""
+        n = ctypes.c_size_t()
+        self.ctx.check(self.lib.slrh_film_bmp(self.h, float(scale), float(sensitivity), None, 0, ctypes.byref(n)),
+                       "slrh_film_bmp")
+        out = np.zeros(n.value, np.uint8)
+        self.ctx.check(self.lib.slrh_film_bmp(self.h, float(scale), float(sensitivity), _ptr(out), out.nbytes,
+                                              ctypes.byref(n)), "slrh_film_bmp")
+        return out.tobytes()
+
     def close(self):
         if self.h:
             self.lib.slrh_film_destroy(self.h)
@@ -465,10 +484,12 @@ def trace_variants():
     return list(zip(d[:n.value].tolist(), r[:n.value].tolist(), f[:n.value].tolist()))
 
 
+_SNAPSHOT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32)
+
 STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "trace_seconds", "trace_launches",
              "extend_rays", "shade_seconds", "other_seconds", "shadow_rays", "nodes_visited", "tris_tested",
              "nodes_shadow", "tris_shadow", "trace_variant", "medium_events", "medium_shadows", "medium_seconds",
-             "medium_lookups", "medium_cells", "medium_rays"]
+             "medium_lookups", "medium_cells", "medium_rays", "medium_guard", "nodes_lds", "inst_entered"]
 
 
 def _fill_frame(fr, f, default_spp):
@@ -503,6 +524,19 @@ class PTRenderer:
                       slots_per_pixel, flags, tune_trace, 0)
         st = np.zeros(len(STAT_KEYS))
         scene.ctx.check(scene.lib.slrh_render(scene.h, ctypes.byref(s), film.h, _ptr(st)), "slrh_render")
+        return dict(zip(STAT_KEYS, st.tolist()))
+
+    def render_progressive(self, scene, film, snapshot=None, seed=1509761209, max_depth=0, slots_per_pixel=0):
+        """slrh_render_progressive: passes of 1, 1, 2, 4, ... samples; snapshot(film, samples_done) is
+        called after each (a true return stops the render)."""
+        def cb(user, f, done):
+            return 1 if (snapshot is not None and snapshot(film, int(done))) else 0
+        fn = _SNAPSHOT_FN(cb)
+        s = _Settings(film.width, film.height, self.spp, max_depth, seed & 0xFFFFFFFF, self.integrator,
+                      slots_per_pixel, 0, 0, 0)
+        st = np.zeros(len(STAT_KEYS))
+        scene.ctx.check(scene.lib.slrh_render_progressive(scene.h, ctypes.byref(s), film.h, fn, None, _ptr(st)),
+                        "slrh_render_progressive")
         return dict(zip(STAT_KEYS, st.tolist()))
 
     def render_batch(self, scene, frames, max_depth=0, slots_per_pixel=0, time_kernels=False, tune_trace=0,
@@ -597,10 +631,11 @@ def build_scene(ctx, desc):
         elif kind == "oren_nayar":
             mid = sc.oren_nayar(sc.spectrum_rgb(m["rgb"]), m["sigma"])
         elif kind == "ggx_conductor":
-            mid = sc.ggx_conductor(sc.spectrum_constant(m["eta"]), sc.spectrum_constant(m["k"]), m["alpha"])
+            mid = sc.ggx_conductor(sc.spectrum_constant(m["eta"]), sc.spectrum_constant(m["k"]), m["alpha"],
+                                   m.get("alpha_y"))
         elif kind == "ggx_dielectric":
             mid = sc.ggx_dielectric(sc.spectrum_constant(m["eta_ext"]), sc.spectrum_constant(m["eta_int"]),
-                                    m["alpha"])
+                                    m["alpha"], m.get("alpha_y"))
         elif kind == "emitter":
             base = mat_ids[m["base"]] if m.get("base", -1) >= 0 else -1
             le = sc.spectrum_constant(m["const"]) if "const" in m else sc.spectrum_rgb(m["rgb"], light=True)

@@ -24,6 +24,21 @@ struct Spec16 {
     float v[kNumL];
 };
 
+// v[i] for a lane-varying i, as an OR of masked values: indexing a register
+// array with a runtime index would move the array to scratch memory. The
+// empty asm keeps each mask opaque, so the optimiser cannot fold the masks
+// back into an indexed load.
+__device__ __forceinline__ float pick16(const float* v, int i) {
+    uint32_t r = 0u;
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) {
+        uint32_t m = i == k ? 0xFFFFFFFFu : 0u;
+        asm volatile("" : "+v"(m));
+        r |= __float_as_uint(v[k]) & m;
+    }
+    return __uint_as_float(r);
+}
+
 __device__ __forceinline__ float importance16(const float* v, int hero) {
     // SampledSpectrum::importance (libSLR/BasicTypes/spectrum_types.h:473-486)
     float sum = 0;
@@ -31,7 +46,7 @@ __device__ __forceinline__ float importance16(const float* v, int hero) {
     for (int i = 0; i < kNumL; ++i) sum += v[i];
     const float primary = 0.9f;
     const float marginal = (1 - primary) / (kNumL - 1);
-    return sum * marginal + v[hero] * (primary - marginal);
+    return sum * marginal + pick16(v, hero) * (primary - marginal);
 }
 
 // concentricSampleDisk (distributions.cpp:57-90)
@@ -270,8 +285,9 @@ struct Dielectric {
         bool entering = q.dir.z >= 0.0f;
         float sign = entering ? 1.0f : -1.0f;
         const int h = q.hero;
-        float eEnterH = entering ? etaExt[h] : etaInt[h];
-        float eExitH = entering ? etaInt[h] : etaExt[h];
+        const float extH = pick16(etaExt, h), intH = pick16(etaInt, h);
+        float eEnterH = entering ? extH : intH;
+        float eExitH = entering ? intH : extH;
         float3 m;
         float mPdf;
         float d = D.sampleVisible(sign * q.dir, u0, u1, &m, &mPdf);
@@ -341,8 +357,9 @@ struct Dielectric {
                 fs[w] = fs[w] / fabsf(nvnl) * (eEn * eEn) * sn;
             }
             const int h = q.hero;
-            float eEnterH = entering ? etaExt[h] : etaInt[h];
-            float eExitH = entering ? etaInt[h] : etaExt[h];
+            const float extH = pick16(etaExt, h), intH = pick16(etaInt, h);
+            float eEnterH = entering ? extH : intH;
+            float eExitH = entering ? intH : extH;
             float3 m = normalize3(-(eEnterH * q.dir + eExitH * wi));
             float dotHV = dot3(q.dir, m);
             if (dotHV * sign <= 0) return;

@@ -717,6 +717,42 @@ int slrh_film_clear(slrh_film* f) {
     return r ? fail(f->ctx, r, err) : SLRH_OK;
 }
 
+namespace {
+// One pixel through ImageSensor::saveImage's arithmetic
+// (libSLR/Core/ImageSensor.cpp:139-195) with the reference's types: the
+// stratum sums as FP32 (DiscretizedSpectrum), times scale in FP32, XYZ by the
+// CMF strata accumulated in FP32 in stratum order and divided by their
+// integral (spectrum_types.h:666-690), XYZ_to_sRGB with its double constants
+// (spectrum_base.h:68-72), clamp at zero, the 1 - exp(-x) tone curve in FP32,
+// sRGB_gamma (spectrum_base.cpp:16-21, double constants, FP32 result) and
+// the byte min(255, uint32(256 * gamma)). out: R, G, B.
+void saveImagePixel(const double* st, float scale, const float* xb, const float* yb, const float* zb, float integ,
+                    uint8_t out[3]) {
+    float v[16];
+    for (int k = 0; k < 16; ++k) v[k] = (float)st[k] * scale;
+    float XYZ[3] = {0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < 16; ++k) {
+        XYZ[0] += xb[k] * v[k];
+        XYZ[1] += yb[k] * v[k];
+        XYZ[2] += zb[k] * v[k];
+    }
+    XYZ[0] /= integ;
+    XYZ[1] /= integ;
+    XYZ[2] /= integ;
+    float RGB[3];
+    RGB[0] = (float)(3.2404542 * XYZ[0] - 1.5371385 * XYZ[1] - 0.4985314 * XYZ[2]);
+    RGB[1] = (float)(-0.9692660 * XYZ[0] + 1.8760108 * XYZ[1] + 0.0415560 * XYZ[2]);
+    RGB[2] = (float)(0.0556434 * XYZ[0] - 0.2040259 * XYZ[1] + 1.0572252 * XYZ[2]);
+    for (int c = 0; c < 3; ++c) {
+        float x = RGB[c] < 0.0f ? 0.0f : RGB[c];
+        x = 1.0f - std::exp(-x);
+        const float g = x <= 0.0031308 ? (float)(12.92 * x) : (float)(1.055 * std::pow((double)x, 1.0 / 2.4) - 0.055);
+        const float q = 256 * g;
+        out[c] = (uint8_t)std::min<uint32_t>(255u, (uint32_t)q);
+    }
+}
+}  // namespace
+
 int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size_t bytes) {
     if (!f || !dst) return SLRH_ERR_INVALID;
     const size_t px = (size_t)f->w * f->h;
@@ -736,32 +772,78 @@ int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size
     float xb[16], yb[16], zb[16], integ;
     cmfStrata(xb, yb, zb, &integ);
     for (size_t p = 0; p < px; ++p) {
-        // DiscretizedSpectrum::getRGB's XYZ step (libSLR/BasicTypes/spectrum_types.h:666-690)
-        double X = 0, Y = 0, Z = 0;
-        for (int k = 0; k < 16; ++k) {
-            double v = st[16 * p + k] * scale;
-            X += xb[k] * v; Y += yb[k] * v; Z += zb[k] * v;
-        }
-        X /= integ; Y /= integ; Z /= integ;
         if (format == SLRH_FILM_XYZ_F64) {
-            double* o = (double*)dst + 3 * p;
-            o[0] = X; o[1] = Y; o[2] = Z;
-        } else {
-            // XYZ -> sRGB, clamp, 1 - exp(-x) tone curve, gamma (ImageSensor::saveImage,
-            // libSLR/Core/ImageSensor.cpp:165-189)
-            double rgb[3] = {3.2404542 * X - 1.5371385 * Y - 0.4985314 * Z,
-                             -0.9692660 * X + 1.8760108 * Y + 0.0415560 * Z,
-                             0.0556434 * X - 0.2040259 * Y + 1.0572252 * Z};
-            uint8_t* o = (uint8_t*)dst + 3 * p;
-            for (int k = 0; k < 3; ++k) {
-                double v = rgb[k] < 0 ? 0 : rgb[k];
-                v = 1.0 - std::exp(-v);
-                v = v <= 0.0031308 ? 12.92 * v : 1.055 * std::pow(v, 1.0 / 2.4) - 0.055;
-                double q = 256.0 * v;
-                o[k] = (uint8_t)std::min(255.0, std::max(0.0, q));
+            // DiscretizedSpectrum::getRGB's XYZ step (libSLR/BasicTypes/spectrum_types.h:666-690), in FP64
+            double X = 0, Y = 0, Z = 0;
+            for (int k = 0; k < 16; ++k) {
+                double v = st[16 * p + k] * scale;
+                X += xb[k] * v; Y += yb[k] * v; Z += zb[k] * v;
             }
+            double* o = (double*)dst + 3 * p;
+            o[0] = X / integ; o[1] = Y / integ; o[2] = Z / integ;
+        } else {
+            saveImagePixel(&st[16 * p], (float)scale, xb, yb, zb, integ, (uint8_t*)dst + 3 * p);
         }
     }
+    return SLRH_OK;
+}
+
+int slrh_film_load(slrh_film* f, const double* strata, size_t bytes) {
+    if (!f || !strata) return SLRH_ERR_INVALID;
+    if (bytes < (size_t)f->w * f->h * 16 * sizeof(double))
+        return fail(f->ctx, SLRH_ERR_INVALID, "film_load: source smaller than the film");
+    std::string err;
+    int r = filmUpload(f->dev, strata, &err);
+    if (r) return fail(f->ctx, r, err);
+    return SLRH_OK;
+}
+
+int slrh_film_bmp(const slrh_film* f, float scale, float sensitivity, void* dst, size_t dst_bytes, size_t* size) {
+    if (!f || !size) return SLRH_ERR_INVALID;
+    // saveBMP's layout (libSLR/Helper/bmp_exporter.cpp:15-54): 54-byte header,
+    // rows bottom-up, BGR, each row 3 * width + width % 4 bytes
+    const uint32_t w = f->w, h = f->h;
+    const uint32_t byteWidth = 3 * w + w % 4;
+    const size_t total = 54 + (size_t)byteWidth * h;
+    *size = total;
+    if (!dst) return SLRH_OK;
+    if (dst_bytes < total) return fail(f->ctx, SLRH_ERR_INVALID, "film_bmp: destination too small");
+    std::vector<double> st;
+    std::string err;
+    int r = filmDownload(f->dev, &st, &err);
+    if (r) return fail(f->ctx, r, err);
+    uint8_t* o = (uint8_t*)dst;
+    memset(o, 0, total);
+    const uint32_t fileSize = (uint32_t)total, dataOffset = 54, infoSize = 40, compress = 0, ppm = 1;
+    const uint32_t dataSize = byteWidth * h;
+    const uint16_t planes = 1, bits = 24;
+    o[0] = 'B';
+    o[1] = 'M';
+    memcpy(o + 2, &fileSize, 4);
+    memcpy(o + 10, &dataOffset, 4);
+    memcpy(o + 14, &infoSize, 4);
+    memcpy(o + 18, &w, 4);
+    memcpy(o + 22, &h, 4);
+    memcpy(o + 26, &planes, 2);
+    memcpy(o + 28, &bits, 2);
+    memcpy(o + 30, &compress, 4);
+    memcpy(o + 34, &dataSize, 4);
+    memcpy(o + 38, &ppm, 4);
+    memcpy(o + 42, &ppm, 4);
+    // ImageSensor::saveImage: an infinite sensitivity counts as 1
+    const float sens = std::isinf(sensitivity) ? 1.0f : sensitivity;
+    const float sc = scale * sens;
+    float xb[16], yb[16], zb[16], integ;
+    cmfStrata(xb, yb, zb, &integ);
+    for (uint32_t i = 0; i < h; ++i)
+        for (uint32_t j = 0; j < w; ++j) {
+            uint8_t rgb[3];
+            saveImagePixel(&st[16 * ((size_t)i * w + j)], sc, xb, yb, zb, integ, rgb);
+            uint8_t* px = o + 54 + (size_t)(h - i - 1) * byteWidth + 3 * j;
+            px[0] = rgb[2];
+            px[1] = rgb[1];
+            px[2] = rgb[0];
+        }
     return SLRH_OK;
 }
 
@@ -791,6 +873,9 @@ void copyStats(const RenderStats& st, double* stats) {
     stats[16] = st.mediumEvents; stats[17] = st.mediumShadows;
     stats[18] = st.mediumSeconds; stats[19] = st.mediumLookups; stats[20] = st.mediumCells;
     stats[21] = st.mediumRays;
+    stats[22] = st.mediumGuard;
+    stats[23] = st.nodesLds;
+    stats[24] = st.instEntered;
 }
 }  // namespace
 
@@ -816,6 +901,32 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     int r = slrh::render(s->dev, cfg, film->dev, &st, &err);
     if (r) return fail(s->ctx, r, err);
     copyStats(st, stats);
+    return SLRH_OK;
+}
+
+int slrh_render_progressive(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
+                            slrh_snapshot_fn snapshot, void* user, double* stats) {
+    if (!s || !set || !film) return SLRH_ERR_INVALID;
+    if (set->spp == 0) return SLRH_OK;
+    // passes of 1, 1, 2, 4, ... samples per pixel (the last cut to spp), so
+    // the film holds 2^k samples per pixel after pass k, where
+    // PTRenderer::render saves its snapshots (PTRenderer.cpp:77-94)
+    double sum[25] = {0};
+    uint32_t done = 0, pass = 0;
+    while (done < set->spp) {
+        slrh_render_settings ps = *set;
+        ps.spp = std::min(set->spp - done, std::max(1u, done));
+        ps.seed = set->seed + 0x9E3779B9u * pass;
+        double st[25] = {0};
+        int r = slrh_render(s, &ps, film, st);
+        if (r) return r;
+        for (int k = 0; k < 25; ++k) sum[k] += st[k];
+        done += ps.spp;
+        ++pass;
+        if (snapshot && snapshot(user, film, done) != 0) break;
+    }
+    sum[15] /= pass;   // the traversal variant, not a sum
+    if (stats) memcpy(stats, sum, sizeof(sum));
     return SLRH_OK;
 }
 

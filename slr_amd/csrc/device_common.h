@@ -123,7 +123,11 @@ struct DevScene {
     float envScale;
     uint32_t hasEnv;
     // heterogeneous medium (volumetric integrator)
-    const float* medDensity;    // lattice densities, x fastest, then y, then z
+    // lattice densities corner-packed: for lattice point (x, y, z), x fastest,
+    // two float4 holding the densities at the corners (x|x+1, y|y+1, z|z+1),
+    // each index clamped to the lattice, in the order z, y, x (x fastest), so
+    // a trilinear lookup is one 32-byte read
+    const float4* medCorners;
     const float* medMajorant;   // maximum density per supervoxel (kMedSuperVoxel^3 lattice cells)
     float medMin[3], medMax[3]; // region
     uint32_t medN[3], medCells[3];

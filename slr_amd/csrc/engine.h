@@ -93,6 +93,9 @@ struct RenderStats {
     double mediumEvents = 0, mediumShadows = 0;   // scattering events, light samples through the medium
     double mediumSeconds = 0;                     // k_medium time (with kernel timing)
     double mediumLookups = 0, mediumCells = 0, mediumRays = 0;   // k_medium work: density lookups, majorant cells, entries
+    double mediumGuard = 0;                       // tracking walks stopped by the step guard
+    double nodesLds = 0;                          // QBVH nodes read from the LDS copy of the tree top
+    double instEntered = 0;                       // instances entered by the traversal
 };
 
 int deviceInit(int device, std::string* err);
@@ -120,6 +123,7 @@ int filmCreate(uint32_t w, uint32_t h, DeviceFilm** out, std::string* err);
 void filmFree(DeviceFilm* f);
 int filmClear(DeviceFilm* f, std::string* err);
 int filmDownload(const DeviceFilm* f, std::vector<double>* strata, std::string* err);
+int filmUpload(DeviceFilm* f, const double* strata, std::string* err);   // w*h*16 doubles, replacing the sums
 void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h);
 
 int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err);

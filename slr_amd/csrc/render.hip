@@ -122,7 +122,8 @@ struct DeviceScene {
     DevBuf<float4> envTexel;
     DevBuf<float2> mengBasis;
     DevBuf<float> envCondPdf, envCondCdf, envMargPdf, envMargCdf;
-    DevBuf<float> medDensity, medMajorant;
+    DevBuf<float> medMajorant;
+    DevBuf<float4> medCorners;
     DevBuf<unsigned long long> counters;
     DevScene view;
     bool hasCamera = false;
@@ -218,6 +219,22 @@ static void cameraDerived(const CameraParams& c, uint32_t w, uint32_t h, DevCame
 }
 
 static int buildLightRecords(DeviceScene* ds, std::string* err);
+
+// Corner-packed lattice (see DevScene::medCorners): entry i = (x, y, z)
+// holds g at (x|ux, y|uy, z|uz) with u = min(l + 1, n - 1), the eight values
+// calcDensity weights for a point whose lower lattice indices are (x, y, z).
+__global__ void k_corner_pack(const float* g, uint32_t nx, uint32_t ny, uint32_t nz, float4* out) {
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(i % nx), y = (uint32_t)((i / nx) % ny), z = (uint32_t)(i / ((size_t)nx * ny));
+        const uint32_t ux = min(x + 1, nx - 1), uy = min(y + 1, ny - 1), uz = min(z + 1, nz - 1);
+        const size_t zl = (size_t)z * ny * nx, zu = (size_t)uz * ny * nx;
+        out[2 * i] = make_float4(g[zl + (size_t)nx * y + x], g[zl + (size_t)nx * y + ux], g[zl + (size_t)nx * uy + x],
+                                 g[zl + (size_t)nx * uy + ux]);
+        out[2 * i + 1] = make_float4(g[zu + (size_t)nx * y + x], g[zu + (size_t)nx * y + ux],
+                                     g[zu + (size_t)nx * uy + x], g[zu + (size_t)nx * uy + ux]);
+    }
+}
 
 int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     std::unique_ptr<DeviceScene> ds(new DeviceScene());
@@ -454,14 +471,27 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         ve.envMapH = e.mapH;
         ve.envScale = e.scale;
     }
-    ve.medDensity = ve.medMajorant = nullptr;
+    ve.medCorners = nullptr;
+    ve.medMajorant = nullptr;
     ve.hasMedium = 0;
     ve.volume = 0;
     if (sd.medium.present) {
         static_assert(kMedSuperVoxel == kSuperVoxel, "supervoxel size");
         const MediumData& m = sd.medium;
-        if ((r = ds->medDensity.upload(m.density, err)) || (r = ds->medMajorant.upload(m.majorant, err))) return r;
-        ve.medDensity = ds->medDensity.p;
+        if ((r = ds->medMajorant.upload(m.majorant, err))) return r;
+        {
+            // the lattice goes up as given and is turned into the
+            // corner-packed form on the device; the plain copy is dropped
+            DevBuf<float> lattice;
+            if ((r = lattice.upload(m.density, err))) return r;
+            const size_t np = (size_t)m.n[0] * m.n[1] * m.n[2];
+            if ((r = ds->medCorners.alloc(2 * np, err))) return r;
+            hipLaunchKernelGGL(k_corner_pack, dim3(2048), dim3(256), 0, 0, lattice.p, m.n[0], m.n[1], m.n[2],
+                               ds->medCorners.p);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipDeviceSynchronize());
+        }
+        ve.medCorners = ds->medCorners.p;
         ve.medMajorant = ds->medMajorant.p;
         for (int a = 0; a < 3; ++a) {
             ve.medMin[a] = m.bmin[a];
@@ -705,6 +735,10 @@ int filmDownload(const DeviceFilm* f, std::vector<double>* strata, std::string* 
     HIP_TRY(hipMemcpy(strata->data(), f->data.p, strata->size() * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
+int filmUpload(DeviceFilm* f, const double* strata, std::string* err) {
+    HIP_TRY(hipMemcpy(f->data.p, strata, (size_t)f->w * f->h * kNumL * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
 void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h) { *w = f->w; *h = f->h; }
 
 // ------------------------------------------------------------------ path state
@@ -727,7 +761,10 @@ enum Stat64 : uint32_t {
     S_MED_LOOKUPS = 9,    // tentative collisions (density lookups) in k_medium
     S_MED_CELLS = 10,     // majorant cells entered in k_medium
     S_MED_RAYS = 11,      // trace entries k_medium tracked through the medium
-    kNumStat = 12
+    S_MED_GUARD = 12,     // tracking walks stopped by the kMaxTrackSteps guard
+    S_NODES_LDS = 13,     // QBVH nodes the traversal read from its LDS copy of the tree top
+    S_INSTS = 14,         // instances entered by the traversal
+    kNumStat = 15
 };
 // Per-slot work code, one byte per slot, rewritten by each stage and turned
 // into slot-ordered queues by k_compact. After shading a slot holds EXT or
@@ -742,36 +779,35 @@ constexpr uint32_t F_HAS_SAMPLE = 1u;    // slot holds a sample whose radiance m
 constexpr uint32_t F_LAST_DELTA = 2u;    // last BSDF sample was a delta lobe
 constexpr uint32_t F_RAD = 4u;           // SF_SP holds the path's radiance so far (zero when clear)
 
-// Per-slot path record: 24 float4 (384 B, three 128-B cache lines), slot s
-// at rec[24 * s]. Each kernel works on a sparse, slot-ordered subset of the
-// slots, so a slot's fields share lines instead of being spread over
-// per-field arrays (where a sparse subset would fetch a whole line per
-// 16-B field):
-//   line 0: what the traversal reads and writes: ray (ro, rd, rinv),
-//           shadow ray (so, sd, sinv), hit
-//   line 1: throughput alpha (16 wavelengths), radiance sp
-//   line 2: pending light-sample contribution nee, info, misc, rng
-// The producers of a ray (regeneration, shading) also store its reciprocal
-// direction and sign bits (rayInv), so the traversal loop starts a ray
-// without the three correctly rounded divisions.
+// Per-slot path record, 20 float4 (320 B) in three arrays, each entry
+// aligned to its size so a slot's fields share cache lines (each kernel works
+// on a sparse, slot-ordered subset of the slots; per-field arrays would fetch
+// a whole line per 16-B field):
+//   ray  (128 B, one line): everything the traversal and the medium walk
+//        read and write: ray, hit, shadow direction, path info, misc, RNG,
+//        job words
+//   spec (128 B, one line): throughput alpha and radiance so far (16
+//        wavelengths each)
+//   nee  (64 B): the pending light-sample contribution
+// A shadow ray starts where the extension ray does (the shading point, tmin
+// 1e-4), so only its direction and length are stored; the traversal forms
+// both rays' reciprocal directions itself.
 enum SlotField : uint32_t {
-    SF_RO = 0,      // ray origin.xyz + tmin
+    SF_RO = 0,      // ray origin.xyz + tmin (also the shadow ray's)
     SF_RD = 1,      // ray direction.xyz + tmax (infinite)
-    SF_RINV = 2,    // rayInv(direction)
-    SF_SO = 3,      // shadow ray origin.xyz + tmin
-    SF_SD = 4,      // shadow ray direction.xyz + tmax
-    SF_SINV = 5,    // rayInv(shadow direction)
-    SF_HIT = 6,     // t, prim, b0, b1
-    SF_HITX = 7,    // instance of the hit (x; scenes with instances only)
-    SF_ALPHA = 8,   // 4 float4
-    SF_SP = 12,     // 4 float4
-    SF_NEE = 16,    // 4 float4
-    SF_INFO = 20,   // pixel, flags | hero << 8 | pathLength << 16, samples started, samples total
-    SF_MISC = 21,   // wl offset, initY, last BSDF pdf, camera weight
-    SF_RNG = 22,    // XORShift state
-    SF_JOB = 23,    // job index, film plane of the slot, seed of its light-sample transmittance streams
-    kSlotVec = 24
+    SF_HIT = 2,     // t, prim, b0, b1
+    SF_SD = 3,      // shadow ray direction.xyz + tmax
+    SF_INFO = 4,    // pixel, flags | hero << 8 | pathLength << 16, samples started, samples total
+    SF_MISC = 5,    // wl offset, initY, last BSDF pdf, camera weight
+    SF_RNG = 6,     // XORShift state
+    SF_JOB = 7,     // job index, film plane of the slot, seed of its light-sample transmittance
+                    // streams, instance of the last hit (scenes with instances)
+    SF_ALPHA = 8,   // 4 float4 (spec array)
+    SF_SP = 12,     // 4 float4 (spec array)
+    SF_NEE = 16,    // 4 float4 (nee array)
 };
+constexpr uint32_t kRayVec = 8, kSpecVec = 8, kNeeVec = 4;
+constexpr uint32_t kSlotRecordBytes = 16 * (kRayVec + kSpecVec + kNeeVec);
 
 // One render job (a frame or a served request) in the slot pool: its camera
 // (resolution included), its film planes and its path-length cap. Jobs own
@@ -788,7 +824,9 @@ static_assert(kCompactSlots == kPoolAlign && kMaxJobs == kPoolMaxJobs, "pool con
 
 struct PathState {
     uint32_t n;          // slots
-    float4* rec;         // n slot records
+    float4* ray;         // n x kRayVec: SF_RO .. SF_JOB
+    float4* spec;        // n x kSpecVec: SF_ALPHA, SF_SP
+    float4* nee;         // n x kNeeVec: SF_NEE
     const DevJob* jobs;  // kMaxJobs entries
     uint32_t* jobActive; // per job: slots not idle, counted by the post-shade compaction
     const uint32_t* blockJob;   // job owning compaction block b (slots 2048 b ..), kNoJob if none
@@ -796,12 +834,16 @@ struct PathState {
     uint8_t* vis;        // 1: the slot's shadow ray was traced and is unoccluded
     uint32_t* q;         // queues: done (n entries), trace (2n), cls0..3 (n each)
     uint32_t* ctr;
-    uint32_t* fetch;     // k_trace's entry cursors, kFetchStride apart (one per queue part)
+    uint32_t* fetch;     // entry cursors, kFetchStride apart: k_trace's (one per queue part), then k_medium's
     unsigned long long* stat;
     uint32_t* spill;
     uint32_t spillStride;
-    __device__ float4& f4(uint32_t s, uint32_t f) const { return rec[(size_t)s * kSlotVec + f]; }
-    __device__ uint4& u4(uint32_t s, uint32_t f) const { return *(uint4*)&rec[(size_t)s * kSlotVec + f]; }
+    // f is a compile-time constant at every use, so the array choice folds away
+    __device__ __forceinline__ float4& f4(uint32_t s, uint32_t f) const {
+        return f < SF_ALPHA ? ray[(size_t)s * kRayVec + f]
+                            : f < SF_NEE ? spec[(size_t)s * kSpecVec + (f - SF_ALPHA)] : nee[(size_t)s * kNeeVec + (f - SF_NEE)];
+    }
+    __device__ __forceinline__ uint4& u4(uint32_t s, uint32_t f) const { return *(uint4*)&f4(s, f); }
 };
 
 // queue offsets in units of n entries
@@ -1031,18 +1073,19 @@ __device__ __forceinline__ float mediumDensity(const DevScene& sc, float3 p) {
     // outside the region (and any NaN coordinate) reads zero
     if (!(px >= 0 && py >= 0 && pz >= 0 && px <= 1 && py <= 1 && pz <= 1)) return 0.0f;
     const uint32_t nx = sc.medN[0], ny = sc.medN[1], nz = sc.medN[2];
-    const uint32_t lx = min((uint32_t)(px * (nx - 1)), nx - 1), ux = min(lx + 1, nx - 1);
-    const uint32_t ly = min((uint32_t)(py * (ny - 1)), ny - 1), uy = min(ly + 1, ny - 1);
-    const uint32_t lz = min((uint32_t)(pz * (nz - 1)), nz - 1), uz = min(lz + 1, nz - 1);
+    // lower lattice indices; the upper ones, min(l + 1, n - 1), are folded
+    // into the corner-packed entry of (lx, ly, lz)
+    const uint32_t lx = min((uint32_t)(px * (nx - 1)), nx - 1);
+    const uint32_t ly = min((uint32_t)(py * (ny - 1)), ny - 1);
+    const uint32_t lz = min((uint32_t)(pz * (nz - 1)), nz - 1);
     const float wux = px * (nx - 1) - lx, wlx = 1 - wux;
     const float wuy = py * (ny - 1) - ly, wly = 1 - wuy;
     const float wuz = pz * (nz - 1) - lz, wlz = 1 - wuz;
-    const float* g = sc.medDensity;
-    const size_t zl = (size_t)lz * ny * nx, zu = (size_t)uz * ny * nx;
-    return g[zl + (size_t)nx * ly + lx] * wlz * wly * wlx + g[zl + (size_t)nx * ly + ux] * wlz * wly * wux +
-           g[zl + (size_t)nx * uy + lx] * wlz * wuy * wlx + g[zl + (size_t)nx * uy + ux] * wlz * wuy * wux +
-           g[zu + (size_t)nx * ly + lx] * wuz * wly * wlx + g[zu + (size_t)nx * ly + ux] * wuz * wly * wux +
-           g[zu + (size_t)nx * uy + lx] * wuz * wuy * wlx + g[zu + (size_t)nx * uy + ux] * wuz * wuy * wux;
+    // the eight corners in one 32-byte read (corner-packed lattice)
+    const float4* c = sc.medCorners + 2 * (((size_t)lz * ny + ly) * nx + lx);
+    const float4 a = c[0], b = c[1];
+    return a.x * wlz * wly * wlx + a.y * wlz * wly * wux + a.z * wlz * wuy * wlx + a.w * wlz * wuy * wux +
+           b.x * wuz * wly * wlx + b.y * wuz * wly * wux + b.z * wuz * wuy * wlx + b.w * wuz * wuy * wux;
 }
 
 // Clips [t0, t1] of the ray to the medium region; false if nothing is left.
@@ -1061,77 +1104,131 @@ __device__ __forceinline__ bool clipToMedium(const DevScene& sc, float3 o, float
 
 // Upper bound on the number of supervoxel steps plus tentative collisions
 // one tracking walk takes before it gives up (a guard; optical depths in the
-// configurations stay far below it).
-constexpr int kMaxTrackSteps = 1 << 16;
+// configurations stay far below it; walks that hit it are counted).
+constexpr uint32_t kMaxTrackSteps = 1u << 16;
 
-// Walks the supervoxels that the part of [t0, t1] inside the medium crosses
-// (3D DDA in lattice space, cells of kMedSuperVoxel lattice intervals) and
-// draws tentative collisions against the piecewise-constant majorant
-// mu = sigma_e * max density of the cell: exponential free paths whose
-// optical depth carries across cells, as the reference's supervoxel
-// traversal does with its trilinear majorant (DensityGridMediumDistribution.cpp:
-// 208-367). collide(t) is called at each tentative collision with the
-// majorant there and returns true to stop the walk (the walk then returns
-// true); false: the walk reached t1.
 // Work counters of the tracking walks (majorant cells entered, density lookups).
 struct MedCount {
     uint32_t cells = 0, lookups = 0;
+    uint32_t guard = 0;   // walks stopped by kMaxTrackSteps (reported; expected 0)
 };
 
-template <typename Collide>
-__device__ __forceinline__ bool trackMedium(const DevScene& sc, float3 o, float3 d, float t0, float t1, uint4& rng,
-                                            Collide&& collide, MedCount* mc = nullptr) {
-    if (!clipToMedium(sc, o, d, &t0, &t1)) return false;
-    const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
-    int cell[3], step[3];
-    float tNext[3], tDelta[3];
+// A walk over the supervoxels that the part of [t0, t1] inside the medium
+// crosses (3D DDA in lattice space, cells of kMedSuperVoxel lattice
+// intervals), drawing tentative collisions against the piecewise-constant
+// majorant mu = sigma_e * max density of the cell: exponential free paths
+// whose optical depth carries across cells, as the reference's supervoxel
+// traversal does with its trilinear majorant (DensityGridMediumDistribution.cpp:
+// 208-367). One advance() is one step: a tentative collision in the current
+// cell, or the move into the next cell.
+struct MediumWalk {
+    enum : uint32_t { kCollide = 0, kMoved = 1, kDone = 2, kGuard = 3 };
+    float t, t1, tau;
+    // per axis: supervoxel, step direction, next boundary and spacing in t
+    // (named per axis, not arrays: a lane-varying axis index into an array
+    // would move the walk state to scratch memory)
+    int cx, cy, cz, sx, sy, sz;
+    float nx, ny, nz, dx, dy, dz;
+    uint32_t steps;
+
+    // Clips [t0, t1] to the medium, sets up the DDA and draws the first free
+    // path; false if the segment misses the medium (no draw then).
+    __device__ __forceinline__ bool init(const DevScene& sc, float3 o, float3 d, float t0, float t1In, uint4& rng) {
+        t1 = t1In;
+        if (!clipToMedium(sc, o, d, &t0, &t1)) return false;
+        const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
+        int cell[3], step[3];
+        float tNext[3], tDelta[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float scale = (float)(sc.medN[a] - 1) / (sc.medMax[a] - sc.medMin[a]);   // lattice units per length
-        const float l0 = (oa[a] + da[a] * t0 - sc.medMin[a]) * scale;
-        int c = (int)floorf(l0 / kMedSuperVoxel);
-        c = max(0, min((int)sc.medCells[a] - 1, c));
-        cell[a] = c;
-        const float dl = da[a] * scale;
-        if (dl > 0.0f) {
-            step[a] = 1;
-            tNext[a] = t0 + ((float)((c + 1) * (int)kMedSuperVoxel) - l0) / dl;
-            tDelta[a] = kMedSuperVoxel / dl;
-        } else if (dl < 0.0f) {
-            step[a] = -1;
-            tNext[a] = t0 + ((float)(c * (int)kMedSuperVoxel) - l0) / dl;
-            tDelta[a] = -(float)kMedSuperVoxel / dl;
-        } else {
-            step[a] = 0;
-            tNext[a] = INFINITY;
-            tDelta[a] = INFINITY;
+        for (int a = 0; a < 3; ++a) {
+            const float scale = (float)(sc.medN[a] - 1) / (sc.medMax[a] - sc.medMin[a]);   // lattice units per length
+            const float l0 = (oa[a] + da[a] * t0 - sc.medMin[a]) * scale;
+            int c = (int)floorf(l0 / kMedSuperVoxel);
+            c = max(0, min((int)sc.medCells[a] - 1, c));
+            cell[a] = c;
+            const float dl = da[a] * scale;
+            if (dl > 0.0f) {
+                step[a] = 1;
+                tNext[a] = t0 + ((float)((c + 1) * (int)kMedSuperVoxel) - l0) / dl;
+                tDelta[a] = kMedSuperVoxel / dl;
+            } else if (dl < 0.0f) {
+                step[a] = -1;
+                tNext[a] = t0 + ((float)(c * (int)kMedSuperVoxel) - l0) / dl;
+                tDelta[a] = -(float)kMedSuperVoxel / dl;
+            } else {
+                step[a] = 0;
+                tNext[a] = INFINITY;
+                tDelta[a] = INFINITY;
+            }
         }
+        cx = cell[0]; cy = cell[1]; cz = cell[2];
+        sx = step[0]; sy = step[1]; sz = step[2];
+        nx = tNext[0]; ny = tNext[1]; nz = tNext[2];
+        dx = tDelta[0]; dy = tDelta[1]; dz = tDelta[2];
+        t = t0;
+        tau = -logf(1.0f - xsFloat(rng));
+        steps = 0;
+        return true;
     }
-    float t = t0;
-    float tau = -logf(1.0f - xsFloat(rng));
-    if (mc) ++mc->cells;
-    for (int it = 0; it < kMaxTrackSteps; ++it) {
-        const float mu =
-            sc.medSigmaE * sc.medMajorant[((size_t)cell[2] * sc.medCells[1] + cell[1]) * sc.medCells[0] + cell[0]];
-        const float tExit = fminf(fminf(tNext[0], tNext[1]), fminf(tNext[2], t1));
+    __device__ __forceinline__ uint32_t cellIndex(const DevScene& sc) const {
+        return ((uint32_t)cz * sc.medCells[1] + (uint32_t)cy) * sc.medCells[0] + (uint32_t)cx;
+    }
+    // One step with the current cell's majorant mu. kCollide: t is a
+    // tentative collision (the caller decides, and redraws tau to go on);
+    // kMoved: the walk entered the next cell (its majorant is needed next);
+    // kDone: the walk left [t0, t1] or the lattice; kGuard: kMaxTrackSteps.
+    __device__ __forceinline__ uint32_t advance(const DevScene& sc, float mu) {
+        if (++steps > kMaxTrackSteps) return kGuard;
+        const float tExit = fminf(fminf(nx, ny), fminf(nz, t1));
         const float span = fmaxf(tExit - t, 0.0f);
         if (mu > 0.0f && tau < mu * span) {
             t += tau / mu;
-            if (mc) ++mc->lookups;
-            if (collide(t, mu)) return true;
-            tau = -logf(1.0f - xsFloat(rng));
-            continue;
+            return kCollide;
         }
         if (mu > 0.0f) tau -= mu * span;
         t = fmaxf(t, tExit);
-        if (t >= t1) return false;
-        const int a = tNext[0] < tNext[1] ? (tNext[0] < tNext[2] ? 0 : 2) : (tNext[1] < tNext[2] ? 1 : 2);
-        cell[a] += step[a];
-        if (cell[a] < 0 || cell[a] >= (int)sc.medCells[a]) return false;
-        tNext[a] += tDelta[a];
-        if (mc) ++mc->cells;
+        if (t >= t1) return kDone;
+        if (nx < ny ? nx < nz : false) {
+            cx += sx;
+            if (cx < 0 || cx >= (int)sc.medCells[0]) return kDone;
+            nx += dx;
+        } else if (nx < ny ? false : ny < nz) {
+            cy += sy;
+            if (cy < 0 || cy >= (int)sc.medCells[1]) return kDone;
+            ny += dy;
+        } else {
+            cz += sz;
+            if (cz < 0 || cz >= (int)sc.medCells[2]) return kDone;
+            nz += dz;
+        }
+        return kMoved;
     }
-    return false;
+};
+
+// The whole walk: collide(t, mu) is called at each tentative collision and
+// returns true to stop the walk (the walk then returns true); false: the
+// walk reached t1 (or the step guard, counted in mc).
+template <typename Collide>
+__device__ __forceinline__ bool trackMedium(const DevScene& sc, float3 o, float3 d, float t0, float t1, uint4& rng,
+                                            Collide&& collide, MedCount* mc = nullptr) {
+    MediumWalk w;
+    if (!w.init(sc, o, d, t0, t1, rng)) return false;
+    if (mc) ++mc->cells;
+    float mu = sc.medSigmaE * sc.medMajorant[w.cellIndex(sc)];
+    for (;;) {
+        const uint32_t r = w.advance(sc, mu);
+        if (r == MediumWalk::kCollide) {
+            if (mc) ++mc->lookups;
+            if (collide(w.t, mu)) return true;
+            w.tau = -logf(1.0f - xsFloat(rng));
+        } else if (r == MediumWalk::kMoved) {
+            if (mc) ++mc->cells;
+            mu = sc.medSigmaE * sc.medMajorant[w.cellIndex(sc)];
+        } else {
+            if (r == MediumWalk::kGuard && mc) ++mc->guard;
+            return false;
+        }
+    }
 }
 
 // Free-path sampling for the hero wavelength by delta tracking (the
@@ -1350,7 +1447,6 @@ __device__ __forceinline__ void startCameraSample(const DevScene& sc, const Path
     const float initY = importance16(one, (int)hero);
     ps.f4(s, SF_RO) = make_float4(org.x, org.y, org.z, 0.0f);
     ps.f4(s, SF_RD) = make_float4(dir.x, dir.y, dir.z, INFINITY);
-    ps.f4(s, SF_RINV) = rayInv(dir);
     misc = make_float4(off, initY, 0.0f, weight);
     info.y = F_HAS_SAMPLE | (hero << 8);   // pathLength 0
     info.z += 1;
@@ -1439,15 +1535,14 @@ void k_trace(DevScene sc, PathState ps) {
     st.treeletNodes = L < sc.numNodes ? (uint32_t)L : sc.numNodes;
     st.treelet = loadTreelet(treelet, sc.nodes, st.treeletNodes);
     TraceCounters tc;
-    traceLoop<T, F, true, I, SLRH_FETCH_PARTS>(sc, st, count, ps.fetch, [&](uint32_t i) { return qin[i]; },
+    traceLoop<T, F, false, I, SLRH_FETCH_PARTS>(sc, st, count, ps.fetch, [&](uint32_t i) { return qin[i]; },
         [&](uint32_t e, RayIn& r) {
             const uint32_t s = e & ~kShadowEntry;
             const bool shadow = (e & kShadowEntry) != 0;
-            // the ray's three consecutive float4: origin + tmin, direction + tmax, rayInv
-            const float4* rp = &ps.f4(s, shadow ? SF_SO : SF_RO);
-            r.o = rp[0];
-            r.d = rp[1];
-            r.inv = rp[2];
+            // origin + tmin (shared by both rays), then the ray's direction + tmax;
+            // the reciprocal direction is formed when the ray starts
+            r.o = ps.f4(s, SF_RO);
+            r.d = ps.f4(s, shadow ? SF_SD : SF_RD);
             return shadow;
         },
         [&](uint32_t e, bool shadow, const HitOut& h) {
@@ -1456,7 +1551,7 @@ void k_trace(DevScene sc, PathState ps) {
                 if (h.prim < 0) ps.vis[s] = 1;
             } else {
                 ps.f4(s, SF_HIT) = make_float4(h.t, __int_as_float(h.prim), h.b0, h.b1);
-                if (I) *(int32_t*)&ps.f4(s, SF_HITX) = h.inst;
+                if (I) ps.u4(s, SF_JOB).w = (uint32_t)h.inst;
                 ps.code[s] = (uint8_t)(CODE_CLS0 + (h.prim >= 0 ? h.aux : (uint32_t)CLS_MISS));
             }
         }, tc);
@@ -1466,6 +1561,8 @@ void k_trace(DevScene sc, PathState ps) {
     statAdd(ps.stat, S_RAYS_SHADOW, tc.raysAny);
     statAdd(ps.stat, S_NODES_SH, tc.nodesAny);
     statAdd(ps.stat, S_TRIS_SH, tc.trisAny);
+    statAdd(ps.stat, S_NODES_LDS, tc.nodesLds);
+    if (I) statAdd(ps.stat, S_INSTS, tc.insts);
     if (tc.overflow) atomicOr(&ps.ctr[C_OVERFLOW], 1u);
 }
 
@@ -1757,7 +1854,7 @@ void k_shade(DevScene sc, PathState ps) {
                 const float4 o4 = ps.f4(s, SF_RO), d4 = ps.f4(s, SF_RD), h = ps.f4(s, SF_HIT);
                 const float3 ro = xyz(o4), rd = xyz(d4);
                 const uint32_t prim = (uint32_t)__float_as_int(h.y);
-                const int32_t hinst = sc.numInstances ? *(const int32_t*)&ps.f4(s, SF_HITX) : -1;
+                const int32_t hinst = sc.numInstances ? (int32_t)ps.u4(s, SF_JOB).w : -1;
                 const SurfPt sp = surfaceAt(sc, prim, h.z, h.w, ro, rd, h.x, hinst);
                 const DevMaterial mat = sc.mats[sp.mat];
                 const int hero = (int)((info.y >> 8) & 0xFFu);
@@ -1796,6 +1893,13 @@ void k_shade(DevScene sc, PathState ps) {
 #pragma unroll
                     for (int k = 0; k < kNumL; ++k) rad[k] += alpha[k] * (Le[k] * edf) * misW;
                 }
+                // the radiance so far is complete for this vertex: it goes to the
+                // record now (and is read back if the path ends here), so its 16
+                // values are not held in registers through light and BSDF sampling
+                if (adds) {
+                    storePlanes(ps, s, SF_SP, rad);
+                    info.y |= F_RAD;
+                }
                 bool alive = true;
                 // Russian roulette (PTRenderer.cpp:267-273), from the second vertex on
                 if (pathLen > 0) {
@@ -1815,9 +1919,13 @@ void k_shade(DevScene sc, PathState ps) {
                     q.dir = dirOut;
                     q.ng = gL;
                     q.hero = hero;
-                    float A[kNumL], B[kNumL];
-                    evalSpectrum(sc, mat.spec0, misc.x, A);
-                    if (CLS != CLS_DIFFUSE) evalSpectrum(sc, mat.spec1, misc.x, B);
+                    // the material's spectra (reflectance, or the two indices of
+                    // refraction) are evaluated where each block needs them, so
+                    // they are not live across both blocks
+                    auto materialSpectra = [&](float* A, float* B) {
+                        evalSpectrum(sc, mat.spec0, misc.x, A);
+                        if (CLS != CLS_DIFFUSE) evalSpectrum(sc, mat.spec1, misc.x, B);
+                    };
                     // ---- next event estimation (PTRenderer.cpp:179-218)
                     const float uSel = xsFloat(rng);
                     const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
@@ -1836,9 +1944,8 @@ void k_shade(DevScene sc, PathState ps) {
 #pragma unroll
                         for (int k = 0; k < kNumL; ++k) c[k] = __builtin_nanf("");
                         storePlanes(ps, s, SF_NEE, c);
-                        ps.f4(s, SF_SO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
+                        ps.f4(s, SF_RO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
                         ps.f4(s, SF_SD) = make_float4(dir.x, dir.y, dir.z, tmax);
-                        ps.f4(s, SF_SINV) = rayInv(dir);
                         shadow = true;
                     };
                     if (sc.hasEnv) {
@@ -1866,6 +1973,8 @@ void k_shade(DevScene sc, PathState ps) {
                             envRadiance(sc, (float)(phi / (2 * kPiD)), (float)(theta / kPiD), misc.x, Le);
                             const float3 sdL = toLocal(sp, sdir);
                             float bpdf = 0.0f;
+                            float A[kNumL], B[kNumL];
+                            materialSpectra(A, B);
                             evalBsdf<CLS>(mat, A, B, q, sdL, fs, &bpdf);
                             const float lightPDF = envSelectProb(sc) * areaPDF;
                             const float bsdfPDF = bpdf * cosL;
@@ -1882,9 +1991,8 @@ void k_shade(DevScene sc, PathState ps) {
                             if (any) {
                                 // Scene::testVisibility toward infinity: [Epsilon, FLT_MAX]
                                 storePlanes(ps, s, SF_NEE, c);
-                                ps.f4(s, SF_SO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
+                                ps.f4(s, SF_RO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
                                 ps.f4(s, SF_SD) = make_float4(sdir.x, sdir.y, sdir.z, FLT_MAX);
-                                ps.f4(s, SF_SINV) = rayInv(sdir);
                                 shadow = true;
                             }
                         }
@@ -1910,6 +2018,8 @@ void k_shade(DevScene sc, PathState ps) {
                             evalSpectrum(sc, lm.emit, misc.x, M);
                             const float3 sdL = toLocal(sp, sdir);
                             float bpdf = 0.0f;
+                            float A[kNumL], B[kNumL];
+                            materialSpectra(A, B);
                             evalBsdf<CLS>(mat, A, B, q, sdL, fs, &bpdf);
                             const float lightPDF = lprob * lightSelectScale(sc) * areaPDF;
                             const float bsdfPDF = bpdf * cosL / dist2;
@@ -1929,9 +2039,8 @@ void k_shade(DevScene sc, PathState ps) {
                                 const float dist = length3(lp - sp.p);
                                 const float3 vdir = divs(lp - sp.p, dist);
                                 storePlanes(ps, s, SF_NEE, c);
-                                ps.f4(s, SF_SO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
+                                ps.f4(s, SF_RO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
                                 ps.f4(s, SF_SD) = make_float4(vdir.x, vdir.y, vdir.z, dist * (1 - 1e-4f));
-                                ps.f4(s, SF_SINV) = rayInv(vdir);
                                 shadow = true;
                             }
                         }
@@ -1941,6 +2050,8 @@ void k_shade(DevScene sc, PathState ps) {
                         const float uc = xsFloat(rng), u0 = xsFloat(rng), u1 = xsFloat(rng);
                         float fs[kNumL];
                         BsdfSample bs;
+                        float A[kNumL], B[kNumL];
+                        materialSpectra(A, B);
                         bool ok = sampleBsdf<CLS>(mat, A, B, q, uc, u0, u1, fs, &bs);
                         bool nonzero = false;
 #pragma unroll
@@ -1952,7 +2063,6 @@ void k_shade(DevScene sc, PathState ps) {
                             const float3 nd = fromLocal(sp, bs.dir);
                             ps.f4(s, SF_RO) = make_float4(sp.p.x, sp.p.y, sp.p.z, 1e-4f);
                             ps.f4(s, SF_RD) = make_float4(nd.x, nd.y, nd.z, INFINITY);
-                            ps.f4(s, SF_RINV) = rayInv(nd);
                             misc.z = bs.pdf;
                             info.y &= ~F_LAST_DELTA;
                             cont = true;
@@ -1963,19 +2073,20 @@ void k_shade(DevScene sc, PathState ps) {
                 if (cont || shadow) {
                     // the path goes on, or it ended with a light sample still to
                     // trace: its sample is filed once the shadow ray is resolved
-                    if (adds) {
-                        storePlanes(ps, s, SF_SP, rad);
-                        info.y |= F_RAD;
-                    }
                     if (cont) storePlanes(ps, s, SF_ALPHA, alpha);
                     ps.u4(s, SF_INFO) = info;
                     ps.f4(s, SF_MISC) = misc;
                     ps.u4(s, SF_RNG) = rng;
                     code = (uint8_t)((cont ? CODE_EXT : CODE_DONE) | (shadow ? CODE_SHADOW : 0));
                 } else {
-                    // nothing added here: the sample's radiance is the stored sum
-                    if (!adds && (info.y & F_RAD)) loadPlanes(ps, s, SF_SP, rad);
-                    code = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
+                    // the sample's radiance is the stored sum
+                    float fin[kNumL];
+                    if (info.y & F_RAD) loadPlanes(ps, s, SF_SP, fin);
+                    else {
+#pragma unroll
+                        for (int k = 0; k < kNumL; ++k) fin[k] = 0.0f;
+                    }
+                    code = retireAndRestart(sc, ps, s, info, misc, rng, fin, samples);
                 }
             }
         }
@@ -1994,59 +2105,197 @@ void k_shade(DevScene sc, PathState ps) {
 //   MODE 1 (after regeneration): trace queue, the slot's extension entry
 //          and then its shadow entry; clears the shadow bit.
 //   MODE 2 (after tracing): one queue per shading class.
+// Hands out queue entries to the lanes of a wave that ask for one: chunks of
+// kFeedChunk entries per atomic on one of kFeedParts cursors (kFetchStride
+// words apart; block b starts on part b % kFeedParts and moves on when its
+// part runs dry), given out in lane order. Wave-uniform state; claim() must
+// be reached by every lane of the wave.
+constexpr uint32_t kFeedParts = SLRH_FETCH_PARTS;
+constexpr uint32_t kFeedChunk = 128;
+struct WaveFeeder {
+    uint32_t next, end, part, partsLeft;
+    bool exhausted;
+    __device__ __forceinline__ void init() {
+        next = end = 0;
+        part = blockIdx.x % kFeedParts;
+        partsLeft = kFeedParts;
+        exhausted = false;
+    }
+    __device__ __forceinline__ bool claim(uint32_t* cursors, uint32_t count, bool want, uint32_t* idx) {
+        const unsigned long long lack = __ballot(want);
+        if (exhausted || lack == 0ull) return false;
+        if (next >= end) {
+            for (;;) {
+                const uint32_t pb = (uint32_t)((uint64_t)count * part / kFeedParts);
+                const uint32_t pe = (uint32_t)((uint64_t)count * (part + 1) / kFeedParts);
+                uint32_t base = 0;
+                if (__lane_id() == 0) base = atomicAdd(cursors + part * kFetchStride, kFeedChunk);
+                base = __shfl(base, 0);
+                if (base < pe - pb) {
+                    next = pb + base;
+                    end = pe - next > kFeedChunk ? next + kFeedChunk : pe;
+                    break;
+                }
+                if (--partsLeft == 0) {
+                    exhausted = true;
+                    return false;
+                }
+                part = part + 1 < kFeedParts ? part + 1 : 0;
+            }
+        }
+        const uint32_t avail = end - next;
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(lack >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)lack, 0u));
+        const uint32_t need = (uint32_t)__popcll(lack);
+        const bool got = want && rank < avail;
+        if (got) *idx = next + rank;
+        next += need < avail ? need : avail;
+        return got;
+    }
+};
+
 // Volumetric integrator, after the traversal: the extension rays' free
 // paths through the medium (Scene::interact, libSLR/Scene/Scene.cpp:66-94:
 // the medium over [tmin, surface hit]) by delta tracking, and the light
 // samples' transmittance (Scene::testVisibility, :114-135) by ratio tracking,
 // which scales the stored contribution before it is resolved. Each trace
-// queue entry has its own lane: the extension ray advances the slot's
+// queue entry is its own walk: the extension ray advances the slot's
 // stream, the light sample draws from a stream hashed for it (shadowStream),
-// so the two long tracking loops of a slot run side by side.
+// so a slot's two walks run side by side.
+//
+// Persistent: a lane advances its walk by one step per loop iteration (one
+// tentative collision with its density lookup, or the move into the next
+// supervoxel, whose majorant is requested then and used in the next
+// iteration) and starts its next queue entry, claimed ahead of time, as soon
+// as the walk ends, so a wave's lanes stay busy while walk lengths differ.
+// Stages: 0 idle, 1 the entry's first words requested, 2 a visible light
+// sample's ray requested, 3 walking.
 __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_TRACE];
     const uint32_t* qin = queue(ps, Q_TRACE);
+    uint32_t* cursors = ps.fetch + kFeedParts * kFetchStride;   // zeroed by the trace compaction
+    WaveFeeder feed;
+    feed.init();
     unsigned long long events = 0, shadows = 0, rays = 0;
     MedCount mc;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
-        const uint32_t e = qin[i];
-        const uint32_t s = e & ~kShadowEntry;
-        if (e & kShadowEntry) {
-            // the light sample (an occluded one needs no transmittance), on a
-            // stream of its own so that it runs beside the slot's extension ray;
-            // everything the entry needs is loaded at once, before the tests
-            const uint8_t vis = ps.vis[s];
-            const float4 o4 = ps.f4(s, SF_SO), d4 = ps.f4(s, SF_SD);
-            const uint32_t slotSeed = ps.u4(s, SF_JOB).z;
-            const uint4 info = ps.u4(s, SF_INFO);
-            if (vis == 0) continue;
-            const float3 o = xyz(o4), d = xyz(d4);
-            float t0 = o4.w, t1 = d4.w;
-            if (!clipToMedium(sc, o, d, &t0, &t1)) continue;
-            ++shadows;
-            ++rays;
-            uint4 rng = shadowStream(slotSeed, info);
-            const float T = mediumTransmittance(sc, o, d, o4.w, d4.w, rng, &mc);
-            if (T != 1.0f) {
-                float c[kNumL];
-                loadPlanes(ps, s, SF_NEE, c);
+    uint32_t stage = 0, s = 0, entNext = 0;
+    bool hasNext = false, shadow = false;
+    uint8_t vis = 0;
+    float4 o4 = make_float4(0, 0, 0, 0), d4 = o4, h = o4;
+    uint4 rng = make_uint4(0, 0, 0, 0), info = rng;
+    uint32_t slotSeed = 0;
+    float T = 1.0f, mu = 0.0f;
+    MediumWalk w;
+    for (;;) {
+        // lanes without a next entry claim one once 16 of them lack one, or
+        // at once for an idle lane (every lane of the wave reaches the claim)
+        uint32_t idx = 0;
+        const unsigned long long lackM = __ballot(!hasNext);
+        const bool ask = !hasNext && (stage == 0 || __popcll(lackM) >= 16);
+        if (feed.claim(cursors, count, ask, &idx)) {
+            entNext = qin[idx];
+            hasNext = true;
+        }
+        // an idle lane starts its next entry; its words are used from the
+        // next iteration on
+        bool started = false;
+        if (stage == 0 && hasNext) {
+            hasNext = false;
+            s = entNext & ~kShadowEntry;
+            shadow = (entNext & kShadowEntry) != 0;
+            if (shadow) {
+                vis = ps.vis[s];   // an occluded light sample needs no transmittance
+            } else {
+                o4 = ps.f4(s, SF_RO);
+                d4 = ps.f4(s, SF_RD);
+                h = ps.f4(s, SF_HIT);
+                rng = ps.u4(s, SF_RNG);
+            }
+            stage = 1;
+            started = true;
+        }
+        // wave-uniform exit: every lane idle, nothing claimed, queue drained
+        if (__ballot(stage != 0 || hasNext) == 0ull) {
+            if (feed.exhausted) break;
+            continue;
+        }
+        if (started) {
+        } else if (stage == 1) {
+            if (shadow) {
+                if (vis == 0) {
+                    stage = 0;
+                } else {
+                    o4 = ps.f4(s, SF_RO);
+                    d4 = ps.f4(s, SF_SD);
+                    slotSeed = ps.u4(s, SF_JOB).z;
+                    info = ps.u4(s, SF_INFO);
+                    stage = 2;
+                }
+            } else if (w.init(sc, xyz(o4), xyz(d4), o4.w, h.x, rng)) {
+                ++rays;
+                ++mc.cells;
+                mu = sc.medSigmaE * sc.medMajorant[w.cellIndex(sc)];
+                stage = 3;
+            } else {
+                stage = 0;   // misses the medium: nothing drawn, nothing written
+            }
+        } else if (stage == 2) {
+            rng = shadowStream(slotSeed, info);
+            T = 1.0f;
+            if (w.init(sc, xyz(o4), xyz(d4), o4.w, d4.w, rng)) {
+                ++rays;
+                ++shadows;
+                ++mc.cells;
+                mu = sc.medSigmaE * sc.medMajorant[w.cellIndex(sc)];
+                stage = 3;
+            } else {
+                stage = 0;
+            }
+        } else if (stage == 3) {
+            const uint32_t r = w.advance(sc, mu);
+            bool done = false;
+            if (r == MediumWalk::kCollide) {
+                ++mc.lookups;
+                const float sigma = sc.medSigmaE * mediumDensity(sc, xyz(o4) + xyz(d4) * w.t);
+                if (!shadow) {
+                    // delta tracking (sampleFreePath)
+                    if (xsFloat(rng) < sigma / mu) {
+                        ps.f4(s, SF_HIT) = make_float4(w.t, __int_as_float(-2), 0.0f, 0.0f);
+                        ps.code[s] = (uint8_t)(CODE_CLS0 + CLS_MEDIUM);
+                        ++events;
+                        done = true;
+                    }
+                } else {
+                    // ratio tracking with Russian roulette (mediumTransmittance)
+                    T *= 1.0f - fminf(sigma / mu, 1.0f);
+                    if (T < 0.1f) {
+                        if (xsFloat(rng) < T) T = 1.0f;
+                        else {
+                            T = 0.0f;
+                            done = true;
+                        }
+                    }
+                }
+                if (!done) w.tau = -logf(1.0f - xsFloat(rng));
+            } else if (r == MediumWalk::kMoved) {
+                ++mc.cells;
+                mu = sc.medSigmaE * sc.medMajorant[w.cellIndex(sc)];
+            } else {
+                if (r == MediumWalk::kGuard) ++mc.guard;
+                done = true;
+            }
+            if (done) {
+                if (!shadow) {
+                    ps.u4(s, SF_RNG) = rng;
+                } else if (T != 1.0f) {
+                    float c[kNumL];
+                    loadPlanes(ps, s, SF_NEE, c);
 #pragma unroll
-                for (int k = 0; k < kNumL; ++k) c[k] *= T;
-                storePlanes(ps, s, SF_NEE, c);
+                    for (int k = 0; k < kNumL; ++k) c[k] *= T;
+                    storePlanes(ps, s, SF_NEE, c);
+                }
+                stage = 0;
             }
-        } else {
-            const float4 o4 = ps.f4(s, SF_RO), d4 = ps.f4(s, SF_RD), h = ps.f4(s, SF_HIT);
-            uint4 rng = ps.u4(s, SF_RNG);
-            const float3 o = xyz(o4), d = xyz(d4);
-            float t0 = o4.w, t1 = h.x;
-            if (!clipToMedium(sc, o, d, &t0, &t1)) continue;
-            float ts;
-            ++rays;
-            if (sampleFreePath(sc, o, d, o4.w, h.x, rng, &ts, &mc)) {
-                ps.f4(s, SF_HIT) = make_float4(ts, __int_as_float(-2), 0.0f, 0.0f);
-                ps.code[s] = (uint8_t)(CODE_CLS0 + CLS_MEDIUM);
-                ++events;
-            }
-            ps.u4(s, SF_RNG) = rng;
         }
     }
     statAdd(ps.stat, S_MED_EVENTS, events);
@@ -2054,6 +2303,7 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
     statAdd(ps.stat, S_MED_LOOKUPS, mc.lookups);
     statAdd(ps.stat, S_MED_CELLS, mc.cells);
     statAdd(ps.stat, S_MED_RAYS, rays);
+    statAdd(ps.stat, S_MED_GUARD, mc.guard);
 }
 
 // Shading at a scattering point in the medium (VolumetricPTRenderer::Job::
@@ -2083,7 +2333,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
         const float3 pos = xyz(o4) + rd * h.x;
         const int hero = (int)((info.y >> 8) & 0xFFu);
         uint32_t pathLen = info.y >> 16;
-        float alpha[kNumL], rad[kNumL], albedo[kNumL];
+        float alpha[kNumL], albedo[kNumL];
         if (pathLen == 0) {
 #pragma unroll
             for (int k = 0; k < kNumL; ++k) alpha[k] = 1.0f;
@@ -2091,7 +2341,13 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
             loadPlanes(ps, s, SF_ALPHA, alpha);
         }
         const bool adds = ps.vis[s] != 0;
-        loadRadiance(ps, s, info.y, adds, rad);
+        if (adds) {
+            // the resolved light sample joins the stored radiance now
+            float rad[kNumL];
+            loadRadiance(ps, s, info.y, true, rad);
+            storePlanes(ps, s, SF_SP, rad);
+            info.y |= F_RAD;
+        }
         bool alive = true;
         if (pathLen > 0) {
             const float p = fminf(importance16(alpha, hero) / misc.y, 1.0f);
@@ -2175,9 +2431,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
                 }
                 if (any) {
                     storePlanes(ps, s, SF_NEE, c);
-                    ps.f4(s, SF_SO) = make_float4(pos.x, pos.y, pos.z, 1e-4f);
+                    ps.f4(s, SF_RO) = make_float4(pos.x, pos.y, pos.z, 1e-4f);
                     ps.f4(s, SF_SD) = make_float4(sdir.x, sdir.y, sdir.z, tmax);
-                    ps.f4(s, SF_SINV) = rayInv(sdir);
                     shadow = true;
                 }
             }
@@ -2198,7 +2453,6 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
                 if (nonzero && value > 0.0f) {
                     ps.f4(s, SF_RO) = make_float4(pos.x, pos.y, pos.z, 1e-4f);
                     ps.f4(s, SF_RD) = make_float4(nd.x, nd.y, nd.z, INFINITY);
-                    ps.f4(s, SF_RINV) = rayInv(nd);
                     misc.z = value;
                     info.y &= ~F_LAST_DELTA;
                     cont = true;
@@ -2207,18 +2461,19 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
         }
         info.y = (info.y & 0xFFFFu) | (pathLen << 16);
         if (cont || shadow) {
-            if (adds) {
-                storePlanes(ps, s, SF_SP, rad);
-                info.y |= F_RAD;
-            }
             if (cont) storePlanes(ps, s, SF_ALPHA, alpha);
             ps.u4(s, SF_INFO) = info;
             ps.f4(s, SF_MISC) = misc;
             ps.u4(s, SF_RNG) = rng;
             code = (uint8_t)((cont ? CODE_EXT : CODE_DONE) | (shadow ? CODE_SHADOW : 0));
         } else {
-            if (!adds && (info.y & F_RAD)) loadPlanes(ps, s, SF_SP, rad);
-            code = retireAndRestart(sc, ps, s, info, misc, rng, rad, samples);
+            float fin[kNumL];
+            if (info.y & F_RAD) loadPlanes(ps, s, SF_SP, fin);
+            else {
+#pragma unroll
+                for (int k = 0; k < kNumL; ++k) fin[k] = 0.0f;
+            }
+            code = retireAndRestart(sc, ps, s, info, misc, rng, fin, samples);
         }
         ps.code[s] = code;
     }
@@ -2299,7 +2554,7 @@ __global__ __launch_bounds__(256) void k_compact(PathState ps, uint32_t groups) 
     __shared__ uint32_t blockBase[NQ];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     // the traversal that consumes this compaction's queue starts its cursor at 0
-    if (MODE == 1 && blockIdx.x == 0 && threadIdx.x < SLRH_FETCH_PARTS) ps.fetch[threadIdx.x * kFetchStride] = 0;
+    if (MODE == 1 && blockIdx.x == 0 && threadIdx.x < 2 * SLRH_FETCH_PARTS) ps.fetch[threadIdx.x * kFetchStride] = 0;
     // entries slot s contributes to queue q
     auto entries = [](int q, uint8_t v) -> uint32_t {
         const bool ext = (v & 0x7F) == CODE_EXT, sh = (v & CODE_SHADOW) != 0;
@@ -2437,7 +2692,7 @@ struct SlotPool {
     // once (more would wait for a second round and leave a tail)
     uint32_t gridShade[kNumClasses] = {}, gridRegen = 0, gridMedium = 0;
     int device = 0;
-    DevBuf<float4> rec;
+    DevBuf<float4> ray, spec, nee;
     DevBuf<uint32_t> q, ctr, fetch, spill, jobActive, blockJob;
     DevBuf<uint8_t> code, vis;
     DevBuf<unsigned long long> stat;
@@ -2476,8 +2731,9 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
     std::unique_ptr<SlotPool> p(new SlotPool());
     const uint32_t blocks = (uint32_t)g_numCUs * 8;
     int r;
-    if ((r = p->rec.alloc((size_t)kSlotVec * capacity, err)) || (r = p->q.alloc((size_t)kNumQueues * capacity, err)) ||
-        (r = p->ctr.alloc(kNumCtr, err)) || (r = p->fetch.alloc(SLRH_FETCH_PARTS * kFetchStride, err)) ||
+    if ((r = p->ray.alloc((size_t)kRayVec * capacity, err)) || (r = p->spec.alloc((size_t)kSpecVec * capacity, err)) ||
+        (r = p->nee.alloc((size_t)kNeeVec * capacity, err)) || (r = p->q.alloc((size_t)kNumQueues * capacity, err)) ||
+        (r = p->ctr.alloc(kNumCtr, err)) || (r = p->fetch.alloc(2 * SLRH_FETCH_PARTS * kFetchStride, err)) ||
         (r = p->code.alloc(capacity, err)) || (r = p->vis.alloc(capacity, err)) ||
         (r = p->stat.alloc(kNumStat * kStatSpread, err)) || (r = p->jobActive.alloc(kMaxJobs, err)) ||
         (r = p->blockJob.alloc(capacity / kCompactSlots, err)) || (r = p->jobs.alloc(kMaxJobs, err)) ||
@@ -2501,7 +2757,9 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
     p->hostBlockJob.assign(capacity / kCompactSlots, kNoJob);
     PathState& ps = p->ps;
     ps.n = capacity;
-    ps.rec = p->rec.p;
+    ps.ray = p->ray.p;
+    ps.spec = p->spec.p;
+    ps.nee = p->nee.p;
     ps.jobs = p->jobs.p;
     ps.jobActive = p->jobActive.p;
     ps.blockJob = p->blockJob.p;
@@ -2927,6 +3185,9 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
         st->mediumLookups = (double)stat[S_MED_LOOKUPS];
         st->mediumCells = (double)stat[S_MED_CELLS];
         st->mediumRays = (double)stat[S_MED_RAYS];
+        st->mediumGuard = (double)stat[S_MED_GUARD];
+        st->nodesLds = (double)stat[S_NODES_LDS];
+        st->instEntered = (double)stat[S_INSTS];
         st->kernelSeconds = ms * 1e-3;
         st->nonFinite = p.hostCtr[C_NONFINITE];
         st->overflow = p.hostCtr[C_OVERFLOW];

@@ -2,7 +2,7 @@
 //
 // Requests (a frame each: camera, resolution, samples, depth cap, seed, film)
 // queue FIFO and are admitted while the pool has room: admission is by
-// path-state memory, i.e. slots (384 B each), in contiguous ranges of
+// path-state memory, i.e. slots (354 B each), in contiguous ranges of
 // kPoolAlign slots. A scheduler thread runs the wavefront in blocks of
 // iterations; between blocks it retires requests whose slots have all gone
 // idle, admits queued requests into the freed slots and applies

@@ -17,7 +17,7 @@ namespace slrh {
 class Server {
 public:
     enum State { kQueued = 0, kRunning = 1, kDone = 2, kCancelled = 3, kFailed = 4 };
-    static constexpr uint64_t kSlotBytes = 384 + 4 * 8 + 2;   // slot record, queue entries, code and flag bytes
+    static constexpr uint64_t kSlotBytes = 320 + 4 * 8 + 2;   // slot record, queue entries, code and flag bytes
     // live paths a request without a fixed slots-per-pixel count aims for
     static constexpr uint64_t kServeSlotsPerRequest = 2u << 20;
 

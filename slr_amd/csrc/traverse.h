@@ -225,6 +225,8 @@ __device__ __forceinline__ bool triTest(const float4 a, const float4 e1v, const 
 struct TraceCounters {
     uint32_t nodes = 0, tris = 0, rays = 0;          // all rays
     uint32_t nodesAny = 0, trisAny = 0, raysAny = 0; // the any-hit share of them
+    uint32_t nodesLds = 0;                           // nodes read from the LDS copy of the tree top
+    uint32_t insts = 0;                              // instances entered (two-level scenes)
     uint32_t overflow = 0;
 };
 
@@ -467,6 +469,7 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
                 float4 nx, fx, ny, fy, nz, fz;
                 uint4 cw;
                 if (ni < st.treeletNodes) {
+                    ++tc.nodesLds;
                     const lds_u8* q = st.treelet + ni * (kTreeletRows * 16u);
                     nx = ldsf4(q, p.nearOff[0]); fx = ldsf4(q, p.nearOff[0] ^ 16u);
                     ny = ldsf4(q, p.nearOff[1]); fy = ldsf4(q, p.nearOff[1] ^ 16u);
@@ -537,6 +540,7 @@ __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& 
                     leafPend = 0;
                     tcur = tend;
                     inst = (int32_t)(__float_as_uint(a.w) & ~kRefInstBit);
+                    ++tc.insts;
                     instRoot = __float_as_uint(e1.w);
                     instCls = __float_as_uint(e2.w);
                     const float4* xr = sc.instXf + kInstXfRows * (uint32_t)inst + 3u;

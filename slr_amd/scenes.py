@@ -267,6 +267,22 @@ def cornell_icosphere(width=1920, height=1080, subdiv=8, seed=7):
     return desc
 
 
+def cornell_sphere(width=32, height=32, material=None, subdiv=3, amplitude=0.05):
+    """Cornell box plus a displaced icosphere (radius 100, smooth vertex
+    normals) of the given material dict, for per-material parity: its
+    shading normals differ from the geometric ones, so the BSDF front end's
+    shading-normal correction is exercised."""
+    desc = cornell_box(width, height)
+    desc["materials"].append(dict(material or {"kind": "lambert", "rgb": (0.75, 0.75, 0.75)}))
+    mat = len(desc["materials"]) - 1
+    p, f = displaced_icosphere(subdiv, 100.0, (300.0, 300.0, 300.0), seed=3, amplitude=amplitude)
+    p *= SCALE
+    n = smooth_normals(p, f)
+    desc["meshes"].append({"pos": p.astype(np.float32), "nrm": n.astype(np.float32),
+                           "tan": _tangent_of(n).astype(np.float32), "idx": f, "mat": mat})
+    return desc
+
+
 def _rotation(axis, angle):
     axis = np.asarray(axis, np.float64)
     axis = axis / np.linalg.norm(axis)
@@ -360,9 +376,11 @@ def instanced_cornell(width=64, height=64, n_instances=8, seed=11, uniform_scale
     return desc
 
 
-def fbm_density(n, seed=5, octaves=5, freq=2.0, bias=0.1, scale=2.5, chunk_points=1 << 20):
+def fbm_density(n, seed=5, octaves=5, freq=2.0, bias=0.0, scale=1.0, chunk_points=1 << 20):
     """(n, n, n) lattice (z, y, x) of max(0, scale * (fBm(p) + bias)) over the
-    unit cube: a seeded Perlin fBm cloud of densities mostly in [0, 1].
+    unit cube: a seeded Perlin fBm cloud. The defaults give the volumetric
+    configuration's density max(0, fBm) (about 60 % of the lattice non-zero,
+    mean 0.09, maximum 0.6 for 5 octaves), which the medium's sigma_t scales.
     Evaluated a few z slices at a time to bound the temporaries."""
     g = (np.arange(n) + 0.0) / (n - 1)
     y, x = np.meshgrid(g, g, indexing="ij")

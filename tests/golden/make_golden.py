@@ -61,13 +61,12 @@ def case(name):
     desc = make()
     cmf = refdrv.cmf_strata()
     threads = max(1, min(8, len(os.sched_getaffinity(0))))
-    out = []
-    for b in range(B):
-        r = refdrv.RefScene(desc)
-        f, _, _ = r.render(W, H, spp, seed + b, threads=threads, volumetric=vol)
-        r.close()
-        out.append(refcases.xyz(f.astype(np.float64), cmf) / spp)
-    out = np.array(out)
+    # all batches on one scene build (seeds seed .. seed + B - 1)
+    r = refdrv.RefScene(desc)
+    films, sec, build_s = r.render_batches(W, H, spp, seed, B, threads=threads, volumetric=vol)
+    r.close()
+    print("%s: %d batches of %dx%d x %d spp in %.1fs (build %.1fs)" % (name, B, W, H, spp, sec, build_s))
+    out = np.array([refcases.xyz(f.astype(np.float64), cmf) / spp for f in films])
     tot = out.sum(axis=(1, 2, 3))
     np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), mean=out.mean(0), se=out.std(0, ddof=1) / np.sqrt(B),
                         tot_mean=tot.mean(), tot_se=tot.std(ddof=1) / np.sqrt(B), batches=B, spp=spp)

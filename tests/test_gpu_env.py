@@ -56,17 +56,17 @@ def test_uniform_environment_furnace(gpu_ctx):
 
 def test_sky_matches_reference_renderer_3sigma(gpu_ctx):
     """Reference PTRenderer image of an open scene under a sky with a sun
-    (32 x 24, 32 batches of 32 spp, tests/golden/ref_sky.npz) vs 16 GPU
+    (32 x 24, 32 batches of 32 spp, tests/golden/ref_sky.npz) vs 32 GPU
     batches of 32 spp."""
     make, W, H, _, spp, _, _ = refcases.CASES["sky"]
     sc = slr_amd.build_scene(gpu_ctx, make())
     cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(16):
+    for b in range(32):
         film = slr_amd.Film(gpu_ctx, W, H)
         slr_amd.PTRenderer(spp).render(sc, film, seed=300 + b)
         gb.append(refcases.xyz(film.strata(), cmf) / spp)
-    refcases.check_3sigma(np.array(gb), refcases.load("sky"))
+    refcases.check_3sigma(np.array(gb), refcases.load("sky"), name="sky")
 
 
 def test_environment_adds_light(gpu_ctx):

@@ -118,16 +118,16 @@ def test_render_matches_flattened_scene(gpu_ctx):
 def test_render_matches_reference_renderer_3sigma(gpu_ctx):
     """Reference PTRenderer image (instances through InternalNode +
     ReferenceNode, 32 x 32, 32 batches of 32 spp, tests/golden/ref_instanced.npz)
-    vs 16 GPU batches of 32 spp in compat mode."""
+    vs 32 GPU batches of 32 spp in compat mode."""
     make, W, H, _, spp, _, _ = refcases.CASES["instanced"]
     sc = slr_amd.build_scene(gpu_ctx, make())
     cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(16):
+    for b in range(32):
         film = slr_amd.Film(gpu_ctx, W, H)
         slr_amd.PTRenderer(spp).render(sc, film, seed=500 + b)
         gb.append(refcases.xyz(film.strata(), cmf) / spp)
-    refcases.check_3sigma(np.array(gb), refcases.load("instanced"))
+    refcases.check_3sigma(np.array(gb), refcases.load("instanced"), name="instanced")
 
 
 def _field(W, H, per_instance):
@@ -153,14 +153,14 @@ def test_instanced_field_matches_reference_renderer_3sigma(gpu_ctx):
     """Instances of four meshes with mixed Lambert, Oren-Nayar, GGX conductor
     and GGX dielectric materials under an equirectangular sky with a sun:
     the reference PTRenderer's image (one prototype per mesh and material,
-    32 x 32, 32 batches of 16 spp, tests/golden/ref_field.npz) vs 16 GPU
+    32 x 32, 32 batches of 16 spp, tests/golden/ref_field.npz) vs 32 GPU
     batches of 16 spp with the materials on the instances, compat mode."""
     _, W, H, _, spp, _, _ = refcases.CASES["field"]
     sc = slr_amd.build_scene(gpu_ctx, dict(_field(W, H, True), compat_instance_shadow=True))
     cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(16):
+    for b in range(32):
         film = slr_amd.Film(gpu_ctx, W, H)
         slr_amd.PTRenderer(spp).render(sc, film, seed=800 + b)
         gb.append(refcases.xyz(film.strata(), cmf) / spp)
-    refcases.check_3sigma(np.array(gb), refcases.load("field"))
+    refcases.check_3sigma(np.array(gb), refcases.load("field"), name="field")

@@ -57,16 +57,16 @@ def _xyz(strata):
 
 def test_matches_reference_renderer_3sigma(gpu_ctx):
     """Reference PTRenderer image of the Cornell box (32 x 32, 32 batches of
-    64 spp, tests/golden/ref_cornell32.npz) vs 16 GPU batches of 64 spp."""
+    64 spp, tests/golden/ref_cornell32.npz) vs 32 GPU batches of 64 spp."""
     make, W, H, _, spp, _, _ = refcases.CASES["cornell32"]
     sc = slr_amd.build_scene(gpu_ctx, make())
     cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(16):
+    for b in range(32):
         film = slr_amd.Film(gpu_ctx, W, H)
         slr_amd.PTRenderer(spp).render(sc, film, seed=100 + b)
         gb.append(refcases.xyz(film.strata(), cmf) / spp)
-    refcases.check_3sigma(np.array(gb), refcases.load("cornell32"))
+    refcases.check_3sigma(np.array(gb), refcases.load("cornell32"), name="cornell32")
 
 
 def test_slots_per_pixel_do_not_change_the_estimate(gpu_ctx):

@@ -95,18 +95,43 @@ def test_absorbing_slab_closed_form(gpu_ctx):
 def test_medium_matches_reference_renderer_3sigma(gpu_ctx):
     """Reference VolumetricPTRenderer image (DensityGridMediumNode in the
     Cornell box, 32 x 32, 32 batches of 16 spp, tests/golden/ref_medium.npz)
-    vs 16 GPU batches of 16 spp in the reference's compat mode."""
+    vs 32 GPU batches of 16 spp in the reference's compat mode."""
     make, W, H, _, spp, _, _ = refcases.CASES["medium"]
     sc = slr_amd.build_scene(gpu_ctx, make())
     cmf = slr_amd.cmf_strata()
     gb = []
-    for b in range(16):
+    for b in range(32):
         film = slr_amd.Film(gpu_ctx, W, H)
         slr_amd.VolumetricPTRenderer(spp).render(sc, film, seed=900 + b)
         gb.append(refcases.xyz(film.strata(), cmf) / spp)
     gb = np.array(gb)
     assert np.isfinite(gb).all()
-    refcases.check_3sigma(gb, refcases.load("medium"))
+    refcases.check_3sigma(gb, refcases.load("medium"), name="medium")
+
+
+def test_thick_medium_matches_reference_renderer_3sigma(gpu_ctx):
+    """The volumetric configuration's regime: density max(0, fBm5) * sigma_t
+    with sigma_t 20 on a 64^3 lattice (optical depth about 9 across the
+    box, about 6 scattering events per sample). Reference
+    VolumetricPTRenderer statistics (tests/golden/ref_medium_thick.npz, 32
+    batches of 16 spp) vs 32 GPU batches, compat mode; no tracking walk may
+    stop at the step guard."""
+    make, W, H, _, spp, _, _ = refcases.CASES["medium_thick"]
+    sc = slr_amd.build_scene(gpu_ctx, make())
+    cmf = slr_amd.cmf_strata()
+    gb = []
+    events = guard = 0.0
+    for b in range(32):
+        film = slr_amd.Film(gpu_ctx, W, H)
+        st = slr_amd.VolumetricPTRenderer(spp).render(sc, film, seed=1900 + b)
+        events += st["medium_events"]
+        guard += st["medium_guard"]
+        gb.append(refcases.xyz(film.strata(), cmf) / spp)
+    gb = np.array(gb)
+    assert np.isfinite(gb).all()
+    assert guard == 0
+    assert events / (32 * W * H * spp) > 1.0
+    refcases.check_3sigma(gb, refcases.load("medium_thick"), name="medium_thick")
 
 
 def test_volumetric_without_medium_is_the_path_tracer(gpu_ctx):

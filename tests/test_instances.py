@@ -119,5 +119,6 @@ def test_fbm_density_matches_direct_evaluation():
     g = np.arange(n) / (n - 1)
     z, y, x = np.meshgrid(g, g, g, indexing="ij")
     p = np.stack([x.ravel(), y.ravel(), z.ravel()], -1)
-    want = np.maximum(0.0, 2.5 * (scenes.fbm(p, 3, octaves=3, freq=2.0) + 0.1)).reshape(n, n, n)
+    # the volumetric configuration's density: max(0, fBm)
+    want = np.maximum(0.0, scenes.fbm(p, 3, octaves=3, freq=2.0)).reshape(n, n, n)
     assert np.allclose(d, want.astype(np.float32), rtol=0, atol=1e-6)
