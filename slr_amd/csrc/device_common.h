@@ -83,7 +83,8 @@ struct DevCamera {
 };
 
 constexpr uint32_t kNoInstance = 0xFFFFFFFFu;
-constexpr uint32_t kLightRecRows = 7;
+constexpr uint32_t kLightRecRows = 8;
+constexpr uint32_t kTriShadeRows = 8;   // per-triangle shading record: one 128-B line
 constexpr uint32_t kInstXfRows = 8;
 
 struct DevScene {
@@ -98,7 +99,12 @@ struct DevScene {
     const float* specPool;
     const uint2* lights;      // emitting triangles: global id, instance (kNoInstance: world space)
     const float4* lightRec;   // per light kLightRecRows float4: world vertices (material, instance and area
-                              // PDF in w), geometric normal, the three object-space vertex normals
+                              // PDF in w), geometric normal, the three object-space vertex normals, the
+                              // header of the light material's emittance spectrum (DevSpectrum bits)
+    const float4* triShade;   // per triangle kTriShadeRows float4: the three vertex normals (material id
+                              // bits in n0.w), the three vertex tangents, the geometric normal (its own
+                              // space: world, or object space for prototype triangles), unused
+    const DevSpectrum* matSpec;   // per material 3 headers: spec0, spec1, emittance (n = 0: none)
     const float* lightCdf;    // numLights + 1 entries
     const float* lightPmf;
     const int32_t* primToLight;

@@ -64,7 +64,8 @@ constexpr int kMinLdsDepth = 8;     // smallest LDS depth among the traversal va
 struct TraceVariant { int ldsDepth, refill; bool fuse; int wavesPerEU; int treelet; };
 constexpr TraceVariant kVariants[] = {{16, 16, true, 0, 256}, {16, 16, true, 0, 0},   {16, 16, true, 0, 128},
                                       {16, 16, true, 0, 512}, {8, 16, true, 0, 256},  {8, 16, true, 0, 512},
-                                      {16, 16, true, 6, 256}, {16, 32, true, 0, 256}, {16, 16, false, 0, 256}};
+                                      {16, 16, true, 6, 256}, {16, 32, true, 0, 256}, {16, 16, false, 0, 256},
+                                      {8, 16, true, 0, 128},  {16, 16, true, 0, 64}};
 constexpr int kBatchTreelet = 128;
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // refill threshold of the ray-batch entry points
@@ -115,7 +116,8 @@ struct DeviceScene {
     DevBuf<DevSpectrum> spectra;
     DevBuf<float> specPool, lightCdf, lightPmf;
     DevBuf<uint2> lights;
-    DevBuf<float4> lightRec;
+    DevBuf<float4> lightRec, triShade;
+    DevBuf<DevSpectrum> matSpec;
     DevBuf<int32_t> primToLight, instMat;
     DevBuf<float4> instXf;
     DevBuf<uint32_t> spill;      // traversal stack spill for ray batches
@@ -219,6 +221,7 @@ static void cameraDerived(const CameraParams& c, uint32_t w, uint32_t h, DevCame
 }
 
 static int buildLightRecords(DeviceScene* ds, std::string* err);
+static int buildTriShade(DeviceScene* ds, std::string* err);
 
 // Corner-packed lattice (see DevScene::medCorners): entry i = (x, y, z)
 // holds g at (x|ux, y|uy, z|uz) with u = min(l + 1, n - 1), the eight values
@@ -368,6 +371,17 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         specs.push_back(d);
     }
     if (pool.empty()) pool.push_back(0.0f);
+    // per material the headers of its spectra, read beside the material so
+    // evaluating one needs no lookup of the header by id
+    std::vector<DevSpectrum> matSpec(std::max<size_t>(1, 3 * sd.materials.size()));
+    for (size_t m = 0; m < sd.materials.size(); ++m) {
+        const int32_t ids[3] = {sd.materials[m].spec0, sd.materials[m].spec1, sd.materials[m].emit};
+        for (int j = 0; j < 3; ++j) {
+            DevSpectrum h{0.0f, 0.0f, 0u, 0u};
+            if (ids[j] >= 0 && (size_t)ids[j] < specs.size()) h = specs[ids[j]];
+            matSpec[3 * m + j] = h;
+        }
+    }
     // lights: every emitting triangle with importance 1, in object order
     // (SurfaceObjectAggregate ctor, libSLR/Core/surface_object.cpp:379-398):
     // the world-space ones, then per instance its prototype's (an instance
@@ -429,7 +443,7 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         (r = ds->specPool.upload(pool, err)) || (r = ds->lights.upload(lights, err)) ||
         (r = ds->lightCdf.upload(cdf, err)) || (r = ds->lightPmf.upload(pmf, err)) ||
         (r = ds->primToLight.upload(primToLight, err)) || (r = ds->instXf.upload(instXf, err)) ||
-        (r = ds->instMat.upload(instMat, err)))
+        (r = ds->instMat.upload(instMat, err)) || (r = ds->matSpec.upload(matSpec, err)))
         return r;
     if ((r = ds->counters.alloc(16, err))) return r;
     DevScene& ve = ds->view;
@@ -509,6 +523,7 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     v.nodes = ds->nodes.p; v.tris = ds->tris.p; v.primVerts = ds->primVerts.p;
     v.vpos = ds->vpos.p; v.vnrm = ds->vnrm.p; v.vtan = ds->vtan.p;
     v.mats = ds->mats.p; v.spectra = ds->spectra.p; v.specPool = ds->specPool.p;
+    v.matSpec = ds->matSpec.p;
     v.lights = ds->lights.p; v.lightCdf = ds->lightCdf.p; v.lightPmf = ds->lightPmf.p;
     v.primToLight = ds->primToLight.p;
     v.instXf = ds->instXf.p;
@@ -524,6 +539,7 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     ds->camera = sd.camera;
     if (sd.hasCamera) cameraDerived(sd.camera, 1, 1, &v.cam);
     if (int rl = buildLightRecords(ds.get(), err)) return rl;
+    if (int rt = buildTriShade(ds.get(), err)) return rt;
     *out = ds.release();
     return 0;
 }
@@ -917,8 +933,9 @@ __device__ __forceinline__ float lambdaOf(int i, float off) {
 }
 
 // RegularContinuousSpectrum::evaluate (libSLR/BasicTypes/spectrum_types.cpp:82-100)
-__device__ __forceinline__ void evalSpectrum(const DevScene& sc, int32_t id, float off, float* out) {
-    const DevSpectrum sp = sc.spectra[id];
+__device__ __forceinline__ void evalSpectrumAt(const DevScene& sc, float lo, float hi, uint32_t offset, uint32_t n,
+                                               float off, float* out) {
+    const DevSpectrum sp{lo, hi, offset, n};
     const float* v = sc.specPool + sp.offset;
     const float span = sp.hi - sp.lo;
     const float nm1 = (float)(sp.n - 1);
@@ -935,6 +952,15 @@ __device__ __forceinline__ void evalSpectrum(const DevScene& sc, int32_t id, flo
         }
         out[i] = r;
     }
+}
+__device__ __forceinline__ void evalSpectrum(const DevScene& sc, int32_t id, float off, float* out) {
+    const DevSpectrum h = sc.spectra[id];
+    evalSpectrumAt(sc, h.lo, h.hi, h.offset, h.n, off, out);
+}
+// spectrum j (0: spec0, 1: spec1, 2: emittance) of material m
+__device__ __forceinline__ void evalMatSpectrum(const DevScene& sc, uint32_t m, uint32_t j, float off, float* out) {
+    const DevSpectrum h = sc.matSpec[3 * m + j];
+    evalSpectrumAt(sc, h.lo, h.hi, h.offset, h.n, off, out);
 }
 
 // An unoccluded shadow ray's contribution (already weighted by the shading
@@ -1624,7 +1650,42 @@ __global__ void k_light_records(DevScene sc, float4* out) {
         R[4] = sc.vnrm[lv.x];
         R[5] = sc.vnrm[lv.y];
         R[6] = sc.vnrm[lv.z];
+        const DevSpectrum e = sc.matSpec[3 * mat + 2];
+        R[7] = make_float4(e.lo, e.hi, __uint_as_float(e.offset), __uint_as_float(e.n));
     }
+}
+
+// Per-triangle shading records (DevScene::triShade), with the expressions
+// calculateSurfacePoint uses, so the values are the same bits.
+__global__ void k_tri_shade(DevScene sc, float4* out) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < sc.numPrims; t += gridDim.x * blockDim.x) {
+        const uint4 pv = sc.primVerts[t];
+        const float3 p0 = xyz(sc.vpos[pv.x]), p1 = xyz(sc.vpos[pv.y]), p2 = xyz(sc.vpos[pv.z]);
+        const float3 g = normalize3(cross3(p1 - p0, p2 - p0));
+        float4* R = out + (size_t)kTriShadeRows * t;
+        const float4 n0 = sc.vnrm[pv.x];
+        R[0] = make_float4(n0.x, n0.y, n0.z, __uint_as_float(pv.w));
+        R[1] = sc.vnrm[pv.y];
+        R[2] = sc.vnrm[pv.z];
+        R[3] = sc.vtan[pv.x];
+        R[4] = sc.vtan[pv.y];
+        R[5] = sc.vtan[pv.z];
+        R[6] = make_float4(g.x, g.y, g.z, 0.0f);
+        R[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+static int buildTriShade(DeviceScene* ds, std::string* err) {
+    const uint32_t n = ds->view.numPrims;
+    ds->view.triShade = nullptr;
+    if (n == 0) return 0;
+    if (int r = ds->triShade.alloc((size_t)n * kTriShadeRows, err)) return r;
+    hipLaunchKernelGGL(k_tri_shade, dim3(std::min(4096u, (n + 255) / 256)), dim3(256), 0, 0, ds->view,
+                       ds->triShade.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    ds->view.triShade = ds->triShade.p;
+    return 0;
 }
 
 static int buildLightRecords(DeviceScene* ds, std::string* err) {
@@ -1647,10 +1708,11 @@ struct LightPoint {
     float3 lp, lg, ln;
     float areaPDF;
     uint32_t mat;
+    float4 emit;        // header of the light material's emittance spectrum (DevSpectrum bits)
 };
 __device__ __forceinline__ LightPoint lightPoint(const DevScene& sc, uint32_t li, float lu0, float lu1) {
     const float4* R = sc.lightRec + (size_t)kLightRecRows * li;
-    const float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3], n0 = R[4], n1 = R[5], n2 = R[6];
+    const float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3], n0 = R[4], n1 = R[5], n2 = R[6], r7 = R[7];
     const float su = sqrtf(lu0);
     const float lb0 = 1.0f - su, lb1 = lu1 * su, lb2 = 1.0f - lb0 - lb1;
     const int32_t linst = __float_as_int(r1.w);
@@ -1661,6 +1723,7 @@ __device__ __forceinline__ LightPoint lightPoint(const DevScene& sc, uint32_t li
     L.lg = xyz(r3);
     L.areaPDF = r2.w;
     L.mat = __float_as_uint(r0.w);
+    L.emit = r7;
     return L;
 }
 
@@ -1687,10 +1750,11 @@ __device__ __forceinline__ void worldVerts(const DevScene& sc, uint32_t prim, in
 __device__ __forceinline__ SurfPt surfaceAt(const DevScene& sc, uint32_t prim, float b0, float b1, float3 o, float3 d,
                                             float t, int32_t inst = -1) {
     SurfPt s;
-    const uint4 pv = sc.primVerts[prim];
-    const float3 p0 = xyz(sc.vpos[pv.x]), p1 = xyz(sc.vpos[pv.y]), p2 = xyz(sc.vpos[pv.z]);
-    const float3 n0 = xyz(sc.vnrm[pv.x]), n1 = xyz(sc.vnrm[pv.y]), n2 = xyz(sc.vnrm[pv.z]);
-    const float3 t0 = xyz(sc.vtan[pv.x]), t1 = xyz(sc.vtan[pv.y]), t2 = xyz(sc.vtan[pv.z]);
+    // the triangle's shading record: one 128-B line (k_tri_shade)
+    const float4* R = sc.triShade + (size_t)kTriShadeRows * prim;
+    const float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3], r4 = R[4], r5 = R[5], r6 = R[6];
+    const float3 n0 = xyz(r0), n1 = xyz(r1), n2 = xyz(r2);
+    const float3 t0 = xyz(r3), t1 = xyz(r4), t2 = xyz(r5);
     const float b2 = 1.0f - b0 - b1;
     s.sz = normalize3(b0 * n0 + b1 * n1 + b2 * n2);
     float3 x = b0 * t0 + b1 * t1 + b2 * t2;
@@ -1698,8 +1762,8 @@ __device__ __forceinline__ SurfPt surfaceAt(const DevScene& sc, uint32_t prim, f
     s.sx = normalize3(x - dNT * s.sz);
     s.sy = cross3(s.sz, s.sx);
     s.p = o + d * t;
-    s.gN = normalize3(cross3(p1 - p0, p2 - p0));
-    s.mat = pv.w;
+    s.gN = xyz(r6);
+    s.mat = __float_as_uint(r0.w);
     s.prim = prim;
     if (inst >= 0) {
         const float4* X = sc.instXf + kInstXfRows * (uint32_t)inst;
@@ -1875,7 +1939,7 @@ void k_shade(DevScene sc, PathState ps) {
                 // emission (PTRenderer.cpp:162-166 first hit, :247-263 implicit with MIS)
                 if (emits) {
                     float Le[kNumL];
-                    evalSpectrum(sc, mat.emit, misc.x, Le);
+                    evalMatSpectrum(sc, sp.mat, 2, misc.x, Le);
                     float misW = 1.0f;
                     if (pathLen > 0 && !(info.y & F_LAST_DELTA)) {
                         const float bsdfPDF = misc.z;
@@ -1922,9 +1986,18 @@ void k_shade(DevScene sc, PathState ps) {
                     // the material's spectra (reflectance, or the two indices of
                     // refraction) are evaluated where each block needs them, so
                     // they are not live across both blocks
-                    auto materialSpectra = [&](float* A, float* B) {
-                        evalSpectrum(sc, mat.spec0, misc.x, A);
-                        if (CLS != CLS_DIFFUSE) evalSpectrum(sc, mat.spec1, misc.x, B);
+                    const uint32_t matId = sp.mat;
+                    const int32_t spec0 = mat.spec0, spec1 = mat.spec1;
+                    const float wlOff = misc.x;
+                    auto materialSpectra = [&sc, matId, spec0, spec1, wlOff](float* A, float* B) {
+                        if (CLS == CLS_DIFFUSE) {
+                            evalMatSpectrum(sc, matId, 0, wlOff, A);
+                        } else {
+                            // (through the material's spectrum ids: the header table
+                            // here leaves an unused stack object in these kernels)
+                            evalSpectrum(sc, spec0, wlOff, A);
+                            evalSpectrum(sc, spec1, wlOff, B);
+                        }
                     };
                     // ---- next event estimation (PTRenderer.cpp:179-218)
                     const float uSel = xsFloat(rng);
@@ -2013,9 +2086,9 @@ void k_shade(DevScene sc, PathState ps) {
                             const float dist = length3(lp - sp.p);
                             tangentNaN(divs(lp - sp.p, dist), dist * (1 - 1e-4f));
                         } else if (cosS * cosL > 0 && lz > 0.0f) {
-                            const DevMaterial lm = sc.mats[L.mat];
                             float M[kNumL], fs[kNumL];
-                            evalSpectrum(sc, lm.emit, misc.x, M);
+                            evalSpectrumAt(sc, L.emit.x, L.emit.y, __float_as_uint(L.emit.z),
+                                           __float_as_uint(L.emit.w), misc.x, M);
                             const float3 sdL = toLocal(sp, sdir);
                             float bpdf = 0.0f;
                             float A[kNumL], B[kNumL];
@@ -2406,7 +2479,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
                 sdir = divs(dv, dist);
                 const float cosL = fabsf(dot3(sdir, lg));
                 if (cosL > 0.0f && dot3(ln, -sdir) > 0.0f) {
-                    evalSpectrum(sc, sc.mats[L.mat].emit, misc.x, Le);
+                    evalSpectrumAt(sc, L.emit.x, L.emit.y, __float_as_uint(L.emit.z), __float_as_uint(L.emit.w),
+                                   misc.x, Le);
                     const float edf = (float)(1.0 / 3.14159265358979323846);
 #pragma unroll
                     for (int k = 0; k < kNumL; ++k) Le[k] *= edf;
@@ -2846,7 +2920,8 @@ static void launchTraceV(uint32_t blocks, hipStream_t s, const DevScene& sc, con
 typedef void (*LaunchFn)(uint32_t, hipStream_t, const DevScene&, const PathState&);
 static const LaunchFn kTraceLaunch[kNumVariants] = {launchTraceV<0>, launchTraceV<1>, launchTraceV<2>,
                                                      launchTraceV<3>, launchTraceV<4>, launchTraceV<5>,
-                                                     launchTraceV<6>, launchTraceV<7>, launchTraceV<8>};
+                                                     launchTraceV<6>, launchTraceV<7>, launchTraceV<8>,
+                                                     launchTraceV<9>, launchTraceV<10>};
 
 int numTraceVariants() { return kNumVariants; }
 void traceVariant(int v, int* ldsDepth, int* refill, int* fused) {
