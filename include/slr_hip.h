@@ -194,6 +194,25 @@ SLRH_API int slrh_scene_create(slrh_context* ctx, slrh_scene** out);
 SLRH_API int slrh_scene_build(slrh_scene* s);
 SLRH_API int slrh_scene_destroy(slrh_scene* s);
 
+/* Accelerators built besides the SBVH collapsed to the QBVH (which renders):
+ * SLRH_ACCEL_STANDARD_BVH adds the binned-SAH binary BVH the reference builds
+ * as StandardBVH(objs, Partitioning::BinnedSAH) (libSLR/Accelerator/
+ * StandardBVH.h:62-193, 16 bins, the same tree node for node) for ray-batch
+ * queries. Scenes without instances only. Call before slrh_scene_build. */
+enum slrh_accel_flags { SLRH_ACCEL_STANDARD_BVH = 1 };
+SLRH_API int slrh_scene_accelerators(slrh_scene* s, uint32_t flags);
+/* Which accelerator slrh_intersect / slrh_occluded traverse: the QBVH
+ * (QBVH::intersect, the default) or the binned-SAH binary BVH
+ * (StandardBVH::intersect / testVisibility, StandardBVH.h:265-330). Hits are
+ * the same up to ties at equal t. */
+enum slrh_query_accel { SLRH_QUERY_QBVH = 0, SLRH_QUERY_STANDARD_BVH = 1 };
+SLRH_API int slrh_scene_query_accel(slrh_scene* s, int accel);
+/* Dump of the binned-SAH BVH in the layout of slrh_scene_sbvh_dump (nodes in
+ * StandardBVH's m_nodes order); stats[0] depth, stats[1] SAH cost
+ * (StandardBVH::calcSAHCost). Any output may be NULL. */
+SLRH_API int slrh_scene_standard_bvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs,
+                                          uint32_t* num_nodes, uint32_t* num_refs, double* stats);
+
 /* stats[0..7]: SBVH nodes, SBVH leaf refs, SBVH depth, SBVH SAH cost,
  * QBVH nodes, QBVH leaf refs, QBVH depth, QBVH SAH cost; stats[8]: build seconds. */
 SLRH_API int slrh_scene_accel_stats(const slrh_scene* s, double* stats);

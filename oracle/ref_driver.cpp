@@ -94,6 +94,7 @@
 #define class struct
 #include <libSLR/Accelerator/SBVH.h>
 #include <libSLR/Accelerator/QBVH.h>
+#include <libSLR/Accelerator/StandardBVH.h>
 #undef class
 #undef private
 #undef protected
@@ -165,6 +166,7 @@ struct RefScene {
     std::map<const SurfaceObject*, int32_t> objToPrim;
     SBVH* sbvh = nullptr;
     QBVH* qbvh = nullptr;
+    StandardBVH* stdbvh = nullptr;   // StandardBVH(objs, BinnedSAH), built beside the SBVH
 
     ~RefScene() {
         envTex.reset();
@@ -392,8 +394,35 @@ int refd_build_accel(void* sp, int quiet) {
     }
     s->sbvh = new SBVH(s->objs);
     s->qbvh = new QBVH(*s->sbvh);
+    s->stdbvh = new StandardBVH(s->objs, StandardBVH::Partitioning::BinnedSAH);
     s->built = true;
     return 0;
+}
+
+// Dumps the StandardBVH as refd_sbvh_dump does the SBVH; stats: nodes, leaf
+// refs, depth, SAH cost.
+int refd_stdbvh_dump(void* sp, float* boxes, uint32_t* info, int32_t* refs, double* stats) {
+    RefScene* s = (RefScene*)sp;
+    if (!s->stdbvh) return -1;
+    const auto& n = s->stdbvh->m_nodes;
+    const auto& l = s->stdbvh->m_objLists;
+    if (stats) {
+        stats[0] = (double)n.size();
+        stats[1] = (double)l.size();
+        stats[2] = s->stdbvh->m_depth;
+        stats[3] = s->stdbvh->m_cost;
+    }
+    for (size_t i = 0; boxes && info && i < n.size(); ++i) {
+        const auto& b = n[i].bbox;
+        float* o = boxes + 6 * i;
+        o[0] = b.minP.x; o[1] = b.minP.y; o[2] = b.minP.z;
+        o[3] = b.maxP.x; o[4] = b.maxP.y; o[5] = b.maxP.z;
+        uint32_t* f = info + 5 * i;
+        f[0] = n[i].c0; f[1] = n[i].c1; f[2] = n[i].offsetFirstLeaf; f[3] = n[i].numLeaves;
+        f[4] = n[i].numLeaves ? 0xFF : (uint32_t)n[i].axis;
+    }
+    for (size_t i = 0; refs && i < l.size(); ++i) refs[i] = s->objToPrim.at(l[i]);
+    return (int)n.size();
 }
 
 // stats: [0]=SBVH nodes, [1]=SBVH leaf refs, [2]=SBVH depth, [3]=SBVH SAH cost,
@@ -539,7 +568,7 @@ int refd_instance_inverse(void* sp, int k, float* rows) {
 }
 
 // Closest hit for n rays, each 8 floats {org xyz, dir xyz, tmin, tmax}.
-// accel: 0 = QBVH, 1 = SBVH. Miss: prim = -1, t = tmax. inst (may be null):
+// accel: 0 = QBVH, 1 = SBVH, 2 = StandardBVH. Miss: prim = -1, t = tmax. inst (may be null):
 // the instance of the hit triangle, -1 in world space.
 int refd_intersect(void* sp, int accel, int n, const float* rays, float* t, int32_t* prim,
                    float* b0, float* b1, int32_t* inst) {
@@ -551,8 +580,9 @@ int refd_intersect(void* sp, int accel, int n, const float* rays, float* t, int3
         RaySegment seg(r[6], r[7]);
         SurfaceInteraction si;
         const SurfaceObject* hit = nullptr;
-        bool ok = accel == 0 ? s->qbvh->intersect(ray, seg, sampler, &si, &hit)
-                             : s->sbvh->intersect(ray, seg, sampler, &si, &hit);
+        bool ok = accel == 0   ? s->qbvh->intersect(ray, seg, sampler, &si, &hit)
+                  : accel == 1 ? s->sbvh->intersect(ray, seg, sampler, &si, &hit)
+                               : s->stdbvh->intersect(ray, seg, sampler, &si, &hit);
         if (ok) {
             t[i] = si.getDistance();
             prim[i] = s->objToPrim.at(si.m_obj);
@@ -575,7 +605,9 @@ int refd_visibility(void* sp, int accel, int n, const float* rays, float* vis) {
         const float* r = rays + 8 * i;
         Ray ray(Point3D(r[0], r[1], r[2]), Vector3D(r[3], r[4], r[5]), 0.0f);
         RaySegment seg(r[6], r[7]);
-        vis[i] = accel == 0 ? s->qbvh->testVisibility(ray, seg) : s->sbvh->testVisibility(ray, seg);
+        vis[i] = accel == 0   ? s->qbvh->testVisibility(ray, seg)
+                 : accel == 1 ? s->sbvh->testVisibility(ray, seg)
+                              : s->stdbvh->testVisibility(ray, seg);
     }
     return 0;
 }

@@ -43,6 +43,7 @@ def lib():
             "refd_accel_stats": (None, [vp, vp]),
             "refd_sbvh_dump": (i32, [vp, vp, vp, vp]),
             "refd_qbvh_dump": (i32, [vp, vp, vp, vp]),
+            "refd_stdbvh_dump": (i32, [vp, vp, vp, vp, vp]),
             "refd_intersect": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp]),
             "refd_visibility": (i32, [vp, i32, i32, vp, vp]),
             "refd_set_environment": (i32, [vp, i32, i32, vp, f32]),
@@ -166,6 +167,18 @@ class RefScene:
         refs = np.zeros(r, np.int32)
         self.L.refd_qbvh_dump(self.h, _p(boxes), _p(info), _p(refs))
         return boxes, info, refs
+
+    def standard_bvh_dump(self):
+        """(boxes (n, 6), info (n, 5), refs, depth, SAH cost) of the
+        reference StandardBVH(objs, BinnedSAH) over the scene's objects."""
+        st = np.zeros(4)
+        self.L.refd_stdbvh_dump(self.h, None, None, None, _p(st))
+        n, r = int(st[0]), int(st[1])
+        boxes = np.zeros((n, 6), np.float32)
+        info = np.zeros((n, 5), np.uint32)
+        refs = np.zeros(r, np.int32)
+        self.L.refd_stdbvh_dump(self.h, _p(boxes), _p(info), _p(refs), None)
+        return boxes, info, refs, int(st[2]), float(st[3])
 
     def medium_density(self, params):
         """Density of the medium at (n, 3) local coordinates (calcDensity)."""

@@ -36,6 +36,7 @@ EXPORTS = [
     "slrh_medium_density_grid", "slrh_medium_query", "slrh_camera_perspective",
     "slrh_scene_create", "slrh_scene_build", "slrh_scene_destroy",
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
+    "slrh_scene_accelerators", "slrh_scene_query_accel", "slrh_scene_standard_bvh_dump",
     "slrh_intersect", "slrh_occluded", "slrh_intersect_noalloc",
     "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read", "slrh_film_load", "slrh_film_bmp",
     "slrh_render_progressive",
@@ -125,6 +126,9 @@ def load_library(path=None):
         "slrh_scene_accel_stats": (i32, [vp, vp]),
         "slrh_scene_sbvh_dump": (i32, [vp, vp, vp, vp, vp, vp]),
         "slrh_scene_qbvh_dump": (i32, [vp, vp, vp, vp, vp, vp]),
+        "slrh_scene_accelerators": (i32, [vp, u32]),
+        "slrh_scene_query_accel": (i32, [vp, i32]),
+        "slrh_scene_standard_bvh_dump": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "slrh_intersect": (i32, [vp, vp, vp, u32, i32, vp]),
         "slrh_occluded": (i32, [vp, vp, vp, u32, i32, vp]),
         "slrh_intersect_noalloc": (i32, [vp, vp, vp, u32, vp]),
@@ -349,6 +353,27 @@ class Scene:
 
     def qbvh_dump(self):
         return self._dump(self.lib.slrh_scene_qbvh_dump, 24, 7, "slrh_scene_qbvh_dump")
+
+    def standard_bvh_dump(self):
+        """(boxes (n, 6), info (n, 5), refs, depth, SAH cost) of the binned-SAH BVH."""
+        nn, nr = ctypes.c_uint32(), ctypes.c_uint32()
+        st = np.zeros(2)
+        fn = self.lib.slrh_scene_standard_bvh_dump
+        self.ctx.check(fn(self.h, None, None, None, ctypes.byref(nn), ctypes.byref(nr), _ptr(st)),
+                       "slrh_scene_standard_bvh_dump")
+        boxes = np.zeros((nn.value, 6), np.float32)
+        info = np.zeros((nn.value, 5), np.uint32)
+        refs = np.zeros(nr.value, np.int32)
+        self.ctx.check(fn(self.h, _ptr(boxes), _ptr(info), _ptr(refs), None, None, None),
+                       "slrh_scene_standard_bvh_dump")
+        return boxes, info, refs, int(st[0]), float(st[1])
+
+    def set_accelerators(self, standard_bvh=False):
+        self.ctx.check(self.lib.slrh_scene_accelerators(self.h, 1 if standard_bvh else 0), "slrh_scene_accelerators")
+
+    def query_accel(self, standard_bvh):
+        """Ray batches through the binned-SAH BVH (True) or the QBVH (False)."""
+        self.ctx.check(self.lib.slrh_scene_query_accel(self.h, 1 if standard_bvh else 0), "slrh_scene_query_accel")
 
     def intersect(self, rays, with_inst=False):
         """rays: (n, 8) float32 {org xyz, dir xyz, tmin, tmax} on the host.
@@ -662,6 +687,8 @@ def build_scene(ctx, desc):
     if med is not None:
         sc.set_medium(med["bounds"], med["density"], sc.spectrum_constant(med["sigma_s"]),
                       sc.spectrum_constant(med["sigma_e"]), med.get("g", 0.0))
+    if desc.get("standard_bvh"):
+        sc.set_accelerators(standard_bvh=True)
     cam = desc["camera"]
     sc.set_camera(cam["l2w"], cam["aspect"], cam["fov_y"], cam.get("lens_radius", 0.0), cam.get("img_dist", 1.0),
                   cam.get("obj_dist", 1.0), cam.get("sensitivity", 1.0))

@@ -33,6 +33,7 @@ struct slrh_scene {
     DeviceScene* dev = nullptr;
     bool built = false;
     uint32_t numMeshes = 0;
+    int queryAccel = SLRH_QUERY_QBVH;   // accelerator slrh_intersect / slrh_occluded use
 };
 
 struct slrh_film {
@@ -520,6 +521,12 @@ int slrh_scene_build(slrh_scene* s) {
         auto code = [&](uint32_t r) { return r < nw ? world[r] : (kRefInstance | (r - nw)); };
         for (uint32_t& r : d.accel.sbvhRefs) r = code(r);
         for (uint32_t& r : d.accel.qbvhRefs) r = code(r);
+        if (d.wantStdBvh) {
+            // the binned-SAH binary BVH over the world triangles
+            // (StandardBVH(objs, Partitioning::BinnedSAH); single-level scenes)
+            buildStandardBvh(positions(world), &d.stdBvh);
+            for (uint32_t& r : d.stdBvh.refs) r = world[r];
+        }
         if (s->ctx->device >= 0) {
             std::string err;
             int r = uploadScene(s->data, &s->dev, &err);
@@ -531,6 +538,46 @@ int slrh_scene_build(slrh_scene* s) {
         return fail(s->ctx, SLRH_ERR_INVALID, std::string("scene build: ") + e.what());
     }
     s->built = true;
+    return SLRH_OK;
+}
+
+int slrh_scene_accelerators(slrh_scene* s, uint32_t flags) {
+    CHECK_SCENE(s);
+    if (flags & ~(uint32_t)SLRH_ACCEL_STANDARD_BVH) return fail(s->ctx, SLRH_ERR_INVALID, "unknown accelerator flag");
+    if ((flags & SLRH_ACCEL_STANDARD_BVH) && !s->data.instances.empty())
+        return fail(s->ctx, SLRH_ERR_STATE, "the binned-SAH BVH is built for scenes without instances");
+    s->data.wantStdBvh = (flags & SLRH_ACCEL_STANDARD_BVH) != 0;
+    return SLRH_OK;
+}
+
+int slrh_scene_query_accel(slrh_scene* s, int accel) {
+    if (!s) return SLRH_ERR_INVALID;
+    if (accel != SLRH_QUERY_QBVH && accel != SLRH_QUERY_STANDARD_BVH)
+        return fail(s->ctx, SLRH_ERR_INVALID, "unknown query accelerator");
+    if (accel == SLRH_QUERY_STANDARD_BVH && !s->data.wantStdBvh)
+        return fail(s->ctx, SLRH_ERR_STATE, "the binned-SAH BVH was not requested (slrh_scene_accelerators)");
+    s->queryAccel = accel;
+    return SLRH_OK;
+}
+
+int slrh_scene_standard_bvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs, uint32_t* nn,
+                                 uint32_t* nr, double* stats) {
+    if (!s) return SLRH_ERR_INVALID;
+    if (!s->built || !s->data.wantStdBvh) return SLRH_ERR_STATE;
+    const StdBvh& b = s->data.stdBvh;
+    if (nn) *nn = (uint32_t)b.nodes.size();
+    if (nr) *nr = (uint32_t)b.refs.size();
+    if (stats) {
+        stats[0] = b.depth;
+        stats[1] = b.cost;
+    }
+    for (size_t i = 0; boxes && info && i < b.nodes.size(); ++i) {
+        const StdBvhNode& n = b.nodes[i];
+        for (int k = 0; k < 3; ++k) { boxes[6 * i + k] = n.box.mn[k]; boxes[6 * i + 3 + k] = n.box.mx[k]; }
+        info[5 * i + 0] = n.c0; info[5 * i + 1] = n.c1; info[5 * i + 2] = n.first; info[5 * i + 3] = n.count;
+        info[5 * i + 4] = n.count ? 0xFFu : n.axis;
+    }
+    for (size_t i = 0; refs && i < b.refs.size(); ++i) refs[i] = (int32_t)b.refs[i];
     return SLRH_OK;
 }
 
@@ -613,7 +660,9 @@ static int doTrace(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, 
     }
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     std::string err;
-    int r = traceBatch(s->dev, a, n, hits == nullptr, mem == SLRH_MEM_DEVICE, stream, nullptr, nullptr, &err);
+    int r = s->queryAccel == SLRH_QUERY_STANDARD_BVH
+                ? traceBatchStdBvh(s->dev, a, n, hits == nullptr, mem == SLRH_MEM_DEVICE, stream, &err)
+                : traceBatch(s->dev, a, n, hits == nullptr, mem == SLRH_MEM_DEVICE, stream, nullptr, nullptr, &err);
     if (r) return fail(s->ctx, r, err);
     return SLRH_OK;
 }

@@ -60,6 +60,8 @@ struct SceneData {
     bool compatTangentNee = false;   // volumetric integrator: NaN for a light sample in the shading plane
     MediumData medium;
     Accel accel;   // world level: world-space triangles and instances, refs as above
+    bool wantStdBvh = false;   // also build the binned-SAH binary BVH (ray-batch queries)
+    StdBvh stdBvh;
 };
 
 struct DeviceScene;
@@ -114,6 +116,11 @@ int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool d
 // scene's trees cannot overflow the traversal stack (nothing is read back).
 constexpr uint32_t kNoAllocMaxRays = 1u << 30;
 int traceBatchNoAlloc(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, void* stream, std::string* err);
+
+// The same queries through the binned-SAH binary BVH (built with the scene
+// when SceneData::wantStdBvh), synchronous, host or device pointers.
+int traceBatchStdBvh(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool devicePtrs, void* stream,
+                     std::string* err);
 
 // mode 0: a transmittance estimate per ray, 1: a free-path distance per ray.
 int mediumQuery(DeviceScene* ds, const TraceArgs& a, uint32_t n, int mode, uint32_t seed, float* out,

@@ -90,6 +90,22 @@ struct Accel {
     uint32_t maxLeafCount = 0;
 };
 
+// Binary BVH node of the binned-SAH builder (StandardBVH::Node,
+// libSLR/Accelerator/StandardBVH.h:24-46): leaf when count > 0.
+struct StdBvhNode {
+    Box box;
+    uint32_t c0 = 0, c1 = 0, first = 0, count = 0;
+    uint8_t axis = 0;
+};
+struct StdBvh {
+    std::vector<StdBvhNode> nodes;   // creation (pre-)order, root first
+    std::vector<uint32_t> refs;      // leaf triangle ids in leaf order
+    uint32_t depth = 0;
+    float cost = 0;
+};
+// StandardBVH(objs, Partitioning::BinnedSAH) over triangles (9 floats each).
+void buildStandardBvh(const std::vector<float>& triPos, StdBvh* out);
+
 // Builds SBVH then QBVH over triangles given as 9 floats each (world space).
 // Deterministic; identical decisions to the reference builder.
 void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads);

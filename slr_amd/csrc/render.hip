@@ -127,6 +127,9 @@ struct DeviceScene {
     DevBuf<float> medMajorant;
     DevBuf<float4> medCorners;
     DevBuf<unsigned long long> counters;
+    // binned-SAH binary BVH for ray-batch queries (when requested): 3 float4
+    // per node, 3 float4 per leaf triangle reference (the QBVH's format)
+    DevBuf<float4> bvh2Nodes, bvh2Tris;
     DevScene view;
     bool hasCamera = false;
     CameraParams camera;
@@ -540,6 +543,33 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     if (sd.hasCamera) cameraDerived(sd.camera, 1, 1, &v.cam);
     if (int rl = buildLightRecords(ds.get(), err)) return rl;
     if (int rt = buildTriShade(ds.get(), err)) return rt;
+    if (sd.wantStdBvh && !sd.stdBvh.nodes.empty()) {
+        // node rows: min xyz + (first child | first leaf ref), max xyz + (second
+        // child | leaf count), {split axis, leaf flag}
+        const StdBvh& b = sd.stdBvh;
+        std::vector<float4> bn(3 * b.nodes.size()), bt(3 * std::max<size_t>(1, b.refs.size()));
+        for (size_t k = 0; k < b.nodes.size(); ++k) {
+            const StdBvhNode& q = b.nodes[k];
+            const bool leaf = q.count > 0 || (q.c0 == 0 && q.c1 == 0);
+            const uint32_t w0 = leaf ? q.first : q.c0, w1 = leaf ? q.count : q.c1;
+            bn[3 * k] = make_float4(q.box.mn[0], q.box.mn[1], q.box.mn[2], __builtin_bit_cast(float, w0));
+            bn[3 * k + 1] = make_float4(q.box.mx[0], q.box.mx[1], q.box.mx[2], __builtin_bit_cast(float, w1));
+            bn[3 * k + 2] = make_float4(__builtin_bit_cast(float, (uint32_t)q.axis),
+                                        __builtin_bit_cast(float, leaf ? 1u : 0u), 0.0f, 0.0f);
+        }
+        for (size_t r = 0; r < b.refs.size(); ++r) {
+            const uint32_t p = b.refs[r];
+            const uint32_t* v = &sd.idx[3 * (size_t)p];
+            const float* p0 = &sd.vpos[3 * (size_t)v[0]];
+            const float* p1 = &sd.vpos[3 * (size_t)v[1]];
+            const float* p2 = &sd.vpos[3 * (size_t)v[2]];
+            const uint32_t cls = sd.materials[sd.triMat[p]].cls;
+            bt[3 * r] = make_float4(p0[0], p0[1], p0[2], __builtin_bit_cast(float, p));
+            bt[3 * r + 1] = make_float4(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], __builtin_bit_cast(float, cls));
+            bt[3 * r + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], 0.0f);
+        }
+        if ((r = ds->bvh2Nodes.upload(bn, err)) || (r = ds->bvh2Tris.upload(bt, err))) return r;
+    }
     *out = ds.release();
     return 0;
 }
@@ -691,6 +721,160 @@ int traceBatch(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool d
     if (nodesVisited) *nodesVisited = (double)cnt[0];
     if (trisTested) *trisTested = (double)cnt[1];
     if (cnt[2]) { if (err) *err = "traversal stack overflow (depth > 64)"; return SLRH_ERR_CAPACITY_CODE; }
+    return 0;
+}
+
+// Ray batches through the binned-SAH binary BVH: StandardBVH::commonProcedure
+// (libSLR/Accelerator/StandardBVH.h:265-293) per ray: a stack of node
+// indices from the root, each popped node's box tested against the current
+// segment with BoundingBox3D::intersect (libSLR/BasicTypes/BoundingBox3D.h:
+// 100-116: near/far per axis from the reciprocal direction, swapped when
+// reversed, the segment narrowed and given up when empty), an internal
+// node's children pushed far then near (near = the first child when the ray
+// points along the positive split axis), a leaf's triangles tested in order
+// with the closest hit narrowing the segment, or, for visibility, the first
+// hit ending the ray. The stack lives in LDS, [depth][lane]; the reference's
+// depth of 64.
+constexpr int kBvh2Block = 128;
+constexpr int kBvh2Stack = 64;
+
+__global__ __launch_bounds__(kBvh2Block) void k_trace_bvh2(DevScene sc, const float4* nodes, const float4* tris,
+                                                          TraceArgs a, uint32_t n, int any,
+                                                          unsigned long long* counters) {
+    __shared__ uint32_t stack[kBvh2Stack * kBvh2Block];
+    uint32_t* st = stack + threadIdx.x;
+    unsigned long long nodeCount = 0, triCount = 0;
+    uint32_t overflow = 0;
+    for (uint32_t i = blockIdx.x * kBvh2Block + threadIdx.x; i < n; i += gridDim.x * kBvh2Block) {
+        RayIn r;
+        r.o = make_float4(a.ox[i], a.oy[i], a.oz[i], a.tmin[i]);
+        r.d = make_float4(a.dx[i], a.dy[i], a.dz[i], a.tmax[i]);
+        const float inv[3] = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};   // Vector3D::reciprocal
+        const float org[3] = {r.o.x, r.o.y, r.o.z};
+        const bool pos[3] = {r.d.x >= 0.0f, r.d.y >= 0.0f, r.d.z >= 0.0f};
+        const float tmin = r.o.w;
+        float tmax = r.d.w, hb1 = 0.0f, hb2 = 0.0f;
+        int32_t prim = -1;
+        int sp = 0;
+        st[0] = 0;
+        sp = 1;
+        bool done = false;
+        while (sp > 0 && !done) {
+            const uint32_t ni = st[--sp * kBvh2Block];
+            const float4 lo = nodes[3 * ni], hi = nodes[3 * ni + 1], info = nodes[3 * ni + 2];
+            ++nodeCount;
+            const float mn[3] = {lo.x, lo.y, lo.z}, mx[3] = {hi.x, hi.y, hi.z};
+            // BoundingBox3D::isValid, then the slab test
+            if (!(mx[0] - mn[0] >= 0 && mx[1] - mn[1] >= 0 && mx[2] - mn[2] >= 0)) continue;
+            float d0 = tmin, d1 = tmax;
+            bool hit = true;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float tn = (mn[k] - org[k]) * inv[k], tf = (mx[k] - org[k]) * inv[k];
+                if (tn > tf) {
+                    const float x = tn;
+                    tn = tf;
+                    tf = x;
+                }
+                d0 = tn > d0 ? tn : d0;
+                d1 = tf < d1 ? tf : d1;
+                if (d0 > d1) hit = false;
+            }
+            if (!hit) continue;
+            const uint32_t w0 = __float_as_uint(lo.w), w1 = __float_as_uint(hi.w);
+            if (__float_as_uint(info.y) == 0u) {
+                const uint32_t axis = __float_as_uint(info.x);
+                const bool positive = axis == 0 ? pos[0] : axis == 1 ? pos[1] : pos[2];
+                if (sp + 2 > kBvh2Stack) { overflow = 1; break; }
+                st[sp++ * kBvh2Block] = positive ? w1 : w0;
+                st[sp++ * kBvh2Block] = positive ? w0 : w1;
+            } else {
+                for (uint32_t k = 0; k < w1; ++k) {
+                    const float4 v0 = tris[3 * (w0 + k)], e1 = tris[3 * (w0 + k) + 1], e2 = tris[3 * (w0 + k) + 2];
+                    ++triCount;
+                    float t, b1, b2;
+                    if (triTest(v0, e1, e2, r, tmin, tmax, &t, &b1, &b2)) {
+                        tmax = t;
+                        prim = (int32_t)__float_as_uint(v0.w);
+                        hb1 = b1;
+                        hb2 = b2;
+                        if (any) {
+                            done = true;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+        if (any) {
+            a.vis[i] = prim >= 0 ? 0.0f : 1.0f;
+        } else {
+            a.t[i] = tmax;
+            a.prim[i] = prim;
+            a.b0[i] = prim >= 0 ? 1.0f - hb1 - hb2 : 0.0f;
+            a.b1[i] = prim >= 0 ? hb1 : 0.0f;
+            if (a.inst) a.inst[i] = -1;
+        }
+    }
+    waveAdd64(&counters[0], nodeCount);
+    waveAdd64(&counters[1], triCount);
+    if (overflow) atomicOr((unsigned int*)&counters[2], 1u);
+}
+
+int traceBatchStdBvh(DeviceScene* ds, const TraceArgs& a, uint32_t n, bool any, bool devicePtrs, void* stream,
+                     std::string* err) {
+    if (n == 0) return 0;
+    if (!ds->bvh2Nodes.p) {
+        if (err) *err = "the scene was built without the binned-SAH BVH";
+        return SLRH_ERR_STATE_CODE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    TraceArgs d = a;
+    std::vector<DevBuf<float>> temps(13);
+    if (!devicePtrs) {
+        const float* hin[8] = {a.ox, a.oy, a.oz, a.dx, a.dy, a.dz, a.tmin, a.tmax};
+        const float** din[8] = {&d.ox, &d.oy, &d.oz, &d.dx, &d.dy, &d.dz, &d.tmin, &d.tmax};
+        for (int k = 0; k < 8; ++k) {
+            if (int r = temps[k].alloc(n, err)) return r;
+            HIP_TRY(hipMemcpyAsync(temps[k].p, hin[k], n * sizeof(float), hipMemcpyHostToDevice, s));
+            *din[k] = temps[k].p;
+        }
+        const int nout = any ? 1 : (a.inst ? 5 : 4);
+        for (int k = 0; k < nout; ++k)
+            if (int r = temps[8 + k].alloc(n, err)) return r;
+        if (any) {
+            d.vis = temps[8].p;
+        } else {
+            d.t = temps[8].p;
+            d.prim = (int32_t*)temps[9].p;
+            d.b0 = temps[10].p;
+            d.b1 = temps[11].p;
+            d.inst = a.inst ? (int32_t*)temps[12].p : nullptr;
+        }
+    }
+    HIP_TRY(hipMemsetAsync(ds->counters.p, 0, 4 * sizeof(unsigned long long), s));
+    const uint32_t blocks = std::max(1u, std::min((uint32_t)g_numCUs * 8, (n + kBvh2Block - 1) / kBvh2Block));
+    hipLaunchKernelGGL(k_trace_bvh2, dim3(blocks), dim3(kBvh2Block), 0, s, ds->view, ds->bvh2Nodes.p,
+                       ds->bvh2Tris.p, d, n, any ? 1 : 0, ds->counters.p);
+    HIP_TRY(hipGetLastError());
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(cnt, ds->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    if (!devicePtrs) {
+        if (any) {
+            HIP_TRY(hipMemcpyAsync(a.vis, d.vis, n * sizeof(float), hipMemcpyDeviceToHost, s));
+        } else {
+            HIP_TRY(hipMemcpyAsync(a.t, d.t, n * sizeof(float), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(a.prim, d.prim, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(a.b0, d.b0, n * sizeof(float), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(a.b1, d.b1, n * sizeof(float), hipMemcpyDeviceToHost, s));
+            if (a.inst) HIP_TRY(hipMemcpyAsync(a.inst, d.inst, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (cnt[2]) {
+        if (err) *err = "binary BVH traversal stack overflow (depth > 64)";
+        return SLRH_ERR_CAPACITY_CODE;
+    }
     return 0;
 }
 

@@ -11,17 +11,23 @@ from tests.conftest import need_ref
 pytestmark = pytest.mark.gpu
 
 
-def _compare(desc, ctx, n, seed, mode="mixed"):
+def _compare(desc, ctx, n, seed, mode="mixed", standard_bvh=False):
     refdrv = need_ref()
-    sc = build_scene(ctx, desc)
+    sc = build_scene(ctx, dict(desc, standard_bvh=standard_bvh))
     ref = refdrv.RefScene(desc)
     ref.build_accel()
+    accel = 2 if standard_bvh else 0
+    sc.query_accel(standard_bvh)
     rays = scenes.random_rays(desc, n, seed=seed, mode=mode)
     t, prim, b0, b1 = sc.intersect(rays)
-    rt, rprim, rb0, rb1 = ref.intersect(rays)
+    rt, rprim, rb0, rb1 = ref.intersect(rays, accel=accel)
     miss = rprim < 0
-    # ties within 1e-6 in t may legitimately pick either triangle
     same = prim == rprim
+    if standard_bvh:
+        # the binary BVH visits triangles in the reference's own order, so
+        # even ties at shared edges resolve to the same triangle
+        assert np.array_equal(prim, rprim), "prim mismatch at %s" % np.nonzero(~same)[0][:10]
+    # ties within 1e-6 in t may legitimately pick either triangle
     with np.errstate(invalid="ignore"):
         tie = np.abs(t - rt) <= 1e-6 * np.maximum(1.0, np.abs(rt))
     assert np.all(same | tie), \
@@ -36,7 +42,7 @@ def _compare(desc, ctx, n, seed, mode="mixed"):
     seg[:, 6] = 1e-4
     seg[:, 7] = np.where(miss, 10.0, rt * 0.999)
     v = sc.occluded(seg)
-    rv = ref.visibility(seg)
+    rv = ref.visibility(seg, accel=accel)
     assert np.array_equal(v, rv)
     return float((~miss).mean())
 
@@ -53,6 +59,37 @@ def test_cornell_axis_aligned_rays(gpu_ctx):
 def test_icosphere_intersect_parity(gpu_ctx):
     frac = _compare(scenes.cornell_icosphere(64, 64, subdiv=5), gpu_ctx, 200000, seed=5)
     assert frac > 0.5
+
+
+@pytest.mark.parametrize("which", ["cornell", "icosphere", "icosphere_axis"])
+def test_standard_bvh_ray_batch_parity(gpu_ctx, which):
+    """Closest hits and visibility through the binned-SAH binary BVH against
+    the reference's StandardBVH(objs, BinnedSAH) traversal
+    (libSLR/Accelerator/StandardBVH.h:266-328): primitive IDs, t and surface
+    parameters bit-exact, ties included."""
+    if which == "cornell":
+        _compare(scenes.cornell_box(64, 64), gpu_ctx, 200000, seed=13, standard_bvh=True)
+    else:
+        _compare(scenes.cornell_icosphere(64, 64, subdiv=5), gpu_ctx, 200000, seed=15,
+                 mode="axis" if which.endswith("axis") else "mixed", standard_bvh=True)
+
+
+def test_standard_bvh_query_switch(gpu_ctx):
+    """Switching the query accelerator back returns the QBVH's hits; a scene
+    built without the binary BVH refuses the switch."""
+    desc = scenes.cornell_box(32, 32)
+    sc = build_scene(gpu_ctx, dict(desc, standard_bvh=True))
+    rays = scenes.random_rays(desc, 10000, seed=21)
+    q = sc.intersect(rays)
+    sc.query_accel(True)
+    b = sc.intersect(rays)
+    sc.query_accel(False)
+    q2 = sc.intersect(rays)
+    assert all(np.array_equal(x, y) for x, y in zip(q, q2))
+    assert np.array_equal(q[1] < 0, b[1] < 0)
+    plain = build_scene(gpu_ctx, desc)
+    with pytest.raises(Exception):
+        plain.query_accel(True)
 
 
 def test_empty_batch(gpu_ctx):

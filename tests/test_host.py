@@ -113,3 +113,41 @@ def test_stat_key_lists_match_the_header():
     assert len(slr_amd.STAT_KEYS) == n_render == 25
     assert len(slr_amd.SERVER_STAT_KEYS) == n_server == 8
     assert len(set(slr_amd.STAT_KEYS)) == len(slr_amd.STAT_KEYS)
+
+
+def _triangle_soup(n, seed, duplicates=False):
+    """Random triangles in one mesh (duplicated triangles make centroid
+    ranges collapse, which takes the builder's median split)."""
+    rng = np.random.RandomState(seed)
+    c = rng.uniform(-1, 1, size=(n, 3))
+    pos = (c[:, None, :] + rng.normal(scale=0.05, size=(n, 3, 3))).reshape(-1, 3)
+    if duplicates:
+        pos[3 * (n // 2):] = pos[:3 * (n - n // 2)]
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    nrm = np.tile([0.0, 0.0, 1.0], (3 * n, 1))
+    return {"materials": [{"kind": "lambert", "rgb": (0.5, 0.5, 0.5)}],
+            "meshes": [{"pos": pos.astype(np.float32), "nrm": nrm.astype(np.float32),
+                        "tan": np.tile([1.0, 0.0, 0.0], (3 * n, 1)).astype(np.float32), "idx": idx, "mat": 0}],
+            "camera": scenes.cornell_box(8, 8)["camera"]}
+
+
+@pytest.mark.parametrize("which", ["cornell", "icosphere", "soup", "soup_duplicates"])
+def test_standard_bvh_matches_reference_bit_for_bit(host_ctx, which):
+    """The binned-SAH binary BVH (StandardBVH(objs, BinnedSAH),
+    libSLR/Accelerator/StandardBVH.h:62-193) node for node: boxes, children,
+    leaf ranges, split axes, leaf order, depth and SAH cost."""
+    refdrv = need_ref()
+    desc = {"cornell": lambda: scenes.cornell_box(8, 8), "icosphere": lambda: scenes.cornell_icosphere(8, 8, subdiv=4),
+            "soup": lambda: _triangle_soup(3000, 1), "soup_duplicates": lambda: _triangle_soup(2000, 2, True)}[which]()
+    desc = dict(desc, standard_bvh=True)
+    sc = slr_amd.build_scene(host_ctx, desc)
+    boxes, info, refs, depth, cost = sc.standard_bvh_dump()
+    ref = refdrv.RefScene(desc)
+    ref.build_accel()
+    rb, ri, rr, rdepth, rcost = ref.standard_bvh_dump()
+    ref.close()
+    assert boxes.shape == rb.shape and len(refs) == len(rr)
+    assert np.array_equal(boxes.view(np.uint32), rb.view(np.uint32))
+    assert np.array_equal(info, ri)
+    assert np.array_equal(refs, rr)
+    assert depth == rdepth and np.float32(cost) == np.float32(rcost)
