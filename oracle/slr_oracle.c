@@ -271,7 +271,10 @@ static void diffuse_value(const bsdf_t* b, v3 wi, v3 wo, int sampled, float* fs)
         float f = on_factor(b, wi, wo, sampled);
         for (int i = 0; i < NL; ++i) fs[i] = b->A[i] * (float)(f / PI_D);
     } else {
-        for (int i = 0; i < NL; ++i) fs[i] = (float)(b->A[i] / PI_D);
+        /* m_R / M_PI: SampledSpectrum::operator/(float) multiplies by the
+           float reciprocal of (float)M_PI (spectrum_types.h:351-357) */
+        const float r = 1.0f / (float)PI_D;
+        for (int i = 0; i < NL; ++i) fs[i] = b->A[i] * r;
     }
 }
 
@@ -754,7 +757,9 @@ static void trace_sample(const scene_t* s, uint32_t st[4], uint32_t px, uint32_t
         if (pathLen > 0) {
             float p = fminf(importance(alpha, hero) / initY, 1.0f);
             if (orc_xs_float(st) < p) {
-                for (int k = 0; k < NL; ++k) alpha[k] /= p;
+                /* alpha /= continueProb multiplies by 1 / p (spectrum_types.h:390-395) */
+                const float r = 1.0f / p;
+                for (int k = 0; k < NL; ++k) alpha[k] *= r;
             } else break;
         }
         ++pathLen;

@@ -156,6 +156,10 @@ __device__ __forceinline__ float3 halfVector(const float3& a, const float3& b) {
 
 // ---------------------------------------------------------------- Lambert / Oren-Nayar
 
+// Lambert's m_R / M_PI: SampledSpectrum::operator/(float) multiplies by the
+// float reciprocal of (float)M_PI (libSLR/BasicTypes/spectrum_types.h:351-357)
+constexpr float kInvPiF = 1.0f / 3.14159265358979323846f;
+
 struct Diffuse {
     const float* R;   // 16 reflectance values
     bool orenNayar;
@@ -185,7 +189,7 @@ struct Diffuse {
         float sn = snCorrection(d, q.ng);
 #pragma unroll
         for (int i = 0; i < kNumL; ++i) fs[i] = (orenNayar ? R[i] * (float)(f / 3.14159265358979323846)
-                                                           : (float)(R[i] / 3.14159265358979323846)) * sn;
+                                                           : R[i] * kInvPiF) * sn;
         return true;
     }
     __device__ void evaluate(const BsdfQuery& q, const float3& d, float* fs, float* pdf) const {
@@ -199,7 +203,7 @@ struct Diffuse {
         float sn = snCorrection(d, q.ng);
 #pragma unroll
         for (int i = 0; i < kNumL; ++i) fs[i] = (orenNayar ? R[i] * (float)(f / 3.14159265358979323846)
-                                                           : (float)(R[i] / 3.14159265358979323846)) * sn;
+                                                           : R[i] * kInvPiF) * sn;
         *pdf = (float)(fabsf(d.z) / 3.14159265358979323846);
     }
 };

@@ -1112,6 +1112,20 @@ __device__ __forceinline__ void storePlanes(const PathState& ps, uint32_t s, uin
 }
 
 // wavelength i of a path: createWithEqualOffsets (libSLR/BasicTypes/spectrum_types.h:59-69)
+// x / y, correctly rounded, from r = 1 / y (correctly rounded): q = x r is
+// within an ulp of x / y, the remainder x - q y is exact in one FMA, and one
+// correction step rounds to the nearest quotient (Markstein's theorem) as
+// long as the remainder does not underflow. Checked against IEEE division
+// for every float x at y = 470 whose quotient is above 1e-30; on random
+// pairs it differs only where the remainder is subnormal. Spectrum lookups
+// divide wavelength offsets of at least an ulp of 360 nm by spans of
+// nanometres, far from that range: one division per spectrum instead of one
+// per wavelength, with the bits of x / y.
+__device__ __forceinline__ float quotient(float x, float y, float r) {
+    const float q = x * r;
+    return fmaf(fmaf(-q, y, x), r, q);
+}
+
 __device__ __forceinline__ float lambdaOf(int i, float off) {
     return kLambdaLowD + (kLambdaHighD - kLambdaLowD) * (i + off) / kNumL;
 }
@@ -1122,10 +1136,11 @@ __device__ __forceinline__ void evalSpectrumAt(const DevScene& sc, float lo, flo
     const DevSpectrum sp{lo, hi, offset, n};
     const float* v = sc.specPool + sp.offset;
     const float span = sp.hi - sp.lo;
+    const float rspan = 1.0f / span;
     const float nm1 = (float)(sp.n - 1);
 #pragma unroll
     for (int i = 0; i < kNumL; ++i) {
-        float binF = (lambdaOf(i, off) - sp.lo) / span * nm1;
+        float binF = quotient(lambdaOf(i, off) - sp.lo, span, rspan) * nm1;
         float r;
         if (binF <= 0.0f) r = v[0];
         else if (binF >= nm1) r = v[sp.n - 1];
@@ -1192,7 +1207,8 @@ __device__ __forceinline__ void envTexelSpectrum(const DevScene& sc, uint32_t te
     }
 #pragma unroll
     for (int i = 0; i < kNumL; ++i) {
-        float p = (lambdaOf(i, off) - kLambdaLowD) / (kLambdaHighD - kLambdaLowD);
+        float p = quotient(lambdaOf(i, off) - kLambdaLowD, kLambdaHighD - kLambdaLowD,
+                           1.0f / (kLambdaHighD - kLambdaLowD));
         p = fminf(fmaxf(p, 0.0f), 1.0f);
         const float binF = p * 94;
         const uint32_t bin = min((uint32_t)binF, 94u);
@@ -1597,7 +1613,8 @@ __device__ __forceinline__ void fileSample(const PathState& ps, uint32_t s, uint
 #pragma unroll
     for (int k = 0; k < kNumL; ++k) {
         float l = lambdaOf(k, misc.x);
-        uint32_t b = min((uint32_t)((l - kLambdaLowD) / (kLambdaHighD - kLambdaLowD) * kNumL), (uint32_t)(kNumL - 1));
+        const float pos = quotient(l - kLambdaLowD, kLambdaHighD - kLambdaLowD, 1.0f / (kLambdaHighD - kLambdaLowD));
+        uint32_t b = min((uint32_t)(pos * kNumL), (uint32_t)(kNumL - 1));
         const float v = c[k] * recBinWidth;
         add[k] += b == (uint32_t)k ? v : 0.0f;
         if (k + 1 < kNumL) add[k + 1] += b == (uint32_t)(k + 1) ? v : 0.0f;
@@ -2153,8 +2170,11 @@ void k_shade(DevScene sc, PathState ps) {
                 if (pathLen > 0) {
                     const float p = fminf(importance16(alpha, hero) / misc.y, 1.0f);
                     if (xsFloat(rng) < p) {
+                        // alpha /= continueProb multiplies by 1 / p
+                        // (libSLR/BasicTypes/spectrum_types.h:390-395)
+                        const float rp = 1.0f / p;
 #pragma unroll
-                        for (int k = 0; k < kNumL; ++k) alpha[k] /= p;
+                        for (int k = 0; k < kNumL; ++k) alpha[k] *= rp;
                     } else alive = false;
                 }
                 if (alive) {
@@ -2609,8 +2629,9 @@ __global__ __launch_bounds__(kBlock) void k_shade_medium(DevScene sc, PathState 
         if (pathLen > 0) {
             const float p = fminf(importance16(alpha, hero) / misc.y, 1.0f);
             if (xsFloat(rng) < p) {
+                const float rp = 1.0f / p;   // alpha /= continueProb (spectrum_types.h:390-395)
 #pragma unroll
-                for (int k = 0; k < kNumL; ++k) alpha[k] /= p;
+                for (int k = 0; k < kNumL; ++k) alpha[k] *= rp;
             } else alive = false;
         }
         if (alive) {
