@@ -74,6 +74,8 @@ struct DevSpectrum {
     float lo, hi;
     uint32_t offset, n;
 };
+// LDS words a shading block gives to the spectrum pool and material headers
+constexpr uint32_t kShadeLdsWords = 4096;
 
 struct DevCamera {
     float l2w[12];        // rows of the 3x4 camera-to-world matrix
@@ -112,6 +114,10 @@ struct DevScene {
     uint32_t numPrims;
     float lightImportance;    // SurfaceObjectAggregate::importance()
     uint32_t numNodes;
+    // the shading kernels' LDS copy of the spectrum pool followed by the
+    // material spectrum headers: specLdsPool words of pool (a multiple of 4),
+    // specLdsWords in all; 0 when they do not fit in kShadeLdsWords
+    uint32_t specLdsPool, specLdsWords;
     // instances: kInstXfRows float4 each: 3 rows object-to-world (affine,
     // row-major, translation in w), then the 4 rows of world-to-object
     const float4* instXf;

@@ -374,6 +374,7 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         specs.push_back(d);
     }
     if (pool.empty()) pool.push_back(0.0f);
+    while (pool.size() % 4) pool.push_back(0.0f);   // whole float4s for the shading kernels' LDS copy
     // per material the headers of its spectra, read beside the material so
     // evaluating one needs no lookup of the header by id
     std::vector<DevSpectrum> matSpec(std::max<size_t>(1, 3 * sd.materials.size()));
@@ -538,6 +539,11 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     v.numLights = (uint32_t)(cdf.size() - 1);
     v.numPrims = (uint32_t)nt;
     v.numNodes = (uint32_t)nn;
+    {
+        const size_t words = pool.size() + 4 * matSpec.size();
+        v.specLdsPool = words <= kShadeLdsWords ? (uint32_t)pool.size() : 0u;
+        v.specLdsWords = words <= kShadeLdsWords ? (uint32_t)words : 0u;
+    }
     ds->hasCamera = sd.hasCamera;
     ds->camera = sd.camera;
     if (sd.hasCamera) cameraDerived(sd.camera, 1, 1, &v.cam);
@@ -1131,26 +1137,29 @@ __device__ __forceinline__ float lambdaOf(int i, float off) {
 }
 
 // RegularContinuousSpectrum::evaluate (libSLR/BasicTypes/spectrum_types.cpp:82-100)
-__device__ __forceinline__ void evalSpectrumAt(const DevScene& sc, float lo, float hi, uint32_t offset, uint32_t n,
-                                               float off, float* out) {
-    const DevSpectrum sp{lo, hi, offset, n};
-    const float* v = sc.specPool + sp.offset;
-    const float span = sp.hi - sp.lo;
+// at the path's 16 wavelengths, from the spectrum's n samples v over
+// [lo, hi]: the first or last sample outside the range, else the linear
+// interpolation of the two samples around the wavelength. Two loads per
+// wavelength whichever case applies (v may be global memory or LDS).
+__device__ __forceinline__ void spectrumValues(const float* v, float lo, float hi, uint32_t n, float off,
+                                               float* out) {
+    const float span = hi - lo;
     const float rspan = 1.0f / span;
-    const float nm1 = (float)(sp.n - 1);
+    const float nm1 = (float)(n - 1);
+    const int32_t last = (int32_t)n - 1;
 #pragma unroll
     for (int i = 0; i < kNumL; ++i) {
-        float binF = quotient(lambdaOf(i, off) - sp.lo, span, rspan) * nm1;
-        float r;
-        if (binF <= 0.0f) r = v[0];
-        else if (binF >= nm1) r = v[sp.n - 1];
-        else {
-            int32_t b = (int32_t)binF;
-            float t = binF - b;
-            r = (1 - t) * v[b] + t * v[b + 1];
-        }
-        out[i] = r;
+        const float binF = quotient(lambdaOf(i, off) - lo, span, rspan) * nm1;
+        const bool below = binF <= 0.0f, above = binF >= nm1;
+        const int32_t b = below ? 0 : above ? last : (int32_t)binF;
+        const float v0 = v[b], v1 = v[min(b + 1, last)];
+        const float t = binF - b;
+        out[i] = (below || above) ? v0 : (1 - t) * v0 + t * v1;
     }
+}
+__device__ __forceinline__ void evalSpectrumAt(const DevScene& sc, float lo, float hi, uint32_t offset, uint32_t n,
+                                               float off, float* out) {
+    spectrumValues(sc.specPool + offset, lo, hi, n, off, out);
 }
 __device__ __forceinline__ void evalSpectrum(const DevScene& sc, int32_t id, float off, float* out) {
     const DevSpectrum h = sc.spectra[id];
@@ -2069,6 +2078,34 @@ void k_shade(DevScene sc, PathState ps) {
         }
         for (uint32_t j = threadIdx.x; j < kMaxJobs; j += blockDim.x) ps.jobActive[j] = 0;
     }
+    // the spectrum pool and the material spectrum headers, copied into LDS
+    // when they fit: a spectrum evaluation is 32 scattered reads per path
+    // vertex, which the vector memory path serves a lane at a time
+    __shared__ float4 specLds4[kShadeLdsWords / 4];
+    const float* specLds = (const float*)specLds4;
+    const bool inLds = CLS != CLS_MISS && sc.specLdsWords != 0;
+    if (inLds) {
+        const uint32_t poolQuads = sc.specLdsPool / 4, quads = sc.specLdsWords / 4;
+        const float4* pool4 = (const float4*)sc.specPool;
+        const float4* hdr4 = (const float4*)sc.matSpec;
+        for (uint32_t j = threadIdx.x; j < quads; j += blockDim.x)
+            specLds4[j] = j < poolQuads ? pool4[j] : hdr4[j - poolQuads];
+        __syncthreads();
+    }
+    auto spectrumAt = [&](float lo, float hi, uint32_t offset, uint32_t n, float off, float* out) {
+        if (inLds) spectrumValues(specLds + offset, lo, hi, n, off, out);
+        else spectrumValues(sc.specPool + offset, lo, hi, n, off, out);
+    };
+    // spectrum j (0: spec0, 1: spec1, 2: emittance) of material m
+    auto matSpectrum = [&](uint32_t m, uint32_t j, float off, float* out) {
+        if (inLds) {
+            const float4 h = specLds4[sc.specLdsPool / 4 + 3 * m + j];
+            spectrumValues(specLds + __float_as_uint(h.z), h.x, h.y, __float_as_uint(h.w), off, out);
+        } else {
+            const DevSpectrum h = sc.matSpec[3 * m + j];
+            spectrumValues(sc.specPool + h.offset, h.lo, h.hi, h.n, off, out);
+        }
+    };
     unsigned long long samples = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
         const uint32_t i = base + threadIdx.x;
@@ -2140,7 +2177,7 @@ void k_shade(DevScene sc, PathState ps) {
                 // emission (PTRenderer.cpp:162-166 first hit, :247-263 implicit with MIS)
                 if (emits) {
                     float Le[kNumL];
-                    evalMatSpectrum(sc, sp.mat, 2, misc.x, Le);
+                    matSpectrum(sp.mat, 2, misc.x, Le);
                     float misW = 1.0f;
                     if (pathLen > 0 && !(info.y & F_LAST_DELTA)) {
                         const float bsdfPDF = misc.z;
@@ -2193,9 +2230,12 @@ void k_shade(DevScene sc, PathState ps) {
                     const uint32_t matId = sp.mat;
                     const int32_t spec0 = mat.spec0, spec1 = mat.spec1;
                     const float wlOff = misc.x;
-                    auto materialSpectra = [&sc, matId, spec0, spec1, wlOff](float* A, float* B) {
+                    auto materialSpectra = [&, matId, spec0, spec1, wlOff](float* A, float* B) {
                         if (CLS == CLS_DIFFUSE) {
-                            evalMatSpectrum(sc, matId, 0, wlOff, A);
+                            matSpectrum(matId, 0, wlOff, A);
+                        } else if (inLds) {
+                            matSpectrum(matId, 0, wlOff, A);
+                            matSpectrum(matId, 1, wlOff, B);
                         } else {
                             // (through the material's spectrum ids: the header table
                             // here leaves an unused stack object in these kernels)
@@ -2291,8 +2331,8 @@ void k_shade(DevScene sc, PathState ps) {
                             tangentNaN(divs(lp - sp.p, dist), dist * (1 - 1e-4f));
                         } else if (cosS * cosL > 0 && lz > 0.0f) {
                             float M[kNumL], fs[kNumL];
-                            evalSpectrumAt(sc, L.emit.x, L.emit.y, __float_as_uint(L.emit.z),
-                                           __float_as_uint(L.emit.w), misc.x, M);
+                            spectrumAt(L.emit.x, L.emit.y, __float_as_uint(L.emit.z), __float_as_uint(L.emit.w),
+                                       misc.x, M);
                             const float3 sdL = toLocal(sp, sdir);
                             float bpdf = 0.0f;
                             float A[kNumL], B[kNumL];
