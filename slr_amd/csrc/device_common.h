@@ -118,6 +118,7 @@ struct DevScene {
     // material spectrum headers: specLdsPool words of pool (a multiple of 4),
     // specLdsWords in all; 0 when they do not fit in kShadeLdsWords
     uint32_t specLdsPool, specLdsWords;
+    uint32_t classMask;   // bit c: some material shades with class c (misses always)
     // instances: kInstXfRows float4 each: 3 rows object-to-world (affine,
     // row-major, translation in w), then the 4 rows of world-to-object
     const float4* instXf;

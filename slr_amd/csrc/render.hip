@@ -539,6 +539,8 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
     v.numLights = (uint32_t)(cdf.size() - 1);
     v.numPrims = (uint32_t)nt;
     v.numNodes = (uint32_t)nn;
+    v.classMask = 1u << CLS_MISS;
+    for (const DevMaterial& m : sd.materials) v.classMask |= 1u << m.cls;
     {
         const size_t words = pool.size() + 4 * matSpec.size();
         v.specLdsPool = words <= kShadeLdsWords ? (uint32_t)pool.size() : 0u;
@@ -3019,6 +3021,12 @@ struct SlotPool {
     std::vector<uint32_t> hostBlockJob;
     uint32_t* hostCtr = nullptr;   // pinned: kNumCtr counters, then kMaxJobs active counts
     hipStream_t stream = nullptr;
+    // the surface shading kernels other than the diffuse one run beside it,
+    // on these streams (forked after classification, joined before the last
+    // compaction); every stream has the default priority
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    bool parallelShade = true;
     hipGraphExec_t graph = nullptr;
     int graphVariant = -1;
     uint32_t graphMode = 0;     // volume | tangentNeeNaN << 1, as the graph was recorded
@@ -3028,6 +3036,11 @@ struct SlotPool {
     ~SlotPool() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (stream) (void)hipStreamDestroy(stream);
+        for (int i = 0; i < 3; ++i) {
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+            if (join[i]) (void)hipEventDestroy(join[i]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
         if (hostCtr) (void)hipHostFree(hostCtr);
     }
 };
@@ -3063,6 +3076,12 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
         return SLRH_ERR_NOMEM_CODE;
     }
     HIP_TRY(hipStreamCreate(&p->stream));
+    HIP_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+    for (int i = 0; i < 3; ++i) {
+        HIP_TRY(hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&p->join[i], hipEventDisableTiming));
+    }
+    if (const char* e = getenv("SLRH_SERIAL_SHADE")) p->parallelShade = e[0] != '1';
     HIP_TRY(hipGetDevice(&p->device));
     p->n = capacity;
     p->blocks = blocks;
@@ -3198,12 +3217,42 @@ static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
         if (ev) (void)hipEventRecord((*ev)[6], s);
     }
     hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
+    // one shading kernel per class the scene has (each reads only its own
+    // queue and slots; the diffuse one also resets the counters that the
+    // last compaction rebuilds, so it always runs); the others on side
+    // streams when parallel, else in order on the main stream
+    const uint32_t mask = p.sc.classMask;
+    const int nside = p.parallelShade ? 3 : 0;
+    if (nside) (void)hipEventRecord(p.fork, s);
+    int used = 0;
+    auto shadeOn = [&]() -> hipStream_t {
+        if (!nside) return s;
+        hipStream_t q = p.side[used];
+        (void)hipStreamWaitEvent(q, p.fork, 0);
+        return q;
+    };
+    auto joinSide = [&](hipStream_t q) {
+        if (q != s) (void)hipEventRecord(p.join[used++], q);
+    };
+    if (mask & (1u << CLS_CONDUCTOR)) {
+        hipStream_t q = shadeOn();
+        hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(p.gridShade[CLS_CONDUCTOR]), dim3(kBlock), 0, q, p.sc, p.ps);
+        joinSide(q);
+    }
+    if (mask & (1u << CLS_DIELECTRIC)) {
+        hipStream_t q = shadeOn();
+        hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(p.gridShade[CLS_DIELECTRIC]), dim3(kBlock), 0, q, p.sc, p.ps);
+        joinSide(q);
+    }
+    {
+        hipStream_t q = shadeOn();
+        hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(p.gridShade[CLS_MISS]), dim3(kBlock), 0, q, p.sc, p.ps);
+        joinSide(q);
+    }
     hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(p.gridShade[CLS_DIFFUSE]), dim3(kBlock), 0, s, p.sc, p.ps);
-    hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(p.gridShade[CLS_CONDUCTOR]), dim3(kBlock), 0, s, p.sc, p.ps);
-    hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(p.gridShade[CLS_DIELECTRIC]), dim3(kBlock), 0, s, p.sc, p.ps);
-    hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(p.gridShade[CLS_MISS]), dim3(kBlock), 0, s, p.sc, p.ps);
     if (p.sc.volume)
         hipLaunchKernelGGL(k_shade_medium, dim3(p.gridShade[CLS_MEDIUM]), dim3(kBlock), 0, s, p.sc, p.ps);
+    for (int i = 0; i < used; ++i) (void)hipStreamWaitEvent(s, p.join[i], 0);
     if (ev) (void)hipEventRecord((*ev)[3], s);
     hipLaunchKernelGGL(k_compact<0>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
     if (ev) (void)hipEventRecord((*ev)[4], s);
