@@ -300,7 +300,10 @@ enum slrh_integrator { SLRH_PT = 0, SLRH_VOL_PT = 1 };
 
 enum slrh_render_flags {
     SLRH_FLAG_TIME_KERNELS = 1,   /* time the kernel groups with HIP events (stats 5, 8, 9) */
-    SLRH_FLAG_NO_GRAPH = 2        /* launch each iteration's kernels instead of replaying a recorded graph */
+    SLRH_FLAG_NO_GRAPH = 2,       /* launch each iteration's kernels instead of replaying a recorded graph */
+    SLRH_FLAG_PARALLEL_SHADE = 4  /* run the conductor, dielectric and miss shading kernels on forked
+                                     streams beside the diffuse one (graph branches); off by default:
+                                     replayed graphs with these branches measured slower than in order */
 };
 
 typedef struct slrh_render_settings {

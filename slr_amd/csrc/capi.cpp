@@ -945,6 +945,7 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     if (cfg.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     cfg.useGraph = (set->flags & SLRH_FLAG_NO_GRAPH) == 0;
     cfg.volumetric = set->integrator == SLRH_VOL_PT;
+    cfg.parallelShade = (set->flags & SLRH_FLAG_PARALLEL_SHADE) != 0;
     RenderStats st;
     std::string err;
     int r = slrh::render(s->dev, cfg, film->dev, &st, &err);
@@ -1025,6 +1026,7 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     opt.useGraph = (set->flags & SLRH_FLAG_NO_GRAPH) == 0;
     opt.traceVariant = set->tune_trace == 0 ? -1 : (int)set->tune_trace - 1;
     opt.volumetric = set->integrator == SLRH_VOL_PT;
+    opt.parallelShade = (set->flags & SLRH_FLAG_PARALLEL_SHADE) != 0;
     if (opt.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     RenderStats st;

@@ -82,6 +82,7 @@ struct RenderConfig {
     bool volumetric = false;   // VolumetricPTRenderer: the medium takes part
     int traceVariant = 0;   // traversal kernel variant (see traceVariant), -1: autotuned
     bool useGraph = true;
+    bool parallelShade = false;
 };
 
 struct RenderStats {
@@ -148,6 +149,7 @@ struct RenderOptions {
     bool useGraph = true;         // replay recorded iteration blocks (ignored when timing kernels)
     int traceVariant = 0;         // -1: chosen by the autotuner
     bool volumetric = false;      // VolumetricPTRenderer: media scatter and attenuate
+    bool parallelShade = false;   // other shading classes on forked streams beside the diffuse kernel
 };
 // The variant the autotuner chose for the scene (0 before any tuning).
 int tunedTraceVariant(const DeviceScene* ds);

@@ -3026,10 +3026,10 @@ struct SlotPool {
     // compaction); every stream has the default priority
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
-    bool parallelShade = true;
+    bool parallelShade = false;   // set per render (RenderOptions::parallelShade)
     hipGraphExec_t graph = nullptr;
     int graphVariant = -1;
-    uint32_t graphMode = 0;     // volume | tangentNeeNaN << 1, as the graph was recorded
+    uint32_t graphMode = 0;     // volume | tangentNeeNaN << 1 | parallelShade << 2, as the graph was recorded
     PathState ps;
     DevScene sc;
     JobRun runs[kMaxJobs];
@@ -3081,7 +3081,6 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
         HIP_TRY(hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&p->join[i], hipEventDisableTiming));
     }
-    if (const char* e = getenv("SLRH_SERIAL_SHADE")) p->parallelShade = e[0] != '1';
     HIP_TRY(hipGetDevice(&p->device));
     p->n = capacity;
     p->blocks = blocks;
@@ -3262,7 +3261,7 @@ static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
 // The kernels take their sizes from device counters, so one recording
 // serves every replay.
 static int launchGraphBlock(SlotPool& p, int tv, std::string* err) {
-    const uint32_t mode = p.sc.volume | p.sc.tangentNeeNaN << 1;
+    const uint32_t mode = p.sc.volume | p.sc.tangentNeeNaN << 1 | (p.parallelShade ? 4u : 0u);
     if (!p.graph || p.graphVariant != tv || p.graphMode != mode) {
         if (p.graph) { (void)hipGraphExecDestroy(p.graph); p.graph = nullptr; }
         hipGraph_t g;
@@ -3373,10 +3372,11 @@ int poolSamples(SlotPool* p, double* samples, double* rays, std::string* err) {
 }
 
 // Live paths the wavefront aims for: the traversal's per-launch tail of long
-// rays is amortised over enough rays (measured on the 1080p/64 spp
-// workload: 2M paths 258 ms per frame, 8M 224 ms, 16M 208 ms); path state is
-// 384 B per slot, 6.4 GB at 16M.
-constexpr uint32_t kTargetSlots = 16u << 20;
+// rays and each launch's fixed costs are amortised over enough rays
+// (measured on the 1080p/64 spp workload with 2, 4, 8 and 16 slots per
+// pixel, i.e. 4M to 33M paths: 164, 138, 125 and 119 ms per frame); path
+// state is 354 B per slot, 11.7 GB at 33M.
+constexpr uint32_t kTargetSlots = 32u << 20;
 
 // Traversal autotuner. For each variant a pilot renders one sample per slot
 // of every job of the render (their cameras and resolutions, as many
@@ -3484,6 +3484,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
     // roulette only) unless the settings set one
     p.sc.volume = opt.volumetric && ds->view.hasMedium ? 1u : 0u;
     p.sc.tangentNeeNaN = opt.volumetric && ds->view.compatTangentNee ? 1u : 0u;
+    p.parallelShade = opt.parallelShade;
     std::vector<JobSpec> jobSpecs(specs, specs + n);
     if (opt.volumetric)
         for (JobSpec& js : jobSpecs)
@@ -3598,6 +3599,7 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
     opt.timeKernels = cfg.timeKernels;
     opt.traceVariant = cfg.traceVariant;
     opt.useGraph = cfg.useGraph;
+    opt.parallelShade = cfg.parallelShade;
     opt.volumetric = cfg.volumetric;
     return renderJobs(ds, &js, 1, opt, st, err);
 }
