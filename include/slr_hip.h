@@ -301,9 +301,11 @@ enum slrh_integrator { SLRH_PT = 0, SLRH_VOL_PT = 1 };
 enum slrh_render_flags {
     SLRH_FLAG_TIME_KERNELS = 1,   /* time the kernel groups with HIP events (stats 5, 8, 9) */
     SLRH_FLAG_NO_GRAPH = 2,       /* launch each iteration's kernels instead of replaying a recorded graph */
-    SLRH_FLAG_PARALLEL_SHADE = 4  /* run the conductor, dielectric and miss shading kernels on forked
+    SLRH_FLAG_PARALLEL_SHADE = 4, /* run the conductor, dielectric and miss shading kernels on forked
                                      streams beside the diffuse one (graph branches); off by default:
                                      replayed graphs with these branches measured slower than in order */
+    SLRH_FLAG_SHADE_BLOCK_128 = 8 /* surface shading kernels in 128-thread workgroups; without it they
+                                     use the autotuner's choice (tune_trace 0) or 256 */
 };
 
 typedef struct slrh_render_settings {
@@ -317,8 +319,11 @@ typedef struct slrh_render_settings {
     uint32_t tune_trace;      /* traversal kernel variant + 1; 0 = automatic: the first render
                                  of a scene with a pool of >= 2^20 slots renders one pilot
                                  sample per slot with each variant (scratch film) and keeps
-                                 the fastest per traced ray for every later render of the
-                                 scene (variant 0 until then) */
+                                 the fastest per traced ray, then pilots the shading
+                                 kernels' workgroup sizes (256, 128) with that variant and
+                                 keeps the faster per shaded vertex, for every later render
+                                 of the scene (variant 0 and 256 until then); neither
+                                 choice changes the film */
     uint32_t reserved;        /* must be 0 */
 } slrh_render_settings;
 
@@ -361,7 +366,7 @@ SLRH_API int slrh_film_bmp(const slrh_film* f, float scale, float sensitivity, v
 
 /* Renders into `film` (accumulating onto what it holds). Replaces
  * PTRenderer::render (libSLR/Renderer/PTRenderer.cpp:26-101). stats, if not
- * NULL, receives 25 doubles: [0] rays traced, [1] samples, [2] device seconds,
+ * NULL, receives 26 doubles: [0] rays traced, [1] samples, [2] device seconds,
  * [3] non-finite samples dropped, [4] wavefront iterations, [5] traversal
  * kernel seconds (extension and shadow rays traced together), [6] traversal
  * launches, [7] extension rays, [8] shading kernel seconds, [9] seconds of
@@ -373,7 +378,9 @@ SLRH_API int slrh_film_bmp(const slrh_film* f, float scale, float sensitivity, v
  * (tentative collisions), [20] majorant cells entered, [21] rays tracked
  * and [22] tracking walks stopped by the step guard (2^16 steps; expected 0),
  * [23] QBVH nodes read from the traversal's LDS copy of the top of the tree
- * (included in [11] and [13]), [24] instances entered by the traversal.
+ * (included in [11] and [13]), [24] instances entered by the traversal,
+ * [25] workgroup size of the surface shading kernels (chosen by the
+ * autotuner with tune_trace 0).
  * Kernel seconds are 0 unless flags bit 0. */
 SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
                          double* stats);

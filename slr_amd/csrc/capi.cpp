@@ -911,6 +911,7 @@ int slrh_trace_variants(int* n, int* ldsDepth, int* refill, int* fused) {
 }
 
 namespace {
+constexpr int kNumRenderStats = 26;   // the doubles slrh_render documents
 void copyStats(const RenderStats& st, double* stats) {
     if (!stats) return;
     stats[0] = st.raysTraced; stats[1] = st.samples; stats[2] = st.kernelSeconds; stats[3] = st.nonFinite;
@@ -925,6 +926,7 @@ void copyStats(const RenderStats& st, double* stats) {
     stats[22] = st.mediumGuard;
     stats[23] = st.nodesLds;
     stats[24] = st.instEntered;
+    stats[25] = st.shadeBlock;
 }
 }  // namespace
 
@@ -946,6 +948,7 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     cfg.useGraph = (set->flags & SLRH_FLAG_NO_GRAPH) == 0;
     cfg.volumetric = set->integrator == SLRH_VOL_PT;
     cfg.parallelShade = (set->flags & SLRH_FLAG_PARALLEL_SHADE) != 0;
+    cfg.shadeShape = (set->flags & SLRH_FLAG_SHADE_BLOCK_128) ? 1 : -1;
     RenderStats st;
     std::string err;
     int r = slrh::render(s->dev, cfg, film->dev, &st, &err);
@@ -961,21 +964,22 @@ int slrh_render_progressive(slrh_scene* s, const slrh_render_settings* set, slrh
     // passes of 1, 1, 2, 4, ... samples per pixel (the last cut to spp), so
     // the film holds 2^k samples per pixel after pass k, where
     // PTRenderer::render saves its snapshots (PTRenderer.cpp:77-94)
-    double sum[25] = {0};
+    double sum[kNumRenderStats] = {0};
     uint32_t done = 0, pass = 0;
     while (done < set->spp) {
         slrh_render_settings ps = *set;
         ps.spp = std::min(set->spp - done, std::max(1u, done));
         ps.seed = set->seed + 0x9E3779B9u * pass;
-        double st[25] = {0};
+        double st[kNumRenderStats] = {0};
         int r = slrh_render(s, &ps, film, st);
         if (r) return r;
-        for (int k = 0; k < 25; ++k) sum[k] += st[k];
+        for (int k = 0; k < kNumRenderStats; ++k) sum[k] += st[k];
         done += ps.spp;
         ++pass;
         if (snapshot && snapshot(user, film, done) != 0) break;
     }
     sum[15] /= pass;   // the traversal variant, not a sum
+    sum[25] /= pass;   // the shading workgroup size
     if (stats) memcpy(stats, sum, sizeof(sum));
     return SLRH_OK;
 }
@@ -1027,6 +1031,7 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     opt.traceVariant = set->tune_trace == 0 ? -1 : (int)set->tune_trace - 1;
     opt.volumetric = set->integrator == SLRH_VOL_PT;
     opt.parallelShade = (set->flags & SLRH_FLAG_PARALLEL_SHADE) != 0;
+    opt.shadeShape = (set->flags & SLRH_FLAG_SHADE_BLOCK_128) ? 1 : -1;
     if (opt.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     RenderStats st;

@@ -83,6 +83,7 @@ struct RenderConfig {
     int traceVariant = 0;   // traversal kernel variant (see traceVariant), -1: autotuned
     bool useGraph = true;
     bool parallelShade = false;
+    int shadeShape = -1;
 };
 
 struct RenderStats {
@@ -92,6 +93,7 @@ struct RenderStats {
     double nodesVisited = 0, trisTested = 0;   // extension rays
     double nodesShadow = 0, trisShadow = 0;    // shadow rays
     double overflow = 0;
+    double shadeBlock = 0;   // workgroup size of the surface shading kernels
     double traceVariant = 0;   // traversal variant the render ran with
     double mediumEvents = 0, mediumShadows = 0;   // scattering events, light samples through the medium
     double mediumSeconds = 0;                     // k_medium time (with kernel timing)
@@ -150,6 +152,7 @@ struct RenderOptions {
     int traceVariant = 0;         // -1: chosen by the autotuner
     bool volumetric = false;      // VolumetricPTRenderer: media scatter and attenuate
     bool parallelShade = false;   // other shading classes on forked streams beside the diffuse kernel
+    int shadeShape = -1;          // shading workgroup shape index; -1: tuned (automatic) or the first
 };
 // The variant the autotuner chose for the scene (0 before any tuning).
 int tunedTraceVariant(const DeviceScene* ds);

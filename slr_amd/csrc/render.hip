@@ -137,6 +137,7 @@ struct DeviceScene {
     std::shared_ptr<struct SlotPool> pool;   // wavefront state kept between renders
     // traversal variant chosen by the autotuner (on a pool of tunedSlots slots)
     int tunedVariant = -1;
+    int tunedShape = 0;   // index into kShadeShapes
     uint32_t tunedSlots = 0;
 };
 
@@ -2064,8 +2065,9 @@ __device__ __forceinline__ bool sampleBsdf(const DevMaterial& m, const float* A,
 #ifndef SLRH_SHADE_MIN_WAVES
 #define SLRH_SHADE_MIN_WAVES 1
 #endif
-template <uint32_t CLS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
+// BS is the workgroup size (the autotuner picks one of kShadeShapes).
+template <uint32_t CLS, uint32_t BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
     CLS == CLS_DIFFUSE ? SLRH_DIFFUSE_WAVES : SLRH_SHADE_MIN_WAVES, CLS == CLS_DIFFUSE ? SLRH_DIFFUSE_WAVES : 10)))
 void k_shade(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_CLS0 + CLS];
@@ -2109,7 +2111,7 @@ void k_shade(DevScene sc, PathState ps) {
         }
     };
     unsigned long long samples = 0;
-    for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+    for (uint32_t base = blockIdx.x * BS; base < count; base += gridDim.x * BS) {
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < count;
         uint8_t code = CODE_IDLE;
@@ -2995,6 +2997,9 @@ __global__ void k_film_reduce(const double* planes, uint32_t numPlanes, size_t p
 
 // ------------------------------------------------------------------ slot pool
 
+// Workgroup sizes of the surface shading kernels the autotuner chooses from.
+constexpr uint32_t kShadeShapes[2] = {256, 128};
+
 // A job in flight: its description, slot range and film planes.
 struct JobRun {
     DevCamera cam;
@@ -3012,6 +3017,8 @@ struct SlotPool {
     // grid of each grid-stride kernel: the blocks that can be resident at
     // once (more would wait for a second round and leave a tail)
     uint32_t gridShade[kNumClasses] = {}, gridRegen = 0, gridMedium = 0;
+    uint32_t gridShade1[kNumClasses] = {};   // the surface shading kernels at kShadeShapes[1]
+    int shadeShape = 0;                      // index into kShadeShapes, set per render
     int device = 0;
     DevBuf<float4> ray, spec, nee;
     DevBuf<uint32_t> q, ctr, fetch, spill, jobActive, blockJob;
@@ -3029,7 +3036,7 @@ struct SlotPool {
     bool parallelShade = false;   // set per render (RenderOptions::parallelShade)
     hipGraphExec_t graph = nullptr;
     int graphVariant = -1;
-    uint32_t graphMode = 0;     // volume | tangentNeeNaN << 1 | parallelShade << 2, as the graph was recorded
+    uint32_t graphMode = 0;     // volume | tangentNeeNaN << 1 | parallelShade << 2 | shadeShape << 3, as recorded
     PathState ps;
     DevScene sc;
     JobRun runs[kMaxJobs];
@@ -3051,9 +3058,10 @@ constexpr uint32_t kGraphIters = 8;
 
 // Blocks of kBlock threads of `kernel` that fit on the device at once
 // (registers and LDS as compiled), at most `cap`.
-static uint32_t residentGrid(const void* kernel, uint32_t cap) {
+static uint32_t residentGrid(const void* kernel, uint32_t cap, uint32_t blockSize = kBlock) {
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || per <= 0) return cap;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)blockSize, 0) != hipSuccess || per <= 0)
+        return cap;
     return std::max(1u, std::min(cap, (uint32_t)g_numCUs * (uint32_t)per));
 }
 
@@ -3084,11 +3092,18 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
     HIP_TRY(hipGetDevice(&p->device));
     p->n = capacity;
     p->blocks = blocks;
-    p->gridShade[CLS_DIFFUSE] = residentGrid((const void*)&k_shade<CLS_DIFFUSE>, blocks);
-    p->gridShade[CLS_CONDUCTOR] = residentGrid((const void*)&k_shade<CLS_CONDUCTOR>, blocks);
-    p->gridShade[CLS_DIELECTRIC] = residentGrid((const void*)&k_shade<CLS_DIELECTRIC>, blocks);
-    p->gridShade[CLS_MISS] = residentGrid((const void*)&k_shade<CLS_MISS>, blocks);
+    p->gridShade[CLS_DIFFUSE] = residentGrid((const void*)&k_shade<CLS_DIFFUSE, kShadeShapes[0]>, blocks);
+    p->gridShade[CLS_CONDUCTOR] = residentGrid((const void*)&k_shade<CLS_CONDUCTOR, kShadeShapes[0]>, blocks);
+    p->gridShade[CLS_DIELECTRIC] = residentGrid((const void*)&k_shade<CLS_DIELECTRIC, kShadeShapes[0]>, blocks);
+    p->gridShade[CLS_MISS] = residentGrid((const void*)&k_shade<CLS_MISS, kShadeShapes[0]>, blocks);
     p->gridShade[CLS_MEDIUM] = residentGrid((const void*)&k_shade_medium, blocks);
+    const uint32_t cap1 = blocks * (kBlock / kShadeShapes[1]);
+    p->gridShade1[CLS_DIFFUSE] = residentGrid((const void*)&k_shade<CLS_DIFFUSE, kShadeShapes[1]>, cap1, kShadeShapes[1]);
+    p->gridShade1[CLS_CONDUCTOR] =
+        residentGrid((const void*)&k_shade<CLS_CONDUCTOR, kShadeShapes[1]>, cap1, kShadeShapes[1]);
+    p->gridShade1[CLS_DIELECTRIC] =
+        residentGrid((const void*)&k_shade<CLS_DIELECTRIC, kShadeShapes[1]>, cap1, kShadeShapes[1]);
+    p->gridShade1[CLS_MISS] = residentGrid((const void*)&k_shade<CLS_MISS, kShadeShapes[1]>, cap1, kShadeShapes[1]);
     p->gridRegen = residentGrid((const void*)&k_regen, blocks);
     p->gridMedium = residentGrid((const void*)&k_medium, blocks);
     p->hostBlockJob.assign(capacity / kCompactSlots, kNoJob);
@@ -3233,22 +3248,31 @@ static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
     auto joinSide = [&](hipStream_t q) {
         if (q != s) (void)hipEventRecord(p.join[used++], q);
     };
+    auto shade = [&](auto cls, hipStream_t q) {
+        constexpr uint32_t C = decltype(cls)::value;
+        if (p.shadeShape == 0)
+            hipLaunchKernelGGL((k_shade<C, kShadeShapes[0]>), dim3(p.gridShade[C]), dim3(kShadeShapes[0]), 0, q,
+                               p.sc, p.ps);
+        else
+            hipLaunchKernelGGL((k_shade<C, kShadeShapes[1]>), dim3(p.gridShade1[C]), dim3(kShadeShapes[1]), 0, q,
+                               p.sc, p.ps);
+    };
     if (mask & (1u << CLS_CONDUCTOR)) {
         hipStream_t q = shadeOn();
-        hipLaunchKernelGGL(k_shade<CLS_CONDUCTOR>, dim3(p.gridShade[CLS_CONDUCTOR]), dim3(kBlock), 0, q, p.sc, p.ps);
+        shade(std::integral_constant<uint32_t, CLS_CONDUCTOR>(), q);
         joinSide(q);
     }
     if (mask & (1u << CLS_DIELECTRIC)) {
         hipStream_t q = shadeOn();
-        hipLaunchKernelGGL(k_shade<CLS_DIELECTRIC>, dim3(p.gridShade[CLS_DIELECTRIC]), dim3(kBlock), 0, q, p.sc, p.ps);
+        shade(std::integral_constant<uint32_t, CLS_DIELECTRIC>(), q);
         joinSide(q);
     }
     {
         hipStream_t q = shadeOn();
-        hipLaunchKernelGGL(k_shade<CLS_MISS>, dim3(p.gridShade[CLS_MISS]), dim3(kBlock), 0, q, p.sc, p.ps);
+        shade(std::integral_constant<uint32_t, CLS_MISS>(), q);
         joinSide(q);
     }
-    hipLaunchKernelGGL(k_shade<CLS_DIFFUSE>, dim3(p.gridShade[CLS_DIFFUSE]), dim3(kBlock), 0, s, p.sc, p.ps);
+    shade(std::integral_constant<uint32_t, CLS_DIFFUSE>(), s);
     if (p.sc.volume)
         hipLaunchKernelGGL(k_shade_medium, dim3(p.gridShade[CLS_MEDIUM]), dim3(kBlock), 0, s, p.sc, p.ps);
     for (int i = 0; i < used; ++i) (void)hipStreamWaitEvent(s, p.join[i], 0);
@@ -3261,7 +3285,8 @@ static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
 // The kernels take their sizes from device counters, so one recording
 // serves every replay.
 static int launchGraphBlock(SlotPool& p, int tv, std::string* err) {
-    const uint32_t mode = p.sc.volume | p.sc.tangentNeeNaN << 1 | (p.parallelShade ? 4u : 0u);
+    const uint32_t mode = p.sc.volume | p.sc.tangentNeeNaN << 1 | (p.parallelShade ? 4u : 0u) |
+                          (uint32_t)p.shadeShape << 3;
     if (!p.graph || p.graphVariant != tv || p.graphMode != mode) {
         if (p.graph) { (void)hipGraphExecDestroy(p.graph); p.graph = nullptr; }
         hipGraph_t g;
@@ -3292,6 +3317,7 @@ int poolOpen(DeviceScene* ds, uint32_t capacity, SlotPool** out, std::string* er
     int r = poolCreate(ds, capacity, &p, err);
     if (r) return r;
     if ((r = poolReset(*p, err))) return r;
+    p->shadeShape = ds->tunedShape;
     *out = p.release();
     return 0;
 }
@@ -3406,12 +3432,16 @@ static int tuneTrace(DeviceScene* ds, SlotPool& p, const JobSpec* specs, uint32_
     }
     std::vector<hipEvent_t> evs;
     auto cleanup = [&]() { for (hipEvent_t e : evs) (void)hipEventDestroy(e); };
-    double best = 0;
-    int bestV = 0;
-    for (int v = 0; v < kNumVariants; ++v) {
-        if ((r = poolReset(p, err))) { cleanup(); return r; }
+    // one pilot with traversal variant v and shading shape sh: kernel
+    // milliseconds of the traversal and of the shading group, traced rays
+    // and extension rays (one shaded vertex each)
+    struct Pilot { double traceMs = 0, shadeMs = 0, rays = 0, ext = 0; };
+    auto pilot = [&](int v, int sh, Pilot* out) -> int {
+        int r;
+        p.shadeShape = sh;
+        if ((r = poolReset(p, err))) return r;
         for (uint32_t i = 0; i < n; ++i)
-            if ((r = poolAdmit(&p, i, pilots[i], planes[i], bases[i], err))) { cleanup(); return r; }
+            if ((r = poolAdmit(&p, i, pilots[i], planes[i], bases[i], err))) return r;
         hipLaunchKernelGGL(k_compact<0>, dim3(compactBlocks(p.n)), dim3(256), 0, p.stream, p.ps, p.n / kCompactSlots);
         std::vector<IterEvents> itEv;
         for (;;) {
@@ -3423,27 +3453,48 @@ static int tuneTrace(DeviceScene* ds, SlotPool& p, const JobSpec* specs, uint32_
                 }
                 launchIteration(p, v, &itEv.back());
             }
-            if ((r = readCounters(p, err))) { cleanup(); return r; }
+            if ((r = readCounters(p, err))) return r;
             if (p.hostCtr[C_OVERFLOW] || (p.hostCtr[C_DONE] == 0 && p.hostCtr[C_PENDING] == 0)) break;
         }
         for (uint32_t i = 0; i < n; ++i)
-            if ((r = finishJob(p, p.runs[i], err))) { cleanup(); return r; }
+            if ((r = finishJob(p, p.runs[i], err))) return r;
         HIP_TRY(hipStreamSynchronize(p.stream));
         unsigned long long stat[kNumStat];
-        if ((r = readStats(&p, stat, err))) { cleanup(); return r; }
-        double ms = 0;
+        if ((r = readStats(&p, stat, err))) return r;
+        *out = Pilot();
         for (auto& e : itEv) {
             float t = 0;
             (void)hipEventElapsedTime(&t, e[1], e[2]);
-            ms += t;
+            out->traceMs += t;
+            (void)hipEventElapsedTime(&t, e[2], e[3]);
+            out->shadeMs += t;
         }
-        const double rays = (double)(stat[S_RAYS_EXT] + stat[S_RAYS_SHADOW]);
-        const double perRay = rays > 0 ? ms / rays : 0;
-        if (v == 0 || perRay < best) { best = perRay; bestV = v; }
+        out->rays = (double)(stat[S_RAYS_EXT] + stat[S_RAYS_SHADOW]);
+        out->ext = (double)stat[S_RAYS_EXT];
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
         evs.clear();
+        return 0;
+    };
+    // traversal variants by traversal time per traced ray
+    double best = 0;
+    int bestV = 0;
+    for (int v = 0; v < kNumVariants; ++v) {
+        Pilot pl;
+        if ((r = pilot(v, 0, &pl))) { cleanup(); return r; }
+        const double perRay = pl.rays > 0 ? pl.traceMs / pl.rays : 0;
+        if (v == 0 || perRay < best) { best = perRay; bestV = v; }
+    }
+    // shading workgroup shape by shading time per shaded vertex
+    int bestS = 0;
+    double bestShade = 0;
+    for (int sh = 0; sh < 2; ++sh) {
+        Pilot pl;
+        if ((r = pilot(bestV, sh, &pl))) { cleanup(); return r; }
+        const double perVertex = pl.ext > 0 ? pl.shadeMs / pl.ext : 0;
+        if (sh == 0 || perVertex < bestShade) { bestShade = perVertex; bestS = sh; }
     }
     ds->tunedVariant = bestV;
+    ds->tunedShape = bestS;
     ds->tunedSlots = p.n;
     return 0;
 }
@@ -3500,6 +3551,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
             if ((r = tuneTrace(ds, p, specs, n, planes.data(), bases.data(), err))) return r;
         tv = ds->tunedVariant >= 0 ? ds->tunedVariant : 0;
     }
+    p.shadeShape = opt.shadeShape >= 0 ? opt.shadeShape : opt.traceVariant < 0 ? ds->tunedShape : 0;
     if ((r = poolReset(p, err))) return r;
 
     std::vector<hipEvent_t> evs;
@@ -3576,6 +3628,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
         st->otherSeconds = ot * 1e-3;
         st->traceLaunches = (double)itEv.size();
         st->traceVariant = tv;
+        st->shadeBlock = kShadeShapes[p.shadeShape];
     }
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     if (p.hostCtr[C_OVERFLOW]) return SLRH_ERR_CAPACITY_CODE;
@@ -3600,6 +3653,7 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
     opt.traceVariant = cfg.traceVariant;
     opt.useGraph = cfg.useGraph;
     opt.parallelShade = cfg.parallelShade;
+    opt.shadeShape = cfg.shadeShape;
     opt.volumetric = cfg.volumetric;
     return renderJobs(ds, &js, 1, opt, st, err);
 }

@@ -159,3 +159,29 @@ def test_parallel_shading_gives_the_same_film(gpu_ctx, graph):
         r.render(sc, b, seed=seed, slots_per_pixel=2, graph=graph)
         assert np.array_equal(a.strata().view(np.uint64), b.strata().view(np.uint64))
         assert a.strata().sum() > 0
+
+
+def test_shading_workgroup_size_keeps_the_film(gpu_ctx):
+    """The shading kernels' workgroup size (256 or 128 threads, one of the
+    autotuner's dimensions) changes which lanes shade which slots, never a
+    slot's arithmetic: films are bit-identical (all shading classes)."""
+    W, H = 64, 40
+    desc = scenes.instanced_field(W, H, n_instances=24, subdiv=3, terrain_n=24, knot=(40, 8),
+                                  env_size=(64, 32), sun_radiance=200.0)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    r = slr_amd.PTRenderer(4)
+    a, b = slr_amd.Film(gpu_ctx, W, H), slr_amd.Film(gpu_ctx, W, H)
+    sa = r.render(sc, a, seed=9, slots_per_pixel=2, tune_trace=1)
+    sb = r.render(sc, b, seed=9, slots_per_pixel=2, tune_trace=1, shade_block_128=True)
+    assert sa["shade_block"] == 256 and sb["shade_block"] == 128
+    assert np.array_equal(a.strata().view(np.uint64), b.strata().view(np.uint64))
+
+
+def test_autotuner_reports_its_shading_workgroup(gpu_ctx):
+    W, H = 1024, 1024
+    desc = scenes.cornell_box(W, H)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    st = slr_amd.PTRenderer(1).render(sc, slr_amd.Film(gpu_ctx, W, H), seed=5, slots_per_pixel=1)
+    assert st["shade_block"] in (128, 256)
+    st2 = slr_amd.PTRenderer(1).render(sc, slr_amd.Film(gpu_ctx, W, H), seed=6, slots_per_pixel=1)
+    assert st2["shade_block"] == st["shade_block"]
