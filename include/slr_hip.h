@@ -304,8 +304,13 @@ enum slrh_render_flags {
     SLRH_FLAG_PARALLEL_SHADE = 4, /* run the conductor, dielectric and miss shading kernels on forked
                                      streams beside the diffuse one (graph branches); off by default:
                                      replayed graphs with these branches measured slower than in order */
-    SLRH_FLAG_SHADE_BLOCK_128 = 8 /* surface shading kernels in 128-thread workgroups; without it they
+    SLRH_FLAG_SHADE_BLOCK_128 = 8, /* surface shading kernels in 128-thread workgroups; without it they
                                      use the autotuner's choice (tune_trace 0) or 256 */
+    SLRH_FLAG_TUNE_POOL = 16      /* with slots_per_pixel 0: the wavefront width comes from a pilot
+                                     (first such render of the scene) that renders two samples per
+                                     slot at 8M, 16M and 32M live paths and keeps the most samples
+                                     per second; without it about 32M. Films depend on the slots per
+                                     pixel, so with this flag they depend on the pilot's timing */
 };
 
 typedef struct slrh_render_settings {
@@ -366,7 +371,7 @@ SLRH_API int slrh_film_bmp(const slrh_film* f, float scale, float sensitivity, v
 
 /* Renders into `film` (accumulating onto what it holds). Replaces
  * PTRenderer::render (libSLR/Renderer/PTRenderer.cpp:26-101). stats, if not
- * NULL, receives 26 doubles: [0] rays traced, [1] samples, [2] device seconds,
+ * NULL, receives 27 doubles: [0] rays traced, [1] samples, [2] device seconds,
  * [3] non-finite samples dropped, [4] wavefront iterations, [5] traversal
  * kernel seconds (extension and shadow rays traced together), [6] traversal
  * launches, [7] extension rays, [8] shading kernel seconds, [9] seconds of
@@ -380,7 +385,7 @@ SLRH_API int slrh_film_bmp(const slrh_film* f, float scale, float sensitivity, v
  * [23] QBVH nodes read from the traversal's LDS copy of the top of the tree
  * (included in [11] and [13]), [24] instances entered by the traversal,
  * [25] workgroup size of the surface shading kernels (chosen by the
- * autotuner with tune_trace 0).
+ * autotuner with tune_trace 0), [26] slots (live paths) per pixel.
  * Kernel seconds are 0 unless flags bit 0. */
 SLRH_API int slrh_render(slrh_scene* s, const slrh_render_settings* settings, slrh_film* film,
                          double* stats);

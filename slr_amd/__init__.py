@@ -80,6 +80,7 @@ FLAG_TIME_KERNELS = 1
 FLAG_NO_GRAPH = 2
 FLAG_PARALLEL_SHADE = 4
 FLAG_SHADE_BLOCK_128 = 8
+FLAG_TUNE_POOL = 16
 
 
 class _Settings(ctypes.Structure):
@@ -517,7 +518,7 @@ STAT_KEYS = ["rays", "samples", "device_seconds", "nonfinite", "iterations", "tr
              "extend_rays", "shade_seconds", "other_seconds", "shadow_rays", "nodes_visited", "tris_tested",
              "nodes_shadow", "tris_shadow", "trace_variant", "medium_events", "medium_shadows", "medium_seconds",
              "medium_lookups", "medium_cells", "medium_rays", "medium_guard", "nodes_lds", "inst_entered",
-             "shade_block"]
+             "shade_block", "slots_per_pixel"]
 
 
 def _fill_frame(fr, f, default_spp):
@@ -546,9 +547,10 @@ class PTRenderer:
     integrator = 0   # SLRH_PT
 
     def render(self, scene, film, seed=1509761209, max_depth=0, slots_per_pixel=0, time_kernels=False,
-               tune_trace=0, graph=True, parallel_shade=False, shade_block_128=False):
+               tune_trace=0, graph=True, parallel_shade=False, shade_block_128=False, tune_pool=False):
         flags = (FLAG_TIME_KERNELS if time_kernels else 0) | (0 if graph else FLAG_NO_GRAPH) | \
-            (FLAG_PARALLEL_SHADE if parallel_shade else 0) | (FLAG_SHADE_BLOCK_128 if shade_block_128 else 0)
+            (FLAG_PARALLEL_SHADE if parallel_shade else 0) | (FLAG_SHADE_BLOCK_128 if shade_block_128 else 0) | \
+            (FLAG_TUNE_POOL if tune_pool else 0)
         s = _Settings(film.width, film.height, self.spp, max_depth, seed & 0xFFFFFFFF, self.integrator,
                       slots_per_pixel, flags, tune_trace, 0)
         st = np.zeros(len(STAT_KEYS))
@@ -569,14 +571,15 @@ class PTRenderer:
         return dict(zip(STAT_KEYS, st.tolist()))
 
     def render_batch(self, scene, frames, max_depth=0, slots_per_pixel=0, time_kernels=False, tune_trace=0,
-                     graph=True, parallel_shade=False, shade_block_128=False):
+                     graph=True, parallel_shade=False, shade_block_128=False, tune_pool=False):
         """frames: list of dicts {film, camera: dict(l2w, aspect, fov_y, lens_radius, img_dist, obj_dist,
         sensitivity), seed, spp (default self.spp)}; all frames in one wavefront."""
         arr = (_Frame * max(1, len(frames)))()
         for i, f in enumerate(frames):
             _fill_frame(arr[i], f, self.spp)
         flags = (FLAG_TIME_KERNELS if time_kernels else 0) | (0 if graph else FLAG_NO_GRAPH) | \
-            (FLAG_PARALLEL_SHADE if parallel_shade else 0) | (FLAG_SHADE_BLOCK_128 if shade_block_128 else 0)
+            (FLAG_PARALLEL_SHADE if parallel_shade else 0) | (FLAG_SHADE_BLOCK_128 if shade_block_128 else 0) | \
+            (FLAG_TUNE_POOL if tune_pool else 0)
         s = _Settings(0, 0, self.spp, max_depth, 0, self.integrator, slots_per_pixel, flags, tune_trace, 0)
         st = np.zeros(len(STAT_KEYS))
         scene.ctx.check(scene.lib.slrh_render_batch(scene.h, arr, len(frames), ctypes.byref(s), _ptr(st)),

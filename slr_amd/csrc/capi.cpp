@@ -911,7 +911,7 @@ int slrh_trace_variants(int* n, int* ldsDepth, int* refill, int* fused) {
 }
 
 namespace {
-constexpr int kNumRenderStats = 26;   // the doubles slrh_render documents
+constexpr int kNumRenderStats = 27;   // the doubles slrh_render documents
 void copyStats(const RenderStats& st, double* stats) {
     if (!stats) return;
     stats[0] = st.raysTraced; stats[1] = st.samples; stats[2] = st.kernelSeconds; stats[3] = st.nonFinite;
@@ -927,6 +927,7 @@ void copyStats(const RenderStats& st, double* stats) {
     stats[23] = st.nodesLds;
     stats[24] = st.instEntered;
     stats[25] = st.shadeBlock;
+    stats[26] = st.slotsPerPixel;
 }
 }  // namespace
 
@@ -949,6 +950,7 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     cfg.volumetric = set->integrator == SLRH_VOL_PT;
     cfg.parallelShade = (set->flags & SLRH_FLAG_PARALLEL_SHADE) != 0;
     cfg.shadeShape = (set->flags & SLRH_FLAG_SHADE_BLOCK_128) ? 1 : -1;
+    cfg.tunePool = (set->flags & SLRH_FLAG_TUNE_POOL) != 0;
     RenderStats st;
     std::string err;
     int r = slrh::render(s->dev, cfg, film->dev, &st, &err);
@@ -980,6 +982,7 @@ int slrh_render_progressive(slrh_scene* s, const slrh_render_settings* set, slrh
     }
     sum[15] /= pass;   // the traversal variant, not a sum
     sum[25] /= pass;   // the shading workgroup size
+    sum[26] /= pass;   // slots per pixel
     if (stats) memcpy(stats, sum, sizeof(sum));
     return SLRH_OK;
 }
@@ -1032,6 +1035,7 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     opt.volumetric = set->integrator == SLRH_VOL_PT;
     opt.parallelShade = (set->flags & SLRH_FLAG_PARALLEL_SHADE) != 0;
     opt.shadeShape = (set->flags & SLRH_FLAG_SHADE_BLOCK_128) ? 1 : -1;
+    opt.tunePool = (set->flags & SLRH_FLAG_TUNE_POOL) != 0;
     if (opt.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     RenderStats st;

@@ -84,6 +84,7 @@ struct RenderConfig {
     bool useGraph = true;
     bool parallelShade = false;
     int shadeShape = -1;
+    bool tunePool = false;
 };
 
 struct RenderStats {
@@ -94,6 +95,7 @@ struct RenderStats {
     double nodesShadow = 0, trisShadow = 0;    // shadow rays
     double overflow = 0;
     double shadeBlock = 0;   // workgroup size of the surface shading kernels
+    double slotsPerPixel = 0;   // live paths per pixel of the wavefront
     double traceVariant = 0;   // traversal variant the render ran with
     double mediumEvents = 0, mediumShadows = 0;   // scattering events, light samples through the medium
     double mediumSeconds = 0;                     // k_medium time (with kernel timing)
@@ -153,6 +155,7 @@ struct RenderOptions {
     bool volumetric = false;      // VolumetricPTRenderer: media scatter and attenuate
     bool parallelShade = false;   // other shading classes on forked streams beside the diffuse kernel
     int shadeShape = -1;          // shading workgroup shape index; -1: tuned (automatic) or the first
+    bool tunePool = false;        // automatic slots per pixel from the pool-width pilot
 };
 // The variant the autotuner chose for the scene (0 before any tuning).
 int tunedTraceVariant(const DeviceScene* ds);

@@ -139,6 +139,7 @@ struct DeviceScene {
     int tunedVariant = -1;
     int tunedShape = 0;   // index into kShadeShapes
     uint32_t tunedSlots = 0;
+    uint32_t tunedPoolWidth = 0;   // live paths chosen by the pool-width pilot (0: not tuned)
 };
 
 struct DeviceFilm {
@@ -3501,6 +3502,53 @@ static int tuneTrace(DeviceScene* ds, SlotPool& p, const JobSpec* specs, uint32_
 
 int tunedTraceVariant(const DeviceScene* ds) { return ds->tunedVariant >= 0 ? ds->tunedVariant : 0; }
 
+// Slots per pixel for a wavefront of totalPixels aiming at `target` live
+// paths: at least 1, at most 16.
+static uint32_t slotsPerPixelFor(uint64_t target, uint64_t totalPixels) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(16, (target + totalPixels - 1) / totalPixels));
+}
+
+// Pool-width tuning (opt-in, RenderOptions::tunePool): for each candidate
+// wavefront width a pilot renders two samples per slot of every job of the
+// render into scratch films, and the width with the most samples per second
+// of device time is kept for the scene. Films depend on the slots per pixel
+// (each slot draws its own random stream), so a tuned width makes them
+// depend on this timing: hence opt-in.
+static const uint32_t kPoolWidths[3] = {8u << 20, 16u << 20, 32u << 20};
+
+static int tunePoolWidth(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOptions& opt,
+                         std::string* err) {
+    uint64_t totalPixels = 0;
+    for (uint32_t i = 0; i < n; ++i) totalPixels += (uint64_t)specs[i].width * specs[i].height;
+    std::vector<std::unique_ptr<DeviceFilm, void (*)(DeviceFilm*)>> scratch;
+    std::vector<JobSpec> pilots(specs, specs + n);
+    int r = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        DeviceFilm* f = nullptr;
+        if ((r = filmCreate(specs[i].width, specs[i].height, &f, err))) return r;
+        scratch.emplace_back(f, filmFree);
+        pilots[i].film = f;
+    }
+    double best = 0;
+    uint32_t bestW = kTargetSlots, lastK = 0;
+    for (uint32_t w : kPoolWidths) {
+        const uint32_t K = slotsPerPixelFor(w, totalPixels);
+        if (K == lastK) continue;   // the same wavefront as a smaller width
+        lastK = K;
+        for (uint32_t i = 0; i < n; ++i) pilots[i].spp = std::min(specs[i].spp, 2 * K);
+        RenderOptions po = opt;
+        po.tunePool = false;
+        po.slotsPerPixel = K;
+        po.timeKernels = false;
+        RenderStats ps;
+        if ((r = renderJobs(ds, pilots.data(), n, po, &ps, err))) return r;
+        const double rate = ps.kernelSeconds > 0 ? ps.samples / ps.kernelSeconds : 0;
+        if (rate > best) { best = rate; bestW = w; }
+    }
+    ds->tunedPoolWidth = bestW;
+    return 0;
+}
+
 int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOptions& opt, RenderStats* st,
                std::string* err) {
     if (n == 0) return 0;
@@ -3510,7 +3558,17 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
     for (uint32_t i = 0; i < n; ++i) totalPixels += (uint64_t)specs[i].width * specs[i].height;
     if (totalPixels == 0) return 0;
     uint32_t K = opt.slotsPerPixel;
-    if (K == 0) K = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(16, (kTargetSlots + totalPixels - 1) / totalPixels));
+    if (K == 0) {
+        uint32_t target = kTargetSlots;
+        if (opt.tunePool) {
+            if (ds->tunedPoolWidth == 0) {
+                const int r = tunePoolWidth(ds, specs, n, opt, err);
+                if (r) return r;
+            }
+            target = ds->tunedPoolWidth;
+        }
+        K = slotsPerPixelFor(target, totalPixels);
+    }
     uint64_t capacity = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const JobSpec& js = specs[i];
@@ -3629,6 +3687,7 @@ int renderJobs(DeviceScene* ds, const JobSpec* specs, uint32_t n, const RenderOp
         st->traceLaunches = (double)itEv.size();
         st->traceVariant = tv;
         st->shadeBlock = kShadeShapes[p.shadeShape];
+        st->slotsPerPixel = K;
     }
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     if (p.hostCtr[C_OVERFLOW]) return SLRH_ERR_CAPACITY_CODE;
@@ -3654,6 +3713,7 @@ int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderSta
     opt.useGraph = cfg.useGraph;
     opt.parallelShade = cfg.parallelShade;
     opt.shadeShape = cfg.shadeShape;
+    opt.tunePool = cfg.tunePool;
     opt.volumetric = cfg.volumetric;
     return renderJobs(ds, &js, 1, opt, st, err);
 }

@@ -185,3 +185,26 @@ def test_autotuner_reports_its_shading_workgroup(gpu_ctx):
     assert st["shade_block"] in (128, 256)
     st2 = slr_amd.PTRenderer(1).render(sc, slr_amd.Film(gpu_ctx, W, H), seed=6, slots_per_pixel=1)
     assert st2["shade_block"] == st["shade_block"]
+
+
+def test_pool_width_tuning(gpu_ctx):
+    """SLRH_FLAG_TUNE_POOL: the first render pilots 8M/16M/32M live paths
+    (8 or 16 slots per pixel at 1024^2) and keeps one for the scene; an
+    explicit slots-per-pixel setting gives the same film as the tuned one."""
+    W, H = 1024, 1024
+    desc = scenes.cornell_box(W, H)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    r = slr_amd.PTRenderer(32)
+    a = slr_amd.Film(gpu_ctx, W, H)
+    st = r.render(sc, a, seed=4, max_depth=4, tune_pool=True)
+    k = int(st["slots_per_pixel"])
+    assert k in (8, 16)
+    assert st["samples"] == W * H * 32
+    st2 = r.render(sc, slr_amd.Film(gpu_ctx, W, H), seed=5, max_depth=4, tune_pool=True)
+    assert int(st2["slots_per_pixel"]) == k
+    b = slr_amd.Film(gpu_ctx, W, H)
+    r.render(sc, b, seed=4, max_depth=4, slots_per_pixel=k)
+    assert np.array_equal(a.strata().view(np.uint64), b.strata().view(np.uint64))
+    # without the flag: about 32M live paths
+    st3 = r.render(sc, slr_amd.Film(gpu_ctx, W, H), seed=6, max_depth=4)
+    assert int(st3["slots_per_pixel"]) == 16

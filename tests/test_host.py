@@ -110,7 +110,7 @@ def test_stat_key_lists_match_the_header():
     hdr = open(os.path.join(ROOT, "include", "slr_hip.h")).read()
     n_render = int(re.search(r"stats, if not\s*\*?\s*NULL, receives (\d+) doubles", hdr).group(1))
     n_server = int(re.search(r"stats\[(\d+)\]: completed requests", hdr).group(1))
-    assert len(slr_amd.STAT_KEYS) == n_render == 26
+    assert len(slr_amd.STAT_KEYS) == n_render == 27
     assert len(slr_amd.SERVER_STAT_KEYS) == n_server == 8
     assert len(set(slr_amd.STAT_KEYS)) == len(slr_amd.STAT_KEYS)
 
