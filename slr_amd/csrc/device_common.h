@@ -143,6 +143,8 @@ struct DevScene {
     const float4* medCorners;
     const float* medMajorant;   // maximum density per supervoxel (kMedSuperVoxel^3 lattice cells)
     float medMin[3], medMax[3]; // region
+    float medScale[3];          // (n - 1) / (max - min): lattice units per length
+    float medInvExt[3];         // 1 / (max - min)
     uint32_t medN[3], medCells[3];
     float medSigmaE;            // extinction per unit density (constant over wavelength)
     int32_t medSigmaS;          // scattering per unit density (spectrum id)

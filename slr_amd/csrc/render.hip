@@ -518,6 +518,9 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
             ve.medMax[a] = m.bmax[a];
             ve.medN[a] = m.n[a];
             ve.medCells[a] = m.cells[a];
+            // the device's own IEEE quotients, computed once
+            ve.medScale[a] = (float)(m.n[a] - 1) / (m.bmax[a] - m.bmin[a]);
+            ve.medInvExt[a] = 1.0f / (m.bmax[a] - m.bmin[a]);
         }
         ve.medSigmaE = m.sigmaEValue;
         ve.medSigmaS = m.sigmaS;
@@ -1121,21 +1124,22 @@ __device__ __forceinline__ void storePlanes(const PathState& ps, uint32_t s, uin
     for (int p = 0; p < 4; ++p) b[p] = make_float4(v[4 * p], v[4 * p + 1], v[4 * p + 2], v[4 * p + 3]);
 }
 
-// wavelength i of a path: createWithEqualOffsets (libSLR/BasicTypes/spectrum_types.h:59-69)
 // x / y, correctly rounded, from r = 1 / y (correctly rounded): q = x r is
 // within an ulp of x / y, the remainder x - q y is exact in one FMA, and one
 // correction step rounds to the nearest quotient (Markstein's theorem) as
 // long as the remainder does not underflow. Checked against IEEE division
 // for every float x at y = 470 whose quotient is above 1e-30; on random
-// pairs it differs only where the remainder is subnormal. Spectrum lookups
-// divide wavelength offsets of at least an ulp of 360 nm by spans of
-// nanometres, far from that range: one division per spectrum instead of one
-// per wavelength, with the bits of x / y.
+// pairs it differs only where the remainder is subnormal. The callers
+// divide many values by one divisor (wavelength offsets by a spectrum's
+// span, coordinates by the medium's extent, lattice distances by a ray's
+// slope), far from that range: one division per divisor instead of one per
+// value, with the bits of x / y.
 __device__ __forceinline__ float quotient(float x, float y, float r) {
     const float q = x * r;
     return fmaf(fmaf(-q, y, x), r, q);
 }
 
+// wavelength i of a path: createWithEqualOffsets (libSLR/BasicTypes/spectrum_types.h:59-69)
 __device__ __forceinline__ float lambdaOf(int i, float off) {
     return kLambdaLowD + (kLambdaHighD - kLambdaLowD) * (i + off) / kNumL;
 }
@@ -1306,9 +1310,11 @@ __device__ __forceinline__ float envAreaPdf(const DevScene& sc, float phi, float
 // trilinear between the lattice points around it, in the reference's term
 // order.
 __device__ __forceinline__ float mediumDensity(const DevScene& sc, float3 p) {
-    const float px = (p.x - sc.medMin[0]) / (sc.medMax[0] - sc.medMin[0]);
-    const float py = (p.y - sc.medMin[1]) / (sc.medMax[1] - sc.medMin[1]);
-    const float pz = (p.z - sc.medMin[2]) / (sc.medMax[2] - sc.medMin[2]);
+    // (p - min) / (max - min) per axis, the IEEE quotients by way of the
+    // region's reciprocal extents
+    const float px = quotient(p.x - sc.medMin[0], sc.medMax[0] - sc.medMin[0], sc.medInvExt[0]);
+    const float py = quotient(p.y - sc.medMin[1], sc.medMax[1] - sc.medMin[1], sc.medInvExt[1]);
+    const float pz = quotient(p.z - sc.medMin[2], sc.medMax[2] - sc.medMin[2], sc.medInvExt[2]);
     // outside the region (and any NaN coordinate) reads zero
     if (!(px >= 0 && py >= 0 && pz >= 0 && px <= 1 && py <= 1 && pz <= 1)) return 0.0f;
     const uint32_t nx = sc.medN[0], ny = sc.medN[1], nz = sc.medN[2];
@@ -1380,20 +1386,22 @@ struct MediumWalk {
         float tNext[3], tDelta[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const float scale = (float)(sc.medN[a] - 1) / (sc.medMax[a] - sc.medMin[a]);   // lattice units per length
+            const float scale = sc.medScale[a];   // lattice units per length
             const float l0 = (oa[a] + da[a] * t0 - sc.medMin[a]) * scale;
             int c = (int)floorf(l0 / kMedSuperVoxel);
             c = max(0, min((int)sc.medCells[a] - 1, c));
             cell[a] = c;
             const float dl = da[a] * scale;
+            // one division per axis; the two quotients by dl from it
+            const float rdl = 1.0f / dl;
             if (dl > 0.0f) {
                 step[a] = 1;
-                tNext[a] = t0 + ((float)((c + 1) * (int)kMedSuperVoxel) - l0) / dl;
-                tDelta[a] = kMedSuperVoxel / dl;
+                tNext[a] = t0 + quotient((float)((c + 1) * (int)kMedSuperVoxel) - l0, dl, rdl);
+                tDelta[a] = quotient(kMedSuperVoxel, dl, rdl);
             } else if (dl < 0.0f) {
                 step[a] = -1;
-                tNext[a] = t0 + ((float)(c * (int)kMedSuperVoxel) - l0) / dl;
-                tDelta[a] = -(float)kMedSuperVoxel / dl;
+                tNext[a] = t0 + quotient((float)(c * (int)kMedSuperVoxel) - l0, dl, rdl);
+                tDelta[a] = -quotient(kMedSuperVoxel, dl, rdl);
             } else {
                 step[a] = 0;
                 tNext[a] = INFINITY;
@@ -2492,10 +2500,32 @@ struct WaveFeeder {
 // as the walk ends, so a wave's lanes stay busy while walk lengths differ.
 // Stages: 0 idle, 1 the entry's first words requested, 2 a visible light
 // sample's ray requested, 3 walking.
-__global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
+// kLdsMajorant: the majorant grid (at most kMajorantLdsCells cells) is
+// copied into LDS by each block of BS threads, one block per CU, so a walk
+// step waits for an LDS read instead of a cache miss; otherwise it is read
+// from global memory.
+constexpr uint32_t kMajorantLdsCells = 32768;   // 128 KB: a 256^3 lattice in 8^3 supervoxels
+constexpr uint32_t kMediumLdsBlock = 1024;      // 16 waves per CU share one copy
+#ifndef SLRH_MEDIUM_INNER
+#define SLRH_MEDIUM_INNER 2
+#endif
+constexpr uint32_t kMediumInnerSteps = SLRH_MEDIUM_INNER;
+#ifndef SLRH_MEDIUM_BATCH
+#define SLRH_MEDIUM_BATCH 16
+#endif
+constexpr int kMediumBatch = SLRH_MEDIUM_BATCH;
+template <bool kLdsMajorant, uint32_t BS>
+__global__ __launch_bounds__(BS) void k_medium(DevScene sc, PathState ps) {
     const uint32_t count = ps.ctr[C_TRACE];
     const uint32_t* qin = queue(ps, Q_TRACE);
     uint32_t* cursors = ps.fetch + kFeedParts * kFetchStride;   // zeroed by the trace compaction
+    __shared__ float majLds[kLdsMajorant ? kMajorantLdsCells : 1];
+    if (kLdsMajorant) {
+        const uint32_t cells = sc.medCells[0] * sc.medCells[1] * sc.medCells[2];
+        for (uint32_t j = threadIdx.x; j < cells; j += BS) majLds[j] = sc.medMajorant[j];
+        __syncthreads();
+    }
+    auto majorant = [&](uint32_t cell) { return kLdsMajorant ? majLds[cell] : sc.medMajorant[cell]; };
     WaveFeeder feed;
     feed.init();
     unsigned long long events = 0, shadows = 0, rays = 0;
@@ -2508,6 +2538,20 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
     uint32_t slotSeed = 0;
     float T = 1.0f, mu = 0.0f;
     MediumWalk w;
+    // the end of a walk: the extension ray's stream, or the light sample's
+    // contribution scaled by its transmittance
+    auto finish = [&]() {
+        if (!shadow) {
+            ps.u4(s, SF_RNG) = rng;
+        } else if (T != 1.0f) {
+            float c[kNumL];
+            loadPlanes(ps, s, SF_NEE, c);
+#pragma unroll
+            for (int k = 0; k < kNumL; ++k) c[k] *= T;
+            storePlanes(ps, s, SF_NEE, c);
+        }
+        stage = 0;
+    };
     for (;;) {
         // lanes without a next entry claim one once 16 of them lack one, or
         // at once for an idle lane (every lane of the wave reaches the claim)
@@ -2520,103 +2564,111 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PathState ps) {
         }
         // an idle lane starts its next entry; its words are used from the
         // next iteration on
-        bool started = false;
+        bool fresh = false;
         if (stage == 0 && hasNext) {
             hasNext = false;
             s = entNext & ~kShadowEntry;
             shadow = (entNext & kShadowEntry) != 0;
             if (shadow) {
                 vis = ps.vis[s];   // an occluded light sample needs no transmittance
+                stage = 1;
             } else {
                 o4 = ps.f4(s, SF_RO);
                 d4 = ps.f4(s, SF_RD);
                 h = ps.f4(s, SF_HIT);
                 rng = ps.u4(s, SF_RNG);
+                stage = 2;
             }
-            stage = 1;
-            started = true;
+            fresh = true;
         }
         // wave-uniform exit: every lane idle, nothing claimed, queue drained
         if (__ballot(stage != 0 || hasNext) == 0ull) {
             if (feed.exhausted) break;
             continue;
         }
-        if (started) {
-        } else if (stage == 1) {
+        if (stage == 1 && !fresh) {
+            // a light sample: its segment once it is known to be unoccluded
+            if (vis == 0) {
+                stage = 0;
+            } else {
+                o4 = ps.f4(s, SF_RO);
+                d4 = ps.f4(s, SF_SD);
+                slotSeed = ps.u4(s, SF_JOB).z;
+                info = ps.u4(s, SF_INFO);
+                stage = 2;
+                fresh = true;
+            }
+        }
+        // The set-up and the collision test are long code paths that a wave
+        // runs for all its lanes at once: lanes that reach one wait until
+        // kMediumBatch lanes of the wave are there or no lane is walking, so
+        // the wave runs each path for many lanes instead of a few every
+        // iteration. A lane's own arithmetic and random draws are unchanged.
+        const bool walking = __ballot(stage == 3) != 0ull;
+        const bool setUp = !walking || __popcll(__ballot(stage == 2 && !fresh)) >= kMediumBatch;
+        if (stage == 2 && !fresh && setUp) {
+            // one code path for both kinds of entry: the extension ray up to
+            // its hit with the slot's stream, the light sample's segment with
+            // its own stream
+            float t1 = h.x;
             if (shadow) {
-                if (vis == 0) {
-                    stage = 0;
-                } else {
-                    o4 = ps.f4(s, SF_RO);
-                    d4 = ps.f4(s, SF_SD);
-                    slotSeed = ps.u4(s, SF_JOB).z;
-                    info = ps.u4(s, SF_INFO);
-                    stage = 2;
-                }
-            } else if (w.init(sc, xyz(o4), xyz(d4), o4.w, h.x, rng)) {
+                rng = shadowStream(slotSeed, info);
+                T = 1.0f;
+                t1 = d4.w;
+            }
+            if (w.init(sc, xyz(o4), xyz(d4), o4.w, t1, rng)) {
                 ++rays;
+                shadows += shadow ? 1 : 0;
                 ++mc.cells;
-                mu = sc.medSigmaE * sc.medMajorant[w.cellIndex(sc)];
+                mu = sc.medSigmaE * majorant(w.cellIndex(sc));
                 stage = 3;
             } else {
                 stage = 0;   // misses the medium: nothing drawn, nothing written
             }
-        } else if (stage == 2) {
-            rng = shadowStream(slotSeed, info);
-            T = 1.0f;
-            if (w.init(sc, xyz(o4), xyz(d4), o4.w, d4.w, rng)) {
-                ++rays;
-                ++shadows;
-                ++mc.cells;
-                mu = sc.medSigmaE * sc.medMajorant[w.cellIndex(sc)];
-                stage = 3;
-            } else {
-                stage = 0;
-            }
-        } else if (stage == 3) {
+        }
+        // walk steps up to the next tentative collision (stage 4)
+        for (uint32_t inner = 0; inner < kMediumInnerSteps && stage == 3; ++inner) {
             const uint32_t r = w.advance(sc, mu);
-            bool done = false;
             if (r == MediumWalk::kCollide) {
                 ++mc.lookups;
-                const float sigma = sc.medSigmaE * mediumDensity(sc, xyz(o4) + xyz(d4) * w.t);
-                if (!shadow) {
-                    // delta tracking (sampleFreePath)
-                    if (xsFloat(rng) < sigma / mu) {
-                        ps.f4(s, SF_HIT) = make_float4(w.t, __int_as_float(-2), 0.0f, 0.0f);
-                        ps.code[s] = (uint8_t)(CODE_CLS0 + CLS_MEDIUM);
-                        ++events;
-                        done = true;
-                    }
-                } else {
-                    // ratio tracking with Russian roulette (mediumTransmittance)
-                    T *= 1.0f - fminf(sigma / mu, 1.0f);
-                    if (T < 0.1f) {
-                        if (xsFloat(rng) < T) T = 1.0f;
-                        else {
-                            T = 0.0f;
-                            done = true;
-                        }
-                    }
-                }
-                if (!done) w.tau = -logf(1.0f - xsFloat(rng));
+                stage = 4;
             } else if (r == MediumWalk::kMoved) {
                 ++mc.cells;
-                mu = sc.medSigmaE * sc.medMajorant[w.cellIndex(sc)];
+                mu = sc.medSigmaE * majorant(w.cellIndex(sc));
             } else {
                 if (r == MediumWalk::kGuard) ++mc.guard;
-                done = true;
+                finish();
+            }
+        }
+        const bool stillWalking = __ballot(stage == 3) != 0ull;
+        const bool test = !stillWalking || __popcll(__ballot(stage == 4)) >= kMediumBatch;
+        if (stage == 4 && test) {
+            bool done = false;
+            const float sigma = sc.medSigmaE * mediumDensity(sc, xyz(o4) + xyz(d4) * w.t);
+            if (!shadow) {
+                // delta tracking (sampleFreePath)
+                if (xsFloat(rng) < sigma / mu) {
+                    ps.f4(s, SF_HIT) = make_float4(w.t, __int_as_float(-2), 0.0f, 0.0f);
+                    ps.code[s] = (uint8_t)(CODE_CLS0 + CLS_MEDIUM);
+                    ++events;
+                    done = true;
+                }
+            } else {
+                // ratio tracking with Russian roulette (mediumTransmittance)
+                T *= 1.0f - fminf(sigma / mu, 1.0f);
+                if (T < 0.1f) {
+                    if (xsFloat(rng) < T) T = 1.0f;
+                    else {
+                        T = 0.0f;
+                        done = true;
+                    }
+                }
             }
             if (done) {
-                if (!shadow) {
-                    ps.u4(s, SF_RNG) = rng;
-                } else if (T != 1.0f) {
-                    float c[kNumL];
-                    loadPlanes(ps, s, SF_NEE, c);
-#pragma unroll
-                    for (int k = 0; k < kNumL; ++k) c[k] *= T;
-                    storePlanes(ps, s, SF_NEE, c);
-                }
-                stage = 0;
+                finish();
+            } else {
+                w.tau = -logf(1.0f - xsFloat(rng));
+                stage = 3;
             }
         }
     }
@@ -3017,7 +3069,7 @@ struct SlotPool {
     uint32_t n = 0, blocks = 0;
     // grid of each grid-stride kernel: the blocks that can be resident at
     // once (more would wait for a second round and leave a tail)
-    uint32_t gridShade[kNumClasses] = {}, gridRegen = 0, gridMedium = 0;
+    uint32_t gridShade[kNumClasses] = {}, gridRegen = 0, gridMedium = 0, gridMediumLds = 0;
     uint32_t gridShade1[kNumClasses] = {};   // the surface shading kernels at kShadeShapes[1]
     int shadeShape = 0;                      // index into kShadeShapes, set per render
     int device = 0;
@@ -3106,7 +3158,9 @@ static int poolCreate(const DeviceScene* ds, uint32_t capacity, std::unique_ptr<
         residentGrid((const void*)&k_shade<CLS_DIELECTRIC, kShadeShapes[1]>, cap1, kShadeShapes[1]);
     p->gridShade1[CLS_MISS] = residentGrid((const void*)&k_shade<CLS_MISS, kShadeShapes[1]>, cap1, kShadeShapes[1]);
     p->gridRegen = residentGrid((const void*)&k_regen, blocks);
-    p->gridMedium = residentGrid((const void*)&k_medium, blocks);
+    p->gridMedium = residentGrid((const void*)&k_medium<false, kBlock>, blocks);
+    p->gridMediumLds = residentGrid((const void*)&k_medium<true, kMediumLdsBlock>, (uint32_t)g_numCUs * 2,
+                                    kMediumLdsBlock);
     p->hostBlockJob.assign(capacity / kCompactSlots, kNoJob);
     PathState& ps = p->ps;
     ps.n = capacity;
@@ -3228,7 +3282,11 @@ static void launchIteration(SlotPool& p, int tv, IterEvents* ev) {
     if (ev) (void)hipEventRecord((*ev)[2], s);
     if (p.sc.volume) {
         if (ev) (void)hipEventRecord((*ev)[5], s);
-        hipLaunchKernelGGL(k_medium, dim3(p.gridMedium), dim3(kBlock), 0, s, p.sc, p.ps);
+        if (p.sc.medCells[0] * p.sc.medCells[1] * p.sc.medCells[2] <= kMajorantLdsCells)
+            hipLaunchKernelGGL((k_medium<true, kMediumLdsBlock>), dim3(p.gridMediumLds), dim3(kMediumLdsBlock), 0, s,
+                               p.sc, p.ps);
+        else
+            hipLaunchKernelGGL((k_medium<false, kBlock>), dim3(p.gridMedium), dim3(kBlock), 0, s, p.sc, p.ps);
         if (ev) (void)hipEventRecord((*ev)[6], s);
     }
     hipLaunchKernelGGL(k_compact<2>, dim3(cblocks), dim3(256), 0, s, p.ps, groups);
