@@ -223,17 +223,22 @@ def cpu_baseline(desc, sample, rays_per_sample, gpu_matched_sps=None):
     return out
 
 
+# kernel choices pinned for profiling runs (--tune-trace, --shade-block-128):
+# the autotuner's choice can differ from one process to the next
+PIN = {}
+
+
 def run_frames(renderer, sc, film, steps, warmup, seed0=0, max_depth=0):
     """warmup + steps renders of one frame, kernels timed; summed stats and
     wall seconds of the timed steps."""
     for i in range(warmup):
         film.clear()
-        renderer.render(sc, film, seed=1000 + seed0 + i, max_depth=max_depth)
+        renderer.render(sc, film, seed=1000 + seed0 + i, max_depth=max_depth, **PIN)
     tot = {}
     t0 = time.perf_counter()
     for i in range(steps):
         film.clear()
-        st = renderer.render(sc, film, seed=seed0 + i, max_depth=max_depth, time_kernels=True)
+        st = renderer.render(sc, film, seed=seed0 + i, max_depth=max_depth, time_kernels=True, **PIN)
         for k, v in st.items():
             tot[k] = tot.get(k, 0.0) + v
     return tot, time.perf_counter() - t0
@@ -366,14 +371,14 @@ def batch_steps(sc, frames, renderer, max_depth, steps, warmup, slots_per_pixel=
     for i in range(warmup):
         for f in frames:
             f["film"].clear()
-        renderer.render_batch(sc, frames, max_depth=max_depth, slots_per_pixel=slots_per_pixel)
+        renderer.render_batch(sc, frames, max_depth=max_depth, slots_per_pixel=slots_per_pixel, **PIN)
     tot = {}
     t0 = time.perf_counter()
     for i in range(steps):
         for f in frames:
             f["film"].clear()
         st = renderer.render_batch(sc, frames, max_depth=max_depth, time_kernels=True,
-                                   slots_per_pixel=slots_per_pixel)
+                                   slots_per_pixel=slots_per_pixel, **PIN)
         for k, v in st.items():
             tot[k] = tot.get(k, 0.0) + v
         log("batch step %d: %.3fs device, %d rays, %d non-finite" % (i, st["device_seconds"], st["rays"],
@@ -522,6 +527,11 @@ def main():
     ap.add_argument("--rate", type=float, default=16.0, help="--config serve: request arrivals per second")
     ap.add_argument("--requests", type=int, default=48, help="--config serve: number of requests")
     ap.add_argument("--mix", default="full", choices=["full", "small"])
+    ap.add_argument("--tune-trace", type=int, default=0,
+                    help="traversal variant + 1 for the timed renders (0: the autotuner's choice)")
+    ap.add_argument("--shade-block-128", action="store_true",
+                    help="shading kernels in 128-thread workgroups (default: the autotuner's choice, or 256 "
+                         "with --tune-trace)")
     ap.add_argument("--slots-per-pixel", type=int, default=0,
                     help="--config c3/c4/batch8: live paths per pixel (0: the library's choice)")
     ap.add_argument("--cpu-sample", default="1920x1080x8")
@@ -530,6 +540,10 @@ def main():
                     help="first run one device-to-device copy of this many MiB (a kernel of known traffic that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
     args = ap.parse_args()
+    if args.tune_trace:
+        PIN["tune_trace"] = args.tune_trace
+    if args.shade_block_128:
+        PIN["shade_block_128"] = True
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (frames, requests or tiles would be "
