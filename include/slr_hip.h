@@ -7,8 +7,11 @@
  * (SLRH_OK or a negative error code); no C++ exception crosses the ABI;
  * slrh_last_error() describes the last failure on a context. Host buffers
  * are owned by the caller, device buffers by the library unless an entry
- * point says it takes device pointers. A context is safe to use from one
- * thread at a time per context.
+ * point says it takes device pointers. Calls on the scenes and films of a
+ * context may come from several threads: they serialise on the context's
+ * lock (a render's snapshot callback runs with it held by the rendering
+ * thread), and slrh_last_error() returns the calling thread's last failure
+ * on that context.
  *
  * Each declaration names the reference interface it replaces
  * (paths relative to the SLR checkout).

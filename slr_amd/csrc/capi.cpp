@@ -14,6 +14,7 @@
 #include <new>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "engine.h"
@@ -21,10 +22,12 @@
 
 using namespace slrh;
 
+// Calls on the scenes and films of one context serialise on its lock
+// (recursive: an entry point may call another); the last error is kept per
+// calling thread and context, like errno.
 struct slrh_context {
     int device = 0;
-    std::string lastError;
-    std::mutex mu;
+    std::recursive_mutex mu;
 };
 
 struct slrh_scene {
@@ -49,10 +52,17 @@ struct slrh_server {
 
 namespace {
 
+thread_local std::unordered_map<const slrh_context*, std::string> tlsLastError;
+
 int fail(slrh_context* c, int code, const std::string& msg) {
-    if (c) c->lastError = msg;
+    if (c) tlsLastError[c] = msg;
     return code;
 }
+
+typedef std::unique_lock<std::recursive_mutex> CtxLock;
+CtxLock lockOf(const slrh_context* c) { return c ? CtxLock(const_cast<slrh_context*>(c)->mu) : CtxLock(); }
+template <typename T>
+CtxLock lockOf(const T* obj) { return obj ? lockOf(obj->ctx) : CtxLock(); }
 
 // makeCoordinateSystem of a unit normal (libSLR/BasicTypes/Vector3D.h:61-67)
 void tangentOf(const float n[3], float t[3]) {
@@ -106,11 +116,16 @@ int slrh_create(int device, slrh_context** out) {
 }
 
 int slrh_destroy(slrh_context* ctx) {
+    tlsLastError.erase(ctx);
     delete ctx;
     return SLRH_OK;
 }
 
-const char* slrh_last_error(const slrh_context* ctx) { return ctx ? ctx->lastError.c_str() : "null context"; }
+const char* slrh_last_error(const slrh_context* ctx) {
+    if (!ctx) return "null context";
+    auto it = tlsLastError.find(ctx);
+    return it == tlsLastError.end() ? "" : it->second.c_str();
+}
 
 int slrh_scene_create(slrh_context* ctx, slrh_scene** out) {
     if (!ctx || !out) return SLRH_ERR_INVALID;
@@ -122,6 +137,7 @@ int slrh_scene_create(slrh_context* ctx, slrh_scene** out) {
 }
 
 int slrh_scene_destroy(slrh_scene* s) {
+    CtxLock lk = lockOf(s);
     if (!s) return SLRH_ERR_INVALID;
     if (s->dev) freeScene(s->dev);
     delete s;
@@ -129,6 +145,7 @@ int slrh_scene_destroy(slrh_scene* s) {
 }
 
 int slrh_spectrum_rgb(slrh_scene* s, int type, float r, float g, float b, int* id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (!id || (type != SLRH_SPECTRUM_REFLECTANCE && type != SLRH_SPECTRUM_LIGHT_SOURCE))
         return fail(s->ctx, SLRH_ERR_INVALID, "slrh_spectrum_rgb: bad type");
@@ -139,6 +156,7 @@ int slrh_spectrum_rgb(slrh_scene* s, int type, float r, float g, float b, int* i
 }
 
 int slrh_spectrum_regular(slrh_scene* s, float lo, float hi, const float* values, int n, int* id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (!id || !values || n < 1 || !(hi > lo) || (n > 1 && !(hi > lo)))
         return fail(s->ctx, SLRH_ERR_INVALID, "slrh_spectrum_regular: bad arguments");
@@ -150,6 +168,7 @@ int slrh_spectrum_regular(slrh_scene* s, float lo, float hi, const float* values
 }
 
 int slrh_spectrum_evaluate(const slrh_scene* s, int id, int n, const float* lambdas, float* out) {
+    CtxLock lk = lockOf(s);
     if (!s || !lambdas || !out || n < 0) return SLRH_ERR_INVALID;
     if (!validSpectrum(s, id)) return fail(s->ctx, SLRH_ERR_INVALID, "spectrum_evaluate: bad id");
     const HostSpectrum& h = s->data.spectra[id];
@@ -172,6 +191,7 @@ int slrh_cmf_strata(float* x, float* y, float* z, float* integral) {
 }
 
 int slrh_material_lambert(slrh_scene* s, int refl, int* id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (!id || !validSpectrum(s, refl)) return fail(s->ctx, SLRH_ERR_INVALID, "lambert: bad spectrum");
     DevMaterial m{};
@@ -180,6 +200,7 @@ int slrh_material_lambert(slrh_scene* s, int refl, int* id) {
 }
 
 int slrh_material_oren_nayar(slrh_scene* s, int refl, float sigma, int* id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (!id || !validSpectrum(s, refl) || !(sigma >= 0)) return fail(s->ctx, SLRH_ERR_INVALID, "oren_nayar: bad args");
     DevMaterial m{};
@@ -191,6 +212,7 @@ int slrh_material_oren_nayar(slrh_scene* s, int refl, float sigma, int* id) {
 }
 
 int slrh_material_ggx_conductor(slrh_scene* s, int eta, int k, float ax, float ay, int* id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (!id || !validSpectrum(s, eta) || !validSpectrum(s, k) || !(ax > 0) || !(ay > 0))
         return fail(s->ctx, SLRH_ERR_INVALID, "ggx_conductor: bad args");
@@ -200,6 +222,7 @@ int slrh_material_ggx_conductor(slrh_scene* s, int eta, int k, float ax, float a
 }
 
 int slrh_material_ggx_dielectric(slrh_scene* s, int etaExt, int etaInt, float ax, float ay, int* id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (!id || !validSpectrum(s, etaExt) || !validSpectrum(s, etaInt) || !(ax > 0) || !(ay > 0))
         return fail(s->ctx, SLRH_ERR_INVALID, "ggx_dielectric: bad args");
@@ -210,6 +233,7 @@ int slrh_material_ggx_dielectric(slrh_scene* s, int etaExt, int etaInt, float ax
 }
 
 int slrh_material_diffuse_emitter(slrh_scene* s, int base, int emittance, int* id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (!id || !validSpectrum(s, emittance) || base >= (int)s->data.materials.size() || base < -1)
         return fail(s->ctx, SLRH_ERR_INVALID, "diffuse_emitter: bad args");
@@ -343,6 +367,7 @@ extern "C" {
 
 int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm, const float* tan,
                   uint32_t ntris, const uint32_t* idx, const uint32_t* mat, int material, int* mesh_id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     int r = addMesh(s, nverts, pos, nrm, tan, ntris, idx, mat, material, -1);
     if (r) return r;
@@ -353,6 +378,7 @@ int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, const float*
 
 int slrh_prototype_add(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm, const float* tan,
                        uint32_t ntris, const uint32_t* idx, const uint32_t* mat, int material, int* proto_id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     SceneData& d = s->data;
     if (d.protos.size() >= (1u << 20)) return fail(s->ctx, SLRH_ERR_CAPACITY, "prototype_add: too many prototypes");
@@ -367,6 +393,7 @@ int slrh_prototype_add(slrh_scene* s, uint32_t nverts, const float* pos, const f
 }
 
 int slrh_instance_add(slrh_scene* s, int proto_id, const float* l2w, int material, int* inst_id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     SceneData& d = s->data;
     if (proto_id < 0 || (size_t)proto_id >= d.protos.size()) return fail(s->ctx, SLRH_ERR_INVALID, "instance_add: bad prototype");
@@ -397,6 +424,7 @@ int slrh_instance_add(slrh_scene* s, int proto_id, const float* l2w, int materia
 }
 
 int slrh_instance_get(const slrh_scene* s, int inst_id, float* l2w, float* w2l) {
+    CtxLock lk = lockOf(s);
     if (!s) return SLRH_ERR_INVALID;
     if (inst_id < 0 || (size_t)inst_id >= s->data.instances.size()) return SLRH_ERR_INVALID;
     const Instance& in = s->data.instances[inst_id];
@@ -407,6 +435,7 @@ int slrh_instance_get(const slrh_scene* s, int inst_id, float* l2w, float* w2l) 
 
 int slrh_medium_density_grid(slrh_scene* s, const float* bounds, const float* density, uint32_t nx, uint32_t ny,
                              uint32_t nz, int sigma_s, int sigma_e, float g, int* medium_id) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     SceneData& d = s->data;
     if (d.medium.present) return fail(s->ctx, SLRH_ERR_CAPACITY, "medium: one medium per scene");
@@ -445,6 +474,7 @@ int slrh_medium_density_grid(slrh_scene* s, const float* bounds, const float* de
 }
 
 int slrh_scene_compat(slrh_scene* s, uint32_t flags) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (flags & ~(uint32_t)(SLRH_COMPAT_INSTANCE_SHADOW | SLRH_COMPAT_VOLPT_TANGENT_NEE))
         return fail(s->ctx, SLRH_ERR_INVALID, "scene_compat: unknown flag");
@@ -455,6 +485,7 @@ int slrh_scene_compat(slrh_scene* s, uint32_t flags) {
 
 int slrh_camera_perspective(slrh_scene* s, const float* l2w, float sensitivity, float aspect, float fovY,
                             float lensRadius, float imgDist, float objDist) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (!l2w || !(aspect > 0) || !(fovY > 0) || !(lensRadius >= 0) || !(imgDist > 0) || !(objDist > 0))
         return fail(s->ctx, SLRH_ERR_INVALID, "camera_perspective: bad args");
@@ -467,6 +498,7 @@ int slrh_camera_perspective(slrh_scene* s, const float* l2w, float sensitivity, 
 }
 
 int slrh_environment(slrh_scene* s, const float* rgb, uint32_t width, uint32_t height, float scale) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (!rgb || width == 0 || height == 0 || !(scale >= 0) || (uint64_t)width * height > (1ull << 28))
         return fail(s->ctx, SLRH_ERR_INVALID, "environment: bad args");
@@ -476,7 +508,7 @@ int slrh_environment(slrh_scene* s, const float* rgb, uint32_t width, uint32_t h
 
 int slrh_scene_build(slrh_scene* s) {
     CHECK_SCENE(s);
-    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    CtxLock lk = lockOf(s);
     try {
         SceneData& d = s->data;
         const size_t nt = d.idx.size() / 3;
@@ -542,6 +574,7 @@ int slrh_scene_build(slrh_scene* s) {
 }
 
 int slrh_scene_accelerators(slrh_scene* s, uint32_t flags) {
+    CtxLock lk = lockOf(s);
     CHECK_SCENE(s);
     if (flags & ~(uint32_t)SLRH_ACCEL_STANDARD_BVH) return fail(s->ctx, SLRH_ERR_INVALID, "unknown accelerator flag");
     if ((flags & SLRH_ACCEL_STANDARD_BVH) && !s->data.instances.empty())
@@ -551,6 +584,7 @@ int slrh_scene_accelerators(slrh_scene* s, uint32_t flags) {
 }
 
 int slrh_scene_query_accel(slrh_scene* s, int accel) {
+    CtxLock lk = lockOf(s);
     if (!s) return SLRH_ERR_INVALID;
     if (accel != SLRH_QUERY_QBVH && accel != SLRH_QUERY_STANDARD_BVH)
         return fail(s->ctx, SLRH_ERR_INVALID, "unknown query accelerator");
@@ -562,6 +596,7 @@ int slrh_scene_query_accel(slrh_scene* s, int accel) {
 
 int slrh_scene_standard_bvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs, uint32_t* nn,
                                  uint32_t* nr, double* stats) {
+    CtxLock lk = lockOf(s);
     if (!s) return SLRH_ERR_INVALID;
     if (!s->built || !s->data.wantStdBvh) return SLRH_ERR_STATE;
     const StdBvh& b = s->data.stdBvh;
@@ -582,6 +617,7 @@ int slrh_scene_standard_bvh_dump(const slrh_scene* s, float* boxes, uint32_t* in
 }
 
 int slrh_scene_accel_stats(const slrh_scene* s, double* st) {
+    CtxLock lk = lockOf(s);
     if (!s || !st) return SLRH_ERR_INVALID;
     if (!s->built) return SLRH_ERR_STATE;
     const Accel& a = s->data.accel;
@@ -599,6 +635,7 @@ int slrh_scene_accel_stats(const slrh_scene* s, double* st) {
 
 int slrh_scene_sbvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs, uint32_t* nn,
                          uint32_t* nr) {
+    CtxLock lk = lockOf(s);
     if (!s) return SLRH_ERR_INVALID;
     if (!s->built) return SLRH_ERR_STATE;
     const Accel& a = s->data.accel;
@@ -616,6 +653,7 @@ int slrh_scene_sbvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int3
 
 int slrh_scene_qbvh_dump(const slrh_scene* s, float* boxes, uint32_t* info, int32_t* refs, uint32_t* nn,
                          uint32_t* nr) {
+    CtxLock lk = lockOf(s);
     if (!s) return SLRH_ERR_INVALID;
     if (!s->built) return SLRH_ERR_STATE;
     const Accel& a = s->data.accel;
@@ -658,7 +696,7 @@ static int doTrace(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, 
         a.vis = vis;
         if (!a.vis) return fail(s->ctx, SLRH_ERR_INVALID, "null visibility array");
     }
-    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    CtxLock lk = lockOf(s);
     std::string err;
     int r = s->queryAccel == SLRH_QUERY_STANDARD_BVH
                 ? traceBatchStdBvh(s->dev, a, n, hits == nullptr, mem == SLRH_MEM_DEVICE, stream, &err)
@@ -668,11 +706,13 @@ static int doTrace(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, 
 }
 
 int slrh_intersect(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, uint32_t n, int mem, void* stream) {
+    CtxLock lk = lockOf(s);
     if (!hits) return s ? fail(s->ctx, SLRH_ERR_INVALID, "null hits") : SLRH_ERR_INVALID;
     return doTrace(s, rays, hits, nullptr, n, mem, stream);
 }
 
 int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* vis, uint32_t n, int mem, void* stream) {
+    CtxLock lk = lockOf(s);
     return doTrace(s, rays, nullptr, vis, n, mem, stream);
 }
 
@@ -708,7 +748,7 @@ int slrh_intersect_noalloc(slrh_scene* s, const slrh_rays* rays, const slrh_hits
     if (stackBound(s->data) > 64)
         return fail(s->ctx, SLRH_ERR_CAPACITY, "slrh_intersect_noalloc: the scene's trees may need a traversal "
                                                 "stack deeper than 64 entries; use slrh_intersect");
-    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    CtxLock lk = lockOf(s);
     std::string err;
     int r = deviceSelect(s->ctx->device, &err);
     if (!r) r = traceBatchNoAlloc(s->dev, a, n, false, stream, &err);
@@ -718,6 +758,7 @@ int slrh_intersect_noalloc(slrh_scene* s, const slrh_rays* rays, const slrh_hits
 
 int slrh_medium_query(slrh_scene* s, const slrh_rays* rays, uint32_t n, int mode, uint32_t seed, float* out,
                       int mem, void* stream) {
+    CtxLock lk = lockOf(s);
     if (!s || !rays || !out) return SLRH_ERR_INVALID;
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "medium query before slrh_scene_build");
     if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
@@ -731,7 +772,6 @@ int slrh_medium_query(slrh_scene* s, const slrh_rays* rays, uint32_t n, int mode
     a.tmin = rays->tmin; a.tmax = rays->tmax;
     if (!a.ox || !a.oy || !a.oz || !a.dx || !a.dy || !a.dz || !a.tmin || !a.tmax)
         return fail(s->ctx, SLRH_ERR_INVALID, "null ray array");
-    std::lock_guard<std::mutex> lk(s->ctx->mu);
     std::string err;
     int r = mediumQuery(s->dev, a, n, mode, seed, out, mem == SLRH_MEM_DEVICE, stream, &err);
     if (r) return fail(s->ctx, r, err);
@@ -753,6 +793,7 @@ int slrh_film_create(slrh_context* ctx, uint32_t w, uint32_t h, slrh_film** out)
 }
 
 int slrh_film_destroy(slrh_film* f) {
+    CtxLock lk = lockOf(f);
     if (!f) return SLRH_ERR_INVALID;
     filmFree(f->dev);
     delete f;
@@ -760,6 +801,7 @@ int slrh_film_destroy(slrh_film* f) {
 }
 
 int slrh_film_clear(slrh_film* f) {
+    CtxLock lk = lockOf(f);
     if (!f) return SLRH_ERR_INVALID;
     std::string err;
     int r = filmClear(f->dev, &err);
@@ -803,6 +845,7 @@ void saveImagePixel(const double* st, float scale, const float* xb, const float*
 }  // namespace
 
 int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size_t bytes) {
+    CtxLock lk = lockOf(f);
     if (!f || !dst) return SLRH_ERR_INVALID;
     const size_t px = (size_t)f->w * f->h;
     size_t need = format == SLRH_FILM_STRATA16_F64 ? px * 16 * 8 : format == SLRH_FILM_XYZ_F64 ? px * 3 * 8
@@ -838,6 +881,7 @@ int slrh_film_read(const slrh_film* f, int format, double scale, void* dst, size
 }
 
 int slrh_film_load(slrh_film* f, const double* strata, size_t bytes) {
+    CtxLock lk = lockOf(f);
     if (!f || !strata) return SLRH_ERR_INVALID;
     if (bytes < (size_t)f->w * f->h * 16 * sizeof(double))
         return fail(f->ctx, SLRH_ERR_INVALID, "film_load: source smaller than the film");
@@ -848,6 +892,7 @@ int slrh_film_load(slrh_film* f, const double* strata, size_t bytes) {
 }
 
 int slrh_film_bmp(const slrh_film* f, float scale, float sensitivity, void* dst, size_t dst_bytes, size_t* size) {
+    CtxLock lk = lockOf(f);
     if (!f || !size) return SLRH_ERR_INVALID;
     // saveBMP's layout (libSLR/Helper/bmp_exporter.cpp:15-54): 54-byte header,
     // rows bottom-up, BGR, each row 3 * width + width % 4 bytes
@@ -940,7 +985,7 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
         return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
     if (set->width != film->w || set->height != film->h) return fail(s->ctx, SLRH_ERR_INVALID, "film size mismatch");
     if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
-    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    CtxLock lk = lockOf(s);
     RenderConfig cfg{set->width, set->height, set->spp, set->max_depth, set->seed, set->slots_per_pixel,
                      (set->flags & 1u) != 0};
     if (set->reserved != 0) return fail(s->ctx, SLRH_ERR_INVALID, "reserved settings field must be 0");
@@ -961,6 +1006,7 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
 
 int slrh_render_progressive(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
                             slrh_snapshot_fn snapshot, void* user, double* stats) {
+    CtxLock lk = lockOf(s);
     if (!s || !set || !film) return SLRH_ERR_INVALID;
     if (set->spp == 0) return SLRH_OK;
     // passes of 1, 1, 2, 4, ... samples per pixel (the last cut to spp), so
@@ -1037,7 +1083,7 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     opt.shadeShape = (set->flags & SLRH_FLAG_SHADE_BLOCK_128) ? 1 : -1;
     opt.tunePool = (set->flags & SLRH_FLAG_TUNE_POOL) != 0;
     if (opt.traceVariant >= numTraceVariants()) return fail(s->ctx, SLRH_ERR_INVALID, "unknown traversal variant");
-    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    CtxLock lk = lockOf(s);
     RenderStats st;
     std::string err;
     int r = renderJobs(s->dev, specs.data(), n, opt, &st, &err);
@@ -1047,6 +1093,7 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
 }
 
 int slrh_server_create(slrh_scene* s, const slrh_server_config* cfg, slrh_server** out) {
+    CtxLock lk = lockOf(s);
     if (!s || !cfg || !out) return SLRH_ERR_INVALID;
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "server before slrh_scene_build");
     if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");

@@ -151,3 +151,38 @@ def test_standard_bvh_matches_reference_bit_for_bit(host_ctx, which):
     assert np.array_equal(info, ri)
     assert np.array_equal(refs, rr)
     assert depth == rdepth and np.float32(cost) == np.float32(rcost)
+
+
+def test_context_calls_from_many_threads(host_ctx):
+    """Scene assembly from several threads on one context serialises on the
+    context's lock, and each thread reads its own last error."""
+    import ctypes
+    import threading
+    lib = host_ctx.lib
+    sc = slr_amd.Scene(host_ctx)
+    errors = []
+
+    def worker(k):
+        try:
+            for i in range(200):
+                sid = ctypes.c_int()
+                assert lib.slrh_spectrum_rgb(sc.h, 0, 0.1 * k, 0.2, 0.3, ctypes.byref(sid)) == 0
+                mid = ctypes.c_int()
+                assert lib.slrh_material_lambert(sc.h, sid.value, ctypes.byref(mid)) == 0
+                # a failure of this thread's own kind, then its own message
+                if k % 2:
+                    assert lib.slrh_material_lambert(sc.h, -5, ctypes.byref(mid)) < 0
+                    assert lib.slrh_last_error(host_ctx.h) == b"lambert: bad spectrum"
+                else:
+                    assert lib.slrh_spectrum_rgb(sc.h, 9, 0.1, 0.2, 0.3, ctypes.byref(sid)) < 0
+                    assert lib.slrh_last_error(host_ctx.h) == b"slrh_spectrum_rgb: bad type"
+        except Exception as e:   # reported on the main thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors[:3]
+    sc.close()
