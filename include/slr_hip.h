@@ -190,6 +190,45 @@ SLRH_API int slrh_environment(slrh_scene* s, const float* rgb, uint32_t width, u
 SLRH_API int slrh_camera_perspective(slrh_scene* s, const float* l2w, float sensitivity, float aspect,
                                      float fov_y, float lens_radius, float img_dist, float obj_dist);
 
+/* Scene file reader: runs the text of a scene written in SLR's scene
+ * language (libSLRSceneGraph/Parser/SceneParser.yy, the builtins of
+ * libSLRSceneGraph/API.cpp:295-1325; readScene in HostProgram/main.cpp:40)
+ * and adds what it describes to `s`: spectra (RGB, regular, irregular —
+ * tabulated over its knots — and the illuminants D65 and E), materials
+ * ("matte" with or without sigma, "microfacet metal", "microfacet glass",
+ * "emitter" with a "diffuse" emitter property), meshes in world space with
+ * the vertex normals and tangents the reference recomputes, and the
+ * perspective camera. Control flow, user functions, model and image loading
+ * and other materials and renderers are refused with a message naming the
+ * line. `info` (may be NULL) receives the render settings the file sets. */
+typedef struct slrh_scene_file_info {
+    uint32_t width, height;   /* setRenderSettings (1024 x 1024 when not set) */
+    uint32_t spp;             /* setRenderer's "samples" (8 when not set) */
+    uint32_t integrator;      /* slrh_integrator: "PT" or "Volumetric PT" */
+    uint32_t rng_seed;        /* setRenderSettings "rngSeed" */
+    float brightness;         /* setRenderSettings "brightness": saveImage's scale is brightness / spp */
+    float sensitivity;        /* the camera's image sensor: its sensitivity, or 1 / (pi r^2) when that is
+                                 not positive (PerspectiveCamera.cpp:24; infinite for a pinhole) */
+    uint32_t num_meshes, num_triangles, num_materials, num_spectra;
+} slrh_scene_file_info;
+SLRH_API int slrh_scene_load_text(slrh_scene* s, const char* text, slrh_scene_file_info* info);
+
+/* Introspection (no reference counterpart; for tests and integrators that
+ * check what a scene holds). slrh_scene_geometry copies the world-level
+ * triangles: on entry *nverts / *ntris are the capacities of the arrays
+ * (arrays may be NULL), on return the counts; pos/nrm/tan 3 floats per
+ * vertex, idx 3 per triangle, mat 1 per triangle. slrh_scene_camera_get:
+ * l2w (16 floats, column-major) and params {sensitivity, aspect, fovY,
+ * lens radius, image distance, object distance}. slrh_material_info: the
+ * material's model (0 Lambert, 1 Oren-Nayar, 2 GGX conductor, 3 GGX
+ * dielectric, 4 emitter without a base), its spectra (-1: none), emittance
+ * spectrum and two parameters (Oren-Nayar A, B; GGX alpha_x, alpha_y). */
+SLRH_API int slrh_scene_geometry(const slrh_scene* s, uint32_t* nverts, uint32_t* ntris, float* pos, float* nrm,
+                                 float* tan, uint32_t* idx, uint32_t* mat);
+SLRH_API int slrh_scene_camera_get(const slrh_scene* s, float* l2w, float* params);
+SLRH_API int slrh_material_info(const slrh_scene* s, int id, int* model, int* spec0, int* spec1, int* emittance,
+                                float* p0, float* p1);
+
 /* Builds the SBVH (libSLR/Accelerator/SBVH.h:380-417), collapses it into the
  * QBVH (libSLR/Accelerator/QBVH.h:252-284) and uploads scene, accelerator and
  * tables to HBM. Replaces SLR::Scene::build (libSLR/Scene/Scene.cpp:17-38). */

@@ -37,6 +37,7 @@ EXPORTS = [
     "slrh_scene_create", "slrh_scene_build", "slrh_scene_destroy",
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
     "slrh_scene_accelerators", "slrh_scene_query_accel", "slrh_scene_standard_bvh_dump",
+    "slrh_scene_load_text", "slrh_scene_geometry", "slrh_scene_camera_get", "slrh_material_info",
     "slrh_intersect", "slrh_occluded", "slrh_intersect_noalloc",
     "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read", "slrh_film_load", "slrh_film_bmp",
     "slrh_render_progressive",
@@ -81,6 +82,13 @@ FLAG_NO_GRAPH = 2
 FLAG_PARALLEL_SHADE = 4
 FLAG_SHADE_BLOCK_128 = 8
 FLAG_TUNE_POOL = 16
+
+
+class _SceneFileInfo(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("integrator", ctypes.c_uint32), ("rng_seed", ctypes.c_uint32), ("brightness", ctypes.c_float),
+                ("sensitivity", ctypes.c_float), ("num_meshes", ctypes.c_uint32), ("num_triangles", ctypes.c_uint32),
+                ("num_materials", ctypes.c_uint32), ("num_spectra", ctypes.c_uint32)]
 
 
 class _Settings(ctypes.Structure):
@@ -132,6 +140,10 @@ def load_library(path=None):
         "slrh_scene_accelerators": (i32, [vp, u32]),
         "slrh_scene_query_accel": (i32, [vp, i32]),
         "slrh_scene_standard_bvh_dump": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "slrh_scene_load_text": (i32, [vp, ctypes.c_char_p, vp]),
+        "slrh_scene_geometry": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "slrh_scene_camera_get": (i32, [vp, vp, vp]),
+        "slrh_material_info": (i32, [vp, i32, pi, pi, pi, pi, vp, vp]),
         "slrh_intersect": (i32, [vp, vp, vp, u32, i32, vp]),
         "slrh_occluded": (i32, [vp, vp, vp, u32, i32, vp]),
         "slrh_intersect_noalloc": (i32, [vp, vp, vp, u32, vp]),
@@ -370,6 +382,42 @@ class Scene:
         self.ctx.check(fn(self.h, _ptr(boxes), _ptr(info), _ptr(refs), None, None, None),
                        "slrh_scene_standard_bvh_dump")
         return boxes, info, refs, int(st[0]), float(st[1])
+
+    def load_text(self, text):
+        """slrh_scene_load_text: adds the scene a scene-language text describes;
+        returns the render settings it sets (a dict)."""
+        info = _SceneFileInfo()
+        self.ctx.check(self.lib.slrh_scene_load_text(self.h, text.encode(), ctypes.byref(info)), "slrh_scene_load_text")
+        self.num_prims += info.num_triangles
+        return {k: getattr(info, k) for k, _ in _SceneFileInfo._fields_}
+
+    def geometry(self):
+        """World-level triangles: dict of pos, nrm, tan (nv, 3), idx (nt, 3), mat (nt,)."""
+        nv, nt = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        fn = self.lib.slrh_scene_geometry
+        self.ctx.check(fn(self.h, ctypes.byref(nv), ctypes.byref(nt), None, None, None, None, None), "slrh_scene_geometry")
+        g = {k: np.zeros((nv.value, 3), np.float32) for k in ("pos", "nrm", "tan")}
+        g["idx"] = np.zeros((nt.value, 3), np.uint32)
+        g["mat"] = np.zeros(nt.value, np.uint32)
+        self.ctx.check(fn(self.h, ctypes.byref(nv), ctypes.byref(nt), _ptr(g["pos"]), _ptr(g["nrm"]), _ptr(g["tan"]),
+                          _ptr(g["idx"]), _ptr(g["mat"])), "slrh_scene_geometry")
+        return g
+
+    def camera(self):
+        """(l2w 4x4 row-major, {sensitivity, aspect, fov_y, lens_radius, img_dist, obj_dist})."""
+        l2w = np.zeros(16, np.float32)
+        p = np.zeros(6, np.float32)
+        self.ctx.check(self.lib.slrh_scene_camera_get(self.h, _ptr(l2w), _ptr(p)), "slrh_scene_camera_get")
+        keys = ("sensitivity", "aspect", "fov_y", "lens_radius", "img_dist", "obj_dist")
+        return l2w.reshape(4, 4).T.copy(), dict(zip(keys, p.tolist()))
+
+    def material_info(self, mid):
+        out = [ctypes.c_int() for _ in range(4)]
+        p0, p1 = ctypes.c_float(), ctypes.c_float()
+        self.ctx.check(self.lib.slrh_material_info(self.h, int(mid), *[ctypes.byref(x) for x in out],
+                                                   ctypes.byref(p0), ctypes.byref(p1)), "slrh_material_info")
+        return {"model": out[0].value, "spec0": out[1].value, "spec1": out[2].value, "emittance": out[3].value,
+                "p0": p0.value, "p1": p1.value}
 
     def set_accelerators(self, standard_bvh=False):
         self.ctx.check(self.lib.slrh_scene_accelerators(self.h, 1 if standard_bvh else 0), "slrh_scene_accelerators")

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "scene_reader.h"
 #include "server.h"
 
 using namespace slrh;
@@ -363,6 +364,11 @@ bool invertMatrix(const float m[4][4], float out[4][4]) {
 }
 }  // namespace
 
+namespace slrh {
+bool invertMatrix4(const float m[4][4], float out[4][4]) { return invertMatrix(m, out); }
+void coordinateTangent(const float n[3], float t[3]) { tangentOf(n, t); }
+}  // namespace slrh
+
 extern "C" {
 
 int slrh_mesh_add(slrh_scene* s, uint32_t nverts, const float* pos, const float* nrm, const float* tan,
@@ -494,6 +500,120 @@ int slrh_camera_perspective(slrh_scene* s, const float* l2w, float sensitivity, 
     c.sensitivity = sensitivity; c.aspect = aspect; c.fovY = fovY;
     c.lensRadius = lensRadius; c.imgDist = imgDist; c.objDist = objDist;
     s->data.hasCamera = true;
+    return SLRH_OK;
+}
+
+int slrh_scene_load_text(slrh_scene* s, const char* text, slrh_scene_file_info* info) {
+    CtxLock lk = lockOf(s);
+    CHECK_SCENE(s);
+    if (!text) return fail(s->ctx, SLRH_ERR_INVALID, "scene_load_text: null text");
+    scenefile::SceneFile sf;
+    std::string err;
+    if (!scenefile::readSceneText(text, &sf, &err)) return fail(s->ctx, SLRH_ERR_INVALID, "scene file: " + err);
+    std::vector<int> specId(sf.spectra.size()), matId(sf.materials.size());
+    int r;
+    for (size_t k = 0; k < sf.spectra.size(); ++k) {
+        const scenefile::Spectrum& sp = sf.spectra[k];
+        if (sp.kind == scenefile::kRgb)
+            r = slrh_spectrum_rgb(s, sp.type == scenefile::kLightSource ? SLRH_SPECTRUM_LIGHT_SOURCE
+                                                                         : SLRH_SPECTRUM_REFLECTANCE,
+                                  sp.rgb[0], sp.rgb[1], sp.rgb[2], &specId[k]);
+        else
+            r = slrh_spectrum_regular(s, sp.lo, sp.hi, sp.values.data(), (int)sp.values.size(), &specId[k]);
+        if (r) return r;
+    }
+    for (size_t k = 0; k < sf.materials.size(); ++k) {
+        const scenefile::Material& m = sf.materials[k];
+        switch (m.kind) {
+            case scenefile::kMatte:
+                r = m.hasSigma ? slrh_material_oren_nayar(s, specId[m.spec0], m.sigma, &matId[k])
+                               : slrh_material_lambert(s, specId[m.spec0], &matId[k]);
+                break;
+            case scenefile::kMicrofacetMetal:
+                r = slrh_material_ggx_conductor(s, specId[m.spec0], specId[m.spec1], m.alphaX, m.alphaY, &matId[k]);
+                break;
+            case scenefile::kMicrofacetGlass:
+                r = slrh_material_ggx_dielectric(s, specId[m.spec0], specId[m.spec1], m.alphaX, m.alphaY, &matId[k]);
+                break;
+            default:
+                r = slrh_material_diffuse_emitter(s, matId[m.base], specId[m.emittance], &matId[k]);
+                break;
+        }
+        if (r) return r;
+    }
+    uint32_t tris = 0;
+    for (const scenefile::Mesh& mesh : sf.meshes) {
+        std::vector<uint32_t> mats(mesh.mat.size());
+        for (size_t t = 0; t < mats.size(); ++t) mats[t] = (uint32_t)matId[mesh.mat[t]];
+        int id;
+        r = slrh_mesh_add(s, (uint32_t)(mesh.pos.size() / 3), mesh.pos.data(), mesh.nrm.data(), mesh.tan.data(),
+                          (uint32_t)mats.size(), mesh.idx.data(), mats.data(), -1, &id);
+        if (r) return r;
+        tris += (uint32_t)mats.size();
+    }
+    const scenefile::Camera& c = sf.camera;
+    if ((r = slrh_camera_perspective(s, c.l2w, c.sensitivity, c.aspect, c.fovY, c.radius, c.imgDist, c.objDist)))
+        return r;
+    if (info) {
+        info->width = sf.width;
+        info->height = sf.height;
+        info->spp = sf.spp;
+        info->integrator = sf.method == "Volumetric PT" ? SLRH_VOL_PT : SLRH_PT;
+        info->rng_seed = sf.rngSeed;
+        info->brightness = sf.brightness;
+        info->sensitivity = c.sensitivity > 0 ? c.sensitivity : (float)(1.0f / (M_PI * c.radius * c.radius));
+        info->num_meshes = (uint32_t)sf.meshes.size();
+        info->num_triangles = tris;
+        info->num_materials = (uint32_t)sf.materials.size();
+        info->num_spectra = (uint32_t)sf.spectra.size();
+    }
+    return SLRH_OK;
+}
+
+int slrh_scene_geometry(const slrh_scene* s, uint32_t* nverts, uint32_t* ntris, float* pos, float* nrm, float* tan,
+                        uint32_t* idx, uint32_t* mat) {
+    CtxLock lk = lockOf(s);
+    if (!s || !nverts || !ntris) return SLRH_ERR_INVALID;
+    const SceneData& d = s->data;
+    const uint32_t nv = (uint32_t)(d.vpos.size() / 3), nt = (uint32_t)d.triMat.size();
+    const uint32_t cv = *nverts, ct = *ntris;
+    *nverts = nv;
+    *ntris = nt;
+    if ((pos || nrm || tan) && cv < nv) return fail(s->ctx, SLRH_ERR_CAPACITY, "scene_geometry: vertex arrays too small");
+    if ((idx || mat) && ct < nt) return fail(s->ctx, SLRH_ERR_CAPACITY, "scene_geometry: triangle arrays too small");
+    if (pos) memcpy(pos, d.vpos.data(), d.vpos.size() * sizeof(float));
+    if (nrm) memcpy(nrm, d.vnrm.data(), d.vnrm.size() * sizeof(float));
+    if (tan) memcpy(tan, d.vtan.data(), d.vtan.size() * sizeof(float));
+    if (idx) memcpy(idx, d.idx.data(), d.idx.size() * sizeof(uint32_t));
+    if (mat) memcpy(mat, d.triMat.data(), d.triMat.size() * sizeof(uint32_t));
+    return SLRH_OK;
+}
+
+int slrh_scene_camera_get(const slrh_scene* s, float* l2w, float* params) {
+    CtxLock lk = lockOf(s);
+    if (!s) return SLRH_ERR_INVALID;
+    if (!s->data.hasCamera) return fail(s->ctx, SLRH_ERR_STATE, "scene has no camera");
+    const CameraParams& c = s->data.camera;
+    if (l2w) memcpy(l2w, c.l2w, sizeof(c.l2w));
+    if (params) {
+        const float p[6] = {c.sensitivity, c.aspect, c.fovY, c.lensRadius, c.imgDist, c.objDist};
+        memcpy(params, p, sizeof(p));
+    }
+    return SLRH_OK;
+}
+
+int slrh_material_info(const slrh_scene* s, int id, int* model, int* spec0, int* spec1, int* emittance, float* p0,
+                       float* p1) {
+    CtxLock lk = lockOf(s);
+    if (!s) return SLRH_ERR_INVALID;
+    if (id < 0 || (size_t)id >= s->data.materials.size()) return fail(s->ctx, SLRH_ERR_INVALID, "material_info: bad id");
+    const DevMaterial& m = s->data.materials[(size_t)id];
+    if (model) *model = (int)m.type;
+    if (spec0) *spec0 = m.spec0;
+    if (spec1) *spec1 = m.spec1;
+    if (emittance) *emittance = m.emit;
+    if (p0) *p0 = m.p0;
+    if (p1) *p1 = m.p1;
     return SLRH_OK;
 }
 

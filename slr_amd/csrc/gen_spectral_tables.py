@@ -1,5 +1,6 @@
 """Regenerates spectral_tables.inc, the two published data tables the
-spectral path needs bit-for-bit:
+spectral path needs bit-for-bit, and spectral_library.inc, the illuminant
+spectra scene files can name:
 
 * the CIE 1931 2-degree standard observer, 360-830 nm at 1 nm (471 samples),
   as tabulated in libSLR/BasicTypes/spectrum_base.cpp:35-221;
@@ -92,6 +93,38 @@ def main():
     with open(os.path.join(HERE, "spectral_tables.inc"), "w") as f:
         f.write("\n".join(out) + "\n")
     print("points", len(points), "cells", len(cells))
+    illuminants()
+
+
+def illuminants():
+    """spectral_library.inc: the illuminant spectra a scene file can name
+    (Spectrum("Illuminant", name, 0)), from the regular tables of
+    libSLR/BasicTypes/spectrum_library.cpp:12-103 (CIE D65 300-830 nm at
+    1 nm, and E scaled to D65's energy)."""
+    lib = open(os.path.join(REF, "libSLR/BasicTypes/spectrum_library.cpp"), encoding="utf-8-sig").read()
+    out = []
+    w = out.append
+    w("// Generated by gen_spectral_tables.py from the reference's spectrum library")
+    w("// (libSLR/BasicTypes/spectrum_library.cpp:12-103): data, not code.")
+    w("#pragma once")
+    w("#include <cstdint>")
+    w("namespace slrh { namespace tables {")
+    w("struct LibrarySpectrum { const char* name; uint32_t numSamples; float minLambda, maxLambda; const float* values; };")
+    entries = []
+    for name in ("D65", "E"):
+        vals = np.float32(_floats(_block(lib, "const float %s_values[] = {" % name)))
+        m = re.search(r"const Data %s = \{\s*DistributionType::Regular,\s*(\d+),\s*([\d.]+)f,\s*([\d.]+)f" % name, lib)
+        n, lo, hi = int(m.group(1)), float(m.group(2)), float(m.group(3))
+        assert n == len(vals), (name, n, len(vals))
+        w("alignas(16) const float kIlluminant%s[%d] = {" % (name, n))
+        for i in range(0, n, 8):
+            w("  " + ", ".join("%.9g" % x for x in vals[i:i + 8]) + ",")
+        w("};")
+        entries.append('{"%s", %d, %.1ff, %.1ff, kIlluminant%s}' % (name, n, lo, hi, name))
+    w("const LibrarySpectrum kIlluminants[] = {" + ", ".join(entries) + "};")
+    w("}}  // namespace slrh::tables")
+    with open(os.path.join(HERE, "spectral_library.inc"), "w") as f:
+        f.write("\n".join(out) + "\n")
 
 
 if __name__ == "__main__":
