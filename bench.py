@@ -1,7 +1,7 @@
 """Benchmark of the spectral path tracer on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c4|batch8|serve]
-                    [--legs relmse,matched,cpu,serve,c3,c4 | --legs none] [--out FILE]
+                    [--legs relmse,matched,cpu,serve,c3,c4,bpt | --legs none] [--out FILE]
 
 Workload (default, "c2"): the Cornell box plus a displaced icosphere at
 subdivision 8 (1,310,720 triangles, GGX conductor alpha 0.2, eta 0.2,
@@ -36,6 +36,12 @@ and the legs below (untimed; --legs selects them):
   c4       the Cornell box around a 256^3 fBm medium, volumetric, 4 x
            1024^2 x 128 spp in one wavefront, one step, with the medium
            tracking kernel's roofline
+  bpt      the bidirectional integrator (BPTRenderer): the Cornell box at
+           512x512 x 16 spp (C1's scene) beside the reference BPTRenderer on
+           the host cores, the C2 scene at 1920x1080 x 4 spp with its
+           traversal roofline, and 32 GPU batches of the Cornell box at
+           32x32 x 64 spp against the reference BPT image statistics
+           (tests/golden/ref_bpt_cornell32.npz)
 
 Other workloads as the headline (--config):
   c1      the Cornell box alone, 512x512, 16 spp, depth 8.
@@ -57,7 +63,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s + samples/s at matched relMSE; p50/p99 latency of served renders"
-ALL_LEGS = ("relmse", "matched", "cpu", "serve", "c3", "c4")
+ALL_LEGS = ("relmse", "matched", "cpu", "serve", "c3", "c4", "bpt")
 
 
 def log(*a):
@@ -101,7 +107,8 @@ def calibration_copy(nbytes):
     hip.hipFree(dst)
 
 
-def trace_roofline(tot, config, instanced=False):
+def trace_roofline(tot, config, instanced=False,
+                   kernel="k_trace (QBVH closest hit for extension rays + any hit for shadow rays, one launch)"):
     """k_trace's algorithmic bytes per launch over its mean launch time (HIP
     events around each launch on its stream), against the HBM peak; beside
     it the same without the node reads served from the LDS copy of the top
@@ -118,7 +125,7 @@ def trace_roofline(tot, config, instanced=False):
     traffic = measured_traffic(config)
     r = {
         "bound": "hbm",
-        "kernel": "k_trace (QBVH closest hit for extension rays + any hit for shadow rays, one launch)",
+        "kernel": kernel,
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -447,6 +454,89 @@ def leg_c4(ctx, steps=1, warmup=1):
     return r
 
 
+def cpu_bpt_baseline(desc, sample, rays_per_sample, gpu_sps):
+    """Reference BPTRenderer on the host cores over the same scene and size
+    as the GPU run (the Cornell box, whose build is instant)."""
+    from oracle import refdrv
+    if not refdrv.available():
+        return None
+    w, h, spp = sample
+    host = host_description()
+    threads = max(1, min(host["nproc"], 64))
+    ref = refdrv.RefScene(desc)
+    log("cpu baseline: reference BPTRenderer %dx%dx%d spp on %d threads" % (w, h, spp, threads))
+    _, sec, _ = ref.render(w, h, spp, 1509761209, threads=threads, bpt=True)
+    ref.close()
+    sps = w * h * spp / sec
+    return {"value": round(sps * rays_per_sample / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "reference",
+            "samples_per_s": round(sps, 1), "seconds": round(sec, 3), "host": host,
+            "sample": "reference libSLR BPTRenderer, the Cornell box at %dx%d, %d spp, %d threads; Mrays/s = "
+                      "samples/s x %.3f rays/sample counted on the GPU for the same estimator" % (
+                          w, h, spp, threads, rays_per_sample),
+            "gpu_over_cpu": round(gpu_sps / sps, 1)}
+
+
+def leg_bpt(ctx, sc_c2, c2_desc, with_cpu):
+    """The bidirectional integrator: C1's scene and size against the
+    reference on the host cores, the C2 scene at 1080p x 4 spp (traversal
+    roofline), and its image against the reference BPT statistics."""
+    import slr_amd
+    from slr_amd import scenes
+    from tests import refcases
+    out = {}
+    W, H, spp = 512, 512, 16
+    desc = scenes.cornell_box(W, H)
+    sc = slr_amd.build_scene(ctx, desc)
+    film = slr_amd.Film(ctx, W, H)
+    r = slr_amd.BPTRenderer(spp)
+    tot, el = run_frames(r, sc, film, 3, 1)
+    c1 = {"workload": "cornell box 512x512 x 16 spp, BPT (no path cap, as the reference)",
+          "mrays_per_s": round(tot["rays"] / el / 1e6, 2), "samples_per_s": round(tot["samples"] / el, 1),
+          "ms_per_frame": round(el / 3 * 1e3, 2), "rays_per_sample": round(tot["rays"] / tot["samples"], 3),
+          "connections_per_sample": round(tot["connections"] / tot["samples"], 3),
+          "lens_contributions_per_sample": round(tot["lens_contributions"] / tot["samples"], 3),
+          "capped_subpaths": int(tot["capped_subpaths"])}
+    out["c1"] = c1
+    film.close()
+    # relMSE against the reference BPT image statistics
+    ref = refcases.load("bpt_cornell32")
+    make, w, h, B, s2, _, _ = refcases.CASES["bpt_cornell32"]
+    cmf = slr_amd.cmf_strata()
+    sc32 = slr_amd.build_scene(ctx, make())
+    gb = []
+    for b in range(B):
+        f = slr_amd.Film(ctx, w, h)
+        r2 = slr_amd.BPTRenderer(s2)
+        r2.render(sc32, f, seed=8100 + b)
+        gb.append(refcases.xyz(f.strata(), cmf) / s2)
+        f.close()
+    gb = np.array(gb)
+    mean = gb.mean(0)
+    se = np.sqrt(gb.var(0, ddof=1) / B + ref["se"] ** 2)
+    out["relmse"] = {"reference": "reference BPTRenderer, Cornell box 32x32, %d batches x %d spp" % (
+        int(ref["batches"]), int(ref["spp"])),
+        "relmse": float(np.mean((mean - ref["mean"]) ** 2 / (ref["mean"] ** 2 + 1e-3))),
+        "outside_3sigma": round(float(np.mean(np.abs(mean - ref["mean"]) / np.maximum(se, 1e-12) > 3.0)), 5)}
+    sc32.close()
+    # the C2 scene at 1080p
+    W2, H2, spp2 = 1920, 1080, 4
+    film = slr_amd.Film(ctx, W2, H2)
+    tot, el = run_frames(slr_amd.BPTRenderer(spp2), sc_c2, film, 2, 1)
+    out["c2"] = {"workload": "C2 scene (1.31M-triangle GGX icosphere in the Cornell box), 1920x1080 x 4 spp, BPT",
+                 "mrays_per_s": round(tot["rays"] / el / 1e6, 2), "samples_per_s": round(tot["samples"] / el, 1),
+                 "ms_per_frame": round(el / 2 * 1e3, 2), "rays_per_sample": round(tot["rays"] / tot["samples"], 3),
+                 "roofline": trace_roofline(tot, "bpt_c2", kernel="k_bpt_trace (closest hit for light and eye "
+                                            "rays, any hit for connection and lens rays, one launch)"),
+                 "device_ms_per_frame": round(tot["device_seconds"] / 2 * 1e3, 2),
+                 "trace_ms_per_frame": round(tot["trace_seconds"] / 2 * 1e3, 2),
+                 "capped_subpaths": int(tot["capped_subpaths"])}
+    film.close()
+    if with_cpu:
+        out["cpu_baseline"] = cpu_bpt_baseline(desc, (W, H, spp), c1["rays_per_sample"], c1["samples_per_s"])
+    sc.close()
+    return out
+
+
 def headline_line(args, result):
     line = json.dumps(result)
     print(line, flush=True)
@@ -654,6 +744,9 @@ def main():
     r = leg("serve", lambda: leg_serve(ctx, desc, sc))
     if r:
         result["serve"] = r
+    r = leg("bpt", lambda: leg_bpt(ctx, sc, desc, "cpu" in legs))
+    if r:
+        result["bpt"] = r
     if "c3" in legs or "c4" in legs:
         film.close()
         sc.close()

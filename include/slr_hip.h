@@ -204,7 +204,7 @@ SLRH_API int slrh_camera_perspective(slrh_scene* s, const float* l2w, float sens
 typedef struct slrh_scene_file_info {
     uint32_t width, height;   /* setRenderSettings (1024 x 1024 when not set) */
     uint32_t spp;             /* setRenderer's "samples" (8 when not set) */
-    uint32_t integrator;      /* slrh_integrator: "PT" or "Volumetric PT" */
+    uint32_t integrator;      /* slrh_integrator: "PT", "Volumetric PT" or "BPT" */
     uint32_t rng_seed;        /* setRenderSettings "rngSeed" */
     float brightness;         /* setRenderSettings "brightness": saveImage's scale is brightness / spp */
     float sensitivity;        /* the camera's image sensor: its sensitivity, or 1 / (pi r^2) when that is
@@ -338,7 +338,21 @@ SLRH_API int slrh_medium_query(slrh_scene* s, const slrh_rays* rays, uint32_t n,
  * scattering by its phase function, light samples attenuated by the
  * medium's transmittance (ratio tracking), no fixed path-length cap (Russian
  * roulette; max_depth still applies when set). */
-enum slrh_integrator { SLRH_PT = 0, SLRH_VOL_PT = 1 };
+/* SLRH_BPT: BPTRenderer (libSLR/Renderer/BPTRenderer.cpp:25-432): per pixel
+ * sample a light subpath from a light chosen as Scene::selectSurfaceLight
+ * does and an eye subpath, both ended by Russian roulette, and every
+ * connection (s, t) between them, weighted by the reference's power-heuristic
+ * MIS over subpath area PDFs and RR probabilities; light subpath vertices
+ * connected to the lens add to the pixel their direction crosses. Media are
+ * ignored (as in BPTRenderer); a scene with an environment light is refused.
+ * max_depth caps the eye subpath's vertices (0: none; the reference has no
+ * cap); at most 64 light subpath vertices are kept per sample (stats[22]
+ * counts subpaths stopped by either cap). stats[16] light and [17] eye
+ * subpath vertices, [19] connections traced, [20] lens contributions filed.
+ * The lens contributions are summed with FP64 atomics in a varying order, so
+ * films of equal seeds agree to rounding (about 1e-15 relative), not
+ * bit-for-bit. slrh_render_batch renders BPT frames one after another. */
+enum slrh_integrator { SLRH_PT = 0, SLRH_VOL_PT = 1, SLRH_BPT = 2 };
 
 enum slrh_render_flags {
     SLRH_FLAG_TIME_KERNELS = 1,   /* time the kernel groups with HIP events (stats 5, 8, 9) */

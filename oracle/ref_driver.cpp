@@ -82,6 +82,7 @@
 #include <libSLR/BSDF/basic_bsdfs.h>
 #include <libSLR/BSDF/OrenNayerBRDF.h>
 #include <libSLR/BSDF/microfacet_bsdfs.h>
+#include <libSLR/Renderer/BPTRenderer.h>
 #include <libSLR/Renderer/PTRenderer.h>
 #include <libSLR/Renderer/VolumetricPTRenderer.h>
 #include <libSLR/Scene/medium_nodes.h>
@@ -158,7 +159,7 @@ struct RefScene {
     uint32_t medN[3];
     std::vector<std::vector<float>> medGrid;   // per z slice, x fastest
     float medSigmaS = 0, medSigmaE = 0, medG = 0;
-    int integrator = 0;   // 0 PTRenderer, 1 VolumetricPTRenderer
+    int integrator = 0;   // 0 PTRenderer, 1 VolumetricPTRenderer, 2 BPTRenderer
 
     // accelerator for ray batches
     bool built = false;
@@ -546,7 +547,7 @@ int refd_medium_track(void* sp, int mode, int n, const float* rays, uint32_t see
     return 0;
 }
 
-// 0: PTRenderer, 1: VolumetricPTRenderer for refd_render.
+// 0: PTRenderer, 1: VolumetricPTRenderer, 2: BPTRenderer for refd_render.
 void refd_set_integrator(void* sp, int integrator) { ((RefScene*)sp)->integrator = integrator; }
 
 // World-to-object matrix (16 floats, row-major 4x4) of instance k as
@@ -679,6 +680,11 @@ double refd_render_batches(void* sp, int w, int h, int spp, uint32_t seed, int b
         envNode.reset(new InfiniteSphereNode(&scene, s->envTex.get(), s->envScale));
         scene.setEnvironmentNode(envNode.get());
     }
+    // BPTRenderer::render adds per-thread separated buffers to the camera's
+    // sensor on every call (ImageSensor::addSeparatedBuffers, which cannot
+    // add them a second time), so each BPT batch renders on a fresh build of
+    // the scene: Scene::destory releases the camera with its sensor.
+    const bool bpt = s->integrator == 2;
     auto t0 = std::chrono::steady_clock::now();
     scene.build(&DefaultAllocator::instance());
     auto t1 = std::chrono::steady_clock::now();
@@ -703,11 +709,18 @@ double refd_render_batches(void* sp, int w, int h, int spp, uint32_t seed, int b
     // batch b: the renderer again with seed + b (PTRenderer::render starts
     // with ImageSensor::init, so each batch's sensor sums start at zero)
     double renderSeconds = 0;
-    const ImageSensor* sensor = scene.getCamera()->getSensor();
     for (int b = 0; b < batches; ++b) {
         settings.addItem(RenderSettingItem::RNGSeed, (int32_t)(seed + (uint32_t)b));
+        if (bpt && b > 0) {
+            scene.destory();
+            scene.build(&DefaultAllocator::instance());
+        }
+        const ImageSensor* sensor = scene.getCamera()->getSensor();
         auto t2 = std::chrono::steady_clock::now();
-        if (s->integrator == 1) {
+        if (bpt) {
+            BPTRenderer renderer((uint32_t)spp);
+            renderer.render(scene, settings);
+        } else if (s->integrator == 1) {
             VolumetricPTRenderer renderer((uint32_t)spp);
             renderer.render(scene, settings);
         } else {
@@ -722,6 +735,14 @@ double refd_render_batches(void* sp, int w, int h, int spp, uint32_t seed, int b
                 DiscretizedSpectrum px = sensor->pixel((uint32_t)x, (uint32_t)y);
                 for (int k = 0; k < (int)NumStrataForStorage; ++k)
                     out[((size_t)y * w + x) * NumStrataForStorage + k] = px[k];
+                // BPT: the light-to-lens sums of each thread's separated
+                // buffer, which saveImage adds with the same scale
+                // (ImageSensor.cpp:139-152)
+                for (int t = 0; bpt && t < threads; ++t) {
+                    DiscretizedSpectrum sp = sensor->pixel((uint32_t)t, (uint32_t)x, (uint32_t)y);
+                    for (int k = 0; k < (int)NumStrataForStorage; ++k)
+                        out[((size_t)y * w + x) * NumStrataForStorage + k] += sp[k];
+                }
             }
     }
 

@@ -220,11 +220,13 @@ class RefScene:
         self.L.refd_visibility(self.h, accel, len(rays), _p(rays), _p(vis))
         return vis
 
-    def render(self, width, height, spp, seed, threads=1, volumetric=False):
-        """Runs PTRenderer (VolumetricPTRenderer with volumetric); returns
-        (film (h, w, 16) float32 sums, seconds, build seconds). Cannot be
-        combined with build_accel on the same object."""
-        self.L.refd_set_integrator(self.h, 1 if volumetric else 0)
+    def render(self, width, height, spp, seed, threads=1, volumetric=False, bpt=False):
+        """Runs PTRenderer (VolumetricPTRenderer with volumetric, BPTRenderer
+        with bpt: its film is the sensor's sums plus its per-thread
+        light-to-lens buffers); returns (film (h, w, 16) float32 sums,
+        seconds, build seconds). Cannot be combined with build_accel on the
+        same object."""
+        self.L.refd_set_integrator(self.h, 2 if bpt else 1 if volumetric else 0)
         film = np.zeros((height, width, 16), np.float32)
         bs = np.zeros(1)
         sec = self.L.refd_render(self.h, width, height, spp, seed & 0xFFFFFFFF, threads, _p(film), 1, _p(bs))
@@ -232,11 +234,12 @@ class RefScene:
             raise RuntimeError("reference render failed")
         return film, sec, bs[0]
 
-    def render_batches(self, width, height, spp, seed, batches, threads=1, volumetric=False):
-        """`batches` renders on one scene build (seeds seed, seed + 1, ...);
-        returns (films (batches, h, w, 16) float32 sums, render seconds,
-        build seconds). Cannot be combined with build_accel or render."""
-        self.L.refd_set_integrator(self.h, 1 if volumetric else 0)
+    def render_batches(self, width, height, spp, seed, batches, threads=1, volumetric=False, bpt=False):
+        """`batches` renders on one scene build (seeds seed, seed + 1, ...;
+        BPT: a fresh build per batch); returns (films (batches, h, w, 16)
+        float32 sums, render seconds, build seconds). Cannot be combined with
+        build_accel or render."""
+        self.L.refd_set_integrator(self.h, 2 if bpt else 1 if volumetric else 0)
         films = np.zeros((batches, height, width, 16), np.float32)
         bs = np.zeros(1)
         sec = self.L.refd_render_batches(self.h, width, height, spp, seed & 0xFFFFFFFF, batches, threads, _p(films), 1,

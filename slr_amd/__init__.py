@@ -641,6 +641,32 @@ class VolumetricPTRenderer(PTRenderer):
     integrator = 1   # SLRH_VOL_PT
 
 
+# the entries slrh_render fills with the bidirectional integrator's counters
+# in place of the medium's (which it does not have)
+BPT_STAT_KEYS = {"medium_events": "light_vertices", "medium_shadows": "eye_vertices", "medium_lookups": "connections",
+                 "medium_cells": "lens_contributions", "medium_guard": "capped_subpaths"}
+
+
+class BPTRenderer(PTRenderer):
+    """BPTRenderer(spp) (libSLR/Renderer/BPTRenderer.h:18-124): light and eye
+    subpaths per pixel sample, every connection between them weighted by the
+    reference's MIS; light vertices seen by the lens add to the pixel they
+    project to. max_depth caps the eye subpath (0: no cap, as the reference)."""
+    integrator = 2   # SLRH_BPT
+
+    @staticmethod
+    def _rename(st):
+        return {BPT_STAT_KEYS.get(k, k): v for k, v in st.items()}
+
+    def render(self, scene, film, seed=1509761209, max_depth=0, slots_per_pixel=0, time_kernels=False, **kw):
+        return self._rename(PTRenderer.render(self, scene, film, seed=seed, max_depth=max_depth,
+                                              slots_per_pixel=slots_per_pixel, time_kernels=time_kernels, **kw))
+
+    def render_batch(self, scene, frames, max_depth=0, slots_per_pixel=0, time_kernels=False, **kw):
+        return self._rename(PTRenderer.render_batch(self, scene, frames, max_depth=max_depth,
+                                                    slots_per_pixel=slots_per_pixel, time_kernels=time_kernels, **kw))
+
+
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
 SERVER_STAT_KEYS = ["completed", "queued", "running", "p50_ms", "p99_ms", "mean_ms", "samples", "busy_seconds"]
 

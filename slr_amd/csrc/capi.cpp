@@ -558,7 +558,7 @@ int slrh_scene_load_text(slrh_scene* s, const char* text, slrh_scene_file_info* 
         info->width = sf.width;
         info->height = sf.height;
         info->spp = sf.spp;
-        info->integrator = sf.method == "Volumetric PT" ? SLRH_VOL_PT : SLRH_PT;
+        info->integrator = sf.method == "Volumetric PT" ? SLRH_VOL_PT : sf.method == "BPT" ? SLRH_BPT : SLRH_PT;
         info->rng_seed = sf.rngSeed;
         info->brightness = sf.brightness;
         info->sensitivity = c.sensitivity > 0 ? c.sensitivity : (float)(1.0f / (M_PI * c.radius * c.radius));
@@ -1093,7 +1093,16 @@ void copyStats(const RenderStats& st, double* stats) {
     stats[24] = st.instEntered;
     stats[25] = st.shadeBlock;
     stats[26] = st.slotsPerPixel;
+    if (st.bptLightVertices + st.bptEyeVertices > 0) {
+        // the bidirectional integrator has no medium: its counters take those entries
+        stats[16] = st.bptLightVertices;
+        stats[17] = st.bptEyeVertices;
+        stats[19] = st.bptConnections;
+        stats[20] = st.bptSplats;
+        stats[22] = st.bptCapped;
+    }
 }
+bool knownIntegrator(uint32_t i) { return i == SLRH_PT || i == SLRH_VOL_PT || i == SLRH_BPT; }
 }  // namespace
 
 int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film, double* stats) {
@@ -1101,8 +1110,7 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "render before slrh_scene_build");
     if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
     if (!s->data.hasCamera) return fail(s->ctx, SLRH_ERR_STATE, "scene has no camera");
-    if (set->integrator != SLRH_PT && set->integrator != SLRH_VOL_PT)
-        return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
+    if (!knownIntegrator(set->integrator)) return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
     if (set->width != film->w || set->height != film->h) return fail(s->ctx, SLRH_ERR_INVALID, "film size mismatch");
     if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
     CtxLock lk = lockOf(s);
@@ -1118,7 +1126,8 @@ int slrh_render(slrh_scene* s, const slrh_render_settings* set, slrh_film* film,
     cfg.tunePool = (set->flags & SLRH_FLAG_TUNE_POOL) != 0;
     RenderStats st;
     std::string err;
-    int r = slrh::render(s->dev, cfg, film->dev, &st, &err);
+    int r = set->integrator == SLRH_BPT ? renderBpt(s->dev, cfg, s->data.camera, film->dev, &st, &err)
+                                        : slrh::render(s->dev, cfg, film->dev, &st, &err);
     if (r) return fail(s->ctx, r, err);
     copyStats(st, stats);
     return SLRH_OK;
@@ -1179,8 +1188,7 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     if (!s || !set || (n && !frames)) return SLRH_ERR_INVALID;
     if (!s->built) return fail(s->ctx, SLRH_ERR_STATE, "render before slrh_scene_build");
     if (!s->dev) return fail(s->ctx, SLRH_ERR_STATE, "host-only context has no device scene");
-    if (set->integrator != SLRH_PT && set->integrator != SLRH_VOL_PT)
-        return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
+    if (!knownIntegrator(set->integrator)) return fail(s->ctx, SLRH_ERR_INVALID, "unknown integrator");
     if (set->reserved != 0) return fail(s->ctx, SLRH_ERR_INVALID, "reserved settings field must be 0");
     if (s->data.materials.empty() || s->data.idx.empty()) return fail(s->ctx, SLRH_ERR_STATE, "empty scene");
     std::vector<JobSpec> specs(n);
@@ -1206,6 +1214,17 @@ int slrh_render_batch(slrh_scene* s, const slrh_frame* frames, uint32_t n, const
     CtxLock lk = lockOf(s);
     RenderStats st;
     std::string err;
+    if (set->integrator == SLRH_BPT) {
+        // the bidirectional integrator renders the frames one after another
+        for (uint32_t i = 0; i < n; ++i) {
+            RenderConfig cfg{specs[i].width, specs[i].height, specs[i].spp, set->max_depth, specs[i].seed,
+                             set->slots_per_pixel, opt.timeKernels};
+            int r = renderBpt(s->dev, cfg, specs[i].camera, specs[i].film, &st, &err);
+            if (r) return fail(s->ctx, r, err);
+        }
+        copyStats(st, stats);
+        return SLRH_OK;
+    }
     int r = renderJobs(s->dev, specs.data(), n, opt, &st, &err);
     if (r) return fail(s->ctx, r, err);
     copyStats(st, stats);

@@ -103,6 +103,10 @@ struct RenderStats {
     double mediumGuard = 0;                       // tracking walks stopped by the step guard
     double nodesLds = 0;                          // QBVH nodes read from the LDS copy of the tree top
     double instEntered = 0;                       // instances entered by the traversal
+    // bidirectional integrator: light and eye subpath vertices, connections
+    // traced (s >= 1, t >= 2), light-to-lens contributions filed (t = 1),
+    // subpaths stopped by the vertex caps
+    double bptLightVertices = 0, bptEyeVertices = 0, bptConnections = 0, bptSplats = 0, bptCapped = 0;
 };
 
 int deviceInit(int device, std::string* err);
@@ -139,6 +143,13 @@ int filmUpload(DeviceFilm* f, const double* strata, std::string* err);   // w*h*
 void filmSize(const DeviceFilm* f, uint32_t* w, uint32_t* h);
 
 int render(DeviceScene* ds, const RenderConfig& cfg, DeviceFilm* film, RenderStats* st, std::string* err);
+
+// BPTRenderer::render (libSLR/Renderer/BPTRenderer.cpp:25-101) on the
+// device: every pixel sample is a light subpath, an eye subpath and all
+// their connections, weighted by the reference's MIS; accumulates into film.
+// cfg.maxDepth caps the eye subpath's vertices (0: no cap).
+int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& camera, DeviceFilm* film, RenderStats* st,
+              std::string* err);
 
 // One frame of a batch: its camera, resolution, samples, depth cap, seed
 // and the film it accumulates into.

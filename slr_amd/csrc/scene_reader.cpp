@@ -979,8 +979,8 @@ void installBuiltins(Interp& in) {
     B["setRenderer"].sigs = {{req("method", Ty::String), opt("config", Ty::Tuple, mkTuple())}};
     B["setRenderer"].procs = {[](Interp& in, const Args& a) {
         const std::string method = a.at("method").s;
-        if (method != "PT" && method != "Volumetric PT")
-            in.error("renderer '" + method + "' is outside the supported subset (PT, Volumetric PT)");
+        if (method != "PT" && method != "Volumetric PT" && method != "BPT")
+            in.error("renderer '" + method + "' is outside the supported subset (PT, Volumetric PT, BPT)");
         Args p;
         if (!mapArgs(*a.at("config").tup, {opt("samples", Ty::Integer, mkInt(8))}, &p))
             in.error("the renderer configuration does not match");

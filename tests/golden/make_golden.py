@@ -9,8 +9,9 @@
   box at 16 x 16: per pixel CIE XYZ, the mean and the standard error over
   32 independent batches of 512 samples per pixel (16384 spp in all).
 * ref_<case>.npz for each case of tests/refcases.py: the reference
-  renderer's image statistics (PTRenderer, or VolumetricPTRenderer for the
-  volumetric cases) that the 3-sigma parity tests compare against.
+  renderer's image statistics (PTRenderer, VolumetricPTRenderer for the
+  volumetric cases, BPTRenderer for the bpt_ cases) that the 3-sigma parity
+  tests compare against.
 
 Run from the repository root: python tests/golden/make_golden.py
 """
@@ -63,7 +64,8 @@ def case(name):
     threads = max(1, min(8, len(os.sched_getaffinity(0))))
     # all batches on one scene build (seeds seed .. seed + B - 1)
     r = refdrv.RefScene(desc)
-    films, sec, build_s = r.render_batches(W, H, spp, seed, B, threads=threads, volumetric=vol)
+    films, sec, build_s = r.render_batches(W, H, spp, seed, B, threads=threads, volumetric=vol is True,
+                                           bpt=vol == "bpt")
     r.close()
     print("%s: %d batches of %dx%d x %d spp in %.1fs (build %.1fs)" % (name, B, W, H, spp, sec, build_s))
     out = np.array([refcases.xyz(f.astype(np.float64), cmf) / spp for f in films])

@@ -23,7 +23,8 @@ def field(W, H, per_instance):
                                   env_size=(64, 32), sun_radiance=200.0, materials_per_instance=per_instance)
 
 
-# name -> (reference scene, W, H, batches, spp per batch, first seed, volumetric integrator)
+# name -> (reference scene, W, H, batches, spp per batch, first seed, integrator: False PTRenderer,
+# True VolumetricPTRenderer, "bpt" BPTRenderer)
 CASES = {
     "cornell32": (lambda: scenes.cornell_box(32, 32), 32, 32, 32, 64, 1000, False),
     "sky": (lambda: scenes.sky_scene(32, 24), 32, 24, 32, 32, 2000, False),
@@ -42,7 +43,21 @@ CASES = {
     "c2": (lambda: scenes.cornell_icosphere(128, 72, subdiv=8), 128, 72, 32, 64, 9000, False),
     "volpt_cornell16": (lambda: dict(scenes.cornell_box(16, 16), compat_volpt_tangent_nee=True), 16, 16, 24, 32, 700,
                         True),
+    # the bidirectional integrator (BPTRenderer): the Cornell box, and a
+    # displaced sphere with smooth normals of each non-Lambert model, each
+    # also rendered by PTRenderer so the two reference integrators can be
+    # checked against each other
+    "bpt_cornell32": (lambda: scenes.cornell_box(32, 32), 32, 32, 32, 64, 11000, "bpt"),
 }
+SPHERE_MATERIALS = {
+    "glass": {"kind": "ggx_dielectric", "eta_ext": 1.0, "eta_int": 1.5, "alpha": 0.1},
+    "metal": {"kind": "ggx_conductor", "eta": 0.2, "k": 3.9, "alpha": 0.08, "alpha_y": 0.4},
+    "on": {"kind": "oren_nayar", "rgb": (0.7, 0.5, 0.3), "sigma": 0.3},
+}
+for _k, (_m, _seed) in {"glass": (SPHERE_MATERIALS["glass"], 12000), "metal": (SPHERE_MATERIALS["metal"], 13000),
+                        "on": (SPHERE_MATERIALS["on"], 14000)}.items():
+    CASES["bpt_" + _k] = ((lambda m=_m: scenes.cornell_sphere(32, 32, m)), 32, 32, 32, 64, _seed, "bpt")
+    CASES["pt_" + _k] = ((lambda m=_m: scenes.cornell_sphere(32, 32, m)), 32, 32, 32, 64, _seed + 500, False)
 
 
 def xyz(strata, cmf):

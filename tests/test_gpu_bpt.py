@@ -1,0 +1,143 @@
+"""The bidirectional integrator (BPTRenderer, libSLR/Renderer/BPTRenderer.cpp)
+on the GPU.
+
+* against the compiled reference BPTRenderer's image statistics
+  (tests/golden/ref_bpt_*.npz: the Cornell box, and a displaced sphere with
+  smooth normals of GGX glass, anisotropic GGX metal and Oren-Nayar), per
+  pixel and XYZ channel within 3 sigma of the combined Monte Carlo error;
+* against the GPU path tracer on the same scene (two unbiased estimators of
+  one image) within 3 sigma;
+* against the white-furnace closed form;
+* its own invariants: every sample filed, no subpath stopped by the vertex
+  caps on these scenes, equal films for equal seeds up to the order of the
+  lens contributions' FP64 atomic adds, batches equal to single renders.
+
+Same-seed equality with the reference is not attempted: the reference draws
+each thread's random numbers in tile order (BPTRenderer.cpp:27-33), so its
+samples cannot be reproduced per pixel.
+"""
+import numpy as np
+import pytest
+
+import slr_amd
+from slr_amd import scenes
+from tests import refcases
+
+pytestmark = pytest.mark.gpu
+
+
+def _batches(ctx, desc, W, H, spp, seed0, B, renderer=slr_amd.BPTRenderer):
+    sc = slr_amd.build_scene(ctx, desc)
+    cmf = slr_amd.cmf_strata()
+    out, stats = [], []
+    for b in range(B):
+        film = slr_amd.Film(ctx, W, H)
+        stats.append(renderer(spp).render(sc, film, seed=seed0 + b))
+        out.append(refcases.xyz(film.strata(), cmf) / spp)
+    return np.array(out), stats
+
+
+@pytest.mark.parametrize("case", ["bpt_cornell32", "bpt_glass", "bpt_metal", "bpt_on"])
+def test_matches_reference_bpt_3sigma(gpu_ctx, case):
+    make, W, H, B, spp, _, _ = refcases.CASES[case]
+    gb, stats = _batches(gpu_ctx, make(), W, H, spp, 900, B)
+    for st in stats:
+        assert st["samples"] == W * H * spp
+        assert st["capped_subpaths"] == 0
+        assert st["nonfinite"] == 0
+    refcases.check_3sigma(gb, refcases.load(case), name=case)
+
+
+def _stats_of(batches):
+    tot = batches.sum(axis=(1, 2, 3))
+    return {"mean": batches.mean(0), "se": batches.std(0, ddof=1) / np.sqrt(len(batches)), "tot_mean": tot.mean(),
+            "tot_se": tot.std(ddof=1) / np.sqrt(len(batches)), "batches": len(batches), "spp": 64}
+
+
+def test_agrees_with_path_tracer(gpu_ctx):
+    """BPT and PT estimate the same image: 128 batches of each on the GPU, on
+    the Oren-Nayar sphere. (With 32 batches the two integrators' differently
+    tailed pixel estimates leave the variance estimates too loose for the
+    tile-cluster bound; on the GGX spheres the path tracer's pixel means are
+    heavy-tailed at this sample count, as the reference's own BPT and PT
+    fixtures show, tests/test_golden.py.)"""
+    W = H = 32
+    desc = scenes.cornell_sphere(W, H, refcases.SPHERE_MATERIALS["on"])
+    bpt, _ = _batches(gpu_ctx, desc, W, H, 64, 50, 128)
+    pt, _ = _batches(gpu_ctx, desc, W, H, 64, 900, 128, renderer=slr_amd.PTRenderer)
+    refcases.check_3sigma(bpt, _stats_of(pt), name="bpt_vs_pt_on")
+    # connecting light subpaths lowers the per-pixel noise of this scene
+    assert np.median(bpt.std(0)) < np.median(pt.std(0))
+
+
+def test_white_furnace(gpu_ctx):
+    W = H = 16
+    spp = 512
+    desc = scenes.furnace(W, H, albedo=0.5, emittance=1.0)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    film = slr_amd.Film(gpu_ctx, W, H)
+    st = slr_amd.BPTRenderer(spp).render(sc, film, seed=5)
+    expect = scenes.pixel_cos4(desc, W, H) * (1.0 / np.pi) / (1 - 0.5) * spp
+    r = film.strata().mean(axis=2) / expect
+    assert abs(r.mean() - 1) < 0.01, r.mean()
+    assert st["samples"] == W * H * spp and st["capped_subpaths"] == 0
+    assert st["light_vertices"] > 0 and st["connections"] > 0 and st["lens_contributions"] > 0
+
+
+def test_same_seed_same_film(gpu_ctx):
+    W = H = 24
+    desc = scenes.cornell_sphere(W, H, refcases.SPHERE_MATERIALS["glass"])
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    films = []
+    for _ in range(2):
+        f = slr_amd.Film(gpu_ctx, W, H)
+        slr_amd.BPTRenderer(32).render(sc, f, seed=77)
+        films.append(f.strata())
+    a, b = films
+    assert np.isfinite(a).all() and a.sum() > 0
+    # the only freedom: the order of the lens contributions' atomic adds
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(a).max())
+
+
+def test_slots_per_pixel_do_not_change_the_estimate(gpu_ctx):
+    W = H = 32
+    desc = scenes.cornell_box(W, H)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    tots = []
+    for k, seed in ((1, 3), (16, 4)):
+        f = slr_amd.Film(gpu_ctx, W, H)
+        st = slr_amd.BPTRenderer(256).render(sc, f, seed=seed, slots_per_pixel=k)
+        assert st["slots_per_pixel"] == k and st["samples"] == W * H * 256
+        tots.append(f.strata().sum())
+    assert abs(tots[0] / tots[1] - 1) < 0.01
+
+
+def test_batch_equals_frames_alone(gpu_ctx):
+    """slrh_render_batch with the bidirectional integrator: each frame's film
+    equals that frame rendered alone with its seed (up to the order of the
+    lens contributions' atomic adds)."""
+    desc = scenes.cornell_box(24, 24)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    frames = []
+    for i, (w, h, yaw) in enumerate(((24, 24, 0.0), (16, 12, 8.0))):
+        frames.append({"film": slr_amd.Film(gpu_ctx, w, h), "camera": scenes.camera_pose(desc, yaw, width=w, height=h),
+                       "seed": 40 + i, "spp": 16})
+    st = slr_amd.BPTRenderer(16).render_batch(sc, frames)
+    assert st["samples"] == 16 * (24 * 24 + 16 * 12)
+    for f in frames:
+        ref = slr_amd.Film(gpu_ctx, f["film"].width, f["film"].height)
+        slr_amd.BPTRenderer(16).render_batch(sc, [dict(f, film=ref)])
+        r = ref.strata()
+        assert r.sum() > 0
+        np.testing.assert_allclose(f["film"].strata(), r, rtol=1e-12, atol=1e-12 * np.abs(r).max())
+
+
+def test_refuses_environment_light(gpu_ctx):
+    desc = scenes.sky_scene(16, 12)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    film = slr_amd.Film(gpu_ctx, 16, 12)
+    with pytest.raises(slr_amd.SlrhError, match="environment"):
+        slr_amd.BPTRenderer(4).render(sc, film, seed=1)
+    # the scene stays usable
+    slr_amd.PTRenderer(4).render(sc, film, seed=1)
+    assert film.strata().sum() > 0

@@ -74,6 +74,13 @@ setRenderer("method": "PT", ("samples": 12,));
     assert info["sensitivity"] == math.inf   # a pinhole without a sensitivity: 1 / (pi r^2)
     assert (info["width"], info["height"], info["spp"]) == (1024, 1024, 8)
     sc.close()
+    # the renderer methods map to the integrators (API.cpp:1222-1254):
+    # PT 0, Volumetric PT 1, BPT 2; samples defaults to 8
+    for method, integ in (("PT", 0), ("Volumetric PT", 1), ("BPT", 2)):
+        sc, info = load(host_ctx, MATTE + one_triangle("(0,0,0)", "(1,0,0)", "(0,1,0)") + CAMERA +
+                        'setRenderer("method": "%s");\n' % method)
+        assert (info["integrator"], info["spp"]) == (integ, 8)
+        sc.close()
 
 
 def test_spectrum_forms(host_ctx):
@@ -141,7 +148,7 @@ addChild(root, cam);
     ("for (i = 0; i < 3; ++i) {}", "line 1: 'for' statements are outside the supported subset"),
     ("\n\nx = load3DModel(\"a.obj\");", "line 3: 'load3DModel' is outside the supported subset"),
     ("m = createSurfaceMaterial(\"metal\", (,));", "material type 'metal' is outside the supported subset"),
-    ("setRenderer(\"method\": \"BPT\");", "renderer 'BPT' is outside the supported subset"),
+    ("setRenderer(\"method\": \"Volumetric BPT\");", "renderer 'Volumetric BPT' is outside the supported subset"),
     ("x = Spectrum(\"Reflectance\");", "the arguments do not match Spectrum()"),
     ("x = (1, 2;", "expected ')'"),
     (MATTE + "addChild(root, createMesh((((1,2,3), (0,0,0), (0,0,0), (0,0)),), ((m, ((0, 1, 2),)),)));",
