@@ -695,3 +695,52 @@ def random_rays(desc, n, seed=1, mode="mixed"):
     r[:, 6] = 0.0
     r[:, 7] = np.inf
     return r
+
+
+def to_scene_text(desc, width, height, spp, method="PT", brightness=1.0):
+    """The scene of `desc` written in SLR's scene language (the subset
+    slrh_scene_load_text reads; libSLRSceneGraph/API.cpp builtins): sRGB
+    spectra, matte and emitter materials, one createMesh per mesh with its
+    vertices in the reference scene's units under setTransform(root,
+    scale(0.01)) as TestScenes/Original_Cornell_Box.txt does, the camera
+    under its own node, renderer and settings. Lambert and emitter materials
+    with RGB values only (the Cornell boxes)."""
+    out = ['setRenderer("method": "%s", ("samples": %d,));' % (method, spp),
+           'setRenderSettings("width": %d, "height": %d, "brightness": %r);' % (width, height, brightness)]
+    for i, m in enumerate(desc["materials"]):
+        if m["kind"] == "lambert" and "rgb" in m:
+            out.append('mat%d = createSurfaceMaterial("matte", (SpectrumTexture(Spectrum("Reflectance", "sRGB", %r, %r, '
+                       '%r)),));' % ((i,) + tuple(float(c) for c in m["rgb"])))
+        elif m["kind"] == "emitter" and "rgb" in m:
+            out.append('mat%d = createSurfaceMaterial("emitter", (mat%d, createEmitterSurfaceProperty("diffuse", '
+                       '(SpectrumTexture(Spectrum("LightSource", "sRGB", %r, %r, %r)),))));'
+                       % ((i, m["base"]) + tuple(float(c) for c in m["rgb"])))
+        else:
+            raise ValueError("to_scene_text: material %r" % (m,))
+    for mesh in desc["meshes"]:
+        pos = np.asarray(mesh["pos"], np.float64) / SCALE
+        nrm = np.asarray(mesh["nrm"], np.float64)
+        tan = np.asarray(mesh["tan"], np.float64)
+        idx = np.asarray(mesh["idx"], np.int64)
+        mat = mesh["mat"]
+        verts = ",\n".join("((%.9g, %.9g, %.9g), (%.9g, %.9g, %.9g), (%.9g, %.9g, %.9g), (0, 0))"
+                           % (tuple(p) + tuple(n) + tuple(t)) for p, n, t in zip(pos, nrm, tan))
+        mats = np.full(len(idx), mat) if np.isscalar(mat) else np.asarray(mat)
+        groups = []
+        for mid in sorted(set(int(x) for x in mats)):
+            tris = ", ".join("(%d, %d, %d)" % tuple(t) for t in idx[mats == mid])
+            groups.append("(mat%d, (%s,))" % (mid, tris))
+        out.append("addChild(root, createMesh((%s), (%s,)));" % (verts, ", ".join(groups)))
+    cam = desc["camera"]
+    l2w = np.asarray(cam["l2w"], np.float64)
+    if not np.allclose(l2w[:3, :3], np.eye(3)):
+        raise ValueError("to_scene_text: camera rotation")
+    t = l2w[:3, 3] / SCALE
+    out.append('cameraNode = createNode();')
+    out.append('addChild(cameraNode, createPerspectiveCamera("aspect": %r, "fovY": %r, "radius": %r, "imgDist": %r, '
+               '"objDist": %r));' % (float(cam["aspect"]), float(cam["fov_y"]), float(cam.get("lens_radius", 0.0)),
+                                     float(cam.get("img_dist", 1.0)), float(cam.get("obj_dist", 1.0))))
+    out.append('setTransform(cameraNode, translate(%.9g, %.9g, %.9g));' % tuple(t))
+    out.append('addChild(root, cameraNode);')
+    out.append('setTransform(root, scale(%r));' % SCALE)
+    return "\n".join(out) + "\n"

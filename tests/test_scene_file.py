@@ -230,3 +230,25 @@ def test_original_cornell_box(host_ctx):
     assert np.array_equal(np.diag(l2w)[:3], [f32(0.01)] * 3)
     assert cam["fov_y"] == float(f32(0.6860478808)) and cam["obj_dist"] == 4.5 and cam["img_dist"] == 1.0
     sc.close()
+
+
+def test_scene_text_writer_round_trip(host_ctx):
+    """slr_amd.scenes.to_scene_text writes a description the reader turns
+    back into the same world: positions within an FP32 ulp of the
+    description's (the file's units times scale(0.01) in FP32), the normals
+    the reference recomputes equal to the flat faces' normals, the camera at
+    the same place."""
+    from slr_amd import scenes
+    desc = scenes.cornell_box(32, 32)
+    sc, info = load(host_ctx, scenes.to_scene_text(desc, 32, 32, 64, method="BPT"))
+    assert (info["width"], info["height"], info["spp"], info["integrator"]) == (32, 32, 64, 2)
+    g = sc.geometry()
+    pos = np.concatenate([m["pos"] for m in desc["meshes"]])
+    nrm = np.concatenate([m["nrm"] for m in desc["meshes"]])
+    assert g["pos"].shape == pos.shape
+    np.testing.assert_allclose(g["pos"], pos, rtol=0, atol=4e-7)
+    np.testing.assert_allclose(g["nrm"], nrm, rtol=0, atol=2e-7)
+    l2w, cam = sc.camera()
+    np.testing.assert_allclose(l2w[:3, 3], np.asarray(desc["camera"]["l2w"])[:3, 3], rtol=1e-6)
+    assert cam["fov_y"] == pytest.approx(desc["camera"]["fov_y"], rel=1e-6)
+    sc.close()
