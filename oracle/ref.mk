@@ -22,7 +22,7 @@ CPPS    := $(filter-out $(foreach e,$(EXCL),%$(e).cpp),$(shell find $(SRC) -name
 CS      := $(filter-out $(foreach e,$(EXCL),%$(e).c),$(shell find $(SRC) -name '*.c'))
 OBJS    := $(patsubst $(SRC)/%.cpp,$(OUT)/obj/%.o,$(CPPS)) $(patsubst $(SRC)/%.c,$(OUT)/obj/%.o,$(CS))
 
-all: $(OUT)/libslr_ref.so $(OUT)/libslr_refdrv.so
+all: $(OUT)/libslr_ref.so $(OUT)/libslr_refdrv.so $(OUT)/adapter_check
 
 $(OUT)/obj/%.o: $(SRC)/%.cpp
 	@mkdir -p $(dir $@)
@@ -40,3 +40,10 @@ $(OUT)/libslr_refdrv.so: oracle/ref_driver.cpp $(OUT)/libslr_ref.so
 
 clean:
 	rm -rf $(OUT)
+
+# a libSLR host program using include/slr_hip_renderer.hpp (needs
+# slr_amd/_lib/libslr_hip.so, built first by slr_amd/build.py)
+# (libslr_refdrv.so supplies the CMF-strata statics libslr_ref.so declares)
+$(OUT)/adapter_check: oracle/adapter_check.cpp include/slr_hip_renderer.hpp include/slr_hip.h $(OUT)/libslr_refdrv.so slr_amd/_lib/libslr_hip.so
+	$(CXX) $(CXXFLAGS) -Iinclude -o $@ oracle/adapter_check.cpp -L$(OUT) -Wl,--no-as-needed -lslr_refdrv -Wl,--as-needed -lslr_ref -Lslr_amd/_lib -lslr_hip \
+		-Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../slr_amd/_lib' -lpthread

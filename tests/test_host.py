@@ -186,3 +186,15 @@ def test_context_calls_from_many_threads(host_ctx):
         t.join()
     assert not errors, errors[:3]
     sc.close()
+
+
+def test_reference_side_adapter_links():
+    """include/slr_hip_renderer.hpp compiles against the reference's headers
+    and the adapter program links libSLR and libslr_hip.so (oracle/ref.mk);
+    without arguments it prints its usage and exits 2 (no device touched)."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "adapter_check")
+    if not os.path.exists(exe):
+        pytest.skip("adapter program not built (needs the reference sources: make -f oracle/ref.mk)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
