@@ -301,6 +301,13 @@ SLRH_API int slrh_intersect(slrh_scene* s, const slrh_rays* rays, const slrh_hit
 SLRH_API int slrh_occluded(slrh_scene* s, const slrh_rays* rays, float* visibility, uint32_t n,
                            int mem, void* stream);
 
+/* Traversal work of the scene's last slrh_intersect / slrh_occluded batch
+ * through the QBVH (no reference counterpart; for measurement): QBVH nodes
+ * popped (each tests its four child boxes) and triangles tested, summed over
+ * the batch; the counts an instrumented QBVH::commonProcedure
+ * (libSLR/Accelerator/QBVH.h:294-341) gives for the same rays. */
+SLRH_API int slrh_scene_query_counts(const slrh_scene* s, double* nodes, double* tris);
+
 /* slrh_intersect for device pointers that allocates nothing: no counterpart
  * in the reference; it is for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid

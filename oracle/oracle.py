@@ -50,6 +50,7 @@ def lib():
             "orc_scene_compat": (None, [vp, i32]),
             "orc_intersect": (i32, [vp, i32, vp, vp, vp, vp, vp]),
             "orc_visibility": (i32, [vp, i32, vp, vp]),
+            "orc_qbvh_trace": (i32, [vp, vp, vp, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp]),
             "orc_render": (i64, [vp, u32, u32, u32, u32, u32, u32, u32, u32, vp, vp]),
         }
         for k, (r, a) in sig.items():
@@ -200,6 +201,32 @@ class OracleScene:
         b0, b1 = np.zeros(n, np.float32), np.zeros(n, np.float32)
         self.L.orc_intersect(self.h, n, _p(rays), _p(t), _p(prim), _p(b0), _p(b1))
         return t, prim, b0, b1
+
+    def qbvh_trace(self, dump, rays, any_hit=False):
+        """QBVH::commonProcedure over a QBVH dump (Scene.qbvh_dump of the
+        same scene: boxes, info, refs): (t, prim, b0, b1) for closest hits or
+        the visibility for any hit, and the nodes popped and triangles
+        tested over all rays."""
+        boxes, info, refs = dump[0], dump[1], dump[2]
+        boxes = np.ascontiguousarray(boxes, np.float32)
+        info = np.ascontiguousarray(info, np.uint32)
+        refs = np.ascontiguousarray(refs, np.int32)
+        # every reference in exactly one leaf (leaf counts fit the 4-bit field)
+        words = info[:, :4].ravel()
+        leaf = (words != 0xFFFFFFFF) & ((words >> 31) == 1)
+        assert int(((words[leaf] >> 27) & 0xF).sum()) == len(refs), "a leaf count does not fit the dump's 4 bits"
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        t, prim = np.zeros(n, np.float32), np.zeros(n, np.int32)
+        b0, b1, vis = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        counts = np.zeros(2)
+        r = self.L.orc_qbvh_trace(self.h, _p(boxes), _p(info), _p(refs), n, _p(rays), 1 if any_hit else 0, _p(t),
+                                  _p(prim), _p(b0), _p(b1), _p(vis), _p(counts))
+        if r != 0:
+            raise RuntimeError("QBVH traversal stack overflow")
+        if any_hit:
+            return vis, counts[0], counts[1]
+        return (t, prim, b0, b1), counts[0], counts[1]
 
     def visibility(self, rays):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

@@ -621,6 +621,95 @@ int orc_intersect(void* p, int n, const float* rays, float* t, int32_t* prim, fl
     return 0;
 }
 
+/* QBVH traversal with counters: QBVH::commonProcedure (libSLR/Accelerator/
+ * QBVH.h:294-341) over the tree as slrh_scene_qbvh_dump writes it (per node
+ * 24 floats: min x/y/z then max x/y/z, 4 children each; 7 words: the four
+ * children as the reference's Children bitfield {idx:27, numLeaves:4,
+ * isLeaf:1}, all ones when invalid, then the top, left and right split
+ * axes), Node::intersect's slab test (:54-75) in scalar form with the same
+ * operations, the triangles of each leaf tested in order with the segment's
+ * far end narrowed to each hit (closest hit), or stopping at the first hit
+ * (any). counts[0] += nodes popped (each tests its four child boxes),
+ * counts[1] += triangles tested: the per-ray work the algorithmic bytes of
+ * a traversal are priced on. Returns -1 on a stack overflow. */
+int orc_qbvh_trace(void* p, const float* boxes, const uint32_t* info, const int32_t* refs, int n,
+                   const float* rays, int any, float* tout, int32_t* pout, float* b0out, float* b1out,
+                   float* vis, double* counts) {
+    const scene_t* s = (const scene_t*)p;
+    static const uint32_t order_table[8] = {0x0123, 0x0132, 0x1023, 0x1032, 0x2301, 0x3201, 0x2310, 0x3210};
+    double nodes = 0, tris = 0;
+    for (int i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        const v3 o = V(r[0], r[1], r[2]), d = V(r[3], r[4], r[5]);
+        const float tmin = r[6];
+        float tmax = r[7];
+        const float inv[3] = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+        const float org[3] = {o.x, o.y, o.z};
+        const int pos[3] = {d.x >= 0, d.y >= 0, d.z >= 0};
+        uint32_t stack[64];
+        int depth = 0, prim = -1, hit = 0;
+        float hb1 = 0, hb2 = 0;
+        stack[depth++] = 0;
+        while (depth > 0 && !hit) {
+            const uint32_t ni = stack[--depth];
+            const float* bx = boxes + 24 * (size_t)ni;
+            const uint32_t* in = info + 7 * (size_t)ni;
+            nodes += 1;
+            uint32_t flags = 0;
+            for (int c = 0; c < 4; ++c) {
+                float tn = tmin, tf = tmax;
+                for (int a = 0; a < 3; ++a) {
+                    const float lo = bx[4 * a + c], hi = bx[12 + 4 * a + c];
+                    const float nearB = inv[a] > 0.0f ? lo : hi, farB = inv[a] > 0.0f ? hi : lo;
+                    const float t0 = (nearB - org[a]) * inv[a], t1 = (farB - org[a]) * inv[a];
+                    /* _mm_max_ps(x, tNear) / _mm_min_ps(x, tFar): the second
+                     * operand when either is NaN */
+                    tn = t0 > tn ? t0 : tn;
+                    tf = t1 < tf ? t1 : tf;
+                }
+                if (tn <= tf) flags |= 1u << c;
+            }
+            if (flags == 0) continue;
+            const uint32_t enc = order_table[4 * pos[in[4]] + 2 * pos[in[5]] + pos[in[6]]];
+            uint32_t ch[4];
+            int ord[4];
+            for (int k = 0; k < 4; ++k) {
+                ord[k] = (int)((enc >> (4 * k)) & 0xF);
+                ch[k] = ((flags >> ord[k]) & 1u) ? in[ord[k]] : 0xFFFFFFFFu;
+            }
+            for (int k = 3; k >= 0; --k) {
+                if (ch[k] == 0xFFFFFFFFu || (ch[k] >> 31)) continue;
+                if (depth >= 64) return -1;
+                stack[depth++] = ch[k] & 0x07FFFFFFu;
+            }
+            for (int k = 0; k < 4 && !hit; ++k) {
+                if (ch[k] == 0xFFFFFFFFu || !(ch[k] >> 31)) continue;
+                const uint32_t first = ch[k] & 0x07FFFFFFu, cnt = (ch[k] >> 27) & 0xFu;
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    const int32_t t = refs[first + j];
+                    float tt, u, v;
+                    tris += 1;
+                    if (tri_test(s, (uint32_t)t, o, d, tmin, tmax, &tt, &u, &v)) {
+                        if (any) { hit = 1; break; }
+                        tmax = tt; prim = t; hb1 = u; hb2 = v;
+                    }
+                }
+            }
+        }
+        if (any) {
+            if (vis) vis[i] = hit ? 0.0f : 1.0f;
+        } else {
+            tout[i] = tmax;
+            pout[i] = prim;
+            b0out[i] = prim >= 0 ? 1.0f - hb1 - hb2 : 0.0f;
+            b1out[i] = prim >= 0 ? hb1 : 0.0f;
+        }
+    }
+    counts[0] += nodes;
+    counts[1] += tris;
+    return 0;
+}
+
 int orc_visibility(void* p, int n, const float* rays, float* vis) {
     const scene_t* s = (const scene_t*)p;
     for (int i = 0; i < n; ++i) {

@@ -38,7 +38,7 @@ EXPORTS = [
     "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
     "slrh_scene_accelerators", "slrh_scene_query_accel", "slrh_scene_standard_bvh_dump",
     "slrh_scene_load_text", "slrh_scene_geometry", "slrh_scene_camera_get", "slrh_material_info",
-    "slrh_intersect", "slrh_occluded", "slrh_intersect_noalloc",
+    "slrh_intersect", "slrh_occluded", "slrh_intersect_noalloc", "slrh_scene_query_counts",
     "slrh_film_create", "slrh_film_destroy", "slrh_film_clear", "slrh_film_read", "slrh_film_load", "slrh_film_bmp",
     "slrh_render_progressive",
     "slrh_environment", "slrh_render", "slrh_render_batch", "slrh_trace_variants",
@@ -145,6 +145,7 @@ def load_library(path=None):
         "slrh_scene_camera_get": (i32, [vp, vp, vp]),
         "slrh_material_info": (i32, [vp, i32, pi, pi, pi, pi, vp, vp]),
         "slrh_intersect": (i32, [vp, vp, vp, u32, i32, vp]),
+        "slrh_scene_query_counts": (i32, [vp, vp, vp]),
         "slrh_occluded": (i32, [vp, vp, vp, u32, i32, vp]),
         "slrh_intersect_noalloc": (i32, [vp, vp, vp, u32, vp]),
         "slrh_film_create": (i32, [vp, u32, u32, pp]),
@@ -460,6 +461,14 @@ class Scene:
         h = _Hits(*(list(outs) + [None] * (5 - len(outs))))
         self.ctx.check(self.lib.slrh_intersect_noalloc(self.h, ctypes.byref(r), ctypes.byref(h), n, stream),
                        "slrh_intersect_noalloc")
+
+    def query_counts(self):
+        """(QBVH nodes popped, triangles tested) of the last intersect /
+        occluded batch (slrh_scene_query_counts)."""
+        n, t = ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(self.lib.slrh_scene_query_counts(self.h, ctypes.byref(n), ctypes.byref(t)),
+                       "slrh_scene_query_counts")
+        return n.value, t.value
 
     def occluded(self, rays):
         rays = _f32(rays).reshape(-1, 8)

@@ -38,6 +38,7 @@ struct slrh_scene {
     bool built = false;
     uint32_t numMeshes = 0;
     int queryAccel = SLRH_QUERY_QBVH;   // accelerator slrh_intersect / slrh_occluded use
+    double lastNodes = 0, lastTris = 0;   // traversal work of the last QBVH ray batch
 };
 
 struct slrh_film {
@@ -818,10 +819,20 @@ static int doTrace(slrh_scene* s, const slrh_rays* rays, const slrh_hits* hits, 
     }
     CtxLock lk = lockOf(s);
     std::string err;
+    s->lastNodes = s->lastTris = 0;
     int r = s->queryAccel == SLRH_QUERY_STANDARD_BVH
                 ? traceBatchStdBvh(s->dev, a, n, hits == nullptr, mem == SLRH_MEM_DEVICE, stream, &err)
-                : traceBatch(s->dev, a, n, hits == nullptr, mem == SLRH_MEM_DEVICE, stream, nullptr, nullptr, &err);
+                : traceBatch(s->dev, a, n, hits == nullptr, mem == SLRH_MEM_DEVICE, stream, &s->lastNodes,
+                             &s->lastTris, &err);
     if (r) return fail(s->ctx, r, err);
+    return SLRH_OK;
+}
+
+int slrh_scene_query_counts(const slrh_scene* s, double* nodes, double* tris) {
+    if (!s) return SLRH_ERR_INVALID;
+    CtxLock lk = lockOf(s);
+    if (nodes) *nodes = s->lastNodes;
+    if (tris) *tris = s->lastTris;
     return SLRH_OK;
 }
 

@@ -96,3 +96,31 @@ def test_empty_batch(gpu_ctx):
     sc = build_scene(gpu_ctx, scenes.cornell_box(8, 8))
     t, prim, b0, b1 = sc.intersect(np.zeros((0, 8), np.float32))
     assert len(t) == 0
+
+
+@pytest.mark.parametrize("name", ["cornell", "icosphere6", "c2"])
+def test_traversal_counters_match_an_instrumented_cpu_traversal(gpu_ctx, name):
+    """The GPU traversal's node and triangle counters (the algorithmic bytes
+    of the roofline are priced on them) equal an independent count over the
+    same rays: QBVH::commonProcedure restated on the CPU with counters
+    (oracle orc_qbvh_trace) over the same tree, for closest and any hits."""
+    from oracle import oracle
+    desc = {"cornell": lambda: scenes.cornell_box(16, 16),
+            "icosphere6": lambda: scenes.cornell_sphere(16, 16, subdiv=6),
+            "c2": lambda: scenes.cornell_icosphere(64, 36, subdiv=8)}[name]()
+    sc = build_scene(gpu_ctx, desc)
+    rays = scenes.random_rays(desc, 1 << 14, seed=29)
+    orc = oracle.OracleScene(desc)
+    dump = sc.qbvh_dump()
+    (t, prim, _, _), n_cpu, t_cpu = orc.qbvh_trace(dump, rays)
+    gt, gprim, _, _ = sc.intersect(rays)
+    n_gpu, t_gpu = sc.query_counts()
+    assert np.array_equal(gt.view(np.uint32), t.view(np.uint32)) and np.array_equal(gprim, prim)
+    assert (n_gpu, t_gpu) == (n_cpu, t_cpu)
+    vis, na_cpu, ta_cpu = orc.qbvh_trace(dump, rays, any_hit=True)
+    gvis = sc.occluded(rays)
+    na_gpu, ta_gpu = sc.query_counts()
+    assert np.array_equal(gvis, vis)
+    assert (na_gpu, ta_gpu) == (na_cpu, ta_cpu)
+    print("counts %s: %.3f nodes, %.3f triangles per closest-hit ray; %.3f, %.3f per any-hit ray" % (
+        name, n_cpu / len(rays), t_cpu / len(rays), na_cpu / len(rays), ta_cpu / len(rays)))

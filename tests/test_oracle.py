@@ -209,3 +209,29 @@ def test_intersect_matches_reference_qbvh():
     seg[:, 6] = 1e-4
     seg[:, 7] = np.where(rp < 0, 10.0, rt * 0.999)
     assert np.array_equal(ref.visibility(seg), orc.visibility(seg))
+
+
+@pytest.mark.parametrize("mode", ["mixed", "axis"])
+def test_instrumented_qbvh_traversal_matches_reference_qbvh(host_ctx, mode):
+    """orc_qbvh_trace (QBVH::commonProcedure restated with counters, over the
+    tree this library builds, which equals the reference's node for node)
+    finds the reference QBVH's closest hits bit for bit, ties included (they
+    are decided by the visiting order), and its any-hit visibility; the
+    GPU's counters are checked against its counts in test_gpu_traversal."""
+    refdrv = need_ref()
+    import slr_amd
+    desc = scenes.cornell_sphere(16, 16, subdiv=5)
+    rays = scenes.random_rays(desc, 4096, seed=17, mode=mode)
+    sc = slr_amd.build_scene(host_ctx, desc)
+    (t, prim, b0, b1), nodes, tris = oracle.OracleScene(desc).qbvh_trace(sc.qbvh_dump(), rays)
+    ref = refdrv.RefScene(desc)
+    ref.build_accel()
+    rt, rp, rb0, rb1 = ref.intersect(rays)[:4]
+    assert np.array_equal(t.view(np.uint32), rt.view(np.uint32))
+    assert np.array_equal(prim, rp)
+    hit = prim >= 0
+    assert np.array_equal(b0[hit].view(np.uint32), rb0[hit].view(np.uint32))
+    assert np.array_equal(b1[hit].view(np.uint32), rb1[hit].view(np.uint32))
+    vis, _, _ = oracle.OracleScene(desc).qbvh_trace(sc.qbvh_dump(), rays, any_hit=True)
+    assert np.array_equal(vis, ref.visibility(rays))
+    assert nodes > len(rays) and tris > 0
