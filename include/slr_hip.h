@@ -258,6 +258,14 @@ SLRH_API int slrh_scene_standard_bvh_dump(const slrh_scene* s, float* boxes, uin
 /* stats[0..7]: SBVH nodes, SBVH leaf refs, SBVH depth, SBVH SAH cost,
  * QBVH nodes, QBVH leaf refs, QBVH depth, QBVH SAH cost; stats[8]: build seconds. */
 SLRH_API int slrh_scene_accel_stats(const slrh_scene* s, double* stats);
+/* stats[0]: seconds of all the scene's SBVH builds and QBVH collapses (world
+ * tree and prototype trees); stats[1]: SBVH nodes of them binned and
+ * partitioned on the device. With a device context, the nodes of at least
+ * 2^16 fragments at the top of a triangle tree keep their fragments in
+ * device memory and are binned and partitioned there, with the same
+ * results as the host builder (the tree is the same node for node); the
+ * host builds the rest of the tree on its threads. */
+SLRH_API int slrh_scene_build_stats(const slrh_scene* s, double* stats);
 /* Topology dumps in the same layout as the reference's node arrays (for
  * parity tests): SBVH boxes[6*n], info[5*n] {c0,c1,offset,numLeaves,axis|0xFF},
  * refs[leaf refs]; QBVH boxes[24*n], info[7*n] {children[4] in the reference's

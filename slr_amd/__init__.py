@@ -35,7 +35,7 @@ EXPORTS = [
     "slrh_mesh_add", "slrh_prototype_add", "slrh_instance_add", "slrh_instance_get", "slrh_scene_compat",
     "slrh_medium_density_grid", "slrh_medium_query", "slrh_camera_perspective",
     "slrh_scene_create", "slrh_scene_build", "slrh_scene_destroy",
-    "slrh_scene_accel_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
+    "slrh_scene_accel_stats", "slrh_scene_build_stats", "slrh_scene_sbvh_dump", "slrh_scene_qbvh_dump",
     "slrh_scene_accelerators", "slrh_scene_query_accel", "slrh_scene_standard_bvh_dump",
     "slrh_scene_load_text", "slrh_scene_geometry", "slrh_scene_camera_get", "slrh_material_info",
     "slrh_intersect", "slrh_occluded", "slrh_intersect_noalloc", "slrh_scene_query_counts",
@@ -135,6 +135,7 @@ def load_library(path=None):
         "slrh_scene_build": (i32, [vp]),
         "slrh_scene_destroy": (i32, [vp]),
         "slrh_scene_accel_stats": (i32, [vp, vp]),
+        "slrh_scene_build_stats": (i32, [vp, vp]),
         "slrh_scene_sbvh_dump": (i32, [vp, vp, vp, vp, vp, vp]),
         "slrh_scene_qbvh_dump": (i32, [vp, vp, vp, vp, vp, vp]),
         "slrh_scene_accelerators": (i32, [vp, u32]),
@@ -354,6 +355,13 @@ class Scene:
         keys = ["sbvh_nodes", "sbvh_refs", "sbvh_depth", "sbvh_cost", "qbvh_nodes", "qbvh_refs", "qbvh_depth",
                 "qbvh_cost", "build_seconds"]
         return dict(zip(keys, st.tolist()))
+
+    def build_stats(self):
+        """{build_seconds, device_nodes}: all SBVH builds of the scene, and
+        their nodes binned on the device (slrh_scene_build_stats)."""
+        st = np.zeros(2)
+        self.ctx.check(self.lib.slrh_scene_build_stats(self.h, _ptr(st)), "slrh_scene_build_stats")
+        return {"build_seconds": float(st[0]), "device_nodes": int(st[1])}
 
     def _dump(self, fn, nbox, ninfo, name):
         nn, nr = ctypes.c_uint32(), ctypes.c_uint32()

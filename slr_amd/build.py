@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libslr_hip.so")
 
-SOURCES = ["capi.cpp", "bvh_build.cpp", "standard_bvh.cpp", "scene_reader.cpp", "spectral.cpp", "server.cpp", "render.hip", "bpt.hip"]
+SOURCES = ["capi.cpp", "bvh_build.cpp", "standard_bvh.cpp", "scene_reader.cpp", "spectral.cpp", "server.cpp", "render.hip", "bpt.hip", "sbvh_gpu.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off keeps every multiply and add separately rounded, as the

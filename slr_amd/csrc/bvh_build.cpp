@@ -14,11 +14,16 @@
 //
 // What differs is the machinery: fragment lists are per-node vectors rather
 // than one shared array shifted with copy_backward (whose 5x budget the
-// reference never checks), and independent subtrees are built on separate
-// threads, then stitched back into DFS pre-order.
+// reference never checks), independent subtrees are built on separate
+// threads, then stitched back into DFS pre-order, and with a device the
+// nodes of at least kDeviceNodeFrags fragments at the top of a triangle tree
+// keep their fragments in device memory, where their bounds, bins and
+// partitions are computed (sbvh_gpu.hip) with the same results.
 #include "host_scene.h"
+#include "sbvh_gpu.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <future>
 #include <memory>
@@ -130,6 +135,31 @@ void splitBoxBounds(const Box& b, int axis, float pos, Box* left, Box* right) {
     right->mn[axis] = std::max(right->mn[axis], pos);
 }
 
+// Nodes with at least this many fragments are binned and partitioned on
+// the device when the build has one.
+constexpr size_t kDeviceNodeFrags = 1u << 16;
+
+// One node's fragments: a host vector, or (dev set) device memory.
+struct NodeFrags {
+    std::vector<Frag> host;
+    DevFrags* dev = nullptr;
+    NodeFrags() {}
+    explicit NodeFrags(std::vector<Frag>&& h) : host(std::move(h)) {}
+    NodeFrags(NodeFrags&& o) noexcept : host(std::move(o.host)), dev(o.dev) { o.dev = nullptr; }
+    NodeFrags& operator=(NodeFrags&& o) noexcept {
+        if (this != &o) {
+            if (dev) SbvhGpu::release(dev);
+            host = std::move(o.host);
+            dev = o.dev;
+            o.dev = nullptr;
+        }
+        return *this;
+    }
+    NodeFrags(const NodeFrags&) = delete;
+    ~NodeFrags() { if (dev) SbvhGpu::release(dev); }
+    size_t size() const { return dev ? SbvhGpu::count(dev) : host.size(); }
+};
+
 // A subtree in DFS pre-order with local indices.
 struct Subtree {
     std::vector<SbvhNode> nodes;
@@ -141,10 +171,16 @@ struct Subtree {
 class Builder {
 public:
     Builder(const std::vector<Tri>& tris, const std::vector<Box>& boxes, const std::vector<float>& boxCosts,
-            float rootArea, int parallelDepth)
-        : m_tris(tris), m_boxes(boxes), m_boxCosts(boxCosts), m_rootArea(rootArea), m_parDepth(parallelDepth) {}
+            float rootArea, int parallelDepth, SbvhGpu* gpu = nullptr)
+        : m_tris(tris), m_boxes(boxes), m_boxCosts(boxCosts), m_rootArea(rootArea), m_parDepth(parallelDepth),
+          m_gpu(gpu) {}
 
-    void build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out);
+    void build(NodeFrags&& frags, uint32_t depth, Subtree* out);
+    uint32_t deviceNodes() const { return m_deviceNodes; }
+    // After a build from device fragments: the host subtrees the device
+    // nodes left behind, built on `threads` threads, put in place of their
+    // placeholders (the tree in DFS pre-order, as one recursive build makes it).
+    void finishDeviceBuild(Subtree* skeleton, int threads);
 
 private:
     // primitives 0..nt-1 are triangles, nt.. the boxes
@@ -164,6 +200,62 @@ private:
     const std::vector<float>& m_boxCosts;
     float m_rootArea;
     int m_parDepth;
+    SbvhGpu* m_gpu;
+    uint32_t m_deviceNodes = 0;   // touched only by the thread that runs device nodes
+    // Subtrees below the device nodes: their fragments wait in a list while
+    // the device nodes are processed (in DFS order, one thread), and a
+    // placeholder node (count kPlaceholder, first = list index) holds their
+    // place; finishDeviceBuild builds them all at once on the host threads.
+    static constexpr uint32_t kPlaceholder = 0xFFFFFFFFu;
+    struct Pending {
+        std::vector<Frag> frags;
+        uint32_t depth = 0;
+        Subtree tree;
+    };
+    std::vector<std::unique_ptr<Pending>> m_pending;
+
+    // the two children of a device node: device children are built now,
+    // host children are left as placeholders
+    void recurseDevice(NodeFrags&& left, NodeFrags&& right, uint32_t depth, Subtree* out, uint32_t nodeIdx,
+                       const Box& parent, int axis) {
+        auto child = [&](NodeFrags&& c) -> uint32_t {
+            const uint32_t idx = (uint32_t)out->nodes.size();
+            if (c.dev) {
+                build(std::move(c), depth, out);
+            } else {
+                out->nodes.emplace_back();
+                SbvhNode& ph = out->nodes.back();
+                ph.count = kPlaceholder;
+                ph.first = (uint32_t)m_pending.size();
+                std::unique_ptr<Pending> p(new Pending());
+                p->frags = std::move(c.host);
+                p->depth = depth;
+                m_pending.push_back(std::move(p));
+            }
+            return idx;
+        };
+        const uint32_t c0 = child(std::move(left));
+        const uint32_t c1 = child(std::move(right));
+        SbvhNode& n = out->nodes[nodeIdx];
+        n.box = parent; n.c0 = c0; n.c1 = c1; n.axis = (uint8_t)axis; n.first = n.count = 0;
+    }
+
+    // a device node's fragments brought to the host, in order
+    void toHost(NodeFrags& nf) const {
+        if (!nf.dev) return;
+        std::vector<uint32_t> prims;
+        std::vector<Box> boxes;
+        m_gpu->download(nf.dev, &prims, &boxes);
+        SbvhGpu::release(nf.dev);
+        nf.dev = nullptr;
+        nf.host.resize(prims.size());
+        for (size_t i = 0; i < prims.size(); ++i) nf.host[i] = Frag{prims[i], boxes[i]};
+    }
+    // a child below the device threshold continues on the host
+    NodeFrags settle(NodeFrags&& nf) const {
+        if (nf.dev && nf.size() < kDeviceNodeFrags) toHost(nf);
+        return std::move(nf);
+    }
 
     static void append(Subtree* dst, const Subtree& src) {
         uint32_t nodeBase = (uint32_t)dst->nodes.size();
@@ -179,8 +271,8 @@ private:
         dst->fragments += src.fragments;
     }
 
-    void recurse(std::vector<Frag>&& left, std::vector<Frag>&& right, uint32_t depth, Subtree* out,
-                 uint32_t nodeIdx, const Box& parent, int axis) {
+    void recurse(NodeFrags&& left, NodeFrags&& right, uint32_t depth, Subtree* out, uint32_t nodeIdx,
+                 const Box& parent, int axis) {
         bool parallel = (int)depth < m_parDepth && left.size() + right.size() > 20000;
         if (parallel) {
             Subtree l, r;
@@ -205,28 +297,39 @@ private:
 };
 
 // SBVH::buildRecursive (SBVH.h:57-349), writing nodes of this subtree into
-// `out` starting at out->nodes.size().
-void Builder::build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out) {
+// `out` starting at out->nodes.size(). A node whose fragments are in device
+// memory gets its bounds, bins and partition from the device (triangles
+// only, unit costs: the cost sums are the counts) and the same decisions.
+void Builder::build(NodeFrags&& nf, uint32_t depth, Subtree* out) {
     uint32_t nodeIdx = (uint32_t)out->nodes.size();
     out->nodes.emplace_back();
     ++depth;
     out->depth = std::max(out->depth, depth);
+    const bool onDev = nf.dev != nullptr;
+    std::vector<Frag>& frags = nf.host;
 
     Box parent, centroids;
     float leafCost = 0.0f;
-    for (const Frag& f : frags) {
-        parent.unify(f.box);
-        centroids.unify(f.box.centroid());
-        leafCost += cost(f.prim);
+    if (onDev) {
+        ++m_deviceNodes;
+        m_gpu->bounds(nf.dev, &parent, &centroids);
+        leafCost = (float)nf.size();   // 1.0f added once per fragment: exact below 2^24
+    } else {
+        for (const Frag& f : frags) {
+            parent.unify(f.box);
+            centroids.unify(f.box.centroid());
+            leafCost += cost(f.prim);
+        }
     }
     const int axisOP = centroids.widestAxis();
     const int axisSP = parent.widestAxis();
     const float areaParent = parent.area();
     const float pcMin = centroids.mn[axisOP], pcMax = centroids.mx[axisOP];
     const float pMin = parent.mn[axisSP], pMax = parent.mx[axisSP];
-    const uint32_t numObjs = (uint32_t)frags.size();
+    const uint32_t numObjs = (uint32_t)nf.size();
 
     auto makeLeaf = [&]() {
+        toHost(nf);
         SbvhNode& n = out->nodes[nodeIdx];
         n.box = parent;
         n.c0 = n.c1 = 0;
@@ -245,12 +348,19 @@ void Builder::build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out) {
     uint32_t planeOP = 0;
     float minCostOP = INFINITY;
     if ((pcMax - pcMin) > 0) {
-        for (const Frag& f : frags) {
-            uint32_t b = (uint32_t)(kObjBins * ((f.box.center(axisOP) - pcMin) / (pcMax - pcMin)));
-            b = std::min(b, kObjBins - 1);
-            ++ob[b].n;
-            ob[b].cost += cost(f.prim);
-            ob[b].box.unify(f.box);
+        if (onDev) {
+            uint32_t cnt[kObjBins];
+            Box box[kObjBins];
+            m_gpu->objectBins(nf.dev, axisOP, pcMin, pcMax, cnt, box);
+            for (uint32_t b = 0; b < kObjBins; ++b) { ob[b].n = cnt[b]; ob[b].cost = (float)cnt[b]; ob[b].box = box[b]; }
+        } else {
+            for (const Frag& f : frags) {
+                uint32_t b = (uint32_t)(kObjBins * ((f.box.center(axisOP) - pcMin) / (pcMax - pcMin)));
+                b = std::min(b, kObjBins - 1);
+                ++ob[b].n;
+                ob[b].cost += cost(f.prim);
+                ob[b].box.unify(f.box);
+            }
         }
         for (uint32_t i = 0; i < kObjBins - 1; ++i) {
             Box b0, b1;
@@ -276,19 +386,30 @@ void Builder::build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out) {
     float minCostSP = INFINITY;
     const float alpha = 1e-5f;
     if (overlapArea / m_rootArea > alpha) {
-        for (const Frag& f : frags) {
-            uint32_t eb = (uint32_t)(kSpBins * ((f.box.mn[axisSP] - pMin) / (pMax - pMin)));
-            uint32_t xb = (uint32_t)(kSpBins * ((f.box.mx[axisSP] - pMin) / (pMax - pMin)));
-            eb = std::min(eb, kSpBins - 1);
-            xb = std::min(xb, kSpBins - 1);
-            ++sb[eb].nIn;
-            ++sb[xb].nOut;
-            sb[eb].cIn += cost(f.prim);
-            sb[xb].cOut += cost(f.prim);
-            for (int b = (int)eb; b <= (int)xb; ++b) {
-                float lo = b * binWidth + pMin;
-                Box c = chopped(f.prim, axisSP, lo, lo + binWidth);
-                sb[b].box.unify(intersect(c, f.box));
+        if (onDev) {
+            uint32_t nIn[kSpBins], nOut[kSpBins];
+            Box box[kSpBins];
+            m_gpu->spatialBins(nf.dev, axisSP, pMin, pMax, binWidth, nIn, nOut, box);
+            for (uint32_t b = 0; b < kSpBins; ++b) {
+                sb[b].nIn = nIn[b]; sb[b].nOut = nOut[b];
+                sb[b].cIn = (float)nIn[b]; sb[b].cOut = (float)nOut[b];
+                sb[b].box = box[b];
+            }
+        } else {
+            for (const Frag& f : frags) {
+                uint32_t eb = (uint32_t)(kSpBins * ((f.box.mn[axisSP] - pMin) / (pMax - pMin)));
+                uint32_t xb = (uint32_t)(kSpBins * ((f.box.mx[axisSP] - pMin) / (pMax - pMin)));
+                eb = std::min(eb, kSpBins - 1);
+                xb = std::min(xb, kSpBins - 1);
+                ++sb[eb].nIn;
+                ++sb[xb].nOut;
+                sb[eb].cIn += cost(f.prim);
+                sb[xb].cOut += cost(f.prim);
+                for (int b = (int)eb; b <= (int)xb; ++b) {
+                    float lo = b * binWidth + pMin;
+                    Box c = chopped(f.prim, axisSP, lo, lo + binWidth);
+                    sb[b].box.unify(intersect(c, f.box));
+                }
             }
         }
         for (uint32_t i = 0; i < kSpBins - 1; ++i) {
@@ -305,43 +426,119 @@ void Builder::build(std::vector<Frag>&& frags, uint32_t depth, Subtree* out) {
 
     if (minCostOP < minCostSP) {
         float pivot = pcMin + (pcMax - pcMin) / kObjBins * (planeOP + 1);
-        auto mid = std::partition(frags.begin(), frags.end(),
-                                  [axisOP, pivot](const Frag& f) { return f.box.center(axisOP) < pivot; });
-        size_t split = std::max<size_t>((size_t)(mid - frags.begin()), 1);
-        if (split >= frags.size()) throw std::runtime_error("SBVH: invalid object partition");
-        std::vector<Frag> left(frags.begin(), frags.begin() + split);
-        std::vector<Frag> right(frags.begin() + split, frags.end());
-        std::vector<Frag>().swap(frags);
+        NodeFrags left, right;
+        if (onDev) {
+            m_gpu->partitionObject(nf.dev, axisOP, pivot, &left.dev, &right.dev);
+            nf = NodeFrags();
+            left = settle(std::move(left));
+            right = settle(std::move(right));
+            recurseDevice(std::move(left), std::move(right), depth, out, nodeIdx, parent, axisOP);
+            return;
+        } else {
+            auto mid = std::partition(frags.begin(), frags.end(),
+                                      [axisOP, pivot](const Frag& f) { return f.box.center(axisOP) < pivot; });
+            size_t split = std::max<size_t>((size_t)(mid - frags.begin()), 1);
+            if (split >= frags.size()) throw std::runtime_error("SBVH: invalid object partition");
+            left.host.assign(frags.begin(), frags.begin() + split);
+            right.host.assign(frags.begin() + split, frags.end());
+            std::vector<Frag>().swap(frags);
+        }
         recurse(std::move(left), std::move(right), depth, out, nodeIdx, parent, axisOP);
         return;
     }
 
-    uint32_t nLeft = 0, nRight = 0;
-    for (uint32_t j = 0; j <= planeSP; ++j) nLeft += sb[j].nIn;
-    for (uint32_t j = planeSP + 1; j < kSpBins; ++j) nRight += sb[j].nOut;
-    std::vector<Frag> left, right;
-    left.reserve(nLeft);
-    right.reserve(nRight);
     const float splitPos = (planeSP + 1) * binWidth + pMin;
-    for (const Frag& f : frags) {
-        uint32_t eb = (uint32_t)(kSpBins * ((f.box.mn[axisSP] - pMin) / (pMax - pMin)));
-        uint32_t xb = (uint32_t)(kSpBins * ((f.box.mx[axisSP] - pMin) / (pMax - pMin)));
-        eb = std::min(eb, kSpBins - 1);
-        xb = std::min(xb, kSpBins - 1);
-        if (xb <= planeSP) {
-            left.push_back(f);
-        } else if (eb > planeSP) {
-            right.push_back(f);
+    NodeFrags left, right;
+    if (onDev) {
+        m_gpu->partitionSpatial(nf.dev, axisSP, pMin, pMax, planeSP, splitPos, &left.dev, &right.dev);
+        nf = NodeFrags();
+        left = settle(std::move(left));
+        right = settle(std::move(right));
+        recurseDevice(std::move(left), std::move(right), depth, out, nodeIdx, parent, axisSP);
+        return;
+    } else {
+        uint32_t nLeft = 0, nRight = 0;
+        for (uint32_t j = 0; j <= planeSP; ++j) nLeft += sb[j].nIn;
+        for (uint32_t j = planeSP + 1; j < kSpBins; ++j) nRight += sb[j].nOut;
+        left.host.reserve(nLeft);
+        right.host.reserve(nRight);
+        for (const Frag& f : frags) {
+            uint32_t eb = (uint32_t)(kSpBins * ((f.box.mn[axisSP] - pMin) / (pMax - pMin)));
+            uint32_t xb = (uint32_t)(kSpBins * ((f.box.mx[axisSP] - pMin) / (pMax - pMin)));
+            eb = std::min(eb, kSpBins - 1);
+            xb = std::min(xb, kSpBins - 1);
+            if (xb <= planeSP) {
+                left.host.push_back(f);
+            } else if (eb > planeSP) {
+                right.host.push_back(f);
+            } else {
+                Box l, r;
+                split(f.prim, axisSP, splitPos, &l, &r);
+                if (l.valid()) left.host.push_back(Frag{f.prim, intersect(l, f.box)});
+                if (r.valid()) right.host.push_back(Frag{f.prim, intersect(r, f.box)});
+            }
+        }
+        if (left.host.empty() || right.host.empty()) throw std::runtime_error("SBVH: invalid spatial partition");
+        std::vector<Frag>().swap(frags);
+    }
+    recurse(std::move(left), std::move(right), depth, out, nodeIdx, parent, axisSP);
+}
+
+void Builder::finishDeviceBuild(Subtree* skeleton, int threads) {
+    // largest subtrees first, so the threads finish together
+    std::vector<size_t> order(m_pending.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [&](size_t a, size_t b) { return m_pending[a]->frags.size() > m_pending[b]->frags.size(); });
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+        for (;;) {
+            const size_t k = next.fetch_add(1);
+            if (k >= order.size()) return;
+            Pending& p = *m_pending[order[k]];
+            build(NodeFrags(std::move(p.frags)), p.depth, &p.tree);
+        }
+    };
+    std::vector<std::future<void>> futs;
+    for (int t = 1; t < std::max(1, threads); ++t) futs.push_back(std::async(std::launch::async, worker));
+    worker();
+    for (auto& f : futs) f.get();
+    // DFS over the skeleton, each placeholder replaced by its subtree
+    Subtree out;
+    out.nodes.reserve(skeleton->nodes.size() + 2 * skeleton->refs.size());
+    out.depth = skeleton->depth;
+    out.fragments = skeleton->fragments;
+    // iterative pre-order emit: (skeleton node, parent slot to patch, which child)
+    struct Item { uint32_t sk; int32_t parent; int side; };
+    std::vector<Item> st;
+    st.push_back(Item{0, -1, 0});
+    while (!st.empty()) {
+        const Item it = st.back();
+        st.pop_back();
+        const SbvhNode& n = skeleton->nodes[it.sk];
+        uint32_t idx = (uint32_t)out.nodes.size();
+        if (n.count == kPlaceholder) {
+            append(&out, m_pending[n.first]->tree);
         } else {
-            Box l, r;
-            split(f.prim, axisSP, splitPos, &l, &r);
-            if (l.valid()) left.push_back(Frag{f.prim, intersect(l, f.box)});
-            if (r.valid()) right.push_back(Frag{f.prim, intersect(r, f.box)});
+            SbvhNode m = n;
+            if (m.count > 0) {
+                m.first = (uint32_t)out.refs.size();
+                out.refs.insert(out.refs.end(), skeleton->refs.begin() + n.first,
+                                skeleton->refs.begin() + n.first + n.count);
+            }
+            out.nodes.push_back(m);
+            if (m.count == 0) {
+                // right child below left on the stack: left is emitted first
+                st.push_back(Item{n.c1, (int32_t)idx, 1});
+                st.push_back(Item{n.c0, (int32_t)idx, 0});
+            }
+        }
+        if (it.parent >= 0) {
+            if (it.side == 0) out.nodes[it.parent].c0 = idx;
+            else out.nodes[it.parent].c1 = idx;
         }
     }
-    if (left.empty() || right.empty()) throw std::runtime_error("SBVH: invalid spatial partition");
-    std::vector<Frag>().swap(frags);
-    recurse(std::move(left), std::move(right), depth, out, nodeIdx, parent, axisSP);
+    *skeleton = std::move(out);
 }
 
 // SBVH::calcSAHCost (SBVH.h:351-377); primCost(ref) is the referenced
@@ -464,12 +661,12 @@ float qbvhCost(const Accel& a, PrimCost&& primCost) {
 
 }  // namespace
 
-void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads) {
-    buildAccel(triPos, std::vector<Box>(), std::vector<float>(), out, numThreads);
+void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads, bool device) {
+    buildAccel(triPos, std::vector<Box>(), std::vector<float>(), out, numThreads, device);
 }
 
 void buildAccel(const std::vector<float>& triPos, const std::vector<Box>& boxes, const std::vector<float>& boxCosts,
-                Accel* out, int numThreads) {
+                Accel* out, int numThreads, bool device) {
     auto t0 = std::chrono::steady_clock::now();
     const size_t nt = triPos.size() / 9, np = nt + boxes.size();
     std::vector<Tri> tris(nt);
@@ -499,11 +696,23 @@ void buildAccel(const std::vector<float>& triPos, const std::vector<Box>& boxes,
     }
     int parDepth = 0;
     for (int t = std::max(1, numThreads); t > 1; t >>= 1) ++parDepth;
-    Builder b(tris, boxes, boxCosts, out->bounds.area(), parDepth + 1);
+    // the device takes the top of a large triangle tree: its root fragments
+    // are the same triangle bounds, made there
+    std::unique_ptr<SbvhGpu> gpu;
+    NodeFrags rootFrags;
+    if (device && boxes.empty() && np >= kDeviceNodeFrags) {
+        gpu = SbvhGpu::create(triPos);
+        rootFrags.dev = gpu->root();
+    } else {
+        rootFrags.host = std::move(frags);
+    }
+    Builder b(tris, boxes, boxCosts, out->bounds.area(), parDepth + 1, gpu.get());
     auto primCost = [&](uint32_t prim) { return prim < nt ? 1.0f : boxCosts[prim - nt]; };
     Subtree st;
     st.nodes.reserve(2 * np);
-    b.build(std::move(frags), 0, &st);
+    b.build(std::move(rootFrags), 0, &st);
+    if (gpu) b.finishDeviceBuild(&st, numThreads);
+    out->deviceNodes = b.deviceNodes();
     out->sbvh = std::move(st.nodes);
     out->sbvhRefs = std::move(st.refs);
     out->sbvhDepth = st.depth;

@@ -641,11 +641,17 @@ int slrh_scene_build(slrh_scene* s) {
             return tp;
         };
         int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        // with a device, the top of each large triangle tree is binned there
+        const bool onDevice = s->ctx->device >= 0;
+        if (onDevice) {
+            std::string err;
+            if (int r = deviceSelect(s->ctx->device, &err)) return fail(s->ctx, r, "scene build: " + err);
+        }
         // one accelerator per prototype, over its object-space triangles
         for (Prototype& p : d.protos) {
             std::vector<uint32_t> ids(p.triCount);
             for (uint32_t i = 0; i < p.triCount; ++i) ids[i] = p.triFirst + i;
-            buildAccel(positions(ids), &p.accel, threads);
+            buildAccel(positions(ids), &p.accel, threads, onDevice);
             for (uint32_t& r : p.accel.sbvhRefs) r = ids[r];
             for (uint32_t& r : p.accel.qbvhRefs) r = ids[r];
         }
@@ -669,7 +675,7 @@ int slrh_scene_build(slrh_scene* s) {
                                                (c & 1) ? b.mx[2] : b.mn[2])));
             boxes.push_back(w);
         }
-        buildAccel(positions(world), boxes, costs, &d.accel, threads);
+        buildAccel(positions(world), boxes, costs, &d.accel, threads, onDevice);
         const uint32_t nw = (uint32_t)world.size();
         auto code = [&](uint32_t r) { return r < nw ? world[r] : (kRefInstance | (r - nw)); };
         for (uint32_t& r : d.accel.sbvhRefs) r = code(r);
@@ -751,6 +757,21 @@ int slrh_scene_accel_stats(const slrh_scene* s, double* st) {
     st[6] = a.qbvhDepth;
     st[7] = a.qbvhCost;
     st[8] = a.buildSeconds;
+    return SLRH_OK;
+}
+
+int slrh_scene_build_stats(const slrh_scene* s, double* st) {
+    CtxLock lk = lockOf(s);
+    if (!s || !st) return SLRH_ERR_INVALID;
+    if (!s->built) return SLRH_ERR_STATE;
+    const SceneData& d = s->data;
+    double secs = d.accel.buildSeconds, dev = d.accel.deviceNodes;
+    for (const Prototype& p : d.protos) {
+        secs += p.accel.buildSeconds;
+        dev += p.accel.deviceNodes;
+    }
+    st[0] = secs;
+    st[1] = dev;
     return SLRH_OK;
 }
 

@@ -88,6 +88,7 @@ struct Accel {
     Box bounds;
     double buildSeconds = 0;
     uint32_t maxLeafCount = 0;
+    uint32_t deviceNodes = 0;   // SBVH nodes binned and partitioned on the device
 };
 
 // Binary BVH node of the binned-SAH builder (StandardBVH::Node,
@@ -108,12 +109,15 @@ void buildStandardBvh(const std::vector<float>& triPos, StdBvh* out);
 
 // Builds SBVH then QBVH over triangles given as 9 floats each (world space).
 // Deterministic; identical decisions to the reference builder.
-void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads);
+// With device set (a device is current on the calling thread), the nodes
+// at the top of a large triangle tree are binned and partitioned there; the
+// tree is the same.
+void buildAccel(const std::vector<float>& triPos, Accel* out, int numThreads, bool device = false);
 // The same over triangles followed by boxes (instances): primitive ids
 // 0..ntris-1 are the triangles, ntris + i is boxes[i], whose intersection
 // cost in the SAH is boxCosts[i] (a triangle's is 1).
 void buildAccel(const std::vector<float>& triPos, const std::vector<Box>& boxes, const std::vector<float>& boxCosts,
-                Accel* out, int numThreads);
+                Accel* out, int numThreads, bool device = false);
 
 // ---------------------------------------------------------------- spectra
 
