@@ -1,24 +1,29 @@
 // Bidirectional path tracing on the device: BPTRenderer
-// (libSLR/Renderer/BPTRenderer.cpp:25-432) as a wavefront over slots, each
-// slot rendering one pixel sample per pass.
+// (libSLR/Renderer/BPTRenderer.cpp:25-432) as a streaming wavefront over
+// slots. A slot owns one pixel and one film plane and renders its share of
+// the pixel's samples one after another: light subpath, eye subpath with its
+// connections, film add, next sample. Slots in either phase share every
+// step, so the steps stay full until the last samples drain.
 //
-//   light phase (until no light ray is left)
-//     trace : closest hit of the light rays, any hit of the rays from the
-//             previous step's light vertices to the lens (one launch)
-//     step  : each unoccluded light-to-lens ray adds its contribution to the
-//             light image (t = 1; the pixel is where the ray crosses the
-//             image plane, so these adds use FP64 atomics); each hit becomes
-//             the subpath's next stored vertex, gets its own light-to-lens
-//             ray, and samples the next direction with Russian roulette or
-//             ends the subpath
-//   eye phase (until no eye ray and no connection is left)
-//     trace : closest hit of the eye rays, any hit of the connection rays
-//     gather: each slot adds its unoccluded connections (s >= 1) to the
-//             sample's radiance; a sample whose eye subpath ended is filed
-//             into the slot's film plane (the slot owns it: no atomics)
-//     step  : each hit becomes the eye vertex: emission with its MIS weight
-//             (s = 0), one connection per stored light vertex (a shadow ray
-//             each), and the next direction or the end of the subpath
+//   trace  : closest hit of the light and eye rays, any hit of the
+//            light-to-lens rays and of the connection rays (one launch)
+//   light  : each unoccluded light-to-lens ray adds its contribution to the
+//            light image (t = 1; the pixel is where the ray crosses the image
+//            plane, so these adds use FP64 atomics); each light-ray hit
+//            becomes the subpath's next stored vertex, gets its own
+//            light-to-lens ray, and samples the next direction with Russian
+//            roulette; a subpath that ends starts the sample's eye subpath
+//            (the camera ray)
+//   gather : each slot adds its unoccluded connections (s >= 1) to the
+//            sample's radiance; a sample whose eye subpath ended is filed
+//            into the slot's film plane (the slot owns it: no atomics) and
+//            the slot's next sample starts (light vertex, first light ray)
+//   eye    : each eye-ray hit becomes the eye vertex: emission with its MIS
+//            weight (s = 0), the next direction or the end of the subpath
+//   connect: one connection per stored light vertex (a shadow ray each)
+//
+// A slot draws its samples from its own random stream in a fixed order, so
+// the film planes do not depend on how the steps interleave the slots.
 //
 // MIS weights are the reference's power heuristic over every (s, t) with
 // the subpaths' area PDFs and Russian-roulette probabilities
@@ -65,28 +70,38 @@ constexpr uint32_t kVtxRows = 9;              // a stored vertex: 9 float4 (144 
 constexpr uint32_t kEyeRows = kVtxRows + 2;   // the eye vertex, then its predecessor's position/PDF and normal/flags
 constexpr uint32_t kConnRows = 6;             // contribution (4 float4), shadow ray (2 float4)
 constexpr uint32_t kSplatRows = 7;            // the same and the pixel
+// trace entries: bit 31 any hit, bit 30 a light-to-lens ray (else a
+// connection ray); connection entries index slot * kLightCap + vertex, so
+// slots are limited to 2^24
 constexpr uint32_t kShadowBit = 0x80000000u;
+constexpr uint32_t kSplatBit = 0x40000000u;
+constexpr uint32_t kMaxBptSlots = 1u << 24;
 constexpr int kBptLdsDepth = 16;
 constexpr uint32_t kBptTreelet = 128;
-constexpr uint32_t kTargetSlots = 1u << 21;   // live samples per pass when the caller leaves it open
+constexpr uint32_t kTargetSlots = 1u << 21;   // slots when the caller leaves it open
+constexpr uint32_t kStepsPerRead = 4;        // wavefront steps between reads of the queue lengths
 
 // vertex flags (row 3 w)
 constexpr uint32_t V_DELTA = 1u;   // sampled by a delta distribution
 constexpr uint32_t V_EDF = 2u;     // the light vertex: its DDF is the light's diffuse EDF
-// slot flags (aux.y bits 0-7)
-constexpr uint32_t B_SAMPLE = 1u;  // the slot holds a sample this pass
+// slot flags (aux.y bits 0-7; bits 8-15 the hero wavelength, 16-31 the
+// number of samples the slot has started)
+constexpr uint32_t B_SAMPLE = 1u;  // the slot holds a sample
 constexpr uint32_t B_END = 2u;     // its eye subpath ended: file after the last connections
 constexpr uint32_t B_CONNECT = 4u; // its eye vertex scatters: connect it to the light subpath
+constexpr uint32_t kMaxSlotSamples = 0xFFFFu;
 // carried flags (mis.y): delta of the last stored vertex and of the one being traced to
 constexpr uint32_t M_DELTA_LAST = 1u, M_DELTA_NEXT = 2u;
 
 enum BptCounter : uint32_t {
-    K_EXT = 0,    // +0/1: extension queue lengths (ping-pong)
-    K_SH = 2,     // +0/1: shadow queue lengths
-    K_CONN = 4,   // +0/1: gather queue lengths
-    K_FETCH = 8,  // traversal fetch cursor
-    K_NONFINITE = 9,
-    K_OVERFLOW = 10,
+    K_LE = 0,     // +0/1: light-ray queue lengths (ping-pong by step parity)
+    K_EE = 2,     // +0/1: eye-ray queue lengths
+    K_SP = 4,     // +0/1: light-to-lens ray queue lengths
+    K_CN = 6,     // +0/1: connection ray queue lengths
+    K_GA = 8,     // +0/1: gather queue lengths (slots whose eye vertex was just made)
+    K_FETCH = 10, // traversal fetch cursor
+    K_NONFINITE = 11,
+    K_OVERFLOW = 12,
     kBptCounters = 16
 };
 enum BptStatId : uint32_t {
@@ -122,14 +137,16 @@ struct BptState {
     uint32_t spillStride;
 };
 
-__device__ __forceinline__ uint32_t* qExt(const BptState& b, uint32_t i) { return b.q + (size_t)i * b.n; }
-__device__ __forceinline__ uint32_t* qSh(const BptState& b, uint32_t i) {
-    return b.q + 2 * (size_t)b.n + (size_t)i * b.n * kLightCap;
+// queues of slot indices (connection entries: slot * kLightCap + light
+// vertex), two of each kind, one per step parity
+__device__ __forceinline__ uint32_t* qLight(const BptState& b, uint32_t p) { return b.q + (size_t)p * b.n; }
+__device__ __forceinline__ uint32_t* qEye(const BptState& b, uint32_t p) { return b.q + (size_t)(2 + p) * b.n; }
+__device__ __forceinline__ uint32_t* qGather(const BptState& b, uint32_t p) { return b.q + (size_t)(4 + p) * b.n; }
+__device__ __forceinline__ uint32_t* qSplat(const BptState& b, uint32_t p) { return b.q + (size_t)(6 + p) * b.n; }
+__device__ __forceinline__ uint32_t* qConn(const BptState& b, uint32_t p) {
+    return b.q + 8 * (size_t)b.n + (size_t)p * b.n * kLightCap;
 }
-__device__ __forceinline__ uint32_t* qConn(const BptState& b, uint32_t i) {
-    return b.q + 2 * (size_t)b.n * (1 + kLightCap) + (size_t)i * b.n;
-}
-__host__ __device__ __forceinline__ size_t queueWords(uint32_t n) { return 2 * (size_t)n * (2 + kLightCap); }
+__host__ __device__ __forceinline__ size_t queueWords(uint32_t n) { return (size_t)n * (8 + 2 * kLightCap); }
 __device__ __forceinline__ float4* splatRows(const BptState& b, uint32_t parity, uint32_t s) {
     return b.splat + ((size_t)parity * b.n + s) * kSplatRows;
 }
@@ -432,114 +449,135 @@ __global__ void k_bpt_seed(BptState b, uint32_t seed) {
         uint32_t h = seed ^ (0x9E3779B9u * (s + 1u));
         h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
         b.rng[s] = xsSeed(h);
+        b.aux[s] = make_uint4(s % b.pixels, 0u, 0u, 0u);
         b.visSplat[s] = 0;
         b.visSplat[(size_t)b.n + s] = 0;
     }
 }
 
-// The pass's sample of every slot: pixel position, wavelengths, lens point,
-// the light vertex (Scene::selectSurfaceLight, SurfaceLight::sampleRay,
-// BPTRenderer.cpp:121-148) with its connection to the lens and the first
-// light ray.
-__global__ __launch_bounds__(kBptBlock) void k_bpt_begin(DevScene sc, BptState b, DevCamera cam, uint32_t pass) {
+// The slot's next sample while it has one left (slot s renders samples
+// plane, plane + planes, ... of its pixel, plane = s / pixels): pixel
+// position, wavelengths, lens point, the light vertex
+// (Scene::selectSurfaceLight, SurfaceLight::sampleRay,
+// BPTRenderer.cpp:121-148) with its connection to the lens (written for the
+// trace of step parity `parity`) and the first light ray. Returns whether a
+// sample started; *splat whether its lens ray is to be traced.
+__device__ __forceinline__ bool startSample(const DevScene& sc, const BptState& b, const DevCamera& cam,
+                                            const LensFrame& lf, uint32_t s, uint32_t parity, uint4& aux,
+                                            bool* splat) {
+    *splat = false;
+    const uint32_t plane = s / b.pixels, pixel = aux.x;
+    const uint32_t started = aux.y >> 16;
+    const uint32_t quota = (b.spp - plane + b.planes - 1) / b.planes;
+    if (started >= quota) {
+        aux.y = started << 16;
+        aux.z = 0;
+        b.aux[s] = aux;
+        return false;
+    }
+    uint4 rng = b.rng[s];
+    const uint32_t px = pixel % cam.width, py = pixel / cam.width;
+    const float fx = px + xsFloat(rng), fy = py + xsFloat(rng);
+    const float off = xsFloat(rng);
+    const float uL = xsFloat(rng);
+    const uint32_t hero = min((uint32_t)(uint16_t)(kNumL * uL), (uint32_t)(kNumL - 1));
+    // light vertex
+    float lprob;
+    const uint32_t li = sampleLight(sc, xsFloat(rng), &lprob);
+    const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
+    const LightPoint lp = lightPoint(sc, li, lu0, lu1);
+    float Le[kNumL];
+    spectrumValues(sc.specPool + __float_as_uint(lp.emit.z), lp.emit.x, lp.emit.y,
+                   __float_as_uint(lp.emit.w), off, Le);
+    const float P0 = lprob * lp.areaPDF;
+    float alpha0[kNumL];
+    const float rP0 = 1.0f / P0;
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) alpha0[k] = Le[k] * rP0;
+    float3 fxv, fyv;
+    coordinateSystem(lp.ln, &fxv, &fyv);
+    const float3 d = cosineHemisphere(xsFloat(rng), xsFloat(rng));
+    const float dirPDF = (float)(d.z / 3.14159265358979323846);
+    const float3 dir = d.x * fxv + d.y * fyv + d.z * lp.ln;
+    Vtx L;
+    L.p = lp.lp; L.P = P0; L.gN = lp.lg; L.cosIn = 0.0f; L.sz = lp.ln; L.tail = 0.0f; L.sx = fxv;
+    L.flags = V_EDF; L.dir = f3(0, 0, 0); L.mat = lp.mat;
+    storeVtx(b.lv + (size_t)s * kLightCap * kVtxRows, L, alpha0);
+    // the lens point (PerspectiveCamera::sample)
+    float lx = 0, ly = 0;
+    concentricDisk(xsFloat(rng), xsFloat(rng), &lx, &ly);
+    const float2 orgL = make_float2(cam.lensRadius * lx, cam.lensRadius * ly);
+    const float* M = cam.l2w;
+    const float3 pe = f3(M[0] * orgL.x + M[1] * orgL.y + M[3], M[4] * orgL.x + M[5] * orgL.y + M[7],
+                         M[8] * orgL.x + M[9] * orgL.y + M[11]);
+    const float lensP = cam.lensRadius > 0.0f
+                            ? (float)(1.0 / (3.14159265358979323846 * cam.lensRadius * cam.lensRadius))
+                            : 1.0f;
+    const float4 lens = make_float4(pe.x, pe.y, pe.z, lensP);
+    b.lens[s] = lens;
+    b.smp[s] = make_float4(fx, fy, orgL.x, orgL.y);
+    // the first light ray and its throughput: alpha0 * Le1 * cos / dirPDF
+    const float sc1 = (float)(1.0 / 3.14159265358979323846) * (fabsf(dot3(dir, lp.lg)) / dirPDF);
+    float a1[kNumL];
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) a1[k] = alpha0[k] * sc1;
+    store16(b.alpha + (size_t)s * 4, a1);
+    float4* g = b.gen + (size_t)s * 4;
+    g[0] = make_float4(lp.lp.x, lp.lp.y, lp.lp.z, 1e-4f);
+    g[1] = make_float4(dir.x, dir.y, dir.z, INFINITY);
+    g[3] = make_float4(dirPDF, 1.0f, d.z, P0);
+    b.mis[s] = make_float4(0.0f, __uint_as_float(0u), off, 0.0f);
+    aux = make_uint4(pixel, B_SAMPLE | (hero << 8) | ((started + 1) << 16), 1u, 0u);
+    b.aux[s] = aux;
+    float z[kNumL];
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) z[k] = 0.0f;
+    store16(b.rad + (size_t)s * 4, z);
+    // light vertex to lens (s = 1, t = 1)
+    float c[kNumL];
+    uint32_t pix = 0;
+    float4 ro, rd;
+    const bool sh = connectLens(sc, cam, lf, lens, orgL, L, alpha0, false, make_float4(0, 0, 0, 1),
+                                make_float4(0, 0, 0, 0), 0u, (int)hero, off, c, &pix, &ro, &rd);
+    if (sh) {
+        float4* sp = splatRows(b, parity, s);
+        store16(sp, c);
+        sp[4] = ro;
+        sp[5] = rd;
+        sp[6] = make_float4(__uint_as_float(pix), 0, 0, 0);
+    }
+    b.rng[s] = rng;
+    *splat = sh;
+    return true;
+}
+
+// Every slot's first sample.
+__global__ __launch_bounds__(kBptBlock) void k_bpt_start(DevScene sc, BptState b, DevCamera cam) {
     const LensFrame lf = lensFrame(cam);
     for (uint32_t base = blockIdx.x * kBptBlock; base < b.n; base += gridDim.x * kBptBlock) {
         const uint32_t s = base + threadIdx.x;
         bool has = false, sh = false;
         if (s < b.n) {
-            const uint32_t plane = s / b.pixels, pixel = s % b.pixels;
-            const uint32_t sampleIdx = pass * b.planes + plane;
-            has = sampleIdx < b.spp;
-            if (!has) b.aux[s] = make_uint4(pixel, 0u, 0u, 0u);
-            if (has) {
-                uint4 rng = b.rng[s];
-                const uint32_t px = pixel % cam.width, py = pixel / cam.width;
-                const float fx = px + xsFloat(rng), fy = py + xsFloat(rng);
-                const float off = xsFloat(rng);
-                const float uL = xsFloat(rng);
-                const uint32_t hero = min((uint32_t)(uint16_t)(kNumL * uL), (uint32_t)(kNumL - 1));
-                // light vertex
-                float lprob;
-                const uint32_t li = sampleLight(sc, xsFloat(rng), &lprob);
-                const float lu0 = xsFloat(rng), lu1 = xsFloat(rng);
-                const LightPoint lp = lightPoint(sc, li, lu0, lu1);
-                float Le[kNumL];
-                spectrumValues(sc.specPool + __float_as_uint(lp.emit.z), lp.emit.x, lp.emit.y,
-                               __float_as_uint(lp.emit.w), off, Le);
-                const float P0 = lprob * lp.areaPDF;
-                float alpha0[kNumL];
-                const float rP0 = 1.0f / P0;
-#pragma unroll
-                for (int k = 0; k < kNumL; ++k) alpha0[k] = Le[k] * rP0;
-                float3 fxv, fyv;
-                coordinateSystem(lp.ln, &fxv, &fyv);
-                const float3 d = cosineHemisphere(xsFloat(rng), xsFloat(rng));
-                const float dirPDF = (float)(d.z / 3.14159265358979323846);
-                const float3 dir = d.x * fxv + d.y * fyv + d.z * lp.ln;
-                Vtx L;
-                L.p = lp.lp; L.P = P0; L.gN = lp.lg; L.cosIn = 0.0f; L.sz = lp.ln; L.tail = 0.0f; L.sx = fxv;
-                L.flags = V_EDF; L.dir = f3(0, 0, 0); L.mat = lp.mat;
-                storeVtx(b.lv + (size_t)s * kLightCap * kVtxRows, L, alpha0);
-                // the lens point (PerspectiveCamera::sample)
-                float lx = 0, ly = 0;
-                concentricDisk(xsFloat(rng), xsFloat(rng), &lx, &ly);
-                const float2 orgL = make_float2(cam.lensRadius * lx, cam.lensRadius * ly);
-                const float* M = cam.l2w;
-                const float3 pe = f3(M[0] * orgL.x + M[1] * orgL.y + M[3], M[4] * orgL.x + M[5] * orgL.y + M[7],
-                                     M[8] * orgL.x + M[9] * orgL.y + M[11]);
-                const float lensP = cam.lensRadius > 0.0f
-                                        ? (float)(1.0 / (3.14159265358979323846 * cam.lensRadius * cam.lensRadius))
-                                        : 1.0f;
-                const float4 lens = make_float4(pe.x, pe.y, pe.z, lensP);
-                b.lens[s] = lens;
-                b.smp[s] = make_float4(fx, fy, orgL.x, orgL.y);
-                // the first light ray and its throughput: alpha0 * Le1 * cos / dirPDF
-                const float sc1 = (float)(1.0 / 3.14159265358979323846) * (fabsf(dot3(dir, lp.lg)) / dirPDF);
-                float a1[kNumL];
-#pragma unroll
-                for (int k = 0; k < kNumL; ++k) a1[k] = alpha0[k] * sc1;
-                store16(b.alpha + (size_t)s * 4, a1);
-                float4* g = b.gen + (size_t)s * 4;
-                g[0] = make_float4(lp.lp.x, lp.lp.y, lp.lp.z, 1e-4f);
-                g[1] = make_float4(dir.x, dir.y, dir.z, INFINITY);
-                g[3] = make_float4(dirPDF, 1.0f, d.z, P0);
-                b.mis[s] = make_float4(0.0f, __uint_as_float(0u), off, 0.0f);
-                b.aux[s] = make_uint4(pixel, B_SAMPLE | (hero << 8), 1u, 0u);
-                float z[kNumL];
-#pragma unroll
-                for (int k = 0; k < kNumL; ++k) z[k] = 0.0f;
-                store16(b.rad + (size_t)s * 4, z);
-                // light vertex to lens (s = 1, t = 1)
-                float c[kNumL];
-                uint32_t pix = 0;
-                float4 ro, rd;
-                sh = connectLens(sc, cam, lf, lens, orgL, L, alpha0, false, make_float4(0, 0, 0, 1),
-                                 make_float4(0, 0, 0, 0), 0u, (int)hero, off, c, &pix, &ro, &rd);
-                if (sh) {
-                    float4* sp = splatRows(b, 0, s);
-                    store16(sp, c);
-                    sp[4] = ro;
-                    sp[5] = rd;
-                    sp[6] = make_float4(__uint_as_float(pix), 0, 0, 0);
-                }
-                b.rng[s] = rng;
-            }
+            uint4 aux = b.aux[s];
+            has = startSample(sc, b, cam, lf, s, 0u, aux, &sh);
         }
-        const uint32_t shPos = waveAppend(&b.ctr[K_SH + 0], sh);
-        if (sh) qSh(b, 0)[shPos] = s;
-        const uint32_t ePos = waveAppend(&b.ctr[K_EXT + 0], has);
-        if (has) qExt(b, 0)[ePos] = s;
+        const uint32_t shPos = waveAppend(&b.ctr[K_SP + 0], sh);
+        if (sh) qSplat(b, 0)[shPos] = s;
+        const uint32_t ePos = waveAppend(&b.ctr[K_LE + 0], has);
+        if (has) qLight(b, 0)[ePos] = s;
     }
 }
 
-// Closest hit for the extension queue, any hit for the shadow queue, in one
-// persistent launch (the traversal of the path tracer).
+// Closest hit for the light and eye rays, any hit for the light-to-lens and
+// connection rays, in one persistent launch (the traversal of the path
+// tracer).
 template <bool I>
-__global__ __launch_bounds__(kBptBlock) void k_bpt_trace(DevScene sc, BptState b, uint32_t cur, int eyePhase) {
+__global__ __launch_bounds__(kBptBlock) void k_bpt_trace(DevScene sc, BptState b, uint32_t cur) {
     __shared__ uint32_t lds[kBptLdsDepth * kBptBlock];
     __shared__ float4 treelet[kBptTreelet * kTreeletRows];
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t ne = b.ctr[K_EXT + cur], ns = b.ctr[K_SH + cur];
+    const uint32_t nl = b.ctr[K_LE + cur], ne = b.ctr[K_EE + cur], nsp = b.ctr[K_SP + cur],
+                   ncn = b.ctr[K_CN + cur];
     const uint32_t gtid = blockIdx.x * kBptBlock + threadIdx.x;
     TraceStack st;
     st.lds = ldsColumn(lds, threadIdx.x);
@@ -552,20 +590,28 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_trace(DevScene sc, BptState b
     // the next step appends to the other queues: they start empty (their
     // contents were consumed before this launch)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        b.ctr[K_EXT + nxt] = 0;
-        b.ctr[K_SH + nxt] = 0;
-        b.ctr[K_CONN + nxt] = 0;
+        b.ctr[K_LE + nxt] = 0;
+        b.ctr[K_EE + nxt] = 0;
+        b.ctr[K_SP + nxt] = 0;
+        b.ctr[K_CN + nxt] = 0;
+        b.ctr[K_GA + nxt] = 0;
     }
-    const uint32_t* qe = qExt(b, cur);
-    const uint32_t* qs = qSh(b, cur);
+    const uint32_t* ql = qLight(b, cur);
+    const uint32_t* qe = qEye(b, cur);
+    const uint32_t* qs = qSplat(b, cur);
+    const uint32_t* qc = qConn(b, cur);
+    const uint32_t n1 = nl, n2 = n1 + ne, n3 = n2 + nsp;
     TraceCounters tc;
     traceLoop<16, true, false, I>(
-        sc, st, ne + ns, &b.ctr[K_FETCH],
-        [&](uint32_t i) { return i < ne ? qe[i] : (qs[i - ne] | kShadowBit); },
+        sc, st, n3 + ncn, &b.ctr[K_FETCH],
+        [&](uint32_t i) {
+            return i < n1 ? ql[i] : i < n2 ? qe[i - n1] : i < n3 ? (qs[i - n2] | kShadowBit | kSplatBit)
+                                                                 : (qc[i - n3] | kShadowBit);
+        },
         [&](uint32_t e, RayIn& r) {
             if (e & kShadowBit) {
-                const uint32_t idx = e & ~kShadowBit;
-                const float4* R = eyePhase ? b.conn + (size_t)idx * kConnRows + 4 : splatRows(b, cur, idx) + 4;
+                const uint32_t idx = e & ~(kShadowBit | kSplatBit);
+                const float4* R = (e & kSplatBit) ? splatRows(b, cur, idx) + 4 : b.conn + (size_t)idx * kConnRows + 4;
                 r.o = R[0];
                 r.d = R[1];
                 return true;
@@ -576,10 +622,10 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_trace(DevScene sc, BptState b
         },
         [&](uint32_t e, bool isAny, const HitOut& h) {
             if (isAny) {
-                const uint32_t idx = e & ~kShadowBit;
+                const uint32_t idx = e & ~(kShadowBit | kSplatBit);
                 const uint8_t v = h.prim >= 0 ? 0 : 1;
-                if (eyePhase) b.vis[idx] = v;
-                else b.visSplat[(size_t)cur * b.n + idx] = v;
+                if (e & kSplatBit) b.visSplat[(size_t)cur * b.n + idx] = v;
+                else b.vis[idx] = v;
             } else {
                 b.gen[(size_t)e * 4 + 2] = make_float4(h.t, __int_as_float(h.prim), h.b0, h.b1);
                 if (I) b.aux[e].w = (uint32_t)h.inst;
@@ -680,15 +726,50 @@ __device__ __forceinline__ bool extendSubpath(const DevScene& sc, const BptState
     return true;
 }
 
-// Light phase step: t = 1 contributions of the traced light-to-lens rays
-// into the light image, then the light rays' hits.
+// The sample's camera ray once its light subpath has ended
+// (PerspectiveCamera::sampleRay, BPTRenderer.cpp:151-169): eye vertex 0 on
+// the lens, then the first eye ray.
+__device__ __forceinline__ void startEye(const BptState& b, const DevCamera& cam, const LensFrame& lf, uint32_t s,
+                                         uint4& aux) {
+    const float4 sm = b.smp[s];
+    const float4 lens = b.lens[s];
+    const float3 orgL = f3(sm.z, sm.w, 0.0f);
+    const float ux = sm.x / cam.width, uy = sm.y / cam.height;
+    const float3 pFocus = f3(cam.opWidth * (0.5f - ux), cam.opHeight * (0.5f - uy), cam.objDist);
+    const float3 dl = normalize3(pFocus - orgL);
+    const float dirPDF = cam.imgDist * cam.imgDist / ((dl.z * dl.z * dl.z) * cam.imgPlaneArea);
+    const float3 dir = dl.x * lf.x + dl.y * lf.y + dl.z * lf.z;
+    const float cosT = fabsf(dot3(dir, lf.z));
+    const float a0 = 1.0f / (lens.w * (kNumL / (kLambdaHighD - kLambdaLowD)));
+    const float a1 = a0 * (cosT / dirPDF);
+    float al[kNumL];
+#pragma unroll
+    for (int k = 0; k < kNumL; ++k) al[k] = a1;
+    store16(b.alpha + (size_t)s * 4, al);
+    float4* g = b.gen + (size_t)s * 4;
+    g[0] = make_float4(lens.x, lens.y, lens.z, 0.0f);
+    g[1] = make_float4(dir.x, dir.y, dir.z, INFINITY);
+    g[3] = make_float4(dirPDF, 1.0f, dl.z, lens.w);
+    const bool pinhole = !(cam.lensRadius > 0.0f);
+    const float4 mis = b.mis[s];
+    b.mis[s] = make_float4(0.0f, __uint_as_float(pinhole ? M_DELTA_LAST : 0u), mis.z, 0.0f);
+    // eye vertex 0 as the predecessor of the first hit
+    float4* ev = b.ev + (size_t)s * kEyeRows;
+    ev[kVtxRows] = make_float4(lens.x, lens.y, lens.z, lens.w);
+    ev[kVtxRows + 1] = make_float4(lf.z.x, lf.z.y, lf.z.z, __uint_as_float(pinhole ? V_DELTA : 0u));
+    aux.z &= 0xFFu;   // eye vertex index 0
+}
+
+// Light step: t = 1 contributions of the traced light-to-lens rays into the
+// light image, then the light rays' hits; a light subpath that ends starts
+// the sample's eye subpath.
 __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b, DevCamera cam, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t ns = b.ctr[K_SH + cur], ne = b.ctr[K_EXT + cur];
+    const uint32_t ns = b.ctr[K_SP + cur], ne = b.ctr[K_LE + cur];
     const LensFrame lf = lensFrame(cam);
     unsigned long long splats = 0, verts = 0, capped = 0;
     for (uint32_t i = blockIdx.x * kBptBlock + threadIdx.x; i < ns; i += gridDim.x * kBptBlock) {
-        const uint32_t s = qSh(b, cur)[i];
+        const uint32_t s = qSplat(b, cur)[i];
         uint8_t& vs = b.visSplat[(size_t)cur * b.n + s];
         if (!vs) continue;
         vs = 0;
@@ -706,10 +787,10 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
     const uint32_t iters = (ne + gridDim.x * kBptBlock - 1) / (gridDim.x * kBptBlock);
     for (uint32_t it = 0; it < iters; ++it) {
         const uint32_t i = it * gridDim.x * kBptBlock + blockIdx.x * kBptBlock + threadIdx.x;
-        bool cont = false, sh = false;
+        bool cont = false, sh = false, eye = false;
         uint32_t s = 0;
         if (i < ne) {
-            s = qExt(b, cur)[i];
+            s = qLight(b, cur)[i];
             const float4* g = b.gen + (size_t)s * 4;
             uint4 aux = b.aux[s];
             const uint32_t k = aux.z & 0xFFu;
@@ -764,120 +845,96 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
                     b.rng[s] = rng;
                 }
             }
+            if (!cont) {
+                startEye(b, cam, lf, s, aux);
+                b.aux[s] = aux;
+                eye = true;
+            }
         }
-        const uint32_t pe = waveAppend(&b.ctr[K_EXT + nxt], cont);
-        if (cont) qExt(b, nxt)[pe] = s;
-        const uint32_t ps = waveAppend(&b.ctr[K_SH + nxt], sh);
-        if (sh) qSh(b, nxt)[ps] = s;
+        const uint32_t pe = waveAppend(&b.ctr[K_LE + nxt], cont);
+        if (cont) qLight(b, nxt)[pe] = s;
+        const uint32_t ps = waveAppend(&b.ctr[K_SP + nxt], sh);
+        if (sh) qSplat(b, nxt)[ps] = s;
+        const uint32_t pq = waveAppend(&b.ctr[K_EE + nxt], eye);
+        if (eye) qEye(b, nxt)[pq] = s;
     }
     statAdd(b.stat, T_SPLAT, splats);
     statAdd(b.stat, T_LIGHT_VERTS, verts);
     statAdd(b.stat, T_LIGHT_CAP, capped);
 }
 
-// The camera ray of every slot holding a sample (PerspectiveCamera::sampleRay,
-// BPTRenderer.cpp:151-169): eye vertex 0 on the lens, then the first ray.
-__global__ __launch_bounds__(kBptBlock) void k_bpt_eye_begin(DevScene sc, BptState b, DevCamera cam, uint32_t cur) {
-    const LensFrame lf = lensFrame(cam);
-    for (uint32_t base = blockIdx.x * kBptBlock; base < b.n; base += gridDim.x * kBptBlock) {
-        const uint32_t s = base + threadIdx.x;
-        uint4 aux = s < b.n ? b.aux[s] : make_uint4(0, 0, 0, 0);
-        const bool has = (aux.y & B_SAMPLE) != 0;
-        if (has) {
-            const float4 sm = b.smp[s];
-            const float4 lens = b.lens[s];
-            const float3 orgL = f3(sm.z, sm.w, 0.0f);
-            const float ux = sm.x / cam.width, uy = sm.y / cam.height;
-            const float3 pFocus = f3(cam.opWidth * (0.5f - ux), cam.opHeight * (0.5f - uy), cam.objDist);
-            const float3 dl = normalize3(pFocus - orgL);
-            const float dirPDF = cam.imgDist * cam.imgDist / ((dl.z * dl.z * dl.z) * cam.imgPlaneArea);
-            const float3 dir = dl.x * lf.x + dl.y * lf.y + dl.z * lf.z;
-            const float cosT = fabsf(dot3(dir, lf.z));
-            const float a0 = 1.0f / (lens.w * (kNumL / (kLambdaHighD - kLambdaLowD)));
-            const float a1 = a0 * (cosT / dirPDF);
-            float al[kNumL];
-#pragma unroll
-            for (int k = 0; k < kNumL; ++k) al[k] = a1;
-            store16(b.alpha + (size_t)s * 4, al);
-            float4* g = b.gen + (size_t)s * 4;
-            g[0] = make_float4(lens.x, lens.y, lens.z, 0.0f);
-            g[1] = make_float4(dir.x, dir.y, dir.z, INFINITY);
-            g[3] = make_float4(dirPDF, 1.0f, dl.z, lens.w);
-            const bool pinhole = !(cam.lensRadius > 0.0f);
-            const float4 mis = b.mis[s];
-            b.mis[s] = make_float4(0.0f, __uint_as_float(pinhole ? M_DELTA_LAST : 0u), mis.z, 0.0f);
-            // eye vertex 0 as the predecessor of the first hit
-            float4* ev = b.ev + (size_t)s * kEyeRows;
-            ev[kVtxRows] = make_float4(lens.x, lens.y, lens.z, lens.w);
-            ev[kVtxRows + 1] = make_float4(lf.z.x, lf.z.y, lf.z.z, __uint_as_float(pinhole ? V_DELTA : 0u));
-            aux.z &= 0xFFu;   // eye vertex index 0
-            b.aux[s] = aux;
-        }
-        const uint32_t pos = waveAppend(&b.ctr[K_EXT + cur], has);
-        if (has) qExt(b, cur)[pos] = s;
-    }
-}
-
 // Gather: unoccluded connections of the last eye vertex join the sample's
-// radiance; a sample whose eye subpath ended is filed into the slot's plane.
-__global__ __launch_bounds__(kBptBlock) void k_bpt_gather(BptState b, uint32_t cur) {
-    const uint32_t nc = b.ctr[K_CONN + cur];
+// radiance; a sample whose eye subpath ended is filed into the slot's plane
+// and the slot's next sample starts.
+__global__ __launch_bounds__(kBptBlock) void k_bpt_gather(DevScene sc, BptState b, DevCamera cam, uint32_t cur) {
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t nc = b.ctr[K_GA + cur];
+    const LensFrame lf = lensFrame(cam);
     unsigned long long samples = 0;
-    for (uint32_t i = blockIdx.x * kBptBlock + threadIdx.x; i < nc; i += gridDim.x * kBptBlock) {
-        const uint32_t s = qConn(b, cur)[i];
-        uint4 aux = b.aux[s];
-        const uint32_t nL = aux.z & 0xFFu;
-        float rad[kNumL];
-        load16(b.rad + (size_t)s * 4, rad);
-        bool changed = false;
-        for (uint32_t j = 0; j < nL; ++j) {
-            const size_t idx = (size_t)s * kLightCap + j;
-            if (!b.vis[idx]) continue;
-            b.vis[idx] = 0;
-            float c[kNumL];
-            load16(b.conn + idx * kConnRows, c);
+    const uint32_t iters = (nc + gridDim.x * kBptBlock - 1) / (gridDim.x * kBptBlock);
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint32_t i = it * gridDim.x * kBptBlock + blockIdx.x * kBptBlock + threadIdx.x;
+        bool next = false, sh = false;
+        uint32_t s = 0;
+        if (i < nc) {
+            s = qGather(b, cur)[i];
+            uint4 aux = b.aux[s];
+            const uint32_t nL = aux.z & 0xFFu;
+            float rad[kNumL];
+            load16(b.rad + (size_t)s * 4, rad);
+            bool changed = false;
+            for (uint32_t j = 0; j < nL; ++j) {
+                const size_t idx = (size_t)s * kLightCap + j;
+                if (!b.vis[idx]) continue;
+                b.vis[idx] = 0;
+                float c[kNumL];
+                load16(b.conn + idx * kConnRows, c);
 #pragma unroll
-            for (int k = 0; k < kNumL; ++k) rad[k] += c[k];
-            changed = true;
-        }
-        if (aux.y & B_END) {
-            bool finite = true, any = false;
-#pragma unroll
-            for (int k = 0; k < kNumL; ++k) {
-                finite = finite && isfinite(rad[k]);
-                any = any || rad[k] != 0.0f;
+                for (int k = 0; k < kNumL; ++k) rad[k] += c[k];
+                changed = true;
             }
-            if (!finite) {
-                atomicAdd(&b.ctr[K_NONFINITE], 1u);
-            } else if (any) {
-                float add[kNumL];
-                binStrata(b.mis[s].z, rad, add);
-                const uint32_t plane = s / b.pixels;
-                double* f = b.planeSums + ((size_t)plane * b.pixels + aux.x) * kNumL;
+            if (aux.y & B_END) {
+                bool finite = true, any = false;
 #pragma unroll
-                for (int k = 0; k < kNumL; k += 2) {
-                    double2 v = *(double2*)(f + k);
-                    v.x += (double)add[k];
-                    v.y += (double)add[k + 1];
-                    *(double2*)(f + k) = v;
+                for (int k = 0; k < kNumL; ++k) {
+                    finite = finite && isfinite(rad[k]);
+                    any = any || rad[k] != 0.0f;
                 }
+                if (!finite) {
+                    atomicAdd(&b.ctr[K_NONFINITE], 1u);
+                } else if (any) {
+                    float add[kNumL];
+                    binStrata(b.mis[s].z, rad, add);
+                    const uint32_t plane = s / b.pixels;
+                    double* f = b.planeSums + ((size_t)plane * b.pixels + aux.x) * kNumL;
+#pragma unroll
+                    for (int k = 0; k < kNumL; k += 2) {
+                        double2 v = *(double2*)(f + k);
+                        v.x += (double)add[k];
+                        v.y += (double)add[k + 1];
+                        *(double2*)(f + k) = v;
+                    }
+                }
+                ++samples;
+                next = startSample(sc, b, cam, lf, s, nxt, aux, &sh);
+            } else if (changed) {
+                store16(b.rad + (size_t)s * 4, rad);
             }
-            aux.y &= ~(B_SAMPLE | B_END);
-            b.aux[s] = aux;
-            ++samples;
-        } else if (changed) {
-            store16(b.rad + (size_t)s * 4, rad);
         }
+        const uint32_t pl = waveAppend(&b.ctr[K_LE + nxt], next);
+        if (next) qLight(b, nxt)[pl] = s;
+        const uint32_t ps = waveAppend(&b.ctr[K_SP + nxt], sh);
+        if (sh) qSplat(b, nxt)[ps] = s;
     }
     statAdd(b.stat, T_SAMPLES, samples);
 }
 
-// Eye phase step: each hit becomes eye vertex k; its emission (s = 0,
+// Eye step: each eye-ray hit becomes eye vertex k; its emission (s = 0,
 // BPTRenderer.cpp:316-336), its connections to the stored light vertices
 // (s >= 1) and the next direction.
 __global__ __launch_bounds__(kBptBlock) void k_bpt_eye(DevScene sc, BptState b, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t ne = b.ctr[K_EXT + cur];
+    const uint32_t ne = b.ctr[K_EE + cur];
     unsigned long long verts = 0, capped = 0;
     const uint32_t iters = (ne + gridDim.x * kBptBlock - 1) / (gridDim.x * kBptBlock);
     for (uint32_t it = 0; it < iters; ++it) {
@@ -885,7 +942,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_eye(DevScene sc, BptState b, 
         bool cont = false, gather = false;
         uint32_t s = 0;
         if (i < ne) {
-            s = qExt(b, cur)[i];
+            s = qEye(b, cur)[i];
             gather = true;
             const float4* g = b.gen + (size_t)s * 4;
             uint4 aux = b.aux[s];
@@ -963,10 +1020,10 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_eye(DevScene sc, BptState b, 
                 b.aux[s] = aux;
             }
         }
-        const uint32_t pe = waveAppend(&b.ctr[K_EXT + nxt], cont);
-        if (cont) qExt(b, nxt)[pe] = s;
-        const uint32_t pc = waveAppend(&b.ctr[K_CONN + nxt], gather);
-        if (gather) qConn(b, nxt)[pc] = s;
+        const uint32_t pe = waveAppend(&b.ctr[K_EE + nxt], cont);
+        if (cont) qEye(b, nxt)[pe] = s;
+        const uint32_t pc = waveAppend(&b.ctr[K_GA + nxt], gather);
+        if (gather) qGather(b, nxt)[pc] = s;
     }
     statAdd(b.stat, T_EYE_VERTS, verts);
     statAdd(b.stat, T_EYE_CAP, capped);
@@ -980,14 +1037,14 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_eye(DevScene sc, BptState b, 
 // sample in k_bpt_gather.
 __global__ __launch_bounds__(kBptBlock) void k_bpt_connect(DevScene sc, BptState b, uint32_t nxt) {
     __shared__ uint32_t incl[kBptBlock], slots[kBptBlock];
-    const uint32_t nc = b.ctr[K_CONN + nxt];
+    const uint32_t nc = b.ctr[K_GA + nxt];
     const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
     unsigned long long conns = 0;
     for (uint32_t base = blockIdx.x * kBptBlock; base < nc; base += gridDim.x * kBptBlock) {
         const uint32_t i = base + threadIdx.x;
         uint32_t s = 0, np = 0;
         if (i < nc) {
-            s = qConn(b, nxt)[i];
+            s = qGather(b, nxt)[i];
             const uint4 aux = b.aux[s];
             np = (aux.y & B_CONNECT) ? (aux.z & 0xFFu) : 0u;
         }
@@ -1047,8 +1104,8 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_connect(DevScene sc, BptState
                     ++conns;
                 }
             }
-            const uint32_t pos = waveAppend(&b.ctr[K_SH + nxt], ok);
-            if (ok) qSh(b, nxt)[pos] = (uint32_t)idx;
+            const uint32_t pos = waveAppend(&b.ctr[K_CN + nxt], ok);
+            if (ok) qConn(b, nxt)[pos] = (uint32_t)idx;
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -1100,8 +1157,12 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
     uint32_t planes = cfg.slotsPerPixel ? std::min(cfg.slotsPerPixel, spp)
                                         : std::max(1u, std::min(spp, kTargetSlots / std::max(1u, pixels)));
     const size_t nSlots = (size_t)pixels * planes;
-    if (nSlots > (1u << 26)) {
-        if (err) *err = "too many slots for the bidirectional integrator (pixels x slots_per_pixel > 2^26)";
+    if (nSlots > kMaxBptSlots) {
+        if (err) *err = "too many slots for the bidirectional integrator (pixels x slots_per_pixel > 2^24)";
+        return SLRH_ERR_CAPACITY_CODE;
+    }
+    if ((spp + planes - 1) / planes > kMaxSlotSamples) {
+        if (err) *err = "too many samples per slot for the bidirectional integrator (spp / slots_per_pixel > 65535)";
         return SLRH_ERR_CAPACITY_CODE;
     }
     const uint32_t n = (uint32_t)nSlots;
@@ -1176,14 +1237,15 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
         HIP_TRY(hipStreamSynchronize(stream));
         return 0;
     };
-    auto launchTrace = [&](uint32_t cur, int eyePhase, uint32_t work) -> int {
+    const uint32_t grid = blocksFor(n);
+    const uint32_t traceGrid = std::min(traceBlocks, blocksFor(4 * n));
+    auto launchStep = [&](uint32_t cur) -> int {
         HIP_TRY(hipMemsetAsync(ctr.p + K_FETCH, 0, sizeof(uint32_t), stream));
         if (cfg.timeKernels) HIP_TRY(hipEventRecord(ta, stream));
-        const uint32_t blocks = std::min(traceBlocks, blocksFor(work));
         if (sc.numInstances)
-            hipLaunchKernelGGL(k_bpt_trace<true>, dim3(blocks), dim3(kBptBlock), 0, stream, sc, b, cur, eyePhase);
+            hipLaunchKernelGGL(k_bpt_trace<true>, dim3(traceGrid), dim3(kBptBlock), 0, stream, sc, b, cur);
         else
-            hipLaunchKernelGGL(k_bpt_trace<false>, dim3(blocks), dim3(kBptBlock), 0, stream, sc, b, cur, eyePhase);
+            hipLaunchKernelGGL(k_bpt_trace<false>, dim3(traceGrid), dim3(kBptBlock), 0, stream, sc, b, cur);
         HIP_TRY(hipGetLastError());
         if (cfg.timeKernels) {
             HIP_TRY(hipEventRecord(tb, stream));
@@ -1193,52 +1255,36 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
             traceSeconds += ms * 1e-3;
         }
         traceLaunches += 1;
+        hipLaunchKernelGGL(k_bpt_light, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_bpt_gather, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_bpt_eye, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cur);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_bpt_connect, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cur ^ 1u);
+        HIP_TRY(hipGetLastError());
+        iterations += 1;
         return 0;
     };
 
     HIP_TRY(hipEventRecord(ev0, stream));
-    hipLaunchKernelGGL(k_bpt_seed, dim3(blocksFor(n)), dim3(kBptBlock), 0, stream, b, cfg.seed);
+    hipLaunchKernelGGL(k_bpt_seed, dim3(grid), dim3(kBptBlock), 0, stream, b, cfg.seed);
     HIP_TRY(hipGetLastError());
-    const uint32_t passes = (spp + planes - 1) / planes;
-    for (uint32_t pass = 0; pass < passes; ++pass) {
-        HIP_TRY(hipMemsetAsync(ctr.p, 0, 8 * sizeof(uint32_t), stream));
-        hipLaunchKernelGGL(k_bpt_begin, dim3(blocksFor(n)), dim3(kBptBlock), 0, stream, sc, b, cam, pass);
-        HIP_TRY(hipGetLastError());
-        // light subpaths
-        uint32_t cur = 0;
-        for (;;) {
-            if (int e = readCounters()) return e;
-            const uint32_t ne = hostCtr[K_EXT + cur], ns = hostCtr[K_SH + cur];
-            if (ne + ns == 0) break;
-            if (int e = launchTrace(cur, 0, ne + ns)) return e;
-            hipLaunchKernelGGL(k_bpt_light, dim3(blocksFor(std::max(ne, ns))), dim3(kBptBlock), 0, stream, sc, b, cam,
-                               cur);
-            HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_bpt_start, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam);
+    HIP_TRY(hipGetLastError());
+    // Steps until every queue is empty (every slot has filed its share of
+    // samples). The kernels take their queue lengths from the device
+    // counters, so the host reads them only every few steps; the steps past
+    // the end find empty queues.
+    uint32_t cur = 0;
+    for (;;) {
+        if (int e = readCounters()) return e;
+        const uint32_t pending = hostCtr[K_LE + cur] + hostCtr[K_EE + cur] + hostCtr[K_SP + cur] +
+                                 hostCtr[K_CN + cur] + hostCtr[K_GA + cur];
+        if (pending == 0) break;
+        for (uint32_t k = 0; k < kStepsPerRead; ++k) {
+            if (int e = launchStep(cur)) return e;
             cur ^= 1u;
-            iterations += 1;
-        }
-        // eye subpaths and connections
-        HIP_TRY(hipMemsetAsync(ctr.p, 0, 8 * sizeof(uint32_t), stream));
-        cur = 0;
-        hipLaunchKernelGGL(k_bpt_eye_begin, dim3(blocksFor(n)), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
-        HIP_TRY(hipGetLastError());
-        for (;;) {
-            if (int e = readCounters()) return e;
-            const uint32_t ne = hostCtr[K_EXT + cur], ns = hostCtr[K_SH + cur], nc = hostCtr[K_CONN + cur];
-            if (ne + ns + nc == 0) break;
-            if (int e = launchTrace(cur, 1, ne + ns)) return e;
-            if (nc) {
-                hipLaunchKernelGGL(k_bpt_gather, dim3(blocksFor(nc)), dim3(kBptBlock), 0, stream, b, cur);
-                HIP_TRY(hipGetLastError());
-            }
-            hipLaunchKernelGGL(k_bpt_eye, dim3(blocksFor(ne)), dim3(kBptBlock), 0, stream, sc, b, cur);
-            HIP_TRY(hipGetLastError());
-            if (ne) {
-                hipLaunchKernelGGL(k_bpt_connect, dim3(blocksFor(ne)), dim3(kBptBlock), 0, stream, sc, b, cur ^ 1u);
-                HIP_TRY(hipGetLastError());
-            }
-            cur ^= 1u;
-            iterations += 1;
         }
     }
     hipLaunchKernelGGL(k_bpt_reduce, dim3(blocksFor(pixels * kNumL)), dim3(kBptBlock), 0, stream, planeSums.p, planes,
