@@ -99,7 +99,6 @@ enum BptCounter : uint32_t {
     K_SP = 4,     // +0/1: light-to-lens ray queue lengths
     K_CN = 6,     // +0/1: connection ray queue lengths
     K_GA = 8,     // +0/1: gather queue lengths (slots whose eye vertex was just made)
-    K_FETCH = 10, // traversal fetch cursor
     K_NONFINITE = 11,
     K_OVERFLOW = 12,
     kBptCounters = 16
@@ -135,6 +134,7 @@ struct BptState {
     double* lightImg;   // [pixels][16]
     uint32_t* spill;
     uint32_t spillStride;
+    uint32_t* fetch;    // the trace's claim cursors, one per part, kFetchStride words apart
 };
 
 // queues of slot indices (connection entries: slot * kLightCap + light
@@ -210,6 +210,7 @@ __device__ __forceinline__ void materialSpectra(const DevScene& sc, uint32_t mat
 // (DiffuseEDF::evaluate/evaluatePDF, libSLR/EDF/basic_edfs.cpp:20-30; its
 // proxy leaves the reverse unset, BPTRenderer.h:30-37), other vertices their
 // BSDF with the subpath's query (BSDFProxy, BPTRenderer.h:40-52).
+template <bool kDiel>
 __device__ __forceinline__ void vertexDdf(const DevScene& sc, const Vtx& v, bool adjoint, int hero, float off,
                                           const float3& dl, float* fs, float* pdf, float* rev) {
     *rev = 0.0f;
@@ -229,7 +230,7 @@ __device__ __forceinline__ void vertexDdf(const DevScene& sc, const Vtx& v, bool
     }
     float A[kNumL], B[kNumL];
     materialSpectra(sc, v.mat, m.type, off, A, B);
-    const BiBsdf b{m.type, A, B, m.p0, m.p1};
+    const BiBsdf<kDiel> b{m.type, A, B, m.p0, m.p1};
     BiQuery q;
     q.dir = v.dir;
     q.ng = vtxLocal(v, v.gN);
@@ -295,6 +296,7 @@ __device__ __forceinline__ LensFrame lensFrame(const DevCamera& cam) {
 // with the IDF proxy): the contribution for the pixel the direction crosses,
 // and the shadow ray from the lens (Scene::testVisibility, Scene.cpp:96-112).
 // Lp0/Lp1: rows 0 and 1 of the vertex below (when hasPrev), pFlags its flags.
+template <bool kDiel>
 __device__ __forceinline__ bool connectLens(const DevScene& sc, const DevCamera& cam, const LensFrame& lf,
                                             const float4& lens, float2 orgL, const Vtx& L, const float* lAlpha,
                                             bool hasPrev, float4 Lp0, float4 Lp1, uint32_t pFlags, int hero,
@@ -316,7 +318,7 @@ __device__ __forceinline__ bool connectLens(const DevScene& sc, const DevCamera&
     if (!valid) return false;
     const float eExt1DirPDF = cam.imgDist * cam.imgDist / ((e.z * e.z * e.z) * cam.imgPlaneArea);
     float lDDF[kNumL], lExt1DirPDF, eExt2DirPDF;
-    vertexDdf(sc, L, true, hero, off, vtxLocal(L, -cv), lDDF, &lExt1DirPDF, &eExt2DirPDF);
+    vertexDdf<kDiel>(sc, L, true, hero, off, vtxLocal(L, -cv), lDDF, &lExt1DirPDF, &eExt2DirPDF);
     bool any = false;
 #pragma unroll
     for (int k = 0; k < kNumL; ++k) any = any || lDDF[k] * G != 0.0f;
@@ -366,6 +368,7 @@ __device__ __forceinline__ bool connectLens(const DevScene& sc, const DevCamera&
 // RR probabilities of the extensions are importances of the DDF values
 // scaled by a cosine over a PDF; importance is linear, so each DDF's
 // importance is taken once.
+template <bool kDiel>
 __device__ __forceinline__ bool connectVertices(const DevScene& sc, const Vtx& E, const float4* eAlpha, bool eBelow,
                                                 float4 Ep0, float4 Ep1, uint32_t eFlagsPrev, const Vtx& L,
                                                 const float4* lAlpha, bool lHasPrev, float4 Lp0, float4 Lp1,
@@ -378,14 +381,14 @@ __device__ __forceinline__ bool connectVertices(const DevScene& sc, const Vtx& E
     const float cosE = fabsf(dot3(cv, E.gN));
     const float G = cosE * cosL / dist2;
     float lExt1DirPDF, eExt2DirPDF, eExt1DirPDF, lExt2DirPDF;
-    vertexDdf(sc, L, true, hero, off, vtxLocal(L, -cv), c, &lExt1DirPDF, &eExt2DirPDF);
+    vertexDdf<kDiel>(sc, L, true, hero, off, vtxLocal(L, -cv), c, &lExt1DirPDF, &eExt2DirPDF);
     bool any = false;
 #pragma unroll
     for (int k = 0; k < kNumL; ++k) any = any || c[k] != 0.0f;
     if (!any) return false;
     const float impL = importance16(c, hero);
     float eDDF[kNumL];
-    vertexDdf(sc, E, false, hero, off, vtxLocal(E, cv), eDDF, &eExt1DirPDF, &lExt2DirPDF);
+    vertexDdf<kDiel>(sc, E, false, hero, off, vtxLocal(E, cv), eDDF, &eExt1DirPDF, &lExt2DirPDF);
     any = false;
 #pragma unroll
     for (int k = 0; k < kNumL; ++k) {
@@ -462,6 +465,7 @@ __global__ void k_bpt_seed(BptState b, uint32_t seed) {
 // BPTRenderer.cpp:121-148) with its connection to the lens (written for the
 // trace of step parity `parity`) and the first light ray. Returns whether a
 // sample started; *splat whether its lens ray is to be traced.
+template <bool kDiel>
 __device__ __forceinline__ bool startSample(const DevScene& sc, const BptState& b, const DevCamera& cam,
                                             const LensFrame& lf, uint32_t s, uint32_t parity, uint4& aux,
                                             bool* splat) {
@@ -537,7 +541,7 @@ __device__ __forceinline__ bool startSample(const DevScene& sc, const BptState& 
     float c[kNumL];
     uint32_t pix = 0;
     float4 ro, rd;
-    const bool sh = connectLens(sc, cam, lf, lens, orgL, L, alpha0, false, make_float4(0, 0, 0, 1),
+    const bool sh = connectLens<kDiel>(sc, cam, lf, lens, orgL, L, alpha0, false, make_float4(0, 0, 0, 1),
                                 make_float4(0, 0, 0, 0), 0u, (int)hero, off, c, &pix, &ro, &rd);
     if (sh) {
         float4* sp = splatRows(b, parity, s);
@@ -552,6 +556,7 @@ __device__ __forceinline__ bool startSample(const DevScene& sc, const BptState& 
 }
 
 // Every slot's first sample.
+template <bool kDiel>
 __global__ __launch_bounds__(kBptBlock) void k_bpt_start(DevScene sc, BptState b, DevCamera cam) {
     const LensFrame lf = lensFrame(cam);
     for (uint32_t base = blockIdx.x * kBptBlock; base < b.n; base += gridDim.x * kBptBlock) {
@@ -559,7 +564,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_start(DevScene sc, BptState b
         bool has = false, sh = false;
         if (s < b.n) {
             uint4 aux = b.aux[s];
-            has = startSample(sc, b, cam, lf, s, 0u, aux, &sh);
+            has = startSample<kDiel>(sc, b, cam, lf, s, 0u, aux, &sh);
         }
         const uint32_t shPos = waveAppend(&b.ctr[K_SP + 0], sh);
         if (sh) qSplat(b, 0)[shPos] = s;
@@ -602,8 +607,8 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_trace(DevScene sc, BptState b
     const uint32_t* qc = qConn(b, cur);
     const uint32_t n1 = nl, n2 = n1 + ne, n3 = n2 + nsp;
     TraceCounters tc;
-    traceLoop<16, true, false, I>(
-        sc, st, n3 + ncn, &b.ctr[K_FETCH],
+    traceLoop<16, true, false, I, SLRH_FETCH_PARTS>(
+        sc, st, n3 + ncn, b.fetch,
         [&](uint32_t i) {
             return i < n1 ? ql[i] : i < n2 ? qe[i - n1] : i < n3 ? (qs[i - n2] | kShadowBit | kSplatBit)
                                                                  : (qc[i - n3] | kShadowBit);
@@ -670,13 +675,14 @@ __device__ __forceinline__ NewVertex hitVertex(const DevScene& sc, const BptStat
 // ends here. On success the next ray, throughput and carried PDFs are
 // stored; T[k-1] (k the new vertex) is formed from the reverse PDF toward
 // the vertex below.
+template <bool kDiel>
 __device__ __forceinline__ bool extendSubpath(const DevScene& sc, const BptState& b, uint32_t s, const NewVertex& nv,
                                               uint32_t mat, const DevMaterial& m, float P_k, bool deltaK, int hero,
                                               float off, bool adjoint, bool tailDefined, float* alpha, uint4& rng) {
     if (m.type == MAT_NONE) return false;
     float A[kNumL], B[kNumL];
     materialSpectra(sc, mat, m.type, off, A, B);
-    const BiBsdf bs{m.type, A, B, m.p0, m.p1};
+    const BiBsdf<kDiel> bs{m.type, A, B, m.p0, m.p1};
     BiQuery q;
     q.dir = nv.dirOut;
     q.ng = nv.gL;
@@ -763,6 +769,7 @@ __device__ __forceinline__ void startEye(const BptState& b, const DevCamera& cam
 // Light step: t = 1 contributions of the traced light-to-lens rays into the
 // light image, then the light rays' hits; a light subpath that ends starts
 // the sample's eye subpath.
+template <bool kDiel>
 __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b, DevCamera cam, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
     const uint32_t ns = b.ctr[K_SP + cur], ne = b.ctr[K_LE + cur];
@@ -831,7 +838,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
                     uint32_t pix = 0;
                     float4 ro, rd;
                     const float4 sm = b.smp[s];
-                    sh = connectLens(sc, cam, lf, b.lens[s], make_float2(sm.z, sm.w), V, alpha, true, pr[0], pr[1],
+                    sh = connectLens<kDiel>(sc, cam, lf, b.lens[s], make_float2(sm.z, sm.w), V, alpha, true, pr[0], pr[1],
                                      __float_as_uint(pr[3].w), hero, off, c, &pix, &ro, &rd);
                     if (sh) {
                         float4* spl = splatRows(b, nxt, s);
@@ -841,7 +848,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
                         spl[6] = make_float4(__uint_as_float(pix), 0, 0, 0);
                     }
                     uint4 rng = b.rng[s];
-                    cont = extendSubpath(sc, b, s, nv, nv.sp.mat, m, V.P, deltaK, hero, off, true, true, alpha, rng);
+                    cont = extendSubpath<kDiel>(sc, b, s, nv, nv.sp.mat, m, V.P, deltaK, hero, off, true, true, alpha, rng);
                     b.rng[s] = rng;
                 }
             }
@@ -866,6 +873,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
 // Gather: unoccluded connections of the last eye vertex join the sample's
 // radiance; a sample whose eye subpath ended is filed into the slot's plane
 // and the slot's next sample starts.
+template <bool kDiel>
 __global__ __launch_bounds__(kBptBlock) void k_bpt_gather(DevScene sc, BptState b, DevCamera cam, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
     const uint32_t nc = b.ctr[K_GA + cur];
@@ -916,7 +924,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_gather(DevScene sc, BptState 
                     }
                 }
                 ++samples;
-                next = startSample(sc, b, cam, lf, s, nxt, aux, &sh);
+                next = startSample<kDiel>(sc, b, cam, lf, s, nxt, aux, &sh);
             } else if (changed) {
                 store16(b.rad + (size_t)s * 4, rad);
             }
@@ -932,6 +940,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_gather(DevScene sc, BptState 
 // Eye step: each eye-ray hit becomes eye vertex k; its emission (s = 0,
 // BPTRenderer.cpp:316-336), its connections to the stored light vertices
 // (s >= 1) and the next direction.
+template <bool kDiel>
 __global__ __launch_bounds__(kBptBlock) void k_bpt_eye(DevScene sc, BptState b, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
     const uint32_t ne = b.ctr[K_EE + cur];
@@ -1013,7 +1022,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_eye(DevScene sc, BptState b, 
                 else aux.y &= ~B_CONNECT;
                 // the next direction, or the end of the eye subpath
                 uint4 rng = b.rng[s];
-                cont = extendSubpath(sc, b, s, nv, nv.sp.mat, m, E.P, deltaK, hero, off, false, k >= 2, alpha, rng);
+                cont = extendSubpath<kDiel>(sc, b, s, nv, nv.sp.mat, m, E.P, deltaK, hero, off, false, k >= 2, alpha, rng);
                 b.rng[s] = rng;
                 if (!cont) aux.y |= B_END;
                 aux.z = (aux.z & 0xFFu) | (k << 8);
@@ -1035,6 +1044,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_eye(DevScene sc, BptState b, 
 // counts) and deals them out 64 at a time, so a long light subpath does not
 // hold the rest of the wave. An unoccluded pair's contribution joins the
 // sample in k_bpt_gather.
+template <bool kDiel>
 __global__ __launch_bounds__(kBptBlock) void k_bpt_connect(DevScene sc, BptState b, uint32_t nxt) {
     __shared__ uint32_t incl[kBptBlock], slots[kBptBlock];
     const uint32_t nc = b.ctr[K_GA + nxt];
@@ -1093,7 +1103,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_connect(DevScene sc, BptState
                 }
                 float4 ro, rd;
                 idx = (size_t)slot * kLightCap + j;
-                ok = connectVertices(sc, E, ev + 5, k >= 2, Ep0, Ep1, __float_as_uint(Ep1.w), L, lr + 5, j > 0, Lp0,
+                ok = connectVertices<kDiel>(sc, E, ev + 5, k >= 2, Ep0, Ep1, __float_as_uint(Ep1.w), L, lr + 5, j > 0, Lp0,
                                      Lp1, lpFlags, hero, off, c, &ro, &rd);
                 b.vis[idx] = 0;
                 if (ok) {
@@ -1132,7 +1142,7 @@ struct BptPool {
     DevBuf<float4> gen, mis, alpha, rad, smp, lens, lv, ev, conn, splat;
     DevBuf<uint4> aux, rng;
     DevBuf<uint8_t> vis, visSplat;
-    DevBuf<uint32_t> q, ctr, spill;
+    DevBuf<uint32_t> q, ctr, spill, fetch;
     DevBuf<unsigned long long> stat;
     DevBuf<double> planeSums, lightImg;
 };
@@ -1189,7 +1199,8 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
             (r = np->rng.alloc(n, err)) || (r = np->vis.alloc((size_t)n * kLightCap, err)) ||
             (r = np->visSplat.alloc((size_t)2 * n, err)) || (r = np->q.alloc(queueWords(n), err)) ||
             (r = np->ctr.alloc(kBptCounters, err)) || (r = np->stat.alloc((size_t)kBptStats * kStatSpread, err)) ||
-            (r = np->spill.alloc((size_t)(kMaxStack - kBptLdsDepth) * traceBlocks * kBptBlock, err)))
+            (r = np->spill.alloc((size_t)(kMaxStack - kBptLdsDepth) * traceBlocks * kBptBlock, err)) ||
+            (r = np->fetch.alloc((size_t)SLRH_FETCH_PARTS * kFetchStride, err)))
             return r;
         np->slots = n;
         np->traceBlocks = traceBlocks;
@@ -1217,7 +1228,7 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
     b.gen = gen.p; b.aux = aux.p; b.mis = mis.p; b.rng = rng.p; b.alpha = alpha.p; b.rad = rad.p; b.smp = smp.p;
     b.lens = lens.p; b.lv = lv.p; b.ev = ev.p; b.conn = conn.p; b.splat = splat.p; b.vis = vis.p;
     b.visSplat = visSplat.p; b.q = q.p; b.ctr = ctr.p; b.stat = stat.p; b.planeSums = planeSums.p;
-    b.lightImg = lightImg.p; b.spill = spill.p; b.spillStride = traceBlocks * kBptBlock;
+    b.lightImg = lightImg.p; b.spill = spill.p; b.spillStride = traceBlocks * kBptBlock; b.fetch = P.fetch.p;
     DevCamera cam;
     cameraDerived(camera, W, H, &cam);
     const DevScene& sc = ds->view;
@@ -1237,10 +1248,12 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
         HIP_TRY(hipStreamSynchronize(stream));
         return 0;
     };
+    // kernels without the GGX dielectric for scenes that have none
+    const bool diel = (ds->matTypeMask & (1u << MAT_GGX_DIELECTRIC)) != 0;
     const uint32_t grid = blocksFor(n);
     const uint32_t traceGrid = std::min(traceBlocks, blocksFor(4 * n));
     auto launchStep = [&](uint32_t cur) -> int {
-        HIP_TRY(hipMemsetAsync(ctr.p + K_FETCH, 0, sizeof(uint32_t), stream));
+        HIP_TRY(hipMemsetAsync(b.fetch, 0, (size_t)SLRH_FETCH_PARTS * kFetchStride * sizeof(uint32_t), stream));
         if (cfg.timeKernels) HIP_TRY(hipEventRecord(ta, stream));
         if (sc.numInstances)
             hipLaunchKernelGGL(k_bpt_trace<true>, dim3(traceGrid), dim3(kBptBlock), 0, stream, sc, b, cur);
@@ -1255,13 +1268,17 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
             traceSeconds += ms * 1e-3;
         }
         traceLaunches += 1;
-        hipLaunchKernelGGL(k_bpt_light, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_bpt_gather, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_bpt_eye, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cur);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_bpt_connect, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cur ^ 1u);
+        if (diel) {
+            hipLaunchKernelGGL(k_bpt_light<true>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
+            hipLaunchKernelGGL(k_bpt_gather<true>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
+            hipLaunchKernelGGL(k_bpt_eye<true>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cur);
+            hipLaunchKernelGGL(k_bpt_connect<true>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cur ^ 1u);
+        } else {
+            hipLaunchKernelGGL(k_bpt_light<false>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
+            hipLaunchKernelGGL(k_bpt_gather<false>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
+            hipLaunchKernelGGL(k_bpt_eye<false>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cur);
+            hipLaunchKernelGGL(k_bpt_connect<false>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cur ^ 1u);
+        }
         HIP_TRY(hipGetLastError());
         iterations += 1;
         return 0;
@@ -1270,7 +1287,8 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
     HIP_TRY(hipEventRecord(ev0, stream));
     hipLaunchKernelGGL(k_bpt_seed, dim3(grid), dim3(kBptBlock), 0, stream, b, cfg.seed);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_bpt_start, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam);
+    if (diel) hipLaunchKernelGGL(k_bpt_start<true>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam);
+    else hipLaunchKernelGGL(k_bpt_start<false>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam);
     HIP_TRY(hipGetLastError());
     // Steps until every queue is empty (every slot has filed its share of
     // samples). The kernels take their queue lengths from the device

@@ -40,6 +40,9 @@ __device__ __forceinline__ float biCorrection(const BiQuery& q, const float3& d)
 
 // A vertex's scattering model: type (MAT_*), the 16-wavelength spectra A, B
 // (reflectance | eta, k | eta_ext, eta_int) and the model parameters.
+// kDiel: the GGX dielectric is compiled in (kernels for scenes without one
+// leave it out and hold far fewer registers).
+template <bool kDiel>
 struct BiBsdf {
     uint32_t type;
     const float* A;
@@ -104,7 +107,7 @@ struct BiBsdf {
             const float denom = 4 * q.dir.z * d.z;
 #pragma unroll
             for (int i = 0; i < kNumL; ++i) fs[i] = fresnelConductor(A[i], B[i], dotHV) * dm * G / denom;
-        } else if (type == MAT_GGX_DIELECTRIC) {
+        } else if ((kDiel && type == MAT_GGX_DIELECTRIC)) {
             const bool entering = q.dir.z >= 0.0f;
             const float sign = entering ? 1.0f : -1.0f;
             const float nvnl = d.z * q.dir.z;
@@ -147,7 +150,7 @@ struct BiBsdf {
             *rev = common * D.pdfVisible(sign * d, m);
             return common * D.pdfVisible(sign * q.dir, m);
         }
-        if (type != MAT_GGX_DIELECTRIC) return 0.0f;
+        if (!kDiel || type != MAT_GGX_DIELECTRIC) return 0.0f;
         const float nvnl = d.z * q.dir.z;
         if (nvnl == 0) return 0.0f;
         const float extH = pick16(A, q.hero), intH = pick16(B, q.hero);
@@ -206,7 +209,7 @@ struct BiBsdf {
 #pragma unroll
             for (int i = 0; i < kNumL; ++i) fs[i] = fresnelConductor(A[i], B[i], dotHV) * dm * G / denom;
             s->revPdf = common * D.pdfVisible(sign * wi, m);
-        } else if (type == MAT_GGX_DIELECTRIC) {
+        } else if ((kDiel && type == MAT_GGX_DIELECTRIC)) {
             const bool entering = q.dir.z >= 0.0f;
             const float sign = entering ? 1.0f : -1.0f;
             const GGX D = ggx();

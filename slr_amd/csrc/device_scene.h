@@ -84,6 +84,7 @@ struct DeviceScene {
     DevBuf<float4> bvh2Nodes, bvh2Tris;
     DevScene view;
     bool hasCamera = false;
+    uint32_t matTypeMask = 0;   // bit t set: a material of type t (MAT_*) is present
     CameraParams camera;
     size_t bytes = 0;
     std::shared_ptr<struct SlotPool> pool;   // wavefront state kept between renders

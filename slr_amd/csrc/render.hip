@@ -368,6 +368,8 @@ int uploadScene(const SceneData& sd, DeviceScene** out, std::string* err) {
         (r = ds->primToLight.upload(primToLight, err)) || (r = ds->instXf.upload(instXf, err)) ||
         (r = ds->instMat.upload(instMat, err)) || (r = ds->matSpec.upload(matSpec, err)))
         return r;
+    ds->matTypeMask = 0;
+    for (const DevMaterial& m : sd.materials) ds->matTypeMask |= 1u << m.type;
     if ((r = ds->counters.alloc(16, err))) return r;
     DevScene& ve = ds->view;
     ve.hasEnv = sd.env.present ? 1u : 0u;
