@@ -606,6 +606,10 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_trace(DevScene sc, BptState b
     const uint32_t* qs = qSplat(b, cur);
     const uint32_t* qc = qConn(b, cur);
     const uint32_t n1 = nl, n2 = n1 + ne, n3 = n2 + nsp;
+    // a short queue (the last steps of a frame) goes out in smaller chunks,
+    // so its rays spread over more waves instead of queueing on few lanes
+    const uint32_t waves = gridDim.x * (kBptBlock / 64);
+    const uint32_t chunk = min(128u, max(16u, ((n3 + ncn) / waves + 15u) & ~15u));
     TraceCounters tc;
     traceLoop<16, true, false, I, SLRH_FETCH_PARTS>(
         sc, st, n3 + ncn, b.fetch,
@@ -636,7 +640,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_trace(DevScene sc, BptState b
                 if (I) b.aux[e].w = (uint32_t)h.inst;
             }
         },
-        tc);
+        tc, chunk);
     statAdd(b.stat, T_EXT_RAYS, tc.rays - tc.raysAny);
     statAdd(b.stat, T_SH_RAYS, tc.raysAny);
     statAdd(b.stat, T_NODES_EXT, tc.nodes - tc.nodesAny);

@@ -273,12 +273,14 @@ __device__ __forceinline__ uint32_t kthSetBit(unsigned long long m, uint32_t k) 
 //   load(e, RayIn& r)   -> bool, requests the ray of entry e (r.inv too when
 //                          kHasInv, else it is computed here); true for any hit
 //   finish(e, anyHit, const HitOut&)
+// chunk: entries a wave claims at a time (wave-uniform; smaller chunks spread
+// a short queue over more waves)
 template <int kRefill, bool kFuse, bool kHasInv, bool kInst = false, uint32_t kParts = 1, typename Entry,
           typename Load, typename Finish>
 __device__ __forceinline__ void traceLoop(const DevScene& sc, const TraceStack& st, uint32_t count,
                                           uint32_t* fetchCtr, Entry&& entry, Load&& load, Finish&& finish,
-                                          TraceCounters& tc) {
-    constexpr uint32_t kChunk = SLRH_FETCH_CHUNK;
+                                          TraceCounters& tc, uint32_t chunk = SLRH_FETCH_CHUNK) {
+    const uint32_t kChunk = chunk;
     uint32_t part = kParts > 1 ? blockIdx.x % kParts : 0;   // wave-uniform
     uint32_t partsLeft = kParts;
     const char* nodes = (const char*)sc.nodes;
