@@ -364,10 +364,11 @@ SLRH_API int slrh_medium_query(slrh_scene* s, const slrh_rays* rays, uint32_t n,
  * cap); at most 64 light subpath vertices are kept per sample (stats[22]
  * counts subpaths stopped by either cap). stats[16] light and [17] eye
  * subpath vertices, [19] connections traced, [20] lens contributions filed.
- * Each of the pixels x slots_per_pixel slots (at most 2^24, automatic: about
- * 2M) renders its pixel's samples one after another from its own random
- * stream, at most 65535 per slot; films of equal seeds and slots_per_pixel
- * are the same however the device interleaves the slots. The lens
+ * Each of the pixels x slots_per_pixel slots (at most 2^24; automatic:
+ * about 4M, 16 KB of device memory each) renders its pixel's samples one
+ * after another from its own random stream, at most 65535 per slot; films
+ * of equal seeds and slots_per_pixel are the same however the device
+ * interleaves the slots. The lens
  * contributions are summed with FP64 atomics in a varying order, so
  * films of equal seeds agree to rounding (about 1e-15 relative), not
  * bit-for-bit. slrh_render_batch renders BPT frames one after another. */

@@ -78,7 +78,7 @@ constexpr uint32_t kSplatBit = 0x40000000u;
 constexpr uint32_t kMaxBptSlots = 1u << 24;
 constexpr int kBptLdsDepth = 16;
 constexpr uint32_t kBptTreelet = 128;
-constexpr uint32_t kTargetSlots = 1u << 21;   // slots when the caller leaves it open
+constexpr uint32_t kTargetSlots = 1u << 22;   // slots when the caller leaves it open (about 64 GB)
 constexpr uint32_t kStepsPerRead = 4;        // wavefront steps between reads of the queue lengths
 
 // vertex flags (row 3 w)
