@@ -70,6 +70,22 @@ def test_agrees_with_path_tracer(gpu_ctx):
     assert np.median(bpt.std(0)) < np.median(pt.std(0))
 
 
+def test_instanced_scene_agrees_with_path_tracer(gpu_ctx):
+    """The instanced traversal of the bidirectional integrator: ten scaled
+    and rotated instances in the Cornell box, one of them the light, so light
+    subpaths start on an instance and connections and lens rays cross them.
+    BPT and PT (itself checked against the reference on instanced scenes,
+    tests/test_gpu_instances.py) estimate the same image."""
+    W = H = 32
+    desc = scenes.instanced_cornell(W, H, n_instances=10, seed=12, uniform_scale=True, instanced_light=True)
+    bpt, stats = _batches(gpu_ctx, desc, W, H, 64, 70, 128)
+    for st in stats:
+        assert st["samples"] == W * H * 64 and st["capped_subpaths"] == 0 and st["nonfinite"] == 0
+        assert st["inst_entered"] > 0
+    pt, _ = _batches(gpu_ctx, desc, W, H, 64, 970, 128, renderer=slr_amd.PTRenderer)
+    refcases.check_3sigma(bpt, _stats_of(pt), name="bpt_vs_pt_instanced")
+
+
 def test_white_furnace(gpu_ctx):
     W = H = 16
     spp = 512
