@@ -810,6 +810,14 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
                 if (k >= kLightCap) {
                     ++capped;
                 } else {
+                    // every read of the slot's state is issued before the
+                    // vertex and splat stores (which the compiler cannot
+                    // move them past)
+                    float4* lvS = b.lv + (size_t)s * kLightCap * kVtxRows;
+                    const float4* pr = lvS + (size_t)(k - 1) * kVtxRows;
+                    const float4 pr0 = pr[0], pr1 = pr[1], pr3 = pr[3];
+                    const float4 sm = b.smp[s], lensS = b.lens[s];
+                    uint4 rng = b.rng[s];
                     const NewVertex nv = hitVertex(sc, b, s, g);
                     const float4 carried = g[3];
                     const float4 mis = b.mis[s];
@@ -831,19 +839,16 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
                     V.flags = deltaK ? V_DELTA : 0u;
                     V.dir = nv.dirOut;
                     V.mat = nv.sp.mat;
-                    float4* lvS = b.lv + (size_t)s * kLightCap * kVtxRows;
                     storeVtx(lvS + (size_t)k * kVtxRows, V, alpha);
                     ++verts;
                     aux.z = k + 1;
                     b.aux[s] = aux;
                     // light vertex k to the lens
-                    const float4* pr = lvS + (size_t)(k - 1) * kVtxRows;
                     float c[kNumL];
                     uint32_t pix = 0;
                     float4 ro, rd;
-                    const float4 sm = b.smp[s];
-                    sh = connectLens<kDiel>(sc, cam, lf, b.lens[s], make_float2(sm.z, sm.w), V, alpha, true, pr[0], pr[1],
-                                     __float_as_uint(pr[3].w), hero, off, c, &pix, &ro, &rd);
+                    sh = connectLens<kDiel>(sc, cam, lf, lensS, make_float2(sm.z, sm.w), V, alpha, true, pr0, pr1,
+                                            __float_as_uint(pr3.w), hero, off, c, &pix, &ro, &rd);
                     if (sh) {
                         float4* spl = splatRows(b, nxt, s);
                         store16(spl, c);
@@ -851,7 +856,6 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
                         spl[5] = rd;
                         spl[6] = make_float4(__uint_as_float(pix), 0, 0, 0);
                     }
-                    uint4 rng = b.rng[s];
                     cont = extendSubpath<kDiel>(sc, b, s, nv, nv.sp.mat, m, V.P, deltaK, hero, off, true, true, alpha, rng);
                     b.rng[s] = rng;
                 }
