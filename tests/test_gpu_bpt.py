@@ -8,9 +8,11 @@ on the GPU.
 * against the GPU path tracer on the same scene (two unbiased estimators of
   one image) within 3 sigma;
 * against the white-furnace closed form;
-* its own invariants: every sample filed, no subpath stopped by the vertex
-  caps on these scenes, equal films for equal seeds up to the order of the
-  lens contributions' FP64 atomic adds, batches equal to single renders.
+* its own invariants: every sample filed (also when spp is not a multiple
+  of the slots per pixel), no subpath stopped by the vertex caps on these
+  scenes and the eye cap counted when set, equal films for equal seeds up to
+  the order of the lens contributions' FP64 atomic adds, batches equal to
+  single renders, capacity limits refused before rendering.
 
 Same-seed equality with the reference is not attempted: the reference draws
 each thread's random numbers in tile order (BPTRenderer.cpp:27-33), so its
@@ -157,3 +159,52 @@ def test_refuses_environment_light(gpu_ctx):
     # the scene stays usable
     slr_amd.PTRenderer(4).render(sc, film, seed=1)
     assert film.strata().sum() > 0
+
+
+def test_ragged_sample_quotas(gpu_ctx):
+    """spp not a multiple of the slots per pixel: the slots of a pixel render
+    3, 2 and 2 of its 7 samples; every sample is filed once."""
+    W = H = 16
+    desc = scenes.cornell_box(W, H)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    tots = []
+    for k, spp, seed in ((3, 7 * 64, 11), (1, 7 * 64, 12)):
+        f = slr_amd.Film(gpu_ctx, W, H)
+        st = slr_amd.BPTRenderer(spp).render(sc, f, seed=seed, slots_per_pixel=k)
+        assert st["slots_per_pixel"] == k and st["samples"] == W * H * spp and st["nonfinite"] == 0
+        tots.append(f.strata().sum())
+    assert abs(tots[0] / tots[1] - 1) < 0.02
+    f = slr_amd.Film(gpu_ctx, W, H)
+    st = slr_amd.BPTRenderer(7).render(sc, f, seed=13, slots_per_pixel=3)
+    assert st["samples"] == W * H * 7
+
+
+def test_eye_depth_cap(gpu_ctx):
+    """max_depth caps the eye subpath: the capped paths are counted and the
+    image loses the energy of the longer paths."""
+    W = H = 16
+    desc = scenes.cornell_box(W, H)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    f1, f2 = slr_amd.Film(gpu_ctx, W, H), slr_amd.Film(gpu_ctx, W, H)
+    capped = slr_amd.BPTRenderer(128).render(sc, f1, seed=21, max_depth=2)
+    full = slr_amd.BPTRenderer(128).render(sc, f2, seed=22)
+    assert capped["capped_subpaths"] > 0 and full["capped_subpaths"] == 0
+    assert capped["samples"] == full["samples"] == W * H * 128
+    assert capped["eye_vertices"] < full["eye_vertices"]
+    assert f1.strata().sum() < f2.strata().sum()
+
+
+def test_capacity_limits(gpu_ctx):
+    """More than 2^24 slots, or more than 65535 samples per slot, are refused
+    before anything is rendered; the scene stays usable."""
+    desc = scenes.cornell_box(64, 64)
+    sc = slr_amd.build_scene(gpu_ctx, desc)
+    big = slr_amd.Film(gpu_ctx, 1024, 1024)
+    with pytest.raises(slr_amd.SlrhError, match="too many slots"):
+        slr_amd.BPTRenderer(32).render(sc, big, seed=1, slots_per_pixel=32)
+    one = slr_amd.Film(gpu_ctx, 1, 1)
+    with pytest.raises(slr_amd.SlrhError, match="samples per slot"):
+        slr_amd.BPTRenderer(70000).render(sc, one, seed=1, slots_per_pixel=1)
+    film = slr_amd.Film(gpu_ctx, 64, 64)
+    st = slr_amd.BPTRenderer(4).render(sc, film, seed=2)
+    assert st["samples"] == 64 * 64 * 4 and film.strata().sum() > 0
