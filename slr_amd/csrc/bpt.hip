@@ -1057,11 +1057,17 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_connect(DevScene sc, BptState
     __shared__ uint32_t incl[kBptBlock], slots[kBptBlock];
     const uint32_t nc = b.ctr[K_GA + nxt];
     const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
+    // entries per wave: 64, or fewer when the queue is short (the last steps
+    // of a frame), so the pairs of a few long light subpaths spread over the
+    // chip instead of queueing in a few waves
+    const uint32_t waves = gridDim.x * (kBptBlock / 64);
+    const uint32_t per = min(64u, max(1u, (nc + waves - 1) / waves));
+    const uint32_t wave = blockIdx.x * (kBptBlock / 64) + (threadIdx.x >> 6);
     unsigned long long conns = 0;
-    for (uint32_t base = blockIdx.x * kBptBlock; base < nc; base += gridDim.x * kBptBlock) {
-        const uint32_t i = base + threadIdx.x;
+    for (uint32_t base = wave * per; base < nc; base += waves * per) {
+        const uint32_t i = base + lane;
         uint32_t s = 0, np = 0;
-        if (i < nc) {
+        if (lane < per && i < nc) {
             s = qGather(b, nxt)[i];
             const uint4 aux = b.aux[s];
             np = (aux.y & B_CONNECT) ? (aux.z & 0xFFu) : 0u;
