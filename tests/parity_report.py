@@ -1,8 +1,8 @@
 """Writes tests/parity_summary.md from the JSON-lines records that the GPU
 parity tests append to $SLRH_PARITY_OUT:
 
-    SLRH_PARITY_OUT=$PWD/gpurun_out/parity.jsonl python -u -m pytest tests -m gpu -x -v -s
-    python tests/parity_report.py gpurun_out/parity.jsonl
+    SLRH_PARITY_OUT=/tmp/parity.jsonl python -u -m pytest tests -m gpu -x -v -s
+    python tests/parity_report.py /tmp/parity.jsonl
 """
 import json
 import os
