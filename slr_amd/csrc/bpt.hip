@@ -212,7 +212,8 @@ __device__ __forceinline__ void materialSpectra(const DevScene& sc, uint32_t mat
 // BSDF with the subpath's query (BSDFProxy, BPTRenderer.h:40-52).
 template <bool kDiel>
 __device__ __forceinline__ void vertexDdf(const DevScene& sc, const Vtx& v, bool adjoint, int hero, float off,
-                                          const float3& dl, float* fs, float* pdf, float* rev) {
+                                          const float3& dl, float* fs, float* pdf, float* rev,
+                                          const float* spA = nullptr, const float* spB = nullptr) {
     *rev = 0.0f;
     if (v.flags & V_EDF) {
         const float e = dl.z > 0.0f ? (float)(1.0 / 3.14159265358979323846) : 0.0f;
@@ -228,8 +229,17 @@ __device__ __forceinline__ void vertexDdf(const DevScene& sc, const Vtx& v, bool
         *pdf = 0.0f;
         return;
     }
+    // the vertex's material spectra, unless the caller has them already
     float A[kNumL], B[kNumL];
-    materialSpectra(sc, v.mat, m.type, off, A, B);
+    if (spA) {
+#pragma unroll
+        for (int k = 0; k < kNumL; ++k) {
+            A[k] = spA[k];
+            B[k] = spB[k];
+        }
+    } else {
+        materialSpectra(sc, v.mat, m.type, off, A, B);
+    }
     const BiBsdf<kDiel> b{m.type, A, B, m.p0, m.p1};
     BiQuery q;
     q.dir = v.dir;
@@ -300,7 +310,8 @@ template <bool kDiel>
 __device__ __forceinline__ bool connectLens(const DevScene& sc, const DevCamera& cam, const LensFrame& lf,
                                             const float4& lens, float2 orgL, const Vtx& L, const float* lAlpha,
                                             bool hasPrev, float4 Lp0, float4 Lp1, uint32_t pFlags, int hero,
-                                            float off, float* c, uint32_t* pixel, float4* ro, float4* rd) {
+                                            float off, float* c, uint32_t* pixel, float4* ro, float4* rd,
+                                            const float* spA = nullptr, const float* spB = nullptr) {
     const float3 pe = xyz(lens);
     const float3 dv = L.p - pe;
     const float dist2 = dot3(dv, dv);
@@ -318,7 +329,7 @@ __device__ __forceinline__ bool connectLens(const DevScene& sc, const DevCamera&
     if (!valid) return false;
     const float eExt1DirPDF = cam.imgDist * cam.imgDist / ((e.z * e.z * e.z) * cam.imgPlaneArea);
     float lDDF[kNumL], lExt1DirPDF, eExt2DirPDF;
-    vertexDdf<kDiel>(sc, L, true, hero, off, vtxLocal(L, -cv), lDDF, &lExt1DirPDF, &eExt2DirPDF);
+    vertexDdf<kDiel>(sc, L, true, hero, off, vtxLocal(L, -cv), lDDF, &lExt1DirPDF, &eExt2DirPDF, spA, spB);
     bool any = false;
 #pragma unroll
     for (int k = 0; k < kNumL; ++k) any = any || lDDF[k] * G != 0.0f;
@@ -682,10 +693,19 @@ __device__ __forceinline__ NewVertex hitVertex(const DevScene& sc, const BptStat
 template <bool kDiel>
 __device__ __forceinline__ bool extendSubpath(const DevScene& sc, const BptState& b, uint32_t s, const NewVertex& nv,
                                               uint32_t mat, const DevMaterial& m, float P_k, bool deltaK, int hero,
-                                              float off, bool adjoint, bool tailDefined, float* alpha, uint4& rng) {
+                                              float off, bool adjoint, bool tailDefined, float* alpha, uint4& rng,
+                                              const float* spA = nullptr, const float* spB = nullptr) {
     if (m.type == MAT_NONE) return false;
     float A[kNumL], B[kNumL];
-    materialSpectra(sc, mat, m.type, off, A, B);
+    if (spA) {
+#pragma unroll
+        for (int k = 0; k < kNumL; ++k) {
+            A[k] = spA[k];
+            B[k] = spB[k];
+        }
+    } else {
+        materialSpectra(sc, mat, m.type, off, A, B);
+    }
     const BiBsdf<kDiel> bs{m.type, A, B, m.p0, m.p1};
     BiQuery q;
     q.dir = nv.dirOut;
@@ -827,6 +847,10 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
                     const DevMaterial m = sc.mats[nv.sp.mat];
                     float alpha[kNumL];
                     load16(b.alpha + (size_t)s * 4, alpha);
+                    // the material's spectra, shared by the lens connection and
+                    // the next direction
+                    float spA[kNumL], spB[kNumL];
+                    if (m.type != MAT_NONE) materialSpectra(sc, nv.sp.mat, m.type, off, spA, spB);
                     Vtx V;
                     V.p = nv.sp.p;
                     V.P = carried.x * nv.cosOut / nv.dist2 * carried.y;
@@ -848,7 +872,7 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
                     uint32_t pix = 0;
                     float4 ro, rd;
                     sh = connectLens<kDiel>(sc, cam, lf, lensS, make_float2(sm.z, sm.w), V, alpha, true, pr0, pr1,
-                                            __float_as_uint(pr3.w), hero, off, c, &pix, &ro, &rd);
+                                            __float_as_uint(pr3.w), hero, off, c, &pix, &ro, &rd, spA, spB);
                     if (sh) {
                         float4* spl = splatRows(b, nxt, s);
                         store16(spl, c);
@@ -856,7 +880,8 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_light(DevScene sc, BptState b
                         spl[5] = rd;
                         spl[6] = make_float4(__uint_as_float(pix), 0, 0, 0);
                     }
-                    cont = extendSubpath<kDiel>(sc, b, s, nv, nv.sp.mat, m, V.P, deltaK, hero, off, true, true, alpha, rng);
+                    cont = extendSubpath<kDiel>(sc, b, s, nv, nv.sp.mat, m, V.P, deltaK, hero, off, true, true, alpha, rng,
+                                                spA, spB);
                     b.rng[s] = rng;
                 }
             }
