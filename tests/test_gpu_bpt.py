@@ -208,3 +208,26 @@ def test_capacity_limits(gpu_ctx):
     film = slr_amd.Film(gpu_ctx, 64, 64)
     st = slr_amd.BPTRenderer(4).render(sc, film, seed=2)
     assert st["samples"] == 64 * 64 * 4 and film.strata().sum() > 0
+
+
+def test_c2_scene_agrees_with_reference_path_tracer_totals(gpu_ctx):
+    """The C2 scene (1.31M-triangle GGX-conductor icosphere in the Cornell
+    box) at 128x72: the bidirectional image against the reference
+    PTRenderer's statistics (tests/golden/ref_c2.npz). The image total agrees
+    within 4 standard errors and the relMSE is small; per pixel the
+    reference path tracer's 32 x 64-spp means are heavy-tailed where the
+    glossy sphere reflects the light (the pixels where the two differ by more
+    than 3 sigma all have the bidirectional estimate above, as missing rare
+    large samples leave a path tracer's mean and its error estimate low), so
+    no per-pixel gate is applied here."""
+    make, W, H, B, spp, _, _ = refcases.CASES["c2"]
+    gb, stats = _batches(gpu_ctx, make(), W, H, spp, 500, 32)
+    for st in stats:
+        assert st["samples"] == W * H * spp and st["nonfinite"] == 0 and st["capped_subpaths"] == 0
+    ref = refcases.load("c2")
+    tot = gb.sum(axis=(1, 2, 3))
+    z = (tot.mean() - float(ref["tot_mean"])) / np.sqrt(tot.var(ddof=1) / len(tot) + float(ref["tot_se"]) ** 2)
+    mean = gb.mean(0)
+    relmse = float(np.mean((mean - ref["mean"]) ** 2 / (ref["mean"] ** 2 + 1e-3)))
+    assert abs(z) < 4, z
+    assert relmse < 5e-3, relmse
