@@ -570,6 +570,7 @@ __device__ __forceinline__ bool startSample(const DevScene& sc, const BptState& 
 template <bool kDiel>
 __global__ __launch_bounds__(kBptBlock) void k_bpt_start(DevScene sc, BptState b, DevCamera cam) {
     const LensFrame lf = lensFrame(cam);
+    if (blockIdx.x == 0 && threadIdx.x < SLRH_FETCH_PARTS) b.fetch[threadIdx.x * kFetchStride] = 0;
     for (uint32_t base = blockIdx.x * kBptBlock; base < b.n; base += gridDim.x * kBptBlock) {
         const uint32_t s = base + threadIdx.x;
         bool has = false, sh = false;
@@ -1082,6 +1083,9 @@ __global__ __launch_bounds__(kBptBlock) void k_bpt_connect(DevScene sc, BptState
     __shared__ uint32_t incl[kBptBlock], slots[kBptBlock];
     const uint32_t nc = b.ctr[K_GA + nxt];
     const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
+    // the next step's traversal starts its claim cursors at 0 (the last
+    // kernel of a step resets them: no memset node in the step graph)
+    if (blockIdx.x == 0 && threadIdx.x < SLRH_FETCH_PARTS) b.fetch[threadIdx.x * kFetchStride] = 0;
     // entries per wave: 64, or fewer when the queue is short (the last steps
     // of a frame), so the pairs of a few long light subpaths spread over the
     // chip instead of queueing in a few waves
@@ -1184,6 +1188,10 @@ struct BptPool {
     DevBuf<uint32_t> q, ctr, spill, fetch;
     DevBuf<unsigned long long> stat;
     DevBuf<double> planeSums, lightImg;
+    hipStream_t stream = nullptr;   // the wavefront's stream (steps replay as a graph on it)
+    ~BptPool() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& camera, DeviceFilm* film,
@@ -1255,7 +1263,8 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
     auto& aux = P.aux; auto& rng = P.rng; auto& vis = P.vis; auto& visSplat = P.visSplat; auto& q = P.q;
     auto& ctr = P.ctr; auto& stat = P.stat; auto& planeSums = P.planeSums; auto& lightImg = P.lightImg;
     auto& spill = P.spill;
-    hipStream_t stream = 0;
+    if (!P.stream) HIP_TRY(hipStreamCreate(&P.stream));
+    hipStream_t stream = P.stream;
     HIP_TRY(hipMemsetAsync(vis.p, 0, (size_t)n * kLightCap, stream));
     HIP_TRY(hipMemsetAsync(stat.p, 0, stat.n * sizeof(unsigned long long), stream));
     HIP_TRY(hipMemsetAsync(planeSums.p, 0, (size_t)planes * pixels * kNumL * sizeof(double), stream));
@@ -1275,12 +1284,19 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
     uint32_t* hostCtr = nullptr;
     HIP_TRY(hipHostMalloc((void**)&hostCtr, kBptCounters * sizeof(uint32_t), hipHostMallocDefault));
     struct HostFree { uint32_t* p; ~HostFree() { if (p) (void)hipHostFree(p); } } hostGuard{hostCtr};
-    hipEvent_t ev0, ev1, ta, tb;
+    hipEvent_t ev0, ev1;
     HIP_TRY(hipEventCreate(&ev0));
     HIP_TRY(hipEventCreate(&ev1));
-    HIP_TRY(hipEventCreate(&ta));
-    HIP_TRY(hipEventCreate(&tb));
-    struct EvFree { hipEvent_t a, b, c, d; ~EvFree() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); (void)hipEventDestroy(d); } } evGuard{ev0, ev1, ta, tb};
+    std::vector<hipEvent_t> traceEvents;   // (start, end) per timed traversal
+    struct EvFree {
+        hipEvent_t a, b;
+        std::vector<hipEvent_t>* v;
+        ~EvFree() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            for (hipEvent_t e : *v) (void)hipEventDestroy(e);
+        }
+    } evGuard{ev0, ev1, &traceEvents};
     double traceSeconds = 0, traceLaunches = 0, iterations = 0;
     auto readCounters = [&]() -> int {
         HIP_TRY(hipMemcpyAsync(hostCtr, ctr.p, kBptCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -1292,20 +1308,22 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
     const uint32_t grid = blocksFor(n);
     const uint32_t traceGrid = std::min(traceBlocks, blocksFor(4 * n));
     auto launchStep = [&](uint32_t cur) -> int {
-        HIP_TRY(hipMemsetAsync(b.fetch, 0, (size_t)SLRH_FETCH_PARTS * kFetchStride * sizeof(uint32_t), stream));
-        if (cfg.timeKernels) HIP_TRY(hipEventRecord(ta, stream));
+        // timed traversals: an event pair each, read after the frame (no
+        // wait between steps)
+        hipEvent_t ta = nullptr, tb = nullptr;
+        if (cfg.timeKernels) {
+            HIP_TRY(hipEventCreate(&ta));
+            traceEvents.push_back(ta);
+            HIP_TRY(hipEventCreate(&tb));
+            traceEvents.push_back(tb);
+            HIP_TRY(hipEventRecord(ta, stream));
+        }
         if (sc.numInstances)
             hipLaunchKernelGGL(k_bpt_trace<true>, dim3(traceGrid), dim3(kBptBlock), 0, stream, sc, b, cur);
         else
             hipLaunchKernelGGL(k_bpt_trace<false>, dim3(traceGrid), dim3(kBptBlock), 0, stream, sc, b, cur);
         HIP_TRY(hipGetLastError());
-        if (cfg.timeKernels) {
-            HIP_TRY(hipEventRecord(tb, stream));
-            HIP_TRY(hipEventSynchronize(tb));
-            float ms = 0;
-            HIP_TRY(hipEventElapsedTime(&ms, ta, tb));
-            traceSeconds += ms * 1e-3;
-        }
+        if (cfg.timeKernels) HIP_TRY(hipEventRecord(tb, stream));
         traceLaunches += 1;
         if (diel) {
             hipLaunchKernelGGL(k_bpt_light<true>, dim3(grid), dim3(kBptBlock), 0, stream, sc, b, cam, cur);
@@ -1333,15 +1351,41 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
     // samples). The kernels take their queue lengths from the device
     // counters, so the host reads them only every few steps; the steps past
     // the end find empty queues.
-    uint32_t cur = 0;
+    // An even number of steps between reads, so every block of steps starts
+    // at parity 0 and one recorded graph of the block serves every replay
+    // (kernel timing launches the steps one by one instead).
+    static_assert(kStepsPerRead % 2 == 0, "step blocks start at parity 0");
+    hipGraphExec_t block = nullptr;
+    struct GraphFree { hipGraphExec_t* g; ~GraphFree() { if (*g) (void)hipGraphExecDestroy(*g); } } graphGuard{&block};
+    if (!cfg.timeKernels) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int e = 0;
+        for (uint32_t k = 0; k < kStepsPerRead && !e; ++k) e = launchStep(k & 1u);
+        const hipError_t ce = hipStreamEndCapture(stream, &graph);
+        if (e) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return e;
+        }
+        HIP_TRY(ce);
+        const hipError_t ie = hipGraphInstantiate(&block, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(ie);
+        // the capture counted its steps; the replays count their own
+        iterations -= kStepsPerRead;
+        traceLaunches -= kStepsPerRead;
+    }
     for (;;) {
         if (int e = readCounters()) return e;
-        const uint32_t pending = hostCtr[K_LE + cur] + hostCtr[K_EE + cur] + hostCtr[K_SP + cur] +
-                                 hostCtr[K_CN + cur] + hostCtr[K_GA + cur];
+        const uint32_t pending = hostCtr[K_LE] + hostCtr[K_EE] + hostCtr[K_SP] + hostCtr[K_CN] + hostCtr[K_GA];
         if (pending == 0) break;
-        for (uint32_t k = 0; k < kStepsPerRead; ++k) {
-            if (int e = launchStep(cur)) return e;
-            cur ^= 1u;
+        if (block) {
+            HIP_TRY(hipGraphLaunch(block, stream));
+            iterations += kStepsPerRead;
+            traceLaunches += kStepsPerRead;
+        } else {
+            for (uint32_t k = 0; k < kStepsPerRead; ++k)
+                if (int e = launchStep(k & 1u)) return e;
         }
     }
     hipLaunchKernelGGL(k_bpt_reduce, dim3(blocksFor(pixels * kNumL)), dim3(kBptBlock), 0, stream, planeSums.p, planes,
@@ -1351,6 +1395,11 @@ int renderBpt(DeviceScene* ds, const RenderConfig& cfg, const CameraParams& came
     HIP_TRY(hipEventSynchronize(ev1));
     float totalMs = 0;
     HIP_TRY(hipEventElapsedTime(&totalMs, ev0, ev1));
+    for (size_t k = 0; k + 1 < traceEvents.size(); k += 2) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, traceEvents[k], traceEvents[k + 1]));
+        traceSeconds += ms * 1e-3;
+    }
     if (int e = readCounters()) return e;
     std::vector<unsigned long long> hs(stat.n);
     HIP_TRY(hipMemcpy(hs.data(), stat.p, stat.n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
